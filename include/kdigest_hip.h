@@ -1,0 +1,150 @@
+/*
+ * kdigest_hip.h — C ABI of the MI355X (gfx950) digest path.
+ *
+ * Plain C types only (no torch, no C++ in signatures).  Every function returns
+ * KD_OK (0) or a negative KD_E* code; kd_last_error() gives a thread-local
+ * message.  Validation happens before any state changes, as in the reference
+ * (kdigest/tdigest.mojo:252-254, kdigest/grouped.mojo:91-97).
+ *
+ * Scale ids: 0..3 = ScaleK0..ScaleK3 (kdigest/scale.mojo:121-243), a runtime
+ * int in place of the reference's compile-time type parameter.
+ *
+ * Citations are of the reference interface each entry point replaces (paths
+ * relative to the reference repository root).
+ */
+#ifndef KDIGEST_HIP_H
+#define KDIGEST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KD_OK 0
+#define KD_EINVAL -1   /* invalid value, weight, key, scale or argument */
+#define KD_EEMPTY -2   /* query on an empty digest */
+#define KD_ERANGE -3   /* quantile q outside [0, 1] */
+#define KD_ESERDE -4   /* unrecognized serialization version */
+#define KD_ETRUNC -5   /* truncated serialized digest */
+#define KD_EDEVICE -6  /* HIP runtime / device failure */
+#define KD_ENOMEM -7   /* out of device or host memory */
+
+typedef struct kd_session kd_session;
+typedef struct kd_tdigest kd_tdigest;
+typedef struct kd_grouped kd_grouped;
+
+/* Thread-local description of the last error on this thread. */
+const char *kd_last_error(void);
+
+/* ---------------------------------------------------------- session ----- */
+/* GpuSession (kdigest/gpu/session.mojo:19-27): one device, one in-order HIP
+ * stream and workspaces that grow geometrically and are reused across calls
+ * (CompressWorkspace.reserve, kdigest/gpu/add_batch.mojo:400-404). */
+int kd_session_create(int device, kd_session **out);
+void kd_session_destroy(kd_session *s);
+int kd_session_sync(kd_session *s);
+
+/* ------------------------------------------------ operator entry points -- */
+/* gpu_compress_batch (kdigest/gpu/add_batch.mojo:629-663) as an FP64
+ * greedy-sweep compress.  means[0..n_prior+n): the first n_prior records carry
+ * weights[i] (prior centroids ++ buffer, in that order), the next n carry
+ * w_uniform.  Result: the centroids of the reference CPU merge pass
+ * (kdigest/tdigest.mojo:391-439) with q ratios over stored_n and the given
+ * normalizer nrm.  Host pointers; out arrays hold n_prior + n entries. */
+int kd_compress(kd_session *s, int scale, double nrm, double stored_n, const double *means,
+                const double *weights, double w_uniform, int64_t n_prior, int64_t n,
+                double *out_means, double *out_weights, int64_t *out_count);
+
+/* gpu_quantile / gpu_cdf (kdigest/gpu/query.mojo:320-342), batched over many
+ * digests.  Digest d has centroids [off[d], off[d+1]) of means/weights and
+ * state[4*d .. 4*d+4) = {stored_total, min, max, eff_factor}, where
+ * eff_factor = e^(-log_scale) with decay and 1 without (the singleton test of
+ * tdigest.mojo:475-483).  Query i reads digest q_digest[i].  Host pointers. */
+int kd_quantile(kd_session *s, int64_t num_digests, const int64_t *off, const double *means,
+                const double *weights, const double *state, const int64_t *q_digest,
+                const double *qs, int64_t nq, double *out);
+int kd_cdf(kd_session *s, int64_t num_digests, const int64_t *off, const double *means,
+           const double *weights, const double *state, const int64_t *q_digest,
+           const double *xs, int64_t nq, double *out);
+
+/* Device-level FP64 scale evaluation, the analogue of gpu_scale_eval
+ * (kdigest/gpu/scale_device.mojo:53-98): out_k[i] = k(q[i], nrm).  Host
+ * pointers; used to pin device math against the host. */
+int kd_scale_eval(kd_session *s, int scale, double nrm, const double *q, int64_t n,
+                  double *out_k);
+/* Device log / asin / sqrt of kd_math.h over an array (fn 0 / 1 / 2; host
+ * pointers). */
+int kd_math_eval(kd_session *s, int fn, const double *x, int64_t n, double *out);
+
+/* Host-side scale functions (kdigest/scale.mojo:105-235), the same code the
+ * device evaluates: normalizer(compression, n), k(q, nrm), q(k, nrm). */
+double kd_scale_normalizer(int scale, double compression, double n);
+double kd_scale_k(int scale, double q, double nrm);
+double kd_scale_q(int scale, double k, double nrm);
+
+/* ------------------------------------------------ TDigest (one digest) -- */
+/* TDigest[S] (kdigest/tdigest.mojo:97-726).  decay_lambda is Decay.rate_lambda
+ * (kdigest/decay.mojo:267-294); 0 disables decay. */
+int kd_tdigest_create(kd_session *s, int scale, double compression, double decay_lambda,
+                      kd_tdigest **out);
+void kd_tdigest_destroy(kd_tdigest *d);
+int kd_tdigest_add(kd_tdigest *d, double value, double weight);              /* :153-161 */
+int kd_tdigest_add_at(kd_tdigest *d, double value, double timestamp, double weight); /* :163-175 */
+/* add_batch (:177-221): below the crossover (60000, or 40000 when
+ * with_session) the per-sample path, otherwise one batch merge (:223-315). */
+int kd_tdigest_add_batch(kd_tdigest *d, const double *values, int64_t n, int with_session);
+/* One batch merge of device-resident values (B semantics, :223-315). */
+int kd_tdigest_add_batch_device(kd_tdigest *d, const double *d_values, int64_t n);
+int kd_tdigest_merge(kd_tdigest *d);                                          /* :391-439 */
+int kd_tdigest_quantile(kd_tdigest *d, const double *qs, int64_t nq, double *out); /* :485-538 */
+int kd_tdigest_cdf(kd_tdigest *d, const double *xs, int64_t nq, double *out);      /* :540-587 */
+int kd_tdigest_merge_into(kd_tdigest *self, kd_tdigest *other);               /* :589-623 */
+/* to_bytes (:627-652): returns the size; writes when cap is large enough. */
+int64_t kd_tdigest_to_bytes(kd_tdigest *d, uint8_t *out, int64_t cap);
+int kd_tdigest_from_bytes(kd_session *s, const uint8_t *data, int64_t n, int scale,
+                          kd_tdigest **out);                                   /* :654-698 */
+int kd_tdigest_from_centroids(kd_session *s, int scale, double compression, const double *means,
+                              const double *weights, int64_t n, double min_v, double max_v,
+                              kd_tdigest **out);                               /* :700-726 */
+/* __len__ (:460-463): number of merged centroids. */
+int64_t kd_tdigest_size(kd_tdigest *d);
+/* Centroids (means, weights) of the merged digest; *count gets the number. */
+int kd_tdigest_centroids(kd_tdigest *d, double *means, double *weights, int64_t cap,
+                         int64_t *count);
+/* {stored_total, log_scale, clock, min, max, compression, lambda,
+ *  total_weight (:465-467), buffered samples}. */
+int kd_tdigest_state(kd_tdigest *d, double *out9);
+
+/* ------------------------------------------ GroupedTDigest (per key) ---- */
+/* GroupedTDigest[S] (kdigest/grouped.mojo:30-190): num_keys digests with
+ * dense ids, state resident in device memory. */
+int kd_grouped_create(kd_session *s, int scale, int64_t num_keys, double compression,
+                      double decay_lambda, kd_grouped **out);
+void kd_grouped_destroy(kd_grouped *g);
+/* add_batch (grouped.mojo:60-162): keys uint32 < num_keys (the reference's
+ * Int32 dense ids), host arrays. */
+int kd_grouped_add_batch(kd_grouped *g, const uint32_t *keys, const double *values, int64_t n);
+/* Same with device-resident keys and values. */
+int kd_grouped_add_batch_device(kd_grouped *g, const uint32_t *d_keys, const double *d_values,
+                                int64_t n);
+/* quantile / cdf (grouped.mojo:164-174), vectorized: query i on key keys[i]. */
+int kd_grouped_quantile(kd_grouped *g, const int64_t *keys, const double *qs, int64_t nq,
+                        double *out);
+int kd_grouped_cdf(kd_grouped *g, const int64_t *keys, const double *xs, int64_t nq,
+                   double *out);
+/* Device-resident query arrays (keys int64, values, output). */
+int kd_grouped_quantile_device(kd_grouped *g, const int64_t *d_keys, const double *d_qs,
+                               int64_t nq, double *d_out);
+int64_t kd_grouped_to_bytes(kd_grouped *g, int64_t key, uint8_t *out, int64_t cap); /* :176-180 */
+int kd_grouped_centroid_count(kd_grouped *g, int64_t key, int64_t *out);            /* :182-185 */
+int kd_grouped_total_weight(kd_grouped *g, int64_t key, double *out);               /* :187-190 */
+int kd_grouped_centroids(kd_grouped *g, int64_t key, double *means, double *weights,
+                         int64_t cap, int64_t *count);
+/* Per-key scalar state {stored_total, log_scale, clock, min, max}. */
+int kd_grouped_key_state(kd_grouped *g, int64_t key, double *out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

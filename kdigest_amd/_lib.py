@@ -1,0 +1,102 @@
+"""ctypes binding of libkdigest_hip.so (the C ABI in include/kdigest_hip.h).
+
+The library is the only compute path: importing this module never falls back
+to a CPU implementation.  If the shared object is missing the import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkdigest_hip.so")
+
+KD_OK, KD_EINVAL, KD_EEMPTY, KD_ERANGE, KD_ESERDE, KD_ETRUNC, KD_EDEVICE, KD_ENOMEM = (
+    0, -1, -2, -3, -4, -5, -6, -7)
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+_i64 = C.c_int64
+_d = C.c_double
+_i = C.c_int
+
+# name -> (restype, argtypes); the exported set of include/kdigest_hip.h.
+SIGNATURES = {
+    "kd_last_error": (C.c_char_p, []),
+    "kd_session_create": (_i, [_i, C.POINTER(_vp)]),
+    "kd_session_destroy": (None, [_vp]),
+    "kd_session_sync": (_i, [_vp]),
+    "kd_compress": (_i, [_vp, _i, _d, _d, _dp, _dp, _d, _i64, _i64, _dp, _dp, _i64p]),
+    "kd_quantile": (_i, [_vp, _i64, _i64p, _dp, _dp, _dp, _i64p, _dp, _i64, _dp]),
+    "kd_cdf": (_i, [_vp, _i64, _i64p, _dp, _dp, _dp, _i64p, _dp, _i64, _dp]),
+    "kd_scale_eval": (_i, [_vp, _i, _d, _dp, _i64, _dp]),
+    "kd_math_eval": (_i, [_vp, _i, _dp, _i64, _dp]),
+    "kd_scale_normalizer": (_d, [_i, _d, _d]),
+    "kd_scale_k": (_d, [_i, _d, _d]),
+    "kd_scale_q": (_d, [_i, _d, _d]),
+    "kd_tdigest_create": (_i, [_vp, _i, _d, _d, C.POINTER(_vp)]),
+    "kd_tdigest_destroy": (None, [_vp]),
+    "kd_tdigest_add": (_i, [_vp, _d, _d]),
+    "kd_tdigest_add_at": (_i, [_vp, _d, _d, _d]),
+    "kd_tdigest_add_batch": (_i, [_vp, _dp, _i64, _i]),
+    "kd_tdigest_add_batch_device": (_i, [_vp, _vp, _i64]),
+    "kd_tdigest_merge": (_i, [_vp]),
+    "kd_tdigest_quantile": (_i, [_vp, _dp, _i64, _dp]),
+    "kd_tdigest_cdf": (_i, [_vp, _dp, _i64, _dp]),
+    "kd_tdigest_merge_into": (_i, [_vp, _vp]),
+    "kd_tdigest_to_bytes": (_i64, [_vp, _u8p, _i64]),
+    "kd_tdigest_from_bytes": (_i, [_vp, _u8p, _i64, _i, C.POINTER(_vp)]),
+    "kd_tdigest_from_centroids": (_i, [_vp, _i, _d, _dp, _dp, _i64, _d, _d, C.POINTER(_vp)]),
+    "kd_tdigest_size": (_i64, [_vp]),
+    "kd_tdigest_centroids": (_i, [_vp, _dp, _dp, _i64, _i64p]),
+    "kd_tdigest_state": (_i, [_vp, _dp]),
+    "kd_grouped_create": (_i, [_vp, _i, _i64, _d, _d, C.POINTER(_vp)]),
+    "kd_grouped_destroy": (None, [_vp]),
+    "kd_grouped_add_batch": (_i, [_vp, _u32p, _dp, _i64]),
+    "kd_grouped_add_batch_device": (_i, [_vp, _vp, _vp, _i64]),
+    "kd_grouped_quantile": (_i, [_vp, _i64p, _dp, _i64, _dp]),
+    "kd_grouped_cdf": (_i, [_vp, _i64p, _dp, _i64, _dp]),
+    "kd_grouped_quantile_device": (_i, [_vp, _vp, _vp, _i64, _vp]),
+    "kd_grouped_to_bytes": (_i64, [_vp, _i64, _u8p, _i64]),
+    "kd_grouped_centroid_count": (_i, [_vp, _i64, _i64p]),
+    "kd_grouped_total_weight": (_i, [_vp, _i64, _dp]),
+    "kd_grouped_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
+    "kd_grouped_key_state": (_i, [_vp, _i64, _dp]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m kdigest_amd.build_ext` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class KdigestError(Exception):
+    """A failed call into the HIP library (message from kd_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != KD_OK:
+        msg = load().kd_last_error().decode(errors="replace")
+        raise KdigestError(rc, msg)

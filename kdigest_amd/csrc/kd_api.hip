@@ -1,0 +1,1153 @@
+// kd_api.hip — the C ABI (include/kdigest_hip.h): session, operator entry
+// points, query kernels and the host layer of TDigest / GroupedTDigest.
+//
+// The host layer keeps only scalar digest state (clock, decay exponent,
+// stored total, min, max) and the per-sample ingest buffer on the host, as the
+// reference does (kdigest/tdigest.mojo:116-151).  Centroids live in device
+// memory; every merge, batch compress and query runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kdigest_hip.h"
+#include "kd_engine.h"
+#include "kd_math.h"
+
+using kd::DevBuf;
+using kd::SegDesc;
+using kd::SegResult;
+
+namespace {
+
+constexpr double kRenormThreshold = 400.0;  // tdigest.mojo:43
+constexpr double kPruneEpsilon = 1e-6;      // tdigest.mojo:47
+constexpr int64_t kCrossover = 60000;       // tdigest.mojo:55
+constexpr int64_t kCrossoverSession = 40000;// tdigest.mojo:58
+constexpr uint8_t kSerdeVersion = 1;        // tdigest.mojo:61
+
+thread_local std::string g_err;
+
+struct KdError : std::runtime_error {
+  int code;
+  KdError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const KdError& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(KD_ENOMEM, "out of memory");
+  } catch (const std::exception& e) {
+    return fail(KD_EDEVICE, e.what());
+  }
+}
+
+void hip_ok(hipError_t e, const char* what) { kd::check_hip(e, what); }
+
+bool is_finite_val(double v) { return std::isfinite(v); }
+
+// ------------------------------------------------------------ kernels
+__global__ void k_scale_eval(int scale, double nrm, const double* q, int64_t n, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = kd_k(scale, q[i], nrm);
+}
+
+__global__ void k_math_eval(int fn, const double* x, int64_t n, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = fn == 0 ? kd_log(x[i]) : (fn == 1 ? kd_asin(x[i]) : kd_sqrt(x[i]));
+}
+
+// Interior walk prefix of quantile/cdf (tdigest.mojo:528-536, 575-585):
+// S[0] = w0/2, S[i+1] = S[i] + (w[i] + w[i+1])/2, accumulated in order so the
+// binary search below sees exactly the sums of the sequential walk.
+__global__ void k_query_prefix(int64_t nd, const int64_t* off, const double* w, double* S) {
+  int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  int64_t a = off[d], b = off[d + 1];
+  if (b <= a) return;
+  double acc = w[a] / 2.0;
+  S[a] = acc;
+  for (int64_t i = a; i + 1 < b; i++) {
+    acc += (w[i] + w[i + 1]) / 2.0;
+    S[i + 1] = acc;
+  }
+}
+
+struct QState {
+  double total, vmin, vmax, eff;
+};
+
+__device__ inline bool singleton(double stored, double eff) { return stored * eff <= 1.0 + 1e-9; }
+
+// tdigest.mojo:485-538 for one query.
+__device__ double quantile_one(const double* m, const double* w, const double* S, int64_t nc,
+                               QState st, double q) {
+  if (nc == 0) return __builtin_nan("");
+  if (nc == 1) return m[0];
+  double n = st.total;
+  if (q <= 0.0) return st.vmin;
+  if (q >= 1.0) return st.vmax;
+  double target = q * n;
+  double first_half = w[0] / 2.0;
+  if (target < first_half) {
+    if (singleton(w[0], st.eff)) return st.vmin;
+    return st.vmin + (m[0] - st.vmin) * (target / first_half);
+  }
+  double lm = m[nc - 1], lw = w[nc - 1];
+  double last_half = lw / 2.0;
+  if (target > n - last_half) {
+    if (singleton(lw, st.eff)) return st.vmax;
+    double frac = (target - (n - last_half)) / last_half;
+    return lm + (st.vmax - lm) * frac;
+  }
+  // First i in [0, nc-2] with target <= S[i+1].
+  int64_t lo = 1, hi = nc;  // search j = i+1 in [1, nc-1]; hi == nc means none
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (target <= S[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= nc) return lm;
+  int64_t i = lo - 1;
+  double dw = (w[i] + w[i + 1]) / 2.0;
+  double frac = (target - S[i]) / dw;
+  return m[i] + (m[i + 1] - m[i]) * frac;
+}
+
+// tdigest.mojo:540-587 for one query.
+__device__ double cdf_one(const double* m, const double* w, const double* S, int64_t nc,
+                          QState st, double x) {
+  if (nc == 0) return __builtin_nan("");
+  if (x < st.vmin) return 0.0;
+  if (x > st.vmax) return 1.0;
+  if (nc == 1) {
+    if (x < m[0]) return 0.0;
+    if (x > m[0]) return 1.0;
+    return 0.5;
+  }
+  double n = st.total;
+  if (x < m[0]) {
+    if (st.vmin == m[0]) return 0.0;
+    double frac = (x - st.vmin) / (m[0] - st.vmin);
+    return (w[0] / 2.0) * frac / n;
+  }
+  double lm = m[nc - 1], lw = w[nc - 1];
+  if (x > lm) {
+    if (st.vmax == lm) return 1.0;
+    double frac = (x - lm) / (st.vmax - lm);
+    return (n - lw / 2.0 + (lw / 2.0) * frac) / n;
+  }
+  // First i in [0, nc-2] with x <= m[i+1].
+  int64_t lo = 1, hi = nc;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (x <= m[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= nc) return 1.0;
+  int64_t i = lo - 1;
+  double dw = (w[i] + w[i + 1]) / 2.0;
+  if (m[i + 1] == m[i]) return (S[i] + dw / 2.0) / n;
+  double frac = (x - m[i]) / (m[i + 1] - m[i]);
+  return (S[i] + dw * frac) / n;
+}
+
+__global__ void k_query(int is_cdf, const int64_t* off, const double* m, const double* w,
+                        const double* S, const double* state, const int64_t* qd,
+                        const double* qs, int64_t nq, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int64_t d = qd[i];
+  int64_t a = off[d], nc = off[d + 1] - a;
+  QState st{state[4 * d], state[4 * d + 1], state[4 * d + 2], state[4 * d + 3]};
+  out[i] = is_cdf ? cdf_one(m + a, w + a, S + a, nc, st, qs[i])
+                  : quantile_one(m + a, w + a, S + a, nc, st, qs[i]);
+}
+
+// Grouped queries: the state arrays are separate per-key vectors.
+__global__ void k_query_grouped(int is_cdf, const int64_t* off, const double* m, const double* w,
+                                const double* S, const double* total, const double* vmin,
+                                const double* vmax, const double* eff, const int64_t* qk,
+                                const double* qs, int64_t nq, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int64_t d = qk[i];
+  int64_t a = off[d], nc = off[d + 1] - a;
+  QState st{total[d], vmin[d], vmax[d], eff[d]};
+  out[i] = is_cdf ? cdf_one(m + a, w + a, S + a, nc, st, qs[i])
+                  : quantile_one(m + a, w + a, S + a, nc, st, qs[i]);
+}
+
+__global__ void k_scale_weights(double* w, int64_t n, double factor) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] *= factor;
+}
+
+// Scale the centroid weights of a list of keys (grouped renormalize).
+__global__ void k_scale_keys(const int64_t* keys, const double* factors, int64_t nk,
+                             const int64_t* off, double* w) {
+  int64_t t = blockIdx.x;
+  if (t >= nk) return;
+  int64_t k = keys[t];
+  for (int64_t i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) w[i] *= factors[t];
+}
+
+// Flags non-finite values and out-of-range keys (validate-then-commit).
+__global__ void k_validate(const uint32_t* keys, const double* vals, int64_t n, int64_t nk,
+                           int* flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = vals[i];
+  bool bad = !(v - v == 0.0);  // NaN or inf
+  if (keys && (int64_t)keys[i] >= nk) bad = true;
+  if (bad) atomicOr(flag, 1);
+}
+
+// New CSR of a grouped digest: untouched keys copy their old centroids,
+// touched keys take the engine's output.
+__global__ void k_gather_csr(int64_t nk, const int64_t* old_off, const double* om,
+                             const double* ow, const int64_t* seg_of, const int64_t* x_off,
+                             const double* em, const double* ew, const int64_t* new_off,
+                             double* nm, double* nw) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
+  int64_t sidx = seg_of[k];
+  const double* sm;
+  const double* sw;
+  if (sidx >= 0) {
+    sm = em + x_off[sidx];
+    sw = ew + x_off[sidx];
+  } else {
+    sm = om + old_off[k];
+    sw = ow + old_off[k];
+  }
+  for (int64_t i = 0; i < cnt; i++) {
+    nm[dst + i] = sm[i];
+    nw[dst + i] = sw[i];
+  }
+}
+
+unsigned blocks_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// =================================================================== session
+struct kd_session {
+  kd::Engine eng;
+  DevBuf q_off, q_m, q_w, q_s, q_state, q_qd, q_q, q_out, e_m, e_w, v_buf, tmp, tmp2;
+  explicit kd_session(int dev) : eng(dev) {}
+  hipStream_t st() { return eng.stream(); }
+  template <typename T>
+  T* buf(DevBuf& b, int64_t n) {
+    return static_cast<T*>(b.reserve(std::max<int64_t>(n, 1) * sizeof(T)));
+  }
+  void h2d(void* d, const void* h, size_t bytes) {
+    if (bytes) hip_ok(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st()), "H2D");
+  }
+  void d2h(void* h, const void* d, size_t bytes) {
+    if (bytes) hip_ok(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st()), "D2H");
+  }
+  void d2d(void* d, const void* s, size_t bytes) {
+    if (bytes) hip_ok(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st()), "D2D");
+  }
+  void sync() { hip_ok(hipStreamSynchronize(st()), "stream sync"); }
+};
+
+extern "C" const char* kd_last_error(void) { return g_err.c_str(); }
+
+extern "C" int kd_session_create(int device, kd_session** out) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      throw KdError(KD_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) throw KdError(KD_EINVAL, "device id out of range");
+    *out = new kd_session(device);
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_session_destroy(kd_session* s) { delete s; }
+
+extern "C" int kd_session_sync(kd_session* s) {
+  return guarded([&] {
+    s->sync();
+    return KD_OK;
+  });
+}
+
+// ======================================================= operator entries
+extern "C" int kd_compress(kd_session* s, int scale, double nrm, double stored_n,
+                           const double* means, const double* weights, double w_uniform,
+                           int64_t n_prior, int64_t n, double* out_means, double* out_weights,
+                           int64_t* out_count) {
+  return guarded([&] {
+    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+    if (n_prior < 0 || n < 0) throw KdError(KD_EINVAL, "negative size");
+    for (int64_t i = 0; i < n_prior + n; i++)
+      if (!is_finite_val(means[i])) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+    for (int64_t i = 0; i < n_prior; i++)
+      if (!(weights[i] > 0.0) || !is_finite_val(weights[i]))
+        throw KdError(KD_EINVAL, "weight must be positive and finite");
+    *out_count = 0;
+    if (n_prior + n == 0) return KD_OK;
+    double* em = s->buf<double>(s->e_m, n_prior);
+    double* ew = s->buf<double>(s->e_w, n_prior);
+    double* v = s->buf<double>(s->v_buf, n);
+    s->h2d(em, means, n_prior * sizeof(double));
+    s->h2d(ew, weights, n_prior * sizeof(double));
+    s->h2d(v, means + n_prior, n * sizeof(double));
+    std::vector<SegDesc> segs(1);
+    segs[0] = SegDesc{0, n, 0, n_prior, 0, 0, w_uniform, stored_n, nrm, 0.0};
+    std::vector<SegResult> res;
+    s->eng.compress(scale, segs, v, em, ew, res);
+    int64_t c = res[0].count;
+    s->d2h(out_means, s->eng.cent_means() + segs[0].x_off, c * sizeof(double));
+    s->d2h(out_weights, s->eng.cent_weights() + segs[0].x_off, c * sizeof(double));
+    s->sync();
+    *out_count = c;
+    return KD_OK;
+  });
+}
+
+static int run_query(kd_session* s, int is_cdf, int64_t nd, const int64_t* off,
+                     const double* means, const double* weights, const double* state,
+                     const int64_t* qd, const double* qs, int64_t nq, double* out) {
+  return guarded([&] {
+    if (nq == 0) return KD_OK;
+    int64_t nc = off[nd];
+    for (int64_t i = 0; i < nq; i++) {
+      if (qd[i] < 0 || qd[i] >= nd) throw KdError(KD_EINVAL, "query digest out of range");
+      if (off[qd[i] + 1] == off[qd[i]])
+        throw KdError(KD_EEMPTY, is_cdf ? "cdf of an empty digest is undefined"
+                                        : "quantile of an empty digest is undefined");
+      if (!is_cdf && (qs[i] < 0.0 || qs[i] > 1.0))
+        throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+    }
+    int64_t* doff = s->buf<int64_t>(s->q_off, nd + 1);
+    double* dm = s->buf<double>(s->q_m, nc);
+    double* dw = s->buf<double>(s->q_w, nc);
+    double* dS = s->buf<double>(s->q_s, nc);
+    double* dst = s->buf<double>(s->q_state, 4 * nd);
+    int64_t* dqd = s->buf<int64_t>(s->q_qd, nq);
+    double* dq = s->buf<double>(s->q_q, nq);
+    double* dout = s->buf<double>(s->q_out, nq);
+    s->h2d(doff, off, (nd + 1) * sizeof(int64_t));
+    s->h2d(dm, means, nc * sizeof(double));
+    s->h2d(dw, weights, nc * sizeof(double));
+    s->h2d(dst, state, 4 * nd * sizeof(double));
+    s->h2d(dqd, qd, nq * sizeof(int64_t));
+    s->h2d(dq, qs, nq * sizeof(double));
+    k_query_prefix<<<blocks_for(nd, 64), 64, 0, s->st()>>>(nd, doff, dw, dS);
+    k_query<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff, dm, dw, dS, dst, dqd, dq, nq, dout);
+    hip_ok(hipGetLastError(), "query launch");
+    s->d2h(out, dout, nq * sizeof(double));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_quantile(kd_session* s, int64_t nd, const int64_t* off, const double* means,
+                           const double* weights, const double* state, const int64_t* qd,
+                           const double* qs, int64_t nq, double* out) {
+  return run_query(s, 0, nd, off, means, weights, state, qd, qs, nq, out);
+}
+
+extern "C" int kd_cdf(kd_session* s, int64_t nd, const int64_t* off, const double* means,
+                      const double* weights, const double* state, const int64_t* qd,
+                      const double* xs, int64_t nq, double* out) {
+  return run_query(s, 1, nd, off, means, weights, state, qd, xs, nq, out);
+}
+
+extern "C" int kd_scale_eval(kd_session* s, int scale, double nrm, const double* q, int64_t n,
+                             double* out_k) {
+  return guarded([&] {
+    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+    if (n == 0) return KD_OK;
+    double* dq = s->buf<double>(s->tmp, n);
+    double* dk = s->buf<double>(s->tmp2, n);
+    s->h2d(dq, q, n * sizeof(double));
+    k_scale_eval<<<blocks_for(n), 256, 0, s->st()>>>(scale, nrm, dq, n, dk);
+    hip_ok(hipGetLastError(), "scale launch");
+    s->d2h(out_k, dk, n * sizeof(double));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_math_eval(kd_session* s, int fn, const double* x, int64_t n, double* out) {
+  return guarded([&] {
+    if (fn < 0 || fn > 2) throw KdError(KD_EINVAL, "fn must be 0 (log), 1 (asin) or 2 (sqrt)");
+    if (n == 0) return KD_OK;
+    double* dx = s->buf<double>(s->tmp, n);
+    double* dy = s->buf<double>(s->tmp2, n);
+    s->h2d(dx, x, n * sizeof(double));
+    k_math_eval<<<blocks_for(n), 256, 0, s->st()>>>(fn, dx, n, dy);
+    hip_ok(hipGetLastError(), "math launch");
+    s->d2h(out, dy, n * sizeof(double));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+// ================================================================ TDigest
+struct kd_tdigest {
+  kd_session* s;
+  int scale;
+  double compression, lambda;
+  int64_t buffer_limit;
+  double stored_total = 0.0, log_scale = 0.0, clock = 0.0, vmin = 0.0, vmax = 0.0;
+  std::vector<double> bm, bw;  // ingest buffer (stored weights)
+  DevBuf cm, cw;               // centroids (device)
+  int64_t nc = 0;
+
+  bool decay() const { return lambda > 0.0; }                       // decay.mojo:292-294
+  double scale_factor() const { return decay() ? std::exp(log_scale) : 1.0; }  // :352-357
+  double effective_count() const {                                   // :373-377
+    return decay() ? stored_total * std::exp(-log_scale) : stored_total;
+  }
+  double* dm() { return static_cast<double*>(cm.p); }
+  double* dw() { return static_cast<double*>(cw.p); }
+
+  void renormalize() {  // tdigest.mojo:359-371
+    double factor = std::exp(-log_scale);
+    if (nc > 0) {
+      k_scale_weights<<<blocks_for(nc), 256, 0, s->st()>>>(dw(), nc, factor);
+      hip_ok(hipGetLastError(), "renormalize launch");
+    }
+    for (double& w : bw) w *= factor;
+    stored_total *= factor;
+    log_scale = 0.0;
+  }
+
+  void advance_clock(double new_clock) {  // tdigest.mojo:324-332
+    if (decay() && new_clock > clock) {
+      log_scale += lambda * (new_clock - clock);
+      if (log_scale > kRenormThreshold) renormalize();
+    }
+    clock = new_clock;
+  }
+
+  // Compress E = centroids ++ buffer (++ device values with weight sf).
+  SegResult run(const double* d_values, int64_t n, double sf) {
+    int64_t ne = nc + (int64_t)bm.size();
+    double* em = s->buf<double>(s->e_m, ne);
+    double* ew = s->buf<double>(s->e_w, ne);
+    s->d2d(em, dm(), nc * sizeof(double));
+    s->d2d(ew, dw(), nc * sizeof(double));
+    s->h2d(em + nc, bm.data(), bm.size() * sizeof(double));
+    s->h2d(ew + nc, bw.data(), bw.size() * sizeof(double));
+    bm.clear();
+    bw.clear();
+    double eff = effective_count();
+    double nrm = kd_normalizer(scale, compression, eff);
+    double floor = decay() ? kPruneEpsilon * std::exp(log_scale) : 0.0;
+    std::vector<SegDesc> segs(1);
+    segs[0] = SegDesc{0, n, 0, ne, 0, 0, sf, stored_total, nrm, floor};
+    std::vector<SegResult> res;
+    s->eng.compress(scale, segs, d_values, em, ew, res);
+    int64_t c = res[0].count;
+    cm.reserve(std::max<int64_t>(c, 1) * sizeof(double));
+    cw.reserve(std::max<int64_t>(c, 1) * sizeof(double));
+    s->d2d(dm(), s->eng.cent_means() + segs[0].x_off, c * sizeof(double));
+    s->d2d(dw(), s->eng.cent_weights() + segs[0].x_off, c * sizeof(double));
+    nc = c;
+    return res[0];
+  }
+
+  void merge() {  // tdigest.mojo:391-439 (+ prune 438-439)
+    if (bm.empty()) return;
+    SegResult r = run(nullptr, 0, 1.0);
+    if (decay() && r.removed > 0.0) stored_total -= r.removed;
+  }
+
+  void insert(double value, double weight) {  // tdigest.mojo:334-350
+    if (stored_total == 0.0 && bm.empty()) {
+      vmin = value;
+      vmax = value;
+    } else {
+      if (value < vmin) vmin = value;
+      if (value > vmax) vmax = value;
+    }
+    double scaled = weight * scale_factor();
+    bm.push_back(value);
+    bw.push_back(scaled);
+    stored_total += scaled;
+    if ((int64_t)bm.size() >= buffer_limit) merge();
+  }
+
+  static void validate(double value, double weight) {  // tdigest.mojo:317-322
+    if (!is_finite_val(value)) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+    if (weight <= 0.0 || !is_finite_val(weight)) throw KdError(KD_EINVAL, "weight must be positive and finite");
+  }
+
+  // B semantics (tdigest.mojo:223-315) of n values already in device memory.
+  // vmin_in/vmax_in: the batch range when the caller knows it (host input),
+  // otherwise taken from the sorted batch.
+  void batch(const double* d_values, int64_t n, bool have_range, double bmin, double bmax) {
+    if (n == 0) return;
+    advance_clock(clock + (double)n);  // :258
+    double sf = scale_factor();        // :259
+    int64_t had_centroids = nc;
+    stored_total += (double)n * sf;    // :282
+    SegResult r = run(d_values, n, sf);
+    if (!have_range) {
+      bmin = r.vmin;
+      bmax = r.vmax;
+    }
+    if (stored_total == (double)n * sf && had_centroids == 0) {  // :285-295
+      vmin = bmin;
+      vmax = bmax;
+    } else {
+      if (bmin < vmin) vmin = bmin;
+      if (bmax > vmax) vmax = bmax;
+    }
+    stored_total = r.total;  // :304-315 (reconcile, then prune)
+  }
+
+  void query(int is_cdf, const double* qs, int64_t nq, double* out) {
+    if (nq == 0) return;
+    double* dq = s->buf<double>(s->q_q, nq);
+    double* dout = s->buf<double>(s->q_out, nq);
+    double* dS = s->buf<double>(s->q_s, nc);
+    int64_t* doff = s->buf<int64_t>(s->q_off, 2);
+    double* dst = s->buf<double>(s->q_state, 4);
+    int64_t* dqd = s->buf<int64_t>(s->q_qd, nq);
+    int64_t hoff[2] = {0, nc};
+    double hst[4] = {stored_total, vmin, vmax, decay() ? std::exp(-log_scale) : 1.0};
+    s->h2d(doff, hoff, sizeof hoff);
+    s->h2d(dst, hst, sizeof hst);
+    s->h2d(dq, qs, nq * sizeof(double));
+    hip_ok(hipMemsetAsync(dqd, 0, nq * sizeof(int64_t), s->st()), "memset");
+    k_query_prefix<<<1, 64, 0, s->st()>>>(1, doff, dw(), dS);
+    k_query<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff, dm(), dw(), dS, dst, dqd, dq, nq, dout);
+    hip_ok(hipGetLastError(), "query launch");
+    s->d2h(out, dout, nq * sizeof(double));
+    s->sync();
+  }
+
+  void download(std::vector<double>& m, std::vector<double>& w) {
+    m.resize(nc);
+    w.resize(nc);
+    s->d2h(m.data(), dm(), nc * sizeof(double));
+    s->d2h(w.data(), dw(), nc * sizeof(double));
+    s->sync();
+  }
+
+  void upload(const double* m, const double* w, int64_t n) {
+    cm.reserve(std::max<int64_t>(n, 1) * sizeof(double));
+    cw.reserve(std::max<int64_t>(n, 1) * sizeof(double));
+    s->h2d(dm(), m, n * sizeof(double));
+    s->h2d(dw(), w, n * sizeof(double));
+    s->sync();
+    nc = n;
+  }
+};
+
+static kd_tdigest* new_digest(kd_session* s, int scale, double compression, double lambda) {
+  if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+  auto* d = new kd_tdigest();
+  d->s = s;
+  d->scale = scale;
+  d->compression = compression;
+  d->lambda = lambda;
+  d->buffer_limit = (int64_t)compression * 5;  // tdigest.mojo:146
+  return d;
+}
+
+extern "C" int kd_tdigest_create(kd_session* s, int scale, double compression, double lambda,
+                                 kd_tdigest** out) {
+  return guarded([&] {
+    *out = new_digest(s, scale, compression, lambda);
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_tdigest_destroy(kd_tdigest* d) { delete d; }
+
+extern "C" int kd_tdigest_add(kd_tdigest* d, double value, double weight) {
+  return guarded([&] {
+    kd_tdigest::validate(value, weight);
+    d->advance_clock(d->clock + 1.0);
+    d->insert(value, weight);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_add_at(kd_tdigest* d, double value, double timestamp, double weight) {
+  return guarded([&] {
+    kd_tdigest::validate(value, weight);
+    if (timestamp > d->clock) d->advance_clock(timestamp);
+    d->insert(value, weight);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_add_batch(kd_tdigest* d, const double* values, int64_t n,
+                                    int with_session) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    int64_t cross = with_session ? kCrossoverSession : kCrossover;
+    if (n < cross) {  // tdigest.mojo:199-202: the per-sample path
+      for (int64_t i = 0; i < n; i++) {
+        kd_tdigest::validate(values[i], 1.0);
+        d->advance_clock(d->clock + 1.0);
+        d->insert(values[i], 1.0);
+      }
+      return KD_OK;
+    }
+    double bmin = values[0], bmax = values[0];
+    for (int64_t i = 0; i < n; i++) {  // :253-254 validate everything first
+      kd_tdigest::validate(values[i], 1.0);
+      if (values[i] < bmin) bmin = values[i];
+      if (values[i] > bmax) bmax = values[i];
+    }
+    double* v = d->s->buf<double>(d->s->v_buf, n);
+    d->s->h2d(v, values, n * sizeof(double));
+    d->batch(v, n, true, bmin, bmax);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_add_batch_device(kd_tdigest* d, const double* d_values, int64_t n) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    int* flag = d->s->buf<int>(d->s->tmp, 1);
+    hip_ok(hipMemsetAsync(flag, 0, sizeof(int), d->s->st()), "memset");
+    k_validate<<<blocks_for(n), 256, 0, d->s->st()>>>(nullptr, d_values, n, 0, flag);
+    int hflag = 0;
+    d->s->d2h(&hflag, flag, sizeof(int));
+    d->s->sync();
+    if (hflag) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+    d->batch(d_values, n, false, 0.0, 0.0);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_merge(kd_tdigest* d) {
+  return guarded([&] {
+    d->merge();
+    d->s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_quantile(kd_tdigest* d, const double* qs, int64_t nq, double* out) {
+  return guarded([&] {
+    for (int64_t i = 0; i < nq; i++)  // :492-493 before the merge
+      if (qs[i] < 0.0 || qs[i] > 1.0) throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+    d->merge();
+    if (d->nc == 0) throw KdError(KD_EEMPTY, "quantile of an empty digest is undefined");
+    d->query(0, qs, nq, out);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_cdf(kd_tdigest* d, const double* xs, int64_t nq, double* out) {
+  return guarded([&] {
+    d->merge();
+    if (d->nc == 0) throw KdError(KD_EEMPTY, "cdf of an empty digest is undefined");
+    d->query(1, xs, nq, out);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_merge_into(kd_tdigest* self, kd_tdigest* other) {
+  return guarded([&] {
+    // tdigest.mojo:589-623
+    other->merge();
+    if (other->effective_count() == 0.0) return KD_OK;
+    self->merge();
+    if (self->decay()) self->renormalize();
+    if (other->decay()) other->renormalize();
+    bool was_empty = self->stored_total == 0.0;
+    std::vector<double> om, ow;
+    other->download(om, ow);
+    self->bm.insert(self->bm.end(), om.begin(), om.end());
+    self->bw.insert(self->bw.end(), ow.begin(), ow.end());
+    self->stored_total += other->stored_total;
+    if (was_empty) {
+      self->vmin = other->vmin;
+      self->vmax = other->vmax;
+    } else {
+      self->vmin = std::min(self->vmin, other->vmin);
+      self->vmax = std::max(self->vmax, other->vmax);
+    }
+    self->clock = std::max(self->clock, other->clock);
+    self->merge();
+    self->s->sync();
+    return KD_OK;
+  });
+}
+
+static void put_f64(uint8_t* p, double v) {  // tdigest.mojo:80-84
+  uint64_t b = kd_d2bits(v);
+  for (int i = 0; i < 8; i++) p[i] = (uint8_t)((b >> (8 * i)) & 0xff);
+}
+
+static int64_t serialize(const double* hdr7, const std::vector<double>& m,
+                         const std::vector<double>& w, uint8_t* out, int64_t cap) {
+  int64_t need = 1 + 8 * 8 + 16 * (int64_t)m.size();
+  if (!out || cap < need) return need;
+  out[0] = kSerdeVersion;
+  for (int i = 0; i < 7; i++) put_f64(out + 1 + 8 * i, hdr7[i]);
+  put_f64(out + 57, (double)m.size());
+  for (size_t i = 0; i < m.size(); i++) {
+    put_f64(out + 65 + 16 * i, m[i]);
+    put_f64(out + 73 + 16 * i, w[i]);
+  }
+  return need;
+}
+
+extern "C" int64_t kd_tdigest_to_bytes(kd_tdigest* d, uint8_t* out, int64_t cap) {
+  int64_t r = 0;
+  int rc = guarded([&] {
+    d->merge();  // :638
+    std::vector<double> m, w;
+    d->download(m, w);
+    double hdr[7] = {d->compression, d->lambda, d->log_scale, d->clock, d->stored_total, d->vmin, d->vmax};
+    r = serialize(hdr, m, w, out, cap);
+    return KD_OK;
+  });
+  return rc == KD_OK ? r : rc;
+}
+
+static int get_f64(const uint8_t* data, int64_t n, int64_t off, double* v) {  // :87-94
+  if (off + 8 > n) return KD_ETRUNC;
+  uint64_t b = 0;
+  for (int i = 0; i < 8; i++) b |= (uint64_t)data[off + i] << (8 * i);
+  *v = kd_bits2d(b);
+  return KD_OK;
+}
+
+extern "C" int kd_tdigest_from_bytes(kd_session* s, const uint8_t* data, int64_t n, int scale,
+                                     kd_tdigest** out) {
+  return guarded([&] {
+    if (n == 0 || data[0] != kSerdeVersion)
+      throw KdError(KD_ESERDE, "unrecognized digest serialization version");
+    double h[8];
+    for (int i = 0; i < 8; i++)
+      if (get_f64(data, n, 1 + 8 * i, &h[i]))
+        throw KdError(KD_ETRUNC, "truncated digest: not enough bytes for a Float64");
+    int64_t count = (int64_t)h[7];
+    std::vector<double> m(std::max<int64_t>(count, 0)), w(std::max<int64_t>(count, 0));
+    int64_t off = 65;
+    for (int64_t i = 0; i < count; i++, off += 16)
+      if (get_f64(data, n, off, &m[i]) || get_f64(data, n, off + 8, &w[i]))
+        throw KdError(KD_ETRUNC, "truncated digest: not enough bytes for a Float64");
+    std::unique_ptr<kd_tdigest> d(new_digest(s, scale, h[0], h[1]));
+    d->upload(m.data(), w.data(), (int64_t)m.size());
+    d->log_scale = h[2];
+    d->clock = h[3];
+    d->stored_total = h[4];
+    d->vmin = h[5];
+    d->vmax = h[6];
+    *out = d.release();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_from_centroids(kd_session* s, int scale, double compression,
+                                         const double* means, const double* weights, int64_t n,
+                                         double min_v, double max_v, kd_tdigest** out) {
+  return guarded([&] {
+    std::unique_ptr<kd_tdigest> d(new_digest(s, scale, compression, 0.0));
+    double total = 0.0;
+    for (int64_t i = 0; i < n; i++) total += weights[i];  // :716-722
+    d->upload(means, weights, n);
+    d->stored_total = total;
+    d->clock = total;
+    d->vmin = min_v;
+    d->vmax = max_v;
+    *out = d.release();
+    return KD_OK;
+  });
+}
+
+extern "C" int64_t kd_tdigest_size(kd_tdigest* d) { return d->nc; }
+
+extern "C" int kd_tdigest_centroids(kd_tdigest* d, double* means, double* weights, int64_t cap,
+                                    int64_t* count) {
+  return guarded([&] {
+    *count = d->nc;
+    if (cap < d->nc) return KD_OK;
+    d->s->d2h(means, d->dm(), d->nc * sizeof(double));
+    d->s->d2h(weights, d->dw(), d->nc * sizeof(double));
+    d->s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_state(kd_tdigest* d, double* o) {
+  o[0] = d->stored_total;
+  o[1] = d->log_scale;
+  o[2] = d->clock;
+  o[3] = d->vmin;
+  o[4] = d->vmax;
+  o[5] = d->compression;
+  o[6] = d->lambda;
+  o[7] = d->effective_count();
+  o[8] = (double)d->bm.size();
+  return KD_OK;
+}
+
+// ========================================================= GroupedTDigest
+struct kd_grouped {
+  kd_session* s;
+  int scale;
+  int64_t num_keys;
+  double compression, lambda;
+  std::vector<double> stored_total, log_scale, clock, vmin, vmax;
+  std::vector<int64_t> hoff;      // host copy of the CSR offsets
+  DevBuf off, cm, cw, S;          // device CSR of centroids + query prefix
+  DevBuf noff, ncm, ncw;          // next CSR (swapped in after a batch)
+  DevBuf counts, koff, vals, keysbuf, valsbuf, segof, xoff, qstate, qtmp;
+  bool prefix_valid = false;
+
+  bool decay() const { return lambda > 0.0; }
+  int64_t* doff() { return static_cast<int64_t*>(off.p); }
+  double* dm() { return static_cast<double*>(cm.p); }
+  double* dw() { return static_cast<double*>(cw.p); }
+  int64_t count(int64_t k) const { return hoff[k + 1] - hoff[k]; }
+  double eff(int64_t k) const { return decay() ? stored_total[k] * std::exp(-log_scale[k]) : stored_total[k]; }
+
+  void check_key(int64_t k) const {
+    if (k < 0 || k >= num_keys) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+  }
+
+  void add_device(const uint32_t* d_keys, const double* d_values, int64_t n);
+  void query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq, double* d_out);
+};
+
+extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, double compression,
+                                 double lambda, kd_grouped** out) {
+  return guarded([&] {
+    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+    if (num_keys <= 0 || num_keys > 0xffffffffLL) throw KdError(KD_EINVAL, "num_keys out of range");
+    std::unique_ptr<kd_grouped> g(new kd_grouped());
+    g->s = s;
+    g->scale = scale;
+    g->num_keys = num_keys;
+    g->compression = compression;
+    g->lambda = lambda;
+    g->stored_total.assign(num_keys, 0.0);
+    g->log_scale.assign(num_keys, 0.0);
+    g->clock.assign(num_keys, 0.0);
+    g->vmin.assign(num_keys, 0.0);
+    g->vmax.assign(num_keys, 0.0);
+    g->hoff.assign(num_keys + 1, 0);
+    g->off.reserve((num_keys + 1) * sizeof(int64_t));
+    hip_ok(hipMemsetAsync(g->off.p, 0, (num_keys + 1) * sizeof(int64_t), s->st()), "memset");
+    g->cm.reserve(sizeof(double));
+    g->cw.reserve(sizeof(double));
+    s->sync();
+    *out = g.release();
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_grouped_destroy(kd_grouped* g) { delete g; }
+
+void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int64_t n) {
+  // grouped.mojo:60-162.  Validate the whole batch first (:91-97).
+  int* flag = s->buf<int>(s->tmp, 1);
+  hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
+  k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
+  int hflag = 0;
+  s->d2h(&hflag, flag, sizeof(int));
+  s->sync();
+  if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
+
+  // Group per key (:99-104) on the device.
+  int64_t* dcnt = static_cast<int64_t*>(counts.reserve(num_keys * sizeof(int64_t)));
+  int64_t* dkoff = static_cast<int64_t*>(koff.reserve((num_keys + 1) * sizeof(int64_t)));
+  double* dv = static_cast<double*>(vals.reserve(std::max<int64_t>(n, 1) * sizeof(double)));
+  s->eng.key_partition(d_keys, d_values, n, num_keys, dcnt, dkoff, dv);
+  std::vector<int64_t> kcnt(num_keys), kof(num_keys + 1);
+  s->d2h(kcnt.data(), dcnt, num_keys * sizeof(int64_t));
+  s->d2h(kof.data(), dkoff, (num_keys + 1) * sizeof(int64_t));
+  s->sync();
+
+  // Per touched key: B-semantics prepare (tdigest.mojo:236-296).
+  std::vector<SegDesc> segs;
+  std::vector<int64_t> seg_of(num_keys, -1), seg_key;
+  std::vector<int64_t> rkeys;
+  std::vector<double> rfac;
+  std::vector<double> sfk;
+  std::vector<char> had_cent;
+  for (int64_t k = 0; k < num_keys; k++) {
+    int64_t m = kcnt[k];
+    if (m == 0) continue;
+    double nc_clock = clock[k] + (double)m;
+    if (decay() && nc_clock > clock[k]) {  // advance_clock (:324-332)
+      log_scale[k] += lambda * (nc_clock - clock[k]);
+      if (log_scale[k] > kRenormThreshold) {  // _renormalize (:359-371)
+        double factor = std::exp(-log_scale[k]);
+        rkeys.push_back(k);
+        rfac.push_back(factor);
+        stored_total[k] *= factor;
+        log_scale[k] = 0.0;
+      }
+    }
+    clock[k] = nc_clock;
+    double sf = decay() ? std::exp(log_scale[k]) : 1.0;
+    had_cent.push_back(count(k) > 0);
+    stored_total[k] += (double)m * sf;
+    double effn = decay() ? stored_total[k] * std::exp(-log_scale[k]) : stored_total[k];
+    SegDesc sd{};
+    sd.v_off = kof[k];
+    sd.v_cnt = m;
+    sd.e_off = hoff[k];
+    sd.e_cnt = count(k);
+    sd.sf = sf;
+    sd.stored_n = stored_total[k];
+    sd.nrm = kd_normalizer(scale, compression, effn);
+    sd.floor = decay() ? kPruneEpsilon * std::exp(log_scale[k]) : 0.0;
+    seg_of[k] = (int64_t)segs.size();
+    segs.push_back(sd);
+    seg_key.push_back(k);
+    sfk.push_back(sf);
+  }
+  if (segs.empty()) return;
+  if (!rkeys.empty()) {
+    int64_t* dk = s->buf<int64_t>(s->q_qd, rkeys.size());
+    double* df = s->buf<double>(s->q_q, rfac.size());
+    s->h2d(dk, rkeys.data(), rkeys.size() * sizeof(int64_t));
+    s->h2d(df, rfac.data(), rfac.size() * sizeof(double));
+    k_scale_keys<<<(unsigned)rkeys.size(), 64, 0, s->st()>>>(dk, df, (int64_t)rkeys.size(), doff(), dw());
+    hip_ok(hipGetLastError(), "renormalize launch");
+  }
+  std::vector<SegResult> res;
+  s->eng.compress(scale, segs, dv, dm(), dw(), res);
+
+  // New CSR: untouched keys keep their centroids.
+  std::vector<int64_t> noffh(num_keys + 1, 0);
+  for (int64_t k = 0; k < num_keys; k++) {
+    int64_t c = seg_of[k] >= 0 ? res[seg_of[k]].count : count(k);
+    noffh[k + 1] = noffh[k] + c;
+  }
+  int64_t total = noffh[num_keys];
+  std::vector<int64_t> xo(segs.size());
+  for (size_t i = 0; i < segs.size(); i++) xo[i] = segs[i].x_off;
+  int64_t* dno = static_cast<int64_t*>(noff.reserve((num_keys + 1) * sizeof(int64_t)));
+  double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
+  double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
+  int64_t* dseg = static_cast<int64_t*>(segof.reserve(num_keys * sizeof(int64_t)));
+  int64_t* dxo = static_cast<int64_t*>(xoff.reserve(std::max<size_t>(xo.size(), 1) * sizeof(int64_t)));
+  s->h2d(dno, noffh.data(), (num_keys + 1) * sizeof(int64_t));
+  s->h2d(dseg, seg_of.data(), num_keys * sizeof(int64_t));
+  s->h2d(dxo, xo.data(), xo.size() * sizeof(int64_t));
+  k_gather_csr<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dm(), dw(), dseg, dxo,
+                                                          s->eng.cent_means(), s->eng.cent_weights(),
+                                                          dno, dnm, dnw);
+  hip_ok(hipGetLastError(), "gather launch");
+  s->sync();
+  std::swap(off, noff);
+  std::swap(cm, ncm);
+  std::swap(cw, ncw);
+  hoff.swap(noffh);
+  prefix_valid = false;
+
+  // Per key: min/max (:285-295) and the reconciled stored total (:298-315).
+  for (size_t i = 0; i < segs.size(); i++) {
+    int64_t k = seg_key[i];
+    double m = (double)segs[i].v_cnt;
+    if (stored_total[k] == m * sfk[i] && !had_cent[i]) {
+      vmin[k] = res[i].vmin;
+      vmax[k] = res[i].vmax;
+    } else {
+      if (res[i].vmin < vmin[k]) vmin[k] = res[i].vmin;
+      if (res[i].vmax > vmax[k]) vmax[k] = res[i].vmax;
+    }
+    stored_total[k] = res[i].total;
+  }
+}
+
+void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq,
+                       double* d_out) {
+  int64_t total = hoff[num_keys];
+  double* dS = static_cast<double*>(S.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
+  double* st = static_cast<double*>(qstate.reserve(4 * num_keys * sizeof(double)));
+  if (!prefix_valid) {
+    k_query_prefix<<<blocks_for(num_keys, 64), 64, 0, s->st()>>>(num_keys, doff(), dw(), dS);
+    std::vector<double> hs(4 * num_keys);
+    for (int64_t k = 0; k < num_keys; k++) {
+      hs[k] = stored_total[k];
+      hs[num_keys + k] = vmin[k];
+      hs[2 * num_keys + k] = vmax[k];
+      hs[3 * num_keys + k] = decay() ? std::exp(-log_scale[k]) : 1.0;
+    }
+    s->h2d(st, hs.data(), hs.size() * sizeof(double));
+    prefix_valid = true;
+  }
+  k_query_grouped<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff(), dm(), dw(), dS, st,
+                                                       st + num_keys, st + 2 * num_keys,
+                                                       st + 3 * num_keys, d_keys, d_q, nq, d_out);
+  hip_ok(hipGetLastError(), "grouped query launch");
+}
+
+extern "C" int kd_grouped_add_batch(kd_grouped* g, const uint32_t* keys, const double* values,
+                                    int64_t n) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    for (int64_t i = 0; i < n; i++) {  // grouped.mojo:91-97
+      if ((int64_t)keys[i] >= g->num_keys) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+      if (!is_finite_val(values[i])) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+    }
+    uint32_t* dk = static_cast<uint32_t*>(g->keysbuf.reserve(n * sizeof(uint32_t)));
+    double* dv = static_cast<double*>(g->valsbuf.reserve(n * sizeof(double)));
+    g->s->h2d(dk, keys, n * sizeof(uint32_t));
+    g->s->h2d(dv, values, n * sizeof(double));
+    g->add_device(dk, dv, n);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_add_batch_device(kd_grouped* g, const uint32_t* d_keys,
+                                           const double* d_values, int64_t n) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    g->add_device(d_keys, d_values, n);
+    return KD_OK;
+  });
+}
+
+static int grouped_query_host(kd_grouped* g, int is_cdf, const int64_t* keys, const double* qs,
+                              int64_t nq, double* out) {
+  return guarded([&] {
+    if (nq <= 0) return KD_OK;
+    for (int64_t i = 0; i < nq; i++) {
+      g->check_key(keys[i]);
+      if (!is_cdf && (qs[i] < 0.0 || qs[i] > 1.0))
+        throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+      if (g->count(keys[i]) == 0)
+        throw KdError(KD_EEMPTY, is_cdf ? "cdf of an empty digest is undefined"
+                                        : "quantile of an empty digest is undefined");
+    }
+    kd_session* s = g->s;
+    int64_t* dk = s->buf<int64_t>(s->q_qd, nq);
+    double* dq = s->buf<double>(s->q_q, nq);
+    double* dout = s->buf<double>(s->q_out, nq);
+    s->h2d(dk, keys, nq * sizeof(int64_t));
+    s->h2d(dq, qs, nq * sizeof(double));
+    g->query(is_cdf, dk, dq, nq, dout);
+    s->d2h(out, dout, nq * sizeof(double));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_quantile(kd_grouped* g, const int64_t* keys, const double* qs,
+                                   int64_t nq, double* out) {
+  return grouped_query_host(g, 0, keys, qs, nq, out);
+}
+
+extern "C" int kd_grouped_cdf(kd_grouped* g, const int64_t* keys, const double* xs, int64_t nq,
+                              double* out) {
+  return grouped_query_host(g, 1, keys, xs, nq, out);
+}
+
+extern "C" int kd_grouped_quantile_device(kd_grouped* g, const int64_t* d_keys, const double* d_qs,
+                                          int64_t nq, double* d_out) {
+  return guarded([&] {
+    if (nq <= 0) return KD_OK;
+    g->query(0, d_keys, d_qs, nq, d_out);
+    return KD_OK;
+  });
+}
+
+extern "C" int64_t kd_grouped_to_bytes(kd_grouped* g, int64_t key, uint8_t* out, int64_t cap) {
+  int64_t r = 0;
+  int rc = guarded([&] {
+    g->check_key(key);
+    int64_t c = g->count(key);
+    std::vector<double> m(c), w(c);
+    g->s->d2h(m.data(), g->dm() + g->hoff[key], c * sizeof(double));
+    g->s->d2h(w.data(), g->dw() + g->hoff[key], c * sizeof(double));
+    g->s->sync();
+    double hdr[7] = {g->compression, g->lambda, g->log_scale[key], g->clock[key],
+                     g->stored_total[key], g->vmin[key], g->vmax[key]};
+    r = serialize(hdr, m, w, out, cap);
+    return KD_OK;
+  });
+  return rc == KD_OK ? r : rc;
+}
+
+extern "C" int kd_grouped_centroid_count(kd_grouped* g, int64_t key, int64_t* out) {
+  return guarded([&] {
+    g->check_key(key);
+    *out = g->count(key);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_total_weight(kd_grouped* g, int64_t key, double* out) {
+  return guarded([&] {
+    g->check_key(key);
+    *out = g->eff(key);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_centroids(kd_grouped* g, int64_t key, double* means, double* weights,
+                                    int64_t cap, int64_t* count) {
+  return guarded([&] {
+    g->check_key(key);
+    int64_t c = g->count(key);
+    *count = c;
+    if (cap < c) return KD_OK;
+    g->s->d2h(means, g->dm() + g->hoff[key], c * sizeof(double));
+    g->s->d2h(weights, g->dw() + g->hoff[key], c * sizeof(double));
+    g->s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_key_state(kd_grouped* g, int64_t key, double* o) {
+  return guarded([&] {
+    g->check_key(key);
+    o[0] = g->stored_total[key];
+    o[1] = g->log_scale[key];
+    o[2] = g->clock[key];
+    o[3] = g->vmin[key];
+    o[4] = g->vmax[key];
+    return KD_OK;
+  });
+}
+
+// ================================================ host scale functions
+extern "C" double kd_scale_normalizer(int scale, double compression, double n) {
+  return kd_normalizer(scale, compression, n);
+}
+
+extern "C" double kd_scale_k(int scale, double q, double nrm) { return kd_k(scale, q, nrm); }
+
+// Inverse maps of scale.mojo:139-142, 163-167, 195-200, 229-235.
+extern "C" double kd_scale_q(int scale, double k, double nrm) {
+  switch (scale) {
+    case KD_K0: return k / nrm;
+    case KD_K1: return (std::sin(k / nrm) + 1.0) / 2.0;
+    case KD_K2: {
+      double w = std::exp(k / nrm);
+      return w / (1.0 + w);
+    }
+    default:
+      if (k <= 0.0) return std::exp(k / nrm) / 2.0;
+      return 1.0 - std::exp(-k / nrm) / 2.0;
+  }
+}

@@ -1,0 +1,95 @@
+// kd_engine.h — internal C++ interface of the gfx950 digest engine.
+//
+// The engine runs the B-semantics compress of many segments at once.  A
+// segment is one digest (or one key of a grouped digest) and holds
+//   * explicit records E (means + weights, in input order: centroids ++ buffer),
+//   * new values V (contiguous, unsorted) that all carry the weight sf.
+// For each segment it produces the greedy-sweep clustering of the stable,
+// mean-sorted union E ++ V (kdigest/tdigest.mojo:391-439), the prune of
+// tdigest.mojo:441-458 and the stored-total reconcile of tdigest.mojo:298-315.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace kd {
+
+// LDS capacities (doubles) of the bucket sort and of the E sort.
+constexpr int kBlock = 256;
+constexpr int kCap = 4096;      // one LDS sort tile of new values
+constexpr int kECap = 4096;     // explicit records per segment
+constexpr int kMaxSpl = 511;    // splitters per partition part
+constexpr int kMaxBkt = 2 * kMaxSpl + 1;
+constexpr int kTile = 1024;     // records per checkpoint tile of the cluster sums
+
+struct SegDesc {
+  int64_t v_off, v_cnt;   // new values in the V buffer
+  int64_t e_off, e_cnt;   // explicit records (input order)
+  int64_t x_off;          // merged region [x_off, x_off + e_cnt + v_cnt), also centroid output
+  int64_t t_off;          // first checkpoint tile of the segment
+  double sf, stored_n, nrm, floor;
+};
+
+struct SegResult {
+  int64_t count;    // centroids after prune
+  double total;     // stored total after reconcile and prune (B semantics)
+  double removed;   // weight dropped by the prune (A semantics subtracts it)
+  double vmin, vmax;// min / max of the new values (unused if v_cnt == 0)
+  int64_t replays;  // tie decisions resolved by an exact replay of the running mean
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* reserve(size_t bytes);
+  void release();
+  ~DevBuf() { release(); }
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+
+  // Compress segments.  d_v: new values, concatenated in segment order at
+  // SegDesc::v_off.  d_em/d_ew: explicit records at SegDesc::e_off.  The
+  // caller fills v_cnt/e_cnt/sf/stored_n/nrm/floor; the engine assigns the
+  // offsets.  Results stay on the device (cent_means()/cent_weights() at
+  // SegDesc::x_off) and per-segment summaries are returned in `res`.
+  void compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
+                const double* d_em, const double* d_ew, std::vector<SegResult>& res,
+                bool sync_results = true);
+
+  const double* cent_means() const { return static_cast<const double*>(cm_.p); }
+  const double* cent_weights() const { return static_cast<const double*>(cw_.p); }
+  const double* sorted_records() const { return static_cast<const double*>(x_.p); }
+  const SegResult* dev_results() const { return static_cast<const SegResult*>(res_.p); }
+
+  // Key partition of (key, value) pairs: counts per key and values grouped by
+  // key (batch order within a key is not kept: all of a key's new values carry
+  // the same weight, so their order among equals cannot change the result).
+  void key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
+                     int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out);
+
+  DevBuf& scratch(int i) { return extra_[i]; }
+
+ private:
+  void sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v);
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  DevBuf seg_, res_, em_s_, ew_s_, se_, pose_, vs_, x_, cm_, cw_, cs_, cnt_, t1_, t2_;
+  DevBuf parts_, spl_, nspl_, bcnt_, bcur_, tiles_, fb_, tsum_, csum_, order_, clist_;
+  DevBuf extra_[4];
+  // Host staging that must outlive the asynchronous copies reading it.
+  std::vector<char> h_parts_, h_tiles_, h_finals_;
+};
+
+void check_hip(hipError_t e, const char* what);
+
+}  // namespace kd

@@ -1,0 +1,1085 @@
+// kd_engine.hip — gfx950 kernels for the FP64 digest path.
+//
+// Pipeline of one compress (all segments of a call in the same launches):
+//   1. k_sort_e         stable LDS sort of each segment's explicit records E
+//   2. value sort       segments <= kCap: one LDS bitonic tile; larger ones:
+//                       sample-sort partition rounds (splitters from a sorted
+//                       sample, equality buckets for repeated splitters) until
+//                       every bucket fits one LDS tile, then k_bucket_sort
+//   3. k_merge_e        merge-path of sorted E into sorted V (E first on ties)
+//   4. k_chain          greedy-sweep cluster boundaries, one lane per segment
+//   5. k_tile_sums /    double-double sums of w*x per 1024-record tile, per
+//      k_tile_scan      segment checkpoint prefixes, then
+//      k_cluster_mean   per-cluster mean = sum / weight
+//   6. k_finish         stored-total reconcile and prune per segment
+//
+// All arithmetic that decides a cluster boundary runs in the same order and
+// with the same rounding as the CPU sweep (kdigest/tdigest.mojo:414-435);
+// the device is compiled with -ffp-contract=off.
+#include "kd_engine.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+#include "kd_math.h"
+
+namespace kd {
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+void* DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return p;
+  release();
+  size_t want = std::max<size_t>(bytes, 256);
+  check_hip(hipMalloc(&p, want), "hipMalloc");
+  cap = want;
+  return p;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+// ----------------------------------------------------------------- helpers
+struct dd {
+  double hi, lo;
+};
+
+__device__ inline dd two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+
+__device__ inline dd dd_add(dd a, dd b) {
+  dd s = two_sum(a.hi, b.hi);
+  dd t = two_sum(a.lo, b.lo);
+  double lo = s.lo + t.hi;
+  dd u = two_sum(s.hi, lo);
+  lo = u.lo + t.lo;
+  return two_sum(u.hi, lo);
+}
+
+__device__ inline dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+
+__device__ inline dd two_prod(double a, double b) {
+  double p = a * b;
+  double e = __builtin_fma(a, b, -p);
+  return {p, e};
+}
+
+// (a.hi + a.lo) / b rounded to double (one correction step).
+__device__ inline double dd_div(dd a, double b) {
+  double q = a.hi / b;
+  dd qb = two_prod(q, b);
+  double r = ((a.hi - qb.hi) - qb.lo) + a.lo;
+  return q + r / b;
+}
+
+// Total order key of a finite double with -0.0 and +0.0 mapped together.
+__device__ inline uint64_t ord_key(double x) {
+  uint64_t b = (x == 0.0) ? 0ull : kd_d2bits(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ inline void bitonic_lds(double* s, int npad) {
+  for (int k = 2; k <= npad; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < (npad >> 1); t += blockDim.x) {
+        int i = 2 * j * (t / j) + (t % j);
+        int l = i + j;
+        double a = s[i], b = s[l];
+        double lo = a < b ? a : b;
+        double hi = a < b ? b : a;
+        bool asc = (i & k) == 0;
+        s[i] = asc ? lo : hi;
+        s[l] = asc ? hi : lo;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ inline int pow2_at_least(int64_t n) {
+  int p = 2;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// ---------------------------------------------------- 1. explicit records
+__global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg,
+                                                    const double* em, const double* ew,
+                                                    double* esm, double* esw, double* se) {
+  __shared__ double key[kECap];
+  __shared__ int idx[kECap];
+  int s = blockIdx.x;
+  if (s >= nseg) return;
+  SegDesc sd = segs[s];
+  int n = (int)sd.e_cnt;
+  if (n == 0) {
+    if (threadIdx.x == 0) se[sd.e_off + s] = 0.0;
+    return;
+  }
+  int npad = pow2_at_least(n);
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+    key[i] = i < n ? em[sd.e_off + i] : __builtin_inf();
+    idx[i] = i < n ? i : 0x7fffffff;
+  }
+  __syncthreads();
+  // Bitonic on (mean, input index): the stable order of the reference's
+  // insertion sort (tdigest.mojo:64-77).
+  for (int k = 2; k <= npad; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < (npad >> 1); t += blockDim.x) {
+        int i = 2 * j * (t / j) + (t % j);
+        int l = i + j;
+        double a = key[i], b = key[l];
+        int ia = idx[i], ib = idx[l];
+        bool a_first = (a < b) || (!(b < a) && ia < ib);
+        bool asc = (i & k) == 0;
+        bool swap = asc ? !a_first : a_first;
+        if (swap) {
+          key[i] = b;
+          key[l] = a;
+          idx[i] = ib;
+          idx[l] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    esm[sd.e_off + i] = em[sd.e_off + idx[i]];
+    esw[sd.e_off + i] = ew[sd.e_off + idx[i]];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    double* p = se + sd.e_off + s;
+    p[0] = 0.0;
+    for (int i = 0; i < n; i++) {
+      acc += ew[sd.e_off + idx[i]];
+      p[i + 1] = acc;
+    }
+  }
+}
+
+// ------------------------------------------------------ 2. value sorting
+struct Part {
+  int32_t seg, src;
+  int64_t off, cnt;
+};
+struct PTile {
+  int32_t part, len;
+  int64_t start;
+};
+struct FBucket {
+  int32_t seg, src, is_eq, pad;
+  int64_t off, cnt, rank;
+};
+
+constexpr int kPTile = 4096;       // elements per partition tile
+constexpr int kPerThread = kPTile / kBlock;
+
+__device__ inline const double* pick(int id, const double* a, const double* b, const double* c) {
+  return id == 0 ? a : (id == 1 ? b : c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_splitters(const Part* parts, const double* s0,
+                                                       const double* s1, const double* s2,
+                                                       double* spl, int* nspl) {
+  __shared__ double s[kCap];
+  Part p = parts[blockIdx.x];
+  const double* src = pick(p.src, s0, s1, s2) + p.off;
+  int S = (int)std::min<int64_t>(p.cnt, kCap);
+  int npad = pow2_at_least(S);
+  for (int i = threadIdx.x; i < npad; i += blockDim.x)
+    s[i] = i < S ? src[((int64_t)i * p.cnt) / S] : __builtin_inf();
+  __syncthreads();
+  bitonic_lds(s, npad);
+  if (threadIdx.x == 0) {
+    int64_t want = p.cnt / (kCap / 4);
+    int ns = (int)std::max<int64_t>(1, std::min<int64_t>(want, kMaxSpl));
+    int k = 0;
+    double* out = spl + (int64_t)blockIdx.x * kMaxSpl;
+    for (int j = 0; j < ns; j++) {
+      double v = s[((int64_t)(j + 1) * S) / (ns + 1)];
+      if (k == 0 || out[k - 1] < v) out[k++] = v;
+    }
+    nspl[blockIdx.x] = k;
+  }
+}
+
+// Bucket of x: 2r for the open interval (spl[r-1], spl[r]), 2r+1 for x == spl[r].
+__device__ inline int classify(const double* spl, int ns, double x) {
+  int lo = 0, hi = ns;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (spl[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < ns && spl[lo] == x) ? 2 * lo + 1 : 2 * lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_hist(const PTile* tiles, const Part* parts,
+                                                         const double* s0, const double* s1,
+                                                         const double* s2, const double* spl,
+                                                         const int* nspl,
+                                                         unsigned long long* bcnt) {
+  __shared__ double lspl[kMaxSpl];
+  __shared__ unsigned int lcnt[kMaxBkt];
+  PTile t = tiles[blockIdx.x];
+  Part p = parts[t.part];
+  int ns = nspl[t.part];
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) lspl[i] = spl[(int64_t)t.part * kMaxSpl + i];
+  for (int i = threadIdx.x; i < kMaxBkt; i += blockDim.x) lcnt[i] = 0;
+  __syncthreads();
+  const double* src = pick(p.src, s0, s1, s2) + t.start;
+  for (int i = threadIdx.x; i < t.len; i += blockDim.x) {
+    int b = classify(lspl, ns, src[i]);
+    atomicAdd(&lcnt[b], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 2 * ns + 1; b += blockDim.x)
+    if (lcnt[b]) atomicAdd(&bcnt[(int64_t)t.part * kMaxBkt + b], (unsigned long long)lcnt[b]);
+}
+
+__global__ void k_bucket_scan(const Part* parts, int nparts, const int* nspl,
+                              const unsigned long long* bcnt, unsigned long long* bcur) {
+  int pi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pi >= nparts) return;
+  int nb = 2 * nspl[pi] + 1;
+  unsigned long long run = (unsigned long long)parts[pi].off;
+  for (int b = 0; b < nb; b++) {
+    bcur[(int64_t)pi * kMaxBkt + b] = run;
+    run += bcnt[(int64_t)pi * kMaxBkt + b];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter(const PTile* tiles, const Part* parts,
+                                                            const double* s0, const double* s1,
+                                                            const double* s2, double* dst,
+                                                            const double* spl, const int* nspl,
+                                                            unsigned long long* bcur) {
+  __shared__ double lspl[kMaxSpl];
+  __shared__ unsigned int lcnt[kMaxBkt];
+  __shared__ unsigned long long lbase[kMaxBkt];
+  PTile t = tiles[blockIdx.x];
+  Part p = parts[t.part];
+  int ns = nspl[t.part];
+  int nb = 2 * ns + 1;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) lspl[i] = spl[(int64_t)t.part * kMaxSpl + i];
+  for (int i = threadIdx.x; i < kMaxBkt; i += blockDim.x) lcnt[i] = 0;
+  __syncthreads();
+  const double* src = pick(p.src, s0, s1, s2) + t.start;
+  double xv[kPerThread];
+  int bk[kPerThread];
+#pragma unroll
+  for (int r = 0; r < kPerThread; r++) {
+    int i = threadIdx.x + r * kBlock;
+    bk[r] = -1;
+    if (i < t.len) {
+      xv[r] = src[i];
+      bk[r] = classify(lspl, ns, xv[r]);
+      atomicAdd(&lcnt[bk[r]], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    unsigned int c = lcnt[b];
+    lbase[b] = c ? atomicAdd(&bcur[(int64_t)t.part * kMaxBkt + b], (unsigned long long)c) : 0ull;
+    lcnt[b] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPerThread; r++) {
+    if (bk[r] >= 0) {
+      unsigned int slot = atomicAdd(&lcnt[bk[r]], 1u);
+      dst[lbase[bk[r]] + slot] = xv[r];
+    }
+  }
+}
+
+// One final bucket (<= kCap values, or all equal) -> sorted values at the
+// segment's destination.
+__global__ __launch_bounds__(kBlock) void k_bucket_sort(const FBucket* fbs, const SegDesc* segs,
+                                                         const double* s0, const double* s1,
+                                                         const double* s2, double* x,
+                                                         double* vs) {
+  __shared__ double s[kCap];
+  FBucket fb = fbs[blockIdx.x];
+  SegDesc sd = segs[fb.seg];
+  const double* src = pick(fb.src, s0, s1, s2) + fb.off;
+  double* dst = (sd.e_cnt == 0 ? x + sd.x_off : vs + sd.v_off) + fb.rank;
+  if (fb.is_eq) {
+    for (int64_t i = threadIdx.x; i < fb.cnt; i += blockDim.x) dst[i] = src[i];
+    return;
+  }
+  int n = (int)fb.cnt;
+  int npad = pow2_at_least(n);
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) s[i] = i < n ? src[i] : __builtin_inf();
+  __syncthreads();
+  bitonic_lds(s, npad);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = s[i];
+}
+
+// ------------------------------------------------------------ 3. E merge
+// Merged order: E records precede new values with an equal mean, which is
+// the stable order of centroids ++ buffer ++ values (tdigest.mojo:263-281).
+__global__ __launch_bounds__(kBlock) void k_merge_e(const SegDesc* segs, const int* eseg, int neseg,
+                                                     const double* vs, const double* esm,
+                                                     double* x, int64_t* pose) {
+  int si = blockIdx.y;
+  if (si >= neseg) return;
+  SegDesc sd = segs[eseg[si]];
+  int64_t tot = sd.v_cnt + sd.e_cnt;
+  const double* e = esm + sd.e_off;
+  const double* v = vs + sd.v_off;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < tot;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    if (it < sd.v_cnt) {
+      double xv = v[it];
+      int64_t lo = 0, hi = sd.e_cnt;  // # E <= xv
+      while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (!(xv < e[mid])) lo = mid + 1;
+        else hi = mid;
+      }
+      x[sd.x_off + it + lo] = xv;
+    } else {
+      int64_t i = it - sd.v_cnt;
+      double xe = e[i];
+      int64_t lo = 0, hi = sd.v_cnt;  // # V < xe
+      while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (v[mid] < xe) lo = mid + 1;
+        else hi = mid;
+      }
+      x[sd.x_off + i + lo] = xe;
+      pose[sd.e_off + i] = i + lo;
+    }
+  }
+}
+
+// ------------------------------------------------------------ 4. chain
+// Weight of the first j merged records: sf * (#V among them) + sum of the E
+// weights among them.  `c` is a cursor (= # E records before j) that moves
+// with the queries, so a monotone walk costs O(1) amortized.
+struct PCur {
+  const int64_t* pos;
+  const double* se;
+  int64_t ne;
+  double sf;
+  int64_t c;
+  __device__ double P(int64_t j) {
+    while (c > 0 && pos[c - 1] >= j) c--;
+    while (c < ne && pos[c] < j) c++;
+    return sf * (double)(j - c) + se[c];
+  }
+};
+
+__device__ inline double kinv_est(int scale, double k, double nrm) {
+  double a = k / nrm;
+  switch (scale) {
+    case KD_K0: return a;
+    case KD_K1: {
+      const double h = 1.5707963267948966;
+      if (a >= h) return 1.0;
+      if (a <= -h) return 0.0;
+      return (sin(a) + 1.0) * 0.5;
+    }
+    case KD_K2: return 1.0 / (1.0 + exp(-a));
+    default: return a <= 0.0 ? exp(a) * 0.5 : 1.0 - exp(-a) * 0.5;
+  }
+}
+
+// Smallest j with P(j) >= T (an estimate: the exact predicate search below
+// corrects any rounding slack).
+__device__ int64_t index_for_weight(PCur& pc, int64_t n, double T) {
+  if (pc.ne == 0) {
+    double j = ceil(T / pc.sf);
+    if (!(j > 0.0)) return 0;
+    if (j > (double)n) return n;
+    return (int64_t)j;
+  }
+  // Largest E record index c with weight-before(c) < T, found by galloping
+  // from the cursor.
+  auto before = [&](int64_t c) { return pc.sf * (double)(pc.pos[c] - c) + pc.se[c]; };
+  int64_t lo = -1, hi = pc.ne;  // before(lo) < T <= before(hi) (virtual ends)
+  int64_t c0 = std::min<int64_t>(std::max<int64_t>(pc.c, 0), pc.ne - 1);
+  if (before(c0) < T) {
+    lo = c0;
+    int64_t step = 1;
+    while (lo + step < pc.ne && before(lo + step) < T) {
+      lo += step;
+      step <<= 1;
+    }
+    hi = std::min<int64_t>(lo + step, pc.ne);
+  } else {
+    hi = c0;
+    int64_t step = 1;
+    while (hi - step >= 0 && !(before(hi - step) < T)) {
+      hi -= step;
+      step <<= 1;
+    }
+    lo = std::max<int64_t>(hi - step, -1);
+  }
+  while (hi - lo > 1) {
+    int64_t mid = (lo + hi) >> 1;
+    if (before(mid) < T) lo = mid;
+    else hi = mid;
+  }
+  // Records before E[lo+1]: V run after E[lo] (or from the start).
+  int64_t base_j = lo < 0 ? 0 : pc.pos[lo] + 1;
+  double base_w = lo < 0 ? 0.0 : pc.sf * (double)(pc.pos[lo] - lo) + pc.se[lo + 1];
+  if (base_w >= T) return base_j;
+  double j = (double)base_j + ceil((T - base_w) / pc.sf);
+  int64_t lim = lo + 1 < pc.ne ? pc.pos[lo + 1] : n;
+  if (j > (double)lim) return lim;
+  return (int64_t)j;
+}
+
+__global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int scale,
+                                               const double* X, const int64_t* pose,
+                                               const double* se, const double* esw,
+                                               int64_t* cs, int64_t* ccount,
+                                               SegResult* res) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  int64_t n = sd.v_cnt + sd.e_cnt;
+  int64_t replays = 0;
+  if (n == 0) {
+    ccount[si] = 0;
+    res[si].replays = 0;
+    return;
+  }
+  const double* x = X + sd.x_off;
+  const double N = sd.stored_n;
+  const double nrm = sd.nrm;
+  PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
+  int64_t s = 0, c = 0;
+  while (s < n) {
+    cs[sd.x_off + c] = s;
+    c++;
+    if (s == n - 1) break;
+    double ks = kd_k(scale, pc.P(s) / N, nrm);
+    // pred(i): candidate i closes the cluster started at s, i.e. NOT
+    // (k(q_right) - k(q_left) <= 1) with q_right = P(i+1)/N.
+    auto pred = [&](int64_t i) -> bool {
+      double span = kd_k(scale, pc.P(i + 1) / N, nrm) - ks;
+      return !(span <= 1.0);
+    };
+    // Estimate from the inverse scale function, then an exact gallop.
+    int64_t g;
+    double qt = kinv_est(scale, ks + 1.0, nrm);
+    if (!(qt < 1.0)) g = n - 1;
+    else if (!(qt > 0.0)) g = s + 1;
+    else g = index_for_weight(pc, n, qt * N) - 1;
+    if (g < s + 1) g = s + 1;
+    if (g > n - 1) g = n - 1;
+    int64_t lo, hi;  // invariant: pred(lo) false (or lo == s), pred(hi) true (or hi == n)
+    if (pred(g)) {
+      hi = g;
+      int64_t step = 1;
+      lo = g - step;
+      while (lo >= s + 1 && pred(lo)) {
+        hi = lo;
+        step <<= 1;
+        lo = hi - step;
+      }
+      if (lo < s) lo = s;
+    } else {
+      lo = g;
+      int64_t step = 1;
+      hi = g + step;
+      while (hi <= n - 1 && !pred(hi)) {
+        lo = hi;
+        step <<= 1;
+        hi = lo + step;
+      }
+      if (hi > n) hi = n;
+      if (hi == n && lo < n - 1) {
+        // hi == n is a virtual "true"; the binary search below keeps it exact.
+      }
+    }
+    while (hi - lo > 1) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (mid >= n) {
+        hi = mid;
+        continue;
+      }
+      if (pred(mid)) hi = mid;
+      else lo = mid;
+    }
+    int64_t b = hi;  // first closing candidate, or n
+    int64_t e = b;
+    if (b < n) {
+      // Tie rule (tdigest.mojo:429): candidate.mean == current.mean absorbs.
+      double xb = x[b], xp = x[b - 1];
+      if (ord_key(xb) - ord_key(xp) <= 2) {
+        if (x[s] == xb) {
+          // Every record in [s, b] equals xb: the running mean is exactly xb
+          // and stays so; absorb the whole run of xb.
+          int64_t l2 = b, h2 = n;
+          while (l2 < h2) {
+            int64_t mid = (l2 + h2) >> 1;
+            if (x[mid] == xb) l2 = mid + 1;
+            else h2 = mid;
+          }
+          e = l2;
+        } else {
+          // Replay the incremental mean of centroid.mojo:34-36 exactly.
+          replays++;
+          PCur wc{pc.pos, pc.se, pc.ne, pc.sf, 0};
+          int64_t ce = 0;
+          while (ce < wc.ne && wc.pos[ce] < s) ce++;
+          double mean = 0.0, wt = 0.0;
+          for (int64_t j = s; j < b; j++) {
+            double w = pc.sf;
+            if (ce < wc.ne && wc.pos[ce] == j) {
+              w = esw[sd.e_off + ce];
+              ce++;
+            }
+            if (j == s) {
+              mean = x[j];
+              wt = w;
+            } else {
+              double nw = wt + w;
+              mean += (x[j] - mean) * (w / nw);
+              wt = nw;
+            }
+          }
+          while (e < n && x[e] == mean) e++;
+        }
+      }
+    }
+    s = e;
+  }
+  ccount[si] = c;
+  res[si].replays = replays;
+}
+
+// ----------------------------------------------------------- 5. sums
+__device__ inline int find_seg_by_tile(const SegDesc* segs, int nseg, int64_t t) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].t_off <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int nseg,
+                                                       const double* X, const int64_t* pose,
+                                                       const double* esw, double* tsum) {
+  __shared__ double wl[kTile];
+  __shared__ double rh[kBlock], rl[kBlock];
+  int64_t t = blockIdx.x;
+  int si = find_seg_by_tile(segs, nseg, t);
+  SegDesc sd = segs[si];
+  int64_t n = sd.v_cnt + sd.e_cnt;
+  int64_t r0 = (t - sd.t_off) * kTile;
+  int len = (int)std::min<int64_t>(kTile, n - r0);
+  for (int i = threadIdx.x; i < kTile; i += blockDim.x) wl[i] = sd.sf;
+  __syncthreads();
+  if (sd.e_cnt > 0) {
+    const int64_t* pos = pose + sd.e_off;
+    // E records inside [r0, r0 + len).
+    int64_t lo = 0, hi = sd.e_cnt;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (pos[mid] < r0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int64_t c = lo + threadIdx.x; c < sd.e_cnt && pos[c] < r0 + len; c += blockDim.x)
+      wl[pos[c] - r0] = esw[sd.e_off + c];
+  }
+  __syncthreads();
+  dd acc{0.0, 0.0};
+  for (int i = threadIdx.x; i < len; i += blockDim.x)
+    acc = dd_add(acc, two_prod(X[sd.x_off + r0 + i], wl[i]));
+  rh[threadIdx.x] = acc.hi;
+  rl[threadIdx.x] = acc.lo;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      dd a{rh[threadIdx.x], rl[threadIdx.x]};
+      dd b{rh[threadIdx.x + w], rl[threadIdx.x + w]};
+      dd c = dd_add(a, b);
+      rh[threadIdx.x] = c.hi;
+      rl[threadIdx.x] = c.lo;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    tsum[2 * t] = rh[0];
+    tsum[2 * t + 1] = rl[0];
+  }
+}
+
+// Per segment: checkpoint prefix cp[k] = sum of tiles < k (k = 0..ntiles).
+__global__ void k_tile_scan(const SegDesc* segs, int nseg, const double* tsum, double* cp) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  int64_t n = sd.v_cnt + sd.e_cnt;
+  int64_t nt = (n + kTile - 1) / kTile;
+  double* o = cp + 2 * (sd.t_off + si);
+  dd acc{0.0, 0.0};
+  o[0] = 0.0;
+  o[1] = 0.0;
+  for (int64_t k = 0; k < nt; k++) {
+    acc = dd_add(acc, dd{tsum[2 * (sd.t_off + k)], tsum[2 * (sd.t_off + k) + 1]});
+    o[2 * (k + 1)] = acc.hi;
+    o[2 * (k + 1) + 1] = acc.lo;
+  }
+}
+
+__global__ void k_scan_counts(const int64_t* cnt, int n, int64_t* base, int64_t* total) {
+  // Single block, chunked exclusive scan.
+  __shared__ int64_t part[1024];
+  int tid = threadIdx.x;
+  int chunk = (n + blockDim.x - 1) / blockDim.x;
+  int lo = std::min(n, tid * chunk), hi = std::min(n, lo + chunk);
+  int64_t s = 0;
+  for (int i = lo; i < hi; i++) s += cnt[i];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < (int)blockDim.x; i++) {
+      int64_t v = part[i];
+      part[i] = run;
+      run += v;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  int64_t run = part[tid];
+  for (int i = lo; i < hi; i++) {
+    base[i] = run;
+    run += cnt[i];
+  }
+}
+
+__device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg, int64_t a,
+                                int64_t b) {
+  // sum over [a, b) of w*x; E positions through the cursor.
+  dd acc{0.0, 0.0};
+  int64_t c = pc.c;
+  while (c > 0 && pc.pos[c - 1] >= a) c--;
+  while (c < pc.ne && pc.pos[c] < a) c++;
+  for (int64_t j = a; j < b; j++) {
+    double w = pc.sf;
+    if (c < pc.ne && pc.pos[c] == j) {
+      w = esw_seg[c];
+      c++;
+    }
+    acc = dd_add(acc, two_prod(x[j], w));
+  }
+  pc.c = c;
+  return acc;
+}
+
+__global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cbase,
+                               const int64_t* ccount, const int64_t* cs, const double* X,
+                               const int64_t* pose, const double* se, const double* esw,
+                               const double* cp, double* cm, double* cw, int64_t total) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (cbase[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  // Skip segments with no clusters that share the same base.
+  while (lo + 1 < nseg && cbase[lo + 1] <= g) lo++;
+  int si = lo;
+  SegDesc sd = segs[si];
+  int64_t c = g - cbase[si];
+  int64_t C = ccount[si];
+  int64_t n = sd.v_cnt + sd.e_cnt;
+  int64_t a = cs[sd.x_off + c];
+  int64_t b = (c + 1 < C) ? cs[sd.x_off + c + 1] : n;
+  const double* x = X + sd.x_off;
+  PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
+  const double* esw_seg = esw + sd.e_off;
+  int64_t ta = a / kTile, tb = b / kTile;
+  dd sum;
+  if (tb - ta <= 1) {
+    sum = direct_sum(x, pc, esw_seg, a, b);
+  } else {
+    dd head = direct_sum(x, pc, esw_seg, a, (ta + 1) * kTile);
+    dd tail = direct_sum(x, pc, esw_seg, tb * kTile, b);
+    const double* cps = cp + 2 * (sd.t_off + si);
+    dd mid = dd_add(dd{cps[2 * tb], cps[2 * tb + 1]},
+                    dd_neg(dd{cps[2 * (ta + 1)], cps[2 * (ta + 1) + 1]}));
+    sum = dd_add(dd_add(head, mid), tail);
+  }
+  double w = pc.P(b) - pc.P(a);
+  double mean = (b - a == 1) ? x[a] : dd_div(sum, w);
+  cm[sd.x_off + c] = mean;
+  cw[sd.x_off + c] = w;
+}
+
+// --------------------------------------------------------- 6. finish
+__global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
+                         double* cw, const double* X, const double* vs, SegResult* res) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  int64_t C = ccount[si];
+  double* m = cm + sd.x_off;
+  double* w = cw + sd.x_off;
+  // _gpu_batch_finish (tdigest.mojo:304-313): total of the new centroids.
+  double total = 0.0;
+  for (int64_t c = 0; c < C; c++) total += w[c];
+  // _prune (tdigest.mojo:441-458) when decay is active (floor > 0).
+  double removed = 0.0;
+  if (sd.floor > 0.0) {
+    int64_t k = 0;
+    for (int64_t c = 0; c < C; c++) {
+      if (w[c] >= sd.floor) {
+        m[k] = m[c];
+        w[k] = w[c];
+        k++;
+      } else {
+        removed += w[c];
+      }
+    }
+    if (removed > 0.0) {
+      total -= removed;
+      C = k;
+    }
+  }
+  SegResult r;
+  r.count = C;
+  r.total = total;
+  r.removed = removed;
+  r.replays = res[si].replays;
+  if (sd.v_cnt > 0) {
+    const double* v = sd.e_cnt == 0 ? X + sd.x_off : vs + sd.v_off;
+    r.vmin = v[0];
+    r.vmax = v[sd.v_cnt - 1];
+  } else {
+    r.vmin = 0.0;
+    r.vmax = 0.0;
+  }
+  res[si] = r;
+}
+
+// ---------------------------------------------------------------- engine
+Engine::Engine(int device) : device_(device) {
+  check_hip(hipSetDevice(device), "hipSetDevice");
+  check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename T>
+static T* as(DevBuf& b, size_t count) {
+  return static_cast<T*>(b.reserve(count * sizeof(T)));
+}
+
+void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v) {
+  int nseg = (int)segs.size();
+  double* t1 = as<double>(t1_, std::max<int64_t>(total_v, 1));
+  double* t2 = as<double>(t2_, std::max<int64_t>(total_v, 1));
+  double* x = static_cast<double*>(x_.p);
+  double* vs = static_cast<double*>(vs_.p);
+  SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
+
+  std::vector<FBucket> finals;
+  std::vector<Part> parts;
+  for (int s = 0; s < nseg; s++) {
+    const SegDesc& sd = segs[s];
+    if (sd.v_cnt == 0) continue;
+    if (sd.v_cnt <= kCap) finals.push_back({s, 0, 0, 0, sd.v_off, sd.v_cnt, 0});
+    else parts.push_back({s, 0, sd.v_off, sd.v_cnt});
+  }
+  int src_id = 0;
+  int round = 0;
+  std::vector<unsigned long long> hcnt;
+  std::vector<int> hns;
+  while (!parts.empty()) {
+    if (++round > 40) throw std::runtime_error("value sort: partition did not converge");
+    int dst_id = (src_id == 1) ? 2 : 1;
+    double* dst = dst_id == 1 ? t1 : t2;
+    int np = (int)parts.size();
+    std::vector<PTile> tiles;
+    for (int p = 0; p < np; p++)
+      for (int64_t o = 0; o < parts[p].cnt; o += kPTile)
+        tiles.push_back({p, (int32_t)std::min<int64_t>(kPTile, parts[p].cnt - o), parts[p].off + o});
+    Part* dparts = as<Part>(parts_, np);
+    PTile* dtiles = as<PTile>(tiles_, tiles.size());
+    h_parts_.assign(reinterpret_cast<const char*>(parts.data()),
+                    reinterpret_cast<const char*>(parts.data() + np));
+    h_tiles_.assign(reinterpret_cast<const char*>(tiles.data()),
+                    reinterpret_cast<const char*>(tiles.data() + tiles.size()));
+    double* dspl = as<double>(spl_, (size_t)np * kMaxSpl);
+    int* dns = as<int>(nspl_, np);
+    auto* dbcnt = as<unsigned long long>(bcnt_, (size_t)np * kMaxBkt);
+    auto* dbcur = as<unsigned long long>(bcur_, (size_t)np * kMaxBkt);
+    check_hip(hipMemcpyAsync(dparts, h_parts_.data(), h_parts_.size(), hipMemcpyHostToDevice, stream_), "H2D parts");
+    check_hip(hipMemcpyAsync(dtiles, h_tiles_.data(), h_tiles_.size(), hipMemcpyHostToDevice, stream_), "H2D tiles");
+    check_hip(hipMemsetAsync(dbcnt, 0, (size_t)np * kMaxBkt * sizeof(unsigned long long), stream_), "memset");
+    const double* s0 = d_v;
+    k_splitters<<<np, kBlock, 0, stream_>>>(dparts, s0, t1, t2, dspl, dns);
+    k_bucket_hist<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dspl, dns, dbcnt);
+    k_bucket_scan<<<(np + 63) / 64, 64, 0, stream_>>>(dparts, np, dns, dbcnt, dbcur);
+    k_bucket_scatter<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dst, dspl, dns, dbcur);
+    hcnt.resize((size_t)np * kMaxBkt);
+    hns.resize(np);
+    check_hip(hipMemcpyAsync(hcnt.data(), dbcnt, hcnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "D2H counts");
+    check_hip(hipMemcpyAsync(hns.data(), dns, np * sizeof(int), hipMemcpyDeviceToHost, stream_), "D2H nspl");
+    check_hip(hipStreamSynchronize(stream_), "partition sync");
+    std::vector<Part> next;
+    for (int p = 0; p < np; p++) {
+      int nb = 2 * hns[p] + 1;
+      int64_t off = parts[p].off;
+      int s = parts[p].seg;
+      for (int b = 0; b < nb; b++) {
+        int64_t c = (int64_t)hcnt[(size_t)p * kMaxBkt + b];
+        if (c == 0) continue;
+        int64_t rank = off - segs[s].v_off;
+        if (b & 1) finals.push_back({s, dst_id, 1, 0, off, c, rank});
+        else if (c <= kCap) finals.push_back({s, dst_id, 0, 0, off, c, rank});
+        else next.push_back({s, dst_id, off, c});
+        off += c;
+      }
+    }
+    parts.swap(next);
+    src_id = dst_id;
+  }
+  if (!finals.empty()) {
+    FBucket* dfb = as<FBucket>(fb_, finals.size());
+    h_finals_.assign(reinterpret_cast<const char*>(finals.data()),
+                     reinterpret_cast<const char*>(finals.data() + finals.size()));
+    check_hip(hipMemcpyAsync(dfb, h_finals_.data(), h_finals_.size(), hipMemcpyHostToDevice, stream_), "H2D finals");
+    k_bucket_sort<<<(unsigned)finals.size(), kBlock, 0, stream_>>>(dfb, dsegs, d_v, t1, t2, x, vs);
+  }
+}
+
+void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
+                      const double* d_em, const double* d_ew, std::vector<SegResult>& res,
+                      bool sync_results) {
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  int nseg = (int)segs.size();
+  res.resize(nseg);
+  if (nseg == 0) return;
+  int64_t total_v = 0, total_e = 0, total_x = 0, total_t = 0;
+  std::vector<int> eseg;
+  int64_t max_e_items = 0;
+  for (int s = 0; s < nseg; s++) {
+    SegDesc& sd = segs[s];
+    if (sd.e_cnt > kECap) throw std::runtime_error("too many explicit records in one digest");
+    total_v = std::max(total_v, sd.v_off + sd.v_cnt);
+    total_e = std::max(total_e, sd.e_off + sd.e_cnt);
+    int64_t n = sd.v_cnt + sd.e_cnt;
+    sd.x_off = total_x;
+    total_x += n;
+    sd.t_off = total_t;
+    total_t += (n + kTile - 1) / kTile;
+    if (sd.e_cnt > 0) {
+      eseg.push_back(s);
+      max_e_items = std::max(max_e_items, n);
+    }
+  }
+  SegDesc* dsegs = as<SegDesc>(seg_, nseg);
+  SegResult* dres = as<SegResult>(res_, nseg);
+  double* esm = as<double>(em_s_, std::max<int64_t>(total_e, 1));
+  double* esw = as<double>(ew_s_, std::max<int64_t>(total_e, 1));
+  double* se = as<double>(se_, total_e + nseg);
+  int64_t* pose = as<int64_t>(pose_, std::max<int64_t>(total_e, 1));
+  double* vs = as<double>(vs_, std::max<int64_t>(total_v, 1));
+  double* x = as<double>(x_, std::max<int64_t>(total_x, 1));
+  double* cm = as<double>(cm_, std::max<int64_t>(total_x, 1));
+  double* cw = as<double>(cw_, std::max<int64_t>(total_x, 1));
+  int64_t* cs = as<int64_t>(cs_, std::max<int64_t>(total_x, 1));
+  int64_t* ccount = as<int64_t>(cnt_, 2 * (size_t)nseg + 1);
+  int64_t* cbase = ccount + nseg;
+  double* tsum = as<double>(tsum_, 2 * std::max<int64_t>(total_t, 1));
+  double* cp = as<double>(csum_, 2 * (total_t + nseg));
+  int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
+  int64_t* dtotal = as<int64_t>(clist_, 1);
+
+  check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
+  if (!eseg.empty())
+    check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
+  if (!eseg.empty())
+    k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
+  else
+    check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
+  sort_values(segs, d_v, total_v);
+  if (!eseg.empty()) {
+    int64_t chunks = std::min<int64_t>((max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
+    for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
+      unsigned ny = (unsigned)std::min<size_t>(65535, eseg.size() - b0);
+      dim3 grid((unsigned)std::max<int64_t>(chunks, 1), ny);
+      k_merge_e<<<grid, kBlock, 0, stream_>>>(dsegs, deseg + b0, (int)ny, vs, esm, x, pose);
+    }
+  }
+  k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount, dres);
+  if (total_t > 0) {
+    k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
+    k_tile_scan<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, tsum, cp);
+  }
+  k_scan_counts<<<1, 1024, 0, stream_>>>(ccount, nseg, cbase, dtotal);
+  int64_t total_c = 0;
+  check_hip(hipMemcpyAsync(&total_c, dtotal, sizeof(int64_t), hipMemcpyDeviceToHost, stream_), "D2H total");
+  check_hip(hipStreamSynchronize(stream_), "sync clusters");
+  if (total_c > 0)
+    k_cluster_mean<<<(unsigned)((total_c + 255) / 256), 256, 0, stream_>>>(
+        dsegs, nseg, cbase, ccount, cs, x, pose, se, esw, cp, cm, cw, total_c);
+  k_finish<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
+  check_hip(hipGetLastError(), "kernel launch");
+  if (sync_results) {
+    check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
+    check_hip(hipStreamSynchronize(stream_), "sync results");
+  }
+}
+
+// ------------------------------------------------------ key partition
+constexpr int kHashSlots = 2048;
+constexpr int kKPTile = 8192;
+
+__device__ inline int probe_insert(unsigned int* hk, unsigned int key) {
+  unsigned int h = (key * 2654435761u) >> 21;  // 11 bits
+  for (int p = 0; p < 16; p++) {
+    int slot = (h + p) & (kHashSlots - 1);
+    unsigned int cur = hk[slot];
+    if (cur == key) return slot;
+    if (cur == 0xffffffffu) {
+      unsigned int old = atomicCAS(&hk[slot], 0xffffffffu, key);
+      if (old == 0xffffffffu || old == key) return slot;
+    }
+  }
+  return -1;
+}
+
+__device__ inline int probe_find(const unsigned int* hk, unsigned int key) {
+  unsigned int h = (key * 2654435761u) >> 21;
+  for (int p = 0; p < 16; p++) {
+    int slot = (h + p) & (kHashSlots - 1);
+    unsigned int cur = hk[slot];
+    if (cur == key) return slot;
+    if (cur == 0xffffffffu) return -1;
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_hist(const uint32_t* keys, int64_t n,
+                                                      unsigned long long* counts) {
+  __shared__ unsigned int hk[kHashSlots];
+  __shared__ unsigned int hc[kHashSlots];
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    hk[i] = 0xffffffffu;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kKPTile;
+  int64_t end = std::min<int64_t>(n, base + kKPTile);
+  for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+    unsigned int k = keys[i];
+    int slot = probe_insert(hk, k);
+    if (slot >= 0) atomicAdd(&hc[slot], 1u);
+    else atomicAdd(&counts[k], 1ull);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x)
+    if (hk[i] != 0xffffffffu && hc[i]) atomicAdd(&counts[hk[i]], (unsigned long long)hc[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_scatter(const uint32_t* keys, const double* vals,
+                                                         int64_t n, unsigned long long* cursor,
+                                                         double* out) {
+  __shared__ unsigned int hk[kHashSlots];
+  __shared__ unsigned int hc[kHashSlots];
+  __shared__ unsigned long long hb[kHashSlots];
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    hk[i] = 0xffffffffu;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kKPTile;
+  int64_t end = std::min<int64_t>(n, base + kKPTile);
+  for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+    int slot = probe_insert(hk, keys[i]);
+    if (slot >= 0) atomicAdd(&hc[slot], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    if (hk[i] != 0xffffffffu && hc[i]) hb[i] = atomicAdd(&cursor[hk[i]], (unsigned long long)hc[i]);
+    hc[i] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+    unsigned int k = keys[i];
+    int slot = probe_find(hk, k);
+    unsigned long long pos;
+    if (slot >= 0) pos = hb[slot] + atomicAdd(&hc[slot], 1u);
+    else pos = atomicAdd(&cursor[k], 1ull);
+    out[pos] = vals[i];
+  }
+}
+
+__global__ void k_excl_scan_u64(const unsigned long long* cnt, int64_t n, int64_t* off,
+                                unsigned long long* cursor) {
+  __shared__ unsigned long long part[1024];
+  int tid = threadIdx.x;
+  int64_t chunk = (n + blockDim.x - 1) / blockDim.x;
+  int64_t lo = std::min<int64_t>(n, tid * chunk), hi = std::min<int64_t>(n, lo + chunk);
+  unsigned long long s = 0;
+  for (int64_t i = lo; i < hi; i++) s += cnt[i];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long run = 0;
+    for (int i = 0; i < (int)blockDim.x; i++) {
+      unsigned long long v = part[i];
+      part[i] = run;
+      run += v;
+    }
+    off[n] = (int64_t)run;
+  }
+  __syncthreads();
+  unsigned long long run = part[tid];
+  for (int64_t i = lo; i < hi; i++) {
+    off[i] = (int64_t)run;
+    cursor[i] = run;
+    run += cnt[i];
+  }
+}
+
+void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
+                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets,
+                           double* d_out) {
+  auto* cnt = reinterpret_cast<unsigned long long*>(d_counts);
+  auto* cur = as<unsigned long long>(extra_[0], num_keys + 1);
+  check_hip(hipMemsetAsync(cnt, 0, num_keys * sizeof(unsigned long long), stream_), "memset counts");
+  if (n > 0) {
+    unsigned g = (unsigned)((n + kKPTile - 1) / kKPTile);
+    k_key_hist<<<g, kBlock, 0, stream_>>>(d_keys, n, cnt);
+    k_excl_scan_u64<<<1, 1024, 0, stream_>>>(cnt, num_keys, d_offsets, cur);
+    k_key_scatter<<<g, kBlock, 0, stream_>>>(d_keys, d_vals, n, cur, d_out);
+  } else {
+    k_excl_scan_u64<<<1, 1024, 0, stream_>>>(cnt, num_keys, d_offsets, cur);
+  }
+  check_hip(hipGetLastError(), "key partition launch");
+}
+
+}  // namespace kd

@@ -1,0 +1,462 @@
+"""TDigest / GroupedTDigest / GpuSession: the reference's public API over the
+HIP library.
+
+Mirrors kdigest/__init__.mojo:10-25: scale functions ScaleK0..ScaleK3
+(scale.mojo), Decay (decay.mojo), TDigest (tdigest.mojo), GroupedTDigest
+(grouped.mojo) and GpuSession (gpu/session.mojo).  Every operation calls the C
+ABI of include/kdigest_hip.h; errors surface as KdigestError, raised before
+any state changes where the reference validates first.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from ._lib import KdigestError, check, load
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+# ------------------------------------------------------------------ scales
+class ScaleFunction:
+    """k(q) scale function (scale.mojo:93-118); `id` selects it in the ABI."""
+
+    id: int = -1
+    name: str = ""
+
+    def normalizer(self, compression: float, n: float) -> float:
+        return load().kd_scale_normalizer(self.id, compression, n)
+
+    def k(self, q: float, normalizer: float) -> float:
+        return load().kd_scale_k(self.id, q, normalizer)
+
+    def q(self, k: float, normalizer: float) -> float:
+        return load().kd_scale_q(self.id, k, normalizer)
+
+    def __repr__(self) -> str:
+        return f"{self.name}()"
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, ScaleFunction) and other.id == self.id
+
+    def __hash__(self) -> int:
+        return self.id
+
+
+class ScaleK0(ScaleFunction):
+    """Linear: k = normalizer * q (scale.mojo:121-142)."""
+    id, name = 0, "ScaleK0"
+
+
+class ScaleK1(ScaleFunction):
+    """k = normalizer * asin(2q - 1) (scale.mojo:145-167)."""
+    id, name = 1, "ScaleK1"
+
+
+class ScaleK2(ScaleFunction):
+    """k = normalizer * log(q / (1 - q)) (scale.mojo:170-200)."""
+    id, name = 2, "ScaleK2"
+
+
+class ScaleK3(ScaleFunction):
+    """Piecewise log scale (scale.mojo:203-235)."""
+    id, name = 3, "ScaleK3"
+
+
+def _scale(s) -> ScaleFunction:
+    if isinstance(s, ScaleFunction):
+        return s
+    if isinstance(s, type) and issubclass(s, ScaleFunction):
+        return s()
+    if isinstance(s, int) and 0 <= s <= 3:
+        return (ScaleK0, ScaleK1, ScaleK2, ScaleK3)[s]()
+    raise ValueError(f"not a scale function: {s!r}")
+
+
+# ------------------------------------------------------------------- decay
+class Decay:
+    """Exponential decay rate lambda (decay.mojo:267-294)."""
+
+    def __init__(self, rate_lambda: float = 0.0):
+        self.rate_lambda = float(rate_lambda)
+
+    @staticmethod
+    def none() -> "Decay":
+        return Decay(0.0)
+
+    @staticmethod
+    def rate(lambda_: float) -> "Decay":
+        return Decay(lambda_)
+
+    @staticmethod
+    def half_life(h: float) -> "Decay":
+        if h <= 0.0:
+            raise ValueError("half_life must be positive")
+        return Decay(math.log(2.0) / h)
+
+    def is_active(self) -> bool:
+        return self.rate_lambda > 0.0
+
+    def __repr__(self) -> str:
+        return f"Decay(lambda={self.rate_lambda})"
+
+
+# ----------------------------------------------------------------- session
+class GpuSession:
+    """One device, one HIP stream and reusable workspaces
+    (gpu/session.mojo:19-27)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        check(L.kd_session_create(device, C.byref(h)))
+        self._h = h.value
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        check(load().kd_session_sync(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().kd_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_sessions: dict[int, GpuSession] = {}
+
+
+def default_session(device: int = 0) -> GpuSession:
+    s = _default_sessions.get(device)
+    if s is None:
+        s = GpuSession(device)
+        _default_sessions[device] = s
+    return s
+
+
+def _device_ptr(t) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return t
+    return int(t.data_ptr())
+
+
+# ----------------------------------------------------------------- TDigest
+class TDigest:
+    """Merging t-digest with pluggable scale and decay (tdigest.mojo:97-726)."""
+
+    def __init__(self, scale=ScaleK1, compression: float = 100.0, decay: Decay | None = None,
+                 session: GpuSession | None = None, _handle=None):
+        self._scale = _scale(scale)
+        self.compression = float(compression)
+        self.decay = decay if decay is not None else Decay.none()
+        self.session = session or default_session()
+        L = load()
+        if _handle is None:
+            h = C.c_void_p()
+            check(L.kd_tdigest_create(self.session.handle, self._scale.id, self.compression,
+                                      self.decay.rate_lambda, C.byref(h)))
+            _handle = h.value
+        self._h = _handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().kd_tdigest_destroy(h)
+            self._h = None
+
+    @property
+    def scale(self) -> ScaleFunction:
+        return self._scale
+
+    # --- ingest
+    def add(self, value: float, weight: float = 1.0) -> None:
+        check(load().kd_tdigest_add(self._h, float(value), float(weight)))
+
+    def add_at(self, value: float, timestamp: float, weight: float = 1.0) -> None:
+        check(load().kd_tdigest_add_at(self._h, float(value), float(timestamp), float(weight)))
+
+    def add_batch(self, values, session: GpuSession | None = None) -> None:
+        """tdigest.mojo:177-221: per-sample path below the crossover (60000,
+        or 40000 with a caller-held session), one batch merge above it."""
+        a = _f64(values)
+        check(load().kd_tdigest_add_batch(self._h, _ptr(a), a.size, int(session is not None)))
+
+    def add_batch_device(self, values) -> None:
+        """One batch merge of a device-resident float64 tensor."""
+        n = int(values.numel())
+        check(load().kd_tdigest_add_batch_device(self._h, _device_ptr(values), n))
+
+    def merge(self) -> None:
+        check(load().kd_tdigest_merge(self._h))
+
+    # --- queries
+    def quantile(self, q):
+        scalar = np.isscalar(q)
+        a = _f64(np.atleast_1d(q))
+        out = np.empty(a.size)
+        check(load().kd_tdigest_quantile(self._h, _ptr(a), a.size, _ptr(out)))
+        return float(out[0]) if scalar else out
+
+    def cdf(self, x):
+        scalar = np.isscalar(x)
+        a = _f64(np.atleast_1d(x))
+        out = np.empty(a.size)
+        check(load().kd_tdigest_cdf(self._h, _ptr(a), a.size, _ptr(out)))
+        return float(out[0]) if scalar else out
+
+    def merge_into(self, other: "TDigest") -> None:
+        check(load().kd_tdigest_merge_into(self._h, other._h))
+
+    # --- state
+    def _state(self) -> np.ndarray:
+        o = np.empty(9)
+        check(load().kd_tdigest_state(self._h, _ptr(o)))
+        return o
+
+    def total_weight(self) -> float:
+        return float(self._state()[7])
+
+    def min(self) -> float:
+        return float(self._state()[3])
+
+    def max(self) -> float:
+        return float(self._state()[4])
+
+    def state(self) -> dict:
+        o = self._state()
+        return dict(stored_total=o[0], log_scale=o[1], clock=o[2], min=o[3], max=o[4],
+                    compression=o[5], decay=o[6], total_weight=o[7], buffered=int(o[8]))
+
+    def __len__(self) -> int:
+        return int(load().kd_tdigest_size(self._h))
+
+    def centroids(self):
+        L = load()
+        n = C.c_int64()
+        check(L.kd_tdigest_centroids(self._h, None, None, 0, C.byref(n)))
+        m = np.empty(n.value)
+        w = np.empty(n.value)
+        check(L.kd_tdigest_centroids(self._h, _ptr(m), _ptr(w), n.value, C.byref(n)))
+        return m, w
+
+    # --- serde (tdigest.mojo:627-698)
+    def to_bytes(self) -> bytes:
+        L = load()
+        need = L.kd_tdigest_to_bytes(self._h, None, 0)
+        if need < 0:
+            check(int(need))
+        buf = (C.c_uint8 * need)()
+        check(0 if L.kd_tdigest_to_bytes(self._h, buf, need) == need else -6)
+        return bytes(buf)
+
+    @staticmethod
+    def from_bytes(data: bytes, scale=ScaleK1, session: GpuSession | None = None) -> "TDigest":
+        sc = _scale(scale)
+        session = session or default_session()
+        L = load()
+        buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(bytes(data) or b"\0")
+        h = C.c_void_p()
+        check(L.kd_tdigest_from_bytes(session.handle, buf, len(data), sc.id, C.byref(h)))
+        st = np.empty(9)
+        L.kd_tdigest_state(h.value, _ptr(st))
+        return TDigest(sc, st[5], Decay.rate(st[6]), session, _handle=h.value)
+
+    @staticmethod
+    def _from_centroids(compression, means, weights, min_v, max_v, scale=ScaleK1,
+                        session: GpuSession | None = None) -> "TDigest":
+        sc = _scale(scale)
+        session = session or default_session()
+        m, w = _f64(means), _f64(weights)
+        h = C.c_void_p()
+        check(load().kd_tdigest_from_centroids(session.handle, sc.id, float(compression), _ptr(m),
+                                               _ptr(w), m.size, float(min_v), float(max_v),
+                                               C.byref(h)))
+        return TDigest(sc, compression, Decay.none(), session, _handle=h.value)
+
+
+# ---------------------------------------------------------- GroupedTDigest
+class GroupedTDigest:
+    """num_keys digests updated together from (key, value) batches
+    (grouped.mojo:30-190).  State is resident in device memory."""
+
+    def __init__(self, num_keys: int, compression: float = 100.0, decay: Decay | None = None,
+                 scale=ScaleK1, session: GpuSession | None = None):
+        self._scale = _scale(scale)
+        self.num_keys = int(num_keys)
+        self.compression = float(compression)
+        self.decay = decay if decay is not None else Decay.none()
+        self.session = session or default_session()
+        h = C.c_void_p()
+        check(load().kd_grouped_create(self.session.handle, self._scale.id, self.num_keys,
+                                       self.compression, self.decay.rate_lambda, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().kd_grouped_destroy(h)
+            self._h = None
+
+    def add_batch(self, keys, values) -> None:
+        k = np.asarray(keys)
+        v = _f64(values)
+        if k.shape[0] != v.shape[0]:
+            raise ValueError("keys and values must have equal length")
+        if k.size and (k.min() < 0 or k.max() >= self.num_keys):
+            raise KdigestError(-1, "key out of range [0, num_keys)")
+        k32 = np.ascontiguousarray(k, dtype=np.uint32)
+        check(load().kd_grouped_add_batch(self._h, k32.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          _ptr(v), v.size))
+
+    def add_batch_device(self, keys, values) -> None:
+        """Device-resident uint32 (or int32) keys and float64 values."""
+        n = int(values.numel())
+        if int(keys.numel()) != n:
+            raise ValueError("keys and values must have equal length")
+        check(load().kd_grouped_add_batch_device(self._h, _device_ptr(keys), _device_ptr(values), n))
+
+    def _query(self, fn, keys, xs):
+        k = np.ascontiguousarray(np.atleast_1d(keys), dtype=np.int64)
+        x = _f64(np.atleast_1d(xs))
+        if k.size == 1 and x.size > 1:
+            k = np.full(x.size, k[0], dtype=np.int64)
+        out = np.empty(x.size)
+        check(fn(self._h, _ptr(k, _i64p), _ptr(x), x.size, _ptr(out)))
+        return out
+
+    def quantile(self, key, q):
+        scalar = np.isscalar(key) and np.isscalar(q)
+        out = self._query(load().kd_grouped_quantile, key, q)
+        return float(out[0]) if scalar else out
+
+    def cdf(self, key, x):
+        scalar = np.isscalar(key) and np.isscalar(x)
+        out = self._query(load().kd_grouped_cdf, key, x)
+        return float(out[0]) if scalar else out
+
+    def quantile_device(self, keys, qs, out) -> None:
+        check(load().kd_grouped_quantile_device(self._h, _device_ptr(keys), _device_ptr(qs),
+                                                int(qs.numel()), _device_ptr(out)))
+
+    def to_bytes(self, key: int) -> bytes:
+        L = load()
+        need = L.kd_grouped_to_bytes(self._h, int(key), None, 0)
+        if need < 0:
+            check(int(need))
+        buf = (C.c_uint8 * need)()
+        L.kd_grouped_to_bytes(self._h, int(key), buf, need)
+        return bytes(buf)
+
+    def centroid_count(self, key: int) -> int:
+        o = C.c_int64()
+        check(load().kd_grouped_centroid_count(self._h, int(key), C.byref(o)))
+        return o.value
+
+    def total_weight(self, key: int) -> float:
+        o = C.c_double()
+        check(load().kd_grouped_total_weight(self._h, int(key), C.byref(o)))
+        return o.value
+
+    def centroids(self, key: int):
+        L = load()
+        n = C.c_int64()
+        check(L.kd_grouped_centroids(self._h, int(key), None, None, 0, C.byref(n)))
+        m = np.empty(n.value)
+        w = np.empty(n.value)
+        check(L.kd_grouped_centroids(self._h, int(key), _ptr(m), _ptr(w), n.value, C.byref(n)))
+        return m, w
+
+    def key_state(self, key: int) -> dict:
+        o = np.empty(5)
+        check(load().kd_grouped_key_state(self._h, int(key), _ptr(o)))
+        return dict(stored_total=o[0], log_scale=o[1], clock=o[2], min=o[3], max=o[4])
+
+
+# ------------------------------------------------------ operator functions
+def gpu_compress(means, prior_weights, n_prior: int, w_uniform: float, scale, nrm: float,
+                 stored_n: float, session: GpuSession | None = None):
+    """gpu_compress_batch (gpu/add_batch.mojo:629-663) as the FP64 greedy
+    sweep: returns (means, weights) of the compressed centroids."""
+    session = session or default_session()
+    m = _f64(means)
+    w = _f64(prior_weights) if n_prior else np.zeros(1)
+    n = m.size - n_prior
+    om = np.empty(max(m.size, 1))
+    ow = np.empty(max(m.size, 1))
+    c = C.c_int64()
+    check(load().kd_compress(session.handle, _scale(scale).id, float(nrm), float(stored_n), _ptr(m),
+                             _ptr(w), float(w_uniform), int(n_prior), int(n), _ptr(om), _ptr(ow),
+                             C.byref(c)))
+    return om[:c.value].copy(), ow[:c.value].copy()
+
+
+def _csr(parts_means: Sequence, parts_weights: Sequence):
+    off = np.zeros(len(parts_means) + 1, dtype=np.int64)
+    for i, p in enumerate(parts_means):
+        off[i + 1] = off[i] + len(p)
+    m = _f64(np.concatenate([np.asarray(p, np.float64) for p in parts_means])) if len(parts_means) else np.zeros(0)
+    w = _f64(np.concatenate([np.asarray(p, np.float64) for p in parts_weights])) if len(parts_weights) else np.zeros(0)
+    return off, m, w
+
+
+def _gpu_query(fn, digests_means, digests_weights, states, q_digest, qs, session):
+    session = session or default_session()
+    off, m, w = _csr(digests_means, digests_weights)
+    st = _f64(np.asarray(states, np.float64).reshape(-1))
+    qd = np.ascontiguousarray(q_digest, dtype=np.int64)
+    q = _f64(qs)
+    out = np.empty(q.size)
+    check(fn(session.handle, len(digests_means), _ptr(off, _i64p), _ptr(m), _ptr(w), _ptr(st),
+             _ptr(qd, _i64p), _ptr(q), q.size, _ptr(out)))
+    return out
+
+
+def gpu_quantile(digests_means, digests_weights, states, q_digest, qs, session=None):
+    """Batched gpu_quantile (gpu/query.mojo:320-331); states rows are
+    (stored_total, min, max, eff_factor)."""
+    return _gpu_query(load().kd_quantile, digests_means, digests_weights, states, q_digest, qs, session)
+
+
+def gpu_cdf(digests_means, digests_weights, states, q_digest, xs, session=None):
+    """Batched gpu_cdf (gpu/query.mojo:334-342)."""
+    return _gpu_query(load().kd_cdf, digests_means, digests_weights, states, q_digest, xs, session)
+
+
+def gpu_scale_eval(scale, nrm: float, qs, session=None) -> np.ndarray:
+    """Device k(q) (gpu/scale_device.mojo:53-98), FP64."""
+    session = session or default_session()
+    q = _f64(qs)
+    out = np.empty(q.size)
+    check(load().kd_scale_eval(session.handle, _scale(scale).id, float(nrm), _ptr(q), q.size, _ptr(out)))
+    return out
+
+
+def gpu_math_eval(fn: str, xs, session=None) -> np.ndarray:
+    session = session or default_session()
+    x = _f64(xs)
+    out = np.empty(x.size)
+    check(load().kd_math_eval(session.handle, {"log": 0, "asin": 1, "sqrt": 2}[fn], _ptr(x), x.size,
+                              _ptr(out)))
+    return out

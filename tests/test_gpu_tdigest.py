@@ -1,0 +1,188 @@
+"""GPU parity of TDigest / GroupedTDigest / queries against the CPU oracle."""
+import numpy as np
+import pytest
+
+from tests.parity import check_centroids, check_values
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp_digest(d, r, tag, integer_weights=True):
+    m, w = d.centroids()
+    rm, rw = r.centroids()
+    scale_abs = max(abs(r.state()["min"]), abs(r.state()["max"]), 1e-300)
+    check_centroids(m, w, rm, rw, scale_abs, integer_weights, tag)
+    st, rs = d.state(), r.state()
+    for k in ("log_scale", "clock", "min", "max"):
+        assert st[k] == rs[k], (tag, k, st[k], rs[k])
+    if integer_weights:
+        assert st["stored_total"] == rs["stored_total"], (tag, st["stored_total"], rs["stored_total"])
+    else:
+        assert abs(st["stored_total"] - rs["stored_total"]) <= 1e-12 * abs(rs["stored_total"]), tag
+
+
+@pytest.mark.parametrize("scale", [0, 1, 2, 3])
+def test_add_batch_b_semantics(kd, oracle, scale):
+    rng = np.random.default_rng(100 + scale)
+    d = kd.TDigest(scale, 200.0)
+    r = oracle.Digest(scale, 200.0)
+    for b in range(4):
+        x = rng.lognormal(0.0, 1.5, 70000 + 1000 * b)
+        d.add_batch(x)
+        r.add_batch(x)
+        _cmp_digest(d, r, f"K{scale} batch {b}")
+    qs = np.concatenate([rng.uniform(0, 1, 500), [0.0, 1.0, 1e-4, 1 - 1e-4]])
+    got = d.quantile(qs)
+    ref = np.array([r.quantile(float(q)) for q in qs])
+    check_values(got, ref, 1.0, "quantile")
+    xs = rng.lognormal(0.0, 1.5, 500)
+    got = d.cdf(xs)
+    ref = np.array([r.cdf(float(x)) for x in xs])
+    check_values(got, ref, 1.0, "cdf")
+
+
+@pytest.mark.parametrize("scale", [1, 2])
+def test_add_batch_b_with_decay(kd, oracle, scale):
+    rng = np.random.default_rng(200 + scale)
+    lam = np.log(2.0) / 1e5
+    d = kd.TDigest(scale, 200.0, kd.Decay.rate(lam))
+    r = oracle.Digest(scale, 200.0, lam)
+    for b in range(5):
+        x = rng.lognormal(0.0, 1.5, 65536)
+        d.add_batch(x)
+        r.add_batch(x)
+        _cmp_digest(d, r, f"decay K{scale} batch {b}", integer_weights=False)
+    qs = rng.uniform(0, 1, 300)
+    check_values(d.quantile(qs), np.array([r.quantile(float(q)) for q in qs]), 1.0, "quantile")
+
+
+@pytest.mark.parametrize("scale", [0, 1, 2, 3])
+def test_add_loop_matches_oracle(kd, oracle, scale):
+    # Below the crossover add_batch is the per-sample path (tdigest.mojo:199-202);
+    # its merges every 5*delta samples run on the GPU.
+    rng = np.random.default_rng(300 + scale)
+    d = kd.TDigest(scale, 100.0)
+    r = oracle.Digest(scale, 100.0)
+    x = rng.standard_normal(20000)
+    for c in range(0, x.size, 4096):
+        d.add_batch(x[c:c + 4096])
+        r.add_batch(x[c:c + 4096])
+    d.merge()
+    r.merge()
+    _cmp_digest(d, r, f"A K{scale}")
+    for q in (0.001, 0.01, 0.5, 0.99, 0.999):
+        check_values([d.quantile(q)], [r.quantile(q)], 1.0, f"q={q}")
+    for x0 in (-3.0, 0.0, 3.0):
+        check_values([d.cdf(x0)], [r.cdf(x0)], 1.0, f"cdf({x0})")
+
+
+def test_add_at_decay_matches_oracle(kd, oracle):
+    rng = np.random.default_rng(5)
+    lam = 1e-3
+    d = kd.TDigest(2, 300.0, kd.Decay.rate(lam))
+    r = oracle.Digest(2, 300.0, lam)
+    t = 0.0
+    for i in range(6000):
+        t += rng.exponential(1e-3)
+        v = rng.standard_t(3)
+        w = rng.uniform(0.1, 10)
+        d.add_at(v, t, w)
+        r.add_at(v, t, w)
+    d.merge()
+    r.merge()
+    _cmp_digest(d, r, "add_at", integer_weights=False)
+
+
+def test_merge_into_matches_oracle(kd, oracle):
+    rng = np.random.default_rng(6)
+    a, b = kd.TDigest(1, 100.0), kd.TDigest(1, 100.0)
+    ra, rb = oracle.Digest(1, 100.0), oracle.Digest(1, 100.0)
+    xa, xb = rng.standard_normal(70000), rng.standard_normal(3000) + 2
+    a.add_batch(xa)
+    ra.add_batch(xa)
+    for v in xb:
+        b.add(v)
+        rb.add(v)
+    a.merge_into(b)
+    ra.merge_into(rb)
+    _cmp_digest(a, ra, "merge_into")
+    _cmp_digest(b, rb, "merge_into other")
+
+
+def test_serde_round_trip_bytes_identical(kd, oracle):
+    rng = np.random.default_rng(8)
+    d = kd.TDigest(2, 100.0, kd.Decay.half_life(5000.0))
+    r = oracle.Digest(2, 100.0, np.log(2.0) / 5000.0)
+    x = rng.uniform(0, 1, 20000)
+    for v in x:
+        d.add(v)
+        r.add(v)
+    bd, br = d.to_bytes(), r.to_bytes()
+    assert bd == br
+    d2 = kd.TDigest.from_bytes(bd, kd.ScaleK2)
+    assert d2.to_bytes() == bd
+    assert len(d2) == len(d)
+
+
+@pytest.mark.parametrize("decay", [0.0, np.log(2.0) / 500.0])
+def test_grouped_accumulates_like_per_key_b(kd, oracle, decay):
+    # test_gpu_grouped.mojo:241-311: per key equal to single-digest B batches.
+    rng = np.random.default_rng(9)
+    nk = 6
+    g = kd.GroupedTDigest(nk, 100.0, kd.Decay.rate(decay), scale=kd.ScaleK2)
+    refs = [oracle.Digest(2, 100.0, decay) for _ in range(nk)]
+    for batch in range(5):
+        keys, vals = [], []
+        for i in range(400):
+            for k in range(nk):
+                if k == 1 and batch % 2 == 1:
+                    continue
+                if k == 2 and batch >= 2:
+                    continue
+                if k == 4 and i >= 10:
+                    continue
+                if k == 5 and batch == 3 and i >= 300:
+                    continue
+                keys.append(k)
+                vals.append(rng.uniform() * (k + 1) + batch)
+        if batch == 4:
+            for _ in range(800):
+                keys.append(3)
+                vals.append(rng.uniform() * 4 + 4)
+        keys = np.array(keys, np.uint32)
+        vals = np.array(vals)
+        g.add_batch(keys, vals)
+        oracle.grouped_add_batch(refs, keys, vals)
+    for k in range(nk):
+        m, w = g.centroids(k)
+        rm, rw = refs[k].centroids()
+        check_centroids(m, w, rm, rw, 10.0, decay == 0.0, f"key {k}")
+        for q in (0.0, 0.01, 0.25, 0.5, 0.75, 0.99, 1.0):
+            check_values([g.quantile(k, q)], [refs[k].quantile(q)], 1.0, f"key {k} q {q}")
+
+
+def test_grouped_many_keys_zipf(kd, oracle):
+    rng = np.random.default_rng(10)
+    nk = 3000
+    p = 1.0 / np.arange(1, nk + 1) ** 1.1
+    p /= p.sum()
+    keys = rng.choice(nk, size=400000, p=p).astype(np.uint32)
+    mu = rng.uniform(-100, 100, nk)
+    vals = np.where(rng.uniform(size=keys.size) < 0.9, rng.standard_normal(keys.size) + mu[keys],
+                    mu[keys] + (1 - rng.uniform(size=keys.size)) ** (-1 / 1.5))
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    g.add_batch(keys, vals)
+    refs = [oracle.Digest(3, 100.0) for _ in range(nk)]
+    oracle.grouped_add_batch(refs, keys, vals, threads=8)
+    for k in range(0, nk, 7):
+        m, w = g.centroids(k)
+        rm, rw = refs[k].centroids()
+        check_centroids(m, w, rm, rw, 200.0, True, f"key {k}")
+    ks = np.arange(0, nk, 7)
+    qk = np.repeat(ks, 4)
+    qs = np.tile([1e-3, 0.01, 0.99, 0.999], ks.size)
+    got = g.quantile(qk, qs)
+    ref = np.array([refs[int(k)].quantile(float(q)) if len(refs[int(k)]) else np.nan
+                    for k, q in zip(qk, qs)])
+    ok = ~np.isnan(ref)
+    check_values(got[ok], ref[ok], 100.0, "grouped quantile")
