@@ -44,6 +44,12 @@ const char *kd_last_error(void);
 int kd_session_create(int device, kd_session **out);
 void kd_session_destroy(kd_session *s);
 int kd_session_sync(kd_session *s);
+/* Per-phase device timing (HIP events on the session stream around each
+ * kernel group).  enable != 0 turns it on and clears the totals. */
+int kd_session_profile(kd_session *s, int enable);
+/* Writes {"phase": [ms_total, launches, algorithmic_bytes], ...} as JSON;
+ * returns the length (writes only when cap is large enough). */
+int64_t kd_session_phase_stats(kd_session *s, char *out, int64_t cap);
 
 /* ------------------------------------------------ operator entry points -- */
 /* gpu_compress_batch (kdigest/gpu/add_batch.mojo:629-663) as an FP64

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -22,6 +23,7 @@
 using kd::DevBuf;
 using kd::SegDesc;
 using kd::SegResult;
+using kd::launched;
 
 namespace {
 
@@ -45,6 +47,10 @@ int fail(int code, const std::string& msg) {
 
 template <typename F>
 int guarded(F&& f) {
+  // hipGetLastError is sticky per thread: drop whatever another library in
+  // the process (or a caller's own HIP use) left behind, so each launch check
+  // below reports only our own launches.
+  (void)hipGetLastError();
   try {
     return f();
   } catch (const KdError& e) {
@@ -291,6 +297,33 @@ extern "C" int kd_session_sync(kd_session* s) {
   });
 }
 
+extern "C" int kd_session_profile(kd_session* s, int enable) {
+  return guarded([&] {
+    s->eng.set_profiling(enable != 0);
+    return KD_OK;
+  });
+}
+
+extern "C" int64_t kd_session_phase_stats(kd_session* s, char* out, int64_t cap) {
+  std::string js = "{";
+  try {
+    bool first = true;
+    for (const auto& kv : s->eng.phase_stats()) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "%s\"%s\": [%.6f, %lld, %.1f]", first ? "" : ", ",
+               kv.first.c_str(), kv.second.ms, (long long)kv.second.launches, kv.second.bytes);
+      js += buf;
+      first = false;
+    }
+  } catch (const std::exception& e) {
+    return fail(KD_EDEVICE, e.what());
+  }
+  js += "}";
+  int64_t need = (int64_t)js.size() + 1;
+  if (out && cap >= need) memcpy(out, js.c_str(), need);
+  return need;
+}
+
 // ======================================================= operator entries
 extern "C" int kd_compress(kd_session* s, int scale, double nrm, double stored_n,
                            const double* means, const double* weights, double w_uniform,
@@ -354,7 +387,9 @@ static int run_query(kd_session* s, int is_cdf, int64_t nd, const int64_t* off,
     s->h2d(dqd, qd, nq * sizeof(int64_t));
     s->h2d(dq, qs, nq * sizeof(double));
     k_query_prefix<<<blocks_for(nd, 64), 64, 0, s->st()>>>(nd, doff, dw, dS);
+    launched("k_query_prefix");
     k_query<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff, dm, dw, dS, dst, dqd, dq, nq, dout);
+    launched("k_query");
     hip_ok(hipGetLastError(), "query launch");
     s->d2h(out, dout, nq * sizeof(double));
     s->sync();
@@ -383,6 +418,7 @@ extern "C" int kd_scale_eval(kd_session* s, int scale, double nrm, const double*
     double* dk = s->buf<double>(s->tmp2, n);
     s->h2d(dq, q, n * sizeof(double));
     k_scale_eval<<<blocks_for(n), 256, 0, s->st()>>>(scale, nrm, dq, n, dk);
+    launched("k_scale_eval");
     hip_ok(hipGetLastError(), "scale launch");
     s->d2h(out_k, dk, n * sizeof(double));
     s->sync();
@@ -398,6 +434,7 @@ extern "C" int kd_math_eval(kd_session* s, int fn, const double* x, int64_t n, d
     double* dy = s->buf<double>(s->tmp2, n);
     s->h2d(dx, x, n * sizeof(double));
     k_math_eval<<<blocks_for(n), 256, 0, s->st()>>>(fn, dx, n, dy);
+    launched("k_math_eval");
     hip_ok(hipGetLastError(), "math launch");
     s->d2h(out, dy, n * sizeof(double));
     s->sync();
@@ -428,6 +465,7 @@ struct kd_tdigest {
     double factor = std::exp(-log_scale);
     if (nc > 0) {
       k_scale_weights<<<blocks_for(nc), 256, 0, s->st()>>>(dw(), nc, factor);
+      launched("k_scale_weights");
       hip_ok(hipGetLastError(), "renormalize launch");
     }
     for (double& w : bw) w *= factor;
@@ -535,7 +573,9 @@ struct kd_tdigest {
     s->h2d(dq, qs, nq * sizeof(double));
     hip_ok(hipMemsetAsync(dqd, 0, nq * sizeof(int64_t), s->st()), "memset");
     k_query_prefix<<<1, 64, 0, s->st()>>>(1, doff, dw(), dS);
+    launched("k_query_prefix");
     k_query<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff, dm(), dw(), dS, dst, dqd, dq, nq, dout);
+    launched("k_query");
     hip_ok(hipGetLastError(), "query launch");
     s->d2h(out, dout, nq * sizeof(double));
     s->sync();
@@ -630,6 +670,7 @@ extern "C" int kd_tdigest_add_batch_device(kd_tdigest* d, const double* d_values
     int* flag = d->s->buf<int>(d->s->tmp, 1);
     hip_ok(hipMemsetAsync(flag, 0, sizeof(int), d->s->st()), "memset");
     k_validate<<<blocks_for(n), 256, 0, d->s->st()>>>(nullptr, d_values, n, 0, flag);
+    launched("k_validate");
     int hflag = 0;
     d->s->d2h(&hflag, flag, sizeof(int));
     d->s->sync();
@@ -868,6 +909,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   int* flag = s->buf<int>(s->tmp, 1);
   hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
   k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
+  launched("k_validate");
   int hflag = 0;
   s->d2h(&hflag, flag, sizeof(int));
   s->sync();
@@ -930,6 +972,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
     s->h2d(dk, rkeys.data(), rkeys.size() * sizeof(int64_t));
     s->h2d(df, rfac.data(), rfac.size() * sizeof(double));
     k_scale_keys<<<(unsigned)rkeys.size(), 64, 0, s->st()>>>(dk, df, (int64_t)rkeys.size(), doff(), dw());
+    launched("k_scale_keys");
     hip_ok(hipGetLastError(), "renormalize launch");
   }
   std::vector<SegResult> res;
@@ -955,11 +998,12 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   k_gather_csr<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dm(), dw(), dseg, dxo,
                                                           s->eng.cent_means(), s->eng.cent_weights(),
                                                           dno, dnm, dnw);
+  launched("k_gather_csr");
   hip_ok(hipGetLastError(), "gather launch");
   s->sync();
-  std::swap(off, noff);
-  std::swap(cm, ncm);
-  std::swap(cw, ncw);
+  off.swap(noff);
+  cm.swap(ncm);
+  cw.swap(ncw);
   hoff.swap(noffh);
   prefix_valid = false;
 
@@ -985,6 +1029,7 @@ void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int
   double* st = static_cast<double*>(qstate.reserve(4 * num_keys * sizeof(double)));
   if (!prefix_valid) {
     k_query_prefix<<<blocks_for(num_keys, 64), 64, 0, s->st()>>>(num_keys, doff(), dw(), dS);
+    launched("k_query_prefix");
     std::vector<double> hs(4 * num_keys);
     for (int64_t k = 0; k < num_keys; k++) {
       hs[k] = stored_total[k];
@@ -998,6 +1043,7 @@ void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int
   k_query_grouped<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff(), dm(), dw(), dS, st,
                                                        st + num_keys, st + 2 * num_keys,
                                                        st + 3 * num_keys, d_keys, d_q, nq, d_out);
+  launched("k_query_grouped");
   hip_ok(hipGetLastError(), "grouped query launch");
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -41,13 +42,40 @@ struct SegResult {
   int64_t replays;  // tie decisions resolved by an exact replay of the running mean
 };
 
+// Owning device allocation, grown on demand (contents are not preserved when
+// it grows).  Move-only: a copy would free the same pointer twice.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
+  void swap(DevBuf& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
   void* reserve(size_t bytes);
   void release();
   ~DevBuf() { release(); }
 };
+
+// Checks the launch that was just enqueued (configuration errors surface here,
+// named after the kernel).
+void launched(const char* kernel);
 
 class Engine {
  public:
@@ -78,6 +106,19 @@ class Engine {
 
   DevBuf& scratch(int i) { return extra_[i]; }
 
+  // Per-phase device time from HIP events recorded on the engine stream
+  // around each kernel group (enabled on demand; off by default).
+  struct PhaseStat {
+    double ms = 0.0;
+    int64_t launches = 0;
+    double bytes = 0.0;  // algorithmic HBM bytes of the launches
+  };
+  void set_profiling(bool on);
+  const std::map<std::string, PhaseStat>& phase_stats() { collect(); return stats_; }
+  int phase_begin(const char* name, double bytes);
+  void phase_end(int id);
+  void collect();
+
  private:
   void sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v);
 
@@ -88,6 +129,16 @@ class Engine {
   DevBuf extra_[4];
   // Host staging that must outlive the asynchronous copies reading it.
   std::vector<char> h_parts_, h_tiles_, h_finals_;
+  bool prof_ = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, PhaseStat> stats_;
+  hipEvent_t get_event();
 };
 
 void check_hip(hipError_t e, const char* what);

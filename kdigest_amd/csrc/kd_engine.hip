@@ -33,6 +33,12 @@ void check_hip(hipError_t e, const char* what) {
   }
 }
 
+void launched(const char* kernel) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("launch of ") + kernel + ": " + hipGetErrorString(e));
+}
+
 void* DevBuf::reserve(size_t bytes) {
   if (bytes <= cap) return p;
   release();
@@ -673,10 +679,15 @@ __global__ void k_scan_counts(const int64_t* cnt, int n, int64_t* base, int64_t*
   }
 }
 
+// Sum over [a, b) of w*x (double-double) and of w (plain, in record order:
+// the running cluster weight of centroid.mojo:34-36); E positions through the
+// cursor.
+// When `run_mean` is given, the running mean of the cluster is replayed with
+// the exact recurrence of centroid.mojo:34-36 as well.
 __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg, int64_t a,
-                                int64_t b) {
-  // sum over [a, b) of w*x; E positions through the cursor.
+                                int64_t b, double* wsum, double* run_mean = nullptr) {
   dd acc{0.0, 0.0};
+  double ws = 0.0, mean = 0.0;
   int64_t c = pc.c;
   while (c > 0 && pc.pos[c - 1] >= a) c--;
   while (c < pc.ne && pc.pos[c] < a) c++;
@@ -686,9 +697,22 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
       w = esw_seg[c];
       c++;
     }
-    acc = dd_add(acc, two_prod(x[j], w));
+    double xj = x[j];
+    if (run_mean) {
+      if (j == a) {
+        mean = xj;
+      } else {
+        double nw = ws + w;
+        mean += (xj - mean) * (w / nw);
+      }
+    } else {
+      acc = dd_add(acc, two_prod(xj, w));
+    }
+    ws = (j == a) ? w : ws + w;
   }
   pc.c = c;
+  *wsum = ws;
+  if (run_mean) *run_mean = mean;
   return acc;
 }
 
@@ -718,17 +742,26 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
   const double* esw_seg = esw + sd.e_off;
   int64_t ta = a / kTile, tb = b / kTile;
   dd sum;
+  double w;
   if (tb - ta <= 1) {
-    sum = direct_sum(x, pc, esw_seg, a, b);
+    // Short cluster: weight and mean replay the running centroid of the CPU
+    // sweep record by record, so both are bit-identical to it.
+    double mean;
+    direct_sum(x, pc, esw_seg, a, b, &w, &mean);
+    cm[sd.x_off + c] = mean;
+    cw[sd.x_off + c] = w;
+    return;
   } else {
-    dd head = direct_sum(x, pc, esw_seg, a, (ta + 1) * kTile);
-    dd tail = direct_sum(x, pc, esw_seg, tb * kTile, b);
+    double wh, wt;
+    dd head = direct_sum(x, pc, esw_seg, a, (ta + 1) * kTile, &wh);
+    dd tail = direct_sum(x, pc, esw_seg, tb * kTile, b, &wt);
     const double* cps = cp + 2 * (sd.t_off + si);
     dd mid = dd_add(dd{cps[2 * tb], cps[2 * tb + 1]},
                     dd_neg(dd{cps[2 * (ta + 1)], cps[2 * (ta + 1) + 1]}));
     sum = dd_add(dd_add(head, mid), tail);
+    // Long cluster: difference of prefix weights (exact for integer weights).
+    w = pc.P(b) - pc.P(a);
   }
-  double w = pc.P(b) - pc.P(a);
   double mean = (b - a == 1) ? x[a] : dd_div(sum, w);
   cm[sd.x_off + c] = mean;
   cw[sd.x_off + c] = w;
@@ -781,12 +814,63 @@ __global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, d
 }
 
 // ---------------------------------------------------------------- engine
+void Engine::set_profiling(bool on) {
+  collect();
+  prof_ = on;
+  stats_.clear();
+}
+
+hipEvent_t Engine::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  check_hip(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+int Engine::phase_begin(const char* name, double bytes) {
+  if (!prof_) return -1;
+  Pending p{name, get_event(), get_event(), bytes};
+  check_hip(hipEventRecord(p.a, stream_), "hipEventRecord");
+  pending_.push_back(p);
+  return (int)pending_.size() - 1;
+}
+
+void Engine::phase_end(int id) {
+  if (id < 0) return;
+  check_hip(hipEventRecord(pending_[id].b, stream_), "hipEventRecord");
+}
+
+void Engine::collect() {
+  if (pending_.empty()) return;
+  check_hip(hipStreamSynchronize(stream_), "profile sync");
+  for (auto& p : pending_) {
+    float ms = 0.f;
+    check_hip(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+    PhaseStat& st = stats_[p.name];
+    st.ms += ms;
+    st.launches += 1;
+    st.bytes += p.bytes;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
 Engine::Engine(int device) : device_(device) {
   check_hip(hipSetDevice(device), "hipSetDevice");
   check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
 }
 
 Engine::~Engine() {
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -821,6 +905,8 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
     double* dst = dst_id == 1 ? t1 : t2;
     int np = (int)parts.size();
     std::vector<PTile> tiles;
+    int64_t round_elems = 0;
+    for (int p = 0; p < np; p++) round_elems += parts[p].cnt;
     for (int p = 0; p < np; p++)
       for (int64_t o = 0; o < parts[p].cnt; o += kPTile)
         tiles.push_back({p, (int32_t)std::min<int64_t>(kPTile, parts[p].cnt - o), parts[p].off + o});
@@ -838,10 +924,26 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
     check_hip(hipMemcpyAsync(dtiles, h_tiles_.data(), h_tiles_.size(), hipMemcpyHostToDevice, stream_), "H2D tiles");
     check_hip(hipMemsetAsync(dbcnt, 0, (size_t)np * kMaxBkt * sizeof(unsigned long long), stream_), "memset");
     const double* s0 = d_v;
-    k_splitters<<<np, kBlock, 0, stream_>>>(dparts, s0, t1, t2, dspl, dns);
-    k_bucket_hist<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dspl, dns, dbcnt);
+    {
+      int ph_ = phase_begin("splitters", 8.0 * (double)np * kCap);
+      k_splitters<<<np, kBlock, 0, stream_>>>(dparts, s0, t1, t2, dspl, dns);
+      launched("k_splitters");
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("bucket_hist", 8.0 * (double)round_elems);
+      k_bucket_hist<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dspl, dns, dbcnt);
+      launched("k_bucket_hist");
+      phase_end(ph_);
+    }
     k_bucket_scan<<<(np + 63) / 64, 64, 0, stream_>>>(dparts, np, dns, dbcnt, dbcur);
-    k_bucket_scatter<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dst, dspl, dns, dbcur);
+    launched("k_bucket_scan");
+    {
+      int ph_ = phase_begin("bucket_scatter", 16.0 * (double)round_elems);
+      k_bucket_scatter<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dst, dspl, dns, dbcur);
+      launched("k_bucket_scatter");
+      phase_end(ph_);
+    }
     hcnt.resize((size_t)np * kMaxBkt);
     hns.resize(np);
     check_hip(hipMemcpyAsync(hcnt.data(), dbcnt, hcnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "D2H counts");
@@ -866,11 +968,18 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
     src_id = dst_id;
   }
   if (!finals.empty()) {
+    int64_t final_elems = 0;
+    for (const auto& f : finals) final_elems += f.cnt;
     FBucket* dfb = as<FBucket>(fb_, finals.size());
     h_finals_.assign(reinterpret_cast<const char*>(finals.data()),
                      reinterpret_cast<const char*>(finals.data() + finals.size()));
     check_hip(hipMemcpyAsync(dfb, h_finals_.data(), h_finals_.size(), hipMemcpyHostToDevice, stream_), "H2D finals");
-    k_bucket_sort<<<(unsigned)finals.size(), kBlock, 0, stream_>>>(dfb, dsegs, d_v, t1, t2, x, vs);
+    {
+      int ph_ = phase_begin("bucket_sort", 16.0 * (double)final_elems);
+      k_bucket_sort<<<(unsigned)finals.size(), kBlock, 0, stream_>>>(dfb, dsegs, d_v, t1, t2, x, vs);
+      launched("k_bucket_sort");
+      phase_end(ph_);
+    }
   }
 }
 
@@ -921,7 +1030,12 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   if (!eseg.empty())
     check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
   if (!eseg.empty())
-    k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
+    {
+      int ph_ = phase_begin("sort_e", 32.0 * (double)total_e);
+      k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
+      launched("k_sort_e");
+      phase_end(ph_);
+    }
   else
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
   sort_values(segs, d_v, total_v);
@@ -930,22 +1044,53 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
       unsigned ny = (unsigned)std::min<size_t>(65535, eseg.size() - b0);
       dim3 grid((unsigned)std::max<int64_t>(chunks, 1), ny);
-      k_merge_e<<<grid, kBlock, 0, stream_>>>(dsegs, deseg + b0, (int)ny, vs, esm, x, pose);
+      {
+        int ph_ = phase_begin("merge_e", 16.0 * (double)total_x);
+        k_merge_e<<<grid, kBlock, 0, stream_>>>(dsegs, deseg + b0, (int)ny, vs, esm, x, pose);
+        launched("k_merge_e");
+        phase_end(ph_);
+      }
     }
   }
-  k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount, dres);
+  {
+    int ph_ = phase_begin("chain", 0.0);
+    k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount, dres);
+    launched("k_chain");
+    phase_end(ph_);
+  }
   if (total_t > 0) {
-    k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
-    k_tile_scan<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, tsum, cp);
+    {
+      int ph_ = phase_begin("tile_sums", 8.0 * (double)total_x);
+      k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
+      launched("k_tile_sums");
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("tile_scan", 32.0 * (double)total_t);
+      k_tile_scan<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, tsum, cp);
+      launched("k_tile_scan");
+      phase_end(ph_);
+    }
   }
   k_scan_counts<<<1, 1024, 0, stream_>>>(ccount, nseg, cbase, dtotal);
+  launched("k_scan_counts");
   int64_t total_c = 0;
   check_hip(hipMemcpyAsync(&total_c, dtotal, sizeof(int64_t), hipMemcpyDeviceToHost, stream_), "D2H total");
   check_hip(hipStreamSynchronize(stream_), "sync clusters");
   if (total_c > 0)
-    k_cluster_mean<<<(unsigned)((total_c + 255) / 256), 256, 0, stream_>>>(
+    {
+      int ph_ = phase_begin("cluster_mean", 16.0 * (double)total_c);
+      k_cluster_mean<<<(unsigned)((total_c + 255) / 256), 256, 0, stream_>>>(
         dsegs, nseg, cbase, ccount, cs, x, pose, se, esw, cp, cm, cw, total_c);
-  k_finish<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
+      launched("k_cluster_mean");
+      phase_end(ph_);
+    }
+  {
+    int ph_ = phase_begin("finish", 0.0);
+    k_finish<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
+    launched("k_finish");
+    phase_end(ph_);
+  }
   check_hip(hipGetLastError(), "kernel launch");
   if (sync_results) {
     check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
@@ -1073,11 +1218,23 @@ void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t
   check_hip(hipMemsetAsync(cnt, 0, num_keys * sizeof(unsigned long long), stream_), "memset counts");
   if (n > 0) {
     unsigned g = (unsigned)((n + kKPTile - 1) / kKPTile);
-    k_key_hist<<<g, kBlock, 0, stream_>>>(d_keys, n, cnt);
+    {
+      int ph_ = phase_begin("key_hist", 4.0 * (double)n);
+      k_key_hist<<<g, kBlock, 0, stream_>>>(d_keys, n, cnt);
+      launched("k_key_hist");
+      phase_end(ph_);
+    }
     k_excl_scan_u64<<<1, 1024, 0, stream_>>>(cnt, num_keys, d_offsets, cur);
-    k_key_scatter<<<g, kBlock, 0, stream_>>>(d_keys, d_vals, n, cur, d_out);
+    launched("k_excl_scan_u64");
+    {
+      int ph_ = phase_begin("key_scatter", 20.0 * (double)n);
+      k_key_scatter<<<g, kBlock, 0, stream_>>>(d_keys, d_vals, n, cur, d_out);
+      launched("k_key_scatter");
+      phase_end(ph_);
+    }
   } else {
     k_excl_scan_u64<<<1, 1024, 0, stream_>>>(cnt, num_keys, d_offsets, cur);
+    launched("k_excl_scan_u64");
   }
   check_hip(hipGetLastError(), "key partition launch");
 }

@@ -132,6 +132,22 @@ class GpuSession:
     def synchronize(self) -> None:
         check(load().kd_session_sync(self._h))
 
+    def profile(self, enable: bool = True) -> None:
+        """Per-phase device timing with HIP events (clears the totals)."""
+        check(load().kd_session_profile(self._h, int(enable)))
+
+    def phase_stats(self) -> dict:
+        """{phase: (total_ms, launches, algorithmic_bytes)} since profile()."""
+        import json
+
+        L = load()
+        need = L.kd_session_phase_stats(self._h, None, 0)
+        if need < 0:
+            check(int(need))
+        buf = C.create_string_buffer(int(need))
+        L.kd_session_phase_stats(self._h, buf, need)
+        return {k: (v[0], int(v[1]), v[2]) for k, v in json.loads(buf.value.decode()).items()}
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             load().kd_session_destroy(self._h)

@@ -34,6 +34,7 @@ def check_centroids(got_m, got_w, ref_m, ref_w, scale_abs: float, integer_weight
         assert bad.size == 0, f"{tag}: weights differ at {bad[:5]}: {got_w[bad[:5]]} vs {ref_w[bad[:5]]}"
     else:
         np.testing.assert_allclose(got_w, ref_w, rtol=W_RTOL, atol=0, err_msg=f"{tag}: weights")
+    scale_abs = max(float(scale_abs), 1e-300)
     err = np.abs(got_m - ref_m)
     tol = MEAN_RTOL * np.maximum(np.abs(ref_m), scale_abs)
     bad = np.nonzero(err > tol)[0]
