@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -63,6 +64,15 @@ int guarded(F&& f) {
 }
 
 void hip_ok(hipError_t e, const char* what) { kd::check_hip(e, what); }
+
+using clk = std::chrono::steady_clock;
+// Milliseconds since t; resets t (consecutive host phases).
+double ms_since(clk::time_point& t) {
+  clk::time_point now = clk::now();
+  double ms = std::chrono::duration<double, std::milli>(now - t).count();
+  t = now;
+  return ms;
+}
 
 bool is_finite_val(double v) { return std::isfinite(v); }
 
@@ -257,6 +267,8 @@ unsigned blocks_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b)
 struct kd_session {
   kd::Engine eng;
   DevBuf q_off, q_m, q_w, q_s, q_state, q_qd, q_q, q_out, e_m, e_w, v_buf, tmp, tmp2;
+  // Grouped-ingest scratch shared by every grouped digest of the session.
+  DevBuf g_counts, g_koff, g_vals, g_keys, g_valsin, g_segof, g_xoff;
   explicit kd_session(int dev) : eng(dev) {}
   hipStream_t st() { return eng.stream(); }
   template <typename T>
@@ -857,7 +869,7 @@ struct kd_grouped {
   std::vector<int64_t> hoff;      // host copy of the CSR offsets
   DevBuf off, cm, cw, S;          // device CSR of centroids + query prefix
   DevBuf noff, ncm, ncw;          // next CSR (swapped in after a batch)
-  DevBuf counts, koff, vals, keysbuf, valsbuf, segof, xoff, qstate, qtmp;
+  DevBuf qstate;
   bool prefix_valid = false;
 
   bool decay() const { return lambda > 0.0; }
@@ -906,6 +918,7 @@ extern "C" void kd_grouped_destroy(kd_grouped* g) { delete g; }
 
 void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int64_t n) {
   // grouped.mojo:60-162.  Validate the whole batch first (:91-97).
+  clk::time_point tp = clk::now();
   int* flag = s->buf<int>(s->tmp, 1);
   hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
   k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
@@ -914,16 +927,18 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   s->d2h(&hflag, flag, sizeof(int));
   s->sync();
   if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
+  s->eng.host_phase("validate", ms_since(tp));
 
   // Group per key (:99-104) on the device.
-  int64_t* dcnt = static_cast<int64_t*>(counts.reserve(num_keys * sizeof(int64_t)));
-  int64_t* dkoff = static_cast<int64_t*>(koff.reserve((num_keys + 1) * sizeof(int64_t)));
-  double* dv = static_cast<double*>(vals.reserve(std::max<int64_t>(n, 1) * sizeof(double)));
+  int64_t* dcnt = static_cast<int64_t*>(s->g_counts.reserve(num_keys * sizeof(int64_t)));
+  int64_t* dkoff = static_cast<int64_t*>(s->g_koff.reserve((num_keys + 1) * sizeof(int64_t)));
+  double* dv = static_cast<double*>(s->g_vals.reserve(std::max<int64_t>(n, 1) * sizeof(double)));
   s->eng.key_partition(d_keys, d_values, n, num_keys, dcnt, dkoff, dv);
   std::vector<int64_t> kcnt(num_keys), kof(num_keys + 1);
   s->d2h(kcnt.data(), dcnt, num_keys * sizeof(int64_t));
   s->d2h(kof.data(), dkoff, (num_keys + 1) * sizeof(int64_t));
   s->sync();
+  s->eng.host_phase("key_partition", ms_since(tp));
 
   // Per touched key: B-semantics prepare (tdigest.mojo:236-296).
   std::vector<SegDesc> segs;
@@ -976,7 +991,9 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
     hip_ok(hipGetLastError(), "renormalize launch");
   }
   std::vector<SegResult> res;
+  s->eng.host_phase("prepare", ms_since(tp));
   s->eng.compress(scale, segs, dv, dm(), dw(), res);
+  s->eng.host_phase("compress", ms_since(tp));
 
   // New CSR: untouched keys keep their centroids.
   std::vector<int64_t> noffh(num_keys + 1, 0);
@@ -990,8 +1007,8 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   int64_t* dno = static_cast<int64_t*>(noff.reserve((num_keys + 1) * sizeof(int64_t)));
   double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
   double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
-  int64_t* dseg = static_cast<int64_t*>(segof.reserve(num_keys * sizeof(int64_t)));
-  int64_t* dxo = static_cast<int64_t*>(xoff.reserve(std::max<size_t>(xo.size(), 1) * sizeof(int64_t)));
+  int64_t* dseg = static_cast<int64_t*>(s->g_segof.reserve(num_keys * sizeof(int64_t)));
+  int64_t* dxo = static_cast<int64_t*>(s->g_xoff.reserve(std::max<size_t>(xo.size(), 1) * sizeof(int64_t)));
   s->h2d(dno, noffh.data(), (num_keys + 1) * sizeof(int64_t));
   s->h2d(dseg, seg_of.data(), num_keys * sizeof(int64_t));
   s->h2d(dxo, xo.data(), xo.size() * sizeof(int64_t));
@@ -1006,6 +1023,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   cw.swap(ncw);
   hoff.swap(noffh);
   prefix_valid = false;
+  s->eng.host_phase("csr", ms_since(tp));
 
   // Per key: min/max (:285-295) and the reconciled stored total (:298-315).
   for (size_t i = 0; i < segs.size(); i++) {
@@ -1055,8 +1073,8 @@ extern "C" int kd_grouped_add_batch(kd_grouped* g, const uint32_t* keys, const d
       if ((int64_t)keys[i] >= g->num_keys) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
       if (!is_finite_val(values[i])) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
     }
-    uint32_t* dk = static_cast<uint32_t*>(g->keysbuf.reserve(n * sizeof(uint32_t)));
-    double* dv = static_cast<double*>(g->valsbuf.reserve(n * sizeof(double)));
+    uint32_t* dk = static_cast<uint32_t*>(g->s->g_keys.reserve(n * sizeof(uint32_t)));
+    double* dv = static_cast<double*>(g->s->g_valsin.reserve(n * sizeof(double)));
     g->s->h2d(dk, keys, n * sizeof(uint32_t));
     g->s->h2d(dv, values, n * sizeof(double));
     g->add_device(dk, dv, n);

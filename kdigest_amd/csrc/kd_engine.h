@@ -118,6 +118,14 @@ class Engine {
   int phase_begin(const char* name, double bytes);
   void phase_end(int id);
   void collect();
+  // Host wall time of a host-side step (recorded as "host:<name>").
+  void host_phase(const char* name, double ms) {
+    if (!prof_) return;
+    PhaseStat& st = stats_[std::string("host:") + name];
+    st.ms += ms;
+    st.launches += 1;
+  }
+  bool profiling() const { return prof_; }
 
  private:
   void sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v);
@@ -125,7 +133,8 @@ class Engine {
   int device_;
   hipStream_t stream_ = nullptr;
   DevBuf seg_, res_, em_s_, ew_s_, se_, pose_, vs_, x_, cm_, cw_, cs_, cnt_, t1_, t2_;
-  DevBuf parts_, spl_, nspl_, bcnt_, bcur_, tiles_, fb_, tsum_, csum_, order_, clist_;
+  DevBuf parts_, parts2_, spl_, nspl_, bcnt_, bcur_, tiles_, tcnt_, fb_, fbl_, route_, tsum_, long_,
+      order_, clist_;
   DevBuf extra_[4];
   // Host staging that must outlive the asynchronous copies reading it.
   std::vector<char> h_parts_, h_tiles_, h_finals_;

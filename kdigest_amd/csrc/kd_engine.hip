@@ -19,6 +19,7 @@
 #include "kd_engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -121,6 +122,102 @@ __device__ inline int pow2_at_least(int64_t n) {
   return p;
 }
 
+// ------------------------------------------------------------ LDS sort
+// Each thread sorts kPer values in registers (bitonic network), writes them
+// as a run, then log2(n / kPer) merge-path rounds double the run length.
+// LDS layout is padded by one slot per 16 so a thread's run of 16 doubles
+// spreads over the banks.
+
+constexpr int kPadCap = kCap + kCap / 16;
+
+__device__ inline int padi(int e) { return e + (e >> 4); }
+
+template <int PER>
+__device__ inline void sort_regs(double (&v)[PER]) {
+#pragma unroll
+  for (int k = 2; k <= PER; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        int l = i ^ j;
+        if (l > i) {
+          double a = v[i], b = v[l];
+          double lo = __builtin_fmin(a, b), hi = __builtin_fmax(a, b);
+          bool asc = (i & k) == 0;
+          v[i] = asc ? lo : hi;
+          v[l] = asc ? hi : lo;
+        }
+      }
+    }
+  }
+}
+
+// Sorts n finite doubles held in `a` at padded indices (n <= capacity of a
+// and b); returns the buffer (a or b) that holds the sorted values.  PER
+// values per thread; all threads of the block must call it.
+template <int PER>
+__device__ double* lds_sort_t(double* a, double* b, int n) {
+  int npad = pow2_at_least(n);
+  if (npad < PER) npad = PER;
+  const int nthr = npad / PER;
+  const int t = threadIdx.x;
+  const double inf = __builtin_inf();
+  for (int t0 = t; t0 < nthr; t0 += blockDim.x) {
+    double v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      int e = t0 * PER + i;
+      v[i] = e < n ? a[padi(e)] : inf;
+    }
+    sort_regs<PER>(v);
+#pragma unroll
+    for (int i = 0; i < PER; i++) a[padi(t0 * PER + i)] = v[i];
+  }
+  __syncthreads();
+  double* src = a;
+  double* dst = b;
+  for (int L = PER; L < npad; L <<= 1) {
+    for (int t0 = t; t0 < nthr; t0 += blockDim.x) {
+      int d0 = t0 * PER;
+      int base = d0 & ~(2 * L - 1);
+      int d = d0 - base;
+      int A = base, B = base + L;
+      int lo = d - L > 0 ? d - L : 0;
+      int hi = d < L ? d : L;
+      while (lo < hi) {  // smallest i with A[i] > B[d-1-i]: ties go to A
+        int mid = (lo + hi) >> 1;
+        if (src[padi(A + mid)] <= src[padi(B + d - mid - 1)]) lo = mid + 1;
+        else hi = mid;
+      }
+      int i = lo, j = d - lo;
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        double x = i < L ? src[padi(A + i)] : inf;
+        double y = j < L ? src[padi(B + j)] : inf;
+        bool take_a = (j >= L) || (i < L && x <= y);
+        dst[padi(d0 + k)] = take_a ? x : y;
+        i += take_a ? 1 : 0;
+        j += take_a ? 0 : 1;
+      }
+    }
+    __syncthreads();
+    double* tmp = src;
+    src = dst;
+    dst = tmp;
+  }
+  return src;
+}
+
+// Picks the values-per-thread so that every thread of the block has a run.
+__device__ inline double* lds_sort(double* a, double* b, int n) {
+  int per = pow2_at_least(n) / (int)blockDim.x;
+  if (per <= 2) return lds_sort_t<2>(a, b, n);
+  if (per <= 4) return lds_sort_t<4>(a, b, n);
+  if (per <= 8) return lds_sort_t<8>(a, b, n);
+  return lds_sort_t<16>(a, b, n);
+}
+
 // ---------------------------------------------------- 1. explicit records
 __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg,
                                                     const double* em, const double* ew,
@@ -203,22 +300,20 @@ __device__ inline const double* pick(int id, const double* a, const double* b, c
 __global__ __launch_bounds__(kBlock) void k_splitters(const Part* parts, const double* s0,
                                                        const double* s1, const double* s2,
                                                        double* spl, int* nspl) {
-  __shared__ double s[kCap];
+  __shared__ double sa[kPadCap], sb[kPadCap];
   Part p = parts[blockIdx.x];
   const double* src = pick(p.src, s0, s1, s2) + p.off;
   int S = (int)std::min<int64_t>(p.cnt, kCap);
-  int npad = pow2_at_least(S);
-  for (int i = threadIdx.x; i < npad; i += blockDim.x)
-    s[i] = i < S ? src[((int64_t)i * p.cnt) / S] : __builtin_inf();
+  for (int i = threadIdx.x; i < S; i += blockDim.x) sa[padi(i)] = src[((int64_t)i * p.cnt) / S];
   __syncthreads();
-  bitonic_lds(s, npad);
+  const double* s = lds_sort(sa, sb, S);
   if (threadIdx.x == 0) {
     int64_t want = p.cnt / (kCap / 4);
     int ns = (int)std::max<int64_t>(1, std::min<int64_t>(want, kMaxSpl));
     int k = 0;
     double* out = spl + (int64_t)blockIdx.x * kMaxSpl;
     for (int j = 0; j < ns; j++) {
-      double v = s[((int64_t)(j + 1) * S) / (ns + 1)];
+      double v = s[padi((int)(((int64_t)(j + 1) * S) / (ns + 1)))];
       if (k == 0 || out[k - 1] < v) out[k++] = v;
     }
     nspl[blockIdx.x] = k;
@@ -240,9 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_bucket_hist(const PTile* tiles, cons
                                                          const double* s0, const double* s1,
                                                          const double* s2, const double* spl,
                                                          const int* nspl,
-                                                         unsigned long long* bcnt) {
+                                                         unsigned long long* bcnt,
+                                                         const int64_t* ntiles) {
   __shared__ double lspl[kMaxSpl];
   __shared__ unsigned int lcnt[kMaxBkt];
+  if ((int64_t)blockIdx.x >= *ntiles) return;
   PTile t = tiles[blockIdx.x];
   Part p = parts[t.part];
   int ns = nspl[t.part];
@@ -275,10 +372,12 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const PTile* tiles, c
                                                             const double* s0, const double* s1,
                                                             const double* s2, double* dst,
                                                             const double* spl, const int* nspl,
-                                                            unsigned long long* bcur) {
+                                                            unsigned long long* bcur,
+                                                            const int64_t* ntiles) {
   __shared__ double lspl[kMaxSpl];
   __shared__ unsigned int lcnt[kMaxBkt];
   __shared__ unsigned long long lbase[kMaxBkt];
+  if ((int64_t)blockIdx.x >= *ntiles) return;
   PTile t = tiles[blockIdx.x];
   Part p = parts[t.part];
   int ns = nspl[t.part];
@@ -317,11 +416,85 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const PTile* tiles, c
 
 // One final bucket (<= kCap values, or all equal) -> sorted values at the
 // segment's destination.
-__global__ __launch_bounds__(kBlock) void k_bucket_sort(const FBucket* fbs, const SegDesc* segs,
-                                                         const double* s0, const double* s1,
-                                                         const double* s2, double* x,
-                                                         double* vs) {
-  __shared__ double s[kCap];
+// Device-side routing of the partition rounds.  Counters: [0] small finals,
+// [1] large finals, [2] next-round parts, [3] elements in next-round parts.
+struct RouteCounters {
+  unsigned long long n_small, n_large, n_next, next_elems;
+};
+
+constexpr int kSmallCap = 1024;
+
+__device__ inline void push_final(const FBucket& f, FBucket* fs, FBucket* fl, RouteCounters* rc,
+                                  int64_t cap) {
+  bool small = !f.is_eq && f.cnt <= kSmallCap;
+  unsigned long long slot = atomicAdd(small ? &rc->n_small : &rc->n_large, 1ull);
+  if ((int64_t)slot < cap) (small ? fs : fl)[slot] = f;
+}
+
+// Segments of at most kCap new values are final buckets of their own; larger
+// ones become the parts of the first partition round.
+__global__ void k_seed(const SegDesc* segs, int nseg, FBucket* fs, FBucket* fl, Part* parts,
+                       RouteCounters* rc, int64_t cap) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  if (sd.v_cnt == 0) return;
+  if (sd.v_cnt <= kCap) {
+    push_final(FBucket{si, 0, 0, 0, sd.v_off, sd.v_cnt, 0}, fs, fl, rc, cap);
+  } else {
+    unsigned long long slot = atomicAdd(&rc->n_next, 1ull);
+    if ((int64_t)slot < cap) parts[slot] = Part{si, 0, sd.v_off, sd.v_cnt};
+    atomicAdd(&rc->next_elems, (unsigned long long)sd.v_cnt);
+  }
+}
+
+__global__ void k_part_tiles(const Part* parts, int np, int64_t* tcnt) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) tcnt[p] = (parts[p].cnt + kPTile - 1) / kPTile;
+}
+
+__global__ void k_make_tiles(const Part* parts, const int64_t* toff, PTile* tiles) {
+  int p = blockIdx.x;
+  Part q = parts[p];
+  int64_t nt = (q.cnt + kPTile - 1) / kPTile;
+  for (int64_t t = threadIdx.x; t < nt; t += blockDim.x)
+    tiles[toff[p] + t] = PTile{p, (int32_t)std::min<int64_t>(kPTile, q.cnt - t * kPTile),
+                               q.off + t * kPTile};
+}
+
+// After the scatter each bucket cursor points at the bucket's end.  Non-empty
+// buckets become final buckets (equality buckets, or at most kCap values) or
+// parts of the next round.
+__global__ void k_route(const Part* parts, const int* nspl, const unsigned long long* bcnt,
+                        const unsigned long long* bcur, const SegDesc* segs, int dst_id,
+                        FBucket* fs, FBucket* fl, Part* next, RouteCounters* rc, int64_t cap) {
+  int p = blockIdx.x;
+  Part q = parts[p];
+  int nb = 2 * nspl[p] + 1;
+  int64_t v_off = segs[q.seg].v_off;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int64_t c = (int64_t)bcnt[(int64_t)p * kMaxBkt + b];
+    if (c == 0) continue;
+    int64_t off = (int64_t)bcur[(int64_t)p * kMaxBkt + b] - c;
+    if ((b & 1) || c <= kCap) {
+      push_final(FBucket{q.seg, dst_id, (b & 1), 0, off, c, off - v_off}, fs, fl, rc, cap);
+    } else {
+      unsigned long long slot = atomicAdd(&rc->n_next, 1ull);
+      if ((int64_t)slot < cap) next[slot] = Part{q.seg, dst_id, off, c};
+      atomicAdd(&rc->next_elems, (unsigned long long)c);
+    }
+  }
+}
+
+// Size classes: buckets of up to kSmallCap values sort in one wave with a
+// 17 KB LDS footprint (many resident per CU); larger ones in a 256-thread
+// block with the full kCap tile.
+template <int CAP, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bucket_sort(const FBucket* fbs, const SegDesc* segs,
+                                                       const double* s0, const double* s1,
+                                                       const double* s2, double* x,
+                                                       double* vs) {
+  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
   FBucket fb = fbs[blockIdx.x];
   SegDesc sd = segs[fb.seg];
   const double* src = pick(fb.src, s0, s1, s2) + fb.off;
@@ -331,11 +504,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_sort(const FBucket* fbs, cons
     return;
   }
   int n = (int)fb.cnt;
-  int npad = pow2_at_least(n);
-  for (int i = threadIdx.x; i < npad; i += blockDim.x) s[i] = i < n ? src[i] : __builtin_inf();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sa[padi(i)] = src[i];
   __syncthreads();
-  bitonic_lds(s, npad);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = s[i];
+  const double* s = lds_sort(sa, sb, n);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = s[padi(i)];
 }
 
 // ------------------------------------------------------------ 3. E merge
@@ -458,7 +630,8 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
                                                const double* X, const int64_t* pose,
                                                const double* se, const double* esw,
                                                int64_t* cs, int64_t* ccount,
-                                               SegResult* res) {
+                                               SegResult* res, int64_t* long_list,
+                                               unsigned long long* long_cnt) {
   int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nseg) return;
   SegDesc sd = segs[si];
@@ -474,6 +647,15 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
   const double nrm = sd.nrm;
   PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
   int64_t s = 0, c = 0;
+  // Clusters spanning three or more checkpoint tiles are summed by
+  // k_long_sum (one block each); the rest by k_cluster_mean.
+  auto emit_long = [&](int64_t a, int64_t b) {
+    if (b / kTile - a / kTile >= 2) {
+      unsigned long long slot = atomicAdd(long_cnt, 1ull);
+      long_list[2 * slot] = si;
+      long_list[2 * slot + 1] = c - 1;
+    }
+  };
   while (s < n) {
     cs[sd.x_off + c] = s;
     c++;
@@ -569,6 +751,7 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
         }
       }
     }
+    emit_long(s, e);
     s = e;
   }
   ccount[si] = c;
@@ -634,23 +817,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int n
   }
 }
 
-// Per segment: checkpoint prefix cp[k] = sum of tiles < k (k = 0..ntiles).
-__global__ void k_tile_scan(const SegDesc* segs, int nseg, const double* tsum, double* cp) {
-  int si = blockIdx.x * blockDim.x + threadIdx.x;
-  if (si >= nseg) return;
-  SegDesc sd = segs[si];
-  int64_t n = sd.v_cnt + sd.e_cnt;
-  int64_t nt = (n + kTile - 1) / kTile;
-  double* o = cp + 2 * (sd.t_off + si);
-  dd acc{0.0, 0.0};
-  o[0] = 0.0;
-  o[1] = 0.0;
-  for (int64_t k = 0; k < nt; k++) {
-    acc = dd_add(acc, dd{tsum[2 * (sd.t_off + k)], tsum[2 * (sd.t_off + k) + 1]});
-    o[2 * (k + 1)] = acc.hi;
-    o[2 * (k + 1) + 1] = acc.lo;
-  }
-}
 
 __global__ void k_scan_counts(const int64_t* cnt, int n, int64_t* base, int64_t* total) {
   // Single block, chunked exclusive scan.
@@ -716,55 +882,132 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
   return acc;
 }
 
+// Clusters up to this many records replay the running centroid of the CPU
+// sweep record by record (bit-identical means); longer ones use an exact
+// double-double sum (means within an ulp or two).
+constexpr int64_t kReplayMax = 256;
+
+// One thread per cluster (grid-stride over the device-side cluster total);
+// clusters spanning three or more tiles are left to k_long_sum.
 __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cbase,
                                const int64_t* ccount, const int64_t* cs, const double* X,
                                const int64_t* pose, const double* se, const double* esw,
-                               const double* cp, double* cm, double* cw, int64_t total) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  int lo = 0, hi = nseg - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (cbase[mid] <= g) lo = mid;
-    else hi = mid - 1;
-  }
-  // Skip segments with no clusters that share the same base.
-  while (lo + 1 < nseg && cbase[lo + 1] <= g) lo++;
-  int si = lo;
-  SegDesc sd = segs[si];
-  int64_t c = g - cbase[si];
-  int64_t C = ccount[si];
-  int64_t n = sd.v_cnt + sd.e_cnt;
-  int64_t a = cs[sd.x_off + c];
-  int64_t b = (c + 1 < C) ? cs[sd.x_off + c + 1] : n;
-  const double* x = X + sd.x_off;
-  PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
-  const double* esw_seg = esw + sd.e_off;
-  int64_t ta = a / kTile, tb = b / kTile;
-  dd sum;
-  double w;
-  if (tb - ta <= 1) {
-    // Short cluster: weight and mean replay the running centroid of the CPU
-    // sweep record by record, so both are bit-identical to it.
-    double mean;
-    direct_sum(x, pc, esw_seg, a, b, &w, &mean);
+                               double* cm, double* cw, const int64_t* total_p) {
+  const int64_t total = *total_p;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {  // last segment whose first cluster index is <= g
+      int mid = (lo + hi + 1) >> 1;
+      if (cbase[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    int si = lo;
+    SegDesc sd = segs[si];
+    int64_t c = g - cbase[si];
+    int64_t C = ccount[si];
+    int64_t n = sd.v_cnt + sd.e_cnt;
+    int64_t a = cs[sd.x_off + c];
+    int64_t b = (c + 1 < C) ? cs[sd.x_off + c + 1] : n;
+    if (b / kTile - a / kTile >= 2) continue;
+    const double* x = X + sd.x_off;
+    PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
+    const double* esw_seg = esw + sd.e_off;
+    double w, mean;
+    if (b - a <= kReplayMax) {
+      direct_sum(x, pc, esw_seg, a, b, &w, &mean);
+    } else {
+      dd sum = direct_sum(x, pc, esw_seg, a, b, &w);
+      mean = dd_div(sum, w);
+    }
     cm[sd.x_off + c] = mean;
     cw[sd.x_off + c] = w;
-    return;
-  } else {
-    double wh, wt;
-    dd head = direct_sum(x, pc, esw_seg, a, (ta + 1) * kTile, &wh);
-    dd tail = direct_sum(x, pc, esw_seg, tb * kTile, b, &wt);
-    const double* cps = cp + 2 * (sd.t_off + si);
-    dd mid = dd_add(dd{cps[2 * tb], cps[2 * tb + 1]},
-                    dd_neg(dd{cps[2 * (ta + 1)], cps[2 * (ta + 1) + 1]}));
-    sum = dd_add(dd_add(head, mid), tail);
-    // Long cluster: difference of prefix weights (exact for integer weights).
-    w = pc.P(b) - pc.P(a);
   }
-  double mean = (b - a == 1) ? x[a] : dd_div(sum, w);
-  cm[sd.x_off + c] = mean;
-  cw[sd.x_off + c] = w;
+}
+
+// One block per long cluster [a, b): head [a, end of tile ta), the full tiles
+// ta+1 .. tb-1 from their tile sums, tail [start of tile tb, b); E records in
+// head and tail are corrected from sf to their own weight.
+__global__ __launch_bounds__(kBlock) void k_long_sum(const SegDesc* segs, const int64_t* ccount,
+                                                     const int64_t* cs, const double* X,
+                                                     const int64_t* pose, const double* se,
+                                                     const double* esw, const double* tsum,
+                                                     const int64_t* long_list,
+                                                     const unsigned long long* long_cnt,
+                                                     double* cm, double* cw) {
+  __shared__ double rh[kBlock], rl[kBlock];
+  const int64_t nl = (int64_t)*long_cnt;
+  for (int64_t li = blockIdx.x; li < nl; li += gridDim.x) {
+    int si = (int)long_list[2 * li];
+    int64_t c = long_list[2 * li + 1];
+    SegDesc sd = segs[si];
+    int64_t n = sd.v_cnt + sd.e_cnt;
+    int64_t C = ccount[si];
+    int64_t a = cs[sd.x_off + c];
+    int64_t b = (c + 1 < C) ? cs[sd.x_off + c + 1] : n;
+    int64_t ta = a / kTile, tb = b / kTile;
+    const double* x = X + sd.x_off;
+    dd acc{0.0, 0.0};
+    for (int64_t j = a + threadIdx.x; j < (ta + 1) * kTile; j += blockDim.x)
+      acc = dd_add(acc, two_prod(x[j], sd.sf));
+    for (int64_t j = tb * kTile + threadIdx.x; j < b; j += blockDim.x)
+      acc = dd_add(acc, two_prod(x[j], sd.sf));
+    for (int64_t t = ta + 1 + threadIdx.x; t < tb; t += blockDim.x)
+      acc = dd_add(acc, dd{tsum[2 * (sd.t_off + t)], tsum[2 * (sd.t_off + t) + 1]});
+    // Weight: sf per new value plus the E weights inside the cluster, summed
+    // directly (a difference of prefix sums would lose relative precision).
+    dd ewt{0.0, 0.0};
+    double ecount = 0.0;
+    if (sd.e_cnt > 0) {
+      const int64_t* pos = pose + sd.e_off;
+      const double* ew = esw + sd.e_off;
+      int64_t lo = 0, hi = sd.e_cnt;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (pos[mid] < a) lo = mid + 1;
+        else hi = mid;
+      }
+      for (int64_t e = lo + threadIdx.x; e < sd.e_cnt && pos[e] < b; e += blockDim.x) {
+        int64_t pe = pos[e];
+        ewt = dd_add(ewt, dd{ew[e], 0.0});
+        ecount += 1.0;
+        // Inside head or tail the record was summed with weight sf above;
+        // inside a full tile its tile sum already used the E weight.
+        if (pe < (ta + 1) * kTile || pe >= tb * kTile) {
+          double xe = x[pe];
+          acc = dd_add(acc, dd_add(two_prod(xe, ew[e]), dd_neg(two_prod(xe, sd.sf))));
+        }
+      }
+    }
+    __shared__ double re[kBlock], rc[kBlock], rf[kBlock];
+    rh[threadIdx.x] = acc.hi;
+    rl[threadIdx.x] = acc.lo;
+    re[threadIdx.x] = ewt.hi;
+    rf[threadIdx.x] = ewt.lo;
+    rc[threadIdx.x] = ecount;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        dd s2 = dd_add(dd{rh[threadIdx.x], rl[threadIdx.x]},
+                       dd{rh[threadIdx.x + w], rl[threadIdx.x + w]});
+        rh[threadIdx.x] = s2.hi;
+        rl[threadIdx.x] = s2.lo;
+        dd e2 = dd_add(dd{re[threadIdx.x], rf[threadIdx.x]},
+                       dd{re[threadIdx.x + w], rf[threadIdx.x + w]});
+        re[threadIdx.x] = e2.hi;
+        rf[threadIdx.x] = e2.lo;
+        rc[threadIdx.x] += rc[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      dd wsum = dd_add(two_prod(sd.sf, (double)(b - a) - rc[0]), dd{re[0], rf[0]});
+      double w = wsum.hi + wsum.lo;
+      cm[sd.x_off + c] = dd_div(dd{rh[0], rl[0]}, w);
+      cw[sd.x_off + c] = w;
+    }
+    __syncthreads();
+  }
 }
 
 // --------------------------------------------------------- 6. finish
@@ -886,43 +1129,75 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
   double* x = static_cast<double*>(x_.p);
   double* vs = static_cast<double*>(vs_.p);
   SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
+  auto* rc = as<RouteCounters>(route_, 1);
+  RouteCounters hrc;
 
-  std::vector<FBucket> finals;
-  std::vector<Part> parts;
-  for (int s = 0; s < nseg; s++) {
-    const SegDesc& sd = segs[s];
-    if (sd.v_cnt == 0) continue;
-    if (sd.v_cnt <= kCap) finals.push_back({s, 0, 0, 0, sd.v_off, sd.v_cnt, 0});
-    else parts.push_back({s, 0, sd.v_off, sd.v_cnt});
+  // Sorts the final buckets listed in (fs, fl); their data sits in buffers
+  // d_v / t1 / t2 that later rounds no longer write.
+  auto sort_finals = [&](const FBucket* fs, const FBucket* fl, const RouteCounters& c) {
+    if (c.n_small > 0) {
+      int ph_ = phase_begin("bucket_sort_small", 0.0);
+      k_bucket_sort<kSmallCap, 64><<<(unsigned)c.n_small, 64, 0, stream_>>>(fs, dsegs, d_v, t1, t2, x, vs);
+      launched("k_bucket_sort_small");
+      phase_end(ph_);
+    }
+    if (c.n_large > 0) {
+      int ph_ = phase_begin("bucket_sort", 0.0);
+      k_bucket_sort<kCap, kBlock><<<(unsigned)c.n_large, kBlock, 0, stream_>>>(fl, dsegs, d_v, t1, t2, x, vs);
+      launched("k_bucket_sort");
+      phase_end(ph_);
+    }
+  };
+
+  int64_t cap = nseg + 1;
+  FBucket* fs = as<FBucket>(fb_, cap);
+  FBucket* fl = as<FBucket>(fbl_, cap);
+  Part* parts = as<Part>(parts_, cap);
+  check_hip(hipMemsetAsync(rc, 0, sizeof(RouteCounters), stream_), "memset counters");
+  k_seed<<<(nseg + 255) / 256, 256, 0, stream_>>>(dsegs, nseg, fs, fl, parts, rc, cap);
+  launched("k_seed");
+  check_hip(hipMemcpyAsync(&hrc, rc, sizeof hrc, hipMemcpyDeviceToHost, stream_), "D2H counters");
+  check_hip(hipStreamSynchronize(stream_), "seed sync");
+  sort_finals(fs, fl, hrc);
+  if (hrc.n_next > 0) {
+    // The rounds below may regrow the final-bucket lists these sorts read.
+    check_hip(hipStreamSynchronize(stream_), "seed sort sync");
   }
+
   int src_id = 0;
   int round = 0;
-  std::vector<unsigned long long> hcnt;
-  std::vector<int> hns;
-  while (!parts.empty()) {
+  // Parts of the current round alternate between two device lists.
+  DevBuf* cur_list = &parts_;
+  DevBuf* nxt_list = &parts2_;
+  while (hrc.n_next > 0) {
     if (++round > 40) throw std::runtime_error("value sort: partition did not converge");
     int dst_id = (src_id == 1) ? 2 : 1;
     double* dst = dst_id == 1 ? t1 : t2;
-    int np = (int)parts.size();
-    std::vector<PTile> tiles;
-    int64_t round_elems = 0;
-    for (int p = 0; p < np; p++) round_elems += parts[p].cnt;
-    for (int p = 0; p < np; p++)
-      for (int64_t o = 0; o < parts[p].cnt; o += kPTile)
-        tiles.push_back({p, (int32_t)std::min<int64_t>(kPTile, parts[p].cnt - o), parts[p].off + o});
-    Part* dparts = as<Part>(parts_, np);
-    PTile* dtiles = as<PTile>(tiles_, tiles.size());
-    h_parts_.assign(reinterpret_cast<const char*>(parts.data()),
-                    reinterpret_cast<const char*>(parts.data() + np));
-    h_tiles_.assign(reinterpret_cast<const char*>(tiles.data()),
-                    reinterpret_cast<const char*>(tiles.data() + tiles.size()));
+    int np = (int)hrc.n_next;
+    int64_t round_elems = (int64_t)hrc.next_elems;
+    int64_t max_tiles = round_elems / kPTile + np;
+    // Each part yields at most kMaxBkt non-empty buckets and at most one per value.
+    int64_t rcap = std::min<int64_t>((int64_t)np * kMaxBkt, round_elems) + 1;
+    Part* dparts = static_cast<Part*>(cur_list->p);
+    int64_t* tcnt = as<int64_t>(tcnt_, 2 * (int64_t)np + 2);
+    int64_t* toff = tcnt + np;
+    int64_t* ntiles = toff + np;  // written by the scan as its total
+    PTile* dtiles = as<PTile>(tiles_, max_tiles);
     double* dspl = as<double>(spl_, (size_t)np * kMaxSpl);
     int* dns = as<int>(nspl_, np);
     auto* dbcnt = as<unsigned long long>(bcnt_, (size_t)np * kMaxBkt);
     auto* dbcur = as<unsigned long long>(bcur_, (size_t)np * kMaxBkt);
-    check_hip(hipMemcpyAsync(dparts, h_parts_.data(), h_parts_.size(), hipMemcpyHostToDevice, stream_), "H2D parts");
-    check_hip(hipMemcpyAsync(dtiles, h_tiles_.data(), h_tiles_.size(), hipMemcpyHostToDevice, stream_), "H2D tiles");
+    fs = as<FBucket>(fb_, rcap);
+    fl = as<FBucket>(fbl_, rcap);
+    Part* dnext = as<Part>(*nxt_list, rcap);
     check_hip(hipMemsetAsync(dbcnt, 0, (size_t)np * kMaxBkt * sizeof(unsigned long long), stream_), "memset");
+    check_hip(hipMemsetAsync(rc, 0, sizeof(RouteCounters), stream_), "memset counters");
+    k_part_tiles<<<(np + 255) / 256, 256, 0, stream_>>>(dparts, np, tcnt);
+    launched("k_part_tiles");
+    k_scan_counts<<<1, 1024, 0, stream_>>>(tcnt, np, toff, ntiles);
+    launched("k_scan_counts");
+    k_make_tiles<<<np, 256, 0, stream_>>>(dparts, toff, dtiles);
+    launched("k_make_tiles");
     const double* s0 = d_v;
     {
       int ph_ = phase_begin("splitters", 8.0 * (double)np * kCap);
@@ -932,7 +1207,8 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
     }
     {
       int ph_ = phase_begin("bucket_hist", 8.0 * (double)round_elems);
-      k_bucket_hist<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dspl, dns, dbcnt);
+      k_bucket_hist<<<(unsigned)max_tiles, kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dspl, dns,
+                                                                   dbcnt, ntiles);
       launched("k_bucket_hist");
       phase_end(ph_);
     }
@@ -940,46 +1216,22 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
     launched("k_bucket_scan");
     {
       int ph_ = phase_begin("bucket_scatter", 16.0 * (double)round_elems);
-      k_bucket_scatter<<<(unsigned)tiles.size(), kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dst, dspl, dns, dbcur);
+      k_bucket_scatter<<<(unsigned)max_tiles, kBlock, 0, stream_>>>(dtiles, dparts, s0, t1, t2, dst, dspl,
+                                                                      dns, dbcur, ntiles);
       launched("k_bucket_scatter");
       phase_end(ph_);
     }
-    hcnt.resize((size_t)np * kMaxBkt);
-    hns.resize(np);
-    check_hip(hipMemcpyAsync(hcnt.data(), dbcnt, hcnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_), "D2H counts");
-    check_hip(hipMemcpyAsync(hns.data(), dns, np * sizeof(int), hipMemcpyDeviceToHost, stream_), "D2H nspl");
-    check_hip(hipStreamSynchronize(stream_), "partition sync");
-    std::vector<Part> next;
-    for (int p = 0; p < np; p++) {
-      int nb = 2 * hns[p] + 1;
-      int64_t off = parts[p].off;
-      int s = parts[p].seg;
-      for (int b = 0; b < nb; b++) {
-        int64_t c = (int64_t)hcnt[(size_t)p * kMaxBkt + b];
-        if (c == 0) continue;
-        int64_t rank = off - segs[s].v_off;
-        if (b & 1) finals.push_back({s, dst_id, 1, 0, off, c, rank});
-        else if (c <= kCap) finals.push_back({s, dst_id, 0, 0, off, c, rank});
-        else next.push_back({s, dst_id, off, c});
-        off += c;
-      }
-    }
-    parts.swap(next);
+    k_route<<<np, 256, 0, stream_>>>(dparts, dns, dbcnt, dbcur, dsegs, dst_id, fs, fl, dnext, rc, rcap);
+    launched("k_route");
+    check_hip(hipMemcpyAsync(&hrc, rc, sizeof hrc, hipMemcpyDeviceToHost, stream_), "D2H counters");
+    check_hip(hipStreamSynchronize(stream_), "route sync");
+    if ((int64_t)hrc.n_small > rcap || (int64_t)hrc.n_large > rcap || (int64_t)hrc.n_next > rcap)
+      throw std::runtime_error("value sort: bucket list overflow");
+    sort_finals(fs, fl, hrc);
+    // The next round reuses fb_/fbl_: wait for this round's sorts to read them.
+    check_hip(hipStreamSynchronize(stream_), "sort sync");
+    std::swap(cur_list, nxt_list);
     src_id = dst_id;
-  }
-  if (!finals.empty()) {
-    int64_t final_elems = 0;
-    for (const auto& f : finals) final_elems += f.cnt;
-    FBucket* dfb = as<FBucket>(fb_, finals.size());
-    h_finals_.assign(reinterpret_cast<const char*>(finals.data()),
-                     reinterpret_cast<const char*>(finals.data() + finals.size()));
-    check_hip(hipMemcpyAsync(dfb, h_finals_.data(), h_finals_.size(), hipMemcpyHostToDevice, stream_), "H2D finals");
-    {
-      int ph_ = phase_begin("bucket_sort", 16.0 * (double)final_elems);
-      k_bucket_sort<<<(unsigned)finals.size(), kBlock, 0, stream_>>>(dfb, dsegs, d_v, t1, t2, x, vs);
-      launched("k_bucket_sort");
-      phase_end(ph_);
-    }
   }
 }
 
@@ -1022,9 +1274,8 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   int64_t* ccount = as<int64_t>(cnt_, 2 * (size_t)nseg + 1);
   int64_t* cbase = ccount + nseg;
   double* tsum = as<double>(tsum_, 2 * std::max<int64_t>(total_t, 1));
-  double* cp = as<double>(csum_, 2 * (total_t + nseg));
   int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
-  int64_t* dtotal = as<int64_t>(clist_, 1);
+  int64_t* dtotal = as<int64_t>(clist_, 2);
 
   check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
   if (!eseg.empty())
@@ -1038,7 +1289,12 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     }
   else
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
-  sort_values(segs, d_v, total_v);
+  {
+    auto t0 = std::chrono::steady_clock::now();
+    sort_values(segs, d_v, total_v);
+    host_phase("sort_values",
+               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
   if (!eseg.empty()) {
     int64_t chunks = std::min<int64_t>((max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
     for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
@@ -1052,39 +1308,42 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
       }
     }
   }
+  int64_t long_cap = total_x / kTile + nseg + 1;
+  int64_t* long_list = as<int64_t>(long_, 2 * long_cap);
+  auto* long_cnt = reinterpret_cast<unsigned long long*>(dtotal + 1);
+  check_hip(hipMemsetAsync(long_cnt, 0, sizeof(unsigned long long), stream_), "memset long count");
   {
     int ph_ = phase_begin("chain", 0.0);
-    k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount, dres);
+    k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount,
+                                                  dres, long_list, long_cnt);
     launched("k_chain");
     phase_end(ph_);
   }
   if (total_t > 0) {
-    {
-      int ph_ = phase_begin("tile_sums", 8.0 * (double)total_x);
-      k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
-      launched("k_tile_sums");
-      phase_end(ph_);
-    }
-    {
-      int ph_ = phase_begin("tile_scan", 32.0 * (double)total_t);
-      k_tile_scan<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, tsum, cp);
-      launched("k_tile_scan");
-      phase_end(ph_);
-    }
+    int ph_ = phase_begin("tile_sums", 8.0 * (double)total_x);
+    k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
+    launched("k_tile_sums");
+    phase_end(ph_);
   }
   k_scan_counts<<<1, 1024, 0, stream_>>>(ccount, nseg, cbase, dtotal);
   launched("k_scan_counts");
-  int64_t total_c = 0;
-  check_hip(hipMemcpyAsync(&total_c, dtotal, sizeof(int64_t), hipMemcpyDeviceToHost, stream_), "D2H total");
-  check_hip(hipStreamSynchronize(stream_), "sync clusters");
-  if (total_c > 0)
-    {
-      int ph_ = phase_begin("cluster_mean", 16.0 * (double)total_c);
-      k_cluster_mean<<<(unsigned)((total_c + 255) / 256), 256, 0, stream_>>>(
-        dsegs, nseg, cbase, ccount, cs, x, pose, se, esw, cp, cm, cw, total_c);
-      launched("k_cluster_mean");
-      phase_end(ph_);
-    }
+  {
+    // Upper bound of the cluster count without a host round trip.
+    unsigned grid = (unsigned)std::min<int64_t>((total_x + 255) / 256, 16384);
+    int ph_ = phase_begin("cluster_mean", 16.0 * (double)total_x / 8.0);
+    k_cluster_mean<<<std::max(grid, 1u), 256, 0, stream_>>>(dsegs, nseg, cbase, ccount, cs, x,
+                                                             pose, se, esw, cm, cw, dtotal);
+    launched("k_cluster_mean");
+    phase_end(ph_);
+  }
+  if (total_t > 2) {
+    unsigned grid = (unsigned)std::min<int64_t>(long_cap, 4096);
+    int ph_ = phase_begin("long_sum", 0.0);
+    k_long_sum<<<grid, kBlock, 0, stream_>>>(dsegs, ccount, cs, x, pose, se, esw, tsum, long_list,
+                                             long_cnt, cm, cw);
+    launched("k_long_sum");
+    phase_end(ph_);
+  }
   {
     int ph_ = phase_begin("finish", 0.0);
     k_finish<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
@@ -1092,10 +1351,11 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     phase_end(ph_);
   }
   check_hip(hipGetLastError(), "kernel launch");
-  if (sync_results) {
+  // The call ends synchronized: host staging read by the asynchronous copies
+  // above (segment table, E segment list) is then free to change.
+  if (sync_results)
     check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
-    check_hip(hipStreamSynchronize(stream_), "sync results");
-  }
+  check_hip(hipStreamSynchronize(stream_), "sync results");
 }
 
 // ------------------------------------------------------ key partition
