@@ -33,6 +33,7 @@ extern "C" {
 typedef struct kd_session kd_session;
 typedef struct kd_tdigest kd_tdigest;
 typedef struct kd_grouped kd_grouped;
+typedef struct kd_batch kd_batch;
 
 /* Thread-local description of the last error on this thread. */
 const char *kd_last_error(void);
@@ -149,6 +150,35 @@ int kd_grouped_centroids(kd_grouped *g, int64_t key, double *means, double *weig
                          int64_t cap, int64_t *count);
 /* Per-key scalar state {stored_total, log_scale, clock, min, max}. */
 int kd_grouped_key_state(kd_grouped *g, int64_t key, double *out5);
+
+/* ------------------------------------------- batched per-sample digests -- */
+/* num_digests independent TDigest[S] fed sample by sample with add_at
+ * (kdigest/tdigest.mojo:163-175, 324-350) and reduced by merge_into
+ * (tdigest.mojo:589-623); state resident in device memory, one workgroup per
+ * digest.  Capacity: 5 * compression <= 1536 buffered samples and at most 1536
+ * centroids per digest. */
+int kd_batch_create(kd_session *s, int scale, int64_t num_digests, double compression,
+                    double decay_lambda, kd_batch **out);
+void kd_batch_destroy(kd_batch *b);
+/* add_at(values[i], times[i], weights[i]) for i in [off[d], off[d+1]) of each
+ * digest d, in order.  Every sample is validated before any digest changes.
+ * Device pointers. */
+int kd_batch_add_at_device(kd_batch *b, const int64_t *d_off, const double *d_values,
+                           const double *d_times, const double *d_weights);
+/* Same with host arrays. */
+int kd_batch_add_at(kd_batch *b, const int64_t *off, const double *values, const double *times,
+                    const double *weights);
+/* merge() of every digest (tdigest.mojo:391-439). */
+int kd_batch_flush(kd_batch *b);
+/* merge() of every digest, then level by level digest i*2^(l+1) absorbs
+ * digest i*2^(l+1) + 2^l with merge_into; the result is digest 0. */
+int kd_batch_merge_tree(kd_batch *b);
+/* {stored_total, log_scale, clock, min, max, buffered samples, error flag}. */
+int kd_batch_state(kd_batch *b, int64_t d, double *out7);
+int kd_batch_centroids(kd_batch *b, int64_t d, double *means, double *weights, int64_t cap,
+                       int64_t *count);
+/* A TDigest holding a copy of digest d (centroids, buffer and state). */
+int kd_batch_to_tdigest(kd_batch *b, int64_t d, kd_tdigest **out);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ behind the C ABI of include/kdigest_hip.h).
 """
 from ._lib import KdigestError, LIB_PATH, load
 from .tdigest import (
+    BatchedDigests,
     Decay,
     GpuSession,
     GroupedTDigest,
@@ -26,6 +27,7 @@ from .tdigest import (
 )
 
 __all__ = [
+    "BatchedDigests",
     "Decay", "GpuSession", "GroupedTDigest", "KdigestError", "LIB_PATH", "ScaleFunction",
     "ScaleK0", "ScaleK1", "ScaleK2", "ScaleK3", "TDigest", "default_session", "gpu_cdf",
     "gpu_compress", "gpu_math_eval", "gpu_quantile", "gpu_scale_eval", "load",

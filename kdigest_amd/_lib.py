@@ -67,6 +67,15 @@ SIGNATURES = {
     "kd_grouped_total_weight": (_i, [_vp, _i64, _dp]),
     "kd_grouped_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
     "kd_grouped_key_state": (_i, [_vp, _i64, _dp]),
+    "kd_batch_create": (_i, [_vp, _i, _i64, _d, _d, C.POINTER(_vp)]),
+    "kd_batch_destroy": (None, [_vp]),
+    "kd_batch_add_at_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "kd_batch_add_at": (_i, [_vp, _i64p, _dp, _dp, _dp]),
+    "kd_batch_flush": (_i, [_vp]),
+    "kd_batch_merge_tree": (_i, [_vp]),
+    "kd_batch_state": (_i, [_vp, _i64, _dp]),
+    "kd_batch_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
+    "kd_batch_to_tdigest": (_i, [_vp, _i64, C.POINTER(_vp)]),
 }
 
 _lib: C.CDLL | None = None

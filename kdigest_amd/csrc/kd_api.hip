@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/kdigest_hip.h"
+#include "kd_batch.h"
 #include "kd_engine.h"
 #include "kd_math.h"
 
@@ -466,15 +467,15 @@ struct kd_tdigest {
   int64_t nc = 0;
 
   bool decay() const { return lambda > 0.0; }                       // decay.mojo:292-294
-  double scale_factor() const { return decay() ? std::exp(log_scale) : 1.0; }  // :352-357
+  double scale_factor() const { return decay() ? kd_exp(log_scale) : 1.0; }  // :352-357
   double effective_count() const {                                   // :373-377
-    return decay() ? stored_total * std::exp(-log_scale) : stored_total;
+    return decay() ? stored_total * kd_exp(-log_scale) : stored_total;
   }
   double* dm() { return static_cast<double*>(cm.p); }
   double* dw() { return static_cast<double*>(cw.p); }
 
   void renormalize() {  // tdigest.mojo:359-371
-    double factor = std::exp(-log_scale);
+    double factor = kd_exp(-log_scale);
     if (nc > 0) {
       k_scale_weights<<<blocks_for(nc), 256, 0, s->st()>>>(dw(), nc, factor);
       launched("k_scale_weights");
@@ -506,7 +507,7 @@ struct kd_tdigest {
     bw.clear();
     double eff = effective_count();
     double nrm = kd_normalizer(scale, compression, eff);
-    double floor = decay() ? kPruneEpsilon * std::exp(log_scale) : 0.0;
+    double floor = decay() ? kPruneEpsilon * kd_exp(log_scale) : 0.0;
     std::vector<SegDesc> segs(1);
     segs[0] = SegDesc{0, n, 0, ne, 0, 0, sf, stored_total, nrm, floor};
     std::vector<SegResult> res;
@@ -579,7 +580,7 @@ struct kd_tdigest {
     double* dst = s->buf<double>(s->q_state, 4);
     int64_t* dqd = s->buf<int64_t>(s->q_qd, nq);
     int64_t hoff[2] = {0, nc};
-    double hst[4] = {stored_total, vmin, vmax, decay() ? std::exp(-log_scale) : 1.0};
+    double hst[4] = {stored_total, vmin, vmax, decay() ? kd_exp(-log_scale) : 1.0};
     s->h2d(doff, hoff, sizeof hoff);
     s->h2d(dst, hst, sizeof hst);
     s->h2d(dq, qs, nq * sizeof(double));
@@ -877,7 +878,7 @@ struct kd_grouped {
   double* dm() { return static_cast<double*>(cm.p); }
   double* dw() { return static_cast<double*>(cw.p); }
   int64_t count(int64_t k) const { return hoff[k + 1] - hoff[k]; }
-  double eff(int64_t k) const { return decay() ? stored_total[k] * std::exp(-log_scale[k]) : stored_total[k]; }
+  double eff(int64_t k) const { return decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k]; }
 
   void check_key(int64_t k) const {
     if (k < 0 || k >= num_keys) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
@@ -954,7 +955,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
     if (decay() && nc_clock > clock[k]) {  // advance_clock (:324-332)
       log_scale[k] += lambda * (nc_clock - clock[k]);
       if (log_scale[k] > kRenormThreshold) {  // _renormalize (:359-371)
-        double factor = std::exp(-log_scale[k]);
+        double factor = kd_exp(-log_scale[k]);
         rkeys.push_back(k);
         rfac.push_back(factor);
         stored_total[k] *= factor;
@@ -962,10 +963,10 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
       }
     }
     clock[k] = nc_clock;
-    double sf = decay() ? std::exp(log_scale[k]) : 1.0;
+    double sf = decay() ? kd_exp(log_scale[k]) : 1.0;
     had_cent.push_back(count(k) > 0);
     stored_total[k] += (double)m * sf;
-    double effn = decay() ? stored_total[k] * std::exp(-log_scale[k]) : stored_total[k];
+    double effn = decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k];
     SegDesc sd{};
     sd.v_off = kof[k];
     sd.v_cnt = m;
@@ -974,7 +975,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
     sd.sf = sf;
     sd.stored_n = stored_total[k];
     sd.nrm = kd_normalizer(scale, compression, effn);
-    sd.floor = decay() ? kPruneEpsilon * std::exp(log_scale[k]) : 0.0;
+    sd.floor = decay() ? kPruneEpsilon * kd_exp(log_scale[k]) : 0.0;
     seg_of[k] = (int64_t)segs.size();
     segs.push_back(sd);
     seg_key.push_back(k);
@@ -1053,7 +1054,7 @@ void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int
       hs[k] = stored_total[k];
       hs[num_keys + k] = vmin[k];
       hs[2 * num_keys + k] = vmax[k];
-      hs[3 * num_keys + k] = decay() ? std::exp(-log_scale[k]) : 1.0;
+      hs[3 * num_keys + k] = decay() ? kd_exp(-log_scale[k]) : 1.0;
     }
     s->h2d(st, hs.data(), hs.size() * sizeof(double));
     prefix_valid = true;
@@ -1214,4 +1215,227 @@ extern "C" double kd_scale_q(int scale, double k, double nrm) {
       if (k <= 0.0) return std::exp(k / nrm) / 2.0;
       return 1.0 - std::exp(-k / nrm) / 2.0;
   }
+}
+
+// ======================================================= batched digests
+// Many independent digests fed sample by sample (add_at) and reduced by a
+// merge_into tree; state resident in device memory (kd_batch.hip).
+struct kd_batch {
+  kd_session* s;
+  int scale;
+  int64_t nd;
+  double compression, lambda;
+  int limit;
+  DevBuf st, cm, cw, bm, bw, order, flag;
+  int64_t* tmp_order = nullptr;
+};
+
+extern "C" int kd_batch_create(kd_session* s, int scale, int64_t num_digests, double compression,
+                               double lambda, kd_batch** out) {
+  return guarded([&] {
+    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+    if (num_digests <= 0) throw KdError(KD_EINVAL, "num_digests must be positive");
+    int64_t limit = (int64_t)compression * 5;  // tdigest.mojo:146
+    if (limit < 1 || limit > kd::batch_capacity_buffer())
+      throw KdError(KD_EINVAL, "compression out of range for the batched path (5*compression <= " +
+                                   std::to_string(kd::batch_capacity_buffer()) + ")");
+    std::unique_ptr<kd_batch> b(new kd_batch());
+    b->s = s;
+    b->scale = scale;
+    b->nd = num_digests;
+    b->compression = compression;
+    b->lambda = lambda;
+    b->limit = (int)limit;
+    size_t cc = (size_t)num_digests * kd::batch_capacity_centroids();
+    size_t bc = (size_t)num_digests * kd::batch_capacity_buffer();
+    b->st.reserve(num_digests * sizeof(kd::BatchState));
+    b->cm.reserve(cc * sizeof(double));
+    b->cw.reserve(cc * sizeof(double));
+    b->bm.reserve(bc * sizeof(double));
+    b->bw.reserve(bc * sizeof(double));
+    hip_ok(hipMemsetAsync(b->st.p, 0, num_digests * sizeof(kd::BatchState), s->st()), "memset");
+    s->sync();
+    *out = b.release();
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_batch_destroy(kd_batch* b) { delete b; }
+
+namespace {
+__global__ void k_validate_samples(const double* v, const double* w, int64_t n, int* flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool bad = !(v[i] - v[i] == 0.0) || !(w[i] > 0.0) || !(w[i] - w[i] == 0.0);
+  if (bad) atomicOr(flag, 1);
+}
+}  // namespace
+
+static void batch_add(kd_batch* b, const int64_t* d_off, const int64_t* h_off, const double* d_v,
+                      const double* d_t, const double* d_w) {
+  kd_session* s = b->s;
+  int64_t n = h_off[b->nd] - h_off[0];
+  if (n <= 0) return;
+  // Validate every sample before any digest changes (tdigest.mojo:317-322).
+  int* flag = static_cast<int*>(b->flag.reserve(sizeof(int)));
+  hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
+  k_validate_samples<<<blocks_for(n), 256, 0, s->st()>>>(d_v + h_off[0], d_w + h_off[0], n, flag);
+  launched("k_validate_samples");
+  int hflag = 0;
+  s->d2h(&hflag, flag, sizeof(int));
+  s->sync();
+  if (hflag) throw KdError(KD_EINVAL, "values must be finite and weights positive and finite");
+  // Largest streams first (LPT): they carry the longest sequential chains.
+  std::vector<int64_t> ord(b->nd);
+  for (int64_t d = 0; d < b->nd; d++) ord[d] = d;
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
+    return (h_off[x + 1] - h_off[x]) > (h_off[y + 1] - h_off[y]);
+  });
+  int64_t* dord = static_cast<int64_t*>(b->order.reserve(b->nd * sizeof(int64_t)));
+  s->h2d(dord, ord.data(), b->nd * sizeof(int64_t));
+  int ph = s->eng.phase_begin("replay", 24.0 * (double)n);
+  kd::launch_replay(s->st(), b->scale, b->compression, b->lambda, b->limit, dord, b->nd, d_off, d_v,
+                    d_t, d_w, static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
+                    static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
+                    static_cast<double*>(b->bw.p));
+  launched("k_replay");
+  s->eng.phase_end(ph);
+  s->sync();
+}
+
+extern "C" int kd_batch_add_at_device(kd_batch* b, const int64_t* d_off, const double* d_values,
+                                      const double* d_times, const double* d_weights) {
+  return guarded([&] {
+    std::vector<int64_t> h(b->nd + 1);
+    b->s->d2h(h.data(), d_off, (b->nd + 1) * sizeof(int64_t));
+    b->s->sync();
+    batch_add(b, d_off, h.data(), d_values, d_times, d_weights);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_add_at(kd_batch* b, const int64_t* off, const double* values,
+                               const double* times, const double* weights) {
+  return guarded([&] {
+    kd_session* s = b->s;
+    int64_t n = off[b->nd];
+    int64_t* doff = s->buf<int64_t>(s->tmp, b->nd + 1);
+    double* dv = s->buf<double>(s->g_vals, n);
+    double* dt = s->buf<double>(s->g_valsin, n);
+    double* dw = s->buf<double>(s->tmp2, n);
+    s->h2d(doff, off, (b->nd + 1) * sizeof(int64_t));
+    s->h2d(dv, values, n * sizeof(double));
+    s->h2d(dt, times, n * sizeof(double));
+    s->h2d(dw, weights, n * sizeof(double));
+    batch_add(b, doff, off, dv, dt, dw);
+    return KD_OK;
+  });
+}
+
+static void batch_check_err(kd_batch* b) {
+  std::vector<kd::BatchState> h(b->nd);
+  b->s->d2h(h.data(), b->st.p, b->nd * sizeof(kd::BatchState));
+  b->s->sync();
+  for (int64_t d = 0; d < b->nd; d++)
+    if (h[d].err)
+      throw KdError(KD_EINVAL, "digest " + std::to_string(d) + " exceeds " +
+                                   std::to_string(kd::batch_capacity_centroids()) +
+                                   " centroids (batched path capacity)");
+}
+
+extern "C" int kd_batch_flush(kd_batch* b) {
+  return guarded([&] {
+    int ph = b->s->eng.phase_begin("flush", 0.0);
+    kd::launch_flush(b->s->st(), b->scale, b->compression, b->lambda, b->nd,
+                     static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
+                     static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
+                     static_cast<double*>(b->bw.p));
+    launched("k_flush");
+    b->s->eng.phase_end(ph);
+    batch_check_err(b);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_merge_tree(kd_batch* b) {
+  return guarded([&] {
+    kd_session* s = b->s;
+    kd::launch_flush(s->st(), b->scale, b->compression, b->lambda, b->nd,
+                     static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
+                     static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
+                     static_cast<double*>(b->bw.p));
+    launched("k_flush");
+    int ph = s->eng.phase_begin("merge_tree", 0.0);
+    for (int64_t stride = 1; stride < b->nd; stride *= 2) {
+      kd::launch_merge_pairs(s->st(), b->scale, b->compression, b->lambda, b->nd, stride,
+                             static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
+                             static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
+                             static_cast<double*>(b->bw.p));
+      launched("k_merge_pairs");
+    }
+    s->eng.phase_end(ph);
+    batch_check_err(b);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_state(kd_batch* b, int64_t d, double* o) {
+  return guarded([&] {
+    if (d < 0 || d >= b->nd) throw KdError(KD_EINVAL, "digest out of range");
+    kd::BatchState h;
+    b->s->d2h(&h, static_cast<kd::BatchState*>(b->st.p) + d, sizeof h);
+    b->s->sync();
+    o[0] = h.stored_total;
+    o[1] = h.log_scale;
+    o[2] = h.clock;
+    o[3] = h.vmin;
+    o[4] = h.vmax;
+    o[5] = (double)h.nb;
+    o[6] = (double)h.err;
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_centroids(kd_batch* b, int64_t d, double* means, double* weights,
+                                  int64_t cap, int64_t* count) {
+  return guarded([&] {
+    if (d < 0 || d >= b->nd) throw KdError(KD_EINVAL, "digest out of range");
+    kd::BatchState h;
+    b->s->d2h(&h, static_cast<kd::BatchState*>(b->st.p) + d, sizeof h);
+    b->s->sync();
+    *count = h.nc;
+    if (cap < h.nc) return KD_OK;
+    int64_t base = d * kd::batch_capacity_centroids();
+    b->s->d2h(means, static_cast<double*>(b->cm.p) + base, h.nc * sizeof(double));
+    b->s->d2h(weights, static_cast<double*>(b->cw.p) + base, h.nc * sizeof(double));
+    b->s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_to_tdigest(kd_batch* b, int64_t d, kd_tdigest** out) {
+  return guarded([&] {
+    if (d < 0 || d >= b->nd) throw KdError(KD_EINVAL, "digest out of range");
+    kd::BatchState h;
+    b->s->d2h(&h, static_cast<kd::BatchState*>(b->st.p) + d, sizeof h);
+    b->s->sync();
+    std::unique_ptr<kd_tdigest> t(new_digest(b->s, b->scale, b->compression, b->lambda));
+    std::vector<double> m(h.nc), w(h.nc), bmv(h.nb), bwv(h.nb);
+    int64_t cb = d * kd::batch_capacity_centroids(), bb = d * kd::batch_capacity_buffer();
+    b->s->d2h(m.data(), static_cast<double*>(b->cm.p) + cb, h.nc * sizeof(double));
+    b->s->d2h(w.data(), static_cast<double*>(b->cw.p) + cb, h.nc * sizeof(double));
+    b->s->d2h(bmv.data(), static_cast<double*>(b->bm.p) + bb, h.nb * sizeof(double));
+    b->s->d2h(bwv.data(), static_cast<double*>(b->bw.p) + bb, h.nb * sizeof(double));
+    b->s->sync();
+    t->upload(m.data(), w.data(), h.nc);
+    t->bm = bmv;
+    t->bw = bwv;
+    t->stored_total = h.stored_total;
+    t->log_scale = h.log_scale;
+    t->clock = h.clock;
+    t->vmin = h.vmin;
+    t->vmax = h.vmax;
+    *out = t.release();
+    return KD_OK;
+  });
 }

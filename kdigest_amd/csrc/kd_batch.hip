@@ -1,0 +1,505 @@
+// kd_batch.hip — many independent digests fed sample by sample (add_at, A
+// semantics of kdigest/tdigest.mojo:163-175, 324-350) and reduced by a
+// pairwise merge_into tree (tdigest.mojo:589-623), all on the GPU.
+//
+// One workgroup per digest keeps the digest's centroids and ingest buffer in
+// LDS for the whole stream.  Per sample, the clock / decay exponent / stored
+// total bookkeeping is inherently sequential and runs in lane 0; the decay
+// factors, the stable sort, the weight prefix, the k values, the
+// next-boundary search and the cluster means run across the block.  Each
+// buffer merge (tdigest.mojo:391-439) is the same greedy sweep as the CPU:
+// the chain of cluster starts is walked by lane 0 over a next-boundary table
+// computed in parallel, and every cluster's running mean and weight are then
+// replayed record by record (bit-identical to the CPU for identical
+// boundaries).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "kd_batch.h"
+#include "kd_math.h"
+
+namespace kd {
+
+namespace {
+
+constexpr int kRB = 256;          // threads per digest
+constexpr int kRCMax = 1536;      // centroids kept per digest
+constexpr int kRBMax = 1536;      // ingest buffer capacity (>= 5 * compression)
+constexpr int kRNMax = kRCMax + kRBMax;
+constexpr double kRenorm = 400.0;   // tdigest.mojo:43
+constexpr double kPruneEps = 1e-6;  // tdigest.mojo:47
+
+// LDS layout (bytes) of one digest.
+constexpr int kOffCm = 0;
+constexpr int kOffCw = kOffCm + 8 * kRCMax;
+constexpr int kOffBm = kOffCw + 8 * kRCMax;
+constexpr int kOffBw = kOffBm + 8 * kRBMax;
+constexpr int kOffWm = kOffBw + 8 * kRBMax;
+constexpr int kOffWw = kOffWm + 8 * kRNMax;
+constexpr int kOffP = kOffWw + 8 * kRNMax;
+constexpr int kOffKv = kOffP + 8 * (kRNMax + 8);
+constexpr int kOffNx = kOffKv + 8 * (kRNMax + 8);
+constexpr int kLds = kOffNx + 4 * kRNMax;
+static_assert(kLds <= 160 * 1024, "LDS budget of one workgroup");
+
+struct Lds {
+  double *cm, *cw, *bm, *bw, *wm, *ww, *P, *kv;
+  int* nx;
+  // Sort scratch and cluster starts alias the k-value region.
+  double* skey;
+  int* sidx;
+  int* starts;
+};
+
+__device__ inline Lds carve(char* pool) {
+  Lds L;
+  L.cm = reinterpret_cast<double*>(pool + kOffCm);
+  L.cw = reinterpret_cast<double*>(pool + kOffCw);
+  L.bm = reinterpret_cast<double*>(pool + kOffBm);
+  L.bw = reinterpret_cast<double*>(pool + kOffBw);
+  L.wm = reinterpret_cast<double*>(pool + kOffWm);
+  L.ww = reinterpret_cast<double*>(pool + kOffWw);
+  L.P = reinterpret_cast<double*>(pool + kOffP);
+  L.kv = reinterpret_cast<double*>(pool + kOffKv);
+  L.nx = reinterpret_cast<int*>(pool + kOffNx);
+  L.skey = L.kv;
+  L.sidx = reinterpret_cast<int*>(pool + kOffKv + 8 * 2048);
+  L.starts = reinterpret_cast<int*>(pool + kOffKv);
+  return L;
+}
+
+__device__ inline uint64_t ord_key(double x) {
+  uint64_t b = (x == 0.0) ? 0ull : kd_d2bits(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Shared scalar state of the digest being processed (lane 0 writes).
+struct Sh {
+  BatchState st;
+  int m, renorm, nclusters;
+  double renorm_ls;
+};
+
+// Stable sort of the buffer by (mean, position): bitonic on the pair.
+__device__ void sort_buffer(Lds& L, int nb) {
+  int npad = 2;
+  while (npad < nb) npad <<= 1;
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+    L.skey[i] = i < nb ? L.bm[i] : __builtin_inf();
+    L.sidx[i] = i < nb ? i : 0x7fffffff;
+  }
+  __syncthreads();
+  for (int k = 2; k <= npad; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < (npad >> 1); t += blockDim.x) {
+        int i = 2 * t - (t & (j - 1));
+        int l = i + j;
+        double a = L.skey[i], b = L.skey[l];
+        int ia = L.sidx[i], ib = L.sidx[l];
+        bool a_first = (a < b) || (!(b < a) && ia < ib);
+        bool asc = (i & k) == 0;
+        if (asc != a_first) {
+          L.skey[i] = b;
+          L.skey[l] = a;
+          L.sidx[i] = ib;
+          L.sidx[l] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// merge() of tdigest.mojo:391-439 (+ prune 441-458) on the digest in LDS.
+// Returns false when the result would not fit kRCMax centroids.
+__device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double lambda) {
+  const int nc = sh.st.nc, nb = sh.st.nb;
+  if (nb == 0) return true;
+  const int n = nc + nb;
+  sort_buffer(L, nb);
+  // work = centroids ++ buffer, stably sorted: centroids first on ties.
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    double v = L.skey[k];
+    int lo = 0, hi = nc;  // # centroids <= v
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (!(v < L.cm[mid])) lo = mid + 1;
+      else hi = mid;
+    }
+    L.wm[k + lo] = v;
+    L.ww[k + lo] = L.bw[L.sidx[k]];
+  }
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    double v = L.cm[i];
+    int lo = 0, hi = nb;  // # buffer < v
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (L.skey[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    L.wm[i + lo] = v;
+    L.ww[i + lo] = L.cw[i];
+  }
+  __syncthreads();
+  // Weight prefix P[j] = sum of the first j weights (blocked scan).
+  {
+    const int chunk = (n + blockDim.x - 1) / blockDim.x;
+    const int a = min(n, (int)threadIdx.x * chunk), b = min(n, a + chunk);
+    double s = 0.0;
+    for (int j = a; j < b; j++) s += L.ww[j];
+    L.kv[threadIdx.x] = s;  // partials (kv is free here)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double run = 0.0;
+      for (int t = 0; t < (int)blockDim.x; t++) {
+        double v = L.kv[t];
+        L.kv[t] = run;
+        run += v;
+      }
+    }
+    __syncthreads();
+    double run = L.kv[threadIdx.x];
+    __syncthreads();
+    for (int j = a; j < b; j++) {
+      L.P[j] = run;
+      run += L.ww[j];
+    }
+    if (b == n && a < b) L.P[n] = run;  // the thread holding the last record
+  }
+  __syncthreads();
+  const double N = sh.st.stored_total;
+  const double eff = lambda > 0.0 ? N * kd_exp(-sh.st.log_scale) : N;
+  const double nrm = kd_normalizer(scale, compression, eff);
+  for (int j = threadIdx.x; j <= n; j += blockDim.x) L.kv[j] = kd_k(scale, L.P[j] / N, nrm);
+  __syncthreads();
+  // nx[s]: first candidate i > s with NOT(k(P[i+1]/N) - k(P[s]/N) <= 1), or n.
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    double ks = L.kv[s];
+    int lo = s + 2, hi = n + 1;  // search j = i + 1
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (!(L.kv[mid] - ks <= 1.0)) hi = mid;
+      else lo = mid + 1;
+    }
+    L.nx[s] = lo - 1;
+  }
+  __syncthreads();
+  // Chain of cluster starts with the tie rule (tdigest.mojo:429), lane 0.
+  if (threadIdx.x == 0) {
+    int s = 0, c = 0;
+    while (s < n) {
+      L.starts[c++] = s;
+      if (s == n - 1) break;
+      int b = L.nx[s];
+      int e = b;
+      if (b < n) {
+        double xb = L.wm[b], xp = L.wm[b - 1];
+        if (ord_key(xb) - ord_key(xp) <= 2) {
+          if (L.wm[s] == xb) {
+            while (e < n && L.wm[e] == xb) e++;
+          } else {
+            double mean = L.wm[s], wt = L.ww[s];
+            for (int j = s + 1; j < b; j++) {
+              double nw = wt + L.ww[j];
+              mean += (L.wm[j] - mean) * (L.ww[j] / nw);
+              wt = nw;
+            }
+            while (e < n && L.wm[e] == mean) e++;
+          }
+        }
+      }
+      s = e;
+    }
+    sh.nclusters = c;
+  }
+  __syncthreads();
+  const int C = sh.nclusters;
+  // Running mean and weight of every cluster, record by record
+  // (centroid.mojo:34-36).  P is free now: means go there; weights go to the
+  // consumed buffer (bm and bw are contiguous: kRNMax doubles).
+  double* om = L.P;
+  double* ow = L.bm;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    int a = L.starts[c], b = (c + 1 < C) ? L.starts[c + 1] : n;
+    double mean = L.wm[a], wt = L.ww[a];
+    for (int j = a + 1; j < b; j++) {
+      double nw = wt + L.ww[j];
+      mean += (L.wm[j] - mean) * (L.ww[j] / nw);
+      wt = nw;
+    }
+    om[c] = mean;
+    ow[c] = wt;
+  }
+  __syncthreads();
+  // Prune (decay active) and install.
+  if (threadIdx.x == 0) {
+    int k = 0;
+    bool fits = true;
+    if (lambda > 0.0) {
+      double floor = kPruneEps * kd_exp(sh.st.log_scale);
+      double removed = 0.0;
+      for (int c = 0; c < C; c++) {
+        if (ow[c] >= floor) {
+          if (k < kRCMax) {
+            L.cm[k] = om[c];
+            L.cw[k] = ow[c];
+          }
+          k++;
+        } else {
+          removed += ow[c];
+        }
+      }
+      if (removed > 0.0) sh.st.stored_total -= removed;
+    } else {
+      for (int c = 0; c < C; c++) {
+        if (k < kRCMax) {
+          L.cm[k] = om[c];
+          L.cw[k] = ow[c];
+        }
+        k++;
+      }
+    }
+    if (k > kRCMax) fits = false;
+    sh.st.nc = fits ? k : kRCMax;
+    sh.st.nb = 0;
+    if (!fits) sh.st.err = 1;
+  }
+  __syncthreads();
+  return sh.st.err == 0;
+}
+
+__device__ void renormalize_lds(Lds& L, Sh& sh, double log_scale) {
+  // tdigest.mojo:359-371.
+  double factor = kd_exp(-log_scale);
+  for (int i = threadIdx.x; i < sh.st.nc; i += blockDim.x) L.cw[i] *= factor;
+  for (int i = threadIdx.x; i < sh.st.nb; i += blockDim.x) L.bw[i] *= factor;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sh.st.stored_total *= factor;
+    sh.st.log_scale = 0.0;
+  }
+  __syncthreads();
+}
+
+__device__ void load_digest(Lds& L, Sh& sh, const BatchState* st, const double* gcm,
+                            const double* gcw, const double* gbm, const double* gbw, int64_t d) {
+  if (threadIdx.x == 0) {
+    sh.st = st[d];
+    sh.renorm = 0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < sh.st.nc; i += blockDim.x) {
+    L.cm[i] = gcm[d * kRCMax + i];
+    L.cw[i] = gcw[d * kRCMax + i];
+  }
+  for (int i = threadIdx.x; i < sh.st.nb; i += blockDim.x) {
+    L.bm[i] = gbm[d * kRBMax + i];
+    L.bw[i] = gbw[d * kRBMax + i];
+  }
+  __syncthreads();
+}
+
+__device__ void store_digest(Lds& L, Sh& sh, BatchState* st, double* gcm, double* gcw,
+                             double* gbm, double* gbw, int64_t d) {
+  for (int i = threadIdx.x; i < sh.st.nc; i += blockDim.x) {
+    gcm[d * kRCMax + i] = L.cm[i];
+    gcw[d * kRCMax + i] = L.cw[i];
+  }
+  for (int i = threadIdx.x; i < sh.st.nb; i += blockDim.x) {
+    gbm[d * kRBMax + i] = L.bm[i];
+    gbw[d * kRBMax + i] = L.bw[i];
+  }
+  if (threadIdx.x == 0) st[d] = sh.st;
+}
+
+}  // namespace
+
+// add_at for each sample of each digest, in order (tdigest.mojo:163-175).
+__global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, double lambda,
+                                                int limit, const int64_t* order, const int64_t* off,
+                                                const double* vals, const double* times,
+                                                const double* wts, BatchState* st, double* gcm,
+                                                double* gcw, double* gbm, double* gbw) {
+  __shared__ __attribute__((aligned(16))) char pool[kLds];
+  __shared__ Sh sh;
+  Lds L = carve(pool);
+  const int64_t d = order[blockIdx.x];
+  load_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
+  if (sh.st.err) return;
+  int64_t p = off[d];
+  const int64_t end = off[d + 1];
+  while (p < end || sh.renorm) {
+    // 1. Lane 0 walks samples: clock and decay exponent per sample
+    //    (advance_clock, tdigest.mojo:324-332), until the buffer is full, a
+    //    renormalization is due, or the samples run out.
+    if (threadIdx.x == 0) {
+      sh.renorm = 0;
+      int m = 0;
+      double ls = sh.st.log_scale, clock = sh.st.clock;
+      int room = limit - sh.st.nb;
+      while (p + m < end && m < room) {
+        double t = times[p + m];
+        if (t > clock) {
+          if (lambda > 0.0) {
+            double nls = ls + lambda * (t - clock);
+            clock = t;
+            if (nls > kRenorm) {
+              sh.renorm = 1;
+              sh.renorm_ls = nls;
+              break;
+            }
+            ls = nls;
+          } else {
+            clock = t;
+          }
+        }
+        L.bw[sh.st.nb + m] = ls;
+        m++;
+      }
+      sh.m = m;
+      sh.st.clock = clock;
+      sh.st.log_scale = ls;
+    }
+    __syncthreads();
+    const int m = sh.m;
+    const int nb0 = sh.st.nb;
+    // 2. Scaled weights w * e^(log_scale) in parallel (tdigest.mojo:345).
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      L.bm[nb0 + i] = vals[p + i];
+      double sf = lambda > 0.0 ? kd_exp(L.bw[nb0 + i]) : 1.0;
+      L.bw[nb0 + i] = wts[p + i] * sf;
+    }
+    __syncthreads();
+    // 3. Insert bookkeeping in order (tdigest.mojo:334-347).
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < m; i++) {
+        double v = L.bm[nb0 + i];
+        if (sh.st.stored_total == 0.0 && sh.st.nb == 0) {
+          sh.st.vmin = v;
+          sh.st.vmax = v;
+        } else {
+          if (v < sh.st.vmin) sh.st.vmin = v;
+          if (v > sh.st.vmax) sh.st.vmax = v;
+        }
+        sh.st.stored_total += L.bw[nb0 + i];
+        sh.st.nb++;
+      }
+    }
+    __syncthreads();
+    p += m;
+    if (sh.renorm) {
+      renormalize_lds(L, sh, sh.renorm_ls);
+      if (threadIdx.x == 0) sh.renorm = 0;
+      __syncthreads();
+      continue;  // the sample that triggered it is inserted by the next walk
+    }
+    if (sh.st.nb >= limit) {
+      if (!merge_lds(L, sh, scale, compression, lambda)) break;
+    }
+  }
+  store_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
+}
+
+// merge() of every digest (tdigest.mojo:391-439).
+__global__ __launch_bounds__(kRB) void k_flush(int scale, double compression, double lambda,
+                                               BatchState* st, double* gcm, double* gcw,
+                                               double* gbm, double* gbw) {
+  __shared__ __attribute__((aligned(16))) char pool[kLds];
+  __shared__ Sh sh;
+  Lds L = carve(pool);
+  const int64_t d = blockIdx.x;
+  load_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
+  if (sh.st.err || sh.st.nb == 0) return;
+  merge_lds(L, sh, scale, compression, lambda);
+  store_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
+}
+
+// self.merge_into(other) for digest pairs (self = base + 2 i stride, other =
+// self + stride); tdigest.mojo:589-623.  Both digests are already merged.
+__global__ __launch_bounds__(kRB) void k_merge_pairs(int scale, double compression, double lambda,
+                                                     int64_t nd, int64_t stride, BatchState* st,
+                                                     double* gcm, double* gcw, double* gbm,
+                                                     double* gbw) {
+  __shared__ __attribute__((aligned(16))) char pool[kLds];
+  __shared__ Sh sh;
+  __shared__ BatchState other;
+  Lds L = carve(pool);
+  const int64_t self = (int64_t)blockIdx.x * 2 * stride;
+  const int64_t oth = self + stride;
+  if (oth >= nd) return;
+  if (threadIdx.x == 0) other = st[oth];
+  __syncthreads();
+  if (other.err) {
+    if (threadIdx.x == 0) st[self].err = 1;
+    return;
+  }
+  // other.merge() happened in k_flush; effective count of other (:602).
+  double other_eff = lambda > 0.0 ? other.stored_total * kd_exp(-other.log_scale)
+                                  : other.stored_total;
+  if (other_eff == 0.0) return;
+  load_digest(L, sh, st, gcm, gcw, gbm, gbw, self);
+  if (sh.st.err) return;
+  // Renormalize both digests (decay active on both: same policy).
+  double ofac = lambda > 0.0 ? kd_exp(-other.log_scale) : 1.0;
+  if (lambda > 0.0) renormalize_lds(L, sh, sh.st.log_scale);
+  // Append other's centroids (renormalized) to the buffer (:611-614).
+  for (int i = threadIdx.x; i < other.nc; i += blockDim.x) {
+    L.bm[i] = gcm[oth * kRCMax + i];
+    double w = gcw[oth * kRCMax + i];
+    L.bw[i] = lambda > 0.0 ? w * ofac : w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ostored = lambda > 0.0 ? other.stored_total * ofac : other.stored_total;
+    bool was_empty = sh.st.stored_total == 0.0;
+    sh.st.nb = other.nc;
+    sh.st.stored_total += ostored;
+    if (was_empty) {
+      sh.st.vmin = other.vmin;
+      sh.st.vmax = other.vmax;
+    } else {
+      sh.st.vmin = fmin(sh.st.vmin, other.vmin);
+      sh.st.vmax = fmax(sh.st.vmax, other.vmax);
+    }
+    sh.st.clock = fmax(sh.st.clock, other.clock);
+  }
+  __syncthreads();
+  merge_lds(L, sh, scale, compression, lambda);
+  store_digest(L, sh, st, gcm, gcw, gbm, gbw, self);
+  // other is renormalized in place (tdigest.mojo:608-609 mutates it).
+  if (lambda > 0.0) {
+    for (int i = threadIdx.x; i < other.nc; i += blockDim.x) gcw[oth * kRCMax + i] *= ofac;
+    if (threadIdx.x == 0) {
+      st[oth].stored_total = other.stored_total * ofac;
+      st[oth].log_scale = 0.0;
+    }
+  }
+}
+
+int batch_capacity_centroids() { return kRCMax; }
+int batch_capacity_buffer() { return kRBMax; }
+
+void launch_replay(hipStream_t s, int scale, double compression, double lambda, int limit,
+                   const int64_t* order, int64_t nd, const int64_t* off, const double* vals,
+                   const double* times, const double* wts, BatchState* st, double* gcm,
+                   double* gcw, double* gbm, double* gbw) {
+  k_replay<<<(unsigned)nd, kRB, 0, s>>>(scale, compression, lambda, limit, order, off, vals, times,
+                                       wts, st, gcm, gcw, gbm, gbw);
+}
+
+void launch_flush(hipStream_t s, int scale, double compression, double lambda, int64_t nd,
+                  BatchState* st, double* gcm, double* gcw, double* gbm, double* gbw) {
+  k_flush<<<(unsigned)nd, kRB, 0, s>>>(scale, compression, lambda, st, gcm, gcw, gbm, gbw);
+}
+
+void launch_merge_pairs(hipStream_t s, int scale, double compression, double lambda, int64_t nd,
+                        int64_t stride, BatchState* st, double* gcm, double* gcw, double* gbm,
+                        double* gbw) {
+  int64_t pairs = (nd + 2 * stride - 1) / (2 * stride);
+  k_merge_pairs<<<(unsigned)pairs, kRB, 0, s>>>(scale, compression, lambda, nd, stride, st, gcm,
+                                               gcw, gbm, gbw);
+}
+
+}  // namespace kd

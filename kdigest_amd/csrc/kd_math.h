@@ -145,6 +145,56 @@ KD_HD double kd_asin(double x) {
   return x < 0.0 ? -res : res;
 }
 
+/* ---------------------------------------------------------------- exp ---- */
+/* fdlibm e_exp.c method: x = k ln2 + r with |r| <= ln2/2 (ln2 split in a
+ * high part with trailing zeros and a low part), exp(r) = 1 + r + r c/(2 - c)
+ * with c = r - r^2 R(r^2) (Remez degree 5), then scaling by 2^k.  Used for
+ * every decay factor e^(+-log_scale) on the host and on the device. */
+KD_HD double kd_exp(double x) {
+  const double invln2 = kd_bits2d(0x3ff71547652b82feull);
+  const double ln2hi = kd_bits2d(0x3fe62e42fee00000ull);
+  const double ln2lo = kd_bits2d(0x3dea39ef35793c76ull);
+  const double p1 = kd_bits2d(0x3fc555555555553eull);
+  const double p2 = kd_bits2d(0xbf66c16c16bebd93ull);
+  const double p3 = kd_bits2d(0x3f11566aaf25de2cull);
+  const double p4 = kd_bits2d(0xbebbbd41c5d26bf1ull);
+  const double p5 = kd_bits2d(0x3e66376972bea4d0ull);
+  const double o_thr = kd_bits2d(0x40862e42fefa39efull);
+  const double u_thr = kd_bits2d(0xc0874910d52d3051ull);
+  if (x != x) return x;
+  if (x > o_thr) return 1.0 / 0.0;
+  if (x < u_thr) return 0.0;
+  double ax = x < 0.0 ? -x : x;
+  if (ax < kd_bits2d(0x3e30000000000000ull)) return 1.0 + x; /* |x| < 2^-28 */
+  int k;
+  double hi, lo;
+  if (ax > kd_bits2d(0x3fd62e42fefa39efull)) { /* |x| > ln2/2 */
+    if (ax < kd_bits2d(0x3ff0a2b23f3bab73ull)) {                            /* |x| < 1.5 ln2 */
+      k = x < 0.0 ? -1 : 1;
+      hi = x < 0.0 ? x + ln2hi : x - ln2hi;
+      lo = x < 0.0 ? -ln2lo : ln2lo;
+    } else {
+      k = (int)(invln2 * x + (x < 0.0 ? -0.5 : 0.5));
+      double dk = (double)k;
+      hi = x - dk * ln2hi;
+      lo = dk * ln2lo;
+    }
+  } else {
+    k = 0;
+    hi = x;
+    lo = 0.0;
+  }
+  double r = hi - lo;
+  double t = r * r;
+  double c = r - t * (p1 + t * (p2 + t * (p3 + t * (p4 + t * p5))));
+  double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+  if (k == 0) return y;
+  if (k >= -1021) return kd_bits2d(kd_d2bits(y) + ((uint64_t)(int64_t)k << 52));
+  /* Subnormal result: scale in two steps. */
+  return kd_bits2d(kd_d2bits(y) + ((uint64_t)(int64_t)(k + 1000) << 52)) *
+         kd_bits2d(0x0170000000000000ull);
+}
+
 /* ------------------------------------------------------ scale functions -- */
 /* Scale ids: 0 = K0 (linear), 1 = K1 (asin), 2 = K2 (log-odds), 3 = K3. */
 enum { KD_K0 = 0, KD_K1 = 1, KD_K2 = 2, KD_K3 = 3 };

@@ -476,3 +476,64 @@ def gpu_math_eval(fn: str, xs, session=None) -> np.ndarray:
     check(load().kd_math_eval(session.handle, {"log": 0, "asin": 1, "sqrt": 2}[fn], _ptr(x), x.size,
                               _ptr(out)))
     return out
+
+
+# ------------------------------------------------------ batched digests
+class BatchedDigests:
+    """num_digests independent TDigest fed sample by sample with add_at
+    (tdigest.mojo:163-175) and reduced by a pairwise merge_into tree
+    (tdigest.mojo:589-623); state resident in device memory, one GPU
+    workgroup per digest."""
+
+    def __init__(self, num_digests: int, compression: float = 100.0, decay: Decay | None = None,
+                 scale=ScaleK1, session: GpuSession | None = None):
+        self._scale = _scale(scale)
+        self.num_digests = int(num_digests)
+        self.compression = float(compression)
+        self.decay = decay if decay is not None else Decay.none()
+        self.session = session or default_session()
+        h = C.c_void_p()
+        check(load().kd_batch_create(self.session.handle, self._scale.id, self.num_digests,
+                                     self.compression, self.decay.rate_lambda, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().kd_batch_destroy(h)
+            self._h = None
+
+    def add_at(self, off, values, times, weights) -> None:
+        """Digest d gets samples [off[d], off[d+1]) in order (host arrays)."""
+        o = np.ascontiguousarray(off, dtype=np.int64)
+        v, t, w = _f64(values), _f64(times), _f64(weights)
+        check(load().kd_batch_add_at(self._h, _ptr(o, _i64p), _ptr(v), _ptr(t), _ptr(w)))
+
+    def add_at_device(self, off, values, times, weights) -> None:
+        check(load().kd_batch_add_at_device(self._h, _device_ptr(off), _device_ptr(values),
+                                            _device_ptr(times), _device_ptr(weights)))
+
+    def flush(self) -> None:
+        check(load().kd_batch_flush(self._h))
+
+    def merge_tree(self) -> None:
+        check(load().kd_batch_merge_tree(self._h))
+
+    def state(self, d: int) -> dict:
+        o = np.empty(7)
+        check(load().kd_batch_state(self._h, int(d), _ptr(o)))
+        return dict(stored_total=o[0], log_scale=o[1], clock=o[2], min=o[3], max=o[4],
+                    buffered=int(o[5]), error=int(o[6]))
+
+    def centroids(self, d: int):
+        L = load()
+        n = C.c_int64()
+        check(L.kd_batch_centroids(self._h, int(d), None, None, 0, C.byref(n)))
+        m, w = np.empty(n.value), np.empty(n.value)
+        check(L.kd_batch_centroids(self._h, int(d), _ptr(m), _ptr(w), n.value, C.byref(n)))
+        return m, w
+
+    def to_tdigest(self, d: int) -> TDigest:
+        h = C.c_void_p()
+        check(load().kd_batch_to_tdigest(self._h, int(d), C.byref(h)))
+        return TDigest(self._scale, self.compression, self.decay, self.session, _handle=h.value)

@@ -7,8 +7,8 @@
  * Built with -O2 -ffp-contract=off so every expression rounds exactly as
  * written, in the operation order of the reference.  k() uses kd_math.h (the
  * same portable FP64 log/asin the device uses; its accuracy against glibc is
- * pinned by the tests); exp() for decay factors is glibc's, as in the host
- * layer of the GPU path.
+ * pinned by the tests).  Decay factors e^(+-log_scale) use kd_exp from the
+ * same header (fdlibm method), as the host layer and the device do.
  *
  * Reference files restated (paths relative to the reference root):
  *   src/kdigest/centroid.mojo:34-40    incremental weighted mean
@@ -66,6 +66,7 @@ double ko_k(int scale, double q, double nrm) { return K_FN(scale, q, nrm); }
 double ko_normalizer(int scale, double c, double n) { return NORM_FN(scale, c, n); }
 double ko_log(double x) { return kd_log(x); }
 double ko_asin(double x) { return kd_asin(x); }
+double ko_exp(double x) { return kd_exp(x); }
 
 /* centroid.mojo:25-40 */
 static inline void c_add_sample(ko_centroid *c, double value, double w) {
@@ -132,7 +133,7 @@ static inline int validate(double value, double weight) {
 
 /* tdigest.mojo:359-371 */
 static void renormalize(ko_digest *d) {
-  double factor = exp(-d->log_scale);
+  double factor = kd_exp(-d->log_scale);
   for (int64_t i = 0; i < d->nc; i++) d->cent[i].weight *= factor;
   for (int64_t i = 0; i < d->nb; i++) d->buf[i].weight *= factor;
   d->stored_total *= factor;
@@ -151,13 +152,13 @@ static void advance_clock(ko_digest *d, double new_clock) {
 /* tdigest.mojo:352-357 */
 static inline double scale_factor(const ko_digest *d) {
   if (!decay_active(d)) return 1.0;
-  return exp(d->log_scale);
+  return kd_exp(d->log_scale);
 }
 
 /* tdigest.mojo:373-377 */
 static inline double effective_count(const ko_digest *d) {
   if (!decay_active(d)) return d->stored_total;
-  return d->stored_total * exp(-d->log_scale);
+  return d->stored_total * kd_exp(-d->log_scale);
 }
 
 /* Stable merge sort by mean: the permutation of the stable insertion sort at
@@ -225,7 +226,7 @@ int64_t ko_sweep(int scale, double nrm, double stored_n, const double *means,
 
 /* tdigest.mojo:441-458 */
 static void prune(ko_digest *d) {
-  double stored_floor = PRUNE_EPSILON * exp(d->log_scale);
+  double stored_floor = PRUNE_EPSILON * kd_exp(d->log_scale);
   double removed = 0.0;
   int64_t k = 0;
   for (int64_t i = 0; i < d->nc; i++) {
@@ -372,7 +373,7 @@ double ko_total_weight(const ko_digest *d) { return effective_count(d); } /* :46
 
 /* tdigest.mojo:475-483 */
 static inline int is_singleton(const ko_digest *d, double stored) {
-  double eff = decay_active(d) ? stored * exp(-d->log_scale) : stored;
+  double eff = decay_active(d) ? stored * kd_exp(-d->log_scale) : stored;
   return eff <= 1.0 + 1e-9;
 }
 

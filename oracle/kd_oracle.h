@@ -99,6 +99,7 @@ double ko_k(int scale, double q, double nrm);
 double ko_normalizer(int scale, double compression, double n);
 double ko_log(double x);
 double ko_asin(double x);
+double ko_exp(double x);
 
 /* Exact rank error: |#(x <= v)/n - q| with x sorted ascending. */
 double ko_rank_error(const double *sorted, int64_t n, double v, double q);

@@ -62,6 +62,7 @@ def lib(variant: str = "kd") -> C.CDLL:
         "ko_normalizer": (d, [i32, d, d]),
         "ko_log": (d, [d]),
         "ko_asin": (d, [d]),
+        "ko_exp": (d, [d]),
         "ko_rank_error": (d, [_dp, i64, d, d]),
     }
     for fn, (res, args) in sig.items():
@@ -224,3 +225,7 @@ def kd_asin(x):
 def rank_error(sorted_data: np.ndarray, v: float, q: float) -> float:
     a = np.ascontiguousarray(sorted_data, np.float64)
     return lib("kd").ko_rank_error(_ptr(a), a.size, v, q)
+
+
+def kd_exp(x):
+    return lib("kd").ko_exp(x)
