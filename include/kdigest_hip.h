@@ -34,6 +34,7 @@ typedef struct kd_session kd_session;
 typedef struct kd_tdigest kd_tdigest;
 typedef struct kd_grouped kd_grouped;
 typedef struct kd_batch kd_batch;
+typedef struct kd_server kd_server;
 
 /* Thread-local description of the last error on this thread. */
 const char *kd_last_error(void);
@@ -179,6 +180,33 @@ int kd_batch_centroids(kd_batch *b, int64_t d, double *means, double *weights, i
                        int64_t *count);
 /* A TDigest holding a copy of digest d (centroids, buffer and state). */
 int kd_batch_to_tdigest(kd_batch *b, int64_t d, kd_tdigest **out);
+
+/* --------------------------------------------------------------- serving -- */
+/* Continuous-batching server over num_keys digests resident in device
+ * memory (GroupedTDigest semantics per key, grouped.mojo:60-190).  Each key
+ * holds at most slot_capacity centroids; requests are admitted in arrival
+ * order while their data fits mem_budget bytes.  kind: 0 ingest batch (one B
+ * merge, tdigest.mojo:223-315), 1 quantile batch, 2 cdf batch
+ * (tdigest.mojo:485-587).  Requests of one key run in arrival order. */
+int kd_server_create(kd_session *s, int scale, int64_t num_keys, double compression,
+                     double decay_lambda, int64_t slot_capacity, int64_t mem_budget,
+                     kd_server **out);
+void kd_server_destroy(kd_server *v);
+/* Copies the request data; validates it before queueing. */
+int kd_server_submit(kd_server *v, int kind, int64_t key, const double *data, int64_t n,
+                     int64_t *id);
+/* Admits and runs one continuous batch: all admitted queries in one launch,
+ * then up to max_rounds rounds of ingests (one per key per round). */
+int kd_server_step(kd_server *v, int max_rounds, int64_t *completed);
+/* *done = 1 when the request completed; query results are copied to out
+ * (cap entries) and the request is then forgotten.  A query that found its
+ * key empty completes with KD_EEMPTY (tdigest.mojo:485-487). */
+int kd_server_result(kd_server *v, int64_t id, double *out, int64_t cap, int *done);
+int kd_server_pending(kd_server *v, int64_t *out);
+/* {stored_total, log_scale, clock, min, max, centroids} of one key. */
+int kd_server_key_state(kd_server *v, int64_t key, double *out6);
+int kd_server_key_centroids(kd_server *v, int64_t key, double *means, double *weights,
+                            int64_t cap, int64_t *count);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@ from ._lib import KdigestError, LIB_PATH, load
 from .tdigest import (
     BatchedDigests,
     Decay,
+    DigestServer,
     GpuSession,
     GroupedTDigest,
     ScaleFunction,
@@ -27,7 +28,7 @@ from .tdigest import (
 )
 
 __all__ = [
-    "BatchedDigests",
+    "BatchedDigests", "DigestServer",
     "Decay", "GpuSession", "GroupedTDigest", "KdigestError", "LIB_PATH", "ScaleFunction",
     "ScaleK0", "ScaleK1", "ScaleK2", "ScaleK3", "TDigest", "default_session", "gpu_cdf",
     "gpu_compress", "gpu_math_eval", "gpu_quantile", "gpu_scale_eval", "load",

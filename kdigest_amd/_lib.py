@@ -76,6 +76,14 @@ SIGNATURES = {
     "kd_batch_state": (_i, [_vp, _i64, _dp]),
     "kd_batch_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
     "kd_batch_to_tdigest": (_i, [_vp, _i64, C.POINTER(_vp)]),
+    "kd_server_create": (_i, [_vp, _i, _i64, _d, _d, _i64, _i64, C.POINTER(_vp)]),
+    "kd_server_destroy": (None, [_vp]),
+    "kd_server_submit": (_i, [_vp, _i, _i64, _dp, _i64, _i64p]),
+    "kd_server_step": (_i, [_vp, _i, _i64p]),
+    "kd_server_result": (_i, [_vp, _i64, _dp, _i64, C.POINTER(_i)]),
+    "kd_server_pending": (_i, [_vp, _i64p]),
+    "kd_server_key_state": (_i, [_vp, _i64, _dp]),
+    "kd_server_key_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
 }
 
 _lib: C.CDLL | None = None

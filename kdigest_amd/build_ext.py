@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkdigest_hip.so")
-SOURCES = ["kd_engine.hip", "kd_batch.hip", "kd_api.hip"]
+SOURCES = ["kd_engine.hip", "kd_batch.hip", "kd_serve.hip", "kd_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

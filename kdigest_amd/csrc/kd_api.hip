@@ -21,11 +21,16 @@
 #include "kd_batch.h"
 #include "kd_engine.h"
 #include "kd_math.h"
+#include "kd_query.h"
+#include "kd_serve.h"
 
 using kd::DevBuf;
 using kd::SegDesc;
 using kd::SegResult;
 using kd::launched;
+using kd::QState;
+using kd::cdf_one;
+using kd::quantile_one;
 
 namespace {
 
@@ -103,85 +108,6 @@ __global__ void k_query_prefix(int64_t nd, const int64_t* off, const double* w, 
     acc += (w[i] + w[i + 1]) / 2.0;
     S[i + 1] = acc;
   }
-}
-
-struct QState {
-  double total, vmin, vmax, eff;
-};
-
-__device__ inline bool singleton(double stored, double eff) { return stored * eff <= 1.0 + 1e-9; }
-
-// tdigest.mojo:485-538 for one query.
-__device__ double quantile_one(const double* m, const double* w, const double* S, int64_t nc,
-                               QState st, double q) {
-  if (nc == 0) return __builtin_nan("");
-  if (nc == 1) return m[0];
-  double n = st.total;
-  if (q <= 0.0) return st.vmin;
-  if (q >= 1.0) return st.vmax;
-  double target = q * n;
-  double first_half = w[0] / 2.0;
-  if (target < first_half) {
-    if (singleton(w[0], st.eff)) return st.vmin;
-    return st.vmin + (m[0] - st.vmin) * (target / first_half);
-  }
-  double lm = m[nc - 1], lw = w[nc - 1];
-  double last_half = lw / 2.0;
-  if (target > n - last_half) {
-    if (singleton(lw, st.eff)) return st.vmax;
-    double frac = (target - (n - last_half)) / last_half;
-    return lm + (st.vmax - lm) * frac;
-  }
-  // First i in [0, nc-2] with target <= S[i+1].
-  int64_t lo = 1, hi = nc;  // search j = i+1 in [1, nc-1]; hi == nc means none
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (target <= S[mid]) hi = mid;
-    else lo = mid + 1;
-  }
-  if (lo >= nc) return lm;
-  int64_t i = lo - 1;
-  double dw = (w[i] + w[i + 1]) / 2.0;
-  double frac = (target - S[i]) / dw;
-  return m[i] + (m[i + 1] - m[i]) * frac;
-}
-
-// tdigest.mojo:540-587 for one query.
-__device__ double cdf_one(const double* m, const double* w, const double* S, int64_t nc,
-                          QState st, double x) {
-  if (nc == 0) return __builtin_nan("");
-  if (x < st.vmin) return 0.0;
-  if (x > st.vmax) return 1.0;
-  if (nc == 1) {
-    if (x < m[0]) return 0.0;
-    if (x > m[0]) return 1.0;
-    return 0.5;
-  }
-  double n = st.total;
-  if (x < m[0]) {
-    if (st.vmin == m[0]) return 0.0;
-    double frac = (x - st.vmin) / (m[0] - st.vmin);
-    return (w[0] / 2.0) * frac / n;
-  }
-  double lm = m[nc - 1], lw = w[nc - 1];
-  if (x > lm) {
-    if (st.vmax == lm) return 1.0;
-    double frac = (x - lm) / (st.vmax - lm);
-    return (n - lw / 2.0 + (lw / 2.0) * frac) / n;
-  }
-  // First i in [0, nc-2] with x <= m[i+1].
-  int64_t lo = 1, hi = nc;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (x <= m[mid]) hi = mid;
-    else lo = mid + 1;
-  }
-  if (lo >= nc) return 1.0;
-  int64_t i = lo - 1;
-  double dw = (w[i] + w[i + 1]) / 2.0;
-  if (m[i + 1] == m[i]) return (S[i] + dw / 2.0) / n;
-  double frac = (x - m[i]) / (m[i + 1] - m[i]);
-  return (S[i] + dw * frac) / n;
 }
 
 __global__ void k_query(int is_cdf, const int64_t* off, const double* m, const double* w,
@@ -359,7 +285,7 @@ extern "C" int kd_compress(kd_session* s, int scale, double nrm, double stored_n
     s->h2d(ew, weights, n_prior * sizeof(double));
     s->h2d(v, means + n_prior, n * sizeof(double));
     std::vector<SegDesc> segs(1);
-    segs[0] = SegDesc{0, n, 0, n_prior, 0, 0, w_uniform, stored_n, nrm, 0.0};
+    segs[0] = SegDesc{0, n, 0, n_prior, 0, 0, 0, w_uniform, stored_n, nrm, 0.0};
     std::vector<SegResult> res;
     s->eng.compress(scale, segs, v, em, ew, res);
     int64_t c = res[0].count;
@@ -509,7 +435,7 @@ struct kd_tdigest {
     double nrm = kd_normalizer(scale, compression, eff);
     double floor = decay() ? kPruneEpsilon * kd_exp(log_scale) : 0.0;
     std::vector<SegDesc> segs(1);
-    segs[0] = SegDesc{0, n, 0, ne, 0, 0, sf, stored_total, nrm, floor};
+    segs[0] = SegDesc{0, n, 0, ne, 0, 0, 0, sf, stored_total, nrm, floor};
     std::vector<SegResult> res;
     s->eng.compress(scale, segs, d_values, em, ew, res);
     int64_t c = res[0].count;
@@ -1436,6 +1362,102 @@ extern "C" int kd_batch_to_tdigest(kd_batch* b, int64_t d, kd_tdigest** out) {
     t->vmin = h.vmin;
     t->vmax = h.vmax;
     *out = t.release();
+    return KD_OK;
+  });
+}
+
+// ================================================================ serving
+struct kd_server {
+  kd_session* s;
+  std::unique_ptr<kd::Server> srv;
+};
+
+extern "C" int kd_server_create(kd_session* s, int scale, int64_t num_keys, double compression,
+                                double lambda, int64_t slot_capacity, int64_t mem_budget,
+                                kd_server** out) {
+  return guarded([&] {
+    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+    if (num_keys <= 0) throw KdError(KD_EINVAL, "num_keys must be positive");
+    if (slot_capacity <= 0 || slot_capacity > kd::kECap)
+      throw KdError(KD_EINVAL, "slot capacity must be in 1.." + std::to_string(kd::kECap));
+    if (mem_budget <= 0) throw KdError(KD_EINVAL, "memory budget must be positive");
+    std::unique_ptr<kd_server> v(new kd_server());
+    v->s = s;
+    v->srv.reset(new kd::Server(s->eng, scale, num_keys, compression, lambda, slot_capacity,
+                                mem_budget));
+    *out = v.release();
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_server_destroy(kd_server* v) { delete v; }
+
+extern "C" int kd_server_submit(kd_server* v, int kind, int64_t key, const double* data, int64_t n,
+                                int64_t* id) {
+  return guarded([&] {
+    if (kind < 0 || kind > 2) throw KdError(KD_EINVAL, "kind must be 0 (ingest), 1 (quantile) or 2 (cdf)");
+    if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    if (n <= 0) throw KdError(KD_EINVAL, "empty request");
+    for (int64_t i = 0; i < n; i++) {
+      if (kind == 0 && !is_finite_val(data[i]))
+        throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+      if (kind == 1 && !(data[i] >= 0.0 && data[i] <= 1.0))
+        throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+    }
+    *id = v->srv->submit(kind, key, data, n);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_step(kd_server* v, int max_rounds, int64_t* completed) {
+  return guarded([&] {
+    *completed = v->srv->step(max_rounds);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_result(kd_server* v, int64_t id, double* out, int64_t cap, int* done) {
+  return guarded([&] {
+    std::vector<double> r;
+    bool empty = false;
+    *done = v->srv->result(id, r, empty, false) ? 1 : 0;
+    if (*done) {
+      if (empty) {
+        v->srv->result(id, r, empty, true);
+        throw KdError(KD_EEMPTY, "query on a key whose digest is empty");
+      }
+      if (cap < (int64_t)r.size()) throw KdError(KD_EINVAL, "output capacity below the request size");
+      if (!r.empty()) memcpy(out, r.data(), r.size() * sizeof(double));
+      v->srv->result(id, r, empty, true);
+    }
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_pending(kd_server* v, int64_t* out) {
+  *out = v->srv->pending();
+  return KD_OK;
+}
+
+extern "C" int kd_server_key_state(kd_server* v, int64_t key, double* out6) {
+  return guarded([&] {
+    if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    v->srv->key_state(key, out6);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_key_centroids(kd_server* v, int64_t key, double* means, double* weights,
+                                       int64_t cap, int64_t* count) {
+  return guarded([&] {
+    if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    std::vector<double> m, w;
+    v->srv->key_centroids(key, m, w);
+    *count = (int64_t)m.size();
+    if (cap >= *count && *count > 0) {
+      memcpy(means, m.data(), m.size() * sizeof(double));
+      memcpy(weights, w.data(), w.size() * sizeof(double));
+    }
     return KD_OK;
   });
 }

@@ -31,6 +31,7 @@ struct SegDesc {
   int64_t e_off, e_cnt;   // explicit records (input order)
   int64_t x_off;          // merged region [x_off, x_off + e_cnt + v_cnt), also centroid output
   int64_t t_off;          // first checkpoint tile of the segment
+  int64_t es_off;         // sorted E records (engine-assigned, compact)
   double sf, stored_n, nrm, floor;
 };
 

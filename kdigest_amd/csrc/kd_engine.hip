@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
   SegDesc sd = segs[s];
   int n = (int)sd.e_cnt;
   if (n == 0) {
-    if (threadIdx.x == 0) se[sd.e_off + s] = 0.0;
+    if (threadIdx.x == 0) se[sd.es_off + s] = 0.0;
     return;
   }
   int npad = pow2_at_least(n);
@@ -261,13 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
     }
   }
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    esm[sd.e_off + i] = em[sd.e_off + idx[i]];
-    esw[sd.e_off + i] = ew[sd.e_off + idx[i]];
+    esm[sd.es_off + i] = em[sd.e_off + idx[i]];
+    esw[sd.es_off + i] = ew[sd.e_off + idx[i]];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double acc = 0.0;
-    double* p = se + sd.e_off + s;
+    double* p = se + sd.es_off + s;
     p[0] = 0.0;
     for (int i = 0; i < n; i++) {
       acc += ew[sd.e_off + idx[i]];
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_e(const SegDesc* segs, const i
   if (si >= neseg) return;
   SegDesc sd = segs[eseg[si]];
   int64_t tot = sd.v_cnt + sd.e_cnt;
-  const double* e = esm + sd.e_off;
+  const double* e = esm + sd.es_off;
   const double* v = vs + sd.v_off;
   for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < tot;
        it += (int64_t)gridDim.x * blockDim.x) {
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_e(const SegDesc* segs, const i
         else hi = mid;
       }
       x[sd.x_off + i + lo] = xe;
-      pose[sd.e_off + i] = i + lo;
+      pose[sd.es_off + i] = i + lo;
     }
   }
 }
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
   const double* x = X + sd.x_off;
   const double N = sd.stored_n;
   const double nrm = sd.nrm;
-  PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
+  PCur pc{pose + sd.es_off, se + sd.es_off + si, sd.e_cnt, sd.sf, 0};
   int64_t s = 0, c = 0;
   // Clusters spanning three or more checkpoint tiles are summed by
   // k_long_sum (one block each); the rest by k_cluster_mean.
@@ -735,7 +735,7 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
           for (int64_t j = s; j < b; j++) {
             double w = pc.sf;
             if (ce < wc.ne && wc.pos[ce] == j) {
-              w = esw[sd.e_off + ce];
+              w = esw[sd.es_off + ce];
               ce++;
             }
             if (j == s) {
@@ -783,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int n
   for (int i = threadIdx.x; i < kTile; i += blockDim.x) wl[i] = sd.sf;
   __syncthreads();
   if (sd.e_cnt > 0) {
-    const int64_t* pos = pose + sd.e_off;
+    const int64_t* pos = pose + sd.es_off;
     // E records inside [r0, r0 + len).
     int64_t lo = 0, hi = sd.e_cnt;
     while (lo < hi) {
@@ -792,7 +792,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int n
       else hi = mid;
     }
     for (int64_t c = lo + threadIdx.x; c < sd.e_cnt && pos[c] < r0 + len; c += blockDim.x)
-      wl[pos[c] - r0] = esw[sd.e_off + c];
+      wl[pos[c] - r0] = esw[sd.es_off + c];
   }
   __syncthreads();
   dd acc{0.0, 0.0};
@@ -887,6 +887,19 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
 // double-double sum (means within an ulp or two).
 constexpr int64_t kReplayMax = 256;
 
+// The running mean of centroid.mojo:34-36 never moves off a value that every
+// record of the cluster shares (x - mean is 0), so a run of equal records
+// keeps that value exactly; a double-double quotient with fractional weights
+// may land an ulp away, and the tie rule of the next sweep (tdigest.mojo:429)
+// compares means for equality.  Zeros: after the second record the running
+// mean is +0 whatever the signs.  Sorted input: x[a] == x[b-1] means all equal.
+__device__ inline bool equal_run_mean(const double* x, int64_t a, int64_t b, double* mean) {
+  double xa = x[a];
+  if (!(xa == x[b - 1])) return false;
+  *mean = (xa == 0.0 && b - a >= 2) ? 0.0 : xa;
+  return true;
+}
+
 // One thread per cluster (grid-stride over the device-side cluster total);
 // clusters spanning three or more tiles are left to k_long_sum.
 __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cbase,
@@ -911,14 +924,14 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
     int64_t b = (c + 1 < C) ? cs[sd.x_off + c + 1] : n;
     if (b / kTile - a / kTile >= 2) continue;
     const double* x = X + sd.x_off;
-    PCur pc{pose + sd.e_off, se + sd.e_off + si, sd.e_cnt, sd.sf, 0};
-    const double* esw_seg = esw + sd.e_off;
+    PCur pc{pose + sd.es_off, se + sd.es_off + si, sd.e_cnt, sd.sf, 0};
+    const double* esw_seg = esw + sd.es_off;
     double w, mean;
     if (b - a <= kReplayMax) {
       direct_sum(x, pc, esw_seg, a, b, &w, &mean);
     } else {
       dd sum = direct_sum(x, pc, esw_seg, a, b, &w);
-      mean = dd_div(sum, w);
+      if (!equal_run_mean(x, a, b, &mean)) mean = dd_div(sum, w);
     }
     cm[sd.x_off + c] = mean;
     cw[sd.x_off + c] = w;
@@ -959,8 +972,8 @@ __global__ __launch_bounds__(kBlock) void k_long_sum(const SegDesc* segs, const 
     dd ewt{0.0, 0.0};
     double ecount = 0.0;
     if (sd.e_cnt > 0) {
-      const int64_t* pos = pose + sd.e_off;
-      const double* ew = esw + sd.e_off;
+      const int64_t* pos = pose + sd.es_off;
+      const double* ew = esw + sd.es_off;
       int64_t lo = 0, hi = sd.e_cnt;
       while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
@@ -1003,7 +1016,9 @@ __global__ __launch_bounds__(kBlock) void k_long_sum(const SegDesc* segs, const 
     if (threadIdx.x == 0) {
       dd wsum = dd_add(two_prod(sd.sf, (double)(b - a) - rc[0]), dd{re[0], rf[0]});
       double w = wsum.hi + wsum.lo;
-      cm[sd.x_off + c] = dd_div(dd{rh[0], rl[0]}, w);
+      double mean;
+      if (!equal_run_mean(x, a, b, &mean)) mean = dd_div(dd{rh[0], rl[0]}, w);
+      cm[sd.x_off + c] = mean;
       cw[sd.x_off + c] = w;
     }
     __syncthreads();
@@ -1249,7 +1264,8 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     SegDesc& sd = segs[s];
     if (sd.e_cnt > kECap) throw std::runtime_error("too many explicit records in one digest");
     total_v = std::max(total_v, sd.v_off + sd.v_cnt);
-    total_e = std::max(total_e, sd.e_off + sd.e_cnt);
+    sd.es_off = total_e;
+    total_e += sd.e_cnt;
     int64_t n = sd.v_cnt + sd.e_cnt;
     sd.x_off = total_x;
     total_x += n;

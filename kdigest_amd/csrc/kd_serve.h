@@ -1,0 +1,56 @@
+// kd_serve.h — continuous-batching server over device-resident per-key digests.
+#pragma once
+
+#include <cstdint>
+#include <deque>
+#include <unordered_map>
+#include <vector>
+
+#include "kd_engine.h"
+
+namespace kd {
+
+class Server {
+ public:
+  // kind of a request: 0 ingest batch, 1 quantile batch, 2 cdf batch.
+  Server(Engine& eng, int scale, int64_t num_keys, double compression, double lambda,
+         int64_t slot_cap, int64_t mem_budget);
+  int64_t submit(int kind, int64_t key, const double* data, int64_t n);
+  // Admits and runs one continuous batch; returns the number of completed requests.
+  int64_t step(int max_rounds);
+  // Copies a completed request's result (query values); false while pending.
+  // empty is set for a query that found its key without centroids.
+  bool result(int64_t id, std::vector<double>& out, bool& empty, bool erase);
+  int64_t pending() const { return (int64_t)pending_.size(); }
+  int64_t last_admitted_bytes() const { return last_admitted_bytes_; }
+  int64_t num_keys() const { return nk_; }
+  void key_state(int64_t key, double* out6) const;
+  void key_centroids(int64_t key, std::vector<double>& m, std::vector<double>& w);
+
+ private:
+  struct Request {
+    int64_t id = 0;
+    int kind = 0;
+    int64_t key = 0;
+    std::vector<double> data, result;
+    bool done = false;
+    bool empty = false;  // query on a key without centroids (no result)
+  };
+  void refresh_state(const std::vector<int64_t>& keys);
+  void renorm_slot(int64_t key, double factor);
+
+  Engine& eng_;
+  int scale_;
+  int64_t nk_;
+  double compression_, lambda_;
+  int64_t cap_, budget_;
+  std::vector<double> stored_, logs_, clock_, vmin_, vmax_;
+  std::vector<int64_t> count_;
+  DevBuf m_, w_, S_, dcount_, dstate_, tmpk_, tmps_, qkind_, qkey_, qx_, qout_, vbuf_, meta_;
+  std::deque<int64_t> pending_;
+  std::unordered_map<int64_t, Request> reqs_;
+  int64_t next_id_ = 1;
+  int64_t last_admitted_bytes_ = 0;
+};
+
+}  // namespace kd

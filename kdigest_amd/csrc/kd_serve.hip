@@ -1,0 +1,378 @@
+// kd_serve.hip — serving loop over device-resident per-key digests.
+//
+// Requests (ingest batches and quantile/cdf query batches, each for one key)
+// are queued in arrival order.  kd_server_step admits requests in that order
+// while their data fits the memory budget, and runs them as one continuous
+// batch:
+//   * every admitted query runs in one launch against the state the key had
+//     before this step's ingests (only queries that precede the key's next
+//     ingest in arrival order are admitted);
+//   * ingests run in rounds, at most one per key per round, so each request
+//     is its own batch merge (B semantics of kdigest/tdigest.mojo:223-315) in
+//     its key's arrival order; every round is one batched compress over all
+//     its keys.
+// Per-key results therefore do not depend on what else shares the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kd_engine.h"
+#include "kd_math.h"
+#include "kd_query.h"
+#include "kd_serve.h"
+
+namespace kd {
+
+namespace {
+
+// Prefix of the interior walk of quantile/cdf (tdigest.mojo:528-536), per key slot.
+__global__ void k_slot_prefix(const int64_t* keys, int64_t nk, int64_t cap, const int64_t* count,
+                              const double* w, double* S) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nk) return;
+  int64_t k = keys[t];
+  int64_t a = k * cap, nc = count[k];
+  if (nc <= 0) return;
+  double acc = w[a] / 2.0;
+  S[a] = acc;
+  for (int64_t i = 0; i + 1 < nc; i++) {
+    acc += (w[a + i] + w[a + i + 1]) / 2.0;
+    S[a + i + 1] = acc;
+  }
+}
+
+// One query (kind 1: quantile tdigest.mojo:485-538, kind 2: cdf :540-587).
+__global__ void k_slot_query(const int* kinds, const int64_t* qkeys, const double* xs, int64_t nq,
+                             int64_t cap, const int64_t* count, const double* m, const double* w,
+                             const double* S, const double* state /* 4 per key */, double* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int64_t k = qkeys[i];
+  int64_t a = k * cap, nc = count[k];
+  QState st{state[4 * k], state[4 * k + 1], state[4 * k + 2], state[4 * k + 3]};
+  out[i] = kinds[i] == 1 ? quantile_one(m + a, w + a, S + a, nc, st, xs[i])
+                         : cdf_one(m + a, w + a, S + a, nc, st, xs[i]);
+}
+
+// Scatters packed per-key query state (stored total, min, max, eff factor).
+__global__ void k_scatter_state(const int64_t* keys, const double* packed, int64_t nk,
+                                double* state) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nk) return;
+  int64_t k = keys[t];
+  for (int j = 0; j < 4; j++) state[4 * k + j] = packed[4 * t + j];
+}
+
+// Installs each segment's compressed centroids into its key slot.
+__global__ void k_install(const int64_t* keys, const int64_t* xoff, const int64_t* counts,
+                          int64_t nseg, int64_t cap, const double* em, const double* ew,
+                          double* m, double* w, int64_t* count) {
+  int64_t sidx = blockIdx.x;
+  if (sidx >= nseg) return;
+  int64_t k = keys[sidx], c = counts[sidx];
+  for (int64_t i = threadIdx.x; i < c; i += blockDim.x) {
+    m[k * cap + i] = em[xoff[sidx] + i];
+    w[k * cap + i] = ew[xoff[sidx] + i];
+  }
+  if (threadIdx.x == 0) count[k] = c;
+}
+
+unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+Server::Server(Engine& eng, int scale, int64_t num_keys, double compression, double lambda,
+               int64_t slot_cap, int64_t mem_budget)
+    : eng_(eng), scale_(scale), nk_(num_keys), compression_(compression), lambda_(lambda),
+      cap_(slot_cap), budget_(mem_budget) {
+  stored_.assign(nk_, 0.0);
+  logs_.assign(nk_, 0.0);
+  clock_.assign(nk_, 0.0);
+  vmin_.assign(nk_, 0.0);
+  vmax_.assign(nk_, 0.0);
+  count_.assign(nk_, 0);
+  size_t slots = (size_t)nk_ * cap_;
+  m_.reserve(slots * sizeof(double));
+  w_.reserve(slots * sizeof(double));
+  S_.reserve(slots * sizeof(double));
+  dcount_.reserve(nk_ * sizeof(int64_t));
+  dstate_.reserve(4 * nk_ * sizeof(double));
+  check_hip(hipMemsetAsync(dcount_.p, 0, nk_ * sizeof(int64_t), eng_.stream()), "memset");
+  std::vector<double> st(4 * nk_);
+  for (int64_t k = 0; k < nk_; k++) {
+    st[4 * k] = 0.0;
+    st[4 * k + 1] = 0.0;
+    st[4 * k + 2] = 0.0;
+    st[4 * k + 3] = 1.0;
+  }
+  check_hip(hipMemcpyAsync(dstate_.p, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
+                           eng_.stream()),
+            "H2D state");
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+}
+
+int64_t Server::submit(int kind, int64_t key, const double* data, int64_t n) {
+  Request r;
+  r.id = next_id_++;
+  r.kind = kind;
+  r.key = key;
+  r.data.assign(data, data + n);
+  pending_.push_back(r.id);
+  reqs_.emplace(r.id, std::move(r));
+  return next_id_ - 1;
+}
+
+void Server::refresh_state(const std::vector<int64_t>& keys) {
+  if (keys.empty()) return;
+  std::vector<double> st(4 * keys.size());
+  for (size_t i = 0; i < keys.size(); i++) {
+    int64_t k = keys[i];
+    st[4 * i] = stored_[k];
+    st[4 * i + 1] = vmin_[k];
+    st[4 * i + 2] = vmax_[k];
+    st[4 * i + 3] = lambda_ > 0.0 ? kd_exp(-logs_[k]) : 1.0;
+  }
+  int64_t* dk = static_cast<int64_t*>(tmpk_.reserve(keys.size() * sizeof(int64_t)));
+  double* dp = static_cast<double*>(tmps_.reserve(st.size() * sizeof(double)));
+  check_hip(hipMemcpyAsync(dk, keys.data(), keys.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           eng_.stream()),
+            "H2D keys");
+  check_hip(hipMemcpyAsync(dp, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
+                           eng_.stream()),
+            "H2D key state");
+  k_scatter_state<<<nblocks((int64_t)keys.size()), 256, 0, eng_.stream()>>>(
+      dk, dp, (int64_t)keys.size(), static_cast<double*>(dstate_.p));
+  launched("k_scatter_state");
+  k_slot_prefix<<<nblocks((int64_t)keys.size(), 64), 64, 0, eng_.stream()>>>(
+      dk, (int64_t)keys.size(), cap_, static_cast<int64_t*>(dcount_.p),
+      static_cast<double*>(w_.p), static_cast<double*>(S_.p));
+  launched("k_slot_prefix");
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+}
+
+int64_t Server::step(int max_rounds) {
+  // ---- admission (arrival order, memory budget, per-key Q* I{0..R}) ----
+  std::vector<int64_t> queries;
+  std::vector<std::vector<int64_t>> rounds(std::max(max_rounds, 1));
+  std::unordered_map<int64_t, int> key_ingests;  // admitted ingests per key this step
+  std::unordered_map<int64_t, bool> key_blocked;  // a later request must wait for the next step
+  int64_t bytes = 0;
+  std::deque<int64_t> keep;
+  bool budget_full = false;
+  int64_t n_empty = 0;
+  for (int64_t id : pending_) {
+    Request& r = reqs_.at(id);
+    bool admit = false;
+    int64_t need = (int64_t)r.data.size() * (r.kind == 0 ? 8 : 16);
+    if (!budget_full && !key_blocked[r.key]) {
+      if (bytes + need > budget_ && bytes > 0) {
+        budget_full = true;  // FIFO: nothing behind it jumps the queue
+      } else if (r.kind == 0) {
+        int& c = key_ingests[r.key];
+        if (c < (int)rounds.size()) {
+          rounds[c].push_back(id);
+          c++;
+          admit = true;
+        } else {
+          key_blocked[r.key] = true;
+        }
+      } else if (key_ingests[r.key] == 0) {
+        if (count_[r.key] == 0) {
+          r.empty = true;  // tdigest.mojo:485-487 raises on an empty digest
+          r.done = true;
+          n_empty++;
+        } else {
+          queries.push_back(id);
+        }
+        admit = true;
+      } else {
+        key_blocked[r.key] = true;  // would need the post-ingest state
+      }
+    }
+    if (admit) bytes += need;
+    else keep.push_back(id);
+  }
+  pending_.swap(keep);
+  last_admitted_bytes_ = bytes;
+  int64_t done = n_empty;
+
+  // ---- queries: one launch ----
+  if (!queries.empty()) {
+    std::vector<int> kinds;
+    std::vector<int64_t> qk;
+    std::vector<double> xs;
+    for (int64_t id : queries) {
+      const Request& r = reqs_.at(id);
+      for (double x : r.data) {
+        kinds.push_back(r.kind);
+        qk.push_back(r.key);
+        xs.push_back(x);
+      }
+    }
+    int64_t nq = (int64_t)xs.size();
+    int* dkind = static_cast<int*>(qkind_.reserve(std::max<int64_t>(nq, 1) * sizeof(int)));
+    int64_t* dqk = static_cast<int64_t*>(qkey_.reserve(std::max<int64_t>(nq, 1) * sizeof(int64_t)));
+    double* dx = static_cast<double*>(qx_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
+    double* dout = static_cast<double*>(qout_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
+    hipStream_t st = eng_.stream();
+    check_hip(hipMemcpyAsync(dkind, kinds.data(), nq * sizeof(int), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(dqk, qk.data(), nq * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(dx, xs.data(), nq * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    int ph = eng_.phase_begin("serve_query", 24.0 * (double)nq);
+    k_slot_query<<<nblocks(nq), 256, 0, st>>>(dkind, dqk, dx, nq, cap_,
+                                              static_cast<int64_t*>(dcount_.p),
+                                              static_cast<double*>(m_.p), static_cast<double*>(w_.p),
+                                              static_cast<double*>(S_.p),
+                                              static_cast<double*>(dstate_.p), dout);
+    launched("k_slot_query");
+    eng_.phase_end(ph);
+    std::vector<double> out(nq);
+    check_hip(hipMemcpyAsync(out.data(), dout, nq * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipStreamSynchronize(st), "sync");
+    int64_t pos = 0;
+    for (int64_t id : queries) {
+      Request& r = reqs_.at(id);
+      r.result.assign(out.begin() + pos, out.begin() + pos + (int64_t)r.data.size());
+      pos += (int64_t)r.data.size();
+      r.done = true;
+      done++;
+    }
+  }
+
+  // ---- ingest rounds ----
+  for (auto& round : rounds) {
+    if (round.empty()) continue;
+    std::sort(round.begin(), round.end(),
+              [&](int64_t a, int64_t b) { return reqs_.at(a).key < reqs_.at(b).key; });
+    std::vector<SegDesc> segs;
+    std::vector<double> vals;
+    std::vector<int64_t> keys;
+    std::vector<char> had;
+    std::vector<double> sfs;
+    for (int64_t id : round) {
+      const Request& r = reqs_.at(id);
+      int64_t k = r.key, n = (int64_t)r.data.size();
+      // _gpu_batch_prepare (tdigest.mojo:236-296).
+      double nclock = clock_[k] + (double)n;
+      if (lambda_ > 0.0 && nclock > clock_[k]) {
+        logs_[k] += lambda_ * (nclock - clock_[k]);
+        if (logs_[k] > 400.0) {  // _renormalize (:359-371)
+          double f = kd_exp(-logs_[k]);
+          renorm_slot(k, f);
+          stored_[k] *= f;
+          logs_[k] = 0.0;
+        }
+      }
+      clock_[k] = nclock;
+      double sf = lambda_ > 0.0 ? kd_exp(logs_[k]) : 1.0;
+      had.push_back(count_[k] > 0);
+      stored_[k] += (double)n * sf;
+      double effn = lambda_ > 0.0 ? stored_[k] * kd_exp(-logs_[k]) : stored_[k];
+      SegDesc sd{};
+      sd.v_off = (int64_t)vals.size();
+      sd.v_cnt = n;
+      sd.e_off = k * cap_;
+      sd.e_cnt = count_[k];
+      sd.sf = sf;
+      sd.stored_n = stored_[k];
+      sd.nrm = kd_normalizer(scale_, compression_, effn);
+      sd.floor = lambda_ > 0.0 ? 1e-6 * kd_exp(logs_[k]) : 0.0;
+      segs.push_back(sd);
+      keys.push_back(k);
+      sfs.push_back(sf);
+      vals.insert(vals.end(), r.data.begin(), r.data.end());
+    }
+    hipStream_t st = eng_.stream();
+    double* dv = static_cast<double*>(vbuf_.reserve(std::max<size_t>(vals.size(), 1) * sizeof(double)));
+    check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    std::vector<SegResult> res;
+    eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
+    std::vector<int64_t> xo(segs.size()), cnt(segs.size());
+    for (size_t i = 0; i < segs.size(); i++) {
+      xo[i] = segs[i].x_off;
+      cnt[i] = res[i].count;
+      if (res[i].count > cap_)
+        throw std::runtime_error("key " + std::to_string(keys[i]) + " exceeds the slot capacity of " +
+                                 std::to_string(cap_) + " centroids");
+    }
+    int64_t ns = (int64_t)segs.size();
+    int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
+    check_hip(hipMemcpyAsync(dmeta, keys.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(dmeta + ns, xo.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(dmeta + 2 * ns, cnt.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
+                                           eng_.cent_means(), eng_.cent_weights(),
+                                           static_cast<double*>(m_.p), static_cast<double*>(w_.p),
+                                           static_cast<int64_t*>(dcount_.p));
+    launched("k_install");
+    check_hip(hipStreamSynchronize(st), "sync");
+    for (size_t i = 0; i < segs.size(); i++) {
+      int64_t k = keys[i];
+      double m = (double)segs[i].v_cnt;
+      if (stored_[k] == m * sfs[i] && !had[i]) {  // :285-295
+        vmin_[k] = res[i].vmin;
+        vmax_[k] = res[i].vmax;
+      } else {
+        if (res[i].vmin < vmin_[k]) vmin_[k] = res[i].vmin;
+        if (res[i].vmax > vmax_[k]) vmax_[k] = res[i].vmax;
+      }
+      stored_[k] = res[i].total;  // :304-315
+      count_[k] = res[i].count;
+    }
+    refresh_state(keys);
+    for (int64_t id : round) {
+      reqs_.at(id).done = true;
+      done++;
+    }
+  }
+  return done;
+}
+
+void Server::renorm_slot(int64_t key, double factor) {
+  // Centroid weights of one key scale by e^(-log_scale) (tdigest.mojo:366-367).
+  std::vector<double> w(count_[key]);
+  double* dw = static_cast<double*>(w_.p) + key * cap_;
+  check_hip(hipMemcpyAsync(w.data(), dw, w.size() * sizeof(double), hipMemcpyDeviceToHost, eng_.stream()), "D2H");
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+  for (double& x : w) x *= factor;
+  check_hip(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, eng_.stream()), "H2D");
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+}
+
+bool Server::result(int64_t id, std::vector<double>& out, bool& empty, bool erase) {
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) throw std::runtime_error("unknown request id");
+  if (!it->second.done) return false;
+  out = it->second.result;
+  empty = it->second.empty;
+  if (erase) reqs_.erase(it);
+  return true;
+}
+
+void Server::key_state(int64_t key, double* o) const {
+  o[0] = stored_[key];
+  o[1] = logs_[key];
+  o[2] = clock_[key];
+  o[3] = vmin_[key];
+  o[4] = vmax_[key];
+  o[5] = (double)count_[key];
+}
+
+void Server::key_centroids(int64_t key, std::vector<double>& m, std::vector<double>& w) {
+  m.resize(count_[key]);
+  w.resize(count_[key]);
+  check_hip(hipMemcpyAsync(m.data(), static_cast<double*>(m_.p) + key * cap_, m.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, eng_.stream()), "D2H");
+  check_hip(hipMemcpyAsync(w.data(), static_cast<double*>(w_.p) + key * cap_, w.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, eng_.stream()), "D2H");
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+}
+
+}  // namespace kd

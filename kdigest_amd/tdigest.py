@@ -537,3 +537,94 @@ class BatchedDigests:
         h = C.c_void_p()
         check(load().kd_batch_to_tdigest(self._h, int(d), C.byref(h)))
         return TDigest(self._scale, self.compression, self.decay, self.session, _handle=h.value)
+
+
+class DigestServer:
+    """Continuous-batching server over num_keys per-key digests kept in device
+    memory (GroupedTDigest semantics per key, grouped.mojo:60-190).
+
+    Requests are ingest batches (one batch merge each, tdigest.mojo:223-315)
+    and quantile / cdf batches (tdigest.mojo:485-587), each for one key.  A
+    step admits queued requests in arrival order while their data fits the
+    memory budget and runs them as one batch; every request's result is the
+    one it would have alone, in its key's arrival order."""
+
+    INGEST, QUANTILE, CDF = 0, 1, 2
+
+    def __init__(self, num_keys: int, compression: float = 100.0, decay: Decay | None = None,
+                 scale=ScaleK1, slot_capacity: int = 1024, mem_budget: int = 64 << 20,
+                 session: GpuSession | None = None):
+        self._scale = _scale(scale)
+        self.num_keys = int(num_keys)
+        self.compression = float(compression)
+        self.decay = decay if decay is not None else Decay.none()
+        self.session = session or default_session()
+        self._n = {}
+        h = C.c_void_p()
+        check(load().kd_server_create(self.session.handle, self._scale.id, self.num_keys,
+                                      self.compression, self.decay.rate_lambda, int(slot_capacity),
+                                      int(mem_budget), C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().kd_server_destroy(h)
+            self._h = None
+
+    def submit(self, kind: int, key: int, data) -> int:
+        a = _f64(data)
+        rid = C.c_int64()
+        check(load().kd_server_submit(self._h, int(kind), int(key), _ptr(a), a.size, C.byref(rid)))
+        self._n[rid.value] = a.size if kind != self.INGEST else 0
+        return rid.value
+
+    def submit_ingest(self, key: int, values) -> int:
+        return self.submit(self.INGEST, key, values)
+
+    def submit_quantile(self, key: int, qs) -> int:
+        return self.submit(self.QUANTILE, key, qs)
+
+    def submit_cdf(self, key: int, xs) -> int:
+        return self.submit(self.CDF, key, xs)
+
+    def step(self, max_rounds: int = 4) -> int:
+        done = C.c_int64()
+        check(load().kd_server_step(self._h, int(max_rounds), C.byref(done)))
+        return done.value
+
+    def result(self, rid: int):
+        """Query values (np.ndarray), an empty array for a finished ingest, or
+        None while the request is pending.  A finished request is forgotten
+        once its result has been read; a query that found its key empty raises
+        KdigestError (code KD_EEMPTY) here."""
+        n = self._n[rid]
+        out = np.empty(max(n, 1))
+        done = C.c_int()
+        rc = load().kd_server_result(self._h, int(rid), _ptr(out), n, C.byref(done))
+        if done.value:
+            del self._n[rid]
+        check(rc)
+        if not done.value:
+            return None
+        return out[:n]
+
+    @property
+    def pending(self) -> int:
+        p = C.c_int64()
+        check(load().kd_server_pending(self._h, C.byref(p)))
+        return p.value
+
+    def key_state(self, key: int) -> dict:
+        o = np.empty(6)
+        check(load().kd_server_key_state(self._h, int(key), _ptr(o)))
+        return dict(stored_total=o[0], log_scale=o[1], clock=o[2], min=o[3], max=o[4],
+                    centroids=int(o[5]))
+
+    def centroids(self, key: int):
+        L = load()
+        n = C.c_int64()
+        check(L.kd_server_key_centroids(self._h, int(key), None, None, 0, C.byref(n)))
+        m, w = np.empty(n.value), np.empty(n.value)
+        check(L.kd_server_key_centroids(self._h, int(key), _ptr(m), _ptr(w), n.value, C.byref(n)))
+        return m, w

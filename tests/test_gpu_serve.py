@@ -1,0 +1,151 @@
+"""Serving loop: every request's result is the same whether it runs alone or
+inside a continuous batch, and equals the oracle replaying the key's requests
+in arrival order (GroupedTDigest semantics, grouped.mojo:60-190: one batch
+merge per ingest, tdigest.mojo:223-315; queries tdigest.mojo:485-587)."""
+import numpy as np
+import pytest
+
+from tests.parity import QUERY_RTOL, check_centroids
+
+pytestmark = pytest.mark.gpu
+
+INGEST, QUANTILE, CDF = 0, 1, 2
+
+
+def _workload(seed, nkeys, nreq, max_ingest):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        key = int(rng.zipf(1.4) - 1) % nkeys
+        kind = int(rng.choice([INGEST, INGEST, QUANTILE, CDF]))
+        if kind == INGEST:
+            n = int(np.exp(rng.uniform(0.0, np.log(max_ingest))))
+            data = key + rng.standard_t(3, n)
+            if rng.uniform() < 0.1:
+                data = np.round(data)  # ties
+        elif kind == QUANTILE:
+            data = rng.uniform(0.0, 1.0, int(rng.integers(1, 24)))
+            data[0] = rng.choice([0.0, 1.0, 1e-6, 0.5])
+        else:
+            data = key + rng.normal(0.0, 3.0, int(rng.integers(1, 24)))
+        reqs.append((kind, key, data))
+    return reqs
+
+
+def _take(kd, srv, i):
+    try:
+        return srv.result(i)
+    except kd.KdigestError as e:
+        assert e.code == -2, e  # KD_EEMPTY: query on an empty key
+        return "empty"
+
+
+def _run_mixed(kd, reqs, nkeys, scale, delta, decay, budget, rounds):
+    srv = kd.DigestServer(nkeys, delta, decay, scale=scale, slot_capacity=2048, mem_budget=budget)
+    ids = [srv.submit(k, key, d) for k, key, d in reqs]
+    steps = 0
+    while srv.pending:
+        assert srv.step(rounds) > 0
+        steps += 1
+    out = [_take(kd, srv, i) for i in ids]
+    assert all(o is not None for o in out)
+    return srv, out, steps
+
+
+def _run_alone(kd, reqs, nkeys, scale, delta, decay):
+    srv = kd.DigestServer(nkeys, delta, decay, scale=scale, slot_capacity=2048, mem_budget=1)
+    out = []
+    for k, key, d in reqs:
+        i = srv.submit(k, key, d)
+        assert srv.step(1) == 1
+        out.append(_take(kd, srv, i))
+    return srv, out
+
+
+def _same_bits(a, b):
+    if isinstance(a, str) or isinstance(b, str):
+        return a == b
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+CASES = [
+    # scale, delta, decay lambda (5e-3 crosses the renormalize threshold)
+    (1, 100.0, 0.0),
+    (2, 200.0, 1e-5),
+    (3, 50.0, 5e-3),
+]
+
+
+@pytest.mark.parametrize("scale,delta,lam", CASES)
+def test_serve_mixed_equals_alone_and_oracle(kd, oracle, scale, delta, lam):
+    nkeys = 37
+    reqs = _workload(5 + scale, nkeys, 400, 30000)
+    decay = kd.Decay.rate(lam)
+    sc = [kd.ScaleK0, kd.ScaleK1, kd.ScaleK2, kd.ScaleK3][scale]
+    mixed, got, steps = _run_mixed(kd, reqs, nkeys, sc, delta, decay, 2 << 20, 3)
+    assert steps > 1  # the budget forces several continuous batches
+    alone, ref_alone = _run_alone(kd, reqs, nkeys, sc, delta, decay)
+    for i, (a, b) in enumerate(zip(got, ref_alone)):
+        assert _same_bits(a, b), (i, reqs[i][0], reqs[i][1], a, b)
+
+    digests = [oracle.Digest(scale, delta, lam) for _ in range(nkeys)]
+    xmax = np.zeros(nkeys)
+    for i, (kind, key, data) in enumerate(reqs):
+        d = digests[key]
+        if kind == INGEST:
+            d.add_batch_b(data)
+            xmax[key] = max(xmax[key], np.abs(data).max())
+            continue
+        if len(d) == 0:
+            assert got[i] == "empty", (i, key)
+            continue
+        fn = d.quantile if kind == QUANTILE else d.cdf
+        exp = np.array([fn(float(x)) for x in data])
+        g = got[i]
+        ok = np.abs(g - exp) <= QUERY_RTOL * np.maximum(1.0, np.abs(exp))
+        assert ok.all(), (i, kind, key, data[~ok], g[~ok], exp[~ok])
+    for key in range(nkeys):
+        st, rs = mixed.key_state(key), digests[key].state()
+        m, w = mixed.centroids(key)
+        rm, rw = digests[key].centroids()
+        check_centroids(m, w, rm, rw, xmax[key], lam == 0.0, f"key {key}")
+        am, aw = alone.centroids(key)
+        assert _same_bits(m, am) and _same_bits(w, aw), key
+        for k in ("clock", "log_scale", "min", "max"):
+            assert st[k] == rs[k], (key, k, st[k], rs[k])
+        assert abs(st["stored_total"] - rs["stored_total"]) <= 1e-12 * max(1.0, abs(rs["stored_total"]))
+
+
+def test_serve_admission_order_and_budget(kd):
+    srv = kd.DigestServer(4, 100.0, mem_budget=8 * 1000)
+    a = srv.submit_ingest(0, np.arange(1000.0))   # fills the budget by itself
+    q = srv.submit_quantile(0, [0.5])             # waits: budget full
+    b = srv.submit_ingest(1, np.arange(10.0))     # FIFO: waits behind q
+    assert srv.step() == 1
+    assert srv.result(a) is not None and srv.result(q) is None and srv.result(b) is None
+    assert srv.pending == 2
+    assert srv.step() == 2
+    assert abs(srv.result(q)[0] - 499.5) < 1.0
+    assert srv.result(b) is not None
+    # One oversized request is still admitted on its own.
+    big = srv.submit_ingest(2, np.ones(50000))
+    assert srv.step() == 1 and srv.result(big) is not None
+    assert srv.key_state(2)["stored_total"] == 50000.0
+
+
+def test_serve_rejects_bad_requests(kd):
+    srv = kd.DigestServer(3, 100.0)
+    with pytest.raises(kd.KdigestError):
+        srv.submit_ingest(3, [1.0])
+    with pytest.raises(kd.KdigestError):
+        srv.submit_ingest(0, [1.0, np.nan])
+    with pytest.raises(kd.KdigestError):
+        srv.submit_quantile(0, [1.5])
+    with pytest.raises(kd.KdigestError):
+        srv.submit(7, 0, [1.0])
+    assert srv.pending == 0
+    r = srv.submit_quantile(1, [0.5])
+    assert srv.step() == 1
+    with pytest.raises(kd.KdigestError):  # empty key
+        srv.result(r)
+    assert srv.pending == 0

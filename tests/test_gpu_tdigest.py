@@ -56,6 +56,22 @@ def test_add_batch_b_with_decay(kd, oracle, scale):
     check_values(d.quantile(qs), np.array([r.quantile(float(q)) for q in qs]), 1.0, "quantile")
 
 
+@pytest.mark.parametrize("scale", [1, 3])
+def test_decay_repeated_values_keep_tie_merges(kd, oracle, scale):
+    """Long clusters of one repeated value under fractional (decayed) weights
+    keep that value as their exact mean, so a later batch holding the value
+    ties with the stored centroid (tdigest.mojo:429) as on the CPU."""
+    rng = np.random.default_rng(300 + scale)
+    lam = 5e-3
+    d = kd.TDigest(scale, 50.0, kd.Decay.rate(lam))
+    r = oracle.Digest(scale, 50.0, lam)
+    for b in range(6):
+        x = np.round(rng.standard_t(3, 70000 + 997 * b))
+        d.add_batch(x)
+        r.add_batch(x)
+        _cmp_digest(d, r, f"ties K{scale} batch {b}", integer_weights=False)
+
+
 @pytest.mark.parametrize("scale", [0, 1, 2, 3])
 def test_add_loop_matches_oracle(kd, oracle, scale):
     # Below the crossover add_batch is the per-sample path (tdigest.mojo:199-202);
