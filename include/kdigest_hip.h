@@ -52,6 +52,15 @@ int kd_session_profile(kd_session *s, int enable);
 /* Writes {"phase": [ms_total, launches, algorithmic_bytes], ...} as JSON;
  * returns the length (writes only when cap is large enough). */
 int64_t kd_session_phase_stats(kd_session *s, char *out, int64_t cap);
+/* Compress strategy for digests receiving more than 4096 new values in one
+ * merge: enable != 0 (the default) compresses them by selection (bucket
+ * counts, rank-space sweep, exact values only at cluster boundaries) and
+ * falls back to a full sort where a boundary may tie; 0 always sorts.  Both
+ * give the same clusters.  No counterpart in the reference (its GPU compress
+ * always sorts, kdigest/gpu/add_batch.mojo:629-663). */
+int kd_session_set_select(kd_session *s, int enable);
+/* Digests compressed so far by selection (out2[0]) and by sorting (out2[1]). */
+int kd_session_path_counts(kd_session *s, int64_t *out2);
 
 /* ------------------------------------------------ operator entry points -- */
 /* gpu_compress_batch (kdigest/gpu/add_batch.mojo:629-663) as an FP64

@@ -30,6 +30,8 @@ SIGNATURES = {
     "kd_session_destroy": (None, [_vp]),
     "kd_session_sync": (_i, [_vp]),
     "kd_session_profile": (_i, [_vp, _i]),
+    "kd_session_set_select": (_i, [_vp, _i]),
+    "kd_session_path_counts": (_i, [_vp, _i64p]),
     "kd_session_phase_stats": (_i64, [_vp, C.c_char_p, _i64]),
     "kd_compress": (_i, [_vp, _i, _d, _d, _dp, _dp, _d, _i64, _i64, _dp, _dp, _i64p]),
     "kd_quantile": (_i, [_vp, _i64, _i64p, _dp, _dp, _dp, _i64p, _dp, _i64, _dp]),

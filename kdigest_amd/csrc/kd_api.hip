@@ -243,6 +243,19 @@ extern "C" int kd_session_profile(kd_session* s, int enable) {
   });
 }
 
+extern "C" int kd_session_set_select(kd_session* s, int enable) {
+  return guarded([&] {
+    s->eng.set_select(enable != 0);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_session_path_counts(kd_session* s, int64_t* out2) {
+  out2[0] = s->eng.path_counts()[0];
+  out2[1] = s->eng.path_counts()[1];
+  return KD_OK;
+}
+
 extern "C" int64_t kd_session_phase_stats(kd_session* s, char* out, int64_t cap) {
   std::string js = "{";
   try {

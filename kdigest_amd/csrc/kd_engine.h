@@ -33,6 +33,10 @@ struct SegDesc {
   int64_t t_off;          // first checkpoint tile of the segment
   int64_t es_off;         // sorted E records (engine-assigned, compact)
   double sf, stored_n, nrm, floor;
+  // Engine-assigned: 1 while the segment is on the selection path (no full
+  // sort); the device resets it to 0 when the segment falls back to sorting.
+  int32_t mode = 0;
+  int32_t sel = -1;       // index among the selection-path segments
 };
 
 struct SegResult {
@@ -41,6 +45,7 @@ struct SegResult {
   double removed;   // weight dropped by the prune (A semantics subtracts it)
   double vmin, vmax;// min / max of the new values (unused if v_cnt == 0)
   int64_t replays;  // tie decisions resolved by an exact replay of the running mean
+  int64_t path;     // 1: compressed by selection without a full sort, 0: sorted
 };
 
 // Owning device allocation, grown on demand (contents are not preserved when
@@ -107,6 +112,14 @@ class Engine {
 
   DevBuf& scratch(int i) { return extra_[i]; }
 
+  // Segments of more than kCap new values are compressed by selection
+  // (bucket counts, rank-space chain, gathered boundary buckets) unless this
+  // is switched off; both paths give the same clusters.
+  void set_select(bool on) { sel_on_ = on; }
+  bool select_enabled() const { return sel_on_; }
+  // Segments compressed so far by selection ([0]) and by sorting ([1]).
+  const int64_t* path_counts() const { return path_counts_; }
+
   // Per-phase device time from HIP events recorded on the engine stream
   // around each kernel group (enabled on demand; off by default).
   struct PhaseStat {
@@ -137,6 +150,17 @@ class Engine {
   DevBuf parts_, parts2_, spl_, nspl_, bcnt_, bcur_, tiles_, tcnt_, fb_, fbl_, route_, tsum_, long_,
       order_, clist_;
   DevBuf extra_[4];
+  // Selection path (large segments compressed without a full sort).
+  DevBuf sel_seg_, sel_st_, sel_spl_, sel_cnt_, sel_bst_, sel_info_, sel_bnd_, sel_ce_, sel_csl_,
+      sel_mk_, sel_ml_, sel_mlc_;
+  std::vector<char> h_sel_;
+  bool sel_on_ = true;
+  int64_t path_counts_[2] = {0, 0};
+  // Radix key partition: pass-1 output, tile count matrix, scan partials,
+  // segment and tile offsets.
+  DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
+  void key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n, int bits,
+                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out);
   // Host staging that must outlive the asynchronous copies reading it.
   std::vector<char> h_parts_, h_tiles_, h_finals_;
   bool prof_ = false;

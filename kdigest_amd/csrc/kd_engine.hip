@@ -122,6 +122,36 @@ __device__ inline int pow2_at_least(int64_t n) {
   return p;
 }
 
+// In-place exclusive scan of v[0..m) (m <= blockDim.x, blockDim.x a multiple
+// of 64 and at most 1024); returns the total.  All threads must call it.
+__device__ inline uint32_t block_excl_scan(uint32_t* v, int m) {
+  __shared__ uint32_t wsum[17];
+  int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  uint32_t x = tid < m ? v[tid] : 0u;
+  uint32_t inc = x;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; w++) {
+      uint32_t t = wsum[w];
+      wsum[w] = run;
+      run += t;
+    }
+    wsum[16] = run;
+  }
+  __syncthreads();
+  uint32_t total = wsum[16];
+  if (tid < m) v[tid] = inc - x + wsum[wid];
+  __syncthreads();
+  return total;
+}
+
 // ------------------------------------------------------------ LDS sort
 // Each thread sorts kPer values in registers (bitonic network), writes them
 // as a run, then log2(n / kPer) merge-path rounds double the run length.
@@ -438,7 +468,7 @@ __global__ void k_seed(const SegDesc* segs, int nseg, FBucket* fs, FBucket* fl, 
   int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nseg) return;
   SegDesc sd = segs[si];
-  if (sd.v_cnt == 0) return;
+  if (sd.v_cnt == 0 || sd.mode == 1) return;
   if (sd.v_cnt <= kCap) {
     push_final(FBucket{si, 0, 0, 0, sd.v_off, sd.v_cnt, 0}, fs, fl, rc, cap);
   } else {
@@ -519,6 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_e(const SegDesc* segs, const i
   int si = blockIdx.y;
   if (si >= neseg) return;
   SegDesc sd = segs[eseg[si]];
+  if (sd.mode == 1) return;  // compressed by selection
   int64_t tot = sd.v_cnt + sd.e_cnt;
   const double* e = esm + sd.es_off;
   const double* v = vs + sd.v_off;
@@ -626,6 +657,62 @@ __device__ int64_t index_for_weight(PCur& pc, int64_t n, double T) {
   return (int64_t)j;
 }
 
+// First index b > s whose record closes the cluster started at s in the
+// greedy sweep (tdigest.mojo:424-435): the smallest b with
+// NOT (k(P(b+1)/N) - k(P(s)/N) <= 1), or n when none does.  Found from the
+// inverse scale function, then an exact gallop and binary search on the
+// monotone predicate (P grows with the index).  Ties are not handled here.
+__device__ inline int64_t chain_next(int scale, PCur& pc, int64_t s, int64_t n, double N,
+                                     double nrm) {
+  double ks = kd_k(scale, pc.P(s) / N, nrm);
+  // pred(i): candidate i closes the cluster started at s, i.e. NOT
+  // (k(q_right) - k(q_left) <= 1) with q_right = P(i+1)/N.
+  auto pred = [&](int64_t i) -> bool {
+    double span = kd_k(scale, pc.P(i + 1) / N, nrm) - ks;
+    return !(span <= 1.0);
+  };
+  // Estimate from the inverse scale function, then an exact gallop.
+  int64_t g;
+  double qt = kinv_est(scale, ks + 1.0, nrm);
+  if (!(qt < 1.0)) g = n - 1;
+  else if (!(qt > 0.0)) g = s + 1;
+  else g = index_for_weight(pc, n, qt * N) - 1;
+  if (g < s + 1) g = s + 1;
+  if (g > n - 1) g = n - 1;
+  int64_t lo, hi;  // invariant: pred(lo) false (or lo == s), pred(hi) true (or hi == n)
+  if (pred(g)) {
+    hi = g;
+    int64_t step = 1;
+    lo = g - step;
+    while (lo >= s + 1 && pred(lo)) {
+      hi = lo;
+      step <<= 1;
+      lo = hi - step;
+    }
+    if (lo < s) lo = s;
+  } else {
+    lo = g;
+    int64_t step = 1;
+    hi = g + step;
+    while (hi <= n - 1 && !pred(hi)) {
+      lo = hi;
+      step <<= 1;
+      hi = lo + step;
+    }
+    if (hi > n) hi = n;  // hi == n is a virtual "true"
+  }
+  while (hi - lo > 1) {
+    int64_t mid = lo + ((hi - lo) >> 1);
+    if (mid >= n) {
+      hi = mid;
+      continue;
+    }
+    if (pred(mid)) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
 __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int scale,
                                                const double* X, const int64_t* pose,
                                                const double* se, const double* esw,
@@ -635,6 +722,7 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
   int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nseg) return;
   SegDesc sd = segs[si];
+  if (sd.mode == 1) return;  // compressed by selection
   int64_t n = sd.v_cnt + sd.e_cnt;
   int64_t replays = 0;
   if (n == 0) {
@@ -660,56 +748,7 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
     cs[sd.x_off + c] = s;
     c++;
     if (s == n - 1) break;
-    double ks = kd_k(scale, pc.P(s) / N, nrm);
-    // pred(i): candidate i closes the cluster started at s, i.e. NOT
-    // (k(q_right) - k(q_left) <= 1) with q_right = P(i+1)/N.
-    auto pred = [&](int64_t i) -> bool {
-      double span = kd_k(scale, pc.P(i + 1) / N, nrm) - ks;
-      return !(span <= 1.0);
-    };
-    // Estimate from the inverse scale function, then an exact gallop.
-    int64_t g;
-    double qt = kinv_est(scale, ks + 1.0, nrm);
-    if (!(qt < 1.0)) g = n - 1;
-    else if (!(qt > 0.0)) g = s + 1;
-    else g = index_for_weight(pc, n, qt * N) - 1;
-    if (g < s + 1) g = s + 1;
-    if (g > n - 1) g = n - 1;
-    int64_t lo, hi;  // invariant: pred(lo) false (or lo == s), pred(hi) true (or hi == n)
-    if (pred(g)) {
-      hi = g;
-      int64_t step = 1;
-      lo = g - step;
-      while (lo >= s + 1 && pred(lo)) {
-        hi = lo;
-        step <<= 1;
-        lo = hi - step;
-      }
-      if (lo < s) lo = s;
-    } else {
-      lo = g;
-      int64_t step = 1;
-      hi = g + step;
-      while (hi <= n - 1 && !pred(hi)) {
-        lo = hi;
-        step <<= 1;
-        hi = lo + step;
-      }
-      if (hi > n) hi = n;
-      if (hi == n && lo < n - 1) {
-        // hi == n is a virtual "true"; the binary search below keeps it exact.
-      }
-    }
-    while (hi - lo > 1) {
-      int64_t mid = lo + ((hi - lo) >> 1);
-      if (mid >= n) {
-        hi = mid;
-        continue;
-      }
-      if (pred(mid)) hi = mid;
-      else lo = mid;
-    }
-    int64_t b = hi;  // first closing candidate, or n
+    int64_t b = chain_next(scale, pc, s, n, N, nrm);  // first closing candidate, or n
     int64_t e = b;
     if (b < n) {
       // Tie rule (tdigest.mojo:429): candidate.mean == current.mean absorbs.
@@ -777,6 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int n
   int64_t t = blockIdx.x;
   int si = find_seg_by_tile(segs, nseg, t);
   SegDesc sd = segs[si];
+  if (sd.mode == 1) return;  // compressed by selection
   int64_t n = sd.v_cnt + sd.e_cnt;
   int64_t r0 = (t - sd.t_off) * kTile;
   int len = (int)std::min<int64_t>(kTile, n - r0);
@@ -917,6 +957,7 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
     }
     int si = lo;
     SegDesc sd = segs[si];
+    if (sd.mode == 1) continue;  // compressed by selection
     int64_t c = g - cbase[si];
     int64_t C = ccount[si];
     int64_t n = sd.v_cnt + sd.e_cnt;
@@ -1025,6 +1066,837 @@ __global__ __launch_bounds__(kBlock) void k_long_sum(const SegDesc* segs, const 
   }
 }
 
+// ------------------------------------------------------ selection path
+// Segments with more than kCap new values are compressed without sorting
+// them.  The new values are bucketed by a sorted splitter set (a regular
+// sample of the values plus every prior centroid mean); the bucket counts fix
+// the rank range of every bucket and the merged position of every prior
+// centroid, and the greedy sweep's cluster boundaries depend on ranks only
+// (P(j), the weight of the first j merged records, is a function of j and of
+// those positions), so the chain is walked in rank space.  Only buckets that
+// hold a record next to a boundary are gathered; a radix select inside them
+// gives the exact values and partial sums at the boundary ranks.  Every other
+// value is added to its cluster as three fixed-point slices whose sums are
+// exact, so the cluster sums do not depend on the order of the additions.
+// A boundary whose two records lie within two ulps of each other (where the
+// sweep's tie rule, tdigest.mojo:429, may apply), or any capacity limit,
+// sends the segment back to the sorting path, which handles ties exactly.
+
+constexpr int kSelSplMax = 8191;     // splitters per segment (buckets: one more)
+constexpr int kSelSample = 8192;     // regular sample sorted in LDS
+constexpr int kSelPad = kSelSample + kSelSample / 16;
+constexpr int kSelCmax = 1024;       // clusters per segment on this path
+constexpr int kSelW = 24;            // bits per slice: exact for 2^29 values per cluster
+constexpr int kSelThreads = 1024;
+constexpr int64_t kSelTileMin = 16384;
+constexpr int64_t kSelMaxV = (int64_t)1 << 29;
+constexpr int kSelLdsVals = 16384;   // marked buckets up to this size are staged in LDS
+constexpr int kSelSmallS = 1023;     // small LDS class of the count and gather passes
+constexpr int kSelSmallC = 256;
+
+struct SelSeg {
+  int32_t si;        // segment index
+  int32_t cap_s;     // splitter capacity
+  int32_t cap_c;     // cluster capacity
+  int32_t pad;
+  int64_t b_off;     // bucket arrays: cap_s + 2 entries
+  int64_t c_off;     // boundary and cluster arrays: cap_c + 1 entries
+  int64_t m_off;     // marked buckets: 2 * cap_c + 2 entries
+  int64_t tile0;     // first tile of the count and gather passes
+  int64_t tile_len;  // values per tile
+};
+
+struct SelState {
+  int32_t nspl, ncl, nmk, fail;
+  unsigned long long kmin, kmax;  // order keys of the smallest / largest new value
+};
+
+struct SelMark {
+  int64_t bucket, goff, cnt, vstart;  // gathered at x_off + goff; first new-value rank
+  unsigned int cursor, pad;
+  double sum_hi, sum_lo, vmin, vmax;
+};
+
+struct SelBound {
+  int64_t b;           // merged rank of the cluster's first record
+  int64_t vr;          // new values before it
+  int32_t v_hi, v_lo;  // record b / record b - 1 is a new value
+  int32_t dup, pad;    // the value at new-value rank vr repeats below that rank
+  double x_hi, x_lo;   // values at new-value ranks vr and vr - 1 (same bucket only)
+  double sl_hi, sl_lo; // sum of the bucket's values below rank vr
+};
+
+__device__ inline double key_value(unsigned long long k) {
+  uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return kd_bits2d(b);
+}
+
+// Bucket of v: the number of splitters <= v.
+__device__ inline int sel_bucket(const double* spl, int S, double v) {
+  int lo = 0, hi = S;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (spl[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Bucket holding new-value rank r: the last b with bst[b] <= r.
+__device__ inline int sel_bucket_of_rank(const unsigned int* bst, int S, int64_t r) {
+  int lo = 0, hi = S;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)bst[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline int sel_tile_seg(const SelSeg* sel, int nsel, int64_t t) {
+  int lo = 0, hi = nsel - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sel[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// 1.5 * 2^k: adding and subtracting it rounds to a multiple of 2^(k - 52).
+__device__ inline double sel_magic(int k) {
+  return kd_bits2d(((uint64_t)(k + 1023) << 52) | 0x0008000000000000ull);
+}
+
+// v = s1 + s2 + r with s_i multiples of 2^(e - i W), |v| < 2^e and
+// |r| <= 2^(e - 2W - 1): the dropped remainder is below 2^-48 of the
+// cluster's magnitude bound, far inside the 1e-12 parity bar for means,
+// while the slice sums stay exact (order-independent) for 2^29 values.
+__device__ inline void sel_slices(double v, int e, double& s1, double& s2) {
+  double m1 = sel_magic(e - kSelW + 52), m2 = sel_magic(e - 2 * kSelW + 52);
+  s1 = (v + m1) - m1;
+  double r1 = v - s1;
+  s2 = (r1 + m2) - m2;
+}
+
+__device__ inline dd dd_mul_d(dd a, double b) {
+  dd p = two_prod(a.hi, b);
+  return two_sum(p.hi, p.lo + a.lo * b);
+}
+
+__global__ void k_sel_init(SelState* st, int nsel) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsel) return;
+  st[j] = SelState{0, 0, 0, 0, ~0ull, 0ull};
+}
+
+// Sample size for a segment of nv new values (about two per splitter).
+__host__ __device__ inline int sel_sample_size(int64_t nv) {
+  int64_t m = nv / 16;
+  if (m < 512) m = 512;
+  if (m > kSelSample) m = kSelSample;
+  if (m > nv) m = nv;
+  return (int)m;
+}
+
+// Splitters: a regular sample of the new values, sorted in LDS, merged with
+// the sorted prior means (each kept exactly), repeats dropped.  Size classes:
+// this instance takes the segments whose sample size and splitter capacity
+// are at most CAP (and not both at most LO).
+template <int CAP, int BLOCK, int LO>
+__global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const SelSeg* sel,
+                                                      const double* vin, const double* esm,
+                                                      double* spl, SelState* st) {
+  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
+  __shared__ uint32_t part[BLOCK];
+  const int j = blockIdx.x;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int64_t nv = sd.v_cnt;
+  const int m = sel_sample_size(nv);
+  const int need = m > g.cap_s ? m : g.cap_s;
+  if (need <= LO || need > CAP) return;
+  const double* v = vin + sd.v_off;
+  for (int i = threadIdx.x; i < m; i += BLOCK)
+    sa[padi(i)] = v[((2 * (int64_t)i + 1) * nv) / (2 * (int64_t)m)];
+  __syncthreads();
+  double* s = lds_sort(sa, sb, m);
+  double* f = (s == sa) ? sb : sa;
+  const int ne = (int)sd.e_cnt;
+  int T = (int)std::min<int64_t>(std::max<int64_t>(nv / 32, 63), (int64_t)(g.cap_s - ne));
+  if (T > m - 1) T = m - 1;
+  // Sample quantiles t/(T+1), t = 1..T, into f.
+  for (int t = threadIdx.x; t < T; t += BLOCK) f[t] = s[padi((int)(((t + 1) * (int64_t)m) / (T + 1)))];
+  __syncthreads();
+  // Merge with the prior means into s: a quantile goes before equal means.
+  const double* e = esm + sd.es_off;
+  for (int t = threadIdx.x; t < T; t += BLOCK) {
+    double x = f[t];
+    int l = 0, h = ne;
+    while (l < h) {
+      int mid = (l + h) >> 1;
+      if (e[mid] < x) l = mid + 1;
+      else h = mid;
+    }
+    s[t + l] = x;
+  }
+  for (int b = threadIdx.x; b < ne; b += BLOCK) {
+    double y = e[b];
+    int l = 0, h = T;
+    while (l < h) {
+      int mid = (l + h) >> 1;
+      if (f[mid] <= y) l = mid + 1;
+      else h = mid;
+    }
+    s[b + l] = y;
+  }
+  __syncthreads();
+  // Drop repeats: keep element k when it exceeds element k - 1.
+  const int L = T + ne;
+  const int per = (L + BLOCK - 1) / BLOCK;
+  const int k0 = (int)threadIdx.x * per < L ? (int)threadIdx.x * per : L;
+  const int k1 = k0 + per < L ? k0 + per : L;
+  uint32_t kept = 0;
+  for (int k = k0; k < k1; k++) kept += (k == 0 || s[k - 1] < s[k]) ? 1u : 0u;
+  part[threadIdx.x] = kept;
+  __syncthreads();
+  uint32_t total = block_excl_scan(part, BLOCK);
+  uint32_t o = part[threadIdx.x];
+  double* out = spl + g.b_off;
+  for (int k = k0; k < k1; k++)
+    if (k == 0 || s[k - 1] < s[k]) out[o++] = s[k];
+  if (threadIdx.x == 0) st[j].nspl = (int)total;
+}
+
+// SMAX: the largest splitter count of the segments this instance serves
+// (their tiles are [t0, t1)); the LDS footprint follows it.
+template <int SMAX>
+__global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, const SelSeg* sel,
+                                                           int nsel, int64_t t0, int64_t t1,
+                                                           const double* vin, const double* spl,
+                                                           SelState* st, unsigned int* cnt) {
+  __shared__ double sp[SMAX];
+  __shared__ unsigned int lc[SMAX + 1];
+  __shared__ unsigned long long smin, smax;
+  const int64_t t = t0 + blockIdx.x;
+  if (t >= t1) return;
+  const int j = sel_tile_seg(sel, nsel, t);
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int S = st[j].nspl;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) sp[i] = spl[g.b_off + i];
+  for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
+  if (threadIdx.x == 0) {
+    smin = ~0ull;
+    smax = 0ull;
+  }
+  __syncthreads();
+  const int64_t a = (t - g.tile0) * g.tile_len;
+  const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
+  const double* v = vin + sd.v_off;
+  unsigned long long kmn = ~0ull, kmx = 0ull;
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+    double x = v[i];
+    atomicAdd(&lc[sel_bucket(sp, S, x)], 1u);
+    unsigned long long k = ord_key(x);
+    kmn = k < kmn ? k : kmn;
+    kmx = k > kmx ? k : kmx;
+  }
+  atomicMin(&smin, kmn);
+  atomicMax(&smax, kmx);
+  __syncthreads();
+  for (int i = threadIdx.x; i <= S; i += blockDim.x)
+    if (lc[i]) atomicAdd(&cnt[g.b_off + i], lc[i]);
+  if (threadIdx.x == 0) {
+    atomicMin(&st[j].kmin, smin);
+    atomicMax(&st[j].kmax, smax);
+  }
+}
+
+// Per segment: bucket ranks, prior-centroid positions, the chain in rank
+// space, the buckets to gather, each other bucket's cluster, and each
+// cluster's slice exponent.
+__global__ __launch_bounds__(kSelThreads) void k_sel_plan(
+    int scale, const SegDesc* segs, const SelSeg* sel, const double* spl, const unsigned int* cnt,
+    unsigned int* bst, int* binfo, const double* esm, const double* se, int64_t* pose,
+    SelState* st, SelBound* bnd, int* ce, double* csl, SelMark* mk, int2* mlist,
+    unsigned int* mlist_cnt) {
+  __shared__ uint32_t part[kSelThreads];
+  __shared__ int s_fail, s_C, s_base;
+  const int j = blockIdx.x;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int tid = threadIdx.x;
+  const int S = st[j].nspl;
+  const int64_t nv = sd.v_cnt;
+  const int ne = (int)sd.e_cnt;
+  const int64_t n = nv + ne;
+  const double* sp = spl + g.b_off;
+  const unsigned int* c = cnt + g.b_off;
+  unsigned int* B = bst + g.b_off;
+  int* info = binfo + g.b_off;
+  SelBound* bd = bnd + g.c_off;
+  if (tid == 0) {
+    s_fail = 0;
+    s_C = 0;
+  }
+  // 1. Exclusive scan of the S + 1 bucket counts; buckets start unmarked.
+  const int nb = S + 1;
+  const int per = (nb + kSelThreads - 1) / kSelThreads;
+  const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
+  uint32_t acc = 0;
+  for (int i = lo0; i < hi0; i++) {
+    acc += c[i];
+    info[i] = 0;
+  }
+  part[tid] = acc;
+  __syncthreads();
+  uint32_t total = block_excl_scan(part, kSelThreads);
+  uint32_t run = part[tid];
+  for (int i = lo0; i < hi0; i++) {
+    B[i] = run;
+    run += c[i];
+  }
+  if (tid == 0) {
+    B[nb] = total;
+    if ((int64_t)total != nv) s_fail = 1;
+  }
+  __syncthreads();
+  // 2. Merged positions of the prior centroids: E precede equal new values.
+  const int64_t* pos = pose + sd.es_off;
+  for (int i = tid; i < ne; i += blockDim.x) {
+    double e = esm[sd.es_off + i];
+    int l = 0, h = S;  // first splitter >= e (e itself is a splitter)
+    while (l < h) {
+      int mid = (l + h) >> 1;
+      if (sp[mid] < e) l = mid + 1;
+      else h = mid;
+    }
+    if (l >= S || !(sp[l] == e)) s_fail = 1;
+    else pose[sd.es_off + i] = i + (int64_t)B[l + 1];
+  }
+  __syncthreads();
+  // 3. Chain in rank space (one lane).
+  if (tid == 0 && !s_fail) {
+    PCur pc{pos, se + sd.es_off + g.si, sd.e_cnt, sd.sf, 0};
+    int C = 0;
+    int64_t s = 0;
+    while (s < n) {
+      if (C >= g.cap_c) {
+        s_fail = 1;
+        break;
+      }
+      bd[C].b = s;
+      C++;
+      if (s == n - 1) break;
+      s = chain_next(scale, pc, s, n, sd.stored_n, sd.nrm);
+    }
+    s_C = C;
+  }
+  __syncthreads();
+  if (s_fail) {
+    if (tid == 0) st[j].fail = 1;
+    return;
+  }
+  const int C = s_C;
+  // 4. Boundary records and the buckets to gather.
+  for (int i = tid; i < C; i += blockDim.x) {
+    int64_t b = bd[i].b;
+    int l = 0, h = ne;  // prior centroids before merged rank b
+    while (l < h) {
+      int mid = (l + h) >> 1;
+      if (pos[mid] < b) l = mid + 1;
+      else h = mid;
+    }
+    int64_t vr = b - l;
+    bd[i].vr = vr;
+    bd[i].v_hi = !(l < ne && pos[l] == b);
+    bd[i].v_lo = b > 0 && !(l > 0 && pos[l - 1] == b - 1);
+    bd[i].dup = 0;
+    if (i > 0) {
+      if (vr < nv) info[sel_bucket_of_rank(B, S, vr)] = -1;
+      if (vr >= 1) info[sel_bucket_of_rank(B, S, vr - 1)] = -1;
+    }
+  }
+  __syncthreads();
+  // 5. Gather layout of the marked buckets, in bucket order.
+  uint32_t mc = 0, mv = 0;
+  for (int i = lo0; i < hi0; i++)
+    if (info[i] < 0) {
+      mc++;
+      mv += c[i];
+    }
+  __syncthreads();
+  part[tid] = mc;
+  __syncthreads();
+  uint32_t nmk = block_excl_scan(part, kSelThreads);
+  uint32_t mbase = part[tid];
+  __syncthreads();
+  part[tid] = mv;
+  __syncthreads();
+  block_excl_scan(part, kSelThreads);
+  uint32_t gbase = part[tid];
+  if (tid == 0) {
+    st[j].nmk = (int)nmk;
+    s_base = (int)atomicAdd(mlist_cnt, nmk);
+  }
+  __syncthreads();
+  for (int i = lo0; i < hi0; i++) {
+    if (info[i] < 0) {
+      SelMark m;
+      m.bucket = i;
+      m.goff = gbase;
+      m.cnt = c[i];
+      m.vstart = B[i];
+      m.cursor = 0;
+      m.pad = 0;
+      m.sum_hi = m.sum_lo = m.vmin = m.vmax = 0.0;
+      mk[g.m_off + mbase] = m;
+      mlist[s_base + mbase] = make_int2(j, (int)mbase);
+      info[i] = -1 - (int)mbase;
+      mbase++;
+      gbase += c[i];
+    }
+  }
+  // 6. Cluster of every other bucket: boundaries i >= 1 with vr_i <= its start.
+  for (int i = lo0; i < hi0; i++) {
+    if (info[i] >= 0) {
+      int64_t r = B[i];
+      int l = 1, h = C;
+      while (l < h) {
+        int mid = (l + h) >> 1;
+        if (bd[mid].vr <= r) l = mid + 1;
+        else h = mid;
+      }
+      info[i] = l - 1;
+    }
+  }
+  // 7. Slice exponent per cluster from the value range of its buckets.
+  const double vmin = key_value(st[j].kmin), vmax = key_value(st[j].kmax);
+  for (int k = tid; k < C; k += blockDim.x) {
+    int64_t va = bd[k].vr, vb = k + 1 < C ? bd[k + 1].vr : nv;
+    int e = 0;
+    if (vb > va) {
+      int bf = sel_bucket_of_rank(B, S, va), bl = sel_bucket_of_rank(B, S, vb - 1);
+      double lo = bf == 0 ? vmin : sp[bf - 1];
+      double hi = bl == S ? vmax : sp[bl];
+      double m = fmax(fabs(lo), fabs(hi));
+      if (m == 0.0) {
+        e = -900;
+      } else {
+        int ex = (int)((kd_d2bits(m) >> 52) & 0x7ff) - 1022;  // m < 2^ex
+        if (ex < -900 || ex > 990) s_fail = 1;
+        e = ex;
+      }
+    }
+    ce[g.c_off + k] = e;
+    csl[2 * (g.c_off + k)] = 0.0;
+    csl[2 * (g.c_off + k) + 1] = 0.0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    st[j].ncl = C;
+    if (s_fail) st[j].fail = 1;
+  }
+}
+
+// Second pass over the new values: marked buckets are gathered, the rest
+// are added to their cluster's slice sums.
+template <int SMAX, int CMAX>
+__global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs, const SelSeg* sel,
+                                                            int nsel, int64_t t0, int64_t t1,
+                                                            const double* vin, const double* spl,
+                                                            const int* binfo, const int* ce,
+                                                            const SelState* st, SelMark* mk,
+                                                            double* csl, double* X) {
+  constexpr int kItems = 8;                 // values per thread per chunk
+  constexpr int kMk = 2 * CMAX + 2;          // gathered buckets of a segment at most
+  __shared__ double sp[SMAX];
+  __shared__ int inf[SMAX + 1];
+  __shared__ int ex[CMAX];
+  __shared__ double acc[2 * CMAX];
+  __shared__ unsigned int lcnt[kMk], lbase[kMk];
+  const int64_t t = t0 + blockIdx.x;
+  if (t >= t1) return;
+  const int j = sel_tile_seg(sel, nsel, t);
+  if (st[j].fail) return;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int S = st[j].nspl, C = st[j].ncl, nmk = st[j].nmk;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) sp[i] = spl[g.b_off + i];
+  for (int i = threadIdx.x; i <= S; i += blockDim.x) inf[i] = binfo[g.b_off + i];
+  for (int i = threadIdx.x; i < C; i += blockDim.x) ex[i] = ce[g.c_off + i];
+  for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) acc[i] = 0.0;
+  const int64_t a = (t - g.tile0) * g.tile_len;
+  const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
+  const double* v = vin + sd.v_off;
+  double* xg = X + sd.x_off;
+  SelMark* m = mk + g.m_off;
+  const int64_t chunk = (int64_t)kItems * kSelThreads;
+  for (int64_t c0 = a; c0 < b; c0 += chunk) {
+    for (int i = threadIdx.x; i < nmk; i += blockDim.x) lcnt[i] = 0;
+    __syncthreads();
+    double xr[kItems];
+    int fr[kItems];
+    unsigned int rr[kItems];
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
+      fr[q] = 0x7fffffff;
+      if (i < b) {
+        double x = v[i];
+        int f = inf[sel_bucket(sp, S, x)];
+        xr[q] = x;
+        fr[q] = f;
+        if (f < 0) {
+          rr[q] = atomicAdd(&lcnt[-1 - f], 1u);
+        } else {
+          double s1, s2;
+          sel_slices(x, ex[f], s1, s2);
+          unsafeAtomicAdd(&acc[2 * f], s1);
+          unsafeAtomicAdd(&acc[2 * f + 1], s2);
+        }
+      }
+    }
+    __syncthreads();
+    // One reservation per gathered bucket and chunk.
+    for (int i = threadIdx.x; i < nmk; i += blockDim.x)
+      if (lcnt[i]) lbase[i] = atomicAdd(&m[i].cursor, lcnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      int f = fr[q];
+      if (f < 0) xg[m[-1 - f].goff + lbase[-1 - f] + rr[q]] = xr[q];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < 2 * C; i += blockDim.x)
+    if (acc[i] != 0.0) unsafeAtomicAdd(&csl[2 * g.c_off + i], acc[i]);
+}
+
+// Deterministic block reduction of per-thread double-doubles (fixed tree).
+__device__ inline dd block_dd_sum(dd v, double* rh, double* rl) {
+  rh[threadIdx.x] = v.hi;
+  rl[threadIdx.x] = v.lo;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      dd s2 = dd_add(dd{rh[threadIdx.x], rl[threadIdx.x]}, dd{rh[threadIdx.x + w], rl[threadIdx.x + w]});
+      rh[threadIdx.x] = s2.hi;
+      rl[threadIdx.x] = s2.lo;
+    }
+    __syncthreads();
+  }
+  dd r{rh[0], rl[0]};
+  __syncthreads();
+  return r;
+}
+
+// First boundary i >= 1 whose new-value rank is at least r.
+__device__ inline int sel_first_bound(const SelBound* bd, int C, int64_t r) {
+  int l = 1, h = C;
+  while (l < h) {
+    int mid = (l + h) >> 1;
+    if (bd[mid].vr < r) l = mid + 1;
+    else h = mid;
+  }
+  return l;
+}
+
+// Per gathered bucket of at most CAP values (and more than LO): sorted in
+// LDS; its sum and extremes, and for every boundary rank inside it the
+// values at that rank and just below and the sum of the values below.
+template <int CAP, int BLOCK, int LO>
+__global__ __launch_bounds__(BLOCK) void k_sel_sorted(const SegDesc* segs, const SelSeg* sel,
+                                                      const SelState* st, const int2* mlist,
+                                                      const unsigned int* mlist_cnt, SelMark* mk,
+                                                      SelBound* bnd, const double* X) {
+  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
+  __shared__ double rh[BLOCK], rl[BLOCK];
+  const unsigned int total = *mlist_cnt;
+  for (unsigned int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+    const int2 e = mlist[gi];
+    const int j = e.x;
+    if (st[j].fail) continue;
+    const SelSeg g = sel[j];
+    SelMark* mp = mk + g.m_off + e.y;
+    const int64_t n = mp->cnt;
+    if (n <= LO || n > CAP) continue;
+    const SegDesc sd = segs[g.si];
+    const double* src = X + sd.x_off + mp->goff;
+    for (int i = threadIdx.x; i < (int)n; i += BLOCK) sa[padi(i)] = src[i];
+    __syncthreads();
+    const double* s = lds_sort(sa, sb, (int)n);
+    dd acc{0.0, 0.0};
+    for (int i = threadIdx.x; i < (int)n; i += BLOCK) acc = dd_add(acc, dd{s[padi(i)], 0.0});
+    dd tot = block_dd_sum(acc, rh, rl);
+    if (threadIdx.x == 0) {
+      mp->sum_hi = tot.hi;
+      mp->sum_lo = tot.lo;
+      mp->vmin = s[padi(0)];
+      mp->vmax = s[padi((int)n - 1)];
+    }
+    const int C = st[j].ncl;
+    SelBound* bd = bnd + g.c_off;
+    const int64_t vstart = mp->vstart;
+    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
+      const int rho = (int)(bd[i].vr - vstart);
+      dd sl{0.0, 0.0};
+      for (int q = threadIdx.x; q < rho; q += BLOCK) sl = dd_add(sl, dd{s[padi(q)], 0.0});
+      dd sls = block_dd_sum(sl, rh, rl);
+      if (threadIdx.x == 0) {
+        bd[i].x_hi = s[padi(rho)];
+        bd[i].x_lo = rho > 0 ? s[padi(rho - 1)] : 0.0;
+        bd[i].dup = rho > 0 && s[padi(rho - 1)] == s[padi(rho)];
+        bd[i].sl_hi = sls.hi;
+        bd[i].sl_lo = sls.lo;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Gathered buckets of more than LO values: radix select on order keys (eight
+// bits per pass) for each boundary rank inside, from LDS when the bucket fits
+// there, else from memory.
+template <int LO>
+__global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, const SelSeg* sel,
+                                                       const SelState* st, const int2* mlist,
+                                                       const unsigned int* mlist_cnt,
+                                                       SelMark* mk, SelBound* bnd,
+                                                       const double* X) {
+  __shared__ double vals[kSelLdsVals];
+  __shared__ uint32_t hist[256];
+  __shared__ double rh[kBlock], rl[kBlock];
+  __shared__ unsigned long long smin, smax, sless;
+  __shared__ unsigned int scnt;
+  __shared__ unsigned long long s_prefix, s_rank;
+  const unsigned int total = *mlist_cnt;
+  for (unsigned int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+    const int2 e = mlist[gi];
+    const int j = e.x;
+    if (st[j].fail) continue;
+    const SelSeg g = sel[j];
+    SelMark* mp = mk + g.m_off + e.y;
+    const int64_t n = mp->cnt, vstart = mp->vstart;
+    if (n <= LO) continue;
+    const SegDesc sd = segs[g.si];
+    const double* src = X + sd.x_off + mp->goff;
+    const double* a = src;
+    if (n <= kSelLdsVals) {
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) vals[i] = src[i];
+      a = vals;
+    }
+    if (threadIdx.x == 0) {
+      smin = ~0ull;
+      smax = 0ull;
+    }
+    __syncthreads();
+    dd sum{0.0, 0.0};
+    unsigned long long kmn = ~0ull, kmx = 0ull;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      double x = a[i];
+      sum = dd_add(sum, dd{x, 0.0});
+      unsigned long long k = ord_key(x);
+      kmn = k < kmn ? k : kmn;
+      kmx = k > kmx ? k : kmx;
+    }
+    atomicMin(&smin, kmn);
+    atomicMax(&smax, kmx);
+    dd tot = block_dd_sum(sum, rh, rl);
+    if (threadIdx.x == 0) {
+      mp->sum_hi = tot.hi;
+      mp->sum_lo = tot.lo;
+      mp->vmin = key_value(smin);
+      mp->vmax = key_value(smax);
+    }
+    const int C = st[j].ncl;
+    SelBound* bd = bnd + g.c_off;
+    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
+      const int64_t rho = bd[i].vr - vstart;
+      if (rho == 0) {
+        if (threadIdx.x == 0) {
+          bd[i].x_hi = key_value(smin);
+          bd[i].sl_hi = 0.0;
+          bd[i].sl_lo = 0.0;
+        }
+        continue;
+      }
+      if (threadIdx.x == 0) {
+        s_prefix = 0ull;
+        s_rank = (unsigned long long)rho;
+      }
+      __syncthreads();
+      unsigned long long pmask = 0ull;
+      for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;  // blockDim.x == 256 bins
+        __syncthreads();
+        const unsigned long long pre = s_prefix;
+        for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+          unsigned long long k = ord_key(a[q]);
+          if ((k & pmask) == pre) atomicAdd(&hist[(k >> shift) & 255], 1u);
+        }
+        __syncthreads();
+        const uint32_t h = hist[threadIdx.x];
+        const unsigned long long r = s_rank;
+        __syncthreads();
+        block_excl_scan(hist, 256);
+        // The digit whose cumulative range holds rank r.
+        const uint32_t cum = hist[threadIdx.x];
+        if ((unsigned long long)cum <= r && r < (unsigned long long)cum + h) {
+          s_rank = r - cum;
+          s_prefix = pre | ((unsigned long long)threadIdx.x << shift);
+        }
+        pmask |= 255ull << shift;
+        __syncthreads();
+      }
+      const unsigned long long K = s_prefix;
+      if (threadIdx.x == 0) {
+        sless = 0ull;
+        scnt = 0;
+      }
+      __syncthreads();
+      dd sl{0.0, 0.0};
+      unsigned long long kl = 0ull;
+      unsigned int cl = 0;
+      for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+        double x = a[q];
+        unsigned long long k = ord_key(x);
+        if (k < K) {
+          sl = dd_add(sl, dd{x, 0.0});
+          kl = k > kl ? k : kl;
+          cl++;
+        }
+      }
+      atomicMax(&sless, kl);
+      atomicAdd(&scnt, cl);
+      dd sls = block_dd_sum(sl, rh, rl);
+      if (threadIdx.x == 0) {
+        bd[i].x_hi = key_value(K);
+        bd[i].x_lo = key_value(sless);
+        bd[i].sl_hi = sls.hi;
+        bd[i].sl_lo = sls.lo;
+        bd[i].dup = (int64_t)scnt != rho;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+// Value of the new value at rank vr - 1 for boundary i (the previous
+// bucket's largest value when vr starts its bucket).
+__device__ inline double sel_value_below(const SelBound& bd, const unsigned int* B, int S,
+                                         const int* info, const SelMark* m) {
+  int bh = sel_bucket_of_rank(B, S, bd.vr);
+  int bl = sel_bucket_of_rank(B, S, bd.vr - 1);
+  if (bd.vr < (int64_t)B[S + 1] && bh == bl) return bd.x_lo;
+  if (info[bl] >= 0) return __builtin_nan("");  // not gathered: cannot happen by construction
+  return m[-1 - info[bl]].vmax;
+}
+
+// Sum of the values of a gathered bucket below its local rank r, where r is
+// 0, the bucket size, or the rank of boundary `bi`.
+__device__ inline dd sel_prefix(const SelMark& m, int64_t r, const SelBound* bd, int bi) {
+  if (r <= 0) return dd{0.0, 0.0};
+  if (r >= m.cnt) return dd{m.sum_hi, m.sum_lo};
+  return dd{bd[bi].sl_hi, bd[bi].sl_lo};
+}
+
+// Tie checks at every boundary, then each cluster's weight and mean.
+__global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const SelSeg* sel,
+                                                         const SelState* st, const SelBound* bnd,
+                                                         const SelMark* mk, const int* binfo,
+                                                         const unsigned int* bst, const double* csl,
+                                                         const double* esm, const double* esw,
+                                                         const int64_t* pose, double* cm,
+                                                         double* cw, int64_t* ccount,
+                                                         SegResult* res) {
+  __shared__ int s_fail;
+  const int j = blockIdx.x;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  if (threadIdx.x == 0) s_fail = st[j].fail;
+  __syncthreads();
+  const int S = st[j].nspl, C = st[j].ncl;
+  const int64_t nv = sd.v_cnt, ne = sd.e_cnt, n = nv + ne;
+  const SelBound* bd = bnd + g.c_off;
+  const unsigned int* B = bst + g.b_off;
+  const int* info = binfo + g.b_off;
+  const SelMark* m = mk + g.m_off;
+  const double* em = esm + sd.es_off;
+  const double* ew = esw + sd.es_off;
+  const int64_t* pos = pose + sd.es_off;
+  if (!s_fail) {
+    for (int i = 1 + threadIdx.x; i < C; i += blockDim.x) {
+      const SelBound b = bd[i];
+      const int64_t ce_ = b.b - b.vr;  // prior centroids before the boundary
+      double xh = b.v_hi ? b.x_hi : em[ce_];
+      double xl = b.v_lo ? sel_value_below(b, B, S, info, m) : em[ce_ - 1];
+      bool dup = b.v_hi && b.v_lo && b.dup;
+      if (dup || xh != xh || xl != xl || !(xl <= xh) || ord_key(xh) - ord_key(xl) <= 2ull) s_fail = 1;
+    }
+  }
+  __syncthreads();
+  if (s_fail) {
+    if (threadIdx.x == 0) segs[g.si].mode = 0;
+    return;
+  }
+  const double vmin = key_value(st[j].kmin), vmax = key_value(st[j].kmax);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const SelBound ba = bd[c];
+    const int64_t bb = c + 1 < C ? bd[c + 1].b : n;
+    const int64_t vra = ba.vr, vrb = c + 1 < C ? bd[c + 1].vr : nv;
+    const int64_t ea = ba.b - vra, eb = bb - vrb;
+    const int64_t nV = vrb - vra;
+    dd ewt{0.0, 0.0}, esum{0.0, 0.0};
+    for (int64_t i = ea; i < eb; i++) {
+      ewt = dd_add(ewt, dd{ew[i], 0.0});
+      esum = dd_add(esum, two_prod(em[i], ew[i]));
+    }
+    dd wsum = dd_add(two_prod(sd.sf, (double)nV), ewt);
+    double w = wsum.hi + wsum.lo;
+    dd sv = two_sum(csl[2 * (g.c_off + c)], csl[2 * (g.c_off + c) + 1]);
+    if (nV > 0) {
+      int bf = sel_bucket_of_rank(B, S, vra), bl = sel_bucket_of_rank(B, S, vrb - 1);
+      if (info[bf] < 0) {
+        const SelMark& ma = m[-1 - info[bf]];
+        int64_t hi_r = std::min<int64_t>(vrb, ma.vstart + ma.cnt) - ma.vstart;
+        dd hi_s = sel_prefix(ma, hi_r, bd, c + 1);
+        dd lo_s = sel_prefix(ma, vra - ma.vstart, bd, c);
+        sv = dd_add(sv, dd_add(hi_s, dd_neg(lo_s)));
+      }
+      if (bl != bf && info[bl] < 0) {
+        const SelMark& mb = m[-1 - info[bl]];
+        sv = dd_add(sv, sel_prefix(mb, vrb - mb.vstart, bd, c + 1));
+      }
+    }
+    dd tot = dd_add(dd_mul_d(sv, sd.sf), esum);
+    // First and last records of the cluster (merged order is sorted).
+    bool first_v = c == 0 ? !(ne > 0 && pos[0] == 0) : (bool)ba.v_hi;
+    double xf = first_v ? (c == 0 ? vmin : ba.x_hi) : em[ea];
+    double xl;
+    if (c + 1 < C) {
+      const SelBound b1 = bd[c + 1];
+      xl = b1.v_lo ? sel_value_below(b1, B, S, info, m) : em[eb - 1];
+    } else {
+      bool last_v = !(ne > 0 && pos[ne - 1] == n - 1);
+      xl = last_v ? vmax : em[ne - 1];
+    }
+    double mean;
+    if (xf == xl) mean = (xf == 0.0 && bb - ba.b >= 2) ? 0.0 : xf;
+    else mean = dd_div(tot, w);
+    cm[sd.x_off + c] = mean;
+    cw[sd.x_off + c] = w;
+  }
+  if (threadIdx.x == 0) {
+    ccount[g.si] = C;
+    res[g.si].vmin = vmin;
+    res[g.si].vmax = vmax;
+    res[g.si].replays = 0;
+  }
+}
+
 // --------------------------------------------------------- 6. finish
 __global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
                          double* cw, const double* X, const double* vs, SegResult* res) {
@@ -1060,7 +1932,11 @@ __global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, d
   r.total = total;
   r.removed = removed;
   r.replays = res[si].replays;
-  if (sd.v_cnt > 0) {
+  r.path = sd.mode;
+  if (sd.mode == 1) {
+    r.vmin = res[si].vmin;
+    r.vmax = res[si].vmax;
+  } else if (sd.v_cnt > 0) {
     const double* v = sd.e_cnt == 0 ? X + sd.x_off : vs + sd.v_off;
     r.vmin = v[0];
     r.vmax = v[sd.v_cnt - 1];
@@ -1260,9 +2136,35 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   int64_t total_v = 0, total_e = 0, total_x = 0, total_t = 0;
   std::vector<int> eseg;
   int64_t max_e_items = 0;
+  std::vector<SelSeg> hsel;
+  int64_t sel_b = 0, sel_c = 0, sel_m = 0, sel_t = 0;
   for (int s = 0; s < nseg; s++) {
     SegDesc& sd = segs[s];
     if (sd.e_cnt > kECap) throw std::runtime_error("too many explicit records in one digest");
+    sd.mode = 0;
+    sd.sel = -1;
+    if (sel_on_ && sd.v_cnt > kCap && sd.v_cnt <= kSelMaxV && sd.e_cnt + 64 <= kSelSplMax) {
+      SelSeg g{};
+      g.si = s;
+      g.cap_s = (int)std::min<int64_t>(kSelSplMax, std::max<int64_t>(63, sd.v_cnt / 32) + sd.e_cnt);
+      // Two consecutive clusters span more than one unit of k, and k is
+      // evaluated on q clamped to [1e-15, 1 - 1e-15].
+      double kr = kd_k(scale, 1.0 - 1e-15, sd.nrm) - kd_k(scale, 1e-15, sd.nrm);
+      int64_t cc = (kr >= 0.0 && kr < 1e6) ? (int64_t)(2.0 * kr) + 4 : kSelCmax;
+      g.cap_c = (int)std::max<int64_t>(2, std::min<int64_t>(kSelCmax, cc));
+      g.b_off = sel_b;
+      sel_b += g.cap_s + 2;
+      g.c_off = sel_c;
+      sel_c += g.cap_c + 1;
+      g.m_off = sel_m;
+      sel_m += 2 * (int64_t)g.cap_c + 2;
+      g.tile_len = std::max<int64_t>(kSelTileMin, (sd.v_cnt + 255) / 256);
+      g.tile0 = sel_t;
+      sel_t += (sd.v_cnt + g.tile_len - 1) / g.tile_len;
+      sd.mode = 1;
+      sd.sel = (int32_t)hsel.size();
+      hsel.push_back(g);
+    }
     total_v = std::max(total_v, sd.v_off + sd.v_cnt);
     sd.es_off = total_e;
     total_e += sd.e_cnt;
@@ -1293,6 +2195,23 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
   int64_t* dtotal = as<int64_t>(clist_, 2);
 
+  int64_t t_small = 0;
+  if (!hsel.empty()) {
+    // Small-footprint class first (splitters and clusters that fit the small
+    // LDS layout of the count and gather passes), then the rest; tiles are
+    // numbered in this order.
+    std::stable_partition(hsel.begin(), hsel.end(), [](const SelSeg& g) {
+      return g.cap_s <= kSelSmallS && g.cap_c <= kSelSmallC;
+    });
+    int64_t t = 0;
+    for (size_t i = 0; i < hsel.size(); i++) {
+      SelSeg& g = hsel[i];
+      g.tile0 = t;
+      t += (segs[g.si].v_cnt + g.tile_len - 1) / g.tile_len;
+      segs[g.si].sel = (int32_t)i;
+      if (g.cap_s <= kSelSmallS && g.cap_c <= kSelSmallC) t_small = t;
+    }
+  }
   check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
   if (!eseg.empty())
     check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
@@ -1305,6 +2224,96 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     }
   else
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
+  if (!hsel.empty()) {
+    const int nsel = (int)hsel.size();
+    h_sel_.resize(hsel.size() * sizeof(SelSeg));
+    memcpy(h_sel_.data(), hsel.data(), h_sel_.size());
+    SelSeg* dsel = as<SelSeg>(sel_seg_, nsel);
+    SelState* dst = as<SelState>(sel_st_, nsel);
+    double* dspl = as<double>(sel_spl_, sel_b);
+    unsigned int* dcnt = as<unsigned int>(sel_cnt_, sel_b);
+    unsigned int* dbst = as<unsigned int>(sel_bst_, sel_b);
+    int* dinfo = as<int>(sel_info_, sel_b);
+    SelBound* dbnd = as<SelBound>(sel_bnd_, sel_c);
+    int* dce = as<int>(sel_ce_, sel_c);
+    double* dcsl = as<double>(sel_csl_, 2 * sel_c);
+    SelMark* dmk = as<SelMark>(sel_mk_, sel_m);
+    int2* dml = as<int2>(sel_ml_, sel_m);
+    unsigned int* dmlc = as<unsigned int>(sel_mlc_, 1);
+    check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
+    check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
+    check_hip(hipMemsetAsync(dmlc, 0, sizeof(unsigned int), stream_), "memset sel list");
+    k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
+    launched("k_sel_init");
+    {
+      int ph_ = phase_begin("sel_sample", 0.0);
+      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      launched("k_sel_sample<1024>");
+      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      launched("k_sel_sample<4096>");
+      k_sel_sample<kSelSample, kBlock, 4096><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
+                                                                          dst);
+      launched("k_sel_sample<8192>");
+      phase_end(ph_);
+    }
+    double sel_bytes = 0.0;
+    for (const SelSeg& g : hsel) sel_bytes += 8.0 * (double)segs[g.si].v_cnt;
+    {
+      int ph_ = phase_begin("sel_count", sel_bytes);
+      if (t_small > 0) {
+        k_sel_count<kSelSmallS><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dst, dcnt);
+        launched("k_sel_count<small>");
+      }
+      if (sel_t > t_small) {
+        k_sel_count<kSelSplMax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dst, dcnt);
+        launched("k_sel_count");
+      }
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("sel_plan", 0.0);
+      k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(scale, dsegs, dsel, dspl, dcnt, dbst, dinfo, esm,
+                                                    se, pose, dst, dbnd, dce, dcsl, dmk, dml, dmlc);
+      launched("k_sel_plan");
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("sel_gather", sel_bytes);
+      if (t_small > 0) {
+        k_sel_gather<kSelSmallS, kSelSmallC><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dinfo, dce, dst, dmk, dcsl, x);
+        launched("k_sel_gather<small>");
+      }
+      if (sel_t > t_small) {
+        k_sel_gather<kSelSplMax, kSelCmax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dinfo, dce, dst, dmk, dcsl, x);
+        launched("k_sel_gather");
+      }
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("sel_select", 0.0);
+      unsigned grid = (unsigned)std::min<int64_t>(sel_m, 4096);
+      k_sel_sorted<1024, 64, 0><<<grid, 64, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
+      launched("k_sel_sorted<1024>");
+      k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk,
+                                                                     dbnd, x);
+      launched("k_sel_sorted<4096>");
+      k_sel_select<4096><<<std::min(grid, 1024u), kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc,
+                                                                         dmk, dbnd, x);
+      launched("k_sel_select");
+      phase_end(ph_);
+    }
+    {
+      int ph_ = phase_begin("sel_finalize", 0.0);
+      k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dbnd, dmk, dinfo, dbst, dcsl,
+                                                   esm, esw, pose, cm, cw, ccount, dres);
+      launched("k_sel_finalize");
+      phase_end(ph_);
+    }
+  }
   {
     auto t0 = std::chrono::steady_clock::now();
     sort_values(segs, d_v, total_v);
@@ -1372,6 +2381,8 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   if (sync_results)
     check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
   check_hip(hipStreamSynchronize(stream_), "sync results");
+  if (sync_results)
+    for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : 1]++;
 }
 
 // ------------------------------------------------------ key partition
@@ -1486,9 +2497,292 @@ __global__ void k_excl_scan_u64(const unsigned long long* cnt, int64_t n, int64_
   }
 }
 
+// ----------------------------------------------- key partition (radix)
+// Most-significant-digit-first counting sort by key: pass 1 splits the batch
+// by the high digit, pass 2 splits every high-digit segment by the low digit,
+// so each key's values end contiguous and in key order.  A pass is: digit
+// counts per tile (k_rx_hist), one exclusive scan of the (segment, digit,
+// tile) count matrix, which yields every tile's destination for every digit,
+// and a scatter staged through LDS in digit order so that consecutive lanes
+// store to consecutive addresses (k_rx_scatter).  Order within a key is not
+// kept (items are ranked with LDS atomics): all new values of a key carry
+// the same weight and are sorted next, so that order cannot matter.
+constexpr int kRxThreads = 512;
+constexpr int kRxItems = 8;
+constexpr int kRxTile = kRxThreads * kRxItems;  // 4096 items per tile
+constexpr int kRxBits = 9;
+constexpr int kRxBins = 1 << kRxBits;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+// The segment owning tile t: the last s with tile_start[s] <= t.
+__device__ inline int rx_seg_of_tile(const uint32_t* tile_start, int nseg, uint32_t t) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_rx_init(uint32_t* seg, uint32_t n) {
+  seg[0] = 0;
+  seg[1] = n;
+}
+
+// tile_start[s] = sum over earlier segments of ceil(len / kRxTile); nseg <= 1024.
+__global__ __launch_bounds__(1024) void k_rx_tiles(const uint32_t* seg_start, int nseg,
+                                                   uint32_t* tile_start) {
+  __shared__ uint32_t v[1024];
+  int s = threadIdx.x;
+  v[s] = s < nseg ? (seg_start[s + 1] - seg_start[s] + kRxTile - 1) / kRxTile : 0u;
+  __syncthreads();
+  uint32_t total = block_excl_scan(v, nseg);
+  if (s < nseg) tile_start[s] = v[s];
+  if (s == 0) tile_start[nseg] = total;
+}
+
+__global__ __launch_bounds__(kRxThreads) void k_rx_hist(const uint32_t* keys,
+                                                        const uint32_t* seg_start,
+                                                        const uint32_t* tile_start, int nseg,
+                                                        int shift, int bits, uint32_t* mat) {
+  __shared__ uint32_t lc[kRxBins];
+  const uint32_t t = blockIdx.x;
+  if (t >= tile_start[nseg]) return;
+  const int s = rx_seg_of_tile(tile_start, nseg, t);
+  const uint32_t lt = t - tile_start[s], nts = tile_start[s + 1] - tile_start[s];
+  const uint32_t a = seg_start[s] + lt * kRxTile;
+  const uint32_t b = min(seg_start[s + 1], a + kRxTile);
+  const int R = 1 << bits;
+  const uint32_t mask = (uint32_t)R - 1u;
+  for (int d = threadIdx.x; d < R; d += blockDim.x) lc[d] = 0;
+  __syncthreads();
+  for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) atomicAdd(&lc[(keys[i] >> shift) & mask], 1u);
+  __syncthreads();
+  const size_t base = (size_t)R * tile_start[s] + lt;
+  for (int d = threadIdx.x; d < R; d += blockDim.x) mat[base + (size_t)d * nts] = lc[d];
+}
+
+// Block sums of the matrix (its live length is R * tile_start[nseg]).
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* a,
+                                                              const uint32_t* tile_start, int nseg,
+                                                              int R, uint32_t* bsum) {
+  __shared__ uint32_t part[kScanThreads];
+  const size_t total = (size_t)R * tile_start[nseg];
+  const size_t c0 = (size_t)blockIdx.x * kScanChunk;
+  uint32_t acc = 0;
+  for (int j = 0; j < kScanItems; j++) {
+    size_t i = c0 + (size_t)j * kScanThreads + threadIdx.x;
+    if (i < total) acc += a[i];
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kScanThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = part[0];
+}
+
+// Exclusive scan of the nb block sums in place (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, int nb) {
+  __shared__ uint32_t v[1024];
+  const int per = (nb + 1023) / 1024;
+  const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+  uint32_t s = 0;
+  for (int i = lo; i < hi; i++) s += bsum[i];
+  v[threadIdx.x] = s;
+  __syncthreads();
+  block_excl_scan(v, 1024);
+  uint32_t run = v[threadIdx.x];
+  for (int i = lo; i < hi; i++) {
+    uint32_t x = bsum[i];
+    bsum[i] = run;
+    run += x;
+  }
+}
+
+// Exclusive scan of each chunk, offset by its block's scanned sum.
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const uint32_t* tile_start,
+                                                            int nseg, int R, const uint32_t* bsum) {
+  __shared__ uint32_t v[kScanThreads];
+  const size_t total = (size_t)R * tile_start[nseg];
+  const size_t c0 = (size_t)blockIdx.x * kScanChunk;
+  if (c0 >= total) return;
+  // Each thread owns kScanItems consecutive entries.
+  const size_t t0 = c0 + (size_t)threadIdx.x * kScanItems;
+  uint32_t x[kScanItems];
+  uint32_t s = 0;
+  for (int j = 0; j < kScanItems; j++) {
+    x[j] = (t0 + j < total) ? a[t0 + j] : 0u;
+    s += x[j];
+  }
+  v[threadIdx.x] = s;
+  __syncthreads();
+  block_excl_scan(v, kScanThreads);
+  uint32_t run = v[threadIdx.x] + bsum[blockIdx.x];
+  for (int j = 0; j < kScanItems; j++) {
+    if (t0 + j < total) a[t0 + j] = run;
+    run += x[j];
+  }
+}
+
+template <bool kWriteKeys>
+__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys, const double* vals,
+                                                           const uint32_t* seg_start,
+                                                           const uint32_t* tile_start, int nseg,
+                                                           int shift, int bits, const uint32_t* dst,
+                                                           uint32_t* keys_out, double* vals_out) {
+  __shared__ uint32_t lc[kRxBins];
+  __shared__ uint32_t gd[kRxBins];
+  __shared__ double sv[kRxTile];
+  __shared__ uint32_t sk[kRxTile];
+  const uint32_t t = blockIdx.x;
+  if (t >= tile_start[nseg]) return;
+  const int s = rx_seg_of_tile(tile_start, nseg, t);
+  const uint32_t lt = t - tile_start[s], nts = tile_start[s + 1] - tile_start[s];
+  const uint32_t a = seg_start[s] + lt * kRxTile;
+  const uint32_t b = min(seg_start[s + 1], a + kRxTile);
+  const int R = 1 << bits;
+  const uint32_t mask = (uint32_t)R - 1u;
+  for (int d = threadIdx.x; d < R; d += blockDim.x) lc[d] = 0;
+  __syncthreads();
+  uint32_t kr[kRxItems], rk[kRxItems];
+  double vr[kRxItems];
+#pragma unroll
+  for (int j = 0; j < kRxItems; j++) {
+    uint32_t i = a + (uint32_t)j * kRxThreads + threadIdx.x;
+    if (i < b) {
+      kr[j] = keys[i];
+      vr[j] = vals[i];
+      rk[j] = atomicAdd(&lc[(kr[j] >> shift) & mask], 1u);
+    }
+  }
+  __syncthreads();
+  block_excl_scan(lc, R);
+  const size_t base = (size_t)R * tile_start[s] + lt;
+  for (int d = threadIdx.x; d < R; d += blockDim.x) gd[d] = dst[base + (size_t)d * nts];
+#pragma unroll
+  for (int j = 0; j < kRxItems; j++) {
+    uint32_t i = a + (uint32_t)j * kRxThreads + threadIdx.x;
+    if (i < b) {
+      uint32_t pos = lc[(kr[j] >> shift) & mask] + rk[j];
+      sv[pos] = vr[j];
+      sk[pos] = kr[j];
+    }
+  }
+  __syncthreads();
+  const int len = (int)(b - a);
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    uint32_t k = sk[i];
+    uint32_t d = (k >> shift) & mask;
+    uint32_t o = gd[d] + ((uint32_t)i - lc[d]);
+    vals_out[o] = sv[i];
+    if (kWriteKeys) keys_out[o] = k;
+  }
+}
+
+// Segments of the next pass: (segment s, digit d) starts where the scanned
+// matrix puts the first tile of s for digit d.
+__global__ void k_rx_segs(const uint32_t* seg_start, const uint32_t* tile_start, int nseg, int R,
+                          const uint32_t* dst, uint32_t* seg_out) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t m = (int64_t)nseg * R;
+  if (idx > m) return;
+  if (idx == m) {
+    seg_out[m] = seg_start[nseg];
+    return;
+  }
+  int s = (int)(idx / R), d = (int)(idx % R);
+  uint32_t nts = tile_start[s + 1] - tile_start[s];
+  seg_out[idx] = nts ? dst[(size_t)R * tile_start[s] + (size_t)d * nts] : seg_start[s];
+}
+
+__global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, int64_t* cnt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > num_keys) return;
+  off[k] = seg[k];
+  if (k < num_keys) cnt[k] = (int64_t)seg[k + 1] - (int64_t)seg[k];
+}
+
+void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n,
+                                 int bits, int64_t num_keys, int64_t* d_counts,
+                                 int64_t* d_offsets, double* d_out) {
+  const int hb = bits > kRxBits ? bits - kRxBits : 0;  // high digit bits (pass 1)
+  const int lb = bits > kRxBits ? kRxBits : bits;      // low digit bits (last pass)
+  const int64_t max_tiles1 = (n + kRxTile - 1) / kRxTile;
+  const int64_t max_tiles2 = max_tiles1 + (1 << hb);
+  const int64_t mat_max = (int64_t)kRxBins * max_tiles2;
+  const int64_t nb_max = (mat_max + kScanChunk - 1) / kScanChunk;
+  uint32_t* mat = as<uint32_t>(rx_mat_, mat_max);
+  uint32_t* bsum = as<uint32_t>(rx_bsum_, nb_max);
+  uint32_t* segA = as<uint32_t>(rx_seg1_, ((int64_t)1 << bits) + 2);
+  uint32_t* segB = as<uint32_t>(rx_seg2_, ((int64_t)1 << bits) + 2);
+  uint32_t* tiles = as<uint32_t>(rx_tile_, ((int64_t)1 << hb) + 2);
+  k_rx_init<<<1, 1, 0, stream_>>>(segA, (uint32_t)n);
+  launched("k_rx_init");
+
+  auto pass = [&](const uint32_t* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
+                  int dbits, int64_t tiles_max, uint32_t* kout, double* vout, uint32_t* seg_out) {
+    const int R = 1 << dbits;
+    k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, tiles);
+    launched("k_rx_tiles");
+    {
+      int ph_ = phase_begin("key_hist", 4.0 * (double)n);
+      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(kin, seg, tiles, nseg, shift,
+                                                                 dbits, mat);
+      launched("k_rx_hist");
+      phase_end(ph_);
+    }
+    const int64_t nb = ((int64_t)R * tiles_max + kScanChunk - 1) / kScanChunk;
+    k_scan_reduce<<<(unsigned)nb, kScanThreads, 0, stream_>>>(mat, tiles, nseg, R, bsum);
+    launched("k_scan_reduce");
+    k_scan_top<<<1, 1024, 0, stream_>>>(bsum, (int)nb);
+    launched("k_scan_top");
+    k_scan_down<<<(unsigned)nb, kScanThreads, 0, stream_>>>(mat, tiles, nseg, R, bsum);
+    launched("k_scan_down");
+    {
+      int ph_ = phase_begin("key_scatter", (kout ? 24.0 : 20.0) * (double)n);
+      if (kout)
+        k_rx_scatter<true><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
+            kin, vin, seg, tiles, nseg, shift, dbits, mat, kout, vout);
+      else
+        k_rx_scatter<false><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
+            kin, vin, seg, tiles, nseg, shift, dbits, mat, nullptr, vout);
+      launched("k_rx_scatter");
+      phase_end(ph_);
+    }
+    const int64_t m = (int64_t)nseg * R + 1;
+    k_rx_segs<<<(unsigned)((m + 255) / 256), 256, 0, stream_>>>(seg, tiles, nseg, R, mat, seg_out);
+    launched("k_rx_segs");
+  };
+
+  if (hb > 0) {
+    uint32_t* k1 = as<uint32_t>(rx_keys_, n);
+    double* v1 = as<double>(rx_vals_, n);
+    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB);
+    pass(k1, v1, segB, 1 << hb, 0, lb, max_tiles2, nullptr, d_out, segA);
+  } else {
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB);
+    std::swap(segA, segB);
+  }
+  k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(segA, num_keys, d_offsets,
+                                                                          d_counts);
+  launched("k_rx_final");
+}
+
 void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
                            int64_t num_keys, int64_t* d_counts, int64_t* d_offsets,
                            double* d_out) {
+  int bits = 0;
+  while (bits < 31 && ((int64_t)1 << bits) < num_keys) bits++;
+  if (n > 0 && n <= ((int64_t)1 << 31) && bits <= 2 * kRxBits) {
+    key_partition_radix(d_keys, d_vals, n, std::max(bits, 1), num_keys, d_counts, d_offsets, d_out);
+    check_hip(hipGetLastError(), "key partition launch");
+    return;
+  }
   auto* cnt = reinterpret_cast<unsigned long long*>(d_counts);
   auto* cur = as<unsigned long long>(extra_[0], num_keys + 1);
   check_hip(hipMemsetAsync(cnt, 0, num_keys * sizeof(unsigned long long), stream_), "memset counts");

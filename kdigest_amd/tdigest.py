@@ -136,6 +136,16 @@ class GpuSession:
         """Per-phase device timing with HIP events (clears the totals)."""
         check(load().kd_session_profile(self._h, int(enable)))
 
+    def set_select(self, enable: bool = True) -> None:
+        """Compress digests receiving more than 4096 new values by selection
+        (default) or always by a full sort; both give the same clusters."""
+        check(load().kd_session_set_select(self.handle, 1 if enable else 0))
+
+    def path_counts(self) -> dict:
+        o = np.zeros(2, np.int64)
+        check(load().kd_session_path_counts(self.handle, _ptr(o, _i64p)))
+        return {"selection": int(o[0]), "sorted": int(o[1])}
+
     def phase_stats(self) -> dict:
         """{phase: (total_ms, launches, algorithmic_bytes)} since profile()."""
         import json

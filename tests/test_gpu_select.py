@@ -1,0 +1,133 @@
+"""Compress by selection (no full sort of large batches) against the sorting
+path and the oracle: the same clusters (counts and integer weights bit-exact,
+means within 1e-12), and a fall back to sorting where a boundary may tie
+(tdigest.mojo:429)."""
+import numpy as np
+import pytest
+
+from tests.parity import check_centroids, check_values
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sessions(kd):
+    sel = kd.GpuSession()
+    srt = kd.GpuSession()
+    srt.set_select(False)
+    return sel, srt
+
+
+def _data(kind, rng, n):
+    if kind == "normal":
+        return rng.standard_normal(n)
+    if kind == "lognormal":
+        return rng.lognormal(0.0, 1.5, n)
+    if kind == "pareto":
+        return (1.0 - rng.uniform(size=n)) ** (-1.0 / 1.5)
+    if kind == "sorted":
+        return np.sort(rng.standard_t(3, n))
+    if kind == "reversed":
+        return np.sort(rng.standard_normal(n))[::-1].copy()
+    if kind == "straddle":
+        # Three groups far apart (one around zero); spacing well above an ulp.
+        return rng.uniform(-1e-3, 1e-3, n) + rng.choice([-1e3, 0.0, 1e3], n)
+    if kind == "tiny":
+        return rng.standard_normal(n) * 1e-200
+    if kind == "periodic":
+        base = rng.standard_normal(1024)
+        return np.tile(base, n // 1024 + 1)[:n] + np.arange(n) * 1e-9
+    raise ValueError(kind)
+
+
+KINDS = ["normal", "lognormal", "pareto", "sorted", "reversed", "straddle", "tiny", "periodic"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("scale", [0, 1, 2, 3])
+def test_select_matches_sort_and_oracle(kd, oracle, sessions, kind, scale):
+    s_sel, s_srt = sessions
+    rng = np.random.default_rng(hash((kind, scale)) % (2**32))
+    before = s_sel.path_counts()
+    a = kd.TDigest(scale, 150.0, session=s_sel)
+    b = kd.TDigest(scale, 150.0, session=s_srt)
+    r = oracle.Digest(scale, 150.0)
+    for n in (70001, 5000, 250000):  # the second and third batches carry prior centroids
+        x = _data(kind, rng, n)
+        a.add_batch(x)
+        b.add_batch(x)
+        r.add_batch(x)
+        scale_abs = float(np.abs(x).max())
+        ma, wa = a.centroids()
+        mb, wb = b.centroids()
+        rm, rw = r.centroids()
+        check_centroids(ma, wa, mb, wb, scale_abs, True, f"{kind} K{scale} n={n} sel vs sort")
+        check_centroids(ma, wa, rm, rw, scale_abs, True, f"{kind} K{scale} n={n} sel vs oracle")
+        sa, rs = a.state(), r.state()
+        for k in ("min", "max", "stored_total"):
+            assert sa[k] == rs[k], (kind, scale, n, k, sa[k], rs[k])
+    after = s_sel.path_counts()
+    assert after["selection"] > before["selection"], (before, after)
+    qs = np.concatenate([rng.uniform(0, 1, 200), [0.0, 1e-4, 0.5, 1 - 1e-4, 1.0]])
+    check_values(a.quantile(qs), np.array([r.quantile(float(q)) for q in qs]),
+                 max(1.0, float(np.abs(x).max())), f"{kind} K{scale} quantile")
+
+
+@pytest.mark.parametrize("lam", [1e-5, 2e-3])
+def test_select_with_decay(kd, oracle, sessions, lam):
+    s_sel, s_srt = sessions
+    rng = np.random.default_rng(77)
+    a = kd.TDigest(2, 200.0, kd.Decay.rate(lam), session=s_sel)
+    r = oracle.Digest(2, 200.0, lam)
+    for i in range(6):
+        x = rng.lognormal(0.0, 1.5, 65536 + 123 * i)
+        a.add_batch(x)
+        r.add_batch(x)
+        ma, wa = a.centroids()
+        rm, rw = r.centroids()
+        check_centroids(ma, wa, rm, rw, float(x.max()), False, f"decay {lam} batch {i}")
+        sa, rs = a.state(), r.state()
+        for k in ("log_scale", "clock", "min", "max"):
+            assert sa[k] == rs[k], (k, sa[k], rs[k])
+
+
+def test_select_ties_fall_back_to_sort(kd, oracle, sessions):
+    s_sel, _ = sessions
+    rng = np.random.default_rng(5)
+    before = s_sel.path_counts()
+    a = kd.TDigest(1, 100.0, session=s_sel)
+    r = oracle.Digest(1, 100.0)
+    x = np.round(rng.standard_normal(100000) * 3.0)  # few distinct values: boundaries tie
+    a.add_batch(x)
+    r.add_batch(x)
+    ma, wa = a.centroids()
+    rm, rw = r.centroids()
+    check_centroids(ma, wa, rm, rw, 20.0, True, "ties")
+    after = s_sel.path_counts()
+    assert after["sorted"] > before["sorted"], (before, after)
+
+
+def test_select_grouped_zipf(kd, oracle, sessions):
+    s_sel, s_srt = sessions
+    rng = np.random.default_rng(12)
+    nk = 400
+    p = 1.0 / np.arange(1, nk + 1) ** 1.1
+    p /= p.sum()
+    keys = rng.choice(nk, size=600000, p=p).astype(np.uint32)
+    mu = rng.uniform(-100, 100, nk)
+    vals = np.where(rng.uniform(size=keys.size) < 0.9, rng.standard_normal(keys.size) + mu[keys],
+                    mu[keys] + (1 - rng.uniform(size=keys.size)) ** (-1 / 1.5))
+    ga = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s_sel)
+    gb = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s_srt)
+    for half in (slice(0, 300000), slice(300000, None)):
+        ga.add_batch(keys[half], vals[half])
+        gb.add_batch(keys[half], vals[half])
+    refs = [oracle.Digest(3, 100.0) for _ in range(nk)]
+    oracle.grouped_add_batch(refs, keys[:300000], vals[:300000], threads=8)
+    oracle.grouped_add_batch(refs, keys[300000:], vals[300000:], threads=8)
+    for k in range(nk):
+        ma, wa = ga.centroids(k)
+        mb, wb = gb.centroids(k)
+        rm, rw = refs[k].centroids()
+        check_centroids(ma, wa, mb, wb, 300.0, True, f"key {k} sel vs sort")
+        check_centroids(ma, wa, rm, rw, 300.0, True, f"key {k} sel vs oracle")

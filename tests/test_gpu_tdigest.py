@@ -202,3 +202,28 @@ def test_grouped_many_keys_zipf(kd, oracle):
                     for k, q in zip(qk, qs)])
     ok = ~np.isnan(ref)
     check_values(got[ok], ref[ok], 100.0, "grouped quantile")
+
+
+@pytest.mark.parametrize("nk", [1, 2, 511, 512, 513, 4097, 1 << 18, (1 << 18) + 1])
+def test_grouped_key_partition_sizes(kd, oracle, nk):
+    """Key grouping at the digit-width edges of the partition (one radix pass
+    up to 512 keys, two passes up to 2^18, hashed counting beyond)."""
+    rng = np.random.default_rng(nk)
+    n = 300_001
+    keys = (rng.zipf(1.3, n) - 1) % nk
+    keys[:3] = [0, nk - 1, nk // 2]
+    keys = keys.astype(np.uint32)
+    vals = rng.standard_normal(n) + keys * 1e-3
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    g.add_batch(keys, vals)
+    cnt = np.bincount(keys, minlength=nk)
+    chosen = sorted(set([0, nk - 1, nk // 2] + list(range(0, nk, max(1, nk // 40)))))
+    for k in chosen:
+        assert g.total_weight(k) == float(cnt[k]), (k, g.total_weight(k), cnt[k])
+        if cnt[k] == 0:
+            continue
+        r = oracle.Digest(3, 100.0)
+        r.add_batch_b(vals[keys == k])
+        m, w = g.centroids(k)
+        rm, rw = r.centroids()
+        check_centroids(m, w, rm, rw, 300.0, True, f"nk {nk} key {k}")
