@@ -7,8 +7,11 @@ ScaleK3, compression 100, no decay.  One step = one GroupedTDigest.add_batch
 of the whole device-resident batch into fresh digests (key grouping, per-key
 sort, greedy-sweep compress, finish).  Secondary lines in the same JSON:
 single-digest ingest (2^26 lognormal(0, 1.5) values in 64 batches of 2^20,
-ScaleK2, compression 200, half-life 1e5) and the 1024 tail quantiles per key
-of the grouped digests.
+ScaleK2, compression 200, half-life 1e5); the 1024 tail quantiles per key of
+the grouped digests; the same grouped batch through plain one-digest-per-launch
+compresses (timed on a stratified key sample, extrapolated); 8192 decayed
+digests fed by add_at and reduced by a merge_into tree; and an open-loop
+serving run over one million device-resident keys.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--keys K]
 """
@@ -70,6 +73,183 @@ def cpu_baseline_grouped(keys_h: np.ndarray, vals_h: np.ndarray, num_keys: int):
     return keys_h.size / dt, dt
 
 
+def plain_per_key(kd, session, keys, vals, nk, ms_grouped, dev):
+    """The same grouped batch as one TDigest batch merge per key (one compress
+    launch sequence per key).  All of the 64 largest keys are timed; the rest
+    on every STRIDE-th key by size rank, scaled by STRIDE."""
+    import torch
+
+    order = torch.argsort(keys.to(torch.int64), stable=False)
+    sv = vals[order]
+    cnt = torch.bincount(keys.to(torch.int64), minlength=nk).cpu().numpy()
+    off = np.zeros(nk + 1, np.int64)
+    off[1:] = np.cumsum(cnt)
+    by_size = np.argsort(-cnt, kind="stable")
+    stride = 97
+    picked = [(int(k), 1.0) for k in by_size[:64]]
+    picked += [(int(k), float(stride)) for k in by_size[64::stride]]
+
+    def one(k):
+        d = kd.TDigest(kd.ScaleK3, 100.0, session=session)
+        if cnt[k] > 0:
+            d.add_batch_device(sv[off[k]:off[k + 1]])
+        return d
+
+    for k, _ in picked[:8]:
+        one(k)
+    session.synchronize()
+    est = 0.0
+    timed = 0.0
+    for k, w in picked:
+        t0 = time.perf_counter()
+        one(k)
+        session.synchronize()
+        dt = time.perf_counter() - t0
+        est += w * dt
+        timed += dt
+    del sv, order
+    return {"workload": "one TDigest.add_batch_device (batch merge) per key of the same batch",
+            "keys_timed": len(picked), "timed_s": timed,
+            "sample": f"the 64 largest keys, then every {stride}th key by size, weighted by {stride}",
+            "est_ms": est * 1e3, "speedup_grouped": est * 1e3 / ms_grouped}
+
+
+def batched_decay(kd, session, dev, nd=8192, seed=3):
+    """8192 digests, geometric sizes (mean 32768, at most 2^22), Student-t(3)
+    values, U(0.1, 10) weights, add_at timestamps with Exp(1e3/s) gaps,
+    lambda 1e-3/s, ScaleK2, compression 300; flush; merge_into tree to one."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    p = 1.0 / 32768.0
+    u = torch.rand(nd, generator=g, dtype=torch.float64, device=dev)
+    sizes = torch.ceil(torch.log1p(-u) / math.log1p(-p)).clamp_(1, 1 << 22).to(torch.int64)
+    off = torch.zeros(nd + 1, dtype=torch.int64, device=dev)
+    off[1:] = torch.cumsum(sizes, 0)
+    n = int(off[-1].item())
+    z = torch.randn(n, generator=g, dtype=torch.float64, device=dev)
+    chi = torch.zeros(n, dtype=torch.float64, device=dev)
+    for _ in range(3):
+        chi += torch.randn(n, generator=g, dtype=torch.float64, device=dev) ** 2
+    vals = z / torch.sqrt(chi / 3.0)
+    del z, chi
+    w = 0.1 + 9.9 * torch.rand(n, generator=g, dtype=torch.float64, device=dev)
+    gaps = -torch.log1p(-torch.rand(n, generator=g, dtype=torch.float64, device=dev)) * 1e-3
+    cs = torch.cumsum(gaps, 0)
+    del gaps
+    seg = torch.repeat_interleave(torch.arange(nd, device=dev), sizes)
+    base = torch.cat([torch.zeros(1, dtype=torch.float64, device=dev), cs])[off[:-1]]
+    times = cs - base[seg]
+    del cs, seg, base
+    torch.cuda.synchronize()
+
+    def run():
+        b = kd.BatchedDigests(nd, 300.0, kd.Decay.rate(1e-3), scale=kd.ScaleK2, session=session)
+        b.add_at_device(off, vals, times, w)
+        b.flush()
+        b.merge_tree()
+        session.synchronize()
+        return b
+
+    run()
+    t0 = time.perf_counter()
+    b = run()
+    dt = time.perf_counter() - t0
+    st = b.state(0)
+    return {"workload": f"{nd} digests, geometric sizes mean 32768 (max 2^22), {n} samples, "
+                        "Student-t(3), weights U(0.1,10), add_at with Exp(1e3/s) gaps, "
+                        "lambda 1e-3/s, ScaleK2, compression 300, flush + merge_into tree",
+            "samples": n, "max_digest": int(sizes.max().item()), "ms": dt * 1e3,
+            "samples_per_s": n / dt, "bytes_per_sample": 56.0,
+            "roofline_frac": 56.0 * n / dt / 1e9 / HBM_PEAK_GBS,
+            "root_centroids": len(b.centroids(0)[0]), "root_total": st["stored_total"]}
+
+
+def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5):
+    """Open-loop serving: Poisson arrivals at `rate` req/s for `seconds`,
+    keys Zipf(1.05) over 1M device-resident digests (ScaleK1, compression
+    100, half-life 1e6 samples); half the requests ingest log-uniform 1..65536
+    N(0,1) values shifted per key, a quarter ask quantiles and a quarter cdf
+    values (log-uniform 1..4096 per request).  Requests are submitted when
+    their arrival time has passed; latency = completion (server clock, when
+    the step that finished the request ends) minus scheduled arrival."""
+    rng = np.random.default_rng(seed)
+    ranks = np.arange(1, nk + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** -1.05)
+    cdf /= cdf[-1]
+    perm = rng.permutation(nk)
+    nreq = int(rate * seconds)
+    arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    keys = perm[np.minimum(np.searchsorted(cdf, rng.uniform(size=nreq)), nk - 1)].astype(np.int64)
+    kinds = rng.choice([0, 0, 1, 2], nreq).astype(np.int32)
+    sizes = np.where(kinds == 0, np.exp(rng.uniform(0, np.log(65536.0), nreq)),
+                     np.exp(rng.uniform(0, np.log(4096.0), nreq))).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    off = np.zeros(nreq + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    shift = (keys % 1000) / 10.0
+    data = np.empty(off[-1])
+    pool = rng.standard_normal(1 << 22)
+    for r in range(nreq):
+        a, b, m = off[r], off[r + 1], sizes[r]
+        if kinds[r] == 0:
+            p0 = rng.integers(0, (1 << 22) - m)
+            data[a:b] = pool[p0:p0 + m] + shift[r]
+        elif kinds[r] == 1:
+            data[a:b] = rng.uniform(0.0, 1.0, m)
+        else:
+            data[a:b] = shift[r] + pool[:m]
+    srv = kd.DigestServer(nk, 100.0, kd.Decay.half_life(1e6), scale=kd.ScaleK1, slot_capacity=128,
+                          mem_budget=64 << 20, session=session)
+    # Warm the buffers (not timed).
+    for k in range(64):
+        srv.submit_ingest(k, pool[:5000] + 1.0)
+    while srv.pending:
+        srv.step(4)
+    srv.take_completed()
+    lat = np.full(nreq, np.nan)
+    first_id = None
+    i = 0
+    ndone = 0
+    t0 = time.perf_counter()
+    c0 = srv.clock()
+    while ndone < nreq:
+        now = time.perf_counter() - t0
+        j = int(np.searchsorted(arr, now, side="right"))
+        if j > i:
+            ids = srv.submit_many(kinds[i:j], keys[i:j], off[i:j + 1] - off[i], data[off[i]:off[j]])
+            if first_id is None:
+                first_id = int(ids[0])
+            i = j
+        if srv.pending == 0:
+            if i < nreq:
+                time.sleep(max(0.0, min(arr[i] - now, 1e-3)))
+            continue
+        srv.step(4)
+        cids, ct = srv.take_completed()
+        for rid, tc in zip(cids.tolist(), ct.tolist()):
+            r = rid - first_id
+            lat[r] = (tc - c0) - arr[r]
+            srv.discard(rid)
+        ndone += len(cids)
+    wall = time.perf_counter() - t0
+    lat *= 1e3
+    st = srv.stats()
+    ing = kinds == 0
+    return {"workload": f"{nk} keys Zipf(1.05), ScaleK1, compression 100, half-life 1e6; "
+                        f"Poisson {rate:.0f} req/s for {seconds:.1f} s, 1/2 ingest (log-uniform "
+                        "1..65536), 1/4 quantile, 1/4 cdf (log-uniform 1..4096)",
+            "requests": nreq, "wall_s": wall, "requests_per_s": nreq / wall,
+            "offered_per_s": rate,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "ingest_p99_ms": float(np.percentile(lat[ing], 99)),
+            "query_p99_ms": float(np.percentile(lat[~ing], 99)),
+            "ingest_samples_per_s": float(sizes[ing].sum()) / wall,
+            "queries_per_s": float(sizes[~ing].sum()) / wall,
+            "admission": st, "mem_budget_bytes": 64 << 20}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,9 +259,13 @@ def main():
     ap.add_argument("--keys", type=int, default=100000)
     ap.add_argument("--single-n", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-queries", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -116,13 +300,16 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phases = session.phase_stats()
+    paths = session.path_counts()
     session.profile(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n * args.steps / (t1 - t0)
     print(f"[bench] grouped ingest {value / 1e9:.3f} G samples/s, {ms_per_step:.2f} ms/step",
           file=sys.stderr, flush=True)
-    dom = max(phases.items(), key=lambda kv: kv[1][0])
-    dom_name, (dom_ms, dom_launches, dom_bytes) = dom
+    # Dominant device kernel group with algorithmic bytes (HIP events on the
+    # session stream around each launch).
+    dev_phases = {k: v for k, v in phases.items() if not k.startswith("host:") and v[2] > 0}
+    dom_name, (dom_ms, dom_launches, dom_bytes) = max(dev_phases.items(), key=lambda kv: kv[1][0])
     dom_avg_ms = dom_ms / max(dom_launches, 1)
     dom_gbs = (dom_bytes / max(dom_launches, 1)) / (dom_avg_ms * 1e-3) / 1e9 if dom_avg_ms > 0 else 0.0
     pipe_gbs = GROUPED_BYTES_PER_SAMPLE * n / (ms_per_step * 1e-3) / 1e9
@@ -157,7 +344,29 @@ def main():
                    "ms": dt * 1e3, "request_p50_us": float(np.percentile(lat, 50)),
                    "request_p99_us": float(np.percentile(lat, 99)),
                    "request": "16 quantiles of one key, host arrays, synchronous"}
+        del qk, qv, out
         print(f"[bench] queries {queries['queries_per_s'] / 1e9:.3f} G/s", file=sys.stderr, flush=True)
+    del digs
+
+    # ------------------------------------------------ plain per-key launches
+    plain = None
+    if not args.no_plain:
+        plain = plain_per_key(kd, session, keys, vals, nk, ms_per_step, dev)
+        print(f"[bench] plain per-key est {plain['est_ms']:.1f} ms "
+              f"({plain['speedup_grouped']:.1f}x)", file=sys.stderr, flush=True)
+
+    # ------------------------------------------------ CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        m = min(args.cpu_sample, n)
+        kh = keys[:m].cpu().numpy().astype(np.uint32)
+        vh = vals[:m].cpu().numpy()
+        rate, dt = cpu_baseline_grouped(kh, vh, nk)
+        cpu = {"value": rate, "unit": "samples/s", "cores": 1, "kind": "port",
+               "sample": f"first {m} pairs of the same grouped workload, oracle per-key batch merge "
+                         f"(C restatement of the reference MergingDigest), {dt:.1f} s"}
+    del keys, vals
+    torch.cuda.empty_cache()
 
     # ------------------------------------------------ single digest
     single = None
@@ -183,18 +392,23 @@ def main():
                               f"of {batch}, ScaleK2, compression 200, half-life 1e5",
                   "samples_per_s": args.single_n / dt, "ms": dt * 1e3, "centroids": len(d),
                   "roofline_frac": 24.0 * args.single_n / dt / 1e9 / HBM_PEAK_GBS}
+        del xs
         print(f"[bench] single {single['samples_per_s'] / 1e9:.3f} G samples/s", file=sys.stderr, flush=True)
 
-    # ------------------------------------------------ CPU baseline
-    cpu = None
-    if not args.no_cpu:
-        m = min(args.cpu_sample, n)
-        kh = keys[:m].cpu().numpy().astype(np.uint32)
-        vh = vals[:m].cpu().numpy()
-        rate, dt = cpu_baseline_grouped(kh, vh, nk)
-        cpu = {"value": rate, "unit": "samples/s", "cores": 1, "kind": "port",
-               "sample": f"first {m} pairs of the same grouped workload, oracle per-key batch merge "
-                         f"(C restatement of the reference MergingDigest), {dt:.1f} s"}
+    # ------------------------------------------------ batched, decayed, merge tree
+    batched = None
+    if not args.no_batched:
+        batched = batched_decay(kd, session, dev)
+        torch.cuda.empty_cache()
+        print(f"[bench] batched {batched['samples_per_s'] / 1e9:.3f} G samples/s", file=sys.stderr,
+              flush=True)
+
+    # ------------------------------------------------ serving
+    serve = None
+    if not args.no_serve:
+        serve = serving(kd, session, seconds=args.serve_seconds)
+        print(f"[bench] serving p50 {serve['p50_ms']:.2f} ms p99 {serve['p99_ms']:.2f} ms",
+              file=sys.stderr, flush=True)
 
     line = {
         "metric": "ingest samples/s (% HBM peak) single + 100k-key grouped; p99 query latency",
@@ -208,7 +422,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (torch Philox, seeded), device-resident before timing",
+        "data": "synthetic (torch Philox / numpy, seeded), device-resident before timing",
         "config": {"workload": f"grouped ingest: {nk} uint32 keys Zipf(1.1), {n} (key, f64) pairs, "
                                "0.9*N(mu_k,1)+0.1*(mu_k+Pareto(1.5)), ScaleK3, compression 100",
                    "keys": nk, "pairs": n},
@@ -218,9 +432,13 @@ def main():
         "pipeline_roofline": {"bytes_per_sample": GROUPED_BYTES_PER_SAMPLE, "achieved": pipe_gbs,
                               "frac": pipe_gbs / HBM_PEAK_GBS},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "compress_paths": paths,
         "cpu_baseline": cpu,
+        "plain_per_key": plain,
         "single_digest": single,
         "grouped_queries": queries,
+        "batched_decay": batched,
+        "serving": serve,
     }
     print(json.dumps(line))
 

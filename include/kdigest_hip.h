@@ -59,8 +59,14 @@ int64_t kd_session_phase_stats(kd_session *s, char *out, int64_t cap);
  * give the same clusters.  No counterpart in the reference (its GPU compress
  * always sorts, kdigest/gpu/add_batch.mojo:629-663). */
 int kd_session_set_select(kd_session *s, int enable);
-/* Digests compressed so far by selection (out2[0]) and by sorting (out2[1]). */
-int kd_session_path_counts(kd_session *s, int64_t *out2);
+/* Digests of at most 4096 merged records are compressed in one block each
+ * (one launch for all of them) unless enable == 0, which routes them through
+ * the multi-kernel sort path.  Same clusters either way.  No counterpart in
+ * the reference. */
+int kd_session_set_fused(kd_session *s, int enable);
+/* Digests compressed so far by selection (out3[0]), by the multi-kernel sort
+ * path (out3[1]) and in one block each (out3[2]). */
+int kd_session_path_counts(kd_session *s, int64_t *out3);
 
 /* ------------------------------------------------ operator entry points -- */
 /* gpu_compress_batch (kdigest/gpu/add_batch.mojo:629-663) as an FP64
@@ -204,6 +210,15 @@ void kd_server_destroy(kd_server *v);
 /* Copies the request data; validates it before queueing. */
 int kd_server_submit(kd_server *v, int kind, int64_t key, const double *data, int64_t n,
                      int64_t *id);
+/* n requests at once: request r has kind kinds[r], key keys[r] and data
+ * data[off[r] .. off[r+1]); all are validated before any is queued. */
+int kd_server_submit_many(kd_server *v, int64_t n, const int *kinds, const int64_t *keys,
+                          const int64_t *off, const double *data, int64_t *ids);
+/* Ids of the requests completed since the last call (at most cap) and the
+ * server clock (kd_server_clock) when each completed. */
+int kd_server_take_completed(kd_server *v, int64_t *ids, double *t, int64_t cap, int64_t *count);
+/* Seconds since the server was created (steady clock). */
+int kd_server_clock(kd_server *v, double *now);
 /* Admits and runs one continuous batch: all admitted queries in one launch,
  * then up to max_rounds rounds of ingests (one per key per round). */
 int kd_server_step(kd_server *v, int max_rounds, int64_t *completed);
@@ -212,6 +227,9 @@ int kd_server_step(kd_server *v, int max_rounds, int64_t *completed);
  * key empty completes with KD_EEMPTY (tdigest.mojo:485-487). */
 int kd_server_result(kd_server *v, int64_t id, double *out, int64_t cap, int *done);
 int kd_server_pending(kd_server *v, int64_t *out);
+/* {steps, requests admitted, bytes admitted, largest step in bytes, ingest
+ * rounds, requests still queued}. */
+int kd_server_stats(kd_server *v, int64_t *out6);
 /* {stored_total, log_scale, clock, min, max, centroids} of one key. */
 int kd_server_key_state(kd_server *v, int64_t key, double *out6);
 int kd_server_key_centroids(kd_server *v, int64_t key, double *means, double *weights,

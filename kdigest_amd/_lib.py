@@ -31,6 +31,7 @@ SIGNATURES = {
     "kd_session_sync": (_i, [_vp]),
     "kd_session_profile": (_i, [_vp, _i]),
     "kd_session_set_select": (_i, [_vp, _i]),
+    "kd_session_set_fused": (_i, [_vp, _i]),
     "kd_session_path_counts": (_i, [_vp, _i64p]),
     "kd_session_phase_stats": (_i64, [_vp, C.c_char_p, _i64]),
     "kd_compress": (_i, [_vp, _i, _d, _d, _dp, _dp, _d, _i64, _i64, _dp, _dp, _i64p]),
@@ -81,9 +82,13 @@ SIGNATURES = {
     "kd_server_create": (_i, [_vp, _i, _i64, _d, _d, _i64, _i64, C.POINTER(_vp)]),
     "kd_server_destroy": (None, [_vp]),
     "kd_server_submit": (_i, [_vp, _i, _i64, _dp, _i64, _i64p]),
+    "kd_server_submit_many": (_i, [_vp, _i64, C.POINTER(_i), _i64p, _i64p, _dp, _i64p]),
+    "kd_server_take_completed": (_i, [_vp, _i64p, _dp, _i64, _i64p]),
+    "kd_server_clock": (_i, [_vp, _dp]),
     "kd_server_step": (_i, [_vp, _i, _i64p]),
     "kd_server_result": (_i, [_vp, _i64, _dp, _i64, C.POINTER(_i)]),
     "kd_server_pending": (_i, [_vp, _i64p]),
+    "kd_server_stats": (_i, [_vp, _i64p]),
     "kd_server_key_state": (_i, [_vp, _i64, _dp]),
     "kd_server_key_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
 }

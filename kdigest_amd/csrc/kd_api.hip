@@ -250,9 +250,15 @@ extern "C" int kd_session_set_select(kd_session* s, int enable) {
   });
 }
 
-extern "C" int kd_session_path_counts(kd_session* s, int64_t* out2) {
-  out2[0] = s->eng.path_counts()[0];
-  out2[1] = s->eng.path_counts()[1];
+extern "C" int kd_session_set_fused(kd_session* s, int enable) {
+  return guarded([&] {
+    s->eng.set_fused(enable != 0);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_session_path_counts(kd_session* s, int64_t* out3) {
+  for (int i = 0; i < 3; i++) out3[i] = s->eng.path_counts()[i];
   return KD_OK;
 }
 
@@ -1422,6 +1428,41 @@ extern "C" int kd_server_submit(kd_server* v, int kind, int64_t key, const doubl
   });
 }
 
+extern "C" int kd_server_submit_many(kd_server* v, int64_t n, const int* kinds, const int64_t* keys,
+                                     const int64_t* off, const double* data, int64_t* ids) {
+  return guarded([&] {
+    for (int64_t r = 0; r < n; r++) {  // validate every request before queueing any
+      int kind = kinds[r];
+      int64_t key = keys[r];
+      if (kind < 0 || kind > 2) throw KdError(KD_EINVAL, "kind must be 0 (ingest), 1 (quantile) or 2 (cdf)");
+      if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+      if (off[r + 1] <= off[r]) throw KdError(KD_EINVAL, "empty request");
+      for (int64_t i = off[r]; i < off[r + 1]; i++) {
+        if (kind == 0 && !is_finite_val(data[i]))
+          throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+        if (kind == 1 && !(data[i] >= 0.0 && data[i] <= 1.0))
+          throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+      }
+    }
+    for (int64_t r = 0; r < n; r++)
+      ids[r] = v->srv->submit(kinds[r], keys[r], data + off[r], off[r + 1] - off[r]);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_take_completed(kd_server* v, int64_t* ids, double* t, int64_t cap,
+                                        int64_t* count) {
+  return guarded([&] {
+    *count = v->srv->take_completed(ids, t, cap);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_clock(kd_server* v, double* now) {
+  *now = v->srv->now();
+  return KD_OK;
+}
+
 extern "C" int kd_server_step(kd_server* v, int max_rounds, int64_t* completed) {
   return guarded([&] {
     *completed = v->srv->step(max_rounds);
@@ -1449,6 +1490,11 @@ extern "C" int kd_server_result(kd_server* v, int64_t id, double* out, int64_t c
 
 extern "C" int kd_server_pending(kd_server* v, int64_t* out) {
   *out = v->srv->pending();
+  return KD_OK;
+}
+
+extern "C" int kd_server_stats(kd_server* v, int64_t* out6) {
+  v->srv->stats(out6);
   return KD_OK;
 }
 

@@ -33,8 +33,9 @@ struct SegDesc {
   int64_t t_off;          // first checkpoint tile of the segment
   int64_t es_off;         // sorted E records (engine-assigned, compact)
   double sf, stored_n, nrm, floor;
-  // Engine-assigned: 1 while the segment is on the selection path (no full
-  // sort); the device resets it to 0 when the segment falls back to sorting.
+  // Engine-assigned path: 0 multi-kernel sort, 1 selection (no full sort;
+  // the device resets it to 0 when the segment falls back to sorting),
+  // 2 one block per digest (at most kCap merged records).
   int32_t mode = 0;
   int32_t sel = -1;       // index among the selection-path segments
 };
@@ -45,7 +46,7 @@ struct SegResult {
   double removed;   // weight dropped by the prune (A semantics subtracts it)
   double vmin, vmax;// min / max of the new values (unused if v_cnt == 0)
   int64_t replays;  // tie decisions resolved by an exact replay of the running mean
-  int64_t path;     // 1: compressed by selection without a full sort, 0: sorted
+  int64_t path;     // 0: multi-kernel sort path, 1: selection, 2: one block per digest
 };
 
 // Owning device allocation, grown on demand (contents are not preserved when
@@ -117,8 +118,12 @@ class Engine {
   // is switched off; both paths give the same clusters.
   void set_select(bool on) { sel_on_ = on; }
   bool select_enabled() const { return sel_on_; }
-  // Segments compressed so far by selection ([0]) and by sorting ([1]).
+  // Segments compressed so far by selection ([0]), by the multi-kernel
+  // sorting path ([1]) and in one block each ([2]).
   const int64_t* path_counts() const { return path_counts_; }
+  // Segments of at most kCap records are compressed in one block each unless
+  // this is switched off (same results either way).
+  void set_fused(bool on) { fused_on_ = on; }
 
   // Per-phase device time from HIP events recorded on the engine stream
   // around each kernel group (enabled on demand; off by default).
@@ -155,7 +160,10 @@ class Engine {
       sel_mk_, sel_ml_, sel_mlc_;
   std::vector<char> h_sel_;
   bool sel_on_ = true;
-  int64_t path_counts_[2] = {0, 0};
+  int64_t path_counts_[3] = {0, 0, 0};
+  bool fused_on_ = true;
+  std::vector<int> small1_, small4_, h_small_;
+  DevBuf small_list_;
   // Radix key partition: pass-1 output, tile count matrix, scan partials,
   // segment and tile offsets.
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;

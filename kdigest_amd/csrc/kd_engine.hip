@@ -468,7 +468,7 @@ __global__ void k_seed(const SegDesc* segs, int nseg, FBucket* fs, FBucket* fl, 
   int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nseg) return;
   SegDesc sd = segs[si];
-  if (sd.v_cnt == 0 || sd.mode == 1) return;
+  if (sd.v_cnt == 0 || sd.mode != 0) return;
   if (sd.v_cnt <= kCap) {
     push_final(FBucket{si, 0, 0, 0, sd.v_off, sd.v_cnt, 0}, fs, fl, rc, cap);
   } else {
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_e(const SegDesc* segs, const i
   int si = blockIdx.y;
   if (si >= neseg) return;
   SegDesc sd = segs[eseg[si]];
-  if (sd.mode == 1) return;  // compressed by selection
+  if (sd.mode != 0) return;  // compressed by selection or in one block
   int64_t tot = sd.v_cnt + sd.e_cnt;
   const double* e = esm + sd.es_off;
   const double* v = vs + sd.v_off;
@@ -713,42 +713,21 @@ __device__ inline int64_t chain_next(int scale, PCur& pc, int64_t s, int64_t n, 
   return hi;
 }
 
-__global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int scale,
-                                               const double* X, const int64_t* pose,
-                                               const double* se, const double* esw,
-                                               int64_t* cs, int64_t* ccount,
-                                               SegResult* res, int64_t* long_list,
-                                               unsigned long long* long_cnt) {
-  int si = blockIdx.x * blockDim.x + threadIdx.x;
-  if (si >= nseg) return;
-  SegDesc sd = segs[si];
-  if (sd.mode == 1) return;  // compressed by selection
-  int64_t n = sd.v_cnt + sd.e_cnt;
-  int64_t replays = 0;
-  if (n == 0) {
-    ccount[si] = 0;
-    res[si].replays = 0;
-    return;
-  }
-  const double* x = X + sd.x_off;
-  const double N = sd.stored_n;
-  const double nrm = sd.nrm;
-  PCur pc{pose + sd.es_off, se + sd.es_off + si, sd.e_cnt, sd.sf, 0};
+// The greedy sweep of tdigest.mojo:414-435 over one segment's merged records
+// x[0..n) (sorted; any address space).  Writes each cluster's first record
+// index to cs[c], calls done(a, b, c) for every cluster [a, b) but the last
+// single record, and returns the cluster count.
+// next(s) gives the first candidate closing the cluster started at s (or n).
+template <class Next, class Done>
+__device__ inline int64_t sweep_segment(const double* x, int64_t n, PCur pc,
+                                        const double* esw_seg, int64_t* cs, int64_t& replays,
+                                        Next next, Done done) {
   int64_t s = 0, c = 0;
-  // Clusters spanning three or more checkpoint tiles are summed by
-  // k_long_sum (one block each); the rest by k_cluster_mean.
-  auto emit_long = [&](int64_t a, int64_t b) {
-    if (b / kTile - a / kTile >= 2) {
-      unsigned long long slot = atomicAdd(long_cnt, 1ull);
-      long_list[2 * slot] = si;
-      long_list[2 * slot + 1] = c - 1;
-    }
-  };
   while (s < n) {
-    cs[sd.x_off + c] = s;
+    cs[c] = s;
     c++;
     if (s == n - 1) break;
-    int64_t b = chain_next(scale, pc, s, n, N, nrm);  // first closing candidate, or n
+    int64_t b = next(s);  // first closing candidate, or n
     int64_t e = b;
     if (b < n) {
       // Tie rule (tdigest.mojo:429): candidate.mean == current.mean absorbs.
@@ -767,14 +746,13 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
         } else {
           // Replay the incremental mean of centroid.mojo:34-36 exactly.
           replays++;
-          PCur wc{pc.pos, pc.se, pc.ne, pc.sf, 0};
           int64_t ce = 0;
-          while (ce < wc.ne && wc.pos[ce] < s) ce++;
+          while (ce < pc.ne && pc.pos[ce] < s) ce++;
           double mean = 0.0, wt = 0.0;
           for (int64_t j = s; j < b; j++) {
             double w = pc.sf;
-            if (ce < wc.ne && wc.pos[ce] == j) {
-              w = esw[sd.es_off + ce];
+            if (ce < pc.ne && pc.pos[ce] == j) {
+              w = esw_seg[ce];
               ce++;
             }
             if (j == s) {
@@ -790,9 +768,41 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
         }
       }
     }
-    emit_long(s, e);
+    done(s, e, c - 1);
     s = e;
   }
+  return c;
+}
+
+__global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int scale,
+                                               const double* X, const int64_t* pose,
+                                               const double* se, const double* esw,
+                                               int64_t* cs, int64_t* ccount,
+                                               SegResult* res, int64_t* long_list,
+                                               unsigned long long* long_cnt) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  if (sd.mode != 0) return;  // compressed by selection or in one block
+  int64_t n = sd.v_cnt + sd.e_cnt;
+  if (n == 0) {
+    ccount[si] = 0;
+    res[si].replays = 0;
+    return;
+  }
+  PCur pc{pose + sd.es_off, se + sd.es_off + si, sd.e_cnt, sd.sf, 0};
+  int64_t replays = 0;
+  // Clusters spanning three or more checkpoint tiles are summed by
+  // k_long_sum (one block each); the rest by k_cluster_mean.
+  int64_t c = sweep_segment(X + sd.x_off, n, pc, esw + sd.es_off, cs + sd.x_off, replays,
+                            [&](int64_t s) { return chain_next(scale, pc, s, n, sd.stored_n, sd.nrm); },
+                            [&](int64_t a, int64_t b, int64_t ci) {
+                              if (b / kTile - a / kTile >= 2) {
+                                unsigned long long slot = atomicAdd(long_cnt, 1ull);
+                                long_list[2 * slot] = si;
+                                long_list[2 * slot + 1] = ci;
+                              }
+                            });
   ccount[si] = c;
   res[si].replays = replays;
 }
@@ -816,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const SegDesc* segs, int n
   int64_t t = blockIdx.x;
   int si = find_seg_by_tile(segs, nseg, t);
   SegDesc sd = segs[si];
-  if (sd.mode == 1) return;  // compressed by selection
+  if (sd.mode != 0) return;  // compressed by selection or in one block
   int64_t n = sd.v_cnt + sd.e_cnt;
   int64_t r0 = (t - sd.t_off) * kTile;
   int len = (int)std::min<int64_t>(kTile, n - r0);
@@ -957,7 +967,7 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
     }
     int si = lo;
     SegDesc sd = segs[si];
-    if (sd.mode == 1) continue;  // compressed by selection
+    if (sd.mode != 0) continue;  // compressed by selection or in one block
     int64_t c = g - cbase[si];
     int64_t C = ccount[si];
     int64_t n = sd.v_cnt + sd.e_cnt;
@@ -976,6 +986,124 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
     }
     cm[sd.x_off + c] = mean;
     cw[sd.x_off + c] = w;
+  }
+}
+
+// Segments of at most CAP merged records compressed in one block each: the
+// new values sorted in LDS, merged with the sorted prior records (prior
+// first on equal means), the sweep walked by one lane, then the cluster
+// means by all lanes with the arithmetic of k_cluster_mean.  Replaces the
+// sort / merge / chain / sums launches for small digests.
+template <int CAP, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDesc* segs,
+                                                          const int* list, int nlist,
+                                                          const double* vin, const double* esm,
+                                                          const double* esw, const double* se,
+                                                          int64_t* pose, double* cm, double* cw,
+                                                          int64_t* ccount, SegResult* res) {
+  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
+  __shared__ uint16_t nxt[CAP];
+  __shared__ int64_t s_C;
+  for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int si = list[li];
+    const SegDesc sd = segs[si];
+    const int nv = (int)sd.v_cnt, ne = (int)sd.e_cnt, n = nv + ne;
+    const double* v = vin + sd.v_off;
+    for (int i = threadIdx.x; i < nv; i += BLOCK) sa[padi(i)] = v[i];
+    __syncthreads();
+    double* s = nv > 0 ? lds_sort(sa, sb, nv) : sa;
+    double* x = (s == sa) ? sb : sa;
+    const double* e = esm + sd.es_off;
+    for (int i = threadIdx.x; i < nv; i += BLOCK) {
+      double xv = s[padi(i)];
+      int lo = 0, hi = ne;  // prior records <= xv
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (!(xv < e[mid])) lo = mid + 1;
+        else hi = mid;
+      }
+      x[i + lo] = xv;
+    }
+    for (int i = threadIdx.x; i < ne; i += BLOCK) {
+      double xe = e[i];
+      int lo = 0, hi = nv;  // new values < xe
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (s[padi(mid)] < xe) lo = mid + 1;
+        else hi = mid;
+      }
+      x[i + lo] = xe;
+      pose[sd.es_off + i] = i + lo;
+    }
+    __syncthreads();
+    const int64_t* pos = pose + sd.es_off;
+    const double* sw = se + sd.es_off + si;
+    const double* ew = esw + sd.es_off;
+    if (threadIdx.x == 0) {
+      SegResult& r = res[si];
+      r.vmin = nv > 0 ? s[padi(0)] : 0.0;
+      r.vmax = nv > 0 ? s[padi(nv - 1)] : 0.0;
+    }
+    __syncthreads();
+    // kv[j] = k(P(j) / N): the left and right ends of every sweep test
+    // (P(j) = sf * (new values before j) + prior weight before j).
+    double* kv = s;
+    for (int j = threadIdx.x; j <= n; j += BLOCK) {
+      int lo = 0, hi = ne;  // prior records before merged index j
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (pos[mid] < j) lo = mid + 1;
+        else hi = mid;
+      }
+      double P = sd.sf * (double)(j - lo) + sw[lo];
+      kv[j] = kd_k(scale, P / sd.stored_n, sd.nrm);
+    }
+    __syncthreads();
+    // nxt[s]: first i > s with NOT (kv[i + 1] - kv[s] <= 1), or n (the
+    // test is monotone in i, so a binary search finds the sweep's candidate).
+    for (int st = threadIdx.x; st < n; st += BLOCK) {
+      const double ks = kv[st];
+      int lo = st + 1, hi = n;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (!(kv[mid + 1] - ks <= 1.0)) hi = mid;
+        else lo = mid + 1;
+      }
+      nxt[st] = (uint16_t)(lo == n ? 0xffff : lo);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t* cs = reinterpret_cast<int64_t*>(s);  // kv is no longer needed
+      int64_t replays = 0;
+      int64_t C = n > 0 ? sweep_segment(x, n, PCur{pos, sw, ne, sd.sf, 0}, ew, cs, replays,
+                                        [&](int64_t st) -> int64_t {
+                                          uint16_t v = nxt[st];
+                                          return v == 0xffff ? (int64_t)n : (int64_t)v;
+                                        },
+                                        [](int64_t, int64_t, int64_t) {})
+                        : 0;
+      res[si].replays = replays;
+      ccount[si] = C;
+      s_C = C;
+    }
+    __syncthreads();
+    const int64_t C = s_C;
+    const int64_t* cs = reinterpret_cast<const int64_t*>(s);
+    for (int64_t c = threadIdx.x; c < C; c += BLOCK) {
+      int64_t a = cs[c];
+      int64_t b = (c + 1 < C) ? cs[c + 1] : n;
+      PCur pc{pos, sw, ne, sd.sf, 0};
+      double w, mean;
+      if (b - a <= kReplayMax) {
+        direct_sum(x, pc, ew, a, b, &w, &mean);
+      } else {
+        dd sum = direct_sum(x, pc, ew, a, b, &w);
+        if (!equal_run_mean(x, a, b, &mean)) mean = dd_div(sum, w);
+      }
+      cm[sd.x_off + c] = mean;
+      cw[sd.x_off + c] = w;
+    }
+    __syncthreads();
   }
 }
 
@@ -1177,6 +1305,13 @@ __device__ inline void sel_slices(double v, int e, double& s1, double& s2) {
   s1 = (v + m1) - m1;
   double r1 = v - s1;
   s2 = (r1 + m2) - m2;
+}
+
+// Smallest e with max(|lo|, |hi|) < 2^e (-900 for zeros).
+__device__ inline int sel_exponent(double lo, double hi) {
+  double m = fmax(fabs(lo), fabs(hi));
+  if (m == 0.0) return -900;
+  return (int)((kd_d2bits(m) >> 52) & 0x7ff) - 1022;
 }
 
 __device__ inline dd dd_mul_d(dd a, double b) {
@@ -1692,18 +1827,27 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
       smax = 0ull;
     }
     __syncthreads();
-    dd sum{0.0, 0.0};
+    // The gathered order is not deterministic: every sum below is of exact
+    // fixed-point slices (with the bucket's magnitude bound), so it does not
+    // depend on the order of the additions.
     unsigned long long kmn = ~0ull, kmx = 0ull;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      double x = a[i];
-      sum = dd_add(sum, dd{x, 0.0});
-      unsigned long long k = ord_key(x);
+      unsigned long long k = ord_key(a[i]);
       kmn = k < kmn ? k : kmn;
       kmx = k > kmx ? k : kmx;
     }
     atomicMin(&smin, kmn);
     atomicMax(&smax, kmx);
-    dd tot = block_dd_sum(sum, rh, rl);
+    __syncthreads();
+    const int ex = sel_exponent(key_value(smin), key_value(smax));
+    double t1 = 0.0, t2 = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      double s1, s2;
+      sel_slices(a[i], ex, s1, s2);
+      t1 += s1;
+      t2 += s2;
+    }
+    dd tot = block_dd_sum(dd{t1, t2}, rh, rl);
     if (threadIdx.x == 0) {
       mp->sum_hi = tot.hi;
       mp->sum_lo = tot.lo;
@@ -1756,18 +1900,22 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
         scnt = 0;
       }
       __syncthreads();
-      dd sl{0.0, 0.0};
+      double l1 = 0.0, l2 = 0.0;
       unsigned long long kl = 0ull;
       unsigned int cl = 0;
       for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
         double x = a[q];
         unsigned long long k = ord_key(x);
         if (k < K) {
-          sl = dd_add(sl, dd{x, 0.0});
+          double s1, s2;
+          sel_slices(x, ex, s1, s2);
+          l1 += s1;
+          l2 += s2;
           kl = k > kl ? k : kl;
           cl++;
         }
       }
+      dd sl{l1, l2};
       atomicMax(&sless, kl);
       atomicAdd(&scnt, cl);
       dd sls = block_dd_sum(sl, rh, rl);
@@ -1933,7 +2081,7 @@ __global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, d
   r.removed = removed;
   r.replays = res[si].replays;
   r.path = sd.mode;
-  if (sd.mode == 1) {
+  if (sd.mode != 0) {
     r.vmin = res[si].vmin;
     r.vmax = res[si].vmax;
   } else if (sd.v_cnt > 0) {
@@ -2138,12 +2286,17 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
   int64_t max_e_items = 0;
   std::vector<SelSeg> hsel;
   int64_t sel_b = 0, sel_c = 0, sel_m = 0, sel_t = 0;
+  small1_.clear();
+  small4_.clear();
   for (int s = 0; s < nseg; s++) {
     SegDesc& sd = segs[s];
     if (sd.e_cnt > kECap) throw std::runtime_error("too many explicit records in one digest");
     sd.mode = 0;
     sd.sel = -1;
-    if (sel_on_ && sd.v_cnt > kCap && sd.v_cnt <= kSelMaxV && sd.e_cnt + 64 <= kSelSplMax) {
+    if (fused_on_ && sd.v_cnt + sd.e_cnt <= kCap) {
+      sd.mode = 2;
+      (sd.v_cnt + sd.e_cnt <= kSmallCap ? small1_ : small4_).push_back(s);
+    } else if (sel_on_ && sd.v_cnt > kCap && sd.v_cnt <= kSelMaxV && sd.e_cnt + 64 <= kSelSplMax) {
       SelSeg g{};
       g.si = s;
       g.cap_s = (int)std::min<int64_t>(kSelSplMax, std::max<int64_t>(63, sd.v_cnt / 32) + sd.e_cnt);
@@ -2314,60 +2467,87 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
       phase_end(ph_);
     }
   }
-  {
+  if (!small1_.empty() || !small4_.empty()) {
+    int* dl = as<int>(small_list_, small1_.size() + small4_.size());
+    h_small_.resize(small1_.size() + small4_.size());
+    std::copy(small1_.begin(), small1_.end(), h_small_.begin());
+    std::copy(small4_.begin(), small4_.end(), h_small_.begin() + small1_.size());
+    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
+                             stream_),
+              "H2D small list");
+    int ph_ = phase_begin("small_compress", 0.0);
+    if (!small1_.empty()) {
+      unsigned grid = (unsigned)std::min<size_t>(small1_.size(), 65535);
+      k_small_compress<kSmallCap, 64><<<grid, 64, 0, stream_>>>(
+          scale, dsegs, dl, (int)small1_.size(), d_v, esm, esw, se, pose, cm, cw, ccount, dres);
+      launched("k_small_compress<1024>");
+    }
+    if (!small4_.empty()) {
+      unsigned grid = (unsigned)std::min<size_t>(small4_.size(), 65535);
+      k_small_compress<kCap, kBlock><<<grid, kBlock, 0, stream_>>>(
+          scale, dsegs, dl + small1_.size(), (int)small4_.size(), d_v, esm, esw, se, pose, cm, cw,
+          ccount, dres);
+      launched("k_small_compress<4096>");
+    }
+    phase_end(ph_);
+  }
+  // The multi-kernel path runs when a segment may need it: one that neither
+  // the selection nor the one-block path took, or a selection fallback.
+  const bool sort_path = !hsel.empty() || (int64_t)(small1_.size() + small4_.size()) < nseg;
+  if (sort_path) {
     auto t0 = std::chrono::steady_clock::now();
     sort_values(segs, d_v, total_v);
     host_phase("sort_values",
                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-  }
-  if (!eseg.empty()) {
-    int64_t chunks = std::min<int64_t>((max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
-    for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
-      unsigned ny = (unsigned)std::min<size_t>(65535, eseg.size() - b0);
-      dim3 grid((unsigned)std::max<int64_t>(chunks, 1), ny);
-      {
-        int ph_ = phase_begin("merge_e", 16.0 * (double)total_x);
-        k_merge_e<<<grid, kBlock, 0, stream_>>>(dsegs, deseg + b0, (int)ny, vs, esm, x, pose);
-        launched("k_merge_e");
-        phase_end(ph_);
+    if (!eseg.empty()) {
+      int64_t chunks = std::min<int64_t>((max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
+      for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
+        unsigned ny = (unsigned)std::min<size_t>(65535, eseg.size() - b0);
+        dim3 grid((unsigned)std::max<int64_t>(chunks, 1), ny);
+        {
+          int ph_ = phase_begin("merge_e", 16.0 * (double)total_x);
+          k_merge_e<<<grid, kBlock, 0, stream_>>>(dsegs, deseg + b0, (int)ny, vs, esm, x, pose);
+          launched("k_merge_e");
+          phase_end(ph_);
+        }
       }
     }
-  }
-  int64_t long_cap = total_x / kTile + nseg + 1;
-  int64_t* long_list = as<int64_t>(long_, 2 * long_cap);
-  auto* long_cnt = reinterpret_cast<unsigned long long*>(dtotal + 1);
-  check_hip(hipMemsetAsync(long_cnt, 0, sizeof(unsigned long long), stream_), "memset long count");
-  {
-    int ph_ = phase_begin("chain", 0.0);
-    k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount,
-                                                  dres, long_list, long_cnt);
-    launched("k_chain");
-    phase_end(ph_);
-  }
-  if (total_t > 0) {
-    int ph_ = phase_begin("tile_sums", 8.0 * (double)total_x);
-    k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
-    launched("k_tile_sums");
-    phase_end(ph_);
-  }
-  k_scan_counts<<<1, 1024, 0, stream_>>>(ccount, nseg, cbase, dtotal);
-  launched("k_scan_counts");
-  {
-    // Upper bound of the cluster count without a host round trip.
-    unsigned grid = (unsigned)std::min<int64_t>((total_x + 255) / 256, 16384);
-    int ph_ = phase_begin("cluster_mean", 16.0 * (double)total_x / 8.0);
-    k_cluster_mean<<<std::max(grid, 1u), 256, 0, stream_>>>(dsegs, nseg, cbase, ccount, cs, x,
-                                                             pose, se, esw, cm, cw, dtotal);
-    launched("k_cluster_mean");
-    phase_end(ph_);
-  }
-  if (total_t > 2) {
-    unsigned grid = (unsigned)std::min<int64_t>(long_cap, 4096);
-    int ph_ = phase_begin("long_sum", 0.0);
-    k_long_sum<<<grid, kBlock, 0, stream_>>>(dsegs, ccount, cs, x, pose, se, esw, tsum, long_list,
-                                             long_cnt, cm, cw);
-    launched("k_long_sum");
-    phase_end(ph_);
+    int64_t long_cap = total_x / kTile + nseg + 1;
+    int64_t* long_list = as<int64_t>(long_, 2 * long_cap);
+    auto* long_cnt = reinterpret_cast<unsigned long long*>(dtotal + 1);
+    check_hip(hipMemsetAsync(long_cnt, 0, sizeof(unsigned long long), stream_), "memset long count");
+    {
+      int ph_ = phase_begin("chain", 0.0);
+      k_chain<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, scale, x, pose, se, esw, cs, ccount,
+                                                    dres, long_list, long_cnt);
+      launched("k_chain");
+      phase_end(ph_);
+    }
+    if (total_t > 0) {
+      int ph_ = phase_begin("tile_sums", 8.0 * (double)total_x);
+      k_tile_sums<<<(unsigned)total_t, kBlock, 0, stream_>>>(dsegs, nseg, x, pose, esw, tsum);
+      launched("k_tile_sums");
+      phase_end(ph_);
+    }
+    k_scan_counts<<<1, 1024, 0, stream_>>>(ccount, nseg, cbase, dtotal);
+    launched("k_scan_counts");
+    {
+      // Upper bound of the cluster count without a host round trip.
+      unsigned grid = (unsigned)std::min<int64_t>((total_x + 255) / 256, 16384);
+      int ph_ = phase_begin("cluster_mean", 16.0 * (double)total_x / 8.0);
+      k_cluster_mean<<<std::max(grid, 1u), 256, 0, stream_>>>(dsegs, nseg, cbase, ccount, cs, x,
+                                                               pose, se, esw, cm, cw, dtotal);
+      launched("k_cluster_mean");
+      phase_end(ph_);
+    }
+    if (total_t > 2) {
+      unsigned grid = (unsigned)std::min<int64_t>(long_cap, 4096);
+      int ph_ = phase_begin("long_sum", 0.0);
+      k_long_sum<<<grid, kBlock, 0, stream_>>>(dsegs, ccount, cs, x, pose, se, esw, tsum, long_list,
+                                               long_cnt, cm, cw);
+      launched("k_long_sum");
+      phase_end(ph_);
+    }
   }
   {
     int ph_ = phase_begin("finish", 0.0);
@@ -2382,7 +2562,7 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
   check_hip(hipStreamSynchronize(stream_), "sync results");
   if (sync_results)
-    for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : 1]++;
+    for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : (r.path == 2 ? 2 : 1)]++;
 }
 
 // ------------------------------------------------------ key partition
@@ -2629,12 +2809,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const u
   }
 }
 
+// Stable within the tile (so every key's values keep their batch order): wave
+// w owns the contiguous items [w * 512, (w + 1) * 512) of the tile, walked in
+// eight rounds of 64; lanes with equal digits are found with ballots, ranked
+// by lane, and counted per wave; one scan over (wave, digit) then orders the
+// waves.
 template <bool kWriteKeys>
 __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys, const double* vals,
                                                            const uint32_t* seg_start,
                                                            const uint32_t* tile_start, int nseg,
                                                            int shift, int bits, const uint32_t* dst,
                                                            uint32_t* keys_out, double* vals_out) {
+  constexpr int kWaves = kRxThreads / 64;
+  constexpr int kPerWave = kRxTile / kWaves;  // 512 items, kRxItems rounds of 64
+  __shared__ uint32_t wc[kWaves][kRxBins];
   __shared__ uint32_t lc[kRxBins];
   __shared__ uint32_t gd[kRxBins];
   __shared__ double sv[kRxTile];
@@ -2647,30 +2835,57 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys,
   const uint32_t b = min(seg_start[s + 1], a + kRxTile);
   const int R = 1 << bits;
   const uint32_t mask = (uint32_t)R - 1u;
-  for (int d = threadIdx.x; d < R; d += blockDim.x) lc[d] = 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kWaves * kRxBins; i += blockDim.x) (&wc[0][0])[i] = 0;
   __syncthreads();
   uint32_t kr[kRxItems], rk[kRxItems];
   double vr[kRxItems];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-  for (int j = 0; j < kRxItems; j++) {
-    uint32_t i = a + (uint32_t)j * kRxThreads + threadIdx.x;
-    if (i < b) {
-      kr[j] = keys[i];
-      vr[j] = vals[i];
-      rk[j] = atomicAdd(&lc[(kr[j] >> shift) & mask], 1u);
+  for (int r = 0; r < kRxItems; r++) {
+    uint32_t i = a + (uint32_t)(w * kPerWave + r * 64 + lane);
+    bool valid = i < b;
+    uint32_t d = 0;
+    if (valid) {
+      kr[r] = keys[i];
+      vr[r] = vals[i];
+      d = (kr[r] >> shift) & mask;
+    }
+    uint64_t peers = __ballot(valid);
+    for (int bit = 0; bit < bits; bit++) {
+      uint64_t bb = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    uint32_t before = (uint32_t)__popcll(peers & below);
+    uint32_t base = valid ? wc[w][d] : 0u;
+    if (valid) {
+      rk[r] = base + before;
+      if (before == 0) wc[w][d] = base + (uint32_t)__popcll(peers);
     }
   }
   __syncthreads();
+  // Per digit: offsets of the waves, and the tile total.
+  for (int d = threadIdx.x; d < R; d += blockDim.x) {
+    uint32_t run = 0;
+    for (int v = 0; v < kWaves; v++) {
+      uint32_t c = wc[v][d];
+      wc[v][d] = run;
+      run += c;
+    }
+    lc[d] = run;
+  }
+  __syncthreads();
   block_excl_scan(lc, R);
-  const size_t base = (size_t)R * tile_start[s] + lt;
-  for (int d = threadIdx.x; d < R; d += blockDim.x) gd[d] = dst[base + (size_t)d * nts];
+  const size_t gbase = (size_t)R * tile_start[s] + lt;
+  for (int d = threadIdx.x; d < R; d += blockDim.x) gd[d] = dst[gbase + (size_t)d * nts];
 #pragma unroll
-  for (int j = 0; j < kRxItems; j++) {
-    uint32_t i = a + (uint32_t)j * kRxThreads + threadIdx.x;
+  for (int r = 0; r < kRxItems; r++) {
+    uint32_t i = a + (uint32_t)(w * kPerWave + r * 64 + lane);
     if (i < b) {
-      uint32_t pos = lc[(kr[j] >> shift) & mask] + rk[j];
-      sv[pos] = vr[j];
-      sk[pos] = kr[j];
+      uint32_t d = (kr[r] >> shift) & mask;
+      uint32_t pos = lc[d] + wc[w][d] + rk[r];
+      sv[pos] = vr[r];
+      sk[pos] = kr[r];
     }
   }
   __syncthreads();

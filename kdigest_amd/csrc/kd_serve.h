@@ -1,6 +1,7 @@
 // kd_serve.h — continuous-batching server over device-resident per-key digests.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
 #include <deque>
 #include <unordered_map>
@@ -22,7 +23,21 @@ class Server {
   // empty is set for a query that found its key without centroids.
   bool result(int64_t id, std::vector<double>& out, bool& empty, bool erase);
   int64_t pending() const { return (int64_t)pending_.size(); }
+  // Seconds since the server was created (the clock of take_completed).
+  double now() const;
+  // Ids of requests completed since the last call, with completion times.
+  int64_t take_completed(int64_t* ids, double* t, int64_t cap);
   int64_t last_admitted_bytes() const { return last_admitted_bytes_; }
+  // {steps, requests admitted, bytes admitted, largest step in bytes,
+  //  ingest rounds, requests still queued}
+  void stats(int64_t* out6) const {
+    out6[0] = steps_;
+    out6[1] = admitted_;
+    out6[2] = admitted_bytes_;
+    out6[3] = max_step_bytes_;
+    out6[4] = rounds_;
+    out6[5] = (int64_t)pending_.size();
+  }
   int64_t num_keys() const { return nk_; }
   void key_state(int64_t key, double* out6) const;
   void key_centroids(int64_t key, std::vector<double>& m, std::vector<double>& w);
@@ -37,6 +52,7 @@ class Server {
     bool empty = false;  // query on a key without centroids (no result)
   };
   void refresh_state(const std::vector<int64_t>& keys);
+  void finish(int64_t id);
   void renorm_slot(int64_t key, double factor);
 
   Engine& eng_;
@@ -51,6 +67,13 @@ class Server {
   std::unordered_map<int64_t, Request> reqs_;
   int64_t next_id_ = 1;
   int64_t last_admitted_bytes_ = 0;
+  int64_t steps_ = 0, admitted_ = 0, admitted_bytes_ = 0, max_step_bytes_ = 0, rounds_ = 0;
+  std::vector<int64_t> completed_;
+  std::vector<double> completed_t_;
+  std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
+  // Host staging of the asynchronous state and install copies.
+  std::vector<int64_t> h_keys_, h_meta_;
+  std::vector<double> h_state_;
 };
 
 }  // namespace kd

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -129,9 +130,14 @@ int64_t Server::submit(int kind, int64_t key, const double* data, int64_t n) {
   return next_id_ - 1;
 }
 
+// Pushes the query state of `keys` (host scalars, centroid prefix sums) to
+// the device.  Asynchronous: the staging vectors are rewritten only after a
+// later synchronization (every compress ends with one, and so does step()).
 void Server::refresh_state(const std::vector<int64_t>& keys) {
   if (keys.empty()) return;
-  std::vector<double> st(4 * keys.size());
+  std::vector<double>& st = h_state_;
+  st.resize(4 * keys.size());
+  h_keys_ = keys;
   for (size_t i = 0; i < keys.size(); i++) {
     int64_t k = keys[i];
     st[4 * i] = stored_[k];
@@ -141,7 +147,7 @@ void Server::refresh_state(const std::vector<int64_t>& keys) {
   }
   int64_t* dk = static_cast<int64_t*>(tmpk_.reserve(keys.size() * sizeof(int64_t)));
   double* dp = static_cast<double*>(tmps_.reserve(st.size() * sizeof(double)));
-  check_hip(hipMemcpyAsync(dk, keys.data(), keys.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+  check_hip(hipMemcpyAsync(dk, h_keys_.data(), keys.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                            eng_.stream()),
             "H2D keys");
   check_hip(hipMemcpyAsync(dp, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -154,10 +160,36 @@ void Server::refresh_state(const std::vector<int64_t>& keys) {
       dk, (int64_t)keys.size(), cap_, static_cast<int64_t*>(dcount_.p),
       static_cast<double*>(w_.p), static_cast<double*>(S_.p));
   launched("k_slot_prefix");
-  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+}
+
+double Server::now() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+void Server::finish(int64_t id) {
+  Request& r = reqs_.at(id);
+  r.done = true;
+  r.data.clear();
+  r.data.shrink_to_fit();
+  completed_.push_back(id);
+  completed_t_.push_back(now());
+}
+
+int64_t Server::take_completed(int64_t* ids, double* t, int64_t cap) {
+  int64_t n = std::min<int64_t>(cap, (int64_t)completed_.size());
+  std::copy(completed_.begin(), completed_.begin() + n, ids);
+  std::copy(completed_t_.begin(), completed_t_.begin() + n, t);
+  completed_.erase(completed_.begin(), completed_.begin() + n);
+  completed_t_.erase(completed_t_.begin(), completed_t_.begin() + n);
+  return n;
+}
+
+static double ms_since(std::chrono::steady_clock::time_point a) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
 }
 
 int64_t Server::step(int max_rounds) {
+  const auto t_step = std::chrono::steady_clock::now();
   // ---- admission (arrival order, memory budget, per-key Q* I{0..R}) ----
   std::vector<int64_t> queries;
   std::vector<std::vector<int64_t>> rounds(std::max(max_rounds, 1));
@@ -186,7 +218,7 @@ int64_t Server::step(int max_rounds) {
       } else if (key_ingests[r.key] == 0) {
         if (count_[r.key] == 0) {
           r.empty = true;  // tdigest.mojo:485-487 raises on an empty digest
-          r.done = true;
+          finish(id);
           n_empty++;
         } else {
           queries.push_back(id);
@@ -196,11 +228,17 @@ int64_t Server::step(int max_rounds) {
         key_blocked[r.key] = true;  // would need the post-ingest state
       }
     }
-    if (admit) bytes += need;
+    if (admit) {
+      bytes += need;
+      admitted_++;
+    }
     else keep.push_back(id);
   }
   pending_.swap(keep);
   last_admitted_bytes_ = bytes;
+  steps_++;
+  admitted_bytes_ += bytes;
+  max_step_bytes_ = std::max(max_step_bytes_, bytes);
   int64_t done = n_empty;
 
   // ---- queries: one launch ----
@@ -241,7 +279,7 @@ int64_t Server::step(int max_rounds) {
       Request& r = reqs_.at(id);
       r.result.assign(out.begin() + pos, out.begin() + pos + (int64_t)r.data.size());
       pos += (int64_t)r.data.size();
-      r.done = true;
+      finish(id);
       done++;
     }
   }
@@ -249,6 +287,7 @@ int64_t Server::step(int max_rounds) {
   // ---- ingest rounds ----
   for (auto& round : rounds) {
     if (round.empty()) continue;
+    rounds_++;
     std::sort(round.begin(), round.end(),
               [&](int64_t a, int64_t b) { return reqs_.at(a).key < reqs_.at(b).key; });
     std::vector<SegDesc> segs;
@@ -294,25 +333,24 @@ int64_t Server::step(int max_rounds) {
     check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
     std::vector<SegResult> res;
     eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
-    std::vector<int64_t> xo(segs.size()), cnt(segs.size());
+    int64_t ns = (int64_t)segs.size();
+    std::vector<int64_t>& meta = h_meta_;
+    meta.resize(3 * ns);
     for (size_t i = 0; i < segs.size(); i++) {
-      xo[i] = segs[i].x_off;
-      cnt[i] = res[i].count;
+      meta[i] = keys[i];
+      meta[ns + i] = segs[i].x_off;
+      meta[2 * ns + i] = res[i].count;
       if (res[i].count > cap_)
         throw std::runtime_error("key " + std::to_string(keys[i]) + " exceeds the slot capacity of " +
                                  std::to_string(cap_) + " centroids");
     }
-    int64_t ns = (int64_t)segs.size();
     int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
-    check_hip(hipMemcpyAsync(dmeta, keys.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemcpyAsync(dmeta + ns, xo.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemcpyAsync(dmeta + 2 * ns, cnt.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(dmeta, meta.data(), 3 * ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
     k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
                                            eng_.cent_means(), eng_.cent_weights(),
                                            static_cast<double*>(m_.p), static_cast<double*>(w_.p),
                                            static_cast<int64_t*>(dcount_.p));
     launched("k_install");
-    check_hip(hipStreamSynchronize(st), "sync");
     for (size_t i = 0; i < segs.size(); i++) {
       int64_t k = keys[i];
       double m = (double)segs[i].v_cnt;
@@ -327,11 +365,11 @@ int64_t Server::step(int max_rounds) {
       count_[k] = res[i].count;
     }
     refresh_state(keys);
-    for (int64_t id : round) {
-      reqs_.at(id).done = true;
-      done++;
-    }
+    for (int64_t id : round) finish(id);
+    done += (int64_t)round.size();
   }
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+  if (eng_.profiling()) eng_.host_phase("serve_step", ms_since(t_step));
   return done;
 }
 

@@ -141,10 +141,15 @@ class GpuSession:
         (default) or always by a full sort; both give the same clusters."""
         check(load().kd_session_set_select(self.handle, 1 if enable else 0))
 
+    def set_fused(self, enable: bool = True) -> None:
+        """Compress digests of at most 4096 merged records in one block each
+        (default) or through the multi-kernel sort path; same clusters."""
+        check(load().kd_session_set_fused(self.handle, 1 if enable else 0))
+
     def path_counts(self) -> dict:
-        o = np.zeros(2, np.int64)
+        o = np.zeros(3, np.int64)
         check(load().kd_session_path_counts(self.handle, _ptr(o, _i64p)))
-        return {"selection": int(o[0]), "sorted": int(o[1])}
+        return {"selection": int(o[0]), "sorted": int(o[1]), "one_block": int(o[2])}
 
     def phase_stats(self) -> dict:
         """{phase: (total_ms, launches, algorithmic_bytes)} since profile()."""
@@ -589,6 +594,40 @@ class DigestServer:
         self._n[rid.value] = a.size if kind != self.INGEST else 0
         return rid.value
 
+    def submit_many(self, kinds, keys, off, data) -> np.ndarray:
+        """Request r: kinds[r], keys[r], data[off[r]:off[r+1]]; returns the ids."""
+        kn = np.ascontiguousarray(kinds, dtype=np.int32)
+        ky = np.ascontiguousarray(keys, dtype=np.int64)
+        of = np.ascontiguousarray(off, dtype=np.int64)
+        d = _f64(data)
+        ids = np.empty(kn.size, np.int64)
+        check(load().kd_server_submit_many(self._h, kn.size, _ptr(kn, C.POINTER(C.c_int)),
+                                           _ptr(ky, _i64p), _ptr(of, _i64p), _ptr(d), _ptr(ids, _i64p)))
+        sizes = np.diff(of)
+        for r, rid in enumerate(ids.tolist()):
+            self._n[rid] = int(sizes[r]) if kn[r] != self.INGEST else 0
+        return ids
+
+    def take_completed(self, cap: int = 1 << 16):
+        """(ids, completion times on the server clock) since the last call."""
+        ids = np.empty(cap, np.int64)
+        t = np.empty(cap)
+        n = C.c_int64()
+        check(load().kd_server_take_completed(self._h, _ptr(ids, _i64p), _ptr(t), cap, C.byref(n)))
+        return ids[:n.value], t[:n.value]
+
+    def clock(self) -> float:
+        t = C.c_double()
+        check(load().kd_server_clock(self._h, C.byref(t)))
+        return t.value
+
+    def discard(self, rid: int) -> None:
+        """Forget a completed request without reading its result."""
+        try:
+            self.result(rid)
+        except KdigestError:
+            pass
+
     def submit_ingest(self, key: int, values) -> int:
         return self.submit(self.INGEST, key, values)
 
@@ -624,6 +663,12 @@ class DigestServer:
         p = C.c_int64()
         check(load().kd_server_pending(self._h, C.byref(p)))
         return p.value
+
+    def stats(self) -> dict:
+        o = np.zeros(6, np.int64)
+        check(load().kd_server_stats(self._h, _ptr(o, _i64p)))
+        return dict(steps=int(o[0]), admitted=int(o[1]), admitted_bytes=int(o[2]),
+                    max_step_bytes=int(o[3]), ingest_rounds=int(o[4]), queued=int(o[5]))
 
     def key_state(self, key: int) -> dict:
         o = np.empty(6)

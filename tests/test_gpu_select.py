@@ -131,3 +131,30 @@ def test_select_grouped_zipf(kd, oracle, sessions):
         rm, rw = refs[k].centroids()
         check_centroids(ma, wa, mb, wb, 300.0, True, f"key {k} sel vs sort")
         check_centroids(ma, wa, rm, rw, 300.0, True, f"key {k} sel vs oracle")
+
+
+def test_one_block_path_matches_sort_path(kd, oracle):
+    """Digests of at most 4096 merged records compressed in one block give
+    the multi-kernel path's clusters bit for bit (same arithmetic, same
+    order), with and without prior centroids and decay."""
+    fused, multi = kd.GpuSession(), kd.GpuSession()
+    multi.set_fused(False)
+    rng = np.random.default_rng(21)
+    for scale, lam in ((1, 0.0), (2, 1e-4), (3, 0.0), (0, 5e-3)):
+        before = fused.path_counts()
+        a = kd.GroupedTDigest(300, 100.0, kd.Decay.rate(lam), scale=scale, session=fused)
+        b = kd.GroupedTDigest(300, 100.0, kd.Decay.rate(lam), scale=scale, session=multi)
+        for _ in range(3):
+            keys = (rng.zipf(1.2, 200000) % 300).astype(np.uint32)
+            vals = rng.standard_t(3, keys.size) + keys
+            vals[::7] = np.round(vals[::7])  # some ties
+            a.add_batch(keys, vals)
+            b.add_batch(keys, vals)
+        for k in range(300):
+            ma, wa = a.centroids(k)
+            mb, wb = b.centroids(k)
+            assert ma.shape == mb.shape, (scale, k)
+            assert np.array_equal(ma.view(np.uint64), mb.view(np.uint64)), (scale, lam, k)
+            assert np.array_equal(wa.view(np.uint64), wb.view(np.uint64)), (scale, lam, k)
+        after = fused.path_counts()
+        assert after["one_block"] > before["one_block"]

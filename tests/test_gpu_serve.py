@@ -149,3 +149,36 @@ def test_serve_rejects_bad_requests(kd):
     with pytest.raises(kd.KdigestError):  # empty key
         srv.result(r)
     assert srv.pending == 0
+
+
+def test_serve_submit_many_and_completions(kd):
+    rng = np.random.default_rng(3)
+    kinds = [0, 0, 1, 2, 0, 1, 2]
+    keys = [1, 2, 1, 2, 1, 7, 1]
+    datas = [rng.standard_normal(1000), rng.standard_normal(50), np.array([0.5, 0.9]),
+             np.array([0.0, 1.0]), rng.standard_normal(3000) + 1.0, np.array([0.5]),
+             np.array([-0.5, 0.5, 2.0])]
+    off = np.cumsum([0] + [len(d) for d in datas])
+    srv = kd.DigestServer(8, 100.0)
+    with pytest.raises(kd.KdigestError):  # one bad request: nothing is queued
+        srv.submit_many([0, 1], [0, 0], [0, 1, 2], [1.0, 2.0])
+    assert srv.pending == 0
+    ids = srv.submit_many(kinds, keys, off, np.concatenate(datas))
+    while srv.pending:
+        srv.step(4)
+    got, t = srv.take_completed()
+    assert sorted(got.tolist()) == sorted(ids.tolist())
+    assert np.all(np.diff(t) >= 0) and t[0] >= 0.0 and t[-1] <= srv.clock()
+    assert srv.take_completed()[0].size == 0
+    one = kd.DigestServer(8, 100.0)
+    for rid, k, key, d in zip(ids.tolist(), kinds, keys, datas):
+        r1 = one.submit(k, key, d)
+        assert one.step(1) == 1
+        one.take_completed()
+        if key == 7:
+            for s_, r_ in ((srv, rid), (one, r1)):
+                with pytest.raises(kd.KdigestError):
+                    s_.result(r_)
+            continue
+        a, b = srv.result(rid), one.result(r1)
+        assert a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64)), (k, key)
