@@ -24,6 +24,9 @@
 #include "kd_query.h"
 #include "kd_serve.h"
 
+static_assert(kd::KD_EEMPTY_SERVE == KD_EEMPTY && kd::KD_ENOMEM_SERVE == KD_ENOMEM,
+              "server failure statuses are the C ABI's codes");
+
 using kd::DevBuf;
 using kd::SegDesc;
 using kd::SegResult;
@@ -1473,16 +1476,17 @@ extern "C" int kd_server_step(kd_server* v, int max_rounds, int64_t* completed) 
 extern "C" int kd_server_result(kd_server* v, int64_t id, double* out, int64_t cap, int* done) {
   return guarded([&] {
     std::vector<double> r;
-    bool empty = false;
-    *done = v->srv->result(id, r, empty, false) ? 1 : 0;
+    int err = 0;
+    std::string msg;
+    *done = v->srv->result(id, r, err, msg, false) ? 1 : 0;
     if (*done) {
-      if (empty) {
-        v->srv->result(id, r, empty, true);
-        throw KdError(KD_EEMPTY, "query on a key whose digest is empty");
+      if (err != 0) {
+        v->srv->result(id, r, err, msg, true);
+        throw KdError(err, msg);
       }
       if (cap < (int64_t)r.size()) throw KdError(KD_EINVAL, "output capacity below the request size");
       if (!r.empty()) memcpy(out, r.data(), r.size() * sizeof(double));
-      v->srv->result(id, r, empty, true);
+      v->srv->result(id, r, err, msg, true);
     }
     return KD_OK;
   });

@@ -4,12 +4,17 @@
 #include <chrono>
 #include <cstdint>
 #include <deque>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "kd_engine.h"
 
 namespace kd {
+
+// Failure statuses of a request (the C ABI's KD_EEMPTY and KD_ENOMEM).
+constexpr int KD_EEMPTY_SERVE = -2;
+constexpr int KD_ENOMEM_SERVE = -7;
 
 class Server {
  public:
@@ -20,8 +25,9 @@ class Server {
   // Admits and runs one continuous batch; returns the number of completed requests.
   int64_t step(int max_rounds);
   // Copies a completed request's result (query values); false while pending.
-  // empty is set for a query that found its key without centroids.
-  bool result(int64_t id, std::vector<double>& out, bool& empty, bool erase);
+  // err: 0, or the status of a request that failed (a query on a key
+  // without centroids, a merge beyond the slot capacity), with a message.
+  bool result(int64_t id, std::vector<double>& out, int& err, std::string& msg, bool erase);
   int64_t pending() const { return (int64_t)pending_.size(); }
   // Seconds since the server was created (the clock of take_completed).
   double now() const;
@@ -49,7 +55,8 @@ class Server {
     int64_t key = 0;
     std::vector<double> data, result;
     bool done = false;
-    bool empty = false;  // query on a key without centroids (no result)
+    int err = 0;  // failure status (no result), see result()
+    std::string msg;
   };
   void refresh_state(const std::vector<int64_t>& keys);
   void finish(int64_t id);

@@ -217,7 +217,8 @@ int64_t Server::step(int max_rounds) {
         }
       } else if (key_ingests[r.key] == 0) {
         if (count_[r.key] == 0) {
-          r.empty = true;  // tdigest.mojo:485-487 raises on an empty digest
+          r.err = KD_EEMPTY_SERVE;  // tdigest.mojo:485-487 raises on an empty digest
+          r.msg = "query on a key whose digest is empty";
           finish(id);
           n_empty++;
         } else {
@@ -295,9 +296,14 @@ int64_t Server::step(int max_rounds) {
     std::vector<int64_t> keys;
     std::vector<char> had;
     std::vector<double> sfs;
+    struct Saved {
+      double stored, logs, clock;
+    };
+    std::vector<Saved> saved;
     for (int64_t id : round) {
       const Request& r = reqs_.at(id);
       int64_t k = r.key, n = (int64_t)r.data.size();
+      saved.push_back(Saved{stored_[k], logs_[k], clock_[k]});
       // _gpu_batch_prepare (tdigest.mojo:236-296).
       double nclock = clock_[k] + (double)n;
       if (lambda_ > 0.0 && nclock > clock_[k]) {
@@ -307,6 +313,10 @@ int64_t Server::step(int max_rounds) {
           renorm_slot(k, f);
           stored_[k] *= f;
           logs_[k] = 0.0;
+          // A failed merge keeps the renormalized weights: the saved state
+          // is rescaled to match them (same digest, other representation).
+          saved.back().stored *= f;
+          saved.back().logs = 0.0;
         }
       }
       clock_[k] = nclock;
@@ -333,25 +343,45 @@ int64_t Server::step(int max_rounds) {
     check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
     std::vector<SegResult> res;
     eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
-    int64_t ns = (int64_t)segs.size();
-    std::vector<int64_t>& meta = h_meta_;
-    meta.resize(3 * ns);
+    // A merge that would not fit its key's slot fails that request alone:
+    // the key keeps its state and nothing is installed for it.
+    std::vector<int64_t> ok;
     for (size_t i = 0; i < segs.size(); i++) {
-      meta[i] = keys[i];
-      meta[ns + i] = segs[i].x_off;
-      meta[2 * ns + i] = res[i].count;
-      if (res[i].count > cap_)
-        throw std::runtime_error("key " + std::to_string(keys[i]) + " exceeds the slot capacity of " +
-                                 std::to_string(cap_) + " centroids");
+      if (res[i].count > cap_) {
+        int64_t k = keys[i];
+        stored_[k] = saved[i].stored;
+        logs_[k] = saved[i].logs;
+        clock_[k] = saved[i].clock;
+        Request& r = reqs_.at(round[i]);
+        r.err = KD_ENOMEM_SERVE;
+        r.msg = "key " + std::to_string(k) + " needs " + std::to_string(res[i].count) +
+                " centroids; the slot capacity is " + std::to_string(cap_);
+      } else {
+        ok.push_back((int64_t)i);
+      }
     }
-    int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
-    check_hip(hipMemcpyAsync(dmeta, meta.data(), 3 * ns * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-    k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
-                                           eng_.cent_means(), eng_.cent_weights(),
-                                           static_cast<double*>(m_.p), static_cast<double*>(w_.p),
-                                           static_cast<int64_t*>(dcount_.p));
-    launched("k_install");
-    for (size_t i = 0; i < segs.size(); i++) {
+    int64_t ns = (int64_t)ok.size();
+    std::vector<int64_t>& meta = h_meta_;
+    meta.resize(3 * std::max<int64_t>(ns, 1));
+    std::vector<int64_t> okeys(ns);
+    for (int64_t q = 0; q < ns; q++) {
+      int64_t i = ok[q];
+      okeys[q] = keys[i];
+      meta[q] = keys[i];
+      meta[ns + q] = segs[i].x_off;
+      meta[2 * ns + q] = res[i].count;
+    }
+    if (ns > 0) {
+      int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
+      check_hip(hipMemcpyAsync(dmeta, meta.data(), 3 * ns * sizeof(int64_t), hipMemcpyHostToDevice, st),
+                "H2D");
+      k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
+                                             eng_.cent_means(), eng_.cent_weights(),
+                                             static_cast<double*>(m_.p), static_cast<double*>(w_.p),
+                                             static_cast<int64_t*>(dcount_.p));
+      launched("k_install");
+    }
+    for (int64_t i : ok) {
       int64_t k = keys[i];
       double m = (double)segs[i].v_cnt;
       if (stored_[k] == m * sfs[i] && !had[i]) {  // :285-295
@@ -364,6 +394,7 @@ int64_t Server::step(int max_rounds) {
       stored_[k] = res[i].total;  // :304-315
       count_[k] = res[i].count;
     }
+    keys.swap(okeys);
     refresh_state(keys);
     for (int64_t id : round) finish(id);
     done += (int64_t)round.size();
@@ -384,12 +415,13 @@ void Server::renorm_slot(int64_t key, double factor) {
   check_hip(hipStreamSynchronize(eng_.stream()), "sync");
 }
 
-bool Server::result(int64_t id, std::vector<double>& out, bool& empty, bool erase) {
+bool Server::result(int64_t id, std::vector<double>& out, int& err, std::string& msg, bool erase) {
   auto it = reqs_.find(id);
   if (it == reqs_.end()) throw std::runtime_error("unknown request id");
   if (!it->second.done) return false;
   out = it->second.result;
-  empty = it->second.empty;
+  err = it->second.err;
+  msg = it->second.msg;
   if (erase) reqs_.erase(it);
   return true;
 }

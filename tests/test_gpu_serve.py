@@ -182,3 +182,22 @@ def test_serve_submit_many_and_completions(kd):
             continue
         a, b = srv.result(rid), one.result(r1)
         assert a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64)), (k, key)
+
+
+def test_serve_slot_overflow_fails_only_that_request(kd):
+    srv = kd.DigestServer(4, 100.0, scale=kd.ScaleK1, slot_capacity=8)
+    rng = np.random.default_rng(4)
+    small = srv.submit_ingest(1, [1.0, 2.0, 3.0])
+    big = srv.submit_ingest(0, rng.standard_normal(20000))  # about 50 centroids
+    while srv.pending:
+        srv.step(4)
+    assert srv.result(small) is not None
+    with pytest.raises(kd.KdigestError) as e:
+        srv.result(big)
+    assert e.value.code == -7  # KD_ENOMEM
+    st = srv.key_state(0)
+    assert st["centroids"] == 0 and st["stored_total"] == 0.0 and st["clock"] == 0.0
+    assert srv.key_state(1)["stored_total"] == 3.0
+    q = srv.submit_quantile(1, [0.5])
+    srv.step(1)
+    assert srv.result(q)[0] == 2.0
