@@ -157,8 +157,9 @@ class Engine {
   DevBuf extra_[4];
   // Selection path (large segments compressed without a full sort).
   DevBuf sel_seg_, sel_st_, sel_spl_, sel_cnt_, sel_bst_, sel_info_, sel_bnd_, sel_ce_, sel_csl_,
-      sel_mk_, sel_ml_, sel_mlc_;
+      sel_mk_, sel_ml_, sel_mlc_, sel_bix_;
   std::vector<char> h_sel_;
+  unsigned int h_fallbacks_ = 0;
   bool sel_on_ = true;
   int64_t path_counts_[3] = {0, 0, 0};
   bool fused_on_ = true;

@@ -1409,7 +1409,8 @@ template <int SMAX>
 __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, const SelSeg* sel,
                                                            int nsel, int64_t t0, int64_t t1,
                                                            const double* vin, const double* spl,
-                                                           SelState* st, unsigned int* cnt) {
+                                                           SelState* st, unsigned int* cnt,
+                                                           uint16_t* bix) {
   __shared__ double sp[SMAX];
   __shared__ unsigned int lc[SMAX + 1];
   __shared__ unsigned long long smin, smax;
@@ -1430,9 +1431,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
   unsigned long long kmn = ~0ull, kmx = 0ull;
+  uint16_t* bx = bix + sd.v_off;
   for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
     double x = v[i];
-    atomicAdd(&lc[sel_bucket(sp, S, x)], 1u);
+    int bk = sel_bucket(sp, S, x);
+    atomicAdd(&lc[bk], 1u);
+    bx[i] = (uint16_t)bk;  // the gather pass reads it back instead of searching again
     unsigned long long k = ord_key(x);
     kmn = k < kmn ? k : kmn;
     kmx = k > kmx ? k : kmx;
@@ -1640,13 +1644,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
 template <int SMAX, int CMAX>
 __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs, const SelSeg* sel,
                                                             int nsel, int64_t t0, int64_t t1,
-                                                            const double* vin, const double* spl,
+                                                            const double* vin, const uint16_t* bix,
                                                             const int* binfo, const int* ce,
                                                             const SelState* st, SelMark* mk,
                                                             double* csl, double* X) {
   constexpr int kItems = 8;                 // values per thread per chunk
   constexpr int kMk = 2 * CMAX + 2;          // gathered buckets of a segment at most
-  __shared__ double sp[SMAX];
   __shared__ int inf[SMAX + 1];
   __shared__ int ex[CMAX];
   __shared__ double acc[2 * CMAX];
@@ -1658,13 +1661,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl, C = st[j].ncl, nmk = st[j].nmk;
-  for (int i = threadIdx.x; i < S; i += blockDim.x) sp[i] = spl[g.b_off + i];
   for (int i = threadIdx.x; i <= S; i += blockDim.x) inf[i] = binfo[g.b_off + i];
   for (int i = threadIdx.x; i < C; i += blockDim.x) ex[i] = ce[g.c_off + i];
   for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) acc[i] = 0.0;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
+  const uint16_t* bx = bix + sd.v_off;
   double* xg = X + sd.x_off;
   SelMark* m = mk + g.m_off;
   const int64_t chunk = (int64_t)kItems * kSelThreads;
@@ -1680,7 +1683,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
       fr[q] = 0x7fffffff;
       if (i < b) {
         double x = v[i];
-        int f = inf[sel_bucket(sp, S, x)];
+        int f = inf[bx[i]];
         xr[q] = x;
         fr[q] = f;
         if (f < 0) {
@@ -1959,7 +1962,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
                                                          const double* esm, const double* esw,
                                                          const int64_t* pose, double* cm,
                                                          double* cw, int64_t* ccount,
-                                                         SegResult* res) {
+                                                         SegResult* res, unsigned int* nfallback) {
   __shared__ int s_fail;
   const int j = blockIdx.x;
   const SelSeg g = sel[j];
@@ -1987,7 +1990,10 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
   }
   __syncthreads();
   if (s_fail) {
-    if (threadIdx.x == 0) segs[g.si].mode = 0;
+    if (threadIdx.x == 0) {
+      segs[g.si].mode = 0;
+      atomicAdd(nfallback, 1u);
+    }
     return;
   }
   const double vmin = key_value(st[j].kmin), vmax = key_value(st[j].kmax);
@@ -2392,10 +2398,11 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     double* dcsl = as<double>(sel_csl_, 2 * sel_c);
     SelMark* dmk = as<SelMark>(sel_mk_, sel_m);
     int2* dml = as<int2>(sel_ml_, sel_m);
-    unsigned int* dmlc = as<unsigned int>(sel_mlc_, 1);
+    uint16_t* dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
+    unsigned int* dmlc = as<unsigned int>(sel_mlc_, 2);  // gathered-bucket list size, fallbacks
     check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
     check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
-    check_hip(hipMemsetAsync(dmlc, 0, sizeof(unsigned int), stream_), "memset sel list");
+    check_hip(hipMemsetAsync(dmlc, 0, 2 * sizeof(unsigned int), stream_), "memset sel list");
     k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
     launched("k_sel_init");
     {
@@ -2412,15 +2419,15 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     double sel_bytes = 0.0;
     for (const SelSeg& g : hsel) sel_bytes += 8.0 * (double)segs[g.si].v_cnt;
     {
-      int ph_ = phase_begin("sel_count", sel_bytes);
+      int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
       if (t_small > 0) {
         k_sel_count<kSelSmallS><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dst, dcnt);
+            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dst, dcnt, dbix);
         launched("k_sel_count<small>");
       }
       if (sel_t > t_small) {
         k_sel_count<kSelSplMax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dst, dcnt);
+            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dst, dcnt, dbix);
         launched("k_sel_count");
       }
       phase_end(ph_);
@@ -2433,15 +2440,15 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
       phase_end(ph_);
     }
     {
-      int ph_ = phase_begin("sel_gather", sel_bytes);
+      int ph_ = phase_begin("sel_gather", sel_bytes * 1.25);
       if (t_small > 0) {
         k_sel_gather<kSelSmallS, kSelSmallC><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dinfo, dce, dst, dmk, dcsl, x);
+            dsegs, dsel, nsel, 0, t_small, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
         launched("k_sel_gather<small>");
       }
       if (sel_t > t_small) {
         k_sel_gather<kSelSplMax, kSelCmax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dinfo, dce, dst, dmk, dcsl, x);
+            dsegs, dsel, nsel, t_small, sel_t, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
         launched("k_sel_gather");
       }
       phase_end(ph_);
@@ -2462,10 +2469,14 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     {
       int ph_ = phase_begin("sel_finalize", 0.0);
       k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dbnd, dmk, dinfo, dbst, dcsl,
-                                                   esm, esw, pose, cm, cw, ccount, dres);
+                                                   esm, esw, pose, cm, cw, ccount, dres, dmlc + 1);
       launched("k_sel_finalize");
       phase_end(ph_);
     }
+    // Whether any selected segment fell back to sorting.
+    check_hip(hipMemcpyAsync(&h_fallbacks_, dmlc + 1, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                             stream_),
+              "D2H fallbacks");
   }
   if (!small1_.empty() || !small4_.empty()) {
     int* dl = as<int>(small_list_, small1_.size() + small4_.size());
@@ -2491,9 +2502,13 @@ void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
     }
     phase_end(ph_);
   }
-  // The multi-kernel path runs when a segment may need it: one that neither
-  // the selection nor the one-block path took, or a selection fallback.
-  const bool sort_path = !hsel.empty() || (int64_t)(small1_.size() + small4_.size()) < nseg;
+  // The multi-kernel path runs when a segment needs it: one that neither the
+  // selection nor the one-block path took, or a selection fallback.
+  bool sort_path = (int64_t)(small1_.size() + small4_.size() + hsel.size()) < nseg;
+  if (!hsel.empty() && !sort_path) {
+    check_hip(hipStreamSynchronize(stream_), "sync fallbacks");
+    sort_path = h_fallbacks_ > 0;
+  }
   if (sort_path) {
     auto t0 = std::chrono::steady_clock::now();
     sort_values(segs, d_v, total_v);
