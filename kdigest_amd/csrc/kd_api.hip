@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kdigest_hip.h"
@@ -76,6 +77,28 @@ void hip_ok(hipError_t e, const char* what) { kd::check_hip(e, what); }
 
 using clk = std::chrono::steady_clock;
 // Milliseconds since t; resets t (consecutive host phases).
+// Host threads for per-key loops (the box's CPU share; at most 16).
+int host_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(n, 16u));
+}
+
+// fn(begin, end, thread) over [0, n) split into nthr contiguous chunks.
+template <class Fn>
+void parallel_for(int64_t n, int nthr, Fn fn) {
+  if (nthr <= 1 || n < 2) {
+    fn(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthr - 1) / nthr;
+  for (int t = 0; t < nthr; t++) {
+    int64_t a = std::min(n, t * chunk), b = std::min(n, a + chunk);
+    th.emplace_back([=, &fn] { fn(a, b, t); });
+  }
+  for (auto& x : th) x.join();
+}
+
 double ms_since(clk::time_point& t) {
   clk::time_point now = clk::now();
   double ms = std::chrono::duration<double, std::milli>(now - t).count();
@@ -199,6 +222,9 @@ struct kd_session {
   DevBuf q_off, q_m, q_w, q_s, q_state, q_qd, q_q, q_out, e_m, e_w, v_buf, tmp, tmp2;
   // Grouped-ingest scratch shared by every grouped digest of the session.
   DevBuf g_counts, g_koff, g_vals, g_keys, g_valsin, g_segof, g_xoff;
+  // Page-locked segment tables and results, reused by every compress call.
+  kd::SegVec segv;
+  kd::ResVec resv;
   explicit kd_session(int dev) : eng(dev) {}
   hipStream_t st() { return eng.stream(); }
   template <typename T>
@@ -306,9 +332,10 @@ extern "C" int kd_compress(kd_session* s, int scale, double nrm, double stored_n
     s->h2d(em, means, n_prior * sizeof(double));
     s->h2d(ew, weights, n_prior * sizeof(double));
     s->h2d(v, means + n_prior, n * sizeof(double));
-    std::vector<SegDesc> segs(1);
+    kd::SegVec& segs = s->segv;
+    segs.resize(1);
     segs[0] = SegDesc{0, n, 0, n_prior, 0, 0, 0, w_uniform, stored_n, nrm, 0.0};
-    std::vector<SegResult> res;
+    kd::ResVec& res = s->resv;
     s->eng.compress(scale, segs, v, em, ew, res);
     int64_t c = res[0].count;
     s->d2h(out_means, s->eng.cent_means() + segs[0].x_off, c * sizeof(double));
@@ -456,9 +483,10 @@ struct kd_tdigest {
     double eff = effective_count();
     double nrm = kd_normalizer(scale, compression, eff);
     double floor = decay() ? kPruneEpsilon * kd_exp(log_scale) : 0.0;
-    std::vector<SegDesc> segs(1);
+    kd::SegVec& segs = s->segv;
+    segs.resize(1);
     segs[0] = SegDesc{0, n, 0, ne, 0, 0, 0, sf, stored_total, nrm, floor};
-    std::vector<SegResult> res;
+    kd::ResVec& res = s->resv;
     s->eng.compress(scale, segs, d_values, em, ew, res);
     int64_t c = res[0].count;
     cm.reserve(std::max<int64_t>(c, 1) * sizeof(double));
@@ -833,6 +861,7 @@ struct kd_grouped {
   }
 
   void add_device(const uint32_t* d_keys, const double* d_values, int64_t n);
+  std::vector<int64_t> h_kcnt, h_kof, h_segof;  // per-key counts / offsets / segment of the last batch
   void query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq, double* d_out);
 };
 
@@ -870,64 +899,89 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   clk::time_point tp = clk::now();
   int* flag = s->buf<int>(s->tmp, 1);
   hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
-  k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
-  launched("k_validate");
+  const bool fused_check = s->eng.key_partition_validates(n, num_keys);
   int hflag = 0;
-  s->d2h(&hflag, flag, sizeof(int));
-  s->sync();
-  if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
-  s->eng.host_phase("validate", ms_since(tp));
+  if (!fused_check) {
+    k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
+    launched("k_validate");
+    s->d2h(&hflag, flag, sizeof(int));
+    s->sync();
+    if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
+    s->eng.host_phase("validate", ms_since(tp));
+  }
 
-  // Group per key (:99-104) on the device.
+  // Group per key (:99-104) on the device; the radix partition validates the
+  // batch in its first pass, before any digest state changes.
   int64_t* dcnt = static_cast<int64_t*>(s->g_counts.reserve(num_keys * sizeof(int64_t)));
   int64_t* dkoff = static_cast<int64_t*>(s->g_koff.reserve((num_keys + 1) * sizeof(int64_t)));
   double* dv = static_cast<double*>(s->g_vals.reserve(std::max<int64_t>(n, 1) * sizeof(double)));
-  s->eng.key_partition(d_keys, d_values, n, num_keys, dcnt, dkoff, dv);
-  std::vector<int64_t> kcnt(num_keys), kof(num_keys + 1);
+  s->eng.key_partition(d_keys, d_values, n, num_keys, dcnt, dkoff, dv, fused_check ? flag : nullptr);
+  std::vector<int64_t>& kcnt = h_kcnt;
+  std::vector<int64_t>& kof = h_kof;
+  kcnt.resize(num_keys);
+  kof.resize(num_keys + 1);
   s->d2h(kcnt.data(), dcnt, num_keys * sizeof(int64_t));
   s->d2h(kof.data(), dkoff, (num_keys + 1) * sizeof(int64_t));
+  if (fused_check) s->d2h(&hflag, flag, sizeof(int));
   s->sync();
+  if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
   s->eng.host_phase("key_partition", ms_since(tp));
 
-  // Per touched key: B-semantics prepare (tdigest.mojo:236-296).
-  std::vector<SegDesc> segs;
-  std::vector<int64_t> seg_of(num_keys, -1), seg_key;
+  // Per touched key: B-semantics prepare (tdigest.mojo:236-296), keys split
+  // over host threads (each key's state is its own).
+  std::vector<int64_t>& seg_of = h_segof;
+  seg_of.assign(num_keys, -1);
+  int64_t nseg = 0;
+  for (int64_t k = 0; k < num_keys; k++)
+    if (kcnt[k] > 0) seg_of[k] = nseg++;
+  kd::SegVec& segs = s->segv;
+  segs.resize(nseg);
+  std::vector<int64_t> seg_key(nseg);
+  std::vector<double> sfk(nseg);
+  std::vector<char> had_cent(nseg);
+  const int nthr = nseg >= 16384 ? host_threads() : 1;
+  std::vector<std::vector<int64_t>> rk(nthr);
+  std::vector<std::vector<double>> rf(nthr);
+  parallel_for(num_keys, nthr, [&](int64_t k0, int64_t k1, int t) {
+    for (int64_t k = k0; k < k1; k++) {
+      int64_t m = kcnt[k];
+      if (m == 0) continue;
+      double nc_clock = clock[k] + (double)m;
+      if (decay() && nc_clock > clock[k]) {  // advance_clock (:324-332)
+        log_scale[k] += lambda * (nc_clock - clock[k]);
+        if (log_scale[k] > kRenormThreshold) {  // _renormalize (:359-371)
+          double factor = kd_exp(-log_scale[k]);
+          rk[t].push_back(k);
+          rf[t].push_back(factor);
+          stored_total[k] *= factor;
+          log_scale[k] = 0.0;
+        }
+      }
+      clock[k] = nc_clock;
+      double sf = decay() ? kd_exp(log_scale[k]) : 1.0;
+      int64_t i = seg_of[k];
+      had_cent[i] = count(k) > 0;
+      stored_total[k] += (double)m * sf;
+      double effn = decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k];
+      SegDesc sd{};
+      sd.v_off = kof[k];
+      sd.v_cnt = m;
+      sd.e_off = hoff[k];
+      sd.e_cnt = count(k);
+      sd.sf = sf;
+      sd.stored_n = stored_total[k];
+      sd.nrm = kd_normalizer(scale, compression, effn);
+      sd.floor = decay() ? kPruneEpsilon * kd_exp(log_scale[k]) : 0.0;
+      segs[i] = sd;
+      seg_key[i] = k;
+      sfk[i] = sf;
+    }
+  });
   std::vector<int64_t> rkeys;
   std::vector<double> rfac;
-  std::vector<double> sfk;
-  std::vector<char> had_cent;
-  for (int64_t k = 0; k < num_keys; k++) {
-    int64_t m = kcnt[k];
-    if (m == 0) continue;
-    double nc_clock = clock[k] + (double)m;
-    if (decay() && nc_clock > clock[k]) {  // advance_clock (:324-332)
-      log_scale[k] += lambda * (nc_clock - clock[k]);
-      if (log_scale[k] > kRenormThreshold) {  // _renormalize (:359-371)
-        double factor = kd_exp(-log_scale[k]);
-        rkeys.push_back(k);
-        rfac.push_back(factor);
-        stored_total[k] *= factor;
-        log_scale[k] = 0.0;
-      }
-    }
-    clock[k] = nc_clock;
-    double sf = decay() ? kd_exp(log_scale[k]) : 1.0;
-    had_cent.push_back(count(k) > 0);
-    stored_total[k] += (double)m * sf;
-    double effn = decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k];
-    SegDesc sd{};
-    sd.v_off = kof[k];
-    sd.v_cnt = m;
-    sd.e_off = hoff[k];
-    sd.e_cnt = count(k);
-    sd.sf = sf;
-    sd.stored_n = stored_total[k];
-    sd.nrm = kd_normalizer(scale, compression, effn);
-    sd.floor = decay() ? kPruneEpsilon * kd_exp(log_scale[k]) : 0.0;
-    seg_of[k] = (int64_t)segs.size();
-    segs.push_back(sd);
-    seg_key.push_back(k);
-    sfk.push_back(sf);
+  for (int t = 0; t < nthr; t++) {
+    rkeys.insert(rkeys.end(), rk[t].begin(), rk[t].end());
+    rfac.insert(rfac.end(), rf[t].begin(), rf[t].end());
   }
   if (segs.empty()) return;
   if (!rkeys.empty()) {
@@ -939,7 +993,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
     launched("k_scale_keys");
     hip_ok(hipGetLastError(), "renormalize launch");
   }
-  std::vector<SegResult> res;
+  kd::ResVec& res = s->resv;
   s->eng.host_phase("prepare", ms_since(tp));
   s->eng.compress(scale, segs, dv, dm(), dw(), res);
   s->eng.host_phase("compress", ms_since(tp));

@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,31 @@ struct SegResult {
   int64_t replays;  // tie decisions resolved by an exact replay of the running mean
   int64_t path;     // 0: multi-kernel sort path, 1: selection, 2: one block per digest
 };
+
+// Page-locked host allocator (hipHostMalloc): copies from and to such memory
+// run at link speed without staging.  Allocation is slow, so vectors using it
+// are kept and reused.
+template <class T>
+struct PinnedAlloc {
+  using value_type = T;
+  PinnedAlloc() = default;
+  template <class U>
+  PinnedAlloc(const PinnedAlloc<U>&) {}
+  T* allocate(size_t n) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess || !p)
+      throw std::bad_alloc();
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+  template <class U>
+  bool operator==(const PinnedAlloc<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+
+using SegVec = std::vector<SegDesc, PinnedAlloc<SegDesc>>;
+using ResVec = std::vector<SegResult, PinnedAlloc<SegResult>>;
 
 // Owning device allocation, grown on demand (contents are not preserved when
 // it grows).  Move-only: a copy would free the same pointer twice.
@@ -96,8 +122,8 @@ class Engine {
   // caller fills v_cnt/e_cnt/sf/stored_n/nrm/floor; the engine assigns the
   // offsets.  Results stay on the device (cent_means()/cent_weights() at
   // SegDesc::x_off) and per-segment summaries are returned in `res`.
-  void compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
-                const double* d_em, const double* d_ew, std::vector<SegResult>& res,
+  void compress(int scale, SegVec& segs, const double* d_v,
+                const double* d_em, const double* d_ew, ResVec& res,
                 bool sync_results = true);
 
   const double* cent_means() const { return static_cast<const double*>(cm_.p); }
@@ -108,8 +134,13 @@ class Engine {
   // Key partition of (key, value) pairs: counts per key and values grouped by
   // key (batch order within a key is not kept: all of a key's new values carry
   // the same weight, so their order among equals cannot change the result).
+  // With d_bad != nullptr and key_partition_validates(n, num_keys), the
+  // batch is also validated on the way (a key >= num_keys or a non-finite
+  // value sets *d_bad; the grouping is then meaningless but stays in bounds).
   void key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
-                     int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out);
+                     int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,
+                     int* d_bad = nullptr);
+  bool key_partition_validates(int64_t n, int64_t num_keys) const;
 
   DevBuf& scratch(int i) { return extra_[i]; }
 
@@ -147,7 +178,7 @@ class Engine {
   bool profiling() const { return prof_; }
 
  private:
-  void sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v);
+  void sort_values(SegVec& segs, const double* d_v, int64_t total_v);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -169,7 +200,8 @@ class Engine {
   // segment and tile offsets.
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
   void key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n, int bits,
-                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out);
+                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,
+                           int* d_bad);
   // Host staging that must outlive the asynchronous copies reading it.
   std::vector<char> h_parts_, h_tiles_, h_finals_;
   bool prof_ = false;

@@ -2167,7 +2167,7 @@ static T* as(DevBuf& b, size_t count) {
   return static_cast<T*>(b.reserve(count * sizeof(T)));
 }
 
-void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t total_v) {
+void Engine::sort_values(SegVec& segs, const double* d_v, int64_t total_v) {
   int nseg = (int)segs.size();
   double* t1 = as<double>(t1_, std::max<int64_t>(total_v, 1));
   double* t2 = as<double>(t2_, std::max<int64_t>(total_v, 1));
@@ -2280,8 +2280,8 @@ void Engine::sort_values(std::vector<SegDesc>& segs, const double* d_v, int64_t 
   }
 }
 
-void Engine::compress(int scale, std::vector<SegDesc>& segs, const double* d_v,
-                      const double* d_em, const double* d_ew, std::vector<SegResult>& res,
+void Engine::compress(int scale, SegVec& segs, const double* d_v,
+                      const double* d_em, const double* d_ew, ResVec& res,
                       bool sync_results) {
   check_hip(hipSetDevice(device_), "hipSetDevice");
   int nseg = (int)segs.size();
@@ -2739,10 +2739,14 @@ __global__ __launch_bounds__(1024) void k_rx_tiles(const uint32_t* seg_start, in
   if (s == 0) tile_start[nseg] = total;
 }
 
+// With vals != nullptr (first pass) it also validates the batch: a key at or
+// above key_limit or a non-finite value sets *bad (grouped.mojo:91-97).
 __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const uint32_t* keys,
                                                         const uint32_t* seg_start,
                                                         const uint32_t* tile_start, int nseg,
-                                                        int shift, int bits, uint32_t* mat) {
+                                                        int shift, int bits, uint32_t* mat,
+                                                        const double* vals, uint32_t key_limit,
+                                                        int* bad) {
   __shared__ uint32_t lc[kRxBins];
   const uint32_t t = blockIdx.x;
   if (t >= tile_start[nseg]) return;
@@ -2754,7 +2758,16 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const uint32_t* keys,
   const uint32_t mask = (uint32_t)R - 1u;
   for (int d = threadIdx.x; d < R; d += blockDim.x) lc[d] = 0;
   __syncthreads();
-  for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) atomicAdd(&lc[(keys[i] >> shift) & mask], 1u);
+  bool wrong = false;
+  for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+    uint32_t k = keys[i];
+    atomicAdd(&lc[(k >> shift) & mask], 1u);
+    if (vals) {
+      double v = vals[i];
+      wrong |= k >= key_limit || !(v - v == 0.0);
+    }
+  }
+  if (wrong) *bad = 1;
   __syncthreads();
   const size_t base = (size_t)R * tile_start[s] + lt;
   for (int d = threadIdx.x; d < R; d += blockDim.x) mat[base + (size_t)d * nts] = lc[d];
@@ -2939,7 +2952,7 @@ __global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, 
 
 void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n,
                                  int bits, int64_t num_keys, int64_t* d_counts,
-                                 int64_t* d_offsets, double* d_out) {
+                                 int64_t* d_offsets, double* d_out, int* d_bad) {
   const int hb = bits > kRxBits ? bits - kRxBits : 0;  // high digit bits (pass 1)
   const int lb = bits > kRxBits ? kRxBits : bits;      // low digit bits (last pass)
   const int64_t max_tiles1 = (n + kRxTile - 1) / kRxTile;
@@ -2955,14 +2968,16 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   launched("k_rx_init");
 
   auto pass = [&](const uint32_t* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
-                  int dbits, int64_t tiles_max, uint32_t* kout, double* vout, uint32_t* seg_out) {
+                  int dbits, int64_t tiles_max, uint32_t* kout, double* vout, uint32_t* seg_out,
+                  bool validate) {
     const int R = 1 << dbits;
     k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, tiles);
     launched("k_rx_tiles");
     {
       int ph_ = phase_begin("key_hist", 4.0 * (double)n);
-      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(kin, seg, tiles, nseg, shift,
-                                                                 dbits, mat);
+      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
+          kin, seg, tiles, nseg, shift, dbits, mat, validate ? vin : nullptr,
+          (uint32_t)std::min<int64_t>(num_keys, 0xffffffffLL), d_bad);
       launched("k_rx_hist");
       phase_end(ph_);
     }
@@ -2992,10 +3007,10 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   if (hb > 0) {
     uint32_t* k1 = as<uint32_t>(rx_keys_, n);
     double* v1 = as<double>(rx_vals_, n);
-    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB);
-    pass(k1, v1, segB, 1 << hb, 0, lb, max_tiles2, nullptr, d_out, segA);
+    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, d_bad != nullptr);
+    pass(k1, v1, segB, 1 << hb, 0, lb, max_tiles2, nullptr, d_out, segA, false);
   } else {
-    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB);
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB, d_bad != nullptr);
     std::swap(segA, segB);
   }
   k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(segA, num_keys, d_offsets,
@@ -3003,13 +3018,22 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   launched("k_rx_final");
 }
 
-void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
-                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets,
-                           double* d_out) {
+static int key_bits(int64_t num_keys) {
   int bits = 0;
   while (bits < 31 && ((int64_t)1 << bits) < num_keys) bits++;
-  if (n > 0 && n <= ((int64_t)1 << 31) && bits <= 2 * kRxBits) {
-    key_partition_radix(d_keys, d_vals, n, std::max(bits, 1), num_keys, d_counts, d_offsets, d_out);
+  return std::max(bits, 1);
+}
+
+bool Engine::key_partition_validates(int64_t n, int64_t num_keys) const {
+  return n > 0 && n <= ((int64_t)1 << 31) && key_bits(num_keys) <= 2 * kRxBits;
+}
+
+void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,
+                           int64_t num_keys, int64_t* d_counts, int64_t* d_offsets,
+                           double* d_out, int* d_bad) {
+  if (key_partition_validates(n, num_keys)) {
+    key_partition_radix(d_keys, d_vals, n, key_bits(num_keys), num_keys, d_counts, d_offsets, d_out,
+                        d_bad);
     check_hip(hipGetLastError(), "key partition launch");
     return;
   }

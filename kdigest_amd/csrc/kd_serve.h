@@ -81,6 +81,8 @@ class Server {
   // Host staging of the asynchronous state and install copies.
   std::vector<int64_t> h_keys_, h_meta_;
   std::vector<double> h_state_;
+  SegVec segv_;
+  ResVec resv_;
 };
 
 }  // namespace kd

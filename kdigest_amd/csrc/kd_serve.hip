@@ -291,7 +291,8 @@ int64_t Server::step(int max_rounds) {
     rounds_++;
     std::sort(round.begin(), round.end(),
               [&](int64_t a, int64_t b) { return reqs_.at(a).key < reqs_.at(b).key; });
-    std::vector<SegDesc> segs;
+    SegVec& segs = segv_;
+    segs.clear();
     std::vector<double> vals;
     std::vector<int64_t> keys;
     std::vector<char> had;
@@ -341,7 +342,7 @@ int64_t Server::step(int max_rounds) {
     hipStream_t st = eng_.stream();
     double* dv = static_cast<double*>(vbuf_.reserve(std::max<size_t>(vals.size(), 1) * sizeof(double)));
     check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
-    std::vector<SegResult> res;
+    ResVec& res = resv_;
     eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
     // A merge that would not fit its key's slot fails that request alone:
     // the key keeps its state and nothing is installed for it.

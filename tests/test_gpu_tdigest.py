@@ -227,3 +227,28 @@ def test_grouped_key_partition_sizes(kd, oracle, nk):
         m, w = g.centroids(k)
         rm, rw = r.centroids()
         check_centroids(m, w, rm, rw, 300.0, True, f"nk {nk} key {k}")
+
+
+@pytest.mark.parametrize("nk", [100, 5000])
+def test_grouped_rejects_bad_batches_without_change(kd, nk):
+    # grouped.mojo:91-97: the whole batch is validated before any key changes.
+    rng = np.random.default_rng(nk)
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK2)
+    keys = rng.integers(0, nk, 50000).astype(np.uint32)
+    vals = rng.standard_normal(50000)
+    g.add_batch(keys, vals)
+    before = [g.centroids(k) for k in range(0, nk, 7)]
+    bad_key = keys.copy()
+    bad_key[12345] = nk
+    bad_val = vals.copy()
+    bad_val[777] = np.inf
+    nan_val = vals.copy()
+    nan_val[-1] = np.nan
+    for k_, v_ in ((bad_key, vals), (keys, bad_val), (keys, nan_val)):
+        with pytest.raises(kd.KdigestError):
+            g.add_batch(k_, v_)
+    after = [g.centroids(k) for k in range(0, nk, 7)]
+    for (m0, w0), (m1, w1) in zip(before, after):
+        assert np.array_equal(m0, m1) and np.array_equal(w0, w1)
+    g.add_batch(keys, vals)  # still usable
+    assert g.total_weight(int(keys[0])) > 0
