@@ -713,11 +713,54 @@ __device__ inline int64_t chain_next(int scale, PCur& pc, int64_t s, int64_t n, 
   return hi;
 }
 
+// End of the cluster started at s whose first closing candidate b < n has a
+// value within 2 ulps of the record before it: the tie rule
+// (tdigest.mojo:429, candidate.mean == current.mean absorbs) may extend it.
+__device__ inline int64_t tie_extend(const double* x, int64_t n, int64_t s, int64_t b,
+                                     const PCur& pc, const double* esw_seg, int64_t& replays) {
+  const double xb = x[b];
+  int64_t e = b;
+  if (x[s] == xb) {
+    // Every record in [s, b] equals xb: the running mean is exactly xb and
+    // stays so; absorb the whole run of xb.
+    int64_t l2 = b, h2 = n;
+    while (l2 < h2) {
+      int64_t mid = (l2 + h2) >> 1;
+      if (x[mid] == xb) l2 = mid + 1;
+      else h2 = mid;
+    }
+    e = l2;
+  } else {
+    // Replay the incremental mean of centroid.mojo:34-36 exactly.
+    replays++;
+    int64_t ce = 0;
+    while (ce < pc.ne && pc.pos[ce] < s) ce++;
+    double mean = 0.0, wt = 0.0;
+    for (int64_t j = s; j < b; j++) {
+      double w = pc.sf;
+      if (ce < pc.ne && pc.pos[ce] == j) {
+        w = esw_seg[ce];
+        ce++;
+      }
+      if (j == s) {
+        mean = x[j];
+        wt = w;
+      } else {
+        double nw = wt + w;
+        mean += (x[j] - mean) * (w / nw);
+        wt = nw;
+      }
+    }
+    while (e < n && x[e] == mean) e++;
+  }
+  return e;
+}
+
 // The greedy sweep of tdigest.mojo:414-435 over one segment's merged records
-// x[0..n) (sorted; any address space).  Writes each cluster's first record
-// index to cs[c], calls done(a, b, c) for every cluster [a, b) but the last
-// single record, and returns the cluster count.
-// next(s) gives the first candidate closing the cluster started at s (or n).
+// x[0..n) (sorted; any address space).  next(s) gives the first candidate
+// closing the cluster started at s (or n).  Writes each cluster's first
+// record index to cs[c], calls done(a, b, c) for every cluster [a, b) but the
+// last single record, and returns the cluster count.
 template <class Next, class Done>
 __device__ inline int64_t sweep_segment(const double* x, int64_t n, PCur pc,
                                         const double* esw_seg, int64_t* cs, int64_t& replays,
@@ -729,45 +772,7 @@ __device__ inline int64_t sweep_segment(const double* x, int64_t n, PCur pc,
     if (s == n - 1) break;
     int64_t b = next(s);  // first closing candidate, or n
     int64_t e = b;
-    if (b < n) {
-      // Tie rule (tdigest.mojo:429): candidate.mean == current.mean absorbs.
-      double xb = x[b], xp = x[b - 1];
-      if (ord_key(xb) - ord_key(xp) <= 2) {
-        if (x[s] == xb) {
-          // Every record in [s, b] equals xb: the running mean is exactly xb
-          // and stays so; absorb the whole run of xb.
-          int64_t l2 = b, h2 = n;
-          while (l2 < h2) {
-            int64_t mid = (l2 + h2) >> 1;
-            if (x[mid] == xb) l2 = mid + 1;
-            else h2 = mid;
-          }
-          e = l2;
-        } else {
-          // Replay the incremental mean of centroid.mojo:34-36 exactly.
-          replays++;
-          int64_t ce = 0;
-          while (ce < pc.ne && pc.pos[ce] < s) ce++;
-          double mean = 0.0, wt = 0.0;
-          for (int64_t j = s; j < b; j++) {
-            double w = pc.sf;
-            if (ce < pc.ne && pc.pos[ce] == j) {
-              w = esw_seg[ce];
-              ce++;
-            }
-            if (j == s) {
-              mean = x[j];
-              wt = w;
-            } else {
-              double nw = wt + w;
-              mean += (x[j] - mean) * (w / nw);
-              wt = nw;
-            }
-          }
-          while (e < n && x[e] == mean) e++;
-        }
-      }
-    }
+    if (b < n && ord_key(x[b]) - ord_key(x[b - 1]) <= 2) e = tie_extend(x, n, s, b, pc, esw_seg, replays);
     done(s, e, c - 1);
     s = e;
   }
@@ -1069,19 +1074,24 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
         if (!(kv[mid + 1] - ks <= 1.0)) hi = mid;
         else lo = mid + 1;
       }
-      nxt[st] = (uint16_t)(lo == n ? 0xffff : lo);
+      // Bit 15 flags a candidate within 2 ulps of its predecessor, where the
+      // tie rule may extend the cluster (n <= 4096 fits in the low bits).
+      bool tie = lo < n && ord_key(x[lo]) - ord_key(x[lo - 1]) <= 2;
+      nxt[st] = (uint16_t)(lo | (tie ? 0x8000 : 0));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       int64_t* cs = reinterpret_cast<int64_t*>(s);  // kv is no longer needed
       int64_t replays = 0;
-      int64_t C = n > 0 ? sweep_segment(x, n, PCur{pos, sw, ne, sd.sf, 0}, ew, cs, replays,
-                                        [&](int64_t st) -> int64_t {
-                                          uint16_t v = nxt[st];
-                                          return v == 0xffff ? (int64_t)n : (int64_t)v;
-                                        },
-                                        [](int64_t, int64_t, int64_t) {})
-                        : 0;
+      const PCur pc{pos, sw, ne, sd.sf, 0};
+      int64_t C = 0, st = 0;
+      while (st < n) {  // the sweep of sweep_segment, walking the table
+        cs[C++] = st;
+        if (st == n - 1) break;
+        const uint16_t v = nxt[st];
+        const int64_t b = v & 0x7fff;
+        st = (v & 0x8000) ? tie_extend(x, n, st, b, pc, ew, replays) : b;
+      }
       res[si].replays = replays;
       ccount[si] = C;
       s_C = C;
