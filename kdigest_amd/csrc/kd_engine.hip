@@ -1462,16 +1462,15 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   }
 }
 
-// Per segment: bucket ranks, prior-centroid positions, the chain in rank
-// space, the buckets to gather, each other bucket's cluster, and each
-// cluster's slice exponent.
-__global__ __launch_bounds__(kSelThreads) void k_sel_plan(
-    int scale, const SegDesc* segs, const SelSeg* sel, const double* spl, const unsigned int* cnt,
-    unsigned int* bst, int* binfo, const double* esm, const double* se, int64_t* pose,
-    SelState* st, SelBound* bnd, int* ce, double* csl, SelMark* mk, int2* mlist,
-    unsigned int* mlist_cnt) {
+// Per segment: bucket ranks and the merged positions of the prior centroids.
+__global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, const SelSeg* sel,
+                                                           const double* spl,
+                                                           const unsigned int* cnt,
+                                                           unsigned int* bst, int* binfo,
+                                                           const double* esm, int64_t* pose,
+                                                           SelState* st) {
   __shared__ uint32_t part[kSelThreads];
-  __shared__ int s_fail, s_C, s_base;
+  __shared__ int s_fail;
   const int j = blockIdx.x;
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
@@ -1479,16 +1478,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   const int S = st[j].nspl;
   const int64_t nv = sd.v_cnt;
   const int ne = (int)sd.e_cnt;
-  const int64_t n = nv + ne;
   const double* sp = spl + g.b_off;
   const unsigned int* c = cnt + g.b_off;
   unsigned int* B = bst + g.b_off;
   int* info = binfo + g.b_off;
-  SelBound* bd = bnd + g.c_off;
-  if (tid == 0) {
-    s_fail = 0;
-    s_C = 0;
-  }
+  if (tid == 0) s_fail = 0;
   // 1. Exclusive scan of the S + 1 bucket counts; buckets start unmarked.
   const int nb = S + 1;
   const int per = (nb + kSelThreads - 1) / kSelThreads;
@@ -1512,7 +1506,6 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   }
   __syncthreads();
   // 2. Merged positions of the prior centroids: E precede equal new values.
-  const int64_t* pos = pose + sd.es_off;
   for (int i = tid; i < ne; i += blockDim.x) {
     double e = esm[sd.es_off + i];
     int l = 0, h = S;  // first splitter >= e (e itself is a splitter)
@@ -1525,29 +1518,62 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     else pose[sd.es_off + i] = i + (int64_t)B[l + 1];
   }
   __syncthreads();
-  // 3. Chain in rank space (one lane).
-  if (tid == 0 && !s_fail) {
-    PCur pc{pos, se + sd.es_off + g.si, sd.e_cnt, sd.sf, 0};
-    int C = 0;
-    int64_t s = 0;
-    while (s < n) {
-      if (C >= g.cap_c) {
-        s_fail = 1;
-        break;
-      }
-      bd[C].b = s;
-      C++;
-      if (s == n - 1) break;
-      s = chain_next(scale, pc, s, n, sd.stored_n, sd.nrm);
+  if (tid == 0 && s_fail) st[j].fail = 1;
+}
+
+// The greedy sweep in rank space (tdigest.mojo:414-435), one lane per
+// segment so that the serial walks of many segments overlap.
+__global__ void k_sel_chain(int scale, const SegDesc* segs, const SelSeg* sel, int nsel,
+                            const double* se, const int64_t* pose, SelState* st, SelBound* bnd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsel || st[j].fail) return;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int64_t n = sd.v_cnt + sd.e_cnt;
+  SelBound* bd = bnd + g.c_off;
+  PCur pc{pose + sd.es_off, se + sd.es_off + g.si, sd.e_cnt, sd.sf, 0};
+  int C = 0;
+  int64_t s = 0;
+  while (s < n) {
+    if (C >= g.cap_c) {
+      st[j].fail = 1;
+      return;
     }
-    s_C = C;
+    bd[C].b = s;
+    C++;
+    if (s == n - 1) break;
+    s = chain_next(scale, pc, s, n, sd.stored_n, sd.nrm);
   }
-  __syncthreads();
-  if (s_fail) {
-    if (tid == 0) st[j].fail = 1;
-    return;
-  }
-  const int C = s_C;
+  st[j].ncl = C;
+}
+
+// Per segment, after the chain: the boundary records, the buckets to gather,
+// each other bucket's cluster, and each cluster's slice exponent.
+__global__ __launch_bounds__(kSelThreads) void k_sel_plan(
+    const SegDesc* segs, const SelSeg* sel, const double* spl, const unsigned int* cnt,
+    const unsigned int* bst, int* binfo, const int64_t* pose, SelState* st, SelBound* bnd, int* ce,
+    double* csl, SelMark* mk, int2* mlist, unsigned int* mlist_cnt) {
+  __shared__ uint32_t part[kSelThreads];
+  __shared__ int s_fail, s_base;
+  const int j = blockIdx.x;
+  if (st[j].fail) return;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int tid = threadIdx.x;
+  const int S = st[j].nspl;
+  const int64_t nv = sd.v_cnt;
+  const int ne = (int)sd.e_cnt;
+  const double* sp = spl + g.b_off;
+  const unsigned int* c = cnt + g.b_off;
+  const unsigned int* B = bst + g.b_off;
+  int* info = binfo + g.b_off;
+  SelBound* bd = bnd + g.c_off;
+  const int64_t* pos = pose + sd.es_off;
+  const int nb = S + 1;
+  const int per = (nb + kSelThreads - 1) / kSelThreads;
+  const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
+  if (tid == 0) s_fail = 0;
+  const int C = st[j].ncl;
   // 4. Boundary records and the buckets to gather.
   for (int i = tid; i < C; i += blockDim.x) {
     int64_t b = bd[i].b;
@@ -2444,8 +2470,13 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     }
     {
       int ph_ = phase_begin("sel_plan", 0.0);
-      k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(scale, dsegs, dsel, dspl, dcnt, dbst, dinfo, esm,
-                                                    se, pose, dst, dbnd, dce, dcsl, dmk, dml, dmlc);
+      k_sel_ranks<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, esm, pose,
+                                                     dst);
+      launched("k_sel_ranks");
+      k_sel_chain<<<(nsel + 63) / 64, 64, 0, stream_>>>(scale, dsegs, dsel, nsel, se, pose, dst, dbnd);
+      launched("k_sel_chain");
+      k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, pose, dst,
+                                                    dbnd, dce, dcsl, dmk, dml, dmlc);
       launched("k_sel_plan");
       phase_end(ph_);
     }
