@@ -61,6 +61,19 @@ def gen_grouped(n: int, num_keys: int, seed: int, dev):
     return keys, vals
 
 
+def traffic_bytes(phase: str):
+    """HBM bytes per launch of a kernel group, from the committed rocprof
+    FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(phase)
+    return None if e is None else e.get("bytes_per_launch")
+
+
 def cpu_baseline_grouped(keys_h: np.ndarray, vals_h: np.ndarray, num_keys: int):
     """The CPU oracle (restatement of the reference MergingDigest) on a
     bounded sample of the grouped workload, one thread."""
@@ -266,6 +279,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -282,6 +297,12 @@ def main():
     n, nk = args.n, args.keys
     print(f"[bench] generating {n} grouped pairs over {nk} keys", file=sys.stderr, flush=True)
     keys, vals = gen_grouped(n, nk, 0, dev)
+    if args.copy_probe:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)  # 1 GiB read + 1 GiB written by one blit kernel
+        torch.cuda.synchronize()
+        del a, b
 
     # ------------------------------------------------ grouped ingest (value)
     for w in range(args.warmup):
@@ -427,8 +448,9 @@ def main():
                                "0.9*N(mu_k,1)+0.1*(mu_k+Pareto(1.5)), ScaleK3, compression 100",
                    "keys": nk, "pairs": n},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name,
-                     "kernel_avg_ms": dom_avg_ms},
+                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic_bytes(dom_name),
+                     "kernel": dom_name, "kernel_avg_ms": dom_avg_ms,
+                     "algorithmic_bytes_per_launch": dom_bytes / max(dom_launches, 1)},
         "pipeline_roofline": {"bytes_per_sample": GROUPED_BYTES_PER_SAMPLE, "achieved": pipe_gbs,
                               "frac": pipe_gbs / HBM_PEAK_GBS},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
