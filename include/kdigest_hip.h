@@ -64,6 +64,12 @@ int kd_session_set_select(kd_session *s, int enable);
  * the multi-kernel sort path.  Same clusters either way.  No counterpart in
  * the reference. */
 int kd_session_set_fused(kd_session *s, int enable);
+/* Launch shape of the grouped key partition (radix by key): items per thread
+ * of a scatter sub-tile (4, 8 or 16) and the target number of chunks per
+ * pass (1..2^20; one workgroup per chunk).  Results do not depend on it.  No
+ * counterpart in the reference (it groups keys on the host,
+ * kdigest/gpu/grouped.mojo:454-538). */
+int kd_session_set_partition_shape(kd_session *s, int items, int64_t chunks);
 /* Digests compressed so far by selection (out3[0]), by the multi-kernel sort
  * path (out3[1]) and in one block each (out3[2]). */
 int kd_session_path_counts(kd_session *s, int64_t *out3);

@@ -32,6 +32,7 @@ SIGNATURES = {
     "kd_session_profile": (_i, [_vp, _i]),
     "kd_session_set_select": (_i, [_vp, _i]),
     "kd_session_set_fused": (_i, [_vp, _i]),
+    "kd_session_set_partition_shape": (_i, [_vp, _i, _i64]),
     "kd_session_path_counts": (_i, [_vp, _i64p]),
     "kd_session_phase_stats": (_i64, [_vp, C.c_char_p, _i64]),
     "kd_compress": (_i, [_vp, _i, _d, _d, _dp, _dp, _d, _i64, _i64, _dp, _dp, _i64p]),

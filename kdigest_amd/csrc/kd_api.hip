@@ -286,6 +286,15 @@ extern "C" int kd_session_set_fused(kd_session* s, int enable) {
   });
 }
 
+extern "C" int kd_session_set_partition_shape(kd_session* s, int items, int64_t chunks) {
+  return guarded([&] {
+    if (items != 4 && items != 8 && items != 16) throw KdError(KD_EINVAL, "items must be 4, 8 or 16");
+    if (chunks < 1 || chunks > (1 << 20)) throw KdError(KD_EINVAL, "chunks must be in [1, 2^20]");
+    s->eng.set_partition_shape(items, chunks);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_session_path_counts(kd_session* s, int64_t* out3) {
   for (int i = 0; i < 3; i++) out3[i] = s->eng.path_counts()[i];
   return KD_OK;

@@ -141,6 +141,14 @@ class Engine {
                      int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,
                      int* d_bad = nullptr);
   bool key_partition_validates(int64_t n, int64_t num_keys) const;
+  // Launch shape of the radix key partition: items per thread of a scatter
+  // sub-tile (4, 8 or 16) and the target number of chunks per pass.
+  void set_partition_shape(int items, int64_t chunks) {
+    rx_items_ = items;
+    rx_chunks_ = chunks;
+  }
+  int partition_items() const { return rx_items_; }
+  int64_t partition_chunks() const { return rx_chunks_; }
 
   DevBuf& scratch(int i) { return extra_[i]; }
 
@@ -199,6 +207,8 @@ class Engine {
   // Radix key partition: pass-1 output, tile count matrix, scan partials,
   // segment and tile offsets.
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
+  int rx_items_ = 8;       // items per thread in a scatter sub-tile (4, 8 or 16)
+  int64_t rx_chunks_ = 4096;  // target number of chunks per pass
   void key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n, int bits,
                            int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,
                            int* d_bad);

@@ -2736,23 +2736,27 @@ __global__ void k_excl_scan_u64(const unsigned long long* cnt, int64_t n, int64_
 // ----------------------------------------------- key partition (radix)
 // Most-significant-digit-first counting sort by key: pass 1 splits the batch
 // by the high digit, pass 2 splits every high-digit segment by the low digit,
-// so each key's values end contiguous and in key order.  A pass is: digit
-// counts per tile (k_rx_hist), one exclusive scan of the (segment, digit,
-// tile) count matrix, which yields every tile's destination for every digit,
-// and a scatter staged through LDS in digit order so that consecutive lanes
-// store to consecutive addresses (k_rx_scatter).  Order within a key is not
-// kept (items are ranked with LDS atomics): all new values of a key carry
-// the same weight and are sorted next, so that order cannot matter.
-constexpr int kRxThreads = 512;
-constexpr int kRxItems = 8;
-constexpr int kRxTile = kRxThreads * kRxItems;  // 4096 items per tile
+// so each key's values end contiguous and in key order.  A pass works on
+// chunks (a few tens of thousands of items, one workgroup each): digit counts
+// per chunk (k_rx_hist), one exclusive scan of the (segment, digit, chunk)
+// count matrix, which yields every chunk's destination for every digit, and a
+// scatter (k_rx_scatter) that walks its chunk in sub-tiles staged through LDS
+// in digit order, so that consecutive lanes store to consecutive addresses,
+// while per-digit cursors in LDS carry the destinations from one sub-tile to
+// the next.  The ranking is stable, so every key's values keep their batch
+// order.  The matrix holds one entry per (digit, chunk): with chunks of 64K
+// items it is a few MB for 2^28 pairs, and each chunk reads its destinations
+// once.
+constexpr int kRxThreads = 256;
+constexpr int kRxWaves = kRxThreads / 64;
 constexpr int kRxBits = 9;
 constexpr int kRxBins = 1 << kRxBits;
+constexpr int kRxHistUnroll = 8;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = kScanThreads * kScanItems;
 
-// The segment owning tile t: the last s with tile_start[s] <= t.
+// The segment owning chunk t: the last s with tile_start[s] <= t.
 __device__ inline int rx_seg_of_tile(const uint32_t* tile_start, int nseg, uint32_t t) {
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
@@ -2768,50 +2772,89 @@ __global__ void k_rx_init(uint32_t* seg, uint32_t n) {
   seg[1] = n;
 }
 
-// tile_start[s] = sum over earlier segments of ceil(len / kRxTile); nseg <= 1024.
+// tile_start[s] = sum over earlier segments of ceil(len / chunk); nseg <= 1024.
 __global__ __launch_bounds__(1024) void k_rx_tiles(const uint32_t* seg_start, int nseg,
-                                                   uint32_t* tile_start) {
+                                                   uint32_t chunk, uint32_t* tile_start) {
   __shared__ uint32_t v[1024];
   int s = threadIdx.x;
-  v[s] = s < nseg ? (seg_start[s + 1] - seg_start[s] + kRxTile - 1) / kRxTile : 0u;
+  v[s] = s < nseg ? (seg_start[s + 1] - seg_start[s] + chunk - 1) / chunk : 0u;
   __syncthreads();
   uint32_t total = block_excl_scan(v, nseg);
   if (s < nseg) tile_start[s] = v[s];
   if (s == 0) tile_start[nseg] = total;
 }
 
-// With vals != nullptr (first pass) it also validates the batch: a key at or
-// above key_limit or a non-finite value sets *bad (grouped.mojo:91-97).
+// Exclusive scan of cnt[0..R) into out (R <= 4 * kRxThreads); all threads call it.
+__device__ inline void rx_scan_bins(const uint32_t* cnt, uint32_t* out, int R) {
+  __shared__ uint32_t wsum[kRxWaves];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int per = (R + kRxThreads - 1) / kRxThreads;
+  const int lo = tid * per;
+  uint32_t x[4];
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    x[j] = (j < per && lo + j < R) ? cnt[lo + j] : 0u;
+    s += x[j];
+  }
+  uint32_t inc = s;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int v = 0; v < w; v++) base += wsum[v];
+  uint32_t run = base + inc - s;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (j < per && lo + j < R) out[lo + j] = run;
+    run += x[j];
+  }
+}
+
+// Digit counts of one chunk, one histogram per wave (so a hot key's digit
+// contends only within its wave).
 __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const uint32_t* keys,
                                                         const uint32_t* seg_start,
                                                         const uint32_t* tile_start, int nseg,
-                                                        int shift, int bits, uint32_t* mat,
-                                                        const double* vals, uint32_t key_limit,
-                                                        int* bad) {
-  __shared__ uint32_t lc[kRxBins];
+                                                        uint32_t chunk, int shift, int bits,
+                                                        uint32_t* mat) {
+  __shared__ uint32_t lc[kRxWaves][kRxBins];
   const uint32_t t = blockIdx.x;
   if (t >= tile_start[nseg]) return;
   const int s = rx_seg_of_tile(tile_start, nseg, t);
   const uint32_t lt = t - tile_start[s], nts = tile_start[s + 1] - tile_start[s];
-  const uint32_t a = seg_start[s] + lt * kRxTile;
-  const uint32_t b = min(seg_start[s + 1], a + kRxTile);
+  const uint32_t a = seg_start[s] + lt * chunk;
+  const uint32_t b = min(seg_start[s + 1], a + chunk);
   const int R = 1 << bits;
   const uint32_t mask = (uint32_t)R - 1u;
-  for (int d = threadIdx.x; d < R; d += blockDim.x) lc[d] = 0;
+  const int w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kRxWaves * kRxBins; i += kRxThreads) (&lc[0][0])[i] = 0;
   __syncthreads();
-  bool wrong = false;
-  for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-    uint32_t k = keys[i];
-    atomicAdd(&lc[(k >> shift) & mask], 1u);
-    if (vals) {
-      double v = vals[i];
-      wrong |= k >= key_limit || !(v - v == 0.0);
+  constexpr uint32_t kStep = kRxThreads * kRxHistUnroll;
+  for (uint32_t i0 = a; i0 < b; i0 += kStep) {
+    uint32_t k[kRxHistUnroll];
+#pragma unroll
+    for (int r = 0; r < kRxHistUnroll; r++) {
+      uint32_t i = i0 + (uint32_t)(r * kRxThreads) + threadIdx.x;
+      k[r] = i < b ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kRxHistUnroll; r++) {
+      uint32_t i = i0 + (uint32_t)(r * kRxThreads) + threadIdx.x;
+      if (i < b) atomicAdd(&lc[w][(k[r] >> shift) & mask], 1u);
     }
   }
-  if (wrong) *bad = 1;
   __syncthreads();
   const size_t base = (size_t)R * tile_start[s] + lt;
-  for (int d = threadIdx.x; d < R; d += blockDim.x) mat[base + (size_t)d * nts] = lc[d];
+  for (int d = threadIdx.x; d < R; d += kRxThreads) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int v = 0; v < kRxWaves; v++) c += lc[v][d];
+    mat[base + (size_t)d * nts] = c;
+  }
 }
 
 // Block sums of the matrix (its live length is R * tile_start[nseg]).
@@ -2878,94 +2921,138 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const u
   }
 }
 
-// Stable within the tile (so every key's values keep their batch order): wave
-// w owns the contiguous items [w * 512, (w + 1) * 512) of the tile, walked in
-// eight rounds of 64; lanes with equal digits are found with ballots, ranked
-// by lane, and counted per wave; one scan over (wave, digit) then orders the
-// waves.
-template <bool kWriteKeys>
+// One chunk, walked in sub-tiles of kRxThreads * ITEMS items.  Stable: wave w
+// owns the contiguous items [w * ITEMS * 64, (w + 1) * ITEMS * 64) of a
+// sub-tile, walked in ITEMS rounds of 64; lanes with equal digits are found
+// with a 64-bit lane mask per (wave, digit) in LDS, ranked by lane and
+// counted per wave; per digit, the waves are
+// then ordered by index and the sub-tiles by position in the chunk.  The next
+// sub-tile's loads are issued before the current one is written out.  With
+// kValidate (first pass) a key at or above key_limit or a non-finite value
+// sets *bad (grouped.mojo:91-97); positions stay in bounds either way because
+// only the key's digit bits place it.
+template <int ITEMS, bool kWriteKeys, bool kValidate>
 __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys, const double* vals,
                                                            const uint32_t* seg_start,
                                                            const uint32_t* tile_start, int nseg,
-                                                           int shift, int bits, const uint32_t* dst,
-                                                           uint32_t* keys_out, double* vals_out) {
-  constexpr int kWaves = kRxThreads / 64;
-  constexpr int kPerWave = kRxTile / kWaves;  // 512 items, kRxItems rounds of 64
-  __shared__ uint32_t wc[kWaves][kRxBins];
+                                                           uint32_t chunk, int shift, int bits,
+                                                           const uint32_t* dst, uint32_t* keys_out,
+                                                           double* vals_out, uint32_t key_limit,
+                                                           int* bad) {
+  constexpr int kSub = kRxThreads * ITEMS;
+  constexpr int kPerWave = ITEMS * 64;
+  constexpr int kPm = kRxWaves * kRxBins;
+  __shared__ uint32_t wc[kRxWaves][kRxBins];
+  __shared__ uint32_t cnt[kRxBins];
   __shared__ uint32_t lc[kRxBins];
-  __shared__ uint32_t gd[kRxBins];
-  __shared__ double sv[kRxTile];
-  __shared__ uint32_t sk[kRxTile];
+  __shared__ uint32_t run[kRxBins];
+  // The lane masks of the ranking and the staged values of the write-out
+  // share this LDS (the masks are all zero again when the ranking ends and
+  // are cleared again after the write-out).
+  __shared__ unsigned long long stage[kSub > kPm ? kSub : kPm];
+  __shared__ uint32_t sk[kSub];
+  double* sv = reinterpret_cast<double*>(stage);
+  auto pm = [&](int wv, uint32_t d) -> unsigned long long& { return stage[wv * kRxBins + (int)d]; };
   const uint32_t t = blockIdx.x;
   if (t >= tile_start[nseg]) return;
   const int s = rx_seg_of_tile(tile_start, nseg, t);
   const uint32_t lt = t - tile_start[s], nts = tile_start[s + 1] - tile_start[s];
-  const uint32_t a = seg_start[s] + lt * kRxTile;
-  const uint32_t b = min(seg_start[s + 1], a + kRxTile);
+  const uint32_t a = seg_start[s] + lt * chunk;
+  const uint32_t b = min(seg_start[s + 1], a + chunk);
   const int R = 1 << bits;
   const uint32_t mask = (uint32_t)R - 1u;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < kWaves * kRxBins; i += blockDim.x) (&wc[0][0])[i] = 0;
-  __syncthreads();
-  uint32_t kr[kRxItems], rk[kRxItems];
-  double vr[kRxItems];
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int r = 0; r < kRxItems; r++) {
-    uint32_t i = a + (uint32_t)(w * kPerWave + r * 64 + lane);
-    bool valid = i < b;
-    uint32_t d = 0;
-    if (valid) {
-      kr[r] = keys[i];
-      vr[r] = vals[i];
-      d = (kr[r] >> shift) & mask;
-    }
-    uint64_t peers = __ballot(valid);
-    for (int bit = 0; bit < bits; bit++) {
-      uint64_t bb = __ballot((d >> bit) & 1u);
-      peers &= ((d >> bit) & 1u) ? bb : ~bb;
-    }
-    uint32_t before = (uint32_t)__popcll(peers & below);
-    uint32_t base = valid ? wc[w][d] : 0u;
-    if (valid) {
-      rk[r] = base + before;
-      if (before == 0) wc[w][d] = base + (uint32_t)__popcll(peers);
-    }
-  }
-  __syncthreads();
-  // Per digit: offsets of the waves, and the tile total.
-  for (int d = threadIdx.x; d < R; d += blockDim.x) {
-    uint32_t run = 0;
-    for (int v = 0; v < kWaves; v++) {
-      uint32_t c = wc[v][d];
-      wc[v][d] = run;
-      run += c;
-    }
-    lc[d] = run;
-  }
-  __syncthreads();
-  block_excl_scan(lc, R);
   const size_t gbase = (size_t)R * tile_start[s] + lt;
-  for (int d = threadIdx.x; d < R; d += blockDim.x) gd[d] = dst[gbase + (size_t)d * nts];
+  for (int d = threadIdx.x; d < R; d += kRxThreads) run[d] = dst[gbase + (size_t)d * nts];
+  for (int i = threadIdx.x; i < kPm; i += kRxThreads) {
+    (&wc[0][0])[i] = 0;
+    stage[i] = 0ull;
+  }
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t kr[ITEMS], rk[ITEMS];
+  double vr[ITEMS];
+  bool wrong = false;
+  auto load = [&](uint32_t sub) {
 #pragma unroll
-  for (int r = 0; r < kRxItems; r++) {
-    uint32_t i = a + (uint32_t)(w * kPerWave + r * 64 + lane);
-    if (i < b) {
-      uint32_t d = (kr[r] >> shift) & mask;
-      uint32_t pos = lc[d] + wc[w][d] + rk[r];
-      sv[pos] = vr[r];
-      sk[pos] = kr[r];
+    for (int r = 0; r < ITEMS; r++) {
+      uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
+      if (i < b) {
+        kr[r] = keys[i];
+        vr[r] = vals[i];
+      }
     }
-  }
+  };
+  load(a);
   __syncthreads();
-  const int len = (int)(b - a);
-  for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    uint32_t k = sk[i];
-    uint32_t d = (k >> shift) & mask;
-    uint32_t o = gd[d] + ((uint32_t)i - lc[d]);
-    vals_out[o] = sv[i];
-    if (kWriteKeys) keys_out[o] = k;
+  for (uint32_t sub = a; sub < b; sub += kSub) {
+    const uint32_t e = min(b, sub + (uint32_t)kSub);
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
+      bool valid = i < e;
+      uint32_t d = valid ? (kr[r] >> shift) & mask : 0u;
+      if (kValidate && valid) wrong |= kr[r] >= key_limit || !(vr[r] - vr[r] == 0.0);
+      // The lanes sharing this lane's digit: every valid lane ORs its bit
+      // into its digit's mask, then reads it back (one wave's LDS operations
+      // complete in issue order, so the read sees every lane's OR, and the
+      // leader's reset lands before the next round's ORs).
+      if (valid) atomicOr(&pm(w, d), 1ull << lane);
+      uint64_t peers = valid ? pm(w, d) : 0ull;
+      uint32_t before = (uint32_t)__popcll(peers & below);
+      uint32_t base = valid ? wc[w][d] : 0u;
+      if (valid) {
+        rk[r] = base + before;
+        if (before == 0) {
+          wc[w][d] = base + (uint32_t)__popcll(peers);
+          pm(w, d) = 0ull;
+        }
+      }
+    }
+    __syncthreads();
+    // Per digit: offsets of the waves, and the sub-tile total.
+    for (int d = threadIdx.x; d < R; d += kRxThreads) {
+      uint32_t acc = 0;
+#pragma unroll
+      for (int v = 0; v < kRxWaves; v++) {
+        uint32_t c = wc[v][d];
+        wc[v][d] = acc;
+        acc += c;
+      }
+      cnt[d] = acc;
+    }
+    __syncthreads();
+    rx_scan_bins(cnt, lc, R);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+      uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
+      if (i < e) {
+        uint32_t d = (kr[r] >> shift) & mask;
+        uint32_t pos = lc[d] + wc[w][d] + rk[r];
+        sv[pos] = vr[r];
+        sk[pos] = kr[r];
+      }
+    }
+    __syncthreads();
+    if (sub + (uint32_t)kSub < b) load(sub + (uint32_t)kSub);
+    const int len = (int)(e - sub);
+    for (int i = threadIdx.x; i < len; i += kRxThreads) {
+      uint32_t k = sk[i];
+      uint32_t d = (k >> shift) & mask;
+      uint32_t o = run[d] + ((uint32_t)i - lc[d]);
+      vals_out[o] = sv[i];
+      if (kWriteKeys) keys_out[o] = k;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < R; d += kRxThreads) {
+      run[d] += cnt[d];
+#pragma unroll
+      for (int v = 0; v < kRxWaves; v++) wc[v][d] = 0;
+    }
+    for (int i = threadIdx.x; i < kPm; i += kRxThreads) stage[i] = 0ull;
+    __syncthreads();
   }
+  if (kValidate && wrong) *bad = 1;
 }
 
 // Segments of the next pass: (segment s, digit d) starts where the scanned
@@ -2996,7 +3083,12 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
                                  int64_t* d_offsets, double* d_out, int* d_bad) {
   const int hb = bits > kRxBits ? bits - kRxBits : 0;  // high digit bits (pass 1)
   const int lb = bits > kRxBits ? kRxBits : bits;      // low digit bits (last pass)
-  const int64_t max_tiles1 = (n + kRxTile - 1) / kRxTile;
+  const int items = rx_items_ == 4 || rx_items_ == 16 ? rx_items_ : 8;
+  const int64_t sub = (int64_t)kRxThreads * items;
+  // Chunk: about n / rx_chunks_ items, a whole number of sub-tiles, at most 2^20.
+  int64_t chunk = (n + std::max<int64_t>(rx_chunks_, 1) - 1) / std::max<int64_t>(rx_chunks_, 1);
+  chunk = std::min<int64_t>(std::max<int64_t>((chunk + sub - 1) / sub * sub, sub), (int64_t)1 << 20);
+  const int64_t max_tiles1 = (n + chunk - 1) / chunk;
   const int64_t max_tiles2 = max_tiles1 + (1 << hb);
   const int64_t mat_max = (int64_t)kRxBins * max_tiles2;
   const int64_t nb_max = (mat_max + kScanChunk - 1) / kScanChunk;
@@ -3005,20 +3097,41 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   uint32_t* segA = as<uint32_t>(rx_seg1_, ((int64_t)1 << bits) + 2);
   uint32_t* segB = as<uint32_t>(rx_seg2_, ((int64_t)1 << bits) + 2);
   uint32_t* tiles = as<uint32_t>(rx_tile_, ((int64_t)1 << hb) + 2);
+  const uint32_t klim = (uint32_t)std::min<int64_t>(num_keys, 0xffffffffLL);
   k_rx_init<<<1, 1, 0, stream_>>>(segA, (uint32_t)n);
   launched("k_rx_init");
 
+  auto scatter = [&](bool first, bool last, unsigned grid, const uint32_t* kin, const double* vin,
+                     const uint32_t* seg, int nseg, int shift, int dbits, uint32_t* kout,
+                     double* vout) {
+    const uint32_t c = (uint32_t)chunk;
+#define KD_RX_LAUNCH(I, WK, VA)                                                                  \
+  k_rx_scatter<I, WK, VA><<<grid, kRxThreads, 0, stream_>>>(kin, vin, seg, tiles, nseg, c, shift, \
+                                                           dbits, mat, kout, vout, klim, d_bad)
+#define KD_RX_ITEMS(WK, VA)            \
+  if (items == 4) KD_RX_LAUNCH(4, WK, VA);       \
+  else if (items == 16) KD_RX_LAUNCH(16, WK, VA); \
+  else KD_RX_LAUNCH(8, WK, VA)
+    const bool validate = first && d_bad != nullptr;
+    if (!last && validate) { KD_RX_ITEMS(true, true); }
+    else if (!last) { KD_RX_ITEMS(true, false); }
+    else if (validate) { KD_RX_ITEMS(false, true); }
+    else { KD_RX_ITEMS(false, false); }
+#undef KD_RX_ITEMS
+#undef KD_RX_LAUNCH
+    launched("k_rx_scatter");
+  };
+
   auto pass = [&](const uint32_t* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
                   int dbits, int64_t tiles_max, uint32_t* kout, double* vout, uint32_t* seg_out,
-                  bool validate) {
+                  bool first) {
     const int R = 1 << dbits;
-    k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, tiles);
+    k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, (uint32_t)chunk, tiles);
     launched("k_rx_tiles");
     {
       int ph_ = phase_begin("key_hist", 4.0 * (double)n);
-      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
-          kin, seg, tiles, nseg, shift, dbits, mat, validate ? vin : nullptr,
-          (uint32_t)std::min<int64_t>(num_keys, 0xffffffffLL), d_bad);
+      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(kin, seg, tiles, nseg,
+                                                                 (uint32_t)chunk, shift, dbits, mat);
       launched("k_rx_hist");
       phase_end(ph_);
     }
@@ -3031,13 +3144,8 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     launched("k_scan_down");
     {
       int ph_ = phase_begin("key_scatter", (kout ? 24.0 : 20.0) * (double)n);
-      if (kout)
-        k_rx_scatter<true><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
-            kin, vin, seg, tiles, nseg, shift, dbits, mat, kout, vout);
-      else
-        k_rx_scatter<false><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
-            kin, vin, seg, tiles, nseg, shift, dbits, mat, nullptr, vout);
-      launched("k_rx_scatter");
+      scatter(first, kout == nullptr, (unsigned)tiles_max, kin, vin, seg, nseg, shift, dbits, kout,
+              vout);
       phase_end(ph_);
     }
     const int64_t m = (int64_t)nseg * R + 1;
@@ -3048,10 +3156,10 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   if (hb > 0) {
     uint32_t* k1 = as<uint32_t>(rx_keys_, n);
     double* v1 = as<double>(rx_vals_, n);
-    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, d_bad != nullptr);
+    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, true);
     pass(k1, v1, segB, 1 << hb, 0, lb, max_tiles2, nullptr, d_out, segA, false);
   } else {
-    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB, d_bad != nullptr);
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB, true);
     std::swap(segA, segB);
   }
   k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(segA, num_keys, d_offsets,

@@ -146,6 +146,11 @@ class GpuSession:
         (default) or through the multi-kernel sort path; same clusters."""
         check(load().kd_session_set_fused(self.handle, 1 if enable else 0))
 
+    def set_partition_shape(self, items: int = 8, chunks: int = 4096) -> None:
+        """Launch shape of the grouped key partition: items per thread of a
+        scatter sub-tile (4, 8, 16) and the target chunks per pass."""
+        check(load().kd_session_set_partition_shape(self.handle, int(items), int(chunks)))
+
     def path_counts(self) -> dict:
         o = np.zeros(3, np.int64)
         check(load().kd_session_path_counts(self.handle, _ptr(o, _i64p)))

@@ -252,3 +252,34 @@ def test_grouped_rejects_bad_batches_without_change(kd, nk):
         assert np.array_equal(m0, m1) and np.array_equal(w0, w1)
     g.add_batch(keys, vals)  # still usable
     assert g.total_weight(int(keys[0])) > 0
+
+
+@pytest.mark.parametrize("items,chunks", [(4, 1), (8, 3), (16, 7), (8, 4096), (4, 1 << 20)])
+def test_grouped_partition_shapes_identical(kd, oracle, items, chunks):
+    """Every launch shape of the key partition (sub-tile width, chunks per
+    pass: one chunk of many sub-tiles up to one sub-tile per chunk) groups
+    the batch identically: centroids are bit-identical to the default shape
+    and match the oracle."""
+    rng = np.random.default_rng(items * 131 + chunks)
+    nk = 3001
+    n = 250_003
+    keys = ((rng.zipf(1.2, n) - 1) % nk).astype(np.uint32)
+    vals = rng.standard_normal(n) * 3.0 + keys * 0.01
+    s = kd.GpuSession(0)
+    s.set_partition_shape(items, chunks)
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s)
+    g.add_batch(keys, vals)
+    base = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    base.add_batch(keys, vals)
+    cnt = np.bincount(keys, minlength=nk)
+    for k in list(range(0, nk, 97)) + [int(np.argmax(cnt))]:
+        m, w = g.centroids(k)
+        bm, bw = base.centroids(k)
+        assert np.array_equal(m, bm) and np.array_equal(w, bw), k
+        assert g.total_weight(k) == float(cnt[k])
+    k = int(np.argmax(cnt))
+    r = oracle.Digest(3, 100.0)
+    r.add_batch_b(vals[keys == k])
+    m, w = g.centroids(k)
+    rm, rw = r.centroids()
+    check_centroids(m, w, rm, rw, 300.0, True, f"key {k}")
