@@ -279,6 +279,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -305,6 +306,12 @@ def main():
         del a, b
 
     # ------------------------------------------------ grouped ingest (value)
+    tuned = None
+    if not args.no_autotune:
+        # Launch shape picked by the library's autotuner on this batch (not
+        # timed; every shape builds identical digests).
+        tuned = session.autotune_grouped(keys, vals, nk, scale=kd.ScaleK3, compression=100.0)
+        print(f"[bench] autotuned shape {tuned}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=session)
         g.add_batch_device(keys, vals)
@@ -455,6 +462,7 @@ def main():
                               "frac": pipe_gbs / HBM_PEAK_GBS},
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
         "compress_paths": paths,
+        "autotuned_shape": tuned,
         "cpu_baseline": cpu,
         "plain_per_key": plain,
         "single_digest": single,

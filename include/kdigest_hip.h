@@ -70,6 +70,33 @@ int kd_session_set_fused(kd_session *s, int enable);
  * counterpart in the reference (it groups keys on the host,
  * kdigest/gpu/grouped.mojo:454-538). */
 int kd_session_set_partition_shape(kd_session *s, int items, int64_t chunks);
+/* Tiles of the selection compress (digests receiving more than 4096 new
+ * values): each tile holds at least tile_min values (1024..2^24) and a digest
+ * is split into at most tiles_max tiles (1..4096).  Results do not depend on
+ * it.  No counterpart in the reference. */
+int kd_session_set_select_shape(kd_session *s, int64_t tile_min, int64_t tiles_max);
+/* Small digests (at most 4096 merged records) compressed on a second stream,
+ * overlapping the selection compress of large ones (enable != 0) or on the
+ * session stream after it (0, the default).  Same results either way.  No
+ * counterpart in the reference. */
+int kd_session_set_overlap(kd_session *s, int enable);
+/* Autotuner of the grouped ingest: times ingests of the device-resident batch
+ * (d_keys, d_values, n pairs, keys below num_keys) into fresh digests of the
+ * given scale and compression under candidate launch shapes (coordinate
+ * descent over partition items and chunks, selection tile_min and tiles_max,
+ * overlap; best of `reps` runs each) and leaves the fastest on the session.
+ * out_shape5 = {items, chunks, tile_min, tiles_max, overlap}; out_ms2 = {ms
+ * with the shape the session had, ms with the chosen one}.  Digests built
+ * with any shape are identical.  No counterpart in the reference (its block
+ * size and per-key path are fixed, kdigest/gpu/grouped.mojo:579-624). */
+int kd_session_autotune_grouped(kd_session *s, int scale, int64_t num_keys, double compression,
+                                const uint32_t *d_keys, const double *d_values, int64_t n,
+                                int reps, int64_t *out_shape5, double *out_ms2);
+/* The same with the batch in host memory (copied to the device once). */
+int kd_session_autotune_grouped_host(kd_session *s, int scale, int64_t num_keys,
+                                     double compression, const uint32_t *keys,
+                                     const double *values, int64_t n, int reps,
+                                     int64_t *out_shape5, double *out_ms2);
 /* Digests compressed so far by selection (out3[0]), by the multi-kernel sort
  * path (out3[1]) and in one block each (out3[2]). */
 int kd_session_path_counts(kd_session *s, int64_t *out3);
@@ -195,6 +222,11 @@ int kd_batch_flush(kd_batch *b);
 /* merge() of every digest, then level by level digest i*2^(l+1) absorbs
  * digest i*2^(l+1) + 2^l with merge_into; the result is digest 0. */
 int kd_batch_merge_tree(kd_batch *b);
+/* Whether kd_batch_merge_tree runs as a HIP graph (enable != 0, the default:
+ * the flush and every level are captured on the first call and the graph is
+ * replayed afterwards) or as eager launches.  Same results either way.  No
+ * counterpart in the reference. */
+int kd_batch_set_graph(kd_batch *b, int enable);
 /* {stored_total, log_scale, clock, min, max, buffered samples, error flag}. */
 int kd_batch_state(kd_batch *b, int64_t d, double *out7);
 int kd_batch_centroids(kd_batch *b, int64_t d, double *means, double *weights, int64_t cap,

@@ -33,6 +33,10 @@ SIGNATURES = {
     "kd_session_set_select": (_i, [_vp, _i]),
     "kd_session_set_fused": (_i, [_vp, _i]),
     "kd_session_set_partition_shape": (_i, [_vp, _i, _i64]),
+    "kd_session_set_select_shape": (_i, [_vp, _i64, _i64]),
+    "kd_session_set_overlap": (_i, [_vp, _i]),
+    "kd_session_autotune_grouped": (_i, [_vp, _i, _i64, _d, _vp, _vp, _i64, _i, _i64p, _dp]),
+    "kd_session_autotune_grouped_host": (_i, [_vp, _i, _i64, _d, _u32p, _dp, _i64, _i, _i64p, _dp]),
     "kd_session_path_counts": (_i, [_vp, _i64p]),
     "kd_session_phase_stats": (_i64, [_vp, C.c_char_p, _i64]),
     "kd_compress": (_i, [_vp, _i, _d, _d, _dp, _dp, _d, _i64, _i64, _dp, _dp, _i64p]),
@@ -77,6 +81,7 @@ SIGNATURES = {
     "kd_batch_add_at": (_i, [_vp, _i64p, _dp, _dp, _dp]),
     "kd_batch_flush": (_i, [_vp]),
     "kd_batch_merge_tree": (_i, [_vp]),
+    "kd_batch_set_graph": (_i, [_vp, _i]),
     "kd_batch_state": (_i, [_vp, _i64, _dp]),
     "kd_batch_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),
     "kd_batch_to_tdigest": (_i, [_vp, _i64, C.POINTER(_vp)]),

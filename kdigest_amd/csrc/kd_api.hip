@@ -295,6 +295,22 @@ extern "C" int kd_session_set_partition_shape(kd_session* s, int items, int64_t 
   });
 }
 
+extern "C" int kd_session_set_select_shape(kd_session* s, int64_t tile_min, int64_t tiles_max) {
+  return guarded([&] {
+    if (tile_min < 1024 || tile_min > (1 << 24)) throw KdError(KD_EINVAL, "tile_min must be in [1024, 2^24]");
+    if (tiles_max < 1 || tiles_max > 4096) throw KdError(KD_EINVAL, "tiles_max must be in [1, 4096]");
+    s->eng.set_select_shape(tile_min, tiles_max);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_session_set_overlap(kd_session* s, int enable) {
+  return guarded([&] {
+    s->eng.set_overlap(enable != 0);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_session_path_counts(kd_session* s, int64_t* out3) {
   for (int i = 0; i < 3; i++) out3[i] = s->eng.path_counts()[i];
   return KD_OK;
@@ -1077,6 +1093,128 @@ void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int
   hip_ok(hipGetLastError(), "grouped query launch");
 }
 
+// ------------------------------------------------------------- autotuner
+// Picks the launch shape of the grouped ingest for a representative batch by
+// timing it: the key partition (items per sub-tile, chunks per pass), the
+// selection tiling (tile_min, tiles_max) and whether the small-digest
+// compress overlaps the selection path on a second stream.  Coordinate
+// descent from the session's current shape: every value of one knob is timed
+// with the others fixed (best of `reps` ingests into fresh digests), the
+// fastest is kept, then the next knob.  Every shape gives the same digests,
+// so the choice only changes speed.
+namespace {
+struct GroupedShape {
+  int items;
+  int64_t chunks, tile_min, tiles_max;
+  int overlap;
+};
+
+GroupedShape current_shape(kd_session* s) {
+  GroupedShape c;
+  c.items = s->eng.partition_items();
+  c.chunks = s->eng.partition_chunks();
+  c.tile_min = s->eng.select_tile_min();
+  c.tiles_max = s->eng.select_tiles_max();
+  c.overlap = s->eng.overlap() ? 1 : 0;
+  return c;
+}
+
+void apply_shape(kd_session* s, const GroupedShape& c) {
+  s->eng.set_partition_shape(c.items, c.chunks);
+  s->eng.set_select_shape(c.tile_min, c.tiles_max);
+  s->eng.set_overlap(c.overlap != 0);
+}
+}  // namespace
+
+extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num_keys,
+                                           double compression, const uint32_t* d_keys,
+                                           const double* d_values, int64_t n, int reps,
+                                           int64_t* out_shape5, double* out_ms2) {
+  return guarded([&] {
+    if (n <= 0) throw KdError(KD_EINVAL, "autotune needs a non-empty batch");
+    if (reps < 1) reps = 1;
+    const bool prof = s->eng.profiling();
+    s->eng.set_profiling(false);
+    auto trial = [&](const GroupedShape& c) {
+      apply_shape(s, c);
+      double best = 1e300;
+      for (int r = 0; r < reps; r++) {
+        kd_grouped* g = nullptr;
+        int rc = kd_grouped_create(s, scale, num_keys, compression, 0.0, &g);
+        if (rc != KD_OK) throw KdError(rc, g_err);
+        std::unique_ptr<kd_grouped> hold(g);
+        s->sync();
+        auto t0 = clk::now();
+        g->add_device(d_keys, d_values, n);
+        s->sync();
+        best = std::min(best, ms_since(t0));
+      }
+      return best;
+    };
+    GroupedShape best = current_shape(s);
+    trial(best);  // warm: buffers at their final sizes, kernels loaded
+    double best_ms = trial(best);
+    const double first_ms = best_ms;
+    const int items[] = {4, 8, 16};
+    const int64_t chunks[] = {1024, 2048, 4096, 8192, 16384};
+    const int64_t tmins[] = {16384, 32768, 65536, 131072};
+    const int64_t tmaxs[] = {16, 32, 64, 128, 256};
+    auto knob = [&](const auto& values, auto set) {
+      for (auto v : values) {
+        GroupedShape c = best;
+        set(c, v);
+        if (memcmp(&c, &best, sizeof c) == 0) continue;
+        double ms = trial(c);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    };
+    knob(items, [](GroupedShape& c, int v) { c.items = v; });
+    knob(chunks, [](GroupedShape& c, int64_t v) { c.chunks = v; });
+    knob(tmins, [](GroupedShape& c, int64_t v) { c.tile_min = v; });
+    knob(tmaxs, [](GroupedShape& c, int64_t v) { c.tiles_max = v; });
+    const int ov[] = {0, 1};
+    knob(ov, [](GroupedShape& c, int v) { c.overlap = v; });
+    apply_shape(s, best);
+    s->eng.set_profiling(prof);
+    if (out_shape5) {
+      out_shape5[0] = best.items;
+      out_shape5[1] = best.chunks;
+      out_shape5[2] = best.tile_min;
+      out_shape5[3] = best.tiles_max;
+      out_shape5[4] = best.overlap;
+    }
+    if (out_ms2) {
+      out_ms2[0] = first_ms;
+      out_ms2[1] = best_ms;
+    }
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_session_autotune_grouped_host(kd_session* s, int scale, int64_t num_keys,
+                                                double compression, const uint32_t* keys,
+                                                const double* values, int64_t n, int reps,
+                                                int64_t* out_shape5, double* out_ms2) {
+  return guarded([&] {
+    if (n <= 0) throw KdError(KD_EINVAL, "autotune needs a non-empty batch");
+    DevBuf dk, dv;
+    dk.reserve(n * sizeof(uint32_t));
+    dv.reserve(n * sizeof(double));
+    s->h2d(dk.p, keys, n * sizeof(uint32_t));
+    s->h2d(dv.p, values, n * sizeof(double));
+    s->sync();
+    int rc = kd_session_autotune_grouped(s, scale, num_keys, compression,
+                                         static_cast<const uint32_t*>(dk.p),
+                                         static_cast<const double*>(dv.p), n, reps, out_shape5,
+                                         out_ms2);
+    if (rc != KD_OK) throw KdError(rc, g_err);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_grouped_add_batch(kd_grouped* g, const uint32_t* keys, const double* values,
                                     int64_t n) {
   return guarded([&] {
@@ -1239,6 +1377,13 @@ struct kd_batch {
   int limit;
   DevBuf st, cm, cw, bm, bw, order, flag;
   int64_t* tmp_order = nullptr;
+  // The merge_into tree (flush + one launch per level) captured once into a
+  // HIP graph over this object's fixed buffers and replayed on later calls.
+  bool use_graph = true;
+  hipGraphExec_t tree_exec = nullptr;
+  ~kd_batch() {
+    if (tree_exec) (void)hipGraphExecDestroy(tree_exec);
+  }
 };
 
 extern "C" int kd_batch_create(kd_session* s, int scale, int64_t num_digests, double compression,
@@ -1368,24 +1513,52 @@ extern "C" int kd_batch_flush(kd_batch* b) {
   });
 }
 
+// Flush of every digest, then the pairwise levels (stride 1, 2, 4, ...).
+static void enqueue_tree(kd_batch* b, hipStream_t st) {
+  auto* bs = static_cast<kd::BatchState*>(b->st.p);
+  auto* cm = static_cast<double*>(b->cm.p);
+  auto* cw = static_cast<double*>(b->cw.p);
+  auto* bm = static_cast<double*>(b->bm.p);
+  auto* bw = static_cast<double*>(b->bw.p);
+  kd::launch_flush(st, b->scale, b->compression, b->lambda, b->nd, bs, cm, cw, bm, bw);
+  launched("k_flush");
+  for (int64_t stride = 1; stride < b->nd; stride *= 2) {
+    kd::launch_merge_pairs(st, b->scale, b->compression, b->lambda, b->nd, stride, bs, cm, cw, bm,
+                           bw);
+    launched("k_merge_pairs");
+  }
+}
+
 extern "C" int kd_batch_merge_tree(kd_batch* b) {
   return guarded([&] {
     kd_session* s = b->s;
-    kd::launch_flush(s->st(), b->scale, b->compression, b->lambda, b->nd,
-                     static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
-                     static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
-                     static_cast<double*>(b->bw.p));
-    launched("k_flush");
-    int ph = s->eng.phase_begin("merge_tree", 0.0);
-    for (int64_t stride = 1; stride < b->nd; stride *= 2) {
-      kd::launch_merge_pairs(s->st(), b->scale, b->compression, b->lambda, b->nd, stride,
-                             static_cast<kd::BatchState*>(b->st.p), static_cast<double*>(b->cm.p),
-                             static_cast<double*>(b->cw.p), static_cast<double*>(b->bm.p),
-                             static_cast<double*>(b->bw.p));
-      launched("k_merge_pairs");
+    if (b->use_graph && !b->tree_exec) {
+      hip_ok(hipStreamBeginCapture(s->st(), hipStreamCaptureModeThreadLocal), "begin capture");
+      hipGraph_t g = nullptr;
+      try {
+        enqueue_tree(b, s->st());
+      } catch (...) {
+        (void)hipStreamEndCapture(s->st(), &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_ok(hipStreamEndCapture(s->st(), &g), "end capture");
+      hipError_t e = hipGraphInstantiate(&b->tree_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      hip_ok(e, "graph instantiate");
     }
+    int ph = s->eng.phase_begin("merge_tree", 0.0);
+    if (b->use_graph) hip_ok(hipGraphLaunch(b->tree_exec, s->st()), "graph launch");
+    else enqueue_tree(b, s->st());
     s->eng.phase_end(ph);
     batch_check_err(b);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_set_graph(kd_batch* b, int enable) {
+  return guarded([&] {
+    b->use_graph = enable != 0;
     return KD_OK;
   });
 }

@@ -331,6 +331,14 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
   int64_t p = off[d];
   const int64_t end = off[d + 1];
   while (p < end || sh.renorm) {
+    // 0. The timestamps the walk below may consume, staged in LDS by all
+    //    lanes (P is free between merges).
+    {
+      const int room = limit - sh.st.nb;
+      const int64_t have = end - p < (int64_t)room ? end - p : (int64_t)room;
+      for (int i = threadIdx.x; i < (int)have; i += blockDim.x) L.P[i] = times[p + i];
+    }
+    __syncthreads();
     // 1. Lane 0 walks samples: clock and decay exponent per sample
     //    (advance_clock, tdigest.mojo:324-332), until the buffer is full, a
     //    renormalization is due, or the samples run out.
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
       double ls = sh.st.log_scale, clock = sh.st.clock;
       int room = limit - sh.st.nb;
       while (p + m < end && m < room) {
-        double t = times[p + m];
+        double t = L.P[m];
         if (t > clock) {
           if (lambda > 0.0) {
             double nls = ls + lambda * (t - clock);

@@ -148,6 +148,19 @@ class Engine {
     rx_chunks_ = chunks;
   }
   int partition_items() const { return rx_items_; }
+  // Tiles of the selection count / gather passes: at least tile_min values,
+  // at most tiles_max tiles per segment (one workgroup per tile).
+  // One-block compress of small segments on a second stream, overlapping the
+  // selection path (off by default: both are bound by the same CU resources
+  // on the grouped workloads measured so far).
+  void set_overlap(bool on) { overlap_ = on; }
+  bool overlap() const { return overlap_; }
+  void set_select_shape(int64_t tile_min, int64_t tiles_max) {
+    sel_tile_min_ = tile_min;
+    sel_tiles_max_ = tiles_max;
+  }
+  int64_t select_tile_min() const { return sel_tile_min_; }
+  int64_t select_tiles_max() const { return sel_tiles_max_; }
   int64_t partition_chunks() const { return rx_chunks_; }
 
   DevBuf& scratch(int i) { return extra_[i]; }
@@ -173,7 +186,7 @@ class Engine {
   };
   void set_profiling(bool on);
   const std::map<std::string, PhaseStat>& phase_stats() { collect(); return stats_; }
-  int phase_begin(const char* name, double bytes);
+  int phase_begin(const char* name, double bytes, hipStream_t on = nullptr);
   void phase_end(int id);
   void collect();
   // Host wall time of a host-side step (recorded as "host:<name>").
@@ -190,6 +203,12 @@ class Engine {
 
   int device_;
   hipStream_t stream_ = nullptr;
+  // Second stream (default priority): the one-block compress of small
+  // segments runs on it, overlapping the selection path of large segments;
+  // fork_/join_ order it after the shared inputs and before the finish.
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
+  bool overlap_ = false;
   DevBuf seg_, res_, em_s_, ew_s_, se_, pose_, vs_, x_, cm_, cw_, cs_, cnt_, t1_, t2_;
   DevBuf parts_, parts2_, spl_, nspl_, bcnt_, bcur_, tiles_, tcnt_, fb_, fbl_, route_, tsum_, long_,
       order_, clist_;
@@ -200,9 +219,13 @@ class Engine {
   std::vector<char> h_sel_;
   unsigned int h_fallbacks_ = 0;
   bool sel_on_ = true;
+  int64_t sel_tile_min_ = 65536, sel_tiles_max_ = 64;
   int64_t path_counts_[3] = {0, 0, 0};
   bool fused_on_ = true;
-  std::vector<int> small1_, small4_, h_small_;
+  // One-block compress size classes (merged records up to 256, 512, 1024,
+  // 2048, 4096): LDS and threads per block grow with the class.
+  static constexpr int kSmallClasses = 5;
+  std::vector<int> small_[kSmallClasses], h_small_;
   DevBuf small_list_;
   // Radix key partition: pass-1 output, tile count matrix, scan partials,
   // segment and tile offsets.
@@ -219,6 +242,7 @@ class Engine {
     std::string name;
     hipEvent_t a, b;
     double bytes;
+    hipStream_t on;
   };
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> pool_;

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -239,12 +240,14 @@ __device__ double* lds_sort_t(double* a, double* b, int n) {
   return src;
 }
 
-// Picks the values-per-thread so that every thread of the block has a run.
+// Picks the values-per-thread so that every thread of the block has a run
+// (at most MAXPER, so a small class keeps its register budget small).
+template <int MAXPER = 16>
 __device__ inline double* lds_sort(double* a, double* b, int n) {
   int per = pow2_at_least(n) / (int)blockDim.x;
-  if (per <= 2) return lds_sort_t<2>(a, b, n);
-  if (per <= 4) return lds_sort_t<4>(a, b, n);
-  if (per <= 8) return lds_sort_t<8>(a, b, n);
+  if (MAXPER <= 2 || per <= 2) return lds_sort_t<2>(a, b, n);
+  if (MAXPER <= 4 || per <= 4) return lds_sort_t<(MAXPER < 4 ? MAXPER : 4)>(a, b, n);
+  if (MAXPER <= 8 || per <= 8) return lds_sort_t<(MAXPER < 8 ? MAXPER : 8)>(a, b, n);
   return lds_sort_t<16>(a, b, n);
 }
 
@@ -905,9 +908,15 @@ __global__ void k_scan_counts(const int64_t* cnt, int n, int64_t* base, int64_t*
 // cursor.
 // When `run_mean` is given, the running mean of the cluster is replayed with
 // the exact recurrence of centroid.mojo:34-36 as well.
+// Records per chunk of the canonical double-double cluster sum: the sum of
+// w*x over [a, b) is the in-order sum of the chunk sums over [a + 16k,
+// a + 16k + 16), each summed in order, so one lane or many lanes (one chunk
+// each) produce the same bits.
+constexpr int64_t kSumChunk = 16;
+
 __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg, int64_t a,
                                 int64_t b, double* wsum, double* run_mean = nullptr) {
-  dd acc{0.0, 0.0};
+  dd acc{0.0, 0.0}, part{0.0, 0.0};
   double ws = 0.0, mean = 0.0;
   int64_t c = pc.c;
   while (c > 0 && pc.pos[c - 1] >= a) c--;
@@ -927,7 +936,11 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
         mean += (xj - mean) * (w / nw);
       }
     } else {
-      acc = dd_add(acc, two_prod(xj, w));
+      part = dd_add(part, two_prod(xj, w));
+      if ((j - a) % kSumChunk == kSumChunk - 1 || j == b - 1) {
+        acc = dd_add(acc, part);
+        part = dd{0.0, 0.0};
+      }
     }
     ws = (j == a) ? w : ws + w;
   }
@@ -935,6 +948,17 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
   *wsum = ws;
   if (run_mean) *run_mean = mean;
   return acc;
+}
+
+// First E record at or after merged index j (binary search over pos).
+__device__ inline int64_t pcur_seek(const PCur& pc, int64_t j) {
+  int64_t lo = 0, hi = pc.ne;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (pc.pos[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
 // Clusters up to this many records replay the running centroid of the CPU
@@ -1006,9 +1030,14 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
                                                           const double* esw, const double* se,
                                                           int64_t* pose, double* cm, double* cw,
                                                           int64_t* ccount, SegResult* res) {
+  // A cluster of more than kReplayMax records: at most CAP / kReplayMax.
+  constexpr int kBigMax = CAP / (int)kReplayMax + 1;
   __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
   __shared__ uint16_t nxt[CAP];
   __shared__ int64_t s_C;
+  __shared__ int s_nbig;
+  __shared__ int s_big[kBigMax];
+  __shared__ double s_bigw[kBigMax];
   for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int si = list[li];
     const SegDesc sd = segs[si];
@@ -1016,7 +1045,7 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
     const double* v = vin + sd.v_off;
     for (int i = threadIdx.x; i < nv; i += BLOCK) sa[padi(i)] = v[i];
     __syncthreads();
-    double* s = nv > 0 ? lds_sort(sa, sb, nv) : sa;
+    double* s = nv > 0 ? lds_sort<(CAP / BLOCK < 16 ? CAP / BLOCK : 16)>(sa, sb, nv) : sa;
     double* x = (s == sa) ? sb : sa;
     const double* e = esm + sd.es_off;
     for (int i = threadIdx.x; i < nv; i += BLOCK) {
@@ -1081,37 +1110,98 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      int64_t* cs = reinterpret_cast<int64_t*>(s);  // kv is no longer needed
+      // Cluster starts go to nxt itself (entries below the current start
+      // are never read again; n <= CAP fits 16 bits).
       int64_t replays = 0;
       const PCur pc{pos, sw, ne, sd.sf, 0};
       int64_t C = 0, st = 0;
       while (st < n) {  // the sweep of sweep_segment, walking the table
-        cs[C++] = st;
+        const uint16_t v = st < n - 1 ? nxt[st] : (uint16_t)0;
+        nxt[C++] = (uint16_t)st;
         if (st == n - 1) break;
-        const uint16_t v = nxt[st];
         const int64_t b = v & 0x7fff;
         st = (v & 0x8000) ? tie_extend(x, n, st, b, pc, ew, replays) : b;
       }
       res[si].replays = replays;
       ccount[si] = C;
       s_C = C;
+      s_nbig = 0;
     }
     __syncthreads();
     const int64_t C = s_C;
-    const int64_t* cs = reinterpret_cast<const int64_t*>(s);
+    const uint16_t* cs = nxt;
+    double* T = s;  // kv is no longer needed: per-record scratch of the means
+    // Clusters of at most kReplayMax records replay the running mean of
+    // centroid.mojo:34-36 (one lane each): the ratios w / W' depend only on
+    // the running weight, so they are formed first (independent divisions)
+    // and the mean recurrence then runs on them.  Longer clusters get their
+    // weight here and their double-double sum from all lanes below.
     for (int64_t c = threadIdx.x; c < C; c += BLOCK) {
-      int64_t a = cs[c];
-      int64_t b = (c + 1 < C) ? cs[c + 1] : n;
+      const int64_t a = cs[c];
+      const int64_t b = (c + 1 < C) ? (int64_t)cs[c + 1] : n;
       PCur pc{pos, sw, ne, sd.sf, 0};
-      double w, mean;
-      if (b - a <= kReplayMax) {
-        direct_sum(x, pc, ew, a, b, &w, &mean);
-      } else {
-        dd sum = direct_sum(x, pc, ew, a, b, &w);
-        if (!equal_run_mean(x, a, b, &mean)) mean = dd_div(sum, w);
+      int64_t ce = ne > 0 ? pcur_seek(pc, a) : 0;
+      double ws = 0.0;
+      for (int64_t j = a; j < b; j++) {
+        double w = sd.sf;
+        if (ce < ne && pos[ce] == j) {
+          w = ew[ce];
+          ce++;
+        }
+        ws = (j == a) ? w : ws + w;
+        if (b - a <= kReplayMax && j > a) T[j] = w / ws;
       }
-      cm[sd.x_off + c] = mean;
-      cw[sd.x_off + c] = w;
+      double mean;
+      if (b - a <= kReplayMax) {
+        mean = x[a];
+        for (int64_t j = a + 1; j < b; j++) mean += (x[j] - mean) * T[j];
+        cm[sd.x_off + c] = mean;
+      } else if (equal_run_mean(x, a, b, &mean)) {
+        cm[sd.x_off + c] = mean;
+      } else {
+        int k = atomicAdd(&s_nbig, 1);
+        s_big[k] = (int)c;
+        s_bigw[k] = ws;
+      }
+      cw[sd.x_off + c] = ws;
+    }
+    __syncthreads();
+    const int nbig = s_nbig;
+    if (nbig > 0) {
+      // Chunk sums of the long clusters (chunk k of cluster [a, b) stored at
+      // T[a + 2k], T[a + 2k + 1]: inside the cluster's own records).
+      for (int q = 0; q < nbig; q++) {
+        const int c = s_big[q];
+        const int64_t a = cs[c];
+        const int64_t b = (c + 1 < C) ? (int64_t)cs[c + 1] : n;
+        const int64_t nch = (b - a + kSumChunk - 1) / kSumChunk;
+        for (int64_t k = threadIdx.x; k < nch; k += BLOCK) {
+          const int64_t j0 = a + k * kSumChunk, j1 = min(b, j0 + kSumChunk);
+          PCur pc{pos, sw, ne, sd.sf, 0};
+          int64_t ce = ne > 0 ? pcur_seek(pc, j0) : 0;
+          dd part{0.0, 0.0};
+          for (int64_t j = j0; j < j1; j++) {
+            double w = sd.sf;
+            if (ce < ne && pos[ce] == j) {
+              w = ew[ce];
+              ce++;
+            }
+            part = dd_add(part, two_prod(x[j], w));
+          }
+          T[a + 2 * k] = part.hi;
+          T[a + 2 * k + 1] = part.lo;
+        }
+      }
+      __syncthreads();
+      for (int q = threadIdx.x; q < nbig; q += BLOCK) {
+        const int c = s_big[q];
+        const int64_t a = cs[c];
+        const int64_t b = (c + 1 < C) ? (int64_t)cs[c + 1] : n;
+        const int64_t nch = (b - a + kSumChunk - 1) / kSumChunk;
+        dd acc{0.0, 0.0};
+        for (int64_t k = 0; k < nch; k++) acc = dd_add(acc, dd{T[a + 2 * k], T[a + 2 * k + 1]});
+        cm[sd.x_off + c] = dd_div(acc, s_bigw[q]);
+      }
     }
     __syncthreads();
   }
@@ -1226,7 +1316,6 @@ constexpr int kSelPad = kSelSample + kSelSample / 16;
 constexpr int kSelCmax = 1024;       // clusters per segment on this path
 constexpr int kSelW = 24;            // bits per slice: exact for 2^29 values per cluster
 constexpr int kSelThreads = 1024;
-constexpr int64_t kSelTileMin = 16384;
 constexpr int64_t kSelMaxV = (int64_t)1 << 29;
 constexpr int kSelLdsVals = 16384;   // marked buckets up to this size are staged in LDS
 constexpr int kSelSmallS = 1023;     // small LDS class of the count and gather passes
@@ -1430,7 +1519,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
-  for (int i = threadIdx.x; i < S; i += blockDim.x) sp[i] = spl[g.b_off + i];
+  // Splitters padded with +inf to P - 1 entries (P a power of two): the
+  // search below then takes log2(P) fixed steps without bounds tests.
+  int P = 1;
+  while (P < S + 1) P <<= 1;
+  const double inf = __builtin_inf();
+  for (int i = threadIdx.x; i < P - 1; i += blockDim.x) sp[i] = i < S ? spl[g.b_off + i] : inf;
   for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
   if (threadIdx.x == 0) {
     smin = ~0ull;
@@ -1442,14 +1536,33 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const double* v = vin + sd.v_off;
   unsigned long long kmn = ~0ull, kmx = 0ull;
   uint16_t* bx = bix + sd.v_off;
-  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-    double x = v[i];
-    int bk = sel_bucket(sp, S, x);
-    atomicAdd(&lc[bk], 1u);
-    bx[i] = (uint16_t)bk;  // the gather pass reads it back instead of searching again
-    unsigned long long k = ord_key(x);
-    kmn = k < kmn ? k : kmn;
-    kmx = k > kmx ? k : kmx;
+  // U values per thread searched together (independent LDS reads
+  // in flight): bucket = number of splitters <= x, as sel_bucket.
+  constexpr int U = 8;
+  for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * kSelThreads) {
+    double x[U];
+    int lo[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
+      x[u] = i < b ? v[i] : inf;
+      lo[u] = 0;
+    }
+    for (int step = P >> 1; step > 0; step >>= 1) {
+#pragma unroll
+      for (int u = 0; u < U; u++) lo[u] += (sp[lo[u] + step - 1] <= x[u]) ? step : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
+      if (i < b) {
+        atomicAdd(&lc[lo[u]], 1u);
+        bx[i] = (uint16_t)lo[u];  // the gather pass reads it back instead of searching again
+        unsigned long long k = ord_key(x[u]);
+        kmn = k < kmn ? k : kmn;
+        kmx = k > kmx ? k : kmx;
+      }
+    }
   }
   atomicMin(&smin, kmn);
   atomicMax(&smax, kmx);
@@ -2155,22 +2268,23 @@ hipEvent_t Engine::get_event() {
   return e;
 }
 
-int Engine::phase_begin(const char* name, double bytes) {
+int Engine::phase_begin(const char* name, double bytes, hipStream_t on) {
   if (!prof_) return -1;
-  Pending p{name, get_event(), get_event(), bytes};
-  check_hip(hipEventRecord(p.a, stream_), "hipEventRecord");
+  Pending p{name, get_event(), get_event(), bytes, on ? on : stream_};
+  check_hip(hipEventRecord(p.a, p.on), "hipEventRecord");
   pending_.push_back(p);
   return (int)pending_.size() - 1;
 }
 
 void Engine::phase_end(int id) {
   if (id < 0) return;
-  check_hip(hipEventRecord(pending_[id].b, stream_), "hipEventRecord");
+  check_hip(hipEventRecord(pending_[id].b, pending_[id].on), "hipEventRecord");
 }
 
 void Engine::collect() {
   if (pending_.empty()) return;
   check_hip(hipStreamSynchronize(stream_), "profile sync");
+  check_hip(hipStreamSynchronize(side_), "profile sync");
   for (auto& p : pending_) {
     float ms = 0.f;
     check_hip(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
@@ -2187,6 +2301,9 @@ void Engine::collect() {
 Engine::Engine(int device) : device_(device) {
   check_hip(hipSetDevice(device), "hipSetDevice");
   check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  check_hip(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
+  check_hip(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+  check_hip(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreate");
 }
 
 Engine::~Engine() {
@@ -2195,6 +2312,9 @@ Engine::~Engine() {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : pool_) (void)hipEventDestroy(e);
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+  if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -2328,8 +2448,7 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
   int64_t max_e_items = 0;
   std::vector<SelSeg> hsel;
   int64_t sel_b = 0, sel_c = 0, sel_m = 0, sel_t = 0;
-  small1_.clear();
-  small4_.clear();
+  for (auto& l : small_) l.clear();
   for (int s = 0; s < nseg; s++) {
     SegDesc& sd = segs[s];
     if (sd.e_cnt > kECap) throw std::runtime_error("too many explicit records in one digest");
@@ -2337,7 +2456,8 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     sd.sel = -1;
     if (fused_on_ && sd.v_cnt + sd.e_cnt <= kCap) {
       sd.mode = 2;
-      (sd.v_cnt + sd.e_cnt <= kSmallCap ? small1_ : small4_).push_back(s);
+      const int64_t nm = sd.v_cnt + sd.e_cnt;
+      small_[nm <= 256 ? 0 : nm <= 512 ? 1 : nm <= 1024 ? 2 : nm <= 2048 ? 3 : 4].push_back(s);
     } else if (sel_on_ && sd.v_cnt > kCap && sd.v_cnt <= kSelMaxV && sd.e_cnt + 64 <= kSelSplMax) {
       SelSeg g{};
       g.si = s;
@@ -2353,7 +2473,7 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
       sel_c += g.cap_c + 1;
       g.m_off = sel_m;
       sel_m += 2 * (int64_t)g.cap_c + 2;
-      g.tile_len = std::max<int64_t>(kSelTileMin, (sd.v_cnt + 255) / 256);
+      g.tile_len = std::max<int64_t>(sel_tile_min_, (sd.v_cnt + sel_tiles_max_ - 1) / sel_tiles_max_);
       g.tile0 = sel_t;
       sel_t += (sd.v_cnt + g.tile_len - 1) / g.tile_len;
       sd.mode = 1;
@@ -2419,6 +2539,50 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     }
   else
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
+  // Small segments first; on the side stream (set_overlap) they overlap the
+  // selection path.
+  bool forked = false;
+  size_t n_small = 0;
+  for (auto& l : small_) n_small += l.size();
+  if (n_small > 0) {
+    // Largest class first: its blocks are the longest, the small ones fill in.
+    int* dl = as<int>(small_list_, n_small);
+    h_small_.clear();
+    for (int c = kSmallClasses - 1; c >= 0; c--) h_small_.insert(h_small_.end(), small_[c].begin(), small_[c].end());
+    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
+                             stream_),
+              "H2D small list");
+    hipStream_t on = stream_;
+    if (overlap_) {
+      check_hip(hipEventRecord(fork_, stream_), "fork");
+      check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork wait");
+      forked = true;
+      on = side_;
+    }
+    int ph_ = phase_begin("small_compress", 0.0, on);
+    int at = 0;
+    for (int c = kSmallClasses - 1; c >= 0; c--) {
+      const int m = (int)small_[c].size();
+      if (m == 0) continue;
+      const unsigned grid = (unsigned)std::min(m, 65535);
+      const int* l = dl + at;
+#define KD_SMALL(CAP, TH)                                                                          \
+  k_small_compress<CAP, TH><<<grid, TH, 0, on>>>(scale, dsegs, l, m, d_v, esm, esw, se, pose, cm,    \
+                                                     cw, ccount, dres)
+      switch (c) {
+        case 0: KD_SMALL(256, 64); break;
+        case 1: KD_SMALL(512, 128); break;
+        case 2: KD_SMALL(1024, 128); break;
+        case 3: KD_SMALL(2048, kBlock); break;
+        default: KD_SMALL(4096, kBlock); break;
+      }
+#undef KD_SMALL
+      launched("k_small_compress");
+      at += m;
+    }
+    phase_end(ph_);
+    if (forked) check_hip(hipEventRecord(join_, side_), "join");
+  }
   if (!hsel.empty()) {
     const int nsel = (int)hsel.size();
     h_sel_.resize(hsel.size() * sizeof(SelSeg));
@@ -2519,33 +2683,10 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
                              stream_),
               "D2H fallbacks");
   }
-  if (!small1_.empty() || !small4_.empty()) {
-    int* dl = as<int>(small_list_, small1_.size() + small4_.size());
-    h_small_.resize(small1_.size() + small4_.size());
-    std::copy(small1_.begin(), small1_.end(), h_small_.begin());
-    std::copy(small4_.begin(), small4_.end(), h_small_.begin() + small1_.size());
-    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
-                             stream_),
-              "H2D small list");
-    int ph_ = phase_begin("small_compress", 0.0);
-    if (!small1_.empty()) {
-      unsigned grid = (unsigned)std::min<size_t>(small1_.size(), 65535);
-      k_small_compress<kSmallCap, 64><<<grid, 64, 0, stream_>>>(
-          scale, dsegs, dl, (int)small1_.size(), d_v, esm, esw, se, pose, cm, cw, ccount, dres);
-      launched("k_small_compress<1024>");
-    }
-    if (!small4_.empty()) {
-      unsigned grid = (unsigned)std::min<size_t>(small4_.size(), 65535);
-      k_small_compress<kCap, kBlock><<<grid, kBlock, 0, stream_>>>(
-          scale, dsegs, dl + small1_.size(), (int)small4_.size(), d_v, esm, esw, se, pose, cm, cw,
-          ccount, dres);
-      launched("k_small_compress<4096>");
-    }
-    phase_end(ph_);
-  }
+  if (forked) check_hip(hipStreamWaitEvent(stream_, join_, 0), "join side stream");
   // The multi-kernel path runs when a segment needs it: one that neither the
   // selection nor the one-block path took, or a selection fallback.
-  bool sort_path = (int64_t)(small1_.size() + small4_.size() + hsel.size()) < nseg;
+  bool sort_path = (int64_t)(n_small + hsel.size()) < nseg;
   if (!hsel.empty() && !sort_path) {
     check_hip(hipStreamSynchronize(stream_), "sync fallbacks");
     sort_path = h_fallbacks_ > 0;

@@ -19,6 +19,7 @@ from ._lib import KdigestError, check, load
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
+_u32p = C.POINTER(C.c_uint32)
 
 
 def _f64(a) -> np.ndarray:
@@ -150,6 +151,43 @@ class GpuSession:
         """Launch shape of the grouped key partition: items per thread of a
         scatter sub-tile (4, 8, 16) and the target chunks per pass."""
         check(load().kd_session_set_partition_shape(self.handle, int(items), int(chunks)))
+
+    def set_overlap(self, enable: bool = True) -> None:
+        """Compress small digests on a second stream, overlapping the
+        selection compress of large ones (off by default)."""
+        check(load().kd_session_set_overlap(self.handle, 1 if enable else 0))
+
+    def autotune_grouped(self, keys, values, num_keys: int, scale: int = 3,
+                         compression: float = 100.0, reps: int = 2) -> dict:
+        """Times the grouped ingest of a device-resident batch (torch tensors
+        or device pointers) under candidate launch shapes and keeps the
+        fastest on this session; returns the shape and both timings."""
+        scale = _scale(scale).id
+        shp = np.zeros(5, np.int64)
+        ms = np.zeros(2)
+        if isinstance(keys, np.ndarray):
+            k = np.ascontiguousarray(keys, dtype=np.uint32)
+            v = _f64(values)
+            if k.size != v.size:
+                raise ValueError("keys and values differ in length")
+            check(load().kd_session_autotune_grouped_host(
+                self.handle, scale, int(num_keys), float(compression), _ptr(k, _u32p), _ptr(v),
+                int(k.size), int(reps), _ptr(shp, _i64p), _ptr(ms)))
+        else:
+            kn, vn = int(keys.numel()), int(values.numel())
+            if kn != vn:
+                raise ValueError("keys and values differ in length")
+            check(load().kd_session_autotune_grouped(
+                self.handle, scale, int(num_keys), float(compression), _device_ptr(keys),
+                _device_ptr(values), kn, int(reps), _ptr(shp, _i64p), _ptr(ms)))
+        return {"items": int(shp[0]), "chunks": int(shp[1]), "tile_min": int(shp[2]),
+                "tiles_max": int(shp[3]), "overlap": bool(shp[4]),
+                "ms_before": float(ms[0]), "ms_after": float(ms[1])}
+
+    def set_select_shape(self, tile_min: int = 65536, tiles_max: int = 64) -> None:
+        """Tiles of the selection compress: at least tile_min values per tile,
+        at most tiles_max tiles per digest."""
+        check(load().kd_session_set_select_shape(self.handle, int(tile_min), int(tiles_max)))
 
     def path_counts(self) -> dict:
         o = np.zeros(3, np.int64)
@@ -535,6 +573,10 @@ class BatchedDigests:
 
     def flush(self) -> None:
         check(load().kd_batch_flush(self._h))
+
+    def set_graph(self, enable: bool = True) -> None:
+        """Run merge_tree as a captured HIP graph (default) or as eager launches."""
+        check(load().kd_batch_set_graph(self._h, 1 if enable else 0))
 
     def merge_tree(self) -> None:
         check(load().kd_batch_merge_tree(self._h))

@@ -108,3 +108,29 @@ def test_batch_rejects_bad_samples_without_change(kd):
     with pytest.raises(kd.KdigestError):
         b.add_at(off, np.array([1.0, np.nan, 3.0, 4.0]), np.arange(4.0), np.ones(4))
     assert b.state(0) == before
+
+
+def test_batch_merge_tree_graph_replays_match_eager(kd):
+    """The merge tree captured as a HIP graph, replayed on later calls with
+    eager work (more add_at samples, state reads) in between: after every
+    replay the digests equal those of an eager twin, bit for bit."""
+    rng = np.random.default_rng(5)
+    nd = 37
+    g = kd.BatchedDigests(nd, 200.0, kd.Decay.rate(1e-3), scale=kd.ScaleK2)
+    e = kd.BatchedDigests(nd, 200.0, kd.Decay.rate(1e-3), scale=kd.ScaleK2)
+    e.set_graph(False)
+    t0 = 0.0
+    for rnd in range(4):
+        sizes = rng.integers(0, 3000, nd)
+        off, vals, times, wts = _streams(rng, sizes, False)
+        times += t0
+        t0 = float(times.max()) + 1.0
+        for b in (g, e):
+            b.add_at(off, vals, times, wts)
+            b.merge_tree()
+        for d in range(nd):
+            mg, wg = g.centroids(d)
+            me, we = e.centroids(d)
+            assert np.array_equal(mg.view(np.uint64), me.view(np.uint64)), (rnd, d)
+            assert np.array_equal(wg.view(np.uint64), we.view(np.uint64)), (rnd, d)
+            assert g.state(d) == e.state(d), (rnd, d)

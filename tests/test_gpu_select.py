@@ -158,3 +158,53 @@ def test_one_block_path_matches_sort_path(kd, oracle):
             assert np.array_equal(wa.view(np.uint64), wb.view(np.uint64)), (scale, lam, k)
         after = fused.path_counts()
         assert after["one_block"] > before["one_block"]
+
+
+@pytest.mark.parametrize("tile_min,tiles_max", [(1024, 4096), (65536, 1), (16384, 256)])
+def test_select_shapes_identical(kd, oracle, tile_min, tiles_max):
+    """The tiling of the selection passes (many small tiles per digest up to
+    one tile per digest) does not change any cluster."""
+    rng = np.random.default_rng(tile_min + tiles_max)
+    nk = 40
+    keys = ((rng.zipf(1.1, 600_000) - 1) % nk).astype(np.uint32)
+    vals = rng.standard_normal(keys.size) + keys * 2.0
+    s = kd.GpuSession(0)
+    s.set_select_shape(tile_min, tiles_max)
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s)
+    g.add_batch(keys, vals)
+    base = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    base.add_batch(keys, vals)
+    assert s.path_counts()["selection"] > 0
+    for k in range(nk):
+        m, w = g.centroids(k)
+        bm, bw = base.centroids(k)
+        assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
+        assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
+    k = 0
+    r = oracle.Digest(3, 100.0)
+    r.add_batch_b(vals[keys == k])
+    rm, rw = r.centroids()
+    m, w = g.centroids(k)
+    check_centroids(m, w, rm, rw, 300.0, True, f"key {k}")
+
+
+def test_autotune_grouped_keeps_results(kd):
+    """The autotuner times candidate launch shapes on a device batch, leaves
+    the fastest on the session, and digests built afterwards equal those of
+    the default shape bit for bit."""
+    rng = np.random.default_rng(3)
+    nk = 500
+    keys = ((rng.zipf(1.1, 400_000) - 1) % nk).astype(np.uint32)
+    vals = rng.standard_normal(keys.size) + keys * 0.5
+    s = kd.GpuSession(0)
+    shape = s.autotune_grouped(keys, vals, nk, scale=kd.ScaleK3, compression=100.0, reps=1)
+    assert shape["items"] in (4, 8, 16) and shape["ms_after"] <= shape["ms_before"] * 1.000001
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s)
+    g.add_batch(keys, vals)
+    base = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    base.add_batch(keys, vals)
+    for k in range(nk):
+        m, w = g.centroids(k)
+        bm, bw = base.centroids(k)
+        assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
+        assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
