@@ -74,14 +74,14 @@ def traffic_bytes(phase: str):
     return None if e is None else e.get("bytes_per_launch")
 
 
-def cpu_baseline_grouped(keys_h: np.ndarray, vals_h: np.ndarray, num_keys: int):
+def cpu_baseline_grouped(keys_h: np.ndarray, vals_h: np.ndarray, num_keys: int, threads: int = 1):
     """The CPU oracle (restatement of the reference MergingDigest) on a
-    bounded sample of the grouped workload, one thread."""
+    bounded sample of the grouped workload; keys split over `threads`."""
     from oracle import oracle as O
 
     digests = [O.Digest(3, 100.0) for _ in range(num_keys)]
     t0 = time.perf_counter()
-    O.grouped_add_batch(digests, keys_h, vals_h, threads=1)
+    O.grouped_add_batch(digests, keys_h, vals_h, threads=threads)
     dt = time.perf_counter() - t0
     return keys_h.size / dt, dt
 
@@ -179,19 +179,9 @@ def batched_decay(kd, session, dev, nd=8192, seed=3):
             "root_centroids": len(b.centroids(0)[0]), "root_total": st["stored_total"]}
 
 
-def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5):
-    """Open-loop serving: Poisson arrivals at `rate` req/s for `seconds`,
-    keys Zipf(1.05) over 1M device-resident digests (ScaleK1, compression
-    100, half-life 1e6 samples); half the requests ingest log-uniform 1..65536
-    N(0,1) values shifted per key, a quarter ask quantiles and a quarter cdf
-    values (log-uniform 1..4096 per request).  Requests are submitted when
-    their arrival time has passed; latency = completion (server clock, when
-    the step that finished the request ends) minus scheduled arrival."""
-    rng = np.random.default_rng(seed)
-    ranks = np.arange(1, nk + 1, dtype=np.float64)
-    cdf = np.cumsum(ranks ** -1.05)
-    cdf /= cdf[-1]
-    perm = rng.permutation(nk)
+def _serve_requests(rng, nk, rate, seconds, cdf, perm, pool):
+    """An open-loop request schedule: Poisson arrivals, Zipf keys, request
+    kinds and sizes as the serving workload states."""
     nreq = int(rate * seconds)
     arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
     keys = perm[np.minimum(np.searchsorted(cdf, rng.uniform(size=nreq)), nk - 1)].astype(np.int64)
@@ -203,24 +193,24 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5):
     off[1:] = np.cumsum(sizes)
     shift = (keys % 1000) / 10.0
     data = np.empty(off[-1])
-    pool = rng.standard_normal(1 << 22)
     for r in range(nreq):
         a, b, m = off[r], off[r + 1], sizes[r]
         if kinds[r] == 0:
-            p0 = rng.integers(0, (1 << 22) - m)
+            p0 = rng.integers(0, pool.size - m)
             data[a:b] = pool[p0:p0 + m] + shift[r]
         elif kinds[r] == 1:
             data[a:b] = rng.uniform(0.0, 1.0, m)
         else:
             data[a:b] = shift[r] + pool[:m]
-    srv = kd.DigestServer(nk, 100.0, kd.Decay.half_life(1e6), scale=kd.ScaleK1, slot_capacity=128,
-                          mem_budget=64 << 20, session=session)
-    # Warm the buffers (not timed).
-    for k in range(64):
-        srv.submit_ingest(k, pool[:5000] + 1.0)
-    while srv.pending:
-        srv.step(4)
-    srv.take_completed()
+    return arr, keys, kinds, sizes, off, data
+
+
+def _serve_run(srv, reqs):
+    """Submits each request once its arrival time has passed and steps the
+    server; returns per-request latency (completion minus arrival, s) and the
+    wall time."""
+    arr, keys, kinds, sizes, off, data = reqs
+    nreq = arr.size
     lat = np.full(nreq, np.nan)
     first_id = None
     i = 0
@@ -246,16 +236,49 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5):
             lat[r] = (tc - c0) - arr[r]
             srv.discard(rid)
         ndone += len(cids)
-    wall = time.perf_counter() - t0
+    return lat, time.perf_counter() - t0
+
+
+def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_seconds=0.5):
+    """Open-loop serving: Poisson arrivals at `rate` req/s for `seconds`,
+    keys Zipf(1.05) over 1M device-resident digests (ScaleK1, compression
+    100, half-life 1e6 samples); half the requests ingest log-uniform 1..65536
+    N(0,1) values shifted per key, a quarter ask quantiles and a quarter cdf
+    values (log-uniform 1..4096 per request).  Requests are submitted when
+    their arrival time has passed; latency = completion (server clock, when
+    the step that finished the request ends) minus scheduled arrival.  A
+    warm-up run of the same mix (its own draws, `warm_seconds` long) goes
+    first and is not measured."""
+    rng = np.random.default_rng(seed)
+    ranks = np.arange(1, nk + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** -1.05)
+    cdf /= cdf[-1]
+    perm = rng.permutation(nk)
+    pool = rng.standard_normal(1 << 22)
+    warm = _serve_requests(rng, nk, rate, warm_seconds, cdf, perm, pool)
+    reqs = _serve_requests(rng, nk, rate, seconds, cdf, perm, pool)
+    srv = kd.DigestServer(nk, 100.0, kd.Decay.half_life(1e6), scale=kd.ScaleK1, slot_capacity=128,
+                          mem_budget=64 << 20, session=session)
+    _serve_run(srv, warm)
+    s0 = srv.stats()
+    lat, wall = _serve_run(srv, reqs)
+    arr, keys, kinds, sizes, off, data = reqs
+    nreq = arr.size
     lat *= 1e3
     st = srv.stats()
+    st = {k: (v - s0[k] if k in ("steps", "admitted", "admitted_bytes", "ingest_rounds") else v)
+          for k, v in st.items()}
     ing = kinds == 0
+    q = int(nreq * 0.25)
     return {"workload": f"{nk} keys Zipf(1.05), ScaleK1, compression 100, half-life 1e6; "
-                        f"Poisson {rate:.0f} req/s for {seconds:.1f} s, 1/2 ingest (log-uniform "
-                        "1..65536), 1/4 quantile, 1/4 cdf (log-uniform 1..4096)",
+                        f"Poisson {rate:.0f} req/s for {seconds:.1f} s after a {warm_seconds:.1f} s "
+                        "warm-up, 1/2 ingest (log-uniform 1..65536), 1/4 quantile, 1/4 cdf "
+                        "(log-uniform 1..4096)",
             "requests": nreq, "wall_s": wall, "requests_per_s": nreq / wall,
             "offered_per_s": rate,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(np.max(lat)),
+            "p99_ms_by_quarter": [float(np.percentile(lat[i * q:(i + 1) * q], 99)) for i in range(4)],
             "ingest_p99_ms": float(np.percentile(lat[ing], 99)),
             "query_p99_ms": float(np.percentile(lat[~ing], 99)),
             "ingest_samples_per_s": float(sizes[ing].sum()) / wall,
@@ -271,7 +294,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--keys", type=int, default=100000)
     ap.add_argument("--single-n", type=int, default=1 << 26)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 27)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
@@ -318,7 +341,7 @@ def main():
         del g
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
     digs = [kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=session) for _ in range(args.steps)]
-    session.profile(True)
+    p0 = session.path_counts()
     torch.cuda.synchronize()
     session.synchronize()
     t0 = time.perf_counter()
@@ -327,13 +350,22 @@ def main():
     session.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    phases = session.phase_stats()
-    paths = session.path_counts()
-    session.profile(False)
+    p1 = session.path_counts()
+    paths = {k: (p1[k] - p0[k]) // args.steps for k in p1}
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n * args.steps / (t1 - t0)
     print(f"[bench] grouped ingest {value / 1e9:.3f} G samples/s, {ms_per_step:.2f} ms/step",
           file=sys.stderr, flush=True)
+    # One more step, outside the timed region, with per-phase HIP events.
+    prof_steps = 2
+    session.profile(True)
+    for _ in range(prof_steps):
+        gp = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=session)
+        gp.add_batch_device(keys, vals)
+    session.synchronize()
+    phases = session.phase_stats()
+    session.profile(False)
+    del gp
     # Dominant device kernel group with algorithmic bytes (HIP events on the
     # session stream around each launch).
     dev_phases = {k: v for k, v in phases.items() if not k.startswith("host:") and v[2] > 0}
@@ -341,6 +373,7 @@ def main():
     dom_avg_ms = dom_ms / max(dom_launches, 1)
     dom_gbs = (dom_bytes / max(dom_launches, 1)) / (dom_avg_ms * 1e-3) / 1e9 if dom_avg_ms > 0 else 0.0
     pipe_gbs = GROUPED_BYTES_PER_SAMPLE * n / (ms_per_step * 1e-3) / 1e9
+    dom_share = dom_ms / prof_steps / ms_per_step
 
     # ------------------------------------------------ tail quantiles per key
     queries = None
@@ -390,9 +423,15 @@ def main():
         kh = keys[:m].cpu().numpy().astype(np.uint32)
         vh = vals[:m].cpu().numpy()
         rate, dt = cpu_baseline_grouped(kh, vh, nk)
+        nthr = max(1, min(16, len(os.sched_getaffinity(0))))
+        rate_mt, dt_mt = cpu_baseline_grouped(kh, vh, nk, threads=nthr)
         cpu = {"value": rate, "unit": "samples/s", "cores": 1, "kind": "port",
                "sample": f"first {m} pairs of the same grouped workload, oracle per-key batch merge "
-                         f"(C restatement of the reference MergingDigest), {dt:.1f} s"}
+                         f"(C restatement of the reference MergingDigest, gcc -O2), {dt:.1f} s",
+               "all_cores": {"value": rate_mt, "cores": nthr, "s": dt_mt,
+                             "note": "same sample, keys partitioned over host threads"},
+               "gpu_over_cpu": value / rate}
+        del kh, vh
     del keys, vals
     torch.cuda.empty_cache()
 
@@ -457,10 +496,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic_bytes(dom_name),
                      "kernel": dom_name, "kernel_avg_ms": dom_avg_ms,
+                     "share_of_step": dom_share,
                      "algorithmic_bytes_per_launch": dom_bytes / max(dom_launches, 1)},
         "pipeline_roofline": {"bytes_per_sample": GROUPED_BYTES_PER_SAMPLE, "achieved": pipe_gbs,
                               "frac": pipe_gbs / HBM_PEAK_GBS},
-        "phases_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "phases_ms_per_step": {k: v[0] / prof_steps for k, v in phases.items()},
+        "phases_note": f"HIP events per kernel group, {prof_steps} extra steps outside the timed region",
         "compress_paths": paths,
         "autotuned_shape": tuned,
         "cpu_baseline": cpu,

@@ -168,14 +168,6 @@ __global__ void k_scale_weights(double* w, int64_t n, double factor) {
 }
 
 // Scale the centroid weights of a list of keys (grouped renormalize).
-__global__ void k_scale_keys(const int64_t* keys, const double* factors, int64_t nk,
-                             const int64_t* off, double* w) {
-  int64_t t = blockIdx.x;
-  if (t >= nk) return;
-  int64_t k = keys[t];
-  for (int64_t i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) w[i] *= factors[t];
-}
-
 // Flags non-finite values and out-of-range keys (validate-then-commit).
 __global__ void k_validate(const uint32_t* keys, const double* vals, int64_t n, int64_t nk,
                            int* flag) {
@@ -187,30 +179,6 @@ __global__ void k_validate(const uint32_t* keys, const double* vals, int64_t n, 
   if (bad) atomicOr(flag, 1);
 }
 
-// New CSR of a grouped digest: untouched keys copy their old centroids,
-// touched keys take the engine's output.
-__global__ void k_gather_csr(int64_t nk, const int64_t* old_off, const double* om,
-                             const double* ow, const int64_t* seg_of, const int64_t* x_off,
-                             const double* em, const double* ew, const int64_t* new_off,
-                             double* nm, double* nw) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nk) return;
-  int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
-  int64_t sidx = seg_of[k];
-  const double* sm;
-  const double* sw;
-  if (sidx >= 0) {
-    sm = em + x_off[sidx];
-    sw = ew + x_off[sidx];
-  } else {
-    sm = om + old_off[k];
-    sw = ow + old_off[k];
-  }
-  for (int64_t i = 0; i < cnt; i++) {
-    nm[dst + i] = sm[i];
-    nw[dst + i] = sw[i];
-  }
-}
 
 unsigned blocks_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
@@ -862,33 +830,148 @@ extern "C" int kd_tdigest_state(kd_tdigest* d, double* o) {
 }
 
 // ========================================================= GroupedTDigest
+// Per-key state and centroids live on the device; add_batch plans, compresses
+// and rebuilds the centroid CSR there (one host sync for the plan summary,
+// one at the end).  Host copies of the offsets and per-key scalars are
+// pulled on demand for the host-side accessors.
 struct kd_grouped {
   kd_session* s;
   int scale;
   int64_t num_keys;
   double compression, lambda;
+  // Host mirrors (valid when !dirty).
   std::vector<double> stored_total, log_scale, clock, vmin, vmax;
   std::vector<int64_t> hoff;      // host copy of the CSR offsets
+  bool dirty = false;
+  int64_t total_cent = 0;         // centroids over all keys
   DevBuf off, cm, cw, S;          // device CSR of centroids + query prefix
   DevBuf noff, ncm, ncw;          // next CSR (swapped in after a batch)
-  DevBuf qstate;
+  DevBuf state;                   // 5 x num_keys: stored total, log scale, clock, min, max
+  DevBuf qstate, seg_key, seg_of, ncnt, tot;
+  int64_t* h_tot = nullptr;       // page-locked new centroid total
   bool prefix_valid = false;
 
+  ~kd_grouped() {
+    if (h_tot) (void)hipHostFree(h_tot);
+  }
   bool decay() const { return lambda > 0.0; }
   int64_t* doff() { return static_cast<int64_t*>(off.p); }
   double* dm() { return static_cast<double*>(cm.p); }
   double* dw() { return static_cast<double*>(cw.p); }
-  int64_t count(int64_t k) const { return hoff[k + 1] - hoff[k]; }
-  double eff(int64_t k) const { return decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k]; }
+  double* dstate(int i) { return static_cast<double*>(state.p) + (size_t)i * num_keys; }
+  void pull() {
+    if (!dirty) return;
+    s->d2h(hoff.data(), doff(), (num_keys + 1) * sizeof(int64_t));
+    s->d2h(stored_total.data(), dstate(0), num_keys * sizeof(double));
+    s->d2h(log_scale.data(), dstate(1), num_keys * sizeof(double));
+    s->d2h(clock.data(), dstate(2), num_keys * sizeof(double));
+    s->d2h(vmin.data(), dstate(3), num_keys * sizeof(double));
+    s->d2h(vmax.data(), dstate(4), num_keys * sizeof(double));
+    s->sync();
+    dirty = false;
+  }
+  int64_t count(int64_t k) {
+    pull();
+    return hoff[k + 1] - hoff[k];
+  }
+  double eff(int64_t k) {
+    pull();
+    return decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k];
+  }
 
   void check_key(int64_t k) const {
     if (k < 0 || k >= num_keys) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
   }
 
   void add_device(const uint32_t* d_keys, const double* d_values, int64_t n);
-  std::vector<int64_t> h_kcnt, h_kof, h_segof;  // per-key counts / offsets / segment of the last batch
   void query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq, double* d_out);
 };
+
+namespace {
+// Per segment of a grouped batch: min / max (grouped.mojo via
+// tdigest.mojo:285-295) and the reconciled stored total (:298-315).
+__global__ void k_grouped_keys(int64_t nseg, const kd::SegDesc* segs, const kd::SegResult* res,
+                               const int64_t* seg_key, double* total, double* vmin, double* vmax) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  const kd::SegDesc sd = segs[i];
+  const kd::SegResult r = res[i];
+  const int64_t k = seg_key[i];
+  if (sd.stored_n == (double)sd.v_cnt * sd.sf && sd.e_cnt == 0) {
+    vmin[k] = r.vmin;
+    vmax[k] = r.vmax;
+  } else {
+    if (r.vmin < vmin[k]) vmin[k] = r.vmin;
+    if (r.vmax > vmax[k]) vmax[k] = r.vmax;
+  }
+  total[k] = r.total;
+}
+
+// Centroid count of every key after the batch.
+__global__ void k_grouped_counts(int64_t nk, const int64_t* off, const int64_t* seg_of,
+                                 const kd::SegResult* res, int64_t* cnt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  int64_t sg = seg_of[k];
+  cnt[k] = sg >= 0 ? res[sg].count : off[k + 1] - off[k];
+}
+
+// Exclusive scan of cnt[0..n) into off[0..n] (one block; off[n] = total).
+__global__ __launch_bounds__(1024) void k_scan_offsets(const int64_t* cnt, int64_t n, int64_t* off) {
+  __shared__ int64_t part[1024];
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t lo = std::min<int64_t>(n, threadIdx.x * per), hi = std::min<int64_t>(n, lo + per);
+  int64_t sum = 0;
+  for (int64_t i = lo; i < hi; i++) sum += cnt[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < 1024; i++) {
+      int64_t v = part[i];
+      part[i] = run;
+      run += v;
+    }
+    off[n] = run;
+  }
+  __syncthreads();
+  int64_t run = part[threadIdx.x];
+  for (int64_t i = lo; i < hi; i++) {
+    off[i] = run;
+    run += cnt[i];
+  }
+}
+
+// New CSR: one wave per key copies its centroids (the engine's output for a
+// touched key, the old ones otherwise).
+__global__ void k_gather_keys(int64_t nk, const int64_t* old_off, const double* om,
+                              const double* ow, const int64_t* seg_of, const kd::SegDesc* segs,
+                              const double* em, const double* ew, const int64_t* new_off,
+                              double* nm, double* nw) {
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (k >= nk) return;
+  const int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
+  const int64_t sg = seg_of[k];
+  const double* sm = sg >= 0 ? em + segs[sg].x_off : om + old_off[k];
+  const double* sw = sg >= 0 ? ew + segs[sg].x_off : ow + old_off[k];
+  for (int64_t i = lane; i < cnt; i += 64) {
+    nm[dst + i] = sm[i];
+    nw[dst + i] = sw[i];
+  }
+}
+
+// Query-side state of every key: stored total, min, max, e^-log_scale.
+__global__ void k_grouped_qstate(int64_t nk, const double* total, const double* logs,
+                                 const double* vmin, const double* vmax, int decay, double* q) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  q[k] = total[k];
+  q[nk + k] = vmin[k];
+  q[2 * nk + k] = vmax[k];
+  q[3 * nk + k] = decay ? kd_exp(-logs[k]) : 1.0;
+}
+}  // namespace
 
 extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, double compression,
                                  double lambda, kd_grouped** out) {
@@ -909,8 +992,13 @@ extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, dou
     g->hoff.assign(num_keys + 1, 0);
     g->off.reserve((num_keys + 1) * sizeof(int64_t));
     hip_ok(hipMemsetAsync(g->off.p, 0, (num_keys + 1) * sizeof(int64_t), s->st()), "memset");
+    g->state.reserve(5 * num_keys * sizeof(double));
+    hip_ok(hipMemsetAsync(g->state.p, 0, 5 * num_keys * sizeof(double), s->st()), "memset");
     g->cm.reserve(sizeof(double));
     g->cw.reserve(sizeof(double));
+    void* p = nullptr;
+    hip_ok(hipHostMalloc(&p, sizeof(int64_t), hipHostMallocDefault), "hipHostMalloc");
+    g->h_tot = static_cast<int64_t*>(p);
     s->sync();
     *out = g.release();
     return KD_OK;
@@ -920,170 +1008,92 @@ extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, dou
 extern "C" void kd_grouped_destroy(kd_grouped* g) { delete g; }
 
 void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int64_t n) {
-  // grouped.mojo:60-162.  Validate the whole batch first (:91-97).
+  // grouped.mojo:60-162.  The whole batch is validated (:91-97) before any
+  // key changes: the partition (or k_validate) sets the flag, and the device
+  // plan changes nothing when it is set.
   clk::time_point tp = clk::now();
   int* flag = s->buf<int>(s->tmp, 1);
   hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
   const bool fused_check = s->eng.key_partition_validates(n, num_keys);
-  int hflag = 0;
   if (!fused_check) {
     k_validate<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, d_values, n, num_keys, flag);
     launched("k_validate");
-    s->d2h(&hflag, flag, sizeof(int));
-    s->sync();
-    if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
-    s->eng.host_phase("validate", ms_since(tp));
   }
-
-  // Group per key (:99-104) on the device; the radix partition validates the
-  // batch in its first pass, before any digest state changes.
+  // Group per key (:99-104) on the device.
   int64_t* dcnt = static_cast<int64_t*>(s->g_counts.reserve(num_keys * sizeof(int64_t)));
   int64_t* dkoff = static_cast<int64_t*>(s->g_koff.reserve((num_keys + 1) * sizeof(int64_t)));
   double* dv = static_cast<double*>(s->g_vals.reserve(std::max<int64_t>(n, 1) * sizeof(double)));
   s->eng.key_partition(d_keys, d_values, n, num_keys, dcnt, dkoff, dv, fused_check ? flag : nullptr);
-  std::vector<int64_t>& kcnt = h_kcnt;
-  std::vector<int64_t>& kof = h_kof;
-  kcnt.resize(num_keys);
-  kof.resize(num_keys + 1);
-  s->d2h(kcnt.data(), dcnt, num_keys * sizeof(int64_t));
-  s->d2h(kof.data(), dkoff, (num_keys + 1) * sizeof(int64_t));
-  if (fused_check) s->d2h(&hflag, flag, sizeof(int));
-  s->sync();
-  if (hflag) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
   s->eng.host_phase("key_partition", ms_since(tp));
 
-  // Per touched key: B-semantics prepare (tdigest.mojo:236-296), keys split
-  // over host threads (each key's state is its own).
-  std::vector<int64_t>& seg_of = h_segof;
-  seg_of.assign(num_keys, -1);
-  int64_t nseg = 0;
-  for (int64_t k = 0; k < num_keys; k++)
-    if (kcnt[k] > 0) seg_of[k] = nseg++;
-  kd::SegVec& segs = s->segv;
-  segs.resize(nseg);
-  std::vector<int64_t> seg_key(nseg);
-  std::vector<double> sfk(nseg);
-  std::vector<char> had_cent(nseg);
-  const int nthr = nseg >= 16384 ? host_threads() : 1;
-  std::vector<std::vector<int64_t>> rk(nthr);
-  std::vector<std::vector<double>> rf(nthr);
-  parallel_for(num_keys, nthr, [&](int64_t k0, int64_t k1, int t) {
-    for (int64_t k = k0; k < k1; k++) {
-      int64_t m = kcnt[k];
-      if (m == 0) continue;
-      double nc_clock = clock[k] + (double)m;
-      if (decay() && nc_clock > clock[k]) {  // advance_clock (:324-332)
-        log_scale[k] += lambda * (nc_clock - clock[k]);
-        if (log_scale[k] > kRenormThreshold) {  // _renormalize (:359-371)
-          double factor = kd_exp(-log_scale[k]);
-          rk[t].push_back(k);
-          rf[t].push_back(factor);
-          stored_total[k] *= factor;
-          log_scale[k] = 0.0;
-        }
-      }
-      clock[k] = nc_clock;
-      double sf = decay() ? kd_exp(log_scale[k]) : 1.0;
-      int64_t i = seg_of[k];
-      had_cent[i] = count(k) > 0;
-      stored_total[k] += (double)m * sf;
-      double effn = decay() ? stored_total[k] * kd_exp(-log_scale[k]) : stored_total[k];
-      SegDesc sd{};
-      sd.v_off = kof[k];
-      sd.v_cnt = m;
-      sd.e_off = hoff[k];
-      sd.e_cnt = count(k);
-      sd.sf = sf;
-      sd.stored_n = stored_total[k];
-      sd.nrm = kd_normalizer(scale, compression, effn);
-      sd.floor = decay() ? kPruneEpsilon * kd_exp(log_scale[k]) : 0.0;
-      segs[i] = sd;
-      seg_key[i] = k;
-      sfk[i] = sf;
-    }
-  });
-  std::vector<int64_t> rkeys;
-  std::vector<double> rfac;
-  for (int t = 0; t < nthr; t++) {
-    rkeys.insert(rkeys.end(), rk[t].begin(), rk[t].end());
-    rfac.insert(rfac.end(), rf[t].begin(), rf[t].end());
-  }
-  if (segs.empty()) return;
-  if (!rkeys.empty()) {
-    int64_t* dk = s->buf<int64_t>(s->q_qd, rkeys.size());
-    double* df = s->buf<double>(s->q_q, rfac.size());
-    s->h2d(dk, rkeys.data(), rkeys.size() * sizeof(int64_t));
-    s->h2d(df, rfac.data(), rfac.size() * sizeof(double));
-    k_scale_keys<<<(unsigned)rkeys.size(), 64, 0, s->st()>>>(dk, df, (int64_t)rkeys.size(), doff(), dw());
-    launched("k_scale_keys");
-    hip_ok(hipGetLastError(), "renormalize launch");
-  }
-  kd::ResVec& res = s->resv;
-  s->eng.host_phase("prepare", ms_since(tp));
-  s->eng.compress(scale, segs, dv, dm(), dw(), res);
+  // Per touched key: B-semantics prepare (tdigest.mojo:236-296), segment
+  // table and compress, all on the device.
+  int64_t* dsk = static_cast<int64_t*>(seg_key.reserve(num_keys * sizeof(int64_t)));
+  int64_t* dso = static_cast<int64_t*>(seg_of.reserve(num_keys * sizeof(int64_t)));
+  kd::GroupedPlanArgs a{};
+  a.scale = scale;
+  a.compression = compression;
+  a.lambda = lambda;
+  a.num_keys = num_keys;
+  a.kcnt = dcnt;
+  a.kof = dkoff;
+  a.off = doff();
+  a.w = dw();
+  a.total = dstate(0);
+  a.logs = dstate(1);
+  a.clock = dstate(2);
+  a.bad = flag;
+  a.seg_key = dsk;
+  a.seg_of = dso;
+  const kd::PlanSummary P = s->eng.compress_grouped(a, dv, dm(), dw());
   s->eng.host_phase("compress", ms_since(tp));
+  if (P.bad == 1) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
+  if (P.bad) throw std::runtime_error("too many explicit records in one digest");
+  if (P.nseg == 0) return;
 
-  // New CSR: untouched keys keep their centroids.
-  std::vector<int64_t> noffh(num_keys + 1, 0);
-  for (int64_t k = 0; k < num_keys; k++) {
-    int64_t c = seg_of[k] >= 0 ? res[seg_of[k]].count : count(k);
-    noffh[k + 1] = noffh[k] + c;
-  }
-  int64_t total = noffh[num_keys];
-  std::vector<int64_t> xo(segs.size());
-  for (size_t i = 0; i < segs.size(); i++) xo[i] = segs[i].x_off;
+  // Per-key min / max / stored total, then the new CSR (untouched keys keep
+  // their centroids).
+  const kd::SegDesc* segs = s->eng.dev_segments();
+  const kd::SegResult* res = s->eng.dev_results();
+  k_grouped_keys<<<blocks_for(P.nseg), 256, 0, s->st()>>>(P.nseg, segs, res, dsk, dstate(0),
+                                                          dstate(3), dstate(4));
+  launched("k_grouped_keys");
+  int64_t* dc = static_cast<int64_t*>(ncnt.reserve(num_keys * sizeof(int64_t)));
+  k_grouped_counts<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dso, res, dc);
+  launched("k_grouped_counts");
   int64_t* dno = static_cast<int64_t*>(noff.reserve((num_keys + 1) * sizeof(int64_t)));
-  double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
-  double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
-  int64_t* dseg = static_cast<int64_t*>(s->g_segof.reserve(num_keys * sizeof(int64_t)));
-  int64_t* dxo = static_cast<int64_t*>(s->g_xoff.reserve(std::max<size_t>(xo.size(), 1) * sizeof(int64_t)));
-  s->h2d(dno, noffh.data(), (num_keys + 1) * sizeof(int64_t));
-  s->h2d(dseg, seg_of.data(), num_keys * sizeof(int64_t));
-  s->h2d(dxo, xo.data(), xo.size() * sizeof(int64_t));
-  k_gather_csr<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dm(), dw(), dseg, dxo,
-                                                          s->eng.cent_means(), s->eng.cent_weights(),
-                                                          dno, dnm, dnw);
-  launched("k_gather_csr");
-  hip_ok(hipGetLastError(), "gather launch");
+  k_scan_offsets<<<1, 1024, 0, s->st()>>>(dc, num_keys, dno);
+  launched("k_scan_offsets");
+  // A compress never grows a digest: the new total is at most the old one
+  // with every touched key's prior centroids replaced by its merged records.
+  const int64_t bound = std::max<int64_t>(total_cent - P.total_e + P.total_x, 1);
+  double* dnm = static_cast<double*>(ncm.reserve(bound * sizeof(double)));
+  double* dnw = static_cast<double*>(ncw.reserve(bound * sizeof(double)));
+  k_gather_keys<<<blocks_for(num_keys * 64), 256, 0, s->st()>>>(
+      num_keys, doff(), dm(), dw(), dso, segs, s->eng.cent_means(), s->eng.cent_weights(), dno,
+      dnm, dnw);
+  launched("k_gather_keys");
+  s->d2h(h_tot, dno + num_keys, sizeof(int64_t));
   s->sync();
   off.swap(noff);
   cm.swap(ncm);
   cw.swap(ncw);
-  hoff.swap(noffh);
+  total_cent = *h_tot;
+  dirty = true;
   prefix_valid = false;
   s->eng.host_phase("csr", ms_since(tp));
-
-  // Per key: min/max (:285-295) and the reconciled stored total (:298-315).
-  for (size_t i = 0; i < segs.size(); i++) {
-    int64_t k = seg_key[i];
-    double m = (double)segs[i].v_cnt;
-    if (stored_total[k] == m * sfk[i] && !had_cent[i]) {
-      vmin[k] = res[i].vmin;
-      vmax[k] = res[i].vmax;
-    } else {
-      if (res[i].vmin < vmin[k]) vmin[k] = res[i].vmin;
-      if (res[i].vmax > vmax[k]) vmax[k] = res[i].vmax;
-    }
-    stored_total[k] = res[i].total;
-  }
 }
 
 void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq,
                        double* d_out) {
-  int64_t total = hoff[num_keys];
-  double* dS = static_cast<double*>(S.reserve(std::max<int64_t>(total, 1) * sizeof(double)));
+  double* dS = static_cast<double*>(S.reserve(std::max<int64_t>(total_cent, 1) * sizeof(double)));
   double* st = static_cast<double*>(qstate.reserve(4 * num_keys * sizeof(double)));
   if (!prefix_valid) {
     k_query_prefix<<<blocks_for(num_keys, 64), 64, 0, s->st()>>>(num_keys, doff(), dw(), dS);
     launched("k_query_prefix");
-    std::vector<double> hs(4 * num_keys);
-    for (int64_t k = 0; k < num_keys; k++) {
-      hs[k] = stored_total[k];
-      hs[num_keys + k] = vmin[k];
-      hs[2 * num_keys + k] = vmax[k];
-      hs[3 * num_keys + k] = decay() ? kd_exp(-log_scale[k]) : 1.0;
-    }
-    s->h2d(st, hs.data(), hs.size() * sizeof(double));
+    k_grouped_qstate<<<blocks_for(num_keys), 256, 0, s->st()>>>(
+        num_keys, dstate(0), dstate(1), dstate(3), dstate(4), decay() ? 1 : 0, st);
+    launched("k_grouped_qstate");
     prefix_valid = true;
   }
   k_query_grouped<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff(), dm(), dw(), dS, st,
@@ -1289,7 +1299,7 @@ extern "C" int64_t kd_grouped_to_bytes(kd_grouped* g, int64_t key, uint8_t* out,
   int64_t r = 0;
   int rc = guarded([&] {
     g->check_key(key);
-    int64_t c = g->count(key);
+    int64_t c = g->count(key);  // pulls the host mirrors
     std::vector<double> m(c), w(c);
     g->s->d2h(m.data(), g->dm() + g->hoff[key], c * sizeof(double));
     g->s->d2h(w.data(), g->dw() + g->hoff[key], c * sizeof(double));
@@ -1335,6 +1345,7 @@ extern "C" int kd_grouped_centroids(kd_grouped* g, int64_t key, double* means, d
 extern "C" int kd_grouped_key_state(kd_grouped* g, int64_t key, double* o) {
   return guarded([&] {
     g->check_key(key);
+    g->pull();
     o[0] = g->stored_total[key];
     o[1] = g->log_scale[key];
     o[2] = g->clock[key];

@@ -50,6 +50,38 @@ struct SegResult {
   int64_t path;     // 0: multi-kernel sort path, 1: selection, 2: one block per digest
 };
 
+// What a compress plan holds, however it was made (on the host from a
+// segment table, or on the device for a grouped batch): segment, class and
+// selection-path counts and the sizes of the buffers the launches need.
+struct PlanSummary {
+  int64_t nseg, nsel, n_mode0, n_eseg, max_e_items;
+  int64_t n_small[5];  // one-block classes: merged records <= 256, 512, 1024, 2048, 4096
+  int64_t t_small, t_mid, sel_t;  // selection tiles: end of class A, of class B, total
+  int64_t sel_b, sel_c, sel_m;    // selection bucket / cluster / mark array sizes
+  int64_t total_e, total_x, total_t, total_v;
+  double sel_v;                   // new values on the selection path
+  int64_t bad;                    // 1: invalid batch; 2: too many prior centroids in a digest
+};
+
+// Inputs of the device-side plan of a grouped batch (per-key B-semantics
+// prepare of tdigest.mojo:236-296 for every key the batch touches, then the
+// segment table, size classes and selection layout of the compress).
+struct GroupedPlanArgs {
+  int scale;
+  double compression, lambda;
+  int64_t num_keys;
+  const int64_t* kcnt;   // new values per key
+  const int64_t* kof;    // first new value of each key (grouped values)
+  const int64_t* off;    // prior centroid CSR (num_keys + 1)
+  double* w;             // prior centroid weights (scaled in place on renormalize)
+  double* total;         // per-key stored total, decay exponent, clock (updated)
+  double* logs;
+  double* clock;
+  const int* bad;        // batch validation flag (no change when set)
+  int64_t* seg_key;      // out: key of each segment
+  int64_t* seg_of;       // out: segment of each key, or -1
+};
+
 // Page-locked host allocator (hipHostMalloc): copies from and to such memory
 // run at link speed without staging.  Allocation is slow, so vectors using it
 // are kept and reused.
@@ -126,6 +158,15 @@ class Engine {
                 const double* d_em, const double* d_ew, ResVec& res,
                 bool sync_results = true);
 
+  // The same compress for a grouped batch planned on the device: the plan
+  // (per-key prepare, segment table, classes) runs in one kernel and only
+  // its summary comes back to the host; the results stay on the device
+  // (dev_results(), dev_segments()).  Returns the summary; summary.bad != 0
+  // means nothing was changed or launched.
+  PlanSummary compress_grouped(const GroupedPlanArgs& a, const double* d_v, const double* d_em,
+                               const double* d_ew);
+  const SegDesc* dev_segments() const { return static_cast<const SegDesc*>(seg_.p); }
+
   const double* cent_means() const { return static_cast<const double*>(cm_.p); }
   const double* cent_weights() const { return static_cast<const double*>(cw_.p); }
   const double* sorted_records() const { return static_cast<const double*>(x_.p); }
@@ -199,7 +240,13 @@ class Engine {
   bool profiling() const { return prof_; }
 
  private:
-  void sort_values(SegVec& segs, const double* d_v, int64_t total_v);
+  void sort_values(int nseg, const double* d_v, int64_t total_v);
+  // Launches of a planned compress: the segment table, small-class lists,
+  // selection segments and E-segment list are on the device already.
+  void execute(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
+               const double* d_ew);
+  DevBuf plan_sum_, plan_part_;
+  PlanSummary* h_plan_ = nullptr;  // page-locked copy of the device plan summary
 
   int device_;
   hipStream_t stream_ = nullptr;

@@ -1353,6 +1353,33 @@ struct SelBound {
   double sl_hi, sl_lo; // sum of the bucket's values below rank vr
 };
 
+// Selection-path sizing of one digest (shared by the host and device plans).
+__host__ __device__ inline int sel_cap_s(int64_t v_cnt, int64_t e_cnt) {
+  int64_t a = v_cnt / 32 > 63 ? v_cnt / 32 : 63;
+  a += e_cnt;
+  return (int)(a < kSelSplMax ? a : kSelSplMax);
+}
+
+// Two consecutive clusters span more than one unit of k, and k is evaluated
+// on q clamped to [1e-15, 1 - 1e-15].
+__host__ __device__ inline int sel_cap_c(int scale, double nrm) {
+  double kr = kd_k(scale, 1.0 - 1e-15, nrm) - kd_k(scale, 1e-15, nrm);
+  int64_t cc = (kr >= 0.0 && kr < 1e6) ? (int64_t)(2.0 * kr) + 4 : kSelCmax;
+  if (cc > kSelCmax) cc = kSelCmax;
+  return (int)(cc < 2 ? 2 : cc);
+}
+
+__host__ __device__ inline int64_t sel_tile_len(int64_t v_cnt, int64_t tile_min, int64_t tiles_max) {
+  int64_t t = (v_cnt + tiles_max - 1) / tiles_max;
+  return t > tile_min ? t : tile_min;
+}
+
+// LDS footprint class of the count and gather passes: 0 few splitters and
+// clusters, 1 many splitters and few clusters, 2 the rest.
+__host__ __device__ inline int sel_class(int cap_s, int cap_c) {
+  return cap_c <= kSelSmallC ? (cap_s <= kSelSmallS ? 0 : 1) : 2;
+}
+
 __device__ inline double key_value(unsigned long long k) {
   uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
   return kd_bits2d(b);
@@ -1510,7 +1537,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
                                                            const double* vin, const double* spl,
                                                            SelState* st, unsigned int* cnt,
                                                            uint16_t* bix) {
-  __shared__ double sp[SMAX];
+  // The search runs on the high 32 bits of each splitter's order key (4-byte
+  // LDS entries: half the bank traffic of the doubles); splitters sharing
+  // those bits with the value are then compared exactly from global memory.
+  __shared__ uint32_t sk[SMAX];
   __shared__ unsigned int lc[SMAX + 1];
   __shared__ unsigned long long smin, smax;
   const int64_t t = t0 + blockIdx.x;
@@ -1519,12 +1549,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
-  // Splitters padded with +inf to P - 1 entries (P a power of two): the
-  // search below then takes log2(P) fixed steps without bounds tests.
+  const double* sg = spl + g.b_off;
+  // Padded with 0xffffffff (above the key of any finite value) to P - 1
+  // entries, P a power of two: log2(P) fixed steps without bounds tests.
   int P = 1;
   while (P < S + 1) P <<= 1;
-  const double inf = __builtin_inf();
-  for (int i = threadIdx.x; i < P - 1; i += blockDim.x) sp[i] = i < S ? spl[g.b_off + i] : inf;
+  for (int i = threadIdx.x; i < P - 1; i += blockDim.x)
+    sk[i] = i < S ? (uint32_t)(ord_key(sg[i]) >> 32) : 0xffffffffu;
   for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
   if (threadIdx.x == 0) {
     smin = ~0ull;
@@ -1536,31 +1567,37 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const double* v = vin + sd.v_off;
   unsigned long long kmn = ~0ull, kmx = 0ull;
   uint16_t* bx = bix + sd.v_off;
-  // U values per thread searched together (independent LDS reads
-  // in flight): bucket = number of splitters <= x, as sel_bucket.
-  constexpr int U = 8;
+  // U values per thread searched together (independent LDS reads in
+  // flight): bucket = number of splitters <= x, as sel_bucket.
+  constexpr int U = 6;
   for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * kSelThreads) {
     double x[U];
+    uint32_t kx[U];
     int lo[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
-      x[u] = i < b ? v[i] : inf;
+      x[u] = i < b ? v[i] : 0.0;
+      kx[u] = (uint32_t)(ord_key(x[u]) >> 32);
       lo[u] = 0;
     }
+    // lo = number of splitters whose high key bits are below the value's:
+    // all of them are below the value.
     for (int step = P >> 1; step > 0; step >>= 1) {
 #pragma unroll
-      for (int u = 0; u < U; u++) lo[u] += (sp[lo[u] + step - 1] <= x[u]) ? step : 0;
+      for (int u = 0; u < U; u++) lo[u] += (sk[lo[u] + step - 1] < kx[u]) ? step : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       if (i < b) {
-        atomicAdd(&lc[lo[u]], 1u);
-        bx[i] = (uint16_t)lo[u];  // the gather pass reads it back instead of searching again
-        unsigned long long k = ord_key(x[u]);
-        kmn = k < kmn ? k : kmn;
-        kmx = k > kmx ? k : kmx;
+        int k = lo[u];
+        while (k < S && sk[k] == kx[u] && sg[k] <= x[u]) k++;
+        atomicAdd(&lc[k], 1u);
+        bx[i] = (uint16_t)k;  // the gather pass reads it back instead of searching again
+        unsigned long long kk = ord_key(x[u]);
+        kmn = kk < kmn ? kk : kmn;
+        kmx = kk > kmx ? kk : kmx;
       }
     }
   }
@@ -1799,9 +1836,14 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
                                                             double* csl, double* X) {
   constexpr int kItems = 8;                 // values per thread per chunk
   constexpr int kMk = 2 * CMAX + 2;          // gathered buckets of a segment at most
+  // Replicas of the cluster accumulators (lane r adds into replica r % kRep):
+  // the central clusters of a large digest hold a third of its values each,
+  // so one copy would serialize a wave's atomic adds.  The slices are exact
+  // fixed-point values, so any grouping of the sums gives the same bits.
+  constexpr int kRep = CMAX <= 256 ? 8 : 1;
   __shared__ int inf[SMAX + 1];
   __shared__ int ex[CMAX];
-  __shared__ double acc[2 * CMAX];
+  __shared__ double acc[kRep * 2 * CMAX];
   __shared__ unsigned int lcnt[kMk], lbase[kMk];
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
@@ -1812,7 +1854,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
   const int S = st[j].nspl, C = st[j].ncl, nmk = st[j].nmk;
   for (int i = threadIdx.x; i <= S; i += blockDim.x) inf[i] = binfo[g.b_off + i];
   for (int i = threadIdx.x; i < C; i += blockDim.x) ex[i] = ce[g.c_off + i];
-  for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < kRep * 2 * CMAX; i += blockDim.x) acc[i] = 0.0;
+  double* acc_r = acc + (threadIdx.x % kRep) * 2 * CMAX;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
@@ -1840,8 +1883,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
         } else {
           double s1, s2;
           sel_slices(x, ex[f], s1, s2);
-          unsafeAtomicAdd(&acc[2 * f], s1);
-          unsafeAtomicAdd(&acc[2 * f + 1], s2);
+          unsafeAtomicAdd(&acc_r[2 * f], s1);
+          unsafeAtomicAdd(&acc_r[2 * f + 1], s2);
         }
       }
     }
@@ -1857,8 +1900,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < 2 * C; i += blockDim.x)
-    if (acc[i] != 0.0) unsafeAtomicAdd(&csl[2 * g.c_off + i], acc[i]);
+  for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) {
+    double sum = 0.0;
+#pragma unroll
+    for (int r = 0; r < kRep; r++) sum += acc[r * 2 * CMAX + i];
+    if (sum != 0.0) unsafeAtomicAdd(&csl[2 * g.c_off + i], sum);
+  }
 }
 
 // Deterministic block reduction of per-thread double-doubles (fixed tree).
@@ -2312,6 +2359,7 @@ Engine::~Engine() {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : pool_) (void)hipEventDestroy(e);
+  if (h_plan_) (void)hipHostFree(h_plan_);
   if (fork_) (void)hipEventDestroy(fork_);
   if (join_) (void)hipEventDestroy(join_);
   if (side_) (void)hipStreamDestroy(side_);
@@ -2323,8 +2371,7 @@ static T* as(DevBuf& b, size_t count) {
   return static_cast<T*>(b.reserve(count * sizeof(T)));
 }
 
-void Engine::sort_values(SegVec& segs, const double* d_v, int64_t total_v) {
-  int nseg = (int)segs.size();
+void Engine::sort_values(int nseg, const double* d_v, int64_t total_v) {
   double* t1 = as<double>(t1_, std::max<int64_t>(total_v, 1));
   double* t2 = as<double>(t2_, std::max<int64_t>(total_v, 1));
   double* x = static_cast<double*>(x_.p);
@@ -2443,11 +2490,10 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
   int nseg = (int)segs.size();
   res.resize(nseg);
   if (nseg == 0) return;
-  int64_t total_v = 0, total_e = 0, total_x = 0, total_t = 0;
+  PlanSummary P{};
+  P.nseg = nseg;
   std::vector<int> eseg;
-  int64_t max_e_items = 0;
   std::vector<SelSeg> hsel;
-  int64_t sel_b = 0, sel_c = 0, sel_m = 0, sel_t = 0;
   for (auto& l : small_) l.clear();
   for (int s = 0; s < nseg; s++) {
     SegDesc& sd = segs[s];
@@ -2461,39 +2507,95 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     } else if (sel_on_ && sd.v_cnt > kCap && sd.v_cnt <= kSelMaxV && sd.e_cnt + 64 <= kSelSplMax) {
       SelSeg g{};
       g.si = s;
-      g.cap_s = (int)std::min<int64_t>(kSelSplMax, std::max<int64_t>(63, sd.v_cnt / 32) + sd.e_cnt);
-      // Two consecutive clusters span more than one unit of k, and k is
-      // evaluated on q clamped to [1e-15, 1 - 1e-15].
-      double kr = kd_k(scale, 1.0 - 1e-15, sd.nrm) - kd_k(scale, 1e-15, sd.nrm);
-      int64_t cc = (kr >= 0.0 && kr < 1e6) ? (int64_t)(2.0 * kr) + 4 : kSelCmax;
-      g.cap_c = (int)std::max<int64_t>(2, std::min<int64_t>(kSelCmax, cc));
-      g.b_off = sel_b;
-      sel_b += g.cap_s + 2;
-      g.c_off = sel_c;
-      sel_c += g.cap_c + 1;
-      g.m_off = sel_m;
-      sel_m += 2 * (int64_t)g.cap_c + 2;
-      g.tile_len = std::max<int64_t>(sel_tile_min_, (sd.v_cnt + sel_tiles_max_ - 1) / sel_tiles_max_);
-      g.tile0 = sel_t;
-      sel_t += (sd.v_cnt + g.tile_len - 1) / g.tile_len;
+      g.cap_s = sel_cap_s(sd.v_cnt, sd.e_cnt);
+      g.cap_c = sel_cap_c(scale, sd.nrm);
+      g.b_off = P.sel_b;
+      P.sel_b += g.cap_s + 2;
+      g.c_off = P.sel_c;
+      P.sel_c += g.cap_c + 1;
+      g.m_off = P.sel_m;
+      P.sel_m += 2 * (int64_t)g.cap_c + 2;
+      g.tile_len = sel_tile_len(sd.v_cnt, sel_tile_min_, sel_tiles_max_);
       sd.mode = 1;
       sd.sel = (int32_t)hsel.size();
       hsel.push_back(g);
+      P.sel_v += (double)sd.v_cnt;
+    } else {
+      P.n_mode0++;
     }
-    total_v = std::max(total_v, sd.v_off + sd.v_cnt);
-    sd.es_off = total_e;
-    total_e += sd.e_cnt;
+    P.total_v = std::max(P.total_v, sd.v_off + sd.v_cnt);
+    sd.es_off = P.total_e;
+    P.total_e += sd.e_cnt;
     int64_t n = sd.v_cnt + sd.e_cnt;
-    sd.x_off = total_x;
-    total_x += n;
-    sd.t_off = total_t;
-    total_t += (n + kTile - 1) / kTile;
+    sd.x_off = P.total_x;
+    P.total_x += n;
+    sd.t_off = P.total_t;
+    P.total_t += (n + kTile - 1) / kTile;
     if (sd.e_cnt > 0) {
       eseg.push_back(s);
-      max_e_items = std::max(max_e_items, n);
+      P.max_e_items = std::max(P.max_e_items, n);
     }
   }
+  P.nsel = (int64_t)hsel.size();
+  P.n_eseg = (int64_t)eseg.size();
+  for (int c = 0; c < kSmallClasses; c++) P.n_small[c] = (int64_t)small_[c].size();
+  if (!hsel.empty()) {
+    // LDS footprint classes of the count and gather passes, in this order:
+    // few splitters and clusters, many splitters and few clusters, the rest;
+    // tiles are numbered in this order.
+    std::stable_sort(hsel.begin(), hsel.end(), [&](const SelSeg& x, const SelSeg& y) {
+      return sel_class(x.cap_s, x.cap_c) < sel_class(y.cap_s, y.cap_c);
+    });
+    int64_t t = 0;
+    for (size_t i = 0; i < hsel.size(); i++) {
+      SelSeg& g = hsel[i];
+      g.tile0 = t;
+      t += (segs[g.si].v_cnt + g.tile_len - 1) / g.tile_len;
+      segs[g.si].sel = (int32_t)i;
+      const int cl = sel_class(g.cap_s, g.cap_c);
+      if (cl == 0) P.t_small = t;
+      if (cl <= 1) P.t_mid = t;
+    }
+    P.sel_t = t;
+  }
+  // Upload the plan: segment table, E-segment list, small-class lists
+  // (largest class first), selection segments.
   SegDesc* dsegs = as<SegDesc>(seg_, nseg);
+  check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
+  int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
+  if (!eseg.empty())
+    check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
+  h_small_.clear();
+  for (int c = kSmallClasses - 1; c >= 0; c--) h_small_.insert(h_small_.end(), small_[c].begin(), small_[c].end());
+  int* dl = as<int>(small_list_, std::max<size_t>(h_small_.size(), 1));
+  if (!h_small_.empty())
+    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
+                             stream_),
+              "H2D small list");
+  if (!hsel.empty()) {
+    h_sel_.resize(hsel.size() * sizeof(SelSeg));
+    memcpy(h_sel_.data(), hsel.data(), h_sel_.size());
+    SelSeg* dsel = as<SelSeg>(sel_seg_, hsel.size());
+    check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
+  }
+  execute(scale, P, d_v, d_em, d_ew);
+  SegResult* dres = static_cast<SegResult*>(res_.p);
+  // The call ends synchronized: host staging read by the asynchronous copies
+  // above (segment table, lists) is then free to change.
+  if (sync_results)
+    check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
+  check_hip(hipStreamSynchronize(stream_), "sync results");
+  if (sync_results)
+    for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : (r.path == 2 ? 2 : 1)]++;
+}
+
+void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
+                     const double* d_ew) {
+  const int nseg = (int)P.nseg;
+  const int64_t total_e = P.total_e, total_x = P.total_x, total_t = P.total_t, total_v = P.total_v;
+  SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
+  int* deseg = static_cast<int*>(order_.p);
+  const int* dl = static_cast<const int*>(small_list_.p);
   SegResult* dres = as<SegResult>(res_, nseg);
   double* esm = as<double>(em_s_, std::max<int64_t>(total_e, 1));
   double* esw = as<double>(ew_s_, std::max<int64_t>(total_e, 1));
@@ -2507,51 +2609,22 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
   int64_t* ccount = as<int64_t>(cnt_, 2 * (size_t)nseg + 1);
   int64_t* cbase = ccount + nseg;
   double* tsum = as<double>(tsum_, 2 * std::max<int64_t>(total_t, 1));
-  int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
   int64_t* dtotal = as<int64_t>(clist_, 2);
 
-  int64_t t_small = 0;
-  if (!hsel.empty()) {
-    // Small-footprint class first (splitters and clusters that fit the small
-    // LDS layout of the count and gather passes), then the rest; tiles are
-    // numbered in this order.
-    std::stable_partition(hsel.begin(), hsel.end(), [](const SelSeg& g) {
-      return g.cap_s <= kSelSmallS && g.cap_c <= kSelSmallC;
-    });
-    int64_t t = 0;
-    for (size_t i = 0; i < hsel.size(); i++) {
-      SelSeg& g = hsel[i];
-      g.tile0 = t;
-      t += (segs[g.si].v_cnt + g.tile_len - 1) / g.tile_len;
-      segs[g.si].sel = (int32_t)i;
-      if (g.cap_s <= kSelSmallS && g.cap_c <= kSelSmallC) t_small = t;
-    }
-  }
-  check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
-  if (!eseg.empty())
-    check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
-  if (!eseg.empty())
-    {
-      int ph_ = phase_begin("sort_e", 32.0 * (double)total_e);
-      k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
-      launched("k_sort_e");
-      phase_end(ph_);
-    }
-  else
+  if (P.n_eseg > 0) {
+    int ph_ = phase_begin("sort_e", 32.0 * (double)total_e);
+    k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
+    launched("k_sort_e");
+    phase_end(ph_);
+  } else {
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
+  }
   // Small segments first; on the side stream (set_overlap) they overlap the
   // selection path.
   bool forked = false;
-  size_t n_small = 0;
-  for (auto& l : small_) n_small += l.size();
+  int64_t n_small = 0;
+  for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
   if (n_small > 0) {
-    // Largest class first: its blocks are the longest, the small ones fill in.
-    int* dl = as<int>(small_list_, n_small);
-    h_small_.clear();
-    for (int c = kSmallClasses - 1; c >= 0; c--) h_small_.insert(h_small_.end(), small_[c].begin(), small_[c].end());
-    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
-                             stream_),
-              "H2D small list");
     hipStream_t on = stream_;
     if (overlap_) {
       check_hip(hipEventRecord(fork_, stream_), "fork");
@@ -2560,9 +2633,9 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
       on = side_;
     }
     int ph_ = phase_begin("small_compress", 0.0, on);
-    int at = 0;
+    int64_t at = 0;
     for (int c = kSmallClasses - 1; c >= 0; c--) {
-      const int m = (int)small_[c].size();
+      const int m = (int)P.n_small[c];
       if (m == 0) continue;
       const unsigned grid = (unsigned)std::min(m, 65535);
       const int* l = dl + at;
@@ -2583,11 +2656,11 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     phase_end(ph_);
     if (forked) check_hip(hipEventRecord(join_, side_), "join");
   }
-  if (!hsel.empty()) {
-    const int nsel = (int)hsel.size();
-    h_sel_.resize(hsel.size() * sizeof(SelSeg));
-    memcpy(h_sel_.data(), hsel.data(), h_sel_.size());
-    SelSeg* dsel = as<SelSeg>(sel_seg_, nsel);
+  const int nsel = (int)P.nsel;
+  if (nsel > 0) {
+    const int64_t sel_b = P.sel_b, sel_c = P.sel_c, sel_m = P.sel_m;
+    const int64_t t_small = P.t_small, t_mid = P.t_mid, sel_t = P.sel_t;
+    SelSeg* dsel = static_cast<SelSeg*>(sel_seg_.p);
     SelState* dst = as<SelState>(sel_st_, nsel);
     double* dspl = as<double>(sel_spl_, sel_b);
     unsigned int* dcnt = as<unsigned int>(sel_cnt_, sel_b);
@@ -2600,7 +2673,6 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     int2* dml = as<int2>(sel_ml_, sel_m);
     uint16_t* dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
     unsigned int* dmlc = as<unsigned int>(sel_mlc_, 2);  // gathered-bucket list size, fallbacks
-    check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
     check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
     check_hip(hipMemsetAsync(dmlc, 0, 2 * sizeof(unsigned int), stream_), "memset sel list");
     k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
@@ -2616,8 +2688,7 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
       launched("k_sel_sample<8192>");
       phase_end(ph_);
     }
-    double sel_bytes = 0.0;
-    for (const SelSeg& g : hsel) sel_bytes += 8.0 * (double)segs[g.si].v_cnt;
+    const double sel_bytes = 8.0 * P.sel_v;
     {
       int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
       if (t_small > 0) {
@@ -2651,9 +2722,14 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
             dsegs, dsel, nsel, 0, t_small, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
         launched("k_sel_gather<small>");
       }
-      if (sel_t > t_small) {
-        k_sel_gather<kSelSplMax, kSelCmax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_small, sel_t, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
+      if (t_mid > t_small) {
+        k_sel_gather<kSelSplMax, kSelSmallC><<<(unsigned)(t_mid - t_small), kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, t_small, t_mid, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
+        launched("k_sel_gather<mid>");
+      }
+      if (sel_t > t_mid) {
+        k_sel_gather<kSelSplMax, kSelCmax><<<(unsigned)(sel_t - t_mid), kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, t_mid, sel_t, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
         launched("k_sel_gather");
       }
       phase_end(ph_);
@@ -2686,20 +2762,20 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
   if (forked) check_hip(hipStreamWaitEvent(stream_, join_, 0), "join side stream");
   // The multi-kernel path runs when a segment needs it: one that neither the
   // selection nor the one-block path took, or a selection fallback.
-  bool sort_path = (int64_t)(n_small + hsel.size()) < nseg;
-  if (!hsel.empty() && !sort_path) {
+  bool sort_path = P.n_mode0 > 0;
+  if (nsel > 0 && !sort_path) {
     check_hip(hipStreamSynchronize(stream_), "sync fallbacks");
     sort_path = h_fallbacks_ > 0;
   }
   if (sort_path) {
     auto t0 = std::chrono::steady_clock::now();
-    sort_values(segs, d_v, total_v);
+    sort_values(nseg, d_v, total_v);
     host_phase("sort_values",
                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-    if (!eseg.empty()) {
-      int64_t chunks = std::min<int64_t>((max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
-      for (size_t b0 = 0; b0 < eseg.size(); b0 += 65535) {
-        unsigned ny = (unsigned)std::min<size_t>(65535, eseg.size() - b0);
+    if (P.n_eseg > 0) {
+      int64_t chunks = std::min<int64_t>((P.max_e_items + kBlock * 4 - 1) / (kBlock * 4), 65535);
+      for (int64_t b0 = 0; b0 < P.n_eseg; b0 += 65535) {
+        unsigned ny = (unsigned)std::min<int64_t>(65535, P.n_eseg - b0);
         dim3 grid((unsigned)std::max<int64_t>(chunks, 1), ny);
         {
           int ph_ = phase_begin("merge_e", 16.0 * (double)total_x);
@@ -2753,13 +2829,375 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     phase_end(ph_);
   }
   check_hip(hipGetLastError(), "kernel launch");
-  // The call ends synchronized: host staging read by the asynchronous copies
-  // above (segment table, E segment list) is then free to change.
-  if (sync_results)
-    check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
-  check_hip(hipStreamSynchronize(stream_), "sync results");
-  if (sync_results)
-    for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : (r.path == 2 ? 2 : 1)]++;
+}
+
+
+// ----------------------------------------------- grouped plan on the device
+// Plans a whole grouped batch on the device: every touched key's B-semantics
+// prepare (tdigest.mojo:236-296 per key, as grouped.mojo:99-157 runs it),
+// its segment descriptor, one-block size class or selection layout, and the
+// prefix offsets of the compress buffers.  Three launches, one thread per
+// key: per-block sums of every counted quantity, one block scanning those
+// sums, then a rewrite pass in which each key takes its block's base plus
+// its exclusive rank inside the block (segments and lists keep key order, as
+// the host plan does).
+constexpr int kPlanThreads = 256;
+constexpr double kPlanRenorm = 400.0;  // tdigest.mojo:43
+constexpr double kPlanPrune = 1e-6;    // tdigest.mojo:47
+
+struct PlanKey {
+  double sf, stored, nrm, floor, logs, clock, factor;
+  bool renorm;
+};
+
+// Per-key prepare of one batch of m new values (the host rule of the grouped
+// add: advance the clock by m, decay exponent, renormalize, scale factor,
+// stored total, normalizer, prune floor).
+__device__ inline PlanKey plan_key(const GroupedPlanArgs& a, int64_t k, int64_t m) {
+  PlanKey p;
+  const bool decay = a.lambda > 0.0;
+  double clock = a.clock[k], logs = a.logs[k], total = a.total[k];
+  const double nc = clock + (double)m;
+  p.renorm = false;
+  p.factor = 1.0;
+  if (decay && nc > clock) {
+    logs += a.lambda * (nc - clock);
+    if (logs > kPlanRenorm) {
+      p.factor = kd_exp(-logs);
+      p.renorm = true;
+      total *= p.factor;
+      logs = 0.0;
+    }
+  }
+  p.clock = nc;
+  p.sf = decay ? kd_exp(logs) : 1.0;
+  total += (double)m * p.sf;
+  const double effn = decay ? total * kd_exp(-logs) : total;
+  p.nrm = kd_normalizer(a.scale, a.compression, effn);
+  p.floor = decay ? kPlanPrune * kd_exp(logs) : 0.0;
+  p.logs = logs;
+  p.stored = total;
+  return p;
+}
+
+// Quantities counted per key, summed per block and scanned.
+enum PlanQ {
+  kQSeg, kQE, kQX, kQT, kQESeg, kQMode0,
+  kQSmall0,                    // one-block classes 0..4
+  kQSel0 = kQSmall0 + 5,       // selection segments per footprint class 0..2
+  kQTile0 = kQSel0 + 3,        // selection tiles per class
+  kQB0 = kQTile0 + 3,          // bucket entries per class
+  kQC0 = kQB0 + 3,             // cluster entries per class
+  kQM0 = kQC0 + 3,             // mark entries per class
+  kQErr = kQM0 + 3,            // digests with too many prior centroids
+  kQCount
+};
+
+struct PlanCfg {
+  int fused, sel_on;
+  int64_t tile_min, tiles_max;
+};
+
+// The quantities of key k (zero when the batch does not touch it); 0 small,
+// 1 selection, 2 multi-kernel path in *path; class in *cls.
+__device__ inline void plan_quant(const GroupedPlanArgs& a, const PlanCfg& c, int64_t k,
+                                  int64_t (&q)[kQCount], PlanKey* pk, int* path, int* cls,
+                                  double* selv) {
+#pragma unroll
+  for (int i = 0; i < kQCount; i++) q[i] = 0;
+  *path = -1;
+  *selv = 0.0;
+  if (k >= a.num_keys) return;
+  const int64_t m = a.kcnt[k];
+  if (m == 0) return;
+  const int64_t e = a.off[k + 1] - a.off[k], n = m + e;
+  *pk = plan_key(a, k, m);
+  q[kQErr] = e > kECap ? 1 : 0;
+  q[kQSeg] = 1;
+  q[kQE] = e;
+  q[kQX] = n;
+  q[kQT] = (n + kTile - 1) / kTile;
+  q[kQESeg] = e > 0 ? 1 : 0;
+  if (c.fused && n <= kCap) {
+    *cls = n <= 256 ? 0 : n <= 512 ? 1 : n <= 1024 ? 2 : n <= 2048 ? 3 : 4;
+    q[kQSmall0 + *cls] = 1;
+    *path = 0;
+  } else if (c.sel_on && m > kCap && m <= kSelMaxV && e + 64 <= kSelSplMax) {
+    const int cs = sel_cap_s(m, e), cc = sel_cap_c(a.scale, pk->nrm);
+    const int cl = sel_class(cs, cc);
+    const int64_t tl = sel_tile_len(m, c.tile_min, c.tiles_max);
+    *cls = cl;
+    q[kQSel0 + cl] = 1;
+    q[kQTile0 + cl] = (m + tl - 1) / tl;
+    q[kQB0 + cl] = cs + 2;
+    q[kQC0 + cl] = cc + 1;
+    q[kQM0 + cl] = 2 * (int64_t)cc + 2;
+    *selv = (double)m;
+    *path = 1;
+  } else {
+    q[kQMode0] = 1;
+    *path = 2;
+  }
+}
+
+// Exclusive block scan of one int64 per thread (kPlanThreads threads).
+__device__ inline int64_t plan_block_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[kPlanThreads / 64 + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int64_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t run = 0;
+    for (int i = 0; i < kPlanThreads / 64; i++) {
+      int64_t t = wsum[i];
+      wsum[i] = run;
+      run += t;
+    }
+    wsum[kPlanThreads / 64] = run;
+  }
+  __syncthreads();
+  int64_t r = inc - v + wsum[w];
+  *total = wsum[kPlanThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+// Block partials: part[b * kQCount + q] sums; pmax[2 b], pmax[2 b + 1]: largest
+// merged size of a digest with prior centroids, largest end of new values;
+// pselv[b]: new values on the selection path.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_count(GroupedPlanArgs a, PlanCfg c,
+                                                             int64_t* part, int64_t* pmax,
+                                                             double* pselv) {
+  const int64_t k = (int64_t)blockIdx.x * kPlanThreads + threadIdx.x;
+  int64_t q[kQCount];
+  PlanKey pk;
+  int path, cls;
+  double selv;
+  plan_quant(a, c, k, q, &pk, &path, &cls, &selv);
+  int64_t me = 0, mv = 0;
+  if (path >= 0) {
+    const int64_t m = a.kcnt[k], e = a.off[k + 1] - a.off[k];
+    me = e > 0 ? m + e : 0;
+    mv = a.kof[k] + m;
+  }
+  __shared__ double sv[kPlanThreads];
+  __shared__ int64_t sm[2][kPlanThreads];
+  for (int i = 0; i < kQCount; i++) {
+    int64_t tot;
+    plan_block_scan(q[i], &tot);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * kQCount + i] = tot;
+  }
+  sv[threadIdx.x] = selv;
+  sm[0][threadIdx.x] = me;
+  sm[1][threadIdx.x] = mv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    int64_t a0 = 0, a1 = 0;
+    for (int i = 0; i < kPlanThreads; i++) {
+      s += sv[i];
+      a0 = sm[0][i] > a0 ? sm[0][i] : a0;
+      a1 = sm[1][i] > a1 ? sm[1][i] : a1;
+    }
+    pselv[blockIdx.x] = s;
+    pmax[2 * blockIdx.x] = a0;
+    pmax[2 * blockIdx.x + 1] = a1;
+  }
+}
+
+// One block: exclusive scans of the block partials into per-block bases
+// (class lists offset by the classes before them), and the summary.
+__global__ __launch_bounds__(1024) void k_plan_top(int64_t nb, const int* bad, int64_t* part,
+                                                   const int64_t* pmax, const double* pselv,
+                                                   PlanSummary* out) {
+  __shared__ int64_t tot[kQCount];
+  // Column q of part: block sums -> exclusive prefixes (in place).
+  for (int q = threadIdx.x; q < kQCount; q += blockDim.x) {
+    int64_t run = 0;
+    for (int64_t b = 0; b < nb; b++) {
+      int64_t v = part[b * kQCount + q];
+      part[b * kQCount + q] = run;
+      run += v;
+    }
+    tot[q] = run;
+  }
+  __syncthreads();
+  // Class bases: small lists largest class first; selection classes 0, 1, 2
+  // with their tile / bucket / cluster / mark offsets in the same order.
+  __shared__ int64_t cbase[kQCount];
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < kQCount; q++) cbase[q] = 0;
+    int64_t b = 0;
+    for (int c = 4; c >= 0; c--) {
+      cbase[kQSmall0 + c] = b;
+      b += tot[kQSmall0 + c];
+    }
+    const int groups[5] = {kQSel0, kQTile0, kQB0, kQC0, kQM0};
+    for (int g = 0; g < 5; g++) {
+      int64_t r = 0;
+      for (int c = 0; c < 3; c++) {
+        cbase[groups[g] + c] = r;
+        r += tot[groups[g] + c];
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < nb * kQCount; i += blockDim.x) part[i] += cbase[i % kQCount];
+  if (threadIdx.x == 0) {
+    PlanSummary P{};
+    P.bad = (bad && *bad) ? 1 : (tot[kQErr] > 0 ? 2 : 0);
+    P.nseg = tot[kQSeg];
+    P.nsel = tot[kQSel0] + tot[kQSel0 + 1] + tot[kQSel0 + 2];
+    P.n_mode0 = tot[kQMode0];
+    P.n_eseg = tot[kQESeg];
+    for (int c = 0; c < 5; c++) P.n_small[c] = tot[kQSmall0 + c];
+    P.t_small = tot[kQTile0];
+    P.t_mid = tot[kQTile0] + tot[kQTile0 + 1];
+    P.sel_t = P.t_mid + tot[kQTile0 + 2];
+    P.sel_b = tot[kQB0] + tot[kQB0 + 1] + tot[kQB0 + 2];
+    P.sel_c = tot[kQC0] + tot[kQC0 + 1] + tot[kQC0 + 2];
+    P.sel_m = tot[kQM0] + tot[kQM0 + 1] + tot[kQM0 + 2];
+    P.total_e = tot[kQE];
+    P.total_x = tot[kQX];
+    P.total_t = tot[kQT];
+    int64_t me = 0, mv = 0;
+    double sv = 0.0;
+    for (int64_t b = 0; b < nb; b++) {
+      me = pmax[2 * b] > me ? pmax[2 * b] : me;
+      mv = pmax[2 * b + 1] > mv ? pmax[2 * b + 1] : mv;
+      sv += pselv[b];
+    }
+    P.max_e_items = me;
+    P.total_v = mv;
+    P.sel_v = sv;
+    *out = P;
+  }
+}
+
+// Rewrite: every touched key's state update, segment descriptor and list
+// entries at its block base plus its rank inside the block.  Nothing happens
+// when the batch failed validation.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_write(GroupedPlanArgs a, PlanCfg c,
+                                                             const int64_t* base,
+                                                             const PlanSummary* sum, SegDesc* segs,
+                                                             int* small_list, SelSeg* sel,
+                                                             int* eseg) {
+  if (sum->bad) return;
+  const int64_t k = (int64_t)blockIdx.x * kPlanThreads + threadIdx.x;
+  int64_t q[kQCount];
+  PlanKey pk;
+  int path, cls = 0;
+  double selv;
+  plan_quant(a, c, k, q, &pk, &path, &cls, &selv);
+  int64_t o[kQCount];
+  const int64_t* bb = base + (int64_t)blockIdx.x * kQCount;
+  for (int i = 0; i < kQCount; i++) {
+    int64_t tot;
+    o[i] = bb[i] + plan_block_scan(q[i], &tot);
+  }
+  if (k >= a.num_keys) return;
+  if (path < 0) {
+    a.seg_of[k] = -1;
+    return;
+  }
+  const int64_t m = a.kcnt[k], e = a.off[k + 1] - a.off[k];
+  if (pk.renorm)  // _renormalize (tdigest.mojo:359-371) of the prior centroids
+    for (int64_t i = a.off[k]; i < a.off[k + 1]; i++) a.w[i] *= pk.factor;
+  a.clock[k] = pk.clock;
+  a.logs[k] = pk.logs;
+  a.total[k] = pk.stored;
+  const int64_t seg = o[kQSeg];
+  SegDesc sd{};
+  sd.v_off = a.kof[k];
+  sd.v_cnt = m;
+  sd.e_off = a.off[k];
+  sd.e_cnt = e;
+  sd.x_off = o[kQX];
+  sd.t_off = o[kQT];
+  sd.es_off = o[kQE];
+  sd.sf = pk.sf;
+  sd.stored_n = pk.stored;
+  sd.nrm = pk.nrm;
+  sd.floor = pk.floor;
+  sd.mode = 0;
+  sd.sel = -1;
+  if (path == 0) {
+    sd.mode = 2;
+    small_list[o[kQSmall0 + cls]] = (int)seg;
+  } else if (path == 1) {
+    SelSeg g{};
+    g.si = (int32_t)seg;
+    g.cap_s = (int)(q[kQB0 + cls] - 2);
+    g.cap_c = (int)(q[kQC0 + cls] - 1);
+    g.tile_len = sel_tile_len(m, c.tile_min, c.tiles_max);
+    g.b_off = o[kQB0 + cls];
+    g.c_off = o[kQC0 + cls];
+    g.m_off = o[kQM0 + cls];
+    g.tile0 = o[kQTile0 + cls];
+    sd.mode = 1;
+    sd.sel = (int32_t)o[kQSel0 + cls];
+    sel[o[kQSel0 + cls]] = g;
+  }
+  segs[seg] = sd;
+  a.seg_key[seg] = k;
+  a.seg_of[k] = seg;
+  if (e > 0) eseg[o[kQESeg]] = (int)seg;
+}
+
+PlanSummary Engine::compress_grouped(const GroupedPlanArgs& a, const double* d_v,
+                                     const double* d_em, const double* d_ew) {
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  const int64_t max_seg = std::max<int64_t>(a.num_keys, 1);
+  SegDesc* dsegs = as<SegDesc>(seg_, max_seg);
+  int* dl = as<int>(small_list_, max_seg);
+  int* deseg = as<int>(order_, max_seg);
+  SelSeg* dsel = as<SelSeg>(sel_seg_, max_seg);
+  const int64_t nb = (a.num_keys + kPlanThreads - 1) / kPlanThreads;
+  int64_t* part = as<int64_t>(plan_part_, nb * kQCount + 3 * nb + 1);
+  int64_t* pmax = part + nb * kQCount;
+  double* pselv = reinterpret_cast<double*>(pmax + 2 * nb);
+  PlanSummary* dsum = as<PlanSummary>(plan_sum_, 1);
+  if (!h_plan_) {
+    void* p = nullptr;
+    check_hip(hipHostMalloc(&p, sizeof(PlanSummary), hipHostMallocDefault), "hipHostMalloc");
+    h_plan_ = static_cast<PlanSummary*>(p);
+  }
+  const PlanCfg cfg{fused_on_ ? 1 : 0, sel_on_ ? 1 : 0, sel_tile_min_, sel_tiles_max_};
+  {
+    int ph_ = phase_begin("plan", 0.0);
+    k_plan_count<<<(unsigned)nb, kPlanThreads, 0, stream_>>>(a, cfg, part, pmax, pselv);
+    launched("k_plan_count");
+    k_plan_top<<<1, 1024, 0, stream_>>>(nb, a.bad, part, pmax, pselv, dsum);
+    launched("k_plan_top");
+    k_plan_write<<<(unsigned)nb, kPlanThreads, 0, stream_>>>(a, cfg, part, dsum, dsegs, dl, dsel,
+                                                             deseg);
+    launched("k_plan_write");
+    phase_end(ph_);
+  }
+  check_hip(hipMemcpyAsync(h_plan_, dsum, sizeof(PlanSummary), hipMemcpyDeviceToHost, stream_),
+            "D2H plan");
+  check_hip(hipStreamSynchronize(stream_), "plan sync");
+  const PlanSummary P = *h_plan_;
+  if (P.bad || P.nseg == 0) return P;
+  execute(a.scale, P, d_v, d_em, d_ew);
+  // execute() reads the fallback count back only when it decides the sort
+  // path from it; otherwise it is still in flight.
+  if (P.nsel > 0 && P.n_mode0 > 0) check_hip(hipStreamSynchronize(stream_), "fallback sync");
+  int64_t n_small = 0;
+  for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
+  path_counts_[2] += n_small;
+  path_counts_[0] += P.nsel;
+  path_counts_[1] += P.n_mode0;
+  if (P.nsel > 0 && h_fallbacks_ > 0) {  // read by execute after its fallback sync
+    path_counts_[0] -= h_fallbacks_;
+    path_counts_[1] += h_fallbacks_;
+  }
+  return P;
 }
 
 // ------------------------------------------------------ key partition
