@@ -916,30 +916,66 @@ __global__ void k_grouped_counts(int64_t nk, const int64_t* off, const int64_t* 
   cnt[k] = sg >= 0 ? res[sg].count : off[k + 1] - off[k];
 }
 
-// Exclusive scan of cnt[0..n) into off[0..n] (one block; off[n] = total).
-__global__ __launch_bounds__(1024) void k_scan_offsets(const int64_t* cnt, int64_t n, int64_t* off) {
-  __shared__ int64_t part[1024];
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t lo = std::min<int64_t>(n, threadIdx.x * per), hi = std::min<int64_t>(n, lo + per);
-  int64_t sum = 0;
-  for (int64_t i = lo; i < hi; i++) sum += cnt[i];
-  part[threadIdx.x] = sum;
+// Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total) in three
+// launches: block sums of 1024-entry chunks, one block scanning those (at
+// most 1024 * 1024 entries), then every chunk scanned from its base.
+constexpr int kOffChunk = 1024;
+
+__device__ inline int64_t block_scan_1024(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[17];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int64_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t run = 0;
-    for (int i = 0; i < 1024; i++) {
-      int64_t v = part[i];
-      part[i] = run;
-      run += v;
+    for (int i = 0; i < 16; i++) {
+      int64_t t = wsum[i];
+      wsum[i] = run;
+      run += t;
     }
-    off[n] = run;
+    wsum[16] = run;
   }
   __syncthreads();
-  int64_t run = part[threadIdx.x];
+  int64_t r = inc - v + wsum[w];
+  *total = wsum[16];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void k_off_sums(const int64_t* cnt, int64_t n, int64_t* bsum) {
+  const int64_t i = (int64_t)blockIdx.x * kOffChunk + threadIdx.x;
+  int64_t tot;
+  block_scan_1024(i < n ? cnt[i] : 0, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Block sums scanned in place; each thread owns a contiguous run of them.
+__global__ __launch_bounds__(1024) void k_off_top(int64_t* bsum, int64_t nb, int64_t* total) {
+  const int64_t per = (nb + 1023) / 1024;
+  const int64_t lo = std::min<int64_t>(nb, threadIdx.x * per), hi = std::min<int64_t>(nb, lo + per);
+  int64_t v = 0;
+  for (int64_t i = lo; i < hi; i++) v += bsum[i];
+  int64_t tot;
+  int64_t run = block_scan_1024(v, &tot);
   for (int64_t i = lo; i < hi; i++) {
-    off[i] = run;
-    run += cnt[i];
+    int64_t x = bsum[i];
+    bsum[i] = run;
+    run += x;
   }
+  if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_off_down(const int64_t* cnt, int64_t n, const int64_t* bsum,
+                                                   int64_t* off) {
+  const int64_t i = (int64_t)blockIdx.x * kOffChunk + threadIdx.x;
+  int64_t tot;
+  int64_t e = block_scan_1024(i < n ? cnt[i] : 0, &tot);
+  if (i < n) off[i] = bsum[blockIdx.x] + e;
 }
 
 // New CSR: one wave per key copies its centroids (the engine's output for a
@@ -1062,23 +1098,29 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   k_grouped_counts<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dso, res, dc);
   launched("k_grouped_counts");
   int64_t* dno = static_cast<int64_t*>(noff.reserve((num_keys + 1) * sizeof(int64_t)));
-  k_scan_offsets<<<1, 1024, 0, s->st()>>>(dc, num_keys, dno);
-  launched("k_scan_offsets");
-  // A compress never grows a digest: the new total is at most the old one
-  // with every touched key's prior centroids replaced by its merged records.
-  const int64_t bound = std::max<int64_t>(total_cent - P.total_e + P.total_x, 1);
-  double* dnm = static_cast<double*>(ncm.reserve(bound * sizeof(double)));
-  double* dnw = static_cast<double*>(ncw.reserve(bound * sizeof(double)));
+  const int64_t nb = (num_keys + kOffChunk - 1) / kOffChunk;
+  int64_t* bsum = static_cast<int64_t*>(tot.reserve((nb + 1) * sizeof(int64_t)));
+  k_off_sums<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum);
+  launched("k_off_sums");
+  k_off_top<<<1, kOffChunk, 0, s->st()>>>(bsum, nb, dno + num_keys);
+  launched("k_off_top");
+  k_off_down<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum, dno);
+  launched("k_off_down");
+  // The exact new total sizes the next CSR.
+  s->d2h(h_tot, dno + num_keys, sizeof(int64_t));
+  s->sync();
+  const int64_t new_total = *h_tot;
+  double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
+  double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
   k_gather_keys<<<blocks_for(num_keys * 64), 256, 0, s->st()>>>(
       num_keys, doff(), dm(), dw(), dso, segs, s->eng.cent_means(), s->eng.cent_weights(), dno,
       dnm, dnw);
   launched("k_gather_keys");
-  s->d2h(h_tot, dno + num_keys, sizeof(int64_t));
   s->sync();
   off.swap(noff);
   cm.swap(ncm);
   cw.swap(ncw);
-  total_cent = *h_tot;
+  total_cent = new_total;
   dirty = true;
   prefix_valid = false;
   s->eng.host_phase("csr", ms_since(tp));

@@ -2619,7 +2619,49 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   } else {
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
   }
-  // Small segments first; on the side stream (set_overlap) they overlap the
+  const int nsel = (int)P.nsel;
+  const int64_t sel_b = P.sel_b, sel_c = P.sel_c, sel_m = P.sel_m;
+  const int64_t t_small = P.t_small, t_mid = P.t_mid, sel_t = P.sel_t;
+  SelSeg* dsel = static_cast<SelSeg*>(sel_seg_.p);
+  SelState* dst = nullptr;
+  double *dspl = nullptr, *dcsl = nullptr;
+  unsigned int *dcnt = nullptr, *dbst = nullptr, *dmlc = nullptr;
+  int *dinfo = nullptr, *dce = nullptr;
+  SelBound* dbnd = nullptr;
+  SelMark* dmk = nullptr;
+  int2* dml = nullptr;
+  uint16_t* dbix = nullptr;
+  if (nsel > 0) {
+    dst = as<SelState>(sel_st_, nsel);
+    dspl = as<double>(sel_spl_, sel_b);
+    dcnt = as<unsigned int>(sel_cnt_, sel_b);
+    dbst = as<unsigned int>(sel_bst_, sel_b);
+    dinfo = as<int>(sel_info_, sel_b);
+    dbnd = as<SelBound>(sel_bnd_, sel_c);
+    dce = as<int>(sel_ce_, sel_c);
+    dcsl = as<double>(sel_csl_, 2 * sel_c);
+    dmk = as<SelMark>(sel_mk_, sel_m);
+    dml = as<int2>(sel_ml_, sel_m);
+    dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
+    dmlc = as<unsigned int>(sel_mlc_, 2);  // gathered-bucket list size, fallbacks
+    check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
+    check_hip(hipMemsetAsync(dmlc, 0, 2 * sizeof(unsigned int), stream_), "memset sel list");
+    k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
+    launched("k_sel_init");
+    {
+      int ph_ = phase_begin("sel_sample", 0.0);
+      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      launched("k_sel_sample<1024>");
+      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      launched("k_sel_sample<4096>");
+      k_sel_sample<kSelSample, kBlock, 4096><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
+                                                                          dst);
+      launched("k_sel_sample<8192>");
+      phase_end(ph_);
+    }
+  }
+  // Small segments after the selection samples (whose blocks then start
+  // first); on the side stream (set_overlap) they overlap the rest of the
   // selection path.
   bool forked = false;
   int64_t n_small = 0;
@@ -2656,38 +2698,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     phase_end(ph_);
     if (forked) check_hip(hipEventRecord(join_, side_), "join");
   }
-  const int nsel = (int)P.nsel;
   if (nsel > 0) {
-    const int64_t sel_b = P.sel_b, sel_c = P.sel_c, sel_m = P.sel_m;
-    const int64_t t_small = P.t_small, t_mid = P.t_mid, sel_t = P.sel_t;
-    SelSeg* dsel = static_cast<SelSeg*>(sel_seg_.p);
-    SelState* dst = as<SelState>(sel_st_, nsel);
-    double* dspl = as<double>(sel_spl_, sel_b);
-    unsigned int* dcnt = as<unsigned int>(sel_cnt_, sel_b);
-    unsigned int* dbst = as<unsigned int>(sel_bst_, sel_b);
-    int* dinfo = as<int>(sel_info_, sel_b);
-    SelBound* dbnd = as<SelBound>(sel_bnd_, sel_c);
-    int* dce = as<int>(sel_ce_, sel_c);
-    double* dcsl = as<double>(sel_csl_, 2 * sel_c);
-    SelMark* dmk = as<SelMark>(sel_mk_, sel_m);
-    int2* dml = as<int2>(sel_ml_, sel_m);
-    uint16_t* dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
-    unsigned int* dmlc = as<unsigned int>(sel_mlc_, 2);  // gathered-bucket list size, fallbacks
-    check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
-    check_hip(hipMemsetAsync(dmlc, 0, 2 * sizeof(unsigned int), stream_), "memset sel list");
-    k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
-    launched("k_sel_init");
-    {
-      int ph_ = phase_begin("sel_sample", 0.0);
-      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
-      launched("k_sel_sample<1024>");
-      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
-      launched("k_sel_sample<4096>");
-      k_sel_sample<kSelSample, kBlock, 4096><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
-                                                                          dst);
-      launched("k_sel_sample<8192>");
-      phase_end(ph_);
-    }
     const double sel_bytes = 8.0 * P.sel_v;
     {
       int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
@@ -3016,15 +3027,23 @@ __global__ __launch_bounds__(1024) void k_plan_top(int64_t nb, const int* bad, i
                                                    const int64_t* pmax, const double* pselv,
                                                    PlanSummary* out) {
   __shared__ int64_t tot[kQCount];
-  // Column q of part: block sums -> exclusive prefixes (in place).
-  for (int q = threadIdx.x; q < kQCount; q += blockDim.x) {
+  // Column q of part: block sums -> exclusive prefixes (in place), one wave
+  // per column, 64 blocks at a time.
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x >> 6; q < kQCount; q += blockDim.x >> 6) {
     int64_t run = 0;
-    for (int64_t b = 0; b < nb; b++) {
-      int64_t v = part[b * kQCount + q];
-      part[b * kQCount + q] = run;
-      run += v;
+    for (int64_t b0 = 0; b0 < nb; b0 += 64) {
+      const int64_t b = b0 + lane;
+      const int64_t v = b < nb ? part[b * kQCount + q] : 0;
+      int64_t inc = v;
+      for (int d = 1; d < 64; d <<= 1) {
+        int64_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+      }
+      if (b < nb) part[b * kQCount + q] = run + inc - v;
+      run += __shfl(inc, 63, 64);
     }
-    tot[q] = run;
+    if (lane == 0) tot[q] = run;
   }
   __syncthreads();
   // Class bases: small lists largest class first; selection classes 0, 1, 2
