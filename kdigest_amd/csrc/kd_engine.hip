@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 
 #include "kd_math.h"
 
@@ -1990,6 +1991,116 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sorted(const SegDesc* segs, const
   }
 }
 
+// Gathered buckets of at most 64 * PER values (and more than LO), one wave
+// each: the values sorted in registers by a bitonic network (lane-crossing
+// stages through shuffles, no LDS and no block barrier), then the same
+// outputs as k_sel_sorted: the bucket's sum and extremes, and for every
+// boundary rank inside it the values at that rank and just below and the sum
+// of the values below (double-double, butterfly over the wave).
+__device__ inline dd wave_dd_sum(dd v) {
+  for (int m = 32; m > 0; m >>= 1) {
+    dd o{__shfl_xor(v.hi, m, 64), __shfl_xor(v.lo, m, 64)};
+    v = dd_add(v, o);
+  }
+  return v;
+}
+
+template <int PER, int LO>
+__global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, const SelSeg* sel,
+                                                         const SelState* st, const int2* mlist,
+                                                         const unsigned int* mlist_cnt, SelMark* mk,
+                                                         SelBound* bnd, const double* X) {
+  const unsigned int total = *mlist_cnt;
+  const int lane = threadIdx.x & 63;
+  const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+  const double inf = __builtin_inf();
+  for (unsigned int gi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gi < total;
+       gi += waves) {
+    const int2 e = mlist[gi];
+    const int j = e.x;
+    if (st[j].fail) continue;
+    const SelSeg g = sel[j];
+    SelMark* mp = mk + g.m_off + e.y;
+    const int64_t n = mp->cnt;
+    if (n <= LO || n > 64 * PER) continue;
+    const SegDesc sd = segs[g.si];
+    const double* src = X + sd.x_off + mp->goff;
+    // Element lane * PER + p sits in v[p].
+    double v[PER];
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+      const int idx = lane * PER + p;
+      v[p] = idx < n ? src[idx] : inf;
+    }
+    for (int k = 2; k <= 64 * PER; k <<= 1) {
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        if (jj < PER) {
+#pragma unroll
+          for (int p = 0; p < PER; p++) {
+            const int q = p ^ jj;
+            if (q > p) {
+              const int idx = lane * PER + p;
+              const bool asc = (idx & k) == 0;
+              const double lo = __builtin_fmin(v[p], v[q]), hi = __builtin_fmax(v[p], v[q]);
+              v[p] = asc ? lo : hi;
+              v[q] = asc ? hi : lo;
+            }
+          }
+        } else {
+          const int lm = jj / PER;
+#pragma unroll
+          for (int p = 0; p < PER; p++) {
+            const int idx = lane * PER + p;
+            const double o = __shfl_xor(v[p], lm, 64);
+            const bool asc = (idx & k) == 0, lower = (idx & jj) == 0;
+            const double lo = __builtin_fmin(v[p], o), hi = __builtin_fmax(v[p], o);
+            v[p] = (asc == lower) ? lo : hi;
+          }
+        }
+      }
+    }
+    dd acc{0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < PER; p++)
+      if (lane * PER + p < n) acc = dd_add(acc, dd{v[p], 0.0});
+    const dd tot = wave_dd_sum(acc);
+    // Element r of the sorted bucket (every lane gets it).
+    auto elem = [&](int r) {
+      double x = v[0];
+#pragma unroll
+      for (int p = 1; p < PER; p++)
+        if ((r % PER) == p) x = v[p];
+      return __shfl(x, r / PER, 64);
+    };
+    const double vmin = elem(0), vmax = elem((int)n - 1);
+    if (lane == 0) {
+      mp->sum_hi = tot.hi;
+      mp->sum_lo = tot.lo;
+      mp->vmin = vmin;
+      mp->vmax = vmax;
+    }
+    const int C = st[j].ncl;
+    SelBound* bd = bnd + g.c_off;
+    const int64_t vstart = mp->vstart;
+    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
+      const int rho = (int)(bd[i].vr - vstart);
+      dd sl{0.0, 0.0};
+#pragma unroll
+      for (int p = 0; p < PER; p++)
+        if (lane * PER + p < rho) sl = dd_add(sl, dd{v[p], 0.0});
+      const dd sls = wave_dd_sum(sl);
+      const double xh = elem(rho), xl = rho > 0 ? elem(rho - 1) : 0.0;
+      if (lane == 0) {
+        bd[i].x_hi = xh;
+        bd[i].x_lo = xl;
+        bd[i].dup = rho > 0 && xl == xh;
+        bd[i].sl_hi = sls.hi;
+        bd[i].sl_lo = sls.lo;
+      }
+    }
+  }
+}
+
 // Gathered buckets of more than LO values: radix select on order keys (eight
 // bits per pass) for each boundary rank inside, from LDS when the bucket fits
 // there, else from memory.
@@ -2748,7 +2859,9 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     {
       int ph_ = phase_begin("sel_select", 0.0);
       unsigned grid = (unsigned)std::min<int64_t>(sel_m, 4096);
-      k_sel_sorted<1024, 64, 0><<<grid, 64, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
+      k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
+      launched("k_sel_sorted_wave<128>");
+      k_sel_sorted<1024, 64, 128><<<grid, 64, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
       launched("k_sel_sorted<1024>");
       k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk,
                                                                      dbnd, x);
@@ -3414,7 +3527,8 @@ __device__ inline void rx_scan_bins(const uint32_t* cnt, uint32_t* out, int R) {
 
 // Digit counts of one chunk, one histogram per wave (so a hot key's digit
 // contends only within its wave).
-__global__ __launch_bounds__(kRxThreads) void k_rx_hist(const uint32_t* keys,
+template <typename KT>
+__global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
                                                         const uint32_t* seg_start,
                                                         const uint32_t* tile_start, int nseg,
                                                         uint32_t chunk, int shift, int bits,
@@ -3528,15 +3642,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const u
 // sub-tile's loads are issued before the current one is written out.  With
 // kValidate (first pass) a key at or above key_limit or a non-finite value
 // sets *bad (grouped.mojo:91-97); positions stay in bounds either way because
-// only the key's digit bits place it.
-template <int ITEMS, bool kWriteKeys, bool kValidate>
-__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys, const double* vals,
+// only the key's digit bits place it.  With kWriteKeys (first of two passes)
+// only the next pass's digit (key & out_mask, at most 16 bits) is written
+// beside each value.
+template <int ITEMS, bool kWriteKeys, bool kValidate, typename KT>
+__global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const double* vals,
                                                            const uint32_t* seg_start,
                                                            const uint32_t* tile_start, int nseg,
                                                            uint32_t chunk, int shift, int bits,
-                                                           const uint32_t* dst, uint32_t* keys_out,
-                                                           double* vals_out, uint32_t key_limit,
-                                                           int* bad) {
+                                                           const uint32_t* dst, uint16_t* keys_out,
+                                                           uint32_t out_mask, double* vals_out,
+                                                           uint32_t key_limit, int* bad) {
   constexpr int kSub = kRxThreads * ITEMS;
   constexpr int kPerWave = ITEMS * 64;
   constexpr int kPm = kRxWaves * kRxBins;
@@ -3639,7 +3755,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const uint32_t* keys,
       uint32_t d = (k >> shift) & mask;
       uint32_t o = run[d] + ((uint32_t)i - lc[d]);
       vals_out[o] = sv[i];
-      if (kWriteKeys) keys_out[o] = k;
+      if (kWriteKeys) keys_out[o] = (uint16_t)(k & out_mask);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < R; d += kRxThreads) {
@@ -3699,35 +3815,42 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   k_rx_init<<<1, 1, 0, stream_>>>(segA, (uint32_t)n);
   launched("k_rx_init");
 
-  auto scatter = [&](bool first, bool last, unsigned grid, const uint32_t* kin, const double* vin,
-                     const uint32_t* seg, int nseg, int shift, int dbits, uint32_t* kout,
+  const uint32_t low_mask = (1u << lb) - 1u;
+  auto scatter = [&](bool first, bool last, unsigned grid, const auto* kin, const double* vin,
+                     const uint32_t* seg, int nseg, int shift, int dbits, uint16_t* kout,
                      double* vout) {
+    using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
     const uint32_t c = (uint32_t)chunk;
-#define KD_RX_LAUNCH(I, WK, VA)                                                                  \
-  k_rx_scatter<I, WK, VA><<<grid, kRxThreads, 0, stream_>>>(kin, vin, seg, tiles, nseg, c, shift, \
-                                                           dbits, mat, kout, vout, klim, d_bad)
+#define KD_RX_LAUNCH(I, WK, VA)                                                                    \
+  k_rx_scatter<I, WK, VA, KT><<<grid, kRxThreads, 0, stream_>>>(kin, vin, seg, tiles, nseg, c,      \
+                                                               shift, dbits, mat, kout, low_mask, \
+                                                               vout, klim, d_bad)
 #define KD_RX_ITEMS(WK, VA)            \
   if (items == 4) KD_RX_LAUNCH(4, WK, VA);       \
   else if (items == 16) KD_RX_LAUNCH(16, WK, VA); \
   else KD_RX_LAUNCH(8, WK, VA)
     const bool validate = first && d_bad != nullptr;
-    if (!last && validate) { KD_RX_ITEMS(true, true); }
-    else if (!last) { KD_RX_ITEMS(true, false); }
-    else if (validate) { KD_RX_ITEMS(false, true); }
-    else { KD_RX_ITEMS(false, false); }
+    if constexpr (std::is_same<KT, uint32_t>::value) {
+      if (!last && validate) { KD_RX_ITEMS(true, true); }
+      else if (!last) { KD_RX_ITEMS(true, false); }
+      else if (validate) { KD_RX_ITEMS(false, true); }
+      else { KD_RX_ITEMS(false, false); }
+    } else {
+      KD_RX_ITEMS(false, false);
+    }
 #undef KD_RX_ITEMS
 #undef KD_RX_LAUNCH
     launched("k_rx_scatter");
   };
 
-  auto pass = [&](const uint32_t* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
-                  int dbits, int64_t tiles_max, uint32_t* kout, double* vout, uint32_t* seg_out,
+  auto pass = [&](const auto* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
+                  int dbits, int64_t tiles_max, uint16_t* kout, double* vout, uint32_t* seg_out,
                   bool first) {
     const int R = 1 << dbits;
     k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, (uint32_t)chunk, tiles);
     launched("k_rx_tiles");
     {
-      int ph_ = phase_begin("key_hist", 4.0 * (double)n);
+      int ph_ = phase_begin("key_hist", (double)sizeof(*kin) * (double)n);
       k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(kin, seg, tiles, nseg,
                                                                  (uint32_t)chunk, shift, dbits, mat);
       launched("k_rx_hist");
@@ -3741,7 +3864,10 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     k_scan_down<<<(unsigned)nb, kScanThreads, 0, stream_>>>(mat, tiles, nseg, R, bsum);
     launched("k_scan_down");
     {
-      int ph_ = phase_begin("key_scatter", (kout ? 24.0 : 20.0) * (double)n);
+      // Bytes: the key (4 B, or the 2 B digit) and value read; the value and
+      // the next digit (pass 1) written.
+      const double b = (double)sizeof(*kin) + 8.0 + 8.0 + (kout ? 2.0 : 0.0);
+      int ph_ = phase_begin("key_scatter", b * (double)n);
       scatter(first, kout == nullptr, (unsigned)tiles_max, kin, vin, seg, nseg, shift, dbits, kout,
               vout);
       phase_end(ph_);
@@ -3752,12 +3878,13 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   };
 
   if (hb > 0) {
-    uint32_t* k1 = as<uint32_t>(rx_keys_, n);
+    uint16_t* k1 = as<uint16_t>(rx_keys_, n);
     double* v1 = as<double>(rx_vals_, n);
     pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, true);
-    pass(k1, v1, segB, 1 << hb, 0, lb, max_tiles2, nullptr, d_out, segA, false);
+    pass((const uint16_t*)k1, v1, segB, 1 << hb, 0, lb, max_tiles2, (uint16_t*)nullptr, d_out, segA,
+         false);
   } else {
-    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, nullptr, d_out, segB, true);
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, (uint16_t*)nullptr, d_out, segB, true);
     std::swap(segA, segB);
   }
   k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(segA, num_keys, d_offsets,
