@@ -1538,10 +1538,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
                                                            const double* vin, const double* spl,
                                                            SelState* st, unsigned int* cnt,
                                                            uint16_t* bix) {
-  // The search runs on the high 32 bits of each splitter's order key (4-byte
-  // LDS entries: half the bank traffic of the doubles); splitters sharing
-  // those bits with the value are then compared exactly from global memory.
-  __shared__ uint32_t sk[SMAX];
+  // The search runs on 32 bits of each splitter's order key (4-byte LDS
+  // entries: half the bank traffic of the doubles): the bits right after the
+  // prefix that every splitter shares, so that nearly every value differs
+  // from every splitter there.  Values below the first or at or above the
+  // last splitter are placed directly; splitters sharing the 32 bits with a
+  // value are compared exactly (from global memory: rare).
+  __shared__ uint32_t sk[SMAX + 1];
   __shared__ unsigned int lc[SMAX + 1];
   __shared__ unsigned long long smin, smax;
   const int64_t t = t0 + blockIdx.x;
@@ -1555,8 +1558,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   // entries, P a power of two: log2(P) fixed steps without bounds tests.
   int P = 1;
   while (P < S + 1) P <<= 1;
-  for (int i = threadIdx.x; i < P - 1; i += blockDim.x)
-    sk[i] = i < S ? (uint32_t)(ord_key(sg[i]) >> 32) : 0xffffffffu;
+  const double sp0 = S > 0 ? sg[0] : 0.0, spl_last = S > 0 ? sg[S - 1] : 0.0;
+  const unsigned long long kf = ord_key(sp0), kl = ord_key(spl_last);
+  const int sh = kf == kl ? 32 : min(32, __clzll(kf ^ kl));
+  for (int i = threadIdx.x; i < P; i += blockDim.x)
+    sk[i] = i < S ? (uint32_t)((ord_key(sg[i]) << sh) >> 32) : 0xffffffffu;
   for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
   if (threadIdx.x == 0) {
     smin = ~0ull;
@@ -1570,7 +1576,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   uint16_t* bx = bix + sd.v_off;
   // U values per thread searched together (independent LDS reads in
   // flight): bucket = number of splitters <= x, as sel_bucket.
-  constexpr int U = 6;
+  constexpr int U = 5;
   for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * kSelThreads) {
     double x[U];
     uint32_t kx[U];
@@ -1579,7 +1585,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
     for (int u = 0; u < U; u++) {
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       x[u] = i < b ? v[i] : 0.0;
-      kx[u] = (uint32_t)(ord_key(x[u]) >> 32);
+      kx[u] = (uint32_t)((ord_key(x[u]) << sh) >> 32);
       lo[u] = 0;
     }
     // lo = number of splitters whose high key bits are below the value's:
@@ -1588,12 +1594,26 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
 #pragma unroll
       for (int u = 0; u < U; u++) lo[u] += (sk[lo[u] + step - 1] < kx[u]) ? step : 0;
     }
+    // The first splitter not below the value in the searched bits: equal
+    // bits (rare) need the exact comparison.
+    uint32_t nx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) nx[u] = sk[lo[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool below = x[u] < sp0, above = !(x[u] < spl_last);
+      lo[u] = below ? 0 : (above ? S : lo[u]);
+      if (!below && !above && nx[u] == kx[u]) {
+        int k = lo[u];
+        while (k < S && sk[k] == kx[u] && sg[k] <= x[u]) k++;
+        lo[u] = k;
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       if (i < b) {
-        int k = lo[u];
-        while (k < S && sk[k] == kx[u] && sg[k] <= x[u]) k++;
+        const int k = lo[u];
         atomicAdd(&lc[k], 1u);
         bx[i] = (uint16_t)k;  // the gather pass reads it back instead of searching again
         unsigned long long kk = ord_key(x[u]);
