@@ -342,30 +342,41 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
     // 1. Lane 0 walks samples: clock and decay exponent per sample
     //    (advance_clock, tdigest.mojo:324-332), until the buffer is full, a
     //    renormalization is due, or the samples run out.
+    //    The state lives in registers for the walk; timestamps are read
+    //    eight at a time (independent LDS loads) and consumed in order.
     if (threadIdx.x == 0) {
-      sh.renorm = 0;
-      int m = 0;
-      double ls = sh.st.log_scale, clock = sh.st.clock;
-      int room = limit - sh.st.nb;
-      while (p + m < end && m < room) {
-        double t = L.P[m];
-        if (t > clock) {
-          if (lambda > 0.0) {
-            double nls = ls + lambda * (t - clock);
-            clock = t;
-            if (nls > kRenorm) {
-              sh.renorm = 1;
-              sh.renorm_ls = nls;
-              break;
+      int m = 0, renorm = 0;
+      double ls = sh.st.log_scale, clock = sh.st.clock, rls = 0.0;
+      const int nb = sh.st.nb;
+      const int have = (int)(end - p < (int64_t)(limit - nb) ? end - p : (int64_t)(limit - nb));
+      double* out = L.bw + nb;
+      while (m < have && !renorm) {
+        double t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t8[u] = m + u < have ? L.P[m + u] : 0.0;
+        const int lim = have - m < 8 ? have - m : 8;
+        for (int u = 0; u < lim; u++) {
+          const double t = t8[u];
+          if (t > clock) {
+            if (lambda > 0.0) {
+              double nls = ls + lambda * (t - clock);
+              clock = t;
+              if (nls > kRenorm) {
+                renorm = 1;
+                rls = nls;
+                break;
+              }
+              ls = nls;
+            } else {
+              clock = t;
             }
-            ls = nls;
-          } else {
-            clock = t;
           }
+          out[m] = ls;
+          m++;
         }
-        L.bw[sh.st.nb + m] = ls;
-        m++;
       }
+      sh.renorm = renorm;
+      sh.renorm_ls = rls;
       sh.m = m;
       sh.st.clock = clock;
       sh.st.log_scale = ls;
@@ -382,18 +393,33 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
     __syncthreads();
     // 3. Insert bookkeeping in order (tdigest.mojo:334-347).
     if (threadIdx.x == 0) {
-      for (int i = 0; i < m; i++) {
-        double v = L.bm[nb0 + i];
-        if (sh.st.stored_total == 0.0 && sh.st.nb == 0) {
-          sh.st.vmin = v;
-          sh.st.vmax = v;
-        } else {
-          if (v < sh.st.vmin) sh.st.vmin = v;
-          if (v > sh.st.vmax) sh.st.vmax = v;
+      double total = sh.st.stored_total, vmin = sh.st.vmin, vmax = sh.st.vmax;
+      int nb = sh.st.nb;
+      for (int i0 = 0; i0 < m; i0 += 8) {
+        double v8[8], w8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          v8[u] = i0 + u < m ? L.bm[nb0 + i0 + u] : 0.0;
+          w8[u] = i0 + u < m ? L.bw[nb0 + i0 + u] : 0.0;
         }
-        sh.st.stored_total += L.bw[nb0 + i];
-        sh.st.nb++;
+        const int lim = m - i0 < 8 ? m - i0 : 8;
+        for (int u = 0; u < lim; u++) {
+          const double v = v8[u];
+          if (total == 0.0 && nb == 0) {
+            vmin = v;
+            vmax = v;
+          } else {
+            if (v < vmin) vmin = v;
+            if (v > vmax) vmax = v;
+          }
+          total += w8[u];
+          nb++;
+        }
       }
+      sh.st.stored_total = total;
+      sh.st.vmin = vmin;
+      sh.st.vmax = vmax;
+      sh.st.nb = nb;
     }
     __syncthreads();
     p += m;

@@ -1370,9 +1370,15 @@ __host__ __device__ inline int sel_cap_c(int scale, double nrm) {
   return (int)(cc < 2 ? 2 : cc);
 }
 
+// Values per tile of the count and gather passes: at most tiles_max tiles,
+// at least tile_min values per tile, except that a digest of more than
+// 2^20 / 64 values gets at least 64 tiles (at most 16384 values each) so a
+// batch holding one large digest still spreads over the GPU.
 __host__ __device__ inline int64_t sel_tile_len(int64_t v_cnt, int64_t tile_min, int64_t tiles_max) {
+  int64_t lo = v_cnt / 64 > 16384 ? v_cnt / 64 : 16384;
+  lo = lo < tile_min ? lo : tile_min;
   int64_t t = (v_cnt + tiles_max - 1) / tiles_max;
-  return t > tile_min ? t : tile_min;
+  return t > lo ? t : lo;
 }
 
 // LDS footprint class of the count and gather passes: 0 few splitters and
