@@ -331,95 +331,143 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
   int64_t p = off[d];
   const int64_t end = off[d + 1];
   while (p < end || sh.renorm) {
-    // 0. The timestamps the walk below may consume, staged in LDS by all
-    //    lanes (P is free between merges).
+    const int nb0 = sh.st.nb;
+    const int have = (int)(end - p < (int64_t)(limit - nb0) ? end - p : (int64_t)(limit - nb0));
+    const int tid = threadIdx.x;
+    const int per = (have + kRB - 1) / kRB;
+    const int i0 = min(have, tid * per), i1 = min(have, i0 + per);
+    // 0. The timestamps this walk may consume, staged in LDS (P, wm, kv and
+    //    ww are free between merges).
+    for (int i = tid; i < have; i += kRB) L.P[i] = times[p + i];
+    __syncthreads();
+    // 1. Clock and decay exponent per sample (advance_clock,
+    //    tdigest.mojo:324-332).  The clock before sample i is the running
+    //    maximum of the earlier timestamps (a block max-scan, exact); sample
+    //    i adds lambda * (t_i - clock) to the exponent when t_i is later.
+    //    Only the exponent's running sum is sequential (lane 0, one addition
+    //    per sample, stopping where it crosses the renormalization
+    //    threshold); the sample that crosses it is consumed by the next walk.
     {
-      const int room = limit - sh.st.nb;
-      const int64_t have = end - p < (int64_t)room ? end - p : (int64_t)room;
-      for (int i = threadIdx.x; i < (int)have; i += blockDim.x) L.P[i] = times[p + i];
+      double lm = -__builtin_inf();
+      for (int i = i0; i < i1; i++) lm = L.P[i] > lm ? L.P[i] : lm;
+      L.wm[tid] = lm;
     }
     __syncthreads();
-    // 1. Lane 0 walks samples: clock and decay exponent per sample
-    //    (advance_clock, tdigest.mojo:324-332), until the buffer is full, a
-    //    renormalization is due, or the samples run out.
-    //    The state lives in registers for the walk; timestamps are read
-    //    eight at a time (independent LDS loads) and consumed in order.
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
+      double run = sh.st.clock;
+      for (int t = 0; t < kRB; t++) {
+        const double x = L.wm[t];
+        L.wm[t] = run;
+        run = x > run ? x : run;
+      }
+    }
+    __syncthreads();
+    {
+      double prev = L.wm[tid];
+      for (int i = i0; i < i1; i++) {
+        const double t = L.P[i];
+        L.bw[nb0 + i] = (lambda > 0.0 && t > prev) ? lambda * (t - prev) : 0.0;
+        prev = t > prev ? t : prev;
+        L.kv[i] = prev;  // the clock after sample i
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
       int m = 0, renorm = 0;
-      double ls = sh.st.log_scale, clock = sh.st.clock, rls = 0.0;
-      const int nb = sh.st.nb;
-      const int have = (int)(end - p < (int64_t)(limit - nb) ? end - p : (int64_t)(limit - nb));
-      double* out = L.bw + nb;
+      double ls = sh.st.log_scale, rls = 0.0;
+      double* term = L.bw + nb0;
       while (m < have && !renorm) {
         double t8[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) t8[u] = m + u < have ? L.P[m + u] : 0.0;
-        const int lim = have - m < 8 ? have - m : 8;
-        for (int u = 0; u < lim; u++) {
-          const double t = t8[u];
-          if (t > clock) {
-            if (lambda > 0.0) {
-              double nls = ls + lambda * (t - clock);
-              clock = t;
-              if (nls > kRenorm) {
-                renorm = 1;
-                rls = nls;
-                break;
-              }
-              ls = nls;
+        for (int u = 0; u < 8; u++) t8[u] = m + u < have ? term[m + u] : 0.0;
+        const int m0 = m;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (m0 + u < have && !renorm) {
+            const double nls = ls + t8[u];
+            if (t8[u] != 0.0 && nls > kRenorm) {
+              renorm = 1;
+              rls = nls;
             } else {
-              clock = t;
+              ls = nls;
+              term[m] = ls;
+              m++;
             }
           }
-          out[m] = ls;
-          m++;
         }
       }
+      // The clock after the last consumed sample, or after the one whose
+      // advance triggers the renormalization.
+      const int last = renorm ? m : m - 1;
+      if (last >= 0) sh.st.clock = L.kv[last];
       sh.renorm = renorm;
       sh.renorm_ls = rls;
       sh.m = m;
-      sh.st.clock = clock;
       sh.st.log_scale = ls;
     }
     __syncthreads();
     const int m = sh.m;
-    const int nb0 = sh.st.nb;
     // 2. Scaled weights w * e^(log_scale) in parallel (tdigest.mojo:345).
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    for (int i = tid; i < m; i += kRB) {
       L.bm[nb0 + i] = vals[p + i];
       double sf = lambda > 0.0 ? kd_exp(L.bw[nb0 + i]) : 1.0;
       L.bw[nb0 + i] = wts[p + i] * sf;
     }
     __syncthreads();
-    // 3. Insert bookkeeping in order (tdigest.mojo:334-347).
-    if (threadIdx.x == 0) {
+    // 3. Insert bookkeeping in order (tdigest.mojo:334-347): min and max by a
+    //    block reduction that keeps the earliest of equal values (as the
+    //    sequential strict comparisons do); the stored total by one addition
+    //    per sample in order (lane 0).
+    {
+      const int pm = (m + kRB - 1) / kRB;
+      const int j0 = min(m, tid * pm), j1 = min(m, j0 + pm);
+      double mn = 0.0, mx = 0.0;
+      int has = 0;
+      for (int i = j0; i < j1; i++) {
+        const double v = L.bm[nb0 + i];
+        if (!has) {
+          mn = v;
+          mx = v;
+          has = 1;
+        } else {
+          if (v < mn) mn = v;
+          if (v > mx) mx = v;
+        }
+      }
+      L.wm[tid] = mn;
+      L.ww[tid] = mx;
+      L.nx[tid] = has;
+    }
+    __syncthreads();
+    if (tid == 0 && m > 0) {
       double total = sh.st.stored_total, vmin = sh.st.vmin, vmax = sh.st.vmax;
-      int nb = sh.st.nb;
-      for (int i0 = 0; i0 < m; i0 += 8) {
-        double v8[8], w8[8];
+      const bool first = total == 0.0 && nb0 == 0;
+      bool init = !first;
+      for (int t = 0; t < kRB; t++) {
+        if (!L.nx[t]) continue;
+        const double a = L.wm[t], c = L.ww[t];
+        if (!init) {
+          vmin = a;
+          vmax = c;
+          init = true;
+        } else {
+          if (a < vmin) vmin = a;
+          if (c > vmax) vmax = c;
+        }
+      }
+      const double* w = L.bw + nb0;
+      for (int i0w = 0; i0w < m; i0w += 8) {
+        double w8[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          v8[u] = i0 + u < m ? L.bm[nb0 + i0 + u] : 0.0;
-          w8[u] = i0 + u < m ? L.bw[nb0 + i0 + u] : 0.0;
-        }
-        const int lim = m - i0 < 8 ? m - i0 : 8;
-        for (int u = 0; u < lim; u++) {
-          const double v = v8[u];
-          if (total == 0.0 && nb == 0) {
-            vmin = v;
-            vmax = v;
-          } else {
-            if (v < vmin) vmin = v;
-            if (v > vmax) vmax = v;
-          }
-          total += w8[u];
-          nb++;
-        }
+        for (int u = 0; u < 8; u++) w8[u] = i0w + u < m ? w[i0w + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (i0w + u < m) total += w8[u];
       }
       sh.st.stored_total = total;
       sh.st.vmin = vmin;
       sh.st.vmax = vmax;
-      sh.st.nb = nb;
+      sh.st.nb = nb0 + m;
     }
     __syncthreads();
     p += m;
