@@ -183,7 +183,11 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
       if (!(L.kv[mid] - ks <= 1.0)) hi = mid;
       else lo = mid + 1;
     }
-    L.nx[s] = lo - 1;
+    // Bit 30 flags a candidate within 2 ulps of the record before it, where
+    // the tie rule (tdigest.mojo:429) may extend the cluster.
+    const int b = lo - 1;
+    const bool tie = b < n && ord_key(L.wm[b]) - ord_key(L.wm[b - 1]) <= 2;
+    L.nx[s] = b | (tie ? 0x40000000 : 0);
   }
   __syncthreads();
   // Chain of cluster starts with the tie rule (tdigest.mojo:429), lane 0.
@@ -192,11 +196,12 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
     while (s < n) {
       L.starts[c++] = s;
       if (s == n - 1) break;
-      int b = L.nx[s];
+      const int v = L.nx[s];
+      const int b = v & 0x3fffffff;
       int e = b;
-      if (b < n) {
-        double xb = L.wm[b], xp = L.wm[b - 1];
-        if (ord_key(xb) - ord_key(xp) <= 2) {
+      if (v & 0x40000000) {
+        double xb = L.wm[b];
+        {
           if (L.wm[s] == xb) {
             while (e < n && L.wm[e] == xb) e++;
           } else {
@@ -233,38 +238,48 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
     ow[c] = wt;
   }
   __syncthreads();
-  // Prune (decay active) and install.
-  if (threadIdx.x == 0) {
-    int k = 0;
-    bool fits = true;
-    if (lambda > 0.0) {
-      double floor = kPruneEps * kd_exp(sh.st.log_scale);
+  // Prune (decay active, tdigest.mojo:441-458) and install: each thread
+  // compacts a contiguous run of clusters at its scanned offset; the pruned
+  // weight is summed in cluster order by lane 0 over the runs that have any.
+  __shared__ int kept[kRB], pruned[kRB];
+  {
+    const double floor = lambda > 0.0 ? kPruneEps * kd_exp(sh.st.log_scale) : 0.0;
+    const int per = (C + kRB - 1) / kRB;
+    const int c0 = min(C, (int)threadIdx.x * per), c1 = min(C, c0 + per);
+    int kc = 0;
+    for (int c = c0; c < c1; c++) kc += (lambda > 0.0 && !(ow[c] >= floor)) ? 0 : 1;
+    kept[threadIdx.x] = kc;
+    pruned[threadIdx.x] = (c1 - c0) - kc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = 0;
       double removed = 0.0;
-      for (int c = 0; c < C; c++) {
-        if (ow[c] >= floor) {
-          if (k < kRCMax) {
-            L.cm[k] = om[c];
-            L.cw[k] = ow[c];
-          }
-          k++;
-        } else {
-          removed += ow[c];
+      for (int t = 0; t < kRB; t++) {
+        const int x = kept[t];
+        kept[t] = run;
+        run += x;
+        if (pruned[t]) {
+          const int a0 = min(C, t * per), a1 = min(C, a0 + per);
+          for (int c = a0; c < a1; c++)
+            if (!(ow[c] >= floor)) removed += ow[c];
         }
       }
       if (removed > 0.0) sh.st.stored_total -= removed;
-    } else {
-      for (int c = 0; c < C; c++) {
-        if (k < kRCMax) {
-          L.cm[k] = om[c];
-          L.cw[k] = ow[c];
-        }
-        k++;
-      }
+      const bool fits = run <= kRCMax;
+      sh.st.nc = fits ? run : kRCMax;
+      sh.st.nb = 0;
+      if (!fits) sh.st.err = 1;
     }
-    if (k > kRCMax) fits = false;
-    sh.st.nc = fits ? k : kRCMax;
-    sh.st.nb = 0;
-    if (!fits) sh.st.err = 1;
+    __syncthreads();
+    int k = kept[threadIdx.x];
+    for (int c = c0; c < c1; c++) {
+      if (lambda > 0.0 && !(ow[c] >= floor)) continue;
+      if (k < kRCMax) {
+        L.cm[k] = om[c];
+        L.cw[k] = ow[c];
+      }
+      k++;
+    }
   }
   __syncthreads();
   return sh.st.err == 0;
