@@ -303,6 +303,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--shape", type=str, default=None,
+                    help="fixed grouped launch shape items,chunks,tile_min,tiles_max,overlap "
+                         "(implies --no-autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -330,7 +333,14 @@ def main():
 
     # ------------------------------------------------ grouped ingest (value)
     tuned = None
-    if not args.no_autotune:
+    if args.shape:
+        it, ch, tmin, tmax, ov = (int(x) for x in args.shape.split(","))
+        session.set_partition_shape(it, ch)
+        session.set_select_shape(tmin, tmax)
+        session.set_overlap(bool(ov))
+        tuned = {"items": it, "chunks": ch, "tile_min": tmin, "tiles_max": tmax, "overlap": bool(ov),
+                 "fixed": True}
+    elif not args.no_autotune:
         # Launch shape picked by the library's autotuner on this batch (not
         # timed; every shape builds identical digests).
         tuned = session.autotune_grouped(keys, vals, nk, scale=kd.ScaleK3, compression=100.0)
