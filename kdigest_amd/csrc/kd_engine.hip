@@ -1547,12 +1547,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   // The search runs on 32 bits of each splitter's order key (4-byte LDS
   // entries: half the bank traffic of the doubles): the bits right after the
   // prefix that every splitter shares, so that nearly every value differs
-  // from every splitter there.  Values below the first or at or above the
+  // from every splitter there.  A table of kSlots slots over the splitters'
+  // key range (monotone float map) gives each value the few splitters it
+  // must still be compared with.  Values below the first or at or above the
   // last splitter are placed directly; splitters sharing the 32 bits with a
   // value are compared exactly (from global memory: rare).
+  constexpr int kSlots = 4096;
   __shared__ uint32_t sk[SMAX + 1];
   __shared__ unsigned int lc[SMAX + 1];
+  __shared__ uint16_t slot[kSlots + 1];
   __shared__ unsigned long long smin, smax;
+  __shared__ int s_occ;
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
   const int j = sel_tile_seg(sel, nsel, t);
@@ -1560,21 +1565,42 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
   const double* sg = spl + g.b_off;
-  // Padded with 0xffffffff (above the key of any finite value) to P - 1
-  // entries, P a power of two: log2(P) fixed steps without bounds tests.
-  int P = 1;
-  while (P < S + 1) P <<= 1;
   const double sp0 = S > 0 ? sg[0] : 0.0, spl_last = S > 0 ? sg[S - 1] : 0.0;
   const unsigned long long kf = ord_key(sp0), kl = ord_key(spl_last);
   const int sh = kf == kl ? 32 : min(32, __clzll(kf ^ kl));
-  for (int i = threadIdx.x; i < P; i += blockDim.x)
-    sk[i] = i < S ? (uint32_t)((ord_key(sg[i]) << sh) >> 32) : 0xffffffffu;
+  const uint32_t k0 = (uint32_t)((kf << sh) >> 32), kS = (uint32_t)((kl << sh) >> 32);
+  const float fscale = (float)kSlots / ((float)(kS - k0) + 1.0f);
+  auto slot_of = [&](uint32_t k) {
+    int h = (int)((float)(k - k0) * fscale);
+    return h < kSlots - 1 ? h : kSlots - 1;
+  };
+  for (int i = threadIdx.x; i < S; i += blockDim.x) sk[i] = (uint32_t)((ord_key(sg[i]) << sh) >> 32);
   for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
   if (threadIdx.x == 0) {
     smin = ~0ull;
     smax = 0ull;
+    s_occ = 0;
   }
   __syncthreads();
+  // slot[h] = number of splitters mapped below h (slot[kSlots] = S): each
+  // splitter fills the slots between its predecessor's and its own.
+  for (int i = threadIdx.x; i <= S; i += blockDim.x) {
+    const int hp = i > 0 ? slot_of(sk[i - 1]) : -1;
+    const int hi = i < S ? slot_of(sk[i]) : kSlots;
+    for (int h = hp + 1; h <= hi; h++) slot[h] = (uint16_t)i;
+  }
+  __syncthreads();
+  {
+    int occ = 0;
+    for (int h = threadIdx.x; h < kSlots; h += blockDim.x) {
+      const int c = (int)slot[h + 1] - (int)slot[h];
+      occ = c > occ ? c : occ;
+    }
+    atomicMax(&s_occ, occ);
+  }
+  __syncthreads();
+  int steps = 0;
+  while ((1 << steps) <= s_occ) steps++;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
@@ -1586,19 +1612,27 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * kSelThreads) {
     double x[U];
     uint32_t kx[U];
-    int lo[U];
+    int lo[U], len[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       x[u] = i < b ? v[i] : 0.0;
-      kx[u] = (uint32_t)((ord_key(x[u]) << sh) >> 32);
-      lo[u] = 0;
+      const bool in = S > 0 && !(x[u] < sp0) && x[u] < spl_last;
+      kx[u] = in ? (uint32_t)((ord_key(x[u]) << sh) >> 32) : k0;
+      const int h = slot_of(kx[u]);
+      lo[u] = slot[h];
+      len[u] = (int)slot[h + 1] - lo[u];
     }
-    // lo = number of splitters whose high key bits are below the value's:
+    // lo = number of splitters whose searched bits are below the value's:
     // all of them are below the value.
-    for (int step = P >> 1; step > 0; step >>= 1) {
+    for (int it = 0; it < steps; it++) {
 #pragma unroll
-      for (int u = 0; u < U; u++) lo[u] += (sk[lo[u] + step - 1] < kx[u]) ? step : 0;
+      for (int u = 0; u < U; u++) {
+        const int half = len[u] >> 1;
+        const bool right = len[u] > 0 && sk[lo[u] + half] < kx[u];
+        lo[u] = right ? lo[u] + half + 1 : lo[u];
+        len[u] = right ? len[u] - half - 1 : half;
+      }
     }
     // The first splitter not below the value in the searched bits: equal
     // bits (rare) need the exact comparison.
