@@ -3616,7 +3616,17 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
 #pragma unroll
     for (int r = 0; r < kRxHistUnroll; r++) {
       uint32_t i = i0 + (uint32_t)(r * kRxThreads) + threadIdx.x;
-      if (i < b) atomicAdd(&lc[w][(k[r] >> shift) & mask], 1u);
+      const bool valid = i < b;
+      const uint32_t d = valid ? (k[r] >> shift) & mask : 0xffffffffu;
+      // A wave whose items all share one digit (a heavy key) adds once.
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      const uint64_t vmask = __ballot(valid);
+      if (__ballot(valid && d != d0) == 0) {
+        if ((threadIdx.x & 63) == 0 && d0 != 0xffffffffu)
+          atomicAdd(&lc[w][d0], (uint32_t)__popcll(vmask));
+      } else if (valid) {
+        atomicAdd(&lc[w][d], 1u);
+      }
     }
   }
   __syncthreads();
@@ -3770,15 +3780,26 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
       // into its digit's mask, then reads it back (one wave's LDS operations
       // complete in issue order, so the read sees every lane's OR, and the
       // leader's reset lands before the next round's ORs).
-      if (valid) atomicOr(&pm(w, d), 1ull << lane);
-      uint64_t peers = valid ? pm(w, d) : 0ull;
+      // A wave whose valid items all share one digit (a heavy key) ranks
+      // them by lane directly, without the masks.
+      const uint32_t dv = valid ? d : 0xffffffffu;
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(dv);
+      const uint64_t vmask = __ballot(valid);
+      const bool uniform = __ballot(valid && dv != d0) == 0;
+      uint64_t peers;
+      if (uniform) {
+        peers = valid ? vmask : 0ull;
+      } else {
+        if (valid) atomicOr(&pm(w, d), 1ull << lane);
+        peers = valid ? pm(w, d) : 0ull;
+      }
       uint32_t before = (uint32_t)__popcll(peers & below);
       uint32_t base = valid ? wc[w][d] : 0u;
       if (valid) {
         rk[r] = base + before;
         if (before == 0) {
           wc[w][d] = base + (uint32_t)__popcll(peers);
-          pm(w, d) = 0ull;
+          if (!uniform) pm(w, d) = 0ull;
         }
       }
     }
