@@ -80,6 +80,12 @@ int kd_session_set_select_shape(kd_session *s, int64_t tile_min, int64_t tiles_m
  * session stream after it (0, the default).  Same results either way.  No
  * counterpart in the reference. */
 int kd_session_set_overlap(kd_session *s, int enable);
+/* Grouped key partition of batches of 2^22 pairs or more (and keys below
+ * 2^17): keys estimated from a sample to hold at least min_pairs of the
+ * batch's pairs (at most 255 of them) are placed by the first radix pass
+ * alone; 0 sends every key through both passes.  Same results either way.
+ * No counterpart in the reference. */
+int kd_session_set_heavy_keys(kd_session *s, int64_t min_pairs);
 /* Autotuner of the grouped ingest: times ingests of the device-resident batch
  * (d_keys, d_values, n pairs, keys below num_keys) into fresh digests of the
  * given scale and compression under candidate launch shapes (coordinate

@@ -279,6 +279,14 @@ extern "C" int kd_session_set_overlap(kd_session* s, int enable) {
   });
 }
 
+extern "C" int kd_session_set_heavy_keys(kd_session* s, int64_t min_pairs) {
+  return guarded([&] {
+    if (min_pairs < 0) throw KdError(KD_EINVAL, "min_pairs must be >= 0");
+    s->eng.set_heavy_keys(min_pairs);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_session_path_counts(kd_session* s, int64_t* out3) {
   for (int i = 0; i < 3; i++) out3[i] = s->eng.path_counts()[i];
   return KD_OK;

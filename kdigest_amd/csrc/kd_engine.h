@@ -189,6 +189,11 @@ class Engine {
     rx_chunks_ = chunks;
   }
   int partition_items() const { return rx_items_; }
+  // Keys of a large batch estimated (from a sample) to hold at least
+  // min_pairs pairs are placed by the first partition pass alone; 0 turns
+  // this off.  Results are the same either way.
+  void set_heavy_keys(int64_t min_pairs) { heavy_min_ = std::max<int64_t>(min_pairs, 0); }
+  int64_t heavy_keys() const { return heavy_min_; }
   // Tiles of the selection count / gather passes: at least tile_min values,
   // at most tiles_max tiles per segment (one workgroup per tile).
   // One-block compress of small segments on a second stream, overlapping the
@@ -278,6 +283,8 @@ class Engine {
   // segment and tile offsets.
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
   int rx_items_ = 8;       // items per thread in a scatter sub-tile (4, 8 or 16)
+  int64_t heavy_min_ = 32768;  // pairs that make a key heavy (0: no heavy split)
+  DevBuf heavy_cnt_, heavy_list_;
   int64_t rx_chunks_ = 4096;  // target number of chunks per pass
   void key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n, int bits,
                            int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,

@@ -3585,15 +3585,180 @@ __device__ inline void rx_scan_bins(const uint32_t* cnt, uint32_t* out, int R) {
   }
 }
 
+// Heavy keys of a large batch (the few keys holding most of its pairs) get
+// a first-pass digit of their own (their rank among the heavy keys, in key
+// order, below kHeavyMax) and land in their final place after one pass;
+// every other key takes digit kHeavyLight + its high bits and goes through
+// the second pass.  They are found from a sample of the keys
+// (k_heavy_sample, k_heavy_classes, k_heavy_flags, k_heavy_list) and kept as
+// a bitmap over the keys with a per-word count of the heavy keys before it,
+// so a key's rank is one word and one byte of LDS away.
+constexpr int kHeavyMax = 255;
+constexpr int kHeavyLight = 256;    // first digit of the keys that take two passes
+constexpr int kHeavyKeyBits = 17;   // heavy split only for keys below 2^17
+constexpr int kHeavyWords = (1 << kHeavyKeyBits) / 64;
+
+// Device layout of the heavy index (one buffer).
+struct HeavyIndex {
+  unsigned long long bm[kHeavyWords];  // bit k % 64 of word k / 64: key k is heavy
+  uint8_t pre[kHeavyWords];            // heavy keys in the words before
+  uint32_t list[kHeavyMax + 1];        // heavy keys in key order
+  unsigned int cls[40];                // sample counts per power-of-two class
+  int nheavy;
+};
+
+// The index staged in LDS by the partition kernels: `words` bitmap words,
+// then their prefix counts (heavy_lds_bytes in all).
+struct HeavyLds {
+  unsigned long long* bm;
+  uint8_t* pre;
+};
+
+__host__ __device__ inline size_t heavy_lds_bytes(int words) {
+  return (size_t)words * 8 + (((size_t)words + 7) & ~(size_t)7);
+}
+
+__device__ inline HeavyLds heavy_stage(unsigned long long* lds, const HeavyIndex* h, int words) {
+  HeavyLds t{lds, reinterpret_cast<uint8_t*>(lds + words)};
+  for (int i = threadIdx.x; i < words; i += blockDim.x) {
+    t.bm[i] = h->bm[i];
+    t.pre[i] = h->pre[i];
+  }
+  return t;
+}
+
+// First-pass digit of key k: its heavy rank, or kHeavyLight + (k >> shift) &
+// lmask.  Keys at or beyond the index (out of range, flagged by validation)
+// are light, so the digit stays below kHeavyLight + lmask + 1.
+__device__ inline uint32_t heavy_digit(const HeavyLds& t, int words, uint32_t k, int shift,
+                                       uint32_t lmask) {
+  const uint32_t w = k >> 6;
+  if (w < (uint32_t)words) {
+    const unsigned long long bits = t.bm[w];
+    const unsigned long long below = (1ull << (k & 63)) - 1ull;
+    if ((bits >> (k & 63)) & 1ull) return (uint32_t)t.pre[w] + (uint32_t)__popcll(bits & below);
+  }
+  return kHeavyLight + ((k >> shift) & lmask);
+}
+
+// Key counts of a sample: block b takes 2048 consecutive keys starting at
+// b * stride, counts them in an LDS hash and adds the counts to cnt.
+__global__ __launch_bounds__(256) void k_heavy_sample(const uint32_t* keys, int64_t n, int64_t stride,
+                                                      int64_t num_keys, unsigned int* cnt) {
+  constexpr int kSlots = 4096;
+  __shared__ uint32_t hk[kSlots];
+  __shared__ uint32_t hc[kSlots];
+  for (int i = threadIdx.x; i < kSlots; i += 256) {
+    hk[i] = 0;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  const int64_t a = (int64_t)blockIdx.x * stride;
+  for (int j = threadIdx.x; j < 2048; j += 256) {
+    const int64_t i = a + j;
+    if (i >= n) break;
+    const uint32_t k = keys[i];
+    if ((int64_t)k >= num_keys) continue;
+    uint32_t h = (k * 2654435761u) >> 20;
+    for (;;) {
+      const uint32_t old = atomicCAS(&hk[h], 0u, k + 1u);
+      if (old == 0u || old == k + 1u) {
+        atomicAdd(&hc[h], 1u);
+        break;
+      }
+      h = (h + 1) & (kSlots - 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kSlots; i += 256)
+    if (hk[i]) atomicAdd(&cnt[hk[i] - 1u], hc[i]);
+}
+
+// Class of a sample count c >= cmin: c in [2^(b-1), 2^b) is class b (1..32);
+// 0 for counts below cmin.
+__device__ inline int heavy_class(unsigned int c, unsigned int cmin) {
+  return (c >= cmin && c > 0u) ? 32 - __clz(c) : 0;
+}
+
+// Keys per class (classes 1..32 in h->cls; zeroed beforehand).
+__global__ __launch_bounds__(256) void k_heavy_classes(const unsigned int* cnt, int64_t num_keys,
+                                                       unsigned int cmin, HeavyIndex* h) {
+  __shared__ unsigned int hist[33];
+  if (threadIdx.x < 33) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+       k < num_keys && k < (int64_t)(blockIdx.x + 1) * 1024; k += 256) {
+    const int b = heavy_class(cnt[k], cmin);
+    if (b) atomicAdd(&hist[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 33 && threadIdx.x > 0 && hist[threadIdx.x])
+    atomicAdd(&h->cls[threadIdx.x], hist[threadIdx.x]);
+}
+
+// Lowest class t such that classes t..32 hold at most kHeavyMax keys (33:
+// none).
+__device__ inline int heavy_threshold(const unsigned int* cls) {
+  unsigned int cum = 0;
+  int thr = 33;
+  for (int b = 32; b >= 1; b--) {
+    if (cum + cls[b] > (unsigned int)kHeavyMax) break;
+    cum += cls[b];
+    thr = b;
+  }
+  return thr;
+}
+
+// Heavy flags as bitmap words: key k is heavy when its class is at least the
+// threshold.  One wave per word (grid covers every word).
+__global__ __launch_bounds__(256) void k_heavy_flags(const unsigned int* cnt, int64_t num_keys,
+                                                     unsigned int cmin, HeavyIndex* h, int words) {
+  __shared__ int s_thr;
+  if (threadIdx.x == 0) s_thr = heavy_threshold(h->cls);
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool f = k < num_keys && heavy_class(cnt[k], cmin) >= s_thr;
+  const unsigned long long m = __ballot(f);
+  const int64_t w = k >> 6;
+  if ((threadIdx.x & 63) == 0 && w < words) h->bm[w] = m;
+}
+
+// Per-word prefix counts and the key-ordered list (one block of 1024
+// threads, two words each: words <= 2048).
+__global__ __launch_bounds__(1024) void k_heavy_list(HeavyIndex* h, int words) {
+  __shared__ uint32_t part[1024];
+  const int w0 = 2 * threadIdx.x;
+  const unsigned long long b0 = w0 < words ? h->bm[w0] : 0ull;
+  const unsigned long long b1 = w0 + 1 < words ? h->bm[w0 + 1] : 0ull;
+  part[threadIdx.x] = (uint32_t)(__popcll(b0) + __popcll(b1));
+  __syncthreads();
+  const uint32_t total = block_excl_scan(part, 1024);
+  uint32_t r = part[threadIdx.x];
+  if (w0 < words) h->pre[w0] = (uint8_t)r;
+  if (w0 + 1 < words) h->pre[w0 + 1] = (uint8_t)(r + (uint32_t)__popcll(b0));
+  for (int q = 0; q < 2; q++) {
+    unsigned long long b = q ? b1 : b0;
+    while (b) {
+      const int i = __ffsll((long long)b) - 1;
+      h->list[r++] = (uint32_t)((int64_t)(w0 + q) * 64 + i);
+      b &= b - 1ull;
+    }
+  }
+  if (threadIdx.x == 0) h->nheavy = (int)total;
+}
+
 // Digit counts of one chunk, one histogram per wave (so a hot key's digit
 // contends only within its wave).
-template <typename KT>
+template <typename KT, bool kHeavy>
 __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
                                                         const uint32_t* seg_start,
                                                         const uint32_t* tile_start, int nseg,
                                                         uint32_t chunk, int shift, int bits,
-                                                        uint32_t* mat) {
+                                                        uint32_t* mat, const HeavyIndex* hix,
+                                                        int hwords, uint32_t lmask) {
   __shared__ uint32_t lc[kRxWaves][kRxBins];
+  // Heavy index (the largest one: the hist kernel's LDS is small anyway).
+  __shared__ unsigned long long hl[kHeavy ? (kHeavyWords * 9 + 7) / 8 : 1];
   const uint32_t t = blockIdx.x;
   if (t >= tile_start[nseg]) return;
   const int s = rx_seg_of_tile(tile_start, nseg, t);
@@ -3604,6 +3769,8 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
   const uint32_t mask = (uint32_t)R - 1u;
   const int w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < kRxWaves * kRxBins; i += kRxThreads) (&lc[0][0])[i] = 0;
+  HeavyLds ht{};
+  if constexpr (kHeavy) ht = heavy_stage(hl, hix, hwords);
   __syncthreads();
   constexpr uint32_t kStep = kRxThreads * kRxHistUnroll;
   for (uint32_t i0 = a; i0 < b; i0 += kStep) {
@@ -3617,7 +3784,14 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
     for (int r = 0; r < kRxHistUnroll; r++) {
       uint32_t i = i0 + (uint32_t)(r * kRxThreads) + threadIdx.x;
       const bool valid = i < b;
-      const uint32_t d = valid ? (k[r] >> shift) & mask : 0xffffffffu;
+      uint32_t d = 0xffffffffu;
+      if (valid) {
+        if constexpr (kHeavy) {
+          d = heavy_digit(ht, hwords, (uint32_t)k[r], shift, lmask);
+        } else {
+          d = (k[r] >> shift) & mask;
+        }
+      }
       // A wave whose items all share one digit (a heavy key) adds once.
       const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
       const uint64_t vmask = __ballot(valid);
@@ -3715,14 +3889,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const u
 // only the key's digit bits place it.  With kWriteKeys (first of two passes)
 // only the next pass's digit (key & out_mask, at most 16 bits) is written
 // beside each value.
-template <int ITEMS, bool kWriteKeys, bool kValidate, typename KT>
+template <int ITEMS, bool kWriteKeys, bool kValidate, typename KT, bool kHeavy>
 __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const double* vals,
                                                            const uint32_t* seg_start,
                                                            const uint32_t* tile_start, int nseg,
                                                            uint32_t chunk, int shift, int bits,
                                                            const uint32_t* dst, uint16_t* keys_out,
                                                            uint32_t out_mask, double* vals_out,
-                                                           uint32_t key_limit, int* bad) {
+                                                           uint32_t key_limit, int* bad,
+                                                           const HeavyIndex* hix, int hwords,
+                                                           uint32_t lmask,
+                                                           const uint32_t* pos_base,
+                                                           double* vals_heavy) {
   constexpr int kSub = kRxThreads * ITEMS;
   constexpr int kPerWave = ITEMS * 64;
   constexpr int kPm = kRxWaves * kRxBins;
@@ -3734,7 +3912,10 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
   // share this LDS (the masks are all zero again when the ranking ends and
   // are cleared again after the write-out).
   __shared__ unsigned long long stage[kSub > kPm ? kSub : kPm];
-  __shared__ uint32_t sk[kSub];
+  // Staged digit (high half) and next-pass key bits (low half) per item.
+  __shared__ uint32_t sdk[kSub];
+  // The heavy index, in dynamic LDS of heavy_lds_bytes(hwords).
+  extern __shared__ unsigned long long rx_dyn[];
   double* sv = reinterpret_cast<double*>(stage);
   auto pm = [&](int wv, uint32_t d) -> unsigned long long& { return stage[wv * kRxBins + (int)d]; };
   const uint32_t t = blockIdx.x;
@@ -3747,13 +3928,17 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
   const uint32_t mask = (uint32_t)R - 1u;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t gbase = (size_t)R * tile_start[s] + lt;
-  for (int d = threadIdx.x; d < R; d += kRxThreads) run[d] = dst[gbase + (size_t)d * nts];
+  const uint32_t pbase = pos_base ? *pos_base : 0u;
+  for (int d = threadIdx.x; d < R; d += kRxThreads) run[d] = pbase + dst[gbase + (size_t)d * nts];
   for (int i = threadIdx.x; i < kPm; i += kRxThreads) {
     (&wc[0][0])[i] = 0;
     stage[i] = 0ull;
   }
+  HeavyLds ht{};
+  if constexpr (kHeavy) ht = heavy_stage(rx_dyn, hix, hwords);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kr[ITEMS], rk[ITEMS];
+  uint16_t dg[ITEMS];
   double vr[ITEMS];
   bool wrong = false;
   auto load = [&](uint32_t sub) {
@@ -3774,7 +3959,15 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
     for (int r = 0; r < ITEMS; r++) {
       uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
       bool valid = i < e;
-      uint32_t d = valid ? (kr[r] >> shift) & mask : 0u;
+      uint32_t d = 0u;
+      if (valid) {
+        if constexpr (kHeavy) {
+          d = heavy_digit(ht, hwords, kr[r], shift, lmask);
+        } else {
+          d = (kr[r] >> shift) & mask;
+        }
+      }
+      dg[r] = (uint16_t)d;
       if (kValidate && valid) wrong |= kr[r] >= key_limit || !(vr[r] - vr[r] == 0.0);
       // The lanes sharing this lane's digit: every valid lane ORs its bit
       // into its digit's mask, then reads it back (one wave's LDS operations
@@ -3822,21 +4015,25 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
     for (int r = 0; r < ITEMS; r++) {
       uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
       if (i < e) {
-        uint32_t d = (kr[r] >> shift) & mask;
+        uint32_t d = dg[r];
         uint32_t pos = lc[d] + wc[w][d] + rk[r];
         sv[pos] = vr[r];
-        sk[pos] = kr[r];
+        sdk[pos] = (d << 16) | (kWriteKeys ? ((uint32_t)kr[r] & out_mask & 0xffffu) : 0u);
       }
     }
     __syncthreads();
     if (sub + (uint32_t)kSub < b) load(sub + (uint32_t)kSub);
     const int len = (int)(e - sub);
     for (int i = threadIdx.x; i < len; i += kRxThreads) {
-      uint32_t k = sk[i];
-      uint32_t d = (k >> shift) & mask;
+      const uint32_t dk = sdk[i];
+      const uint32_t d = dk >> 16, k = dk & 0xffffu;
       uint32_t o = run[d] + ((uint32_t)i - lc[d]);
-      vals_out[o] = sv[i];
-      if (kWriteKeys) keys_out[o] = (uint16_t)(k & out_mask);
+      if (kHeavy && d < (uint32_t)kHeavyLight) {
+        vals_heavy[o] = sv[i];  // final place: no second pass
+      } else {
+        vals_out[o] = sv[i];
+        if (kWriteKeys) keys_out[o] = (uint16_t)(k & out_mask);
+      }
     }
     __syncthreads();
     for (int d = threadIdx.x; d < R; d += kRxThreads) {
@@ -3853,7 +4050,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
 // Segments of the next pass: (segment s, digit d) starts where the scanned
 // matrix puts the first tile of s for digit d.
 __global__ void k_rx_segs(const uint32_t* seg_start, const uint32_t* tile_start, int nseg, int R,
-                          const uint32_t* dst, uint32_t* seg_out) {
+                          const uint32_t* dst, uint32_t* seg_out, const uint32_t* pos_base) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t m = (int64_t)nseg * R;
   if (idx > m) return;
@@ -3863,14 +4060,34 @@ __global__ void k_rx_segs(const uint32_t* seg_start, const uint32_t* tile_start,
   }
   int s = (int)(idx / R), d = (int)(idx % R);
   uint32_t nts = tile_start[s + 1] - tile_start[s];
-  seg_out[idx] = nts ? dst[(size_t)R * tile_start[s] + (size_t)d * nts] : seg_start[s];
+  const uint32_t pbase = pos_base ? *pos_base : 0u;
+  seg_out[idx] = nts ? pbase + dst[(size_t)R * tile_start[s] + (size_t)d * nts] : seg_start[s];
 }
 
-__global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, int64_t* cnt) {
+// Per key: start and count of its values.  A heavy key's values are the
+// first-pass segment of its digit (seg1); the other keys' are second-pass
+// segments (seg, indexed by key).
+__global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, int64_t* cnt,
+                           const uint32_t* seg1, const HeavyIndex* hix, uint32_t total) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k > num_keys) return;
-  off[k] = seg[k];
-  if (k < num_keys) cnt[k] = (int64_t)seg[k + 1] - (int64_t)seg[k];
+  if (k == num_keys) {
+    off[k] = total;
+    return;
+  }
+  int r = -1;
+  if (hix) {  // num_keys <= 2^kHeavyKeyBits when the index is given
+    const unsigned long long bits = hix->bm[k >> 6];
+    if ((bits >> (k & 63)) & 1ull)
+      r = (int)hix->pre[k >> 6] + __popcll(bits & ((1ull << (k & 63)) - 1ull));
+  }
+  if (r >= 0) {
+    off[k] = seg1[r];
+    cnt[k] = (int64_t)seg1[r + 1] - (int64_t)seg1[r];
+  } else {
+    off[k] = seg[k];
+    cnt[k] = (int64_t)seg[k + 1] - (int64_t)seg[k];
+  }
 }
 
 void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n,
@@ -3878,6 +4095,9 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
                                  int64_t* d_offsets, double* d_out, int* d_bad) {
   const int hb = bits > kRxBits ? bits - kRxBits : 0;  // high digit bits (pass 1)
   const int lb = bits > kRxBits ? kRxBits : bits;      // low digit bits (last pass)
+  // Heavy keys get their own first-pass digit when the batch is large
+  // enough for a handful of keys to hold most of it.
+  const bool heavy = heavy_min_ > 0 && hb > 0 && bits <= kHeavyKeyBits && n >= ((int64_t)1 << 22);
   const int items = rx_items_ == 4 || rx_items_ == 16 ? rx_items_ : 8;
   const int64_t sub = (int64_t)kRxThreads * items;
   // Chunk: about n / rx_chunks_ items, a whole number of sub-tiles, at most 2^20.
@@ -3889,35 +4109,64 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   const int64_t nb_max = (mat_max + kScanChunk - 1) / kScanChunk;
   uint32_t* mat = as<uint32_t>(rx_mat_, mat_max);
   uint32_t* bsum = as<uint32_t>(rx_bsum_, nb_max);
-  uint32_t* segA = as<uint32_t>(rx_seg1_, ((int64_t)1 << bits) + 2);
-  uint32_t* segB = as<uint32_t>(rx_seg2_, ((int64_t)1 << bits) + 2);
+  uint32_t* segA = as<uint32_t>(rx_seg1_, ((int64_t)1 << bits) + kRxBins + 2);
+  uint32_t* segB = as<uint32_t>(rx_seg2_, ((int64_t)1 << bits) + kRxBins + 2);
   uint32_t* tiles = as<uint32_t>(rx_tile_, ((int64_t)1 << hb) + 2);
   const uint32_t klim = (uint32_t)std::min<int64_t>(num_keys, 0xffffffffLL);
+  HeavyIndex* hix = nullptr;
+  const int hwords = (int)((num_keys + 63) / 64);
+  if (heavy) {
+    unsigned int* hc = as<unsigned int>(heavy_cnt_, num_keys);
+    hix = reinterpret_cast<HeavyIndex*>(as<uint8_t>(heavy_list_, sizeof(HeavyIndex)));
+    check_hip(hipMemsetAsync(hc, 0, num_keys * sizeof(unsigned int), stream_), "memset heavy counts");
+    check_hip(hipMemsetAsync(hix->cls, 0, sizeof(hix->cls), stream_), "memset heavy classes");
+    const int nsb = 256;
+    const int64_t stride = std::max<int64_t>(2048, n / nsb);
+    const int64_t sampled = std::min<int64_t>(n, (int64_t)nsb * 2048);
+    // Heavy: an estimated heavy_min_ values or more (a sample count of at
+    // least heavy_min_ * sampled / n).
+    const unsigned int cmin = (unsigned int)std::max<double>(
+        1.0, std::ceil((double)heavy_min_ * (double)sampled / (double)n));
+    k_heavy_sample<<<nsb, 256, 0, stream_>>>(d_keys, n, stride, num_keys, hc);
+    launched("k_heavy_sample");
+    k_heavy_classes<<<(unsigned)((num_keys + 1023) / 1024), 256, 0, stream_>>>(hc, num_keys, cmin, hix);
+    launched("k_heavy_classes");
+    k_heavy_flags<<<(unsigned)((hwords * 64 + 255) / 256), 256, 0, stream_>>>(hc, num_keys, cmin, hix,
+                                                                              hwords);
+    launched("k_heavy_flags");
+    k_heavy_list<<<1, 1024, 0, stream_>>>(hix, hwords);
+    launched("k_heavy_list");
+  }
   k_rx_init<<<1, 1, 0, stream_>>>(segA, (uint32_t)n);
   launched("k_rx_init");
 
   const uint32_t low_mask = (1u << lb) - 1u;
+  const uint32_t high_mask = (1u << hb) - 1u;
   auto scatter = [&](bool first, bool last, unsigned grid, const auto* kin, const double* vin,
                      const uint32_t* seg, int nseg, int shift, int dbits, uint16_t* kout,
-                     double* vout) {
+                     double* vout, const uint32_t* pbase) {
     using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
     const uint32_t c = (uint32_t)chunk;
-#define KD_RX_LAUNCH(I, WK, VA)                                                                    \
-  k_rx_scatter<I, WK, VA, KT><<<grid, kRxThreads, 0, stream_>>>(kin, vin, seg, tiles, nseg, c,      \
-                                                               shift, dbits, mat, kout, low_mask, \
-                                                               vout, klim, d_bad)
-#define KD_RX_ITEMS(WK, VA)            \
-  if (items == 4) KD_RX_LAUNCH(4, WK, VA);       \
-  else if (items == 16) KD_RX_LAUNCH(16, WK, VA); \
-  else KD_RX_LAUNCH(8, WK, VA)
+#define KD_RX_LAUNCH(I, WK, VA, HV)                                                               \
+  k_rx_scatter<I, WK, VA, KT, HV><<<grid, kRxThreads, HV ? heavy_lds_bytes(hwords) : 0, stream_>>>( \
+      kin, vin, seg, tiles, nseg, c, shift, dbits, mat, kout, low_mask, vout, klim, d_bad, hix,   \
+      hwords, high_mask, pbase, d_out)
+#define KD_RX_ITEMS(WK, VA, HV)            \
+  if (items == 4) KD_RX_LAUNCH(4, WK, VA, HV);       \
+  else if (items == 16) KD_RX_LAUNCH(16, WK, VA, HV); \
+  else KD_RX_LAUNCH(8, WK, VA, HV)
     const bool validate = first && d_bad != nullptr;
     if constexpr (std::is_same<KT, uint32_t>::value) {
-      if (!last && validate) { KD_RX_ITEMS(true, true); }
-      else if (!last) { KD_RX_ITEMS(true, false); }
-      else if (validate) { KD_RX_ITEMS(false, true); }
-      else { KD_RX_ITEMS(false, false); }
+      if (!last && heavy) {
+        if (validate) { KD_RX_ITEMS(true, true, true); }
+        else { KD_RX_ITEMS(true, false, true); }
+      }
+      else if (!last && validate) { KD_RX_ITEMS(true, true, false); }
+      else if (!last) { KD_RX_ITEMS(true, false, false); }
+      else if (validate) { KD_RX_ITEMS(false, true, false); }
+      else { KD_RX_ITEMS(false, false, false); }
     } else {
-      KD_RX_ITEMS(false, false);
+      KD_RX_ITEMS(false, false, false);
     }
 #undef KD_RX_ITEMS
 #undef KD_RX_LAUNCH
@@ -3926,14 +4175,19 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
 
   auto pass = [&](const auto* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
                   int dbits, int64_t tiles_max, uint16_t* kout, double* vout, uint32_t* seg_out,
-                  bool first) {
+                  bool first, bool hv, const uint32_t* pbase) {
     const int R = 1 << dbits;
     k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, (uint32_t)chunk, tiles);
     launched("k_rx_tiles");
     {
       int ph_ = phase_begin("key_hist", (double)sizeof(*kin) * (double)n);
-      k_rx_hist<<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(kin, seg, tiles, nseg,
-                                                                 (uint32_t)chunk, shift, dbits, mat);
+      using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
+      if (hv)
+        k_rx_hist<KT, true><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
+            kin, seg, tiles, nseg, (uint32_t)chunk, shift, dbits, mat, hix, hwords, high_mask);
+      else
+        k_rx_hist<KT, false><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
+            kin, seg, tiles, nseg, (uint32_t)chunk, shift, dbits, mat, nullptr, 0, 0u);
       launched("k_rx_hist");
       phase_end(ph_);
     }
@@ -3950,26 +4204,37 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
       const double b = (double)sizeof(*kin) + 8.0 + 8.0 + (kout ? 2.0 : 0.0);
       int ph_ = phase_begin("key_scatter", b * (double)n);
       scatter(first, kout == nullptr, (unsigned)tiles_max, kin, vin, seg, nseg, shift, dbits, kout,
-              vout);
+              vout, pbase);
       phase_end(ph_);
     }
     const int64_t m = (int64_t)nseg * R + 1;
-    k_rx_segs<<<(unsigned)((m + 255) / 256), 256, 0, stream_>>>(seg, tiles, nseg, R, mat, seg_out);
+    k_rx_segs<<<(unsigned)((m + 255) / 256), 256, 0, stream_>>>(seg, tiles, nseg, R, mat, seg_out,
+                                                               pbase);
     launched("k_rx_segs");
   };
 
   if (hb > 0) {
     uint16_t* k1 = as<uint16_t>(rx_keys_, n);
     double* v1 = as<double>(rx_vals_, n);
-    pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, true);
-    pass((const uint16_t*)k1, v1, segB, 1 << hb, 0, lb, max_tiles2, (uint16_t*)nullptr, d_out, segA,
-         false);
+    if (heavy) {
+      // Pass 1 over digits 0..255 (heavy keys, straight to the output) and
+      // 256 + high bits; pass 2 over the light segments only, placed after
+      // the heavy values.
+      pass(d_keys, d_vals, segA, 1, lb, kRxBits, max_tiles1, k1, v1, segB, true, true, nullptr);
+      pass((const uint16_t*)k1, v1, segB + kHeavyLight, 1 << hb, 0, lb, max_tiles2,
+           (uint16_t*)nullptr, d_out, segA, false, false, segB + kHeavyLight);
+    } else {
+      pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, true, false, nullptr);
+      pass((const uint16_t*)k1, v1, segB, 1 << hb, 0, lb, max_tiles2, (uint16_t*)nullptr, d_out,
+           segA, false, false, nullptr);
+    }
   } else {
-    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, (uint16_t*)nullptr, d_out, segB, true);
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, (uint16_t*)nullptr, d_out, segB, true, false,
+         nullptr);
     std::swap(segA, segB);
   }
-  k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(segA, num_keys, d_offsets,
-                                                                          d_counts);
+  k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(
+      segA, num_keys, d_offsets, d_counts, segB, hix, (uint32_t)n);
   launched("k_rx_final");
 }
 

@@ -157,6 +157,11 @@ class GpuSession:
         selection compress of large ones (off by default)."""
         check(load().kd_session_set_overlap(self.handle, 1 if enable else 0))
 
+    def set_heavy_keys(self, min_pairs: int) -> None:
+        """Keys of a large grouped batch estimated to hold at least min_pairs
+        pairs are placed by the first partition pass alone (0: off)."""
+        check(load().kd_session_set_heavy_keys(self.handle, int(min_pairs)))
+
     def autotune_grouped(self, keys, values, num_keys: int, scale: int = 3,
                          compression: float = 100.0, reps: int = 2) -> dict:
         """Times the grouped ingest of a device-resident batch (torch tensors

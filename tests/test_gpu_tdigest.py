@@ -254,6 +254,53 @@ def test_grouped_rejects_bad_batches_without_change(kd, nk):
     assert g.total_weight(int(keys[0])) > 0
 
 
+@pytest.mark.parametrize("nk,zipf,min_pairs", [(3001, 1.3, 32768), (1 << 17, 1.1, 1),
+                                               (1 << 17, 3.0, 1 << 20), (1 << 18, 1.2, 1)])
+def test_grouped_heavy_keys_identical(kd, oracle, nk, zipf, min_pairs):
+    """Batches of 2^22 pairs and more place the keys holding many pairs in the
+    first partition pass alone; grouping is identical with that split off
+    (bit-identical centroids, two batches so the second merges; with 2^18
+    keys the first pass has no digits to spare and the split stays off), the
+    heaviest key matches the oracle, and a bad key in such a batch is still rejected
+    without change."""
+    rng = np.random.default_rng(nk + int(zipf * 10))
+    n = (1 << 22) + 12345
+    keys = ((rng.zipf(zipf, n) - 1) % nk).astype(np.uint32)
+    vals = rng.standard_normal(n) * 2.0 + (keys % 13) * 0.5
+    on = kd.GpuSession(0)
+    on.set_heavy_keys(min_pairs)  # 1: the 255-key limit applies
+    off = kd.GpuSession(0)
+    off.set_heavy_keys(0)
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK2, session=on)
+    h = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK2, session=off)
+    for half in (slice(0, n // 2), slice(n // 2, n)):
+        g.add_batch(keys[half], vals[half])
+        h.add_batch(keys[half], vals[half])
+    cnt = np.bincount(keys, minlength=nk)
+    top = [int(k) for k in np.argsort(cnt)[::-1][:300]]
+    chosen = sorted(set(top + list(range(0, nk, max(1, nk // 200))) + [nk - 1]))
+    for k in chosen:
+        m, w = g.centroids(k)
+        hm, hw = h.centroids(k)
+        assert np.array_equal(m, hm) and np.array_equal(w, hw), k
+        assert g.total_weight(k) == float(cnt[k]), k
+    k = top[0]
+    r = oracle.Digest(2, 100.0)
+    sel = vals[keys == k]
+    r.add_batch_b(sel[: np.count_nonzero(keys[: n // 2] == k)])
+    r.add_batch_b(sel[np.count_nonzero(keys[: n // 2] == k):])
+    m, w = g.centroids(k)
+    rm, rw = r.centroids()
+    check_centroids(m, w, rm, rw, 300.0, True, f"key {k}")
+    bad = keys.copy()
+    bad[n - 7] = nk
+    before = g.centroids(top[0])
+    with pytest.raises(kd.KdigestError):
+        g.add_batch(bad, vals)
+    after = g.centroids(top[0])
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+
+
 @pytest.mark.parametrize("items,chunks", [(4, 1), (8, 3), (16, 7), (8, 4096), (4, 1 << 20)])
 def test_grouped_partition_shapes_identical(kd, oracle, items, chunks):
     """Every launch shape of the key partition (sub-tile width, chunks per
