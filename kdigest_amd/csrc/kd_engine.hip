@@ -962,6 +962,13 @@ __device__ inline int64_t pcur_seek(const PCur& pc, int64_t j) {
   return lo;
 }
 
+// A positive normal power of two: its multiples by integers below 2^53 are
+// exact, so a running sum of n such weights is exactly n * w.
+__device__ inline bool pow2_weight(double w) {
+  const uint64_t b = kd_d2bits(w);
+  return (b & 0x000fffffffffffffull) == 0 && ((b >> 52) & 0x7ff) != 0 && ((b >> 52) & 0x7ff) != 0x7ff;
+}
+
 // Clusters up to this many records replay the running centroid of the CPU
 // sweep record by record (bit-identical means); longer ones use an exact
 // double-double sum (means within an ulp or two).
@@ -1133,38 +1140,99 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
     const uint16_t* cs = nxt;
     double* T = s;  // kv is no longer needed: per-record scratch of the means
     // Clusters of at most kReplayMax records replay the running mean of
-    // centroid.mojo:34-36 (one lane each): the ratios w / W' depend only on
-    // the running weight, so they are formed first (independent divisions)
-    // and the mean recurrence then runs on them.  Longer clusters get their
-    // weight here and their double-double sum from all lanes below.
+    // centroid.mojo:34-36: the ratios w / W' depend only on the running
+    // weight, so each cluster's weight chain (one lane each) stores W' per
+    // record, all lanes then form the ratios (independent divisions), and
+    // the mean recurrence runs on them (one lane each).  Longer clusters get
+    // their weight here and their double-double sum from all lanes below.
+    // With no prior records and a power-of-two weight per new value, every
+    // running weight is exactly (j - a + 1) * sf: no chain is needed.
+    const bool exact_w = ne == 0 && pow2_weight(sd.sf);
     for (int64_t c = threadIdx.x; c < C; c += BLOCK) {
       const int64_t a = cs[c];
       const int64_t b = (c + 1 < C) ? (int64_t)cs[c + 1] : n;
-      PCur pc{pos, sw, ne, sd.sf, 0};
-      int64_t ce = ne > 0 ? pcur_seek(pc, a) : 0;
+      const bool replay = b - a <= kReplayMax;
       double ws = 0.0;
-      for (int64_t j = a; j < b; j++) {
-        double w = sd.sf;
-        if (ce < ne && pos[ce] == j) {
-          w = ew[ce];
-          ce++;
+      if (exact_w) {
+        ws = (double)(b - a) * sd.sf;
+      } else if (ne == 0) {
+        ws = sd.sf;
+        if (replay) T[a] = ws;
+        for (int64_t j = a + 1; j < b; j++) {
+          ws += sd.sf;
+          if (replay) T[j] = ws;
         }
-        ws = (j == a) ? w : ws + w;
-        if (b - a <= kReplayMax && j > a) T[j] = w / ws;
+      } else {
+        PCur pc{pos, sw, ne, sd.sf, 0};
+        int64_t ce = pcur_seek(pc, a);
+        for (int64_t j = a; j < b; j++) {
+          double w = sd.sf;
+          if (ce < ne && pos[ce] == j) {
+            w = ew[ce];
+            ce++;
+          }
+          ws = (j == a) ? w : ws + w;
+          if (replay) T[j] = ws;
+        }
       }
       double mean;
-      if (b - a <= kReplayMax) {
-        mean = x[a];
-        for (int64_t j = a + 1; j < b; j++) mean += (x[j] - mean) * T[j];
-        cm[sd.x_off + c] = mean;
-      } else if (equal_run_mean(x, a, b, &mean)) {
-        cm[sd.x_off + c] = mean;
-      } else {
-        int k = atomicAdd(&s_nbig, 1);
-        s_big[k] = (int)c;
-        s_bigw[k] = ws;
+      if (!replay) {
+        if (equal_run_mean(x, a, b, &mean)) {
+          cm[sd.x_off + c] = mean;
+        } else {
+          int k = atomicAdd(&s_nbig, 1);
+          s_big[k] = (int)c;
+          s_bigw[k] = ws;
+        }
       }
       cw[sd.x_off + c] = ws;
+    }
+    __syncthreads();
+    // w / W' for every record (also where unused: cluster starts, long
+    // clusters, whose T entries are rewritten below).
+    for (int j = threadIdx.x; j < n; j += BLOCK) {
+      double w = sd.sf;
+      double W;
+      if (exact_w) {
+        int lo = 0, hi = (int)C - 1;  // the cluster holding j: last start <= j
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if ((int)cs[mid] <= j) lo = mid;
+          else hi = mid - 1;
+        }
+        W = (double)(j - (int)cs[lo] + 1) * sd.sf;
+      } else {
+        W = T[j];
+        if (ne > 0) {
+          int lo = 0, hi = ne;  // the prior record at merged index j, if any
+          while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (pos[mid] < j) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < ne && pos[lo] == j) w = ew[lo];
+        }
+      }
+      T[j] = w / W;
+    }
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < C; c += BLOCK) {
+      const int a = (int)cs[c];
+      const int b = (c + 1 < C) ? (int)cs[c + 1] : n;
+      if (b - a > kReplayMax) continue;
+      double mean = x[a];
+      int j = a + 1;
+      // Four records at a time: their loads do not wait for the chain.
+      for (; j + 4 <= b; j += 4) {
+        const double x0 = x[j], x1 = x[j + 1], x2 = x[j + 2], x3 = x[j + 3];
+        const double t0 = T[j], t1 = T[j + 1], t2 = T[j + 2], t3 = T[j + 3];
+        mean += (x0 - mean) * t0;
+        mean += (x1 - mean) * t1;
+        mean += (x2 - mean) * t2;
+        mean += (x3 - mean) * t3;
+      }
+      for (; j < b; j++) mean += (x[j] - mean) * T[j];
+      cm[sd.x_off + c] = mean;
     }
     __syncthreads();
     const int nbig = s_nbig;
