@@ -284,7 +284,7 @@ class Engine {
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
   int rx_items_ = 8;       // items per thread in a scatter sub-tile (4, 8 or 16)
   int64_t heavy_min_ = 32768;  // pairs that make a key heavy (0: no heavy split)
-  DevBuf heavy_cnt_, heavy_list_;
+  DevBuf heavy_cnt_, heavy_list_, heavy_dig_;
   int64_t rx_chunks_ = 4096;  // target number of chunks per pass
   void key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n, int bits,
                            int64_t num_keys, int64_t* d_counts, int64_t* d_offsets, double* d_out,

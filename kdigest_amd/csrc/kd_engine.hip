@@ -3660,7 +3660,8 @@ __device__ inline void rx_scan_bins(const uint32_t* cnt, uint32_t* out, int R) {
 // the second pass.  They are found from a sample of the keys
 // (k_heavy_sample, k_heavy_classes, k_heavy_flags, k_heavy_list) and kept as
 // a bitmap over the keys with a per-word count of the heavy keys before it,
-// so a key's rank is one word and one byte of LDS away.
+// and expanded into a per-key table of first-pass digits (k_heavy_digits),
+// read through the cache by the partition kernels.
 constexpr int kHeavyMax = 255;
 constexpr int kHeavyLight = 256;    // first digit of the keys that take two passes
 constexpr int kHeavyKeyBits = 17;   // heavy split only for keys below 2^17
@@ -3675,16 +3676,12 @@ struct HeavyIndex {
   int nheavy;
 };
 
-// The index staged in LDS by the partition kernels: `words` bitmap words,
-// then their prefix counts (heavy_lds_bytes in all).
+// The index staged in LDS by the histogram kernel: `words` bitmap words,
+// then their prefix counts.
 struct HeavyLds {
   unsigned long long* bm;
   uint8_t* pre;
 };
-
-__host__ __device__ inline size_t heavy_lds_bytes(int words) {
-  return (size_t)words * 8 + (((size_t)words + 7) & ~(size_t)7);
-}
 
 __device__ inline HeavyLds heavy_stage(unsigned long long* lds, const HeavyIndex* h, int words) {
   HeavyLds t{lds, reinterpret_cast<uint8_t*>(lds + words)};
@@ -3695,11 +3692,10 @@ __device__ inline HeavyLds heavy_stage(unsigned long long* lds, const HeavyIndex
   return t;
 }
 
-// First-pass digit of key k: its heavy rank, or kHeavyLight + (k >> shift) &
-// lmask.  Keys at or beyond the index (out of range, flagged by validation)
-// are light, so the digit stays below kHeavyLight + lmask + 1.
-__device__ inline uint32_t heavy_digit(const HeavyLds& t, int words, uint32_t k, int shift,
-                                       uint32_t lmask) {
+// First-pass digit of key k from the staged index: its heavy rank, or
+// kHeavyLight + (k >> shift) & lmask (the same digit as k_heavy_digits).
+__device__ inline uint32_t heavy_digit_lds(const HeavyLds& t, int words, uint32_t k, int shift,
+                                           uint32_t lmask) {
   const uint32_t w = k >> 6;
   if (w < (uint32_t)words) {
     const unsigned long long bits = t.bm[w];
@@ -3707,6 +3703,14 @@ __device__ inline uint32_t heavy_digit(const HeavyLds& t, int words, uint32_t k,
     if ((bits >> (k & 63)) & 1ull) return (uint32_t)t.pre[w] + (uint32_t)__popcll(bits & below);
   }
   return kHeavyLight + ((k >> shift) & lmask);
+}
+
+// First-pass digit of key k from the per-key digit table (k_heavy_digits);
+// keys at or beyond it (out of range, flagged by validation) take the light
+// digit of their high bits, so the digit stays below kHeavyLight + lmask + 1.
+__device__ inline uint32_t heavy_digit(const uint16_t* dtab, uint32_t nk, uint32_t k, int shift,
+                                       uint32_t lmask) {
+  return k < nk ? (uint32_t)dtab[k] : kHeavyLight + ((k >> shift) & lmask);
 }
 
 // Key counts of a sample: block b takes 2048 consecutive keys starting at
@@ -3815,6 +3819,18 @@ __global__ __launch_bounds__(1024) void k_heavy_list(HeavyIndex* h, int words) {
   if (threadIdx.x == 0) h->nheavy = (int)total;
 }
 
+// dtab[k]: key k's heavy rank, or kHeavyLight + (k >> shift) & lmask.
+__global__ __launch_bounds__(256) void k_heavy_digits(const HeavyIndex* h, int64_t num_keys,
+                                                      int shift, uint32_t lmask, uint16_t* dtab) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= num_keys) return;
+  const unsigned long long bits = h->bm[k >> 6];
+  const unsigned long long below = (1ull << (k & 63)) - 1ull;
+  dtab[k] = ((bits >> (k & 63)) & 1ull)
+                ? (uint16_t)(h->pre[k >> 6] + __popcll(bits & below))
+                : (uint16_t)(kHeavyLight + (((uint32_t)k >> shift) & lmask));
+}
+
 // Digit counts of one chunk, one histogram per wave (so a hot key's digit
 // contends only within its wave).
 template <typename KT, bool kHeavy>
@@ -3825,7 +3841,8 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
                                                         uint32_t* mat, const HeavyIndex* hix,
                                                         int hwords, uint32_t lmask) {
   __shared__ uint32_t lc[kRxWaves][kRxBins];
-  // Heavy index (the largest one: the hist kernel's LDS is small anyway).
+  // The heavy index in LDS (its largest size: this kernel's LDS is small),
+  // where the table lookup of the scatter would stall the stream of keys.
   __shared__ unsigned long long hl[kHeavy ? (kHeavyWords * 9 + 7) / 8 : 1];
   const uint32_t t = blockIdx.x;
   if (t >= tile_start[nseg]) return;
@@ -3855,7 +3872,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_hist(const KT* keys,
       uint32_t d = 0xffffffffu;
       if (valid) {
         if constexpr (kHeavy) {
-          d = heavy_digit(ht, hwords, (uint32_t)k[r], shift, lmask);
+          d = heavy_digit_lds(ht, hwords, (uint32_t)k[r], shift, lmask);
         } else {
           d = (k[r] >> shift) & mask;
         }
@@ -3965,7 +3982,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
                                                            const uint32_t* dst, uint16_t* keys_out,
                                                            uint32_t out_mask, double* vals_out,
                                                            uint32_t key_limit, int* bad,
-                                                           const HeavyIndex* hix, int hwords,
+                                                           const uint16_t* dtab, uint32_t dkeys,
                                                            uint32_t lmask,
                                                            const uint32_t* pos_base,
                                                            double* vals_heavy) {
@@ -3982,8 +3999,6 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
   __shared__ unsigned long long stage[kSub > kPm ? kSub : kPm];
   // Staged digit (high half) and next-pass key bits (low half) per item.
   __shared__ uint32_t sdk[kSub];
-  // The heavy index, in dynamic LDS of heavy_lds_bytes(hwords).
-  extern __shared__ unsigned long long rx_dyn[];
   double* sv = reinterpret_cast<double*>(stage);
   auto pm = [&](int wv, uint32_t d) -> unsigned long long& { return stage[wv * kRxBins + (int)d]; };
   const uint32_t t = blockIdx.x;
@@ -4002,8 +4017,6 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
     (&wc[0][0])[i] = 0;
     stage[i] = 0ull;
   }
-  HeavyLds ht{};
-  if constexpr (kHeavy) ht = heavy_stage(rx_dyn, hix, hwords);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kr[ITEMS], rk[ITEMS];
   uint16_t dg[ITEMS];
@@ -4030,7 +4043,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
       uint32_t d = 0u;
       if (valid) {
         if constexpr (kHeavy) {
-          d = heavy_digit(ht, hwords, kr[r], shift, lmask);
+          d = heavy_digit(dtab, dkeys, kr[r], shift, lmask);
         } else {
           d = (kr[r] >> shift) & mask;
         }
@@ -4136,7 +4149,7 @@ __global__ void k_rx_segs(const uint32_t* seg_start, const uint32_t* tile_start,
 // first-pass segment of its digit (seg1); the other keys' are second-pass
 // segments (seg, indexed by key).
 __global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, int64_t* cnt,
-                           const uint32_t* seg1, const HeavyIndex* hix, uint32_t total) {
+                           const uint32_t* seg1, const uint16_t* dtab, uint32_t total) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k > num_keys) return;
   if (k == num_keys) {
@@ -4144,11 +4157,7 @@ __global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, 
     return;
   }
   int r = -1;
-  if (hix) {  // num_keys <= 2^kHeavyKeyBits when the index is given
-    const unsigned long long bits = hix->bm[k >> 6];
-    if ((bits >> (k & 63)) & 1ull)
-      r = (int)hix->pre[k >> 6] + __popcll(bits & ((1ull << (k & 63)) - 1ull));
-  }
+  if (dtab && dtab[k] < kHeavyLight) r = dtab[k];
   if (r >= 0) {
     off[k] = seg1[r];
     cnt[k] = (int64_t)seg1[r + 1] - (int64_t)seg1[r];
@@ -4182,6 +4191,8 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   uint32_t* tiles = as<uint32_t>(rx_tile_, ((int64_t)1 << hb) + 2);
   const uint32_t klim = (uint32_t)std::min<int64_t>(num_keys, 0xffffffffLL);
   HeavyIndex* hix = nullptr;
+  uint16_t* dtab = nullptr;
+  const uint32_t dkeys = heavy ? (uint32_t)num_keys : 0u;
   const int hwords = (int)((num_keys + 63) / 64);
   if (heavy) {
     unsigned int* hc = as<unsigned int>(heavy_cnt_, num_keys);
@@ -4204,6 +4215,10 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     launched("k_heavy_flags");
     k_heavy_list<<<1, 1024, 0, stream_>>>(hix, hwords);
     launched("k_heavy_list");
+    dtab = as<uint16_t>(heavy_dig_, num_keys);
+    k_heavy_digits<<<(unsigned)((num_keys + 255) / 256), 256, 0, stream_>>>(
+        hix, num_keys, lb, (1u << hb) - 1u, dtab);
+    launched("k_heavy_digits");
   }
   k_rx_init<<<1, 1, 0, stream_>>>(segA, (uint32_t)n);
   launched("k_rx_init");
@@ -4216,9 +4231,9 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
     const uint32_t c = (uint32_t)chunk;
 #define KD_RX_LAUNCH(I, WK, VA, HV)                                                               \
-  k_rx_scatter<I, WK, VA, KT, HV><<<grid, kRxThreads, HV ? heavy_lds_bytes(hwords) : 0, stream_>>>( \
-      kin, vin, seg, tiles, nseg, c, shift, dbits, mat, kout, low_mask, vout, klim, d_bad, hix,   \
-      hwords, high_mask, pbase, d_out)
+  k_rx_scatter<I, WK, VA, KT, HV><<<grid, kRxThreads, 0, stream_>>>(                             \
+      kin, vin, seg, tiles, nseg, c, shift, dbits, mat, kout, low_mask, vout, klim, d_bad, dtab,  \
+      dkeys, high_mask, pbase, d_out)
 #define KD_RX_ITEMS(WK, VA, HV)            \
   if (items == 4) KD_RX_LAUNCH(4, WK, VA, HV);       \
   else if (items == 16) KD_RX_LAUNCH(16, WK, VA, HV); \
@@ -4241,6 +4256,9 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     launched("k_rx_scatter");
   };
 
+  // Pairs taking the second pass (all but the heavy keys'): known on the
+  // device only; read back for the phase byte counts when profiling.
+  int64_t n_light = n;
   auto pass = [&](const auto* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
                   int dbits, int64_t tiles_max, uint16_t* kout, double* vout, uint32_t* seg_out,
                   bool first, bool hv, const uint32_t* pbase) {
@@ -4248,7 +4266,7 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, (uint32_t)chunk, tiles);
     launched("k_rx_tiles");
     {
-      int ph_ = phase_begin("key_hist", (double)sizeof(*kin) * (double)n);
+      int ph_ = phase_begin("key_hist", (double)sizeof(*kin) * (first ? (double)n : (double)n_light));
       using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
       if (hv)
         k_rx_hist<KT, true><<<(unsigned)tiles_max, kRxThreads, 0, stream_>>>(
@@ -4266,11 +4284,21 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     launched("k_scan_top");
     k_scan_down<<<(unsigned)nb, kScanThreads, 0, stream_>>>(mat, tiles, nseg, R, bsum);
     launched("k_scan_down");
+    if (hv && prof_) {
+      // The first light digit's offset in the scanned matrix (one segment:
+      // tiles_max tiles, digit-major) is the heavy keys' pair count.
+      uint32_t h = 0;
+      check_hip(hipMemcpyAsync(&h, mat + (size_t)kHeavyLight * tiles_max, sizeof(h),
+                               hipMemcpyDeviceToHost, stream_), "heavy count");
+      check_hip(hipStreamSynchronize(stream_), "heavy count");
+      n_light = n - (int64_t)h;
+    }
     {
       // Bytes: the key (4 B, or the 2 B digit) and value read; the value and
-      // the next digit (pass 1) written.
-      const double b = (double)sizeof(*kin) + 8.0 + 8.0 + (kout ? 2.0 : 0.0);
-      int ph_ = phase_begin("key_scatter", b * (double)n);
+      // the next digit (first of two passes, light keys only) written.
+      const double items = first ? (double)n : (double)n_light;
+      const double b = ((double)sizeof(*kin) + 16.0) * items + (kout ? 2.0 * (double)n_light : 0.0);
+      int ph_ = phase_begin("key_scatter", b);
       scatter(first, kout == nullptr, (unsigned)tiles_max, kin, vin, seg, nseg, shift, dbits, kout,
               vout, pbase);
       phase_end(ph_);
@@ -4302,7 +4330,7 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     std::swap(segA, segB);
   }
   k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(
-      segA, num_keys, d_offsets, d_counts, segB, hix, (uint32_t)n);
+      segA, num_keys, d_offsets, d_counts, segB, dtab, (uint32_t)n);
   launched("k_rx_final");
 }
 
