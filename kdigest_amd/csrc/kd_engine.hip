@@ -294,18 +294,31 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
       __syncthreads();
     }
   }
+  // The sorted weights replace the keys in LDS for the prefix below.
+  __syncthreads();
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double wv = ew[sd.e_off + idx[i]];
     esm[sd.es_off + i] = em[sd.e_off + idx[i]];
-    esw[sd.es_off + i] = ew[sd.e_off + idx[i]];
+    esw[sd.es_off + i] = wv;
+    key[i] = wv;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // Weight prefix in sorted order, summed one record at a time (as the
+  // sweep does) by the first wave through lane broadcasts.
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
     double acc = 0.0;
     double* p = se + sd.es_off + s;
-    p[0] = 0.0;
-    for (int i = 0; i < n; i++) {
-      acc += ew[sd.e_off + idx[i]];
-      p[i + 1] = acc;
+    if (lane == 0) p[0] = 0.0;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const double wl = c0 + lane < n ? key[c0 + lane] : 0.0;
+      const int cnt = n - c0 < 64 ? n - c0 : 64;
+      double mine = 0.0;
+      for (int l = 0; l < cnt; l++) {
+        acc += __shfl(wl, l, 64);
+        if (lane == l) mine = acc;
+      }
+      if (lane < cnt) p[c0 + lane + 1] = mine;
     }
   }
 }
@@ -1553,8 +1566,21 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
   const int need = m > g.cap_s ? m : g.cap_s;
   if (need <= LO || need > CAP) return;
   const double* v = vin + sd.v_off;
-  for (int i = threadIdx.x; i < m; i += BLOCK)
-    sa[padi(i)] = v[((2 * (int64_t)i + 1) * nv) / (2 * (int64_t)m)];
+  // Sample i sits at ((2i + 1) nv) / (2m); eight loads in flight per thread
+  // (each is a separate cache line).
+  for (int i0 = 0; i0 < m; i0 += 8 * BLOCK) {
+    double xs[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * BLOCK + (int)threadIdx.x;
+      xs[u] = i < m ? v[((2 * (int64_t)i + 1) * nv) / (2 * (int64_t)m)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * BLOCK + (int)threadIdx.x;
+      if (i < m) sa[padi(i)] = xs[u];
+    }
+  }
   __syncthreads();
   double* s = lds_sort(sa, sb, m);
   double* f = (s == sa) ? sb : sa;
@@ -1565,7 +1591,11 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
   for (int t = threadIdx.x; t < T; t += BLOCK) f[t] = s[padi((int)(((t + 1) * (int64_t)m) / (T + 1)))];
   __syncthreads();
   // Merge with the prior means into s: a quantile goes before equal means.
-  const double* e = esm + sd.es_off;
+  // The prior means are staged after the quantiles in f (T + ne <= cap_s <
+  // CAP), so the searches below stay in LDS.
+  double* e = f + T;
+  for (int b = threadIdx.x; b < ne; b += BLOCK) e[b] = esm[sd.es_off + b];
+  __syncthreads();
   for (int t = threadIdx.x; t < T; t += BLOCK) {
     double x = f[t];
     int l = 0, h = ne;
@@ -1800,30 +1830,159 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   if (tid == 0 && s_fail) st[j].fail = 1;
 }
 
-// The greedy sweep in rank space (tdigest.mojo:414-435), one lane per
-// segment so that the serial walks of many segments overlap.
-__global__ void k_sel_chain(int scale, const SegDesc* segs, const SelSeg* sel, int nsel,
-                            const double* se, const int64_t* pose, SelState* st, SelBound* bnd) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nsel || st[j].fail) return;
-  const SelSeg g = sel[j];
-  const SegDesc sd = segs[g.si];
+// The greedy sweep in rank space (tdigest.mojo:414-435), one wave per
+// segment: the boundary of each cluster is the first candidate b > s with
+// NOT (k(P(b+1)/N) - k(P(s)/N) <= 1) (n when none), found by the 64 lanes
+// testing 64 candidates at once, first around the estimate from the inverse
+// scale function and then evenly spread over the bracket that is left, so a
+// cluster costs about one scale-function evaluation of latency.  The prior
+// records' merged positions and weight prefixes are staged in LDS when they
+// fit.  Same boundaries as chain_next (the predicate is evaluated with the
+// same arithmetic; only the order of the probes differs).
+constexpr int kChainWaves = 4;  // segments per block
+constexpr int kChainLds = 512;  // prior records staged per segment
+
+__global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const SegDesc* segs,
+                                                               const SelSeg* sel, int nsel,
+                                                               const double* se,
+                                                               const int64_t* pose, SelState* st,
+                                                               SelBound* bnd) {
+  __shared__ int64_t lpos[kChainWaves][kChainLds];
+  __shared__ double lse[kChainWaves][kChainLds + 1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = blockIdx.x * kChainWaves + wv;
+  const bool active = j < nsel && !st[j].fail;
+  SelSeg g{};
+  SegDesc sd{};
+  int64_t ne = 0;
+  const int64_t* pos = nullptr;
+  const double* sep = nullptr;
+  if (active) {
+    g = sel[j];
+    sd = segs[g.si];
+    ne = sd.e_cnt;
+    pos = pose + sd.es_off;
+    sep = se + sd.es_off + g.si;
+    if (ne <= kChainLds) {
+      for (int i = lane; i < ne; i += 64) lpos[wv][i] = pos[i];
+      for (int i = lane; i <= ne; i += 64) lse[wv][i] = sep[i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  if (ne <= kChainLds) {
+    pos = lpos[wv];
+    sep = lse[wv];
+  }
   const int64_t n = sd.v_cnt + sd.e_cnt;
+  const double N = sd.stored_n, nrm = sd.nrm, sf = sd.sf;
+  // Prior records before the current start s (the walk only moves up).
+  int64_t cur = 0;
+  // First index c >= cur whose predicate fails (monotone over [cur, ne):
+  // true then false), galloping up from the cursor: O(log distance).
+  auto gallop = [&](auto below) -> int64_t {
+    if (cur >= ne || !below(cur)) return cur;
+    int64_t a = cur, step = 1, b = cur + 1;  // below(a); b: unknown or ne
+    while (b < ne && below(b)) {
+      a = b;
+      step <<= 1;
+      b = a + step;
+    }
+    if (b > ne) b = ne;
+    while (b - a > 1) {
+      const int64_t mid = a + ((b - a) >> 1);
+      if (below(mid)) a = mid;
+      else b = mid;
+    }
+    return b;
+  };
+  // P(j) for j >= s: weight of the merged records before index j (PCur::P).
+  auto P = [&](int64_t jj) -> double {
+    const int64_t c = gallop([&](int64_t i) { return pos[i] < jj; });
+    return sf * (double)(jj - c) + sep[c];
+  };
+  auto kv = [&](int64_t jj) -> double { return kd_k(scale, P(jj) / N, nrm); };
+  // Smallest index with P >= T, estimated (index_for_weight), for T at or
+  // above the weight before s.
+  auto index_est = [&](double T) -> int64_t {
+    // Largest prior record c with weight-before(c) < T.
+    const int64_t lo = gallop([&](int64_t i) { return sf * (double)(pos[i] - i) + sep[i] < T; }) - 1;
+    const int64_t base_j = lo < 0 ? 0 : pos[lo] + 1;
+    const double base_w = lo < 0 ? 0.0 : sf * (double)(pos[lo] - lo) + sep[lo + 1];
+    if (base_w >= T) return base_j;
+    const double jd = (double)base_j + ceil((T - base_w) / sf);
+    const int64_t lim = lo + 1 < ne ? pos[lo + 1] : n;
+    if (!(jd < (double)lim)) return lim;
+    return (int64_t)jd;
+  };
   SelBound* bd = bnd + g.c_off;
-  PCur pc{pose + sd.es_off, se + sd.es_off + g.si, sd.e_cnt, sd.sf, 0};
   int C = 0;
   int64_t s = 0;
+  double ks = kv(0);
   while (s < n) {
     if (C >= g.cap_c) {
-      st[j].fail = 1;
+      if (lane == 0) st[j].fail = 1;
       return;
     }
-    bd[C].b = s;
+    if (lane == 0) bd[C].b = s;
     C++;
     if (s == n - 1) break;
-    s = chain_next(scale, pc, s, n, sd.stored_n, sd.nrm);
+    // Bracket: pred(lo) false (or lo == s), pred(hi) true (or hi == n);
+    // klo = k(P(lo + 1) / N) when lo > s.
+    int64_t lo = s, hi = n;
+    double klo = 0.0;
+    bool klo_set = false;
+    const double qt = kinv_est(scale, ks + 1.0, nrm);
+    int64_t gg;
+    if (!(qt < 1.0)) gg = n - 1;
+    else if (!(qt > 0.0)) gg = s + 1;
+    else gg = index_est(qt * N) - 1;
+    int64_t base = gg - 31;
+    bool first = true;
+    while (hi - lo > 1) {
+      // Candidates: a window of 64 at the estimate, then 64 spread evenly.
+      const int64_t span = hi - lo - 1;
+      int64_t cand;
+      if (first) {
+        if (base > hi - 64) base = hi - 64;
+        if (base < lo + 1) base = lo + 1;
+        cand = base + lane;
+      } else {
+        cand = lo + 1 + (span >= 64 ? (int64_t)lane * span / 64 : (int64_t)lane);
+      }
+      first = false;
+      const bool live = cand < hi;
+      double kr = 0.0;
+      bool p = true;  // lanes past hi count as true
+      if (live) {
+        kr = kv(cand + 1);
+        p = !(kr - ks <= 1.0);
+      }
+      // Candidates increase with the lane: the first true lane closes the
+      // bracket from above, the live false lanes before it from below.
+      const uint64_t tm = __ballot(p);
+      const uint64_t fm = __ballot(live && !p);
+      uint64_t below = fm;
+      if (tm != 0ull) {
+        const int ft = __ffsll((long long)tm) - 1;
+        const int64_t cand_ft = __shfl(cand, ft, 64);
+        if (__shfl((int)live, ft, 64) != 0) hi = cand_ft;
+        below = fm & ((1ull << ft) - 1ull);  // ft < 64; ft == 0 leaves none
+      }
+      if (below != 0ull) {
+        const int lf = 63 - __clzll((long long)below);
+        lo = __shfl(cand, lf, 64);
+        klo = __shfl(kr, lf, 64);
+        klo_set = true;
+      }
+    }
+    // Next start: hi.  Its k value was computed for candidate lo = hi - 1
+    // (k(P(lo + 1) / N)) unless lo is still the previous start.
+    s = hi;
+    while (cur < ne && pos[cur] < s) cur++;
+    if (s < n) ks = klo_set ? klo : kv(s);
   }
-  st[j].ncl = C;
+  if (lane == 0) st[j].ncl = C;
 }
 
 // Per segment, after the chain: the boundary records, the buckets to gather,
@@ -2487,9 +2646,18 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
 }
 
 // --------------------------------------------------------- 6. finish
-__global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
-                         double* cw, const double* X, const double* vs, SegResult* res) {
-  int si = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per segment (kFinishWaves per block): the weights are loaded 64
+// at a time and summed in centroid order through lane broadcasts (the same
+// sequential sums as one lane walking them), and the prune compacts by
+// ballot.
+constexpr int kFinishWaves = 4;
+
+__global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* segs, int nseg,
+                                                             const int64_t* ccount, double* cm,
+                                                             double* cw, const double* X,
+                                                             const double* vs, SegResult* res) {
+  const int lane = threadIdx.x & 63;
+  const int si = blockIdx.x * kFinishWaves + (threadIdx.x >> 6);
   if (si >= nseg) return;
   SegDesc sd = segs[si];
   int64_t C = ccount[si];
@@ -2497,25 +2665,43 @@ __global__ void k_finish(const SegDesc* segs, int nseg, const int64_t* ccount, d
   double* w = cw + sd.x_off;
   // _gpu_batch_finish (tdigest.mojo:304-313): total of the new centroids.
   double total = 0.0;
-  for (int64_t c = 0; c < C; c++) total += w[c];
-  // _prune (tdigest.mojo:441-458) when decay is active (floor > 0).
+  for (int64_t c0 = 0; c0 < C; c0 += 64) {
+    const double wl = c0 + lane < C ? w[c0 + lane] : 0.0;
+    const int cnt = (int)std::min<int64_t>(64, C - c0);
+    for (int l = 0; l < cnt; l++) total += __shfl(wl, l, 64);
+  }
+  // _prune (tdigest.mojo:441-458) when decay is active (floor > 0): kept
+  // centroids move down in order (each chunk is read before it is written,
+  // and writes land at or below the reads), pruned weights are summed in
+  // order.
   double removed = 0.0;
   if (sd.floor > 0.0) {
     int64_t k = 0;
-    for (int64_t c = 0; c < C; c++) {
-      if (w[c] >= sd.floor) {
-        m[k] = m[c];
-        w[k] = w[c];
-        k++;
-      } else {
-        removed += w[c];
+    for (int64_t c0 = 0; c0 < C; c0 += 64) {
+      const bool valid = c0 + lane < C;
+      const double wl = valid ? w[c0 + lane] : 0.0;
+      const double ml = valid ? m[c0 + lane] : 0.0;
+      const bool keep = valid && wl >= sd.floor;
+      const uint64_t km = __ballot(keep);
+      uint64_t rm = __ballot(valid && !keep);
+      while (rm) {
+        const int l = __ffsll((long long)rm) - 1;
+        removed += __shfl(wl, l, 64);
+        rm &= rm - 1ull;
       }
+      if (keep) {
+        const int64_t dst = k + __popcll(km & ((1ull << lane) - 1ull));
+        m[dst] = ml;
+        w[dst] = wl;
+      }
+      k += __popcll(km);
     }
     if (removed > 0.0) {
       total -= removed;
       C = k;
     }
   }
+  if (lane != 0) return;
   SegResult r;
   r.count = C;
   r.total = total;
@@ -2893,7 +3079,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       launched("k_sel_sample<1024>");
       k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl, dst);
       launched("k_sel_sample<4096>");
-      k_sel_sample<kSelSample, kBlock, 4096><<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
+      // The largest class fills a CU's LDS anyway: a block of kSelThreads.
+      k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
                                                                           dst);
       launched("k_sel_sample<8192>");
       phase_end(ph_);
@@ -2958,7 +3145,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_ranks<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, esm, pose,
                                                      dst);
       launched("k_sel_ranks");
-      k_sel_chain<<<(nsel + 63) / 64, 64, 0, stream_>>>(scale, dsegs, dsel, nsel, se, pose, dst, dbnd);
+      k_sel_chain<<<(nsel + kChainWaves - 1) / kChainWaves, 64 * kChainWaves, 0, stream_>>>(
+          scale, dsegs, dsel, nsel, se, pose, dst, dbnd);
       launched("k_sel_chain");
       k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, pose, dst,
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);
@@ -3076,7 +3264,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   }
   {
     int ph_ = phase_begin("finish", 0.0);
-    k_finish<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
+    k_finish<<<(nseg + kFinishWaves - 1) / kFinishWaves, 64 * kFinishWaves, 0, stream_>>>(
+        dsegs, nseg, ccount, cm, cw, x, vs, dres);
     launched("k_finish");
     phase_end(ph_);
   }
