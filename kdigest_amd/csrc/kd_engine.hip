@@ -1993,6 +1993,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     double* csl, SelMark* mk, int2* mlist, unsigned int* mlist_cnt) {
   __shared__ uint32_t part[kSelThreads];
   __shared__ int s_fail, s_base;
+  // Bucket ranks and boundary ranks in LDS: every search below runs there.
+  __shared__ unsigned int sB[kSelSplMax + 2];
+  __shared__ int64_t svr[kSelCmax];
   const int j = blockIdx.x;
   if (st[j].fail) return;
   const SelSeg g = sel[j];
@@ -2012,6 +2015,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
   if (tid == 0) s_fail = 0;
   const int C = st[j].ncl;
+  for (int i = tid; i <= S + 1; i += blockDim.x) sB[i] = B[i];
+  __syncthreads();
   // 4. Boundary records and the buckets to gather.
   for (int i = tid; i < C; i += blockDim.x) {
     int64_t b = bd[i].b;
@@ -2023,12 +2028,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     }
     int64_t vr = b - l;
     bd[i].vr = vr;
+    svr[i] = vr;
     bd[i].v_hi = !(l < ne && pos[l] == b);
     bd[i].v_lo = b > 0 && !(l > 0 && pos[l - 1] == b - 1);
     bd[i].dup = 0;
     if (i > 0) {
-      if (vr < nv) info[sel_bucket_of_rank(B, S, vr)] = -1;
-      if (vr >= 1) info[sel_bucket_of_rank(B, S, vr - 1)] = -1;
+      if (vr < nv) info[sel_bucket_of_rank(sB, S, vr)] = -1;
+      if (vr >= 1) info[sel_bucket_of_rank(sB, S, vr - 1)] = -1;
     }
   }
   __syncthreads();
@@ -2071,17 +2077,24 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
       gbase += c[i];
     }
   }
-  // 6. Cluster of every other bucket: boundaries i >= 1 with vr_i <= its start.
+  // 6. Cluster of every other bucket: boundaries i >= 1 with vr_i <= its
+  // start (a thread's buckets are consecutive, so a cursor follows them).
+  int l6 = -1;
   for (int i = lo0; i < hi0; i++) {
     if (info[i] >= 0) {
-      int64_t r = B[i];
-      int l = 1, h = C;
-      while (l < h) {
-        int mid = (l + h) >> 1;
-        if (bd[mid].vr <= r) l = mid + 1;
-        else h = mid;
+      const int64_t r = sB[i];
+      if (l6 < 0) {
+        int l = 1, h = C;
+        while (l < h) {
+          int mid = (l + h) >> 1;
+          if (svr[mid] <= r) l = mid + 1;
+          else h = mid;
+        }
+        l6 = l;
+      } else {
+        while (l6 < C && svr[l6] <= r) l6++;
       }
-      info[i] = l - 1;
+      info[i] = l6 - 1;
     }
   }
   // 7. Slice exponent per cluster from the value range of its buckets.
@@ -2090,7 +2103,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     int64_t va = bd[k].vr, vb = k + 1 < C ? bd[k + 1].vr : nv;
     int e = 0;
     if (vb > va) {
-      int bf = sel_bucket_of_rank(B, S, va), bl = sel_bucket_of_rank(B, S, vb - 1);
+      int bf = sel_bucket_of_rank(sB, S, va), bl = sel_bucket_of_rank(sB, S, vb - 1);
       double lo = bf == 0 ? vmin : sp[bf - 1];
       double hi = bl == S ? vmax : sp[bl];
       double m = fmax(fabs(lo), fabs(hi));
