@@ -2659,11 +2659,70 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
 }
 
 // --------------------------------------------------------- 6. finish
-// One wave per segment (kFinishWaves per block): the weights are loaded 64
-// at a time and summed in centroid order through lane broadcasts (the same
-// sequential sums as one lane walking them), and the prune compacts by
-// ballot.
+// Result of a segment: count, total, extremes.
+__device__ inline void finish_result(const SegDesc& sd, int si, int64_t C, double total,
+                                     double removed, const double* X, const double* vs,
+                                     SegResult* res) {
+  SegResult r;
+  r.count = C;
+  r.total = total;
+  r.removed = removed;
+  r.replays = res[si].replays;
+  r.path = sd.mode;
+  if (sd.mode != 0) {
+    r.vmin = res[si].vmin;
+    r.vmax = res[si].vmax;
+  } else if (sd.v_cnt > 0) {
+    const double* v = sd.e_cnt == 0 ? X + sd.x_off : vs + sd.v_off;
+    r.vmin = v[0];
+    r.vmax = v[sd.v_cnt - 1];
+  } else {
+    r.vmin = 0.0;
+    r.vmax = 0.0;
+  }
+  res[si] = r;
+}
+
+// Many segments: one lane each, walking its centroids.
+__global__ void k_finish_lane(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
+                              double* cw, const double* X, const double* vs, SegResult* res) {
+  int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nseg) return;
+  SegDesc sd = segs[si];
+  int64_t C = ccount[si];
+  double* m = cm + sd.x_off;
+  double* w = cw + sd.x_off;
+  // _gpu_batch_finish (tdigest.mojo:304-313): total of the new centroids.
+  double total = 0.0;
+  for (int64_t c = 0; c < C; c++) total += w[c];
+  // _prune (tdigest.mojo:441-458) when decay is active (floor > 0).
+  double removed = 0.0;
+  if (sd.floor > 0.0) {
+    int64_t k = 0;
+    for (int64_t c = 0; c < C; c++) {
+      if (w[c] >= sd.floor) {
+        m[k] = m[c];
+        w[k] = w[c];
+        k++;
+      } else {
+        removed += w[c];
+      }
+    }
+    if (removed > 0.0) {
+      total -= removed;
+      C = k;
+    }
+  }
+  finish_result(sd, si, C, total, removed, X, vs, res);
+}
+
+// Few segments (a lone digest's centroids are a long dependent walk for one
+// lane): one wave per segment (kFinishWaves per block); the weights are
+// loaded 64 at a time and summed in centroid order through lane broadcasts
+// (the same sequential sums as one lane walking them), and the prune
+// compacts by ballot.
 constexpr int kFinishWaves = 4;
+constexpr int kFinishWaveSegs = 2048;  // at most this many segments take the wave form
 
 __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* segs, int nseg,
                                                              const int64_t* ccount, double* cm,
@@ -2714,25 +2773,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* seg
       C = k;
     }
   }
-  if (lane != 0) return;
-  SegResult r;
-  r.count = C;
-  r.total = total;
-  r.removed = removed;
-  r.replays = res[si].replays;
-  r.path = sd.mode;
-  if (sd.mode != 0) {
-    r.vmin = res[si].vmin;
-    r.vmax = res[si].vmax;
-  } else if (sd.v_cnt > 0) {
-    const double* v = sd.e_cnt == 0 ? X + sd.x_off : vs + sd.v_off;
-    r.vmin = v[0];
-    r.vmax = v[sd.v_cnt - 1];
-  } else {
-    r.vmin = 0.0;
-    r.vmax = 0.0;
-  }
-  res[si] = r;
+  if (lane == 0) finish_result(sd, si, C, total, removed, X, vs, res);
 }
 
 // ---------------------------------------------------------------- engine
@@ -3277,8 +3318,11 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   }
   {
     int ph_ = phase_begin("finish", 0.0);
-    k_finish<<<(nseg + kFinishWaves - 1) / kFinishWaves, 64 * kFinishWaves, 0, stream_>>>(
-        dsegs, nseg, ccount, cm, cw, x, vs, dres);
+    if (nseg <= kFinishWaveSegs)
+      k_finish<<<(nseg + kFinishWaves - 1) / kFinishWaves, 64 * kFinishWaves, 0, stream_>>>(
+          dsegs, nseg, ccount, cm, cw, x, vs, dres);
+    else
+      k_finish_lane<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
     launched("k_finish");
     phase_end(ph_);
   }
