@@ -1840,6 +1840,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
 // fit.  Same boundaries as chain_next (the predicate is evaluated with the
 // same arithmetic; only the order of the probes differs).
 constexpr int kChainWaves = 4;  // segments per block
+constexpr int kChainGroups = 6;  // clusters tested per speculative pass
+// First lane of each speculative group (and 64).
+__constant__ const int kChainGroup[kChainGroups + 1] = {0, 4, 12, 22, 34, 48, 64};
 constexpr int kChainLds = 512;  // prior records staged per segment
 
 __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const SegDesc* segs,
@@ -1917,16 +1920,17 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
   };
   SelBound* bd = bnd + g.c_off;
   int C = 0;
-  int64_t s = 0;
-  double ks = kv(0);
-  while (s < n) {
-    if (C >= g.cap_c) {
-      if (lane == 0) st[j].fail = 1;
-      return;
-    }
-    if (lane == 0) bd[C].b = s;
+  // Records a cluster start; false when the segment's capacity is used up.
+  auto record = [&](int64_t b) -> bool {
+    if (C >= g.cap_c) return false;
+    if (lane == 0) bd[C].b = b;
     C++;
-    if (s == n - 1) break;
+    return true;
+  };
+  // Exact search for the cluster started at s (k value ks): a window of 64
+  // candidates at the estimate, then 64 spread evenly over the bracket left.
+  // Returns the first closing candidate (n when none) and its k value.
+  auto exact_next = [&](int64_t s, double ks, double* ks_next) -> int64_t {
     // Bracket: pred(lo) false (or lo == s), pred(hi) true (or hi == n);
     // klo = k(P(lo + 1) / N) when lo > s.
     int64_t lo = s, hi = n;
@@ -1940,7 +1944,6 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     int64_t base = gg - 31;
     bool first = true;
     while (hi - lo > 1) {
-      // Candidates: a window of 64 at the estimate, then 64 spread evenly.
       const int64_t span = hi - lo - 1;
       int64_t cand;
       if (first) {
@@ -1976,11 +1979,96 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
         klo_set = true;
       }
     }
-    // Next start: hi.  Its k value was computed for candidate lo = hi - 1
-    // (k(P(lo + 1) / N)) unless lo is still the previous start.
-    s = hi;
+    // k of the next start hi: computed for candidate lo = hi - 1 unless lo
+    // is still s.
+    *ks_next = hi < n ? (klo_set ? klo : kv(hi)) : 0.0;
+    return hi;
+  };
+  int64_t s = 0;
+  double ks = kv(0);
+  if (!record(0)) {
+    if (lane == 0) st[j].fail = 1;
+    return;
+  }
+  while (s < n - 1) {
+    // Speculative pass: lane group d tests consecutive candidates around
+    // the estimated boundary of the cluster d + 1 ahead (level ks + 1 + d).
+    // Each cluster's true level is at most one record's k step below the
+    // previous one plus one, so the estimate of group d lands within a
+    // record or two for d = 0 and drifts by up to about a record per
+    // cluster after that: the groups widen downwards (4, 8, 10, 12, 14 and
+    // 16 lanes, each window ending four records above its estimate).  The
+    // groups are then checked in order with the exact predicate and the
+    // exact k of each new start; a group whose window does not bracket its
+    // boundary ends the pass.
+    const int grp = (lane >= 4) + (lane >= 12) + (lane >= 22) + (lane >= 34) + (lane >= 48);
+    const int gstart = kChainGroup[grp];
+    const int gwidth = kChainGroup[grp + 1] - gstart;
+    const double qs = kinv_est(scale, ks + 1.0 + (double)grp, nrm);
+    int64_t gs;
+    if (!(qs < 1.0)) gs = n - 1;
+    else if (!(qs > 0.0)) gs = s + 1;
+    else gs = index_est(qs * N) - 1;
+    const int64_t cand = gs + (grp == 0 ? -1 : 5 - gwidth) + (lane - gstart);
+    const bool low = cand < s;    // before the current start: not evaluated
+    const bool over = cand >= n;  // past the last record: a virtual true
+    double kr = 0.0;
+    if (!low && !over) kr = kv(cand + 1);
+    int64_t sp = s;
+    double ksp = ks;
+    int done = 0;
+    bool end = false;
+    for (int d = 0; d < kChainGroups; d++) {
+      const uint64_t gm = ((d + 1 < kChainGroups ? 1ull << kChainGroup[d + 1] : 0ull) - 1ull) &
+                          ~((1ull << kChainGroup[d]) - 1ull);
+      const bool valid = cand > sp;
+      const bool p = over || (!low && !(kr - ksp <= 1.0));
+      const uint64_t tm = __ballot(valid && p) & gm;
+      if (tm == 0ull) break;
+      const int f = __ffsll((long long)tm) - 1;
+      const int64_t b = __shfl(cand, f, 64);
+      // Bracketed when b is the first candidate after sp or the lane before
+      // f (candidate b - 1 >= sp, evaluated) is in the group: then its
+      // k(P(b) / N) is the next start's k.
+      double kb;
+      const double kprev = __shfl(kr, f > 0 ? f - 1 : 0, 64);
+      if (f == kChainGroup[d]) {
+        if (b != sp + 1) break;
+        kb = b < n ? kv(b) : 0.0;
+      } else {
+        kb = kprev;
+      }
+      done++;
+      if (b >= n) {  // the open cluster runs to the end
+        end = true;
+        break;
+      }
+      if (!record(b)) {
+        if (lane == 0) st[j].fail = 1;
+        return;
+      }
+      sp = b;
+      ksp = kb;
+      if (b == n - 1) {  // the last record closes its own cluster
+        end = true;
+        break;
+      }
+    }
+    if (end) break;
+    if (done == 0) {
+      double kn;
+      const int64_t b = exact_next(s, ks, &kn);
+      if (b >= n) break;
+      if (!record(b)) {
+        if (lane == 0) st[j].fail = 1;
+        return;
+      }
+      sp = b;
+      ksp = kn;
+    }
+    s = sp;
+    ks = ksp;
     while (cur < ne && pos[cur] < s) cur++;
-    if (s < n) ks = klo_set ? klo : kv(s);
   }
   if (lane == 0) st[j].ncl = C;
 }
