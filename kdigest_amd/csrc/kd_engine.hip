@@ -1645,17 +1645,18 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   // The search runs on 32 bits of each splitter's order key (4-byte LDS
   // entries: half the bank traffic of the doubles): the bits right after the
   // prefix that every splitter shares, so that nearly every value differs
-  // from every splitter there.  A table of kSlots slots over the splitters'
-  // key range (monotone float map) gives each value the few splitters it
-  // must still be compared with.  Values below the first or at or above the
-  // last splitter are placed directly; splitters sharing the 32 bits with a
-  // value are compared exactly (from global memory: rare).
+  // from every splitter there.  A table of kSlots slots over the key range
+  // of the central splitters (monotone float map) gives each value the few
+  // splitters it must still be compared with, by a binary search that each
+  // wave ends when none of its values has any left.  Values below the first
+  // or at or above the last splitter are placed directly; splitters sharing
+  // the 32 bits with a value are compared exactly (from global memory:
+  // rare).
   constexpr int kSlots = 4096;
   __shared__ uint32_t sk[SMAX + 1];
   __shared__ unsigned int lc[SMAX + 1];
   __shared__ uint16_t slot[kSlots + 1];
   __shared__ unsigned long long smin, smax;
-  __shared__ int s_occ;
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
   const int j = sel_tile_seg(sel, nsel, t);
@@ -1666,20 +1667,26 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
   const double sp0 = S > 0 ? sg[0] : 0.0, spl_last = S > 0 ? sg[S - 1] : 0.0;
   const unsigned long long kf = ord_key(sp0), kl = ord_key(spl_last);
   const int sh = kf == kl ? 32 : min(32, __clzll(kf ^ kl));
-  const uint32_t k0 = (uint32_t)((kf << sh) >> 32), kS = (uint32_t)((kl << sh) >> 32);
-  const float fscale = (float)kSlots / ((float)(kS - k0) + 1.0f);
-  auto slot_of = [&](uint32_t k) {
-    int h = (int)((float)(k - k0) * fscale);
-    return h < kSlots - 1 ? h : kSlots - 1;
-  };
   for (int i = threadIdx.x; i < S; i += blockDim.x) sk[i] = (uint32_t)((ord_key(sg[i]) << sh) >> 32);
   for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
   if (threadIdx.x == 0) {
     smin = ~0ull;
     smax = 0ull;
-    s_occ = 0;
   }
   __syncthreads();
+  // The slots span the splitters tl..th: all but the outer 1/256 on each
+  // side when there are many, since a heavy tail would otherwise squeeze
+  // the bulk of the splitters into a few slots.  Keys outside that range
+  // land in the two edge slots (clamped), whose searches stay exact.
+  const int tl = S >= 512 ? S / 256 : 0;
+  const int th = S >= 512 ? S - 1 - S / 256 : (S > 0 ? S - 1 : 0);
+  const uint32_t k0 = S > 0 ? sk[tl] : 0u, kS = S > 0 ? sk[th] : 0u;
+  const float fscale = (float)kSlots / ((float)(kS - k0) + 1.0f);
+  auto slot_of = [&](uint32_t k) {
+    if (k <= k0) return 0;
+    int h = (int)((float)(k - k0) * fscale);
+    return h < kSlots - 1 ? h : kSlots - 1;
+  };
   // slot[h] = number of splitters mapped below h (slot[kSlots] = S): each
   // splitter fills the slots between its predecessor's and its own.
   for (int i = threadIdx.x; i <= S; i += blockDim.x) {
@@ -1688,17 +1695,6 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
     for (int h = hp + 1; h <= hi; h++) slot[h] = (uint16_t)i;
   }
   __syncthreads();
-  {
-    int occ = 0;
-    for (int h = threadIdx.x; h < kSlots; h += blockDim.x) {
-      const int c = (int)slot[h + 1] - (int)slot[h];
-      occ = c > occ ? c : occ;
-    }
-    atomicMax(&s_occ, occ);
-  }
-  __syncthreads();
-  int steps = 0;
-  while ((1 << steps) <= s_occ) steps++;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
@@ -1716,14 +1712,18 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
       int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       x[u] = i < b ? v[i] : 0.0;
       const bool in = S > 0 && !(x[u] < sp0) && x[u] < spl_last;
-      kx[u] = in ? (uint32_t)((ord_key(x[u]) << sh) >> 32) : k0;
-      const int h = slot_of(kx[u]);
-      lo[u] = slot[h];
-      len[u] = (int)slot[h + 1] - lo[u];
+      kx[u] = in ? (uint32_t)((ord_key(x[u]) << sh) >> 32) : 0u;
+      const int h = in ? slot_of(kx[u]) : 0;
+      lo[u] = in ? (int)slot[h] : 0;
+      len[u] = in ? (int)slot[h + 1] - lo[u] : 0;
     }
     // lo = number of splitters whose searched bits are below the value's:
     // all of them are below the value.
-    for (int it = 0; it < steps; it++) {
+    for (;;) {
+      bool more = false;
+#pragma unroll
+      for (int u = 0; u < U; u++) more = more || len[u] > 0;
+      if (__ballot(more) == 0ull) break;
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int half = len[u] >> 1;
