@@ -44,8 +44,13 @@ void launched(const char* kernel) {
 
 void* DevBuf::reserve(size_t bytes) {
   if (bytes <= cap) return p;
+  // Growth is geometric (a buffer that has grown grows by half again), so
+  // a stream of slowly growing requests reallocates, and with hipFree
+  // synchronizes the device, only a few times.
+  const size_t old = cap;
   release();
   size_t want = std::max<size_t>(bytes, 256);
+  if (old > 0) want = std::max<size_t>(want, old + old / 2);
   check_hip(hipMalloc(&p, want), "hipMalloc");
   cap = want;
   return p;
