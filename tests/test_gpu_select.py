@@ -73,6 +73,37 @@ def test_select_matches_sort_and_oracle(kd, oracle, sessions, kind, scale):
                  max(1.0, float(np.abs(x).max())), f"{kind} K{scale} quantile")
 
 
+@pytest.mark.parametrize("scale", [0, 1, 2, 3])
+def test_select_many_prior_centroids(kd, oracle, sessions, scale):
+    """Compression 1500: batches merging into more than 512 prior
+    centroids (the chain then reads their positions from memory instead of
+    LDS) give the sort path's clusters and the oracle's."""
+    s_sel, s_srt = sessions
+    rng = np.random.default_rng(900 + scale)
+    a = kd.TDigest(scale, 1500.0, session=s_sel)
+    b = kd.TDigest(scale, 1500.0, session=s_srt)
+    r = oracle.Digest(scale, 1500.0)
+    big_prior = False
+    sizes = []
+    for n in (300001, 200000, 400000):
+        x = rng.standard_t(4, n) * 10.0
+        prior = len(a)
+        sizes.append(prior)
+        before = s_sel.path_counts()["selection"]
+        a.add_batch(x)
+        b.add_batch(x)
+        r.add_batch(x)
+        if prior > 512 and s_sel.path_counts()["selection"] > before:
+            big_prior = True
+        ma, wa = a.centroids()
+        mb, wb = b.centroids()
+        rm, rw = r.centroids()
+        scale_abs = float(np.abs(x).max())
+        check_centroids(ma, wa, mb, wb, scale_abs, True, f"K{scale} n={n} sel vs sort")
+        check_centroids(ma, wa, rm, rw, scale_abs, True, f"K{scale} n={n} sel vs oracle")
+    assert big_prior, f"no selection merge with more than 512 prior centroids: {sizes}"
+
+
 @pytest.mark.parametrize("lam", [1e-5, 2e-3])
 def test_select_with_decay(kd, oracle, sessions, lam):
     s_sel, s_srt = sessions
