@@ -3324,7 +3324,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       unsigned grid = (unsigned)std::min<int64_t>(sel_m, 4096);
       k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
-      k_sel_sorted<1024, 64, 128><<<grid, 64, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
+      k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd,
+                                                                    x);
       launched("k_sel_sorted<1024>");
       k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk,
                                                                      dbnd, x);
