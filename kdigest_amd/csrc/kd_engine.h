@@ -248,8 +248,14 @@ class Engine {
   void sort_values(int nseg, const double* d_v, int64_t total_v);
   // Launches of a planned compress: the segment table, small-class lists,
   // selection segments and E-segment list are on the device already.
+  // part 0: every launch, deciding on the sort path for selection
+  // fallbacks with a sync; 1: the same without that sync (fallen-back
+  // segments are left unfinished and defer_pending_ is set, for the caller to
+  // run part 2 once it has synchronized); 2: the sort path and finish of the
+  // fallen-back segments only.
   void execute(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
-               const double* d_ew);
+               const double* d_ew, int part = 0);
+  bool defer_pending_ = false;
   DevBuf plan_sum_, plan_part_;
   PlanSummary* h_plan_ = nullptr;  // page-locked copy of the device plan summary
 
@@ -269,7 +275,7 @@ class Engine {
   DevBuf sel_seg_, sel_st_, sel_spl_, sel_cnt_, sel_bst_, sel_info_, sel_bnd_, sel_ce_, sel_csl_,
       sel_mk_, sel_ml_, sel_mlc_, sel_bix_;
   std::vector<char> h_sel_;
-  unsigned int h_fallbacks_ = 0;
+  unsigned int* h_fallbacks_ = nullptr;  // pinned: selection fallbacks of the last batch
   bool sel_on_ = true;
   int64_t sel_tile_min_ = 65536, sel_tiles_max_ = 64;
   int64_t path_counts_[3] = {0, 0, 0};

@@ -2777,11 +2777,14 @@ __device__ inline void finish_result(const SegDesc& sd, int si, int64_t C, doubl
 }
 
 // Many segments: one lane each, walking its centroids.
+// only: -1 every segment, 0 the sort path's (mode 0), 1 the others.
 __global__ void k_finish_lane(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
-                              double* cw, const double* X, const double* vs, SegResult* res) {
+                              double* cw, const double* X, const double* vs, SegResult* res,
+                              int only) {
   int si = blockIdx.x * blockDim.x + threadIdx.x;
   if (si >= nseg) return;
   SegDesc sd = segs[si];
+  if (only >= 0 && (sd.mode == 0) != (only == 0)) return;
   int64_t C = ccount[si];
   double* m = cm + sd.x_off;
   double* w = cw + sd.x_off;
@@ -2820,11 +2823,13 @@ constexpr int kFinishWaveSegs = 2048;  // at most this many segments take the wa
 __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* segs, int nseg,
                                                              const int64_t* ccount, double* cm,
                                                              double* cw, const double* X,
-                                                             const double* vs, SegResult* res) {
+                                                             const double* vs, SegResult* res,
+                                                             int only) {
   const int lane = threadIdx.x & 63;
   const int si = blockIdx.x * kFinishWaves + (threadIdx.x >> 6);
   if (si >= nseg) return;
   SegDesc sd = segs[si];
+  if (only >= 0 && (sd.mode == 0) != (only == 0)) return;
   int64_t C = ccount[si];
   double* m = cm + sd.x_off;
   double* w = cw + sd.x_off;
@@ -2923,6 +2928,10 @@ Engine::Engine(int device) : device_(device) {
   check_hip(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
   check_hip(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
   check_hip(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreate");
+  void* p = nullptr;
+  check_hip(hipHostMalloc(&p, sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
+  h_fallbacks_ = static_cast<unsigned int*>(p);
+  *h_fallbacks_ = 0;
 }
 
 Engine::~Engine() {
@@ -2932,6 +2941,7 @@ Engine::~Engine() {
   }
   for (auto e : pool_) (void)hipEventDestroy(e);
   if (h_plan_) (void)hipHostFree(h_plan_);
+  if (h_fallbacks_) (void)hipHostFree(h_fallbacks_);
   if (fork_) (void)hipEventDestroy(fork_);
   if (join_) (void)hipEventDestroy(join_);
   if (side_) (void)hipStreamDestroy(side_);
@@ -3150,19 +3160,29 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     SelSeg* dsel = as<SelSeg>(sel_seg_, hsel.size());
     check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
   }
-  execute(scale, P, d_v, d_em, d_ew);
+  defer_pending_ = false;
+  execute(scale, P, d_v, d_em, d_ew, 1);
   SegResult* dres = static_cast<SegResult*>(res_.p);
   // The call ends synchronized: host staging read by the asynchronous copies
   // above (segment table, lists) is then free to change.
   if (sync_results)
     check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
   check_hip(hipStreamSynchronize(stream_), "sync results");
+  // Selection fallbacks (rare): their segments go through the sort path now.
+  if (defer_pending_ && *h_fallbacks_ > 0) {
+    execute(scale, P, d_v, d_em, d_ew, 2);
+    if (sync_results)
+      check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
+    check_hip(hipStreamSynchronize(stream_), "sync results");
+  }
+  defer_pending_ = false;
   if (sync_results)
     for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : (r.path == 2 ? 2 : 1)]++;
 }
 
 void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
-                     const double* d_ew) {
+                     const double* d_ew, int part) {
+  const bool fb_only = part == 2;
   const int nseg = (int)P.nseg;
   const int64_t total_e = P.total_e, total_x = P.total_x, total_t = P.total_t, total_v = P.total_v;
   SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
@@ -3183,12 +3203,13 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   double* tsum = as<double>(tsum_, 2 * std::max<int64_t>(total_t, 1));
   int64_t* dtotal = as<int64_t>(clist_, 2);
 
-  if (P.n_eseg > 0) {
+  // (Part 2 reuses the sorted prior records of part 1.)
+  if (P.n_eseg > 0 && !fb_only) {
     int ph_ = phase_begin("sort_e", 32.0 * (double)total_e);
     k_sort_e<<<nseg, kBlock, 0, stream_>>>(dsegs, nseg, d_em, d_ew, esm, esw, se);
     launched("k_sort_e");
     phase_end(ph_);
-  } else {
+  } else if (!fb_only) {
     check_hip(hipMemsetAsync(se, 0, (total_e + nseg) * sizeof(double), stream_), "memset se");
   }
   const int nsel = (int)P.nsel;
@@ -3203,7 +3224,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   SelMark* dmk = nullptr;
   int2* dml = nullptr;
   uint16_t* dbix = nullptr;
-  if (nsel > 0) {
+  if (nsel > 0 && !fb_only) {
     dst = as<SelState>(sel_st_, nsel);
     dspl = as<double>(sel_spl_, sel_b);
     dcnt = as<unsigned int>(sel_cnt_, sel_b);
@@ -3239,7 +3260,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   bool forked = false;
   int64_t n_small = 0;
   for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
-  if (n_small > 0) {
+  if (n_small > 0 && !fb_only) {
     hipStream_t on = stream_;
     if (overlap_) {
       check_hip(hipEventRecord(fork_, stream_), "fork");
@@ -3271,7 +3292,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     phase_end(ph_);
     if (forked) check_hip(hipEventRecord(join_, side_), "join");
   }
-  if (nsel > 0) {
+  if (nsel > 0 && !fb_only) {
     const double sel_bytes = 8.0 * P.sel_v;
     {
       int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
@@ -3342,18 +3363,26 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       launched("k_sel_finalize");
       phase_end(ph_);
     }
-    // Whether any selected segment fell back to sorting.
-    check_hip(hipMemcpyAsync(&h_fallbacks_, dmlc + 1, sizeof(unsigned int), hipMemcpyDeviceToHost,
+    // Whether any selected segment fell back to sorting (pinned: the copy
+    // does not wait for the stream).
+    check_hip(hipMemcpyAsync(h_fallbacks_, dmlc + 1, sizeof(unsigned int), hipMemcpyDeviceToHost,
                              stream_),
               "D2H fallbacks");
   }
   if (forked) check_hip(hipStreamWaitEvent(stream_, join_, 0), "join side stream");
   // The multi-kernel path runs when a segment needs it: one that neither the
   // selection nor the one-block path took, or a selection fallback.
-  bool sort_path = P.n_mode0 > 0;
+  bool sort_path = P.n_mode0 > 0 || fb_only;
+  int finish_only = fb_only ? 0 : -1;  // finish: mode-0 segments, or every one
   if (nsel > 0 && !sort_path) {
-    check_hip(hipStreamSynchronize(stream_), "sync fallbacks");
-    sort_path = h_fallbacks_ > 0;
+    if (part == 1) {
+      // Decided after the caller's sync: finish the other segments now.
+      defer_pending_ = true;
+      finish_only = 1;
+    } else {
+      check_hip(hipStreamSynchronize(stream_), "sync fallbacks");
+      sort_path = *h_fallbacks_ > 0;
+    }
   }
   if (sort_path) {
     auto t0 = std::chrono::steady_clock::now();
@@ -3414,9 +3443,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     int ph_ = phase_begin("finish", 0.0);
     if (nseg <= kFinishWaveSegs)
       k_finish<<<(nseg + kFinishWaves - 1) / kFinishWaves, 64 * kFinishWaves, 0, stream_>>>(
-          dsegs, nseg, ccount, cm, cw, x, vs, dres);
+          dsegs, nseg, ccount, cm, cw, x, vs, dres, finish_only);
     else
-      k_finish_lane<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres);
+      k_finish_lane<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres,
+                                                          finish_only);
     launched("k_finish");
     phase_end(ph_);
   }
@@ -3793,9 +3823,9 @@ PlanSummary Engine::compress_grouped(const GroupedPlanArgs& a, const double* d_v
   path_counts_[2] += n_small;
   path_counts_[0] += P.nsel;
   path_counts_[1] += P.n_mode0;
-  if (P.nsel > 0 && h_fallbacks_ > 0) {  // read by execute after its fallback sync
-    path_counts_[0] -= h_fallbacks_;
-    path_counts_[1] += h_fallbacks_;
+  if (P.nsel > 0 && *h_fallbacks_ > 0) {  // read by execute after its fallback sync
+    path_counts_[0] -= *h_fallbacks_;
+    path_counts_[1] += *h_fallbacks_;
   }
   return P;
 }
