@@ -286,6 +286,35 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_s
             "admission": st, "mem_budget_bytes": 64 << 20}
 
 
+def single_digest(kd, session, dev, single_n: int):
+    """One digest fed 2^20-value device batches (ScaleK2, compression 200,
+    half-life 1e5 samples): one warm run, then one timed run."""
+    import torch
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    xs = torch.exp(1.5 * torch.randn(single_n, generator=gen, dtype=torch.float64, device=dev))
+    batch = 1 << 20
+
+    def run_single():
+        d = kd.TDigest(kd.ScaleK2, 200.0, kd.Decay.half_life(1e5), session=session)
+        for a in range(0, single_n, batch):
+            d.add_batch_device(xs[a:a + batch])
+        return d
+
+    run_single()
+    session.synchronize()
+    t0 = time.perf_counter()
+    d = run_single()
+    session.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": f"one digest, {single_n} lognormal(0,1.5) FP64 values in batches "
+                        f"of {batch}, ScaleK2, compression 200, half-life 1e5",
+            "samples_per_s": single_n / dt, "ms": dt * 1e3, "centroids": len(d),
+            "batches": (single_n + batch - 1) // batch,
+            "roofline_frac": 24.0 * single_n / dt / 1e9 / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +335,8 @@ def main():
     ap.add_argument("--shape", type=str, default=None,
                     help="fixed grouped launch shape items,chunks,tile_min,tiles_max,overlap "
                          "(implies --no-autotune)")
+    ap.add_argument("--only-single", action="store_true",
+                    help="measure only the single-digest line (for its kernel profile)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -321,6 +352,13 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     session = kd.default_session(0)
+    if args.only_single:
+        single = single_digest(kd, session, dev, args.single_n)
+        print(json.dumps({"metric": "single-digest ingest samples/s", "value": single["samples_per_s"],
+                          "unit": "samples/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f64", "data": "synthetic (torch Philox, seeded), device-resident",
+                          "config": {"workload": single["workload"]}, "single_digest": single}))
+        return
     n, nk = args.n, args.keys
     print(f"[bench] generating {n} grouped pairs over {nk} keys", file=sys.stderr, flush=True)
     keys, vals = gen_grouped(n, nk, 0, dev)
@@ -448,28 +486,7 @@ def main():
     # ------------------------------------------------ single digest
     single = None
     if not args.no_single:
-        gen = torch.Generator(device=dev)
-        gen.manual_seed(1)
-        xs = torch.exp(1.5 * torch.randn(args.single_n, generator=gen, dtype=torch.float64, device=dev))
-        batch = 1 << 20
-
-        def run_single():
-            d = kd.TDigest(kd.ScaleK2, 200.0, kd.Decay.half_life(1e5), session=session)
-            for a in range(0, args.single_n, batch):
-                d.add_batch_device(xs[a:a + batch])
-            return d
-
-        run_single()
-        session.synchronize()
-        t0 = time.perf_counter()
-        d = run_single()
-        session.synchronize()
-        dt = time.perf_counter() - t0
-        single = {"workload": f"one digest, {args.single_n} lognormal(0,1.5) FP64 values in batches "
-                              f"of {batch}, ScaleK2, compression 200, half-life 1e5",
-                  "samples_per_s": args.single_n / dt, "ms": dt * 1e3, "centroids": len(d),
-                  "roofline_frac": 24.0 * args.single_n / dt / 1e9 / HBM_PEAK_GBS}
-        del xs
+        single = single_digest(kd, session, dev, args.single_n)
         print(f"[bench] single {single['samples_per_s'] / 1e9:.3f} G samples/s", file=sys.stderr, flush=True)
 
     # ------------------------------------------------ batched, decayed, merge tree
