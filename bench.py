@@ -331,6 +331,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-wire", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", type=str, default=None,
                     help="fixed grouped launch shape items,chunks,tile_min,tiles_max,overlap "
@@ -455,6 +456,35 @@ def main():
                    "request": "16 quantiles of one key, host arrays, synchronous"}
         del qk, qv, out
         print(f"[bench] queries {queries['queries_per_s'] / 1e9:.3f} G/s", file=sys.stderr, flush=True)
+    # ------------------------------------------------ wire format, every key
+    wire = None
+    if not args.no_wire:
+        g = digs[-1]
+        nbytes = g.wire_size()
+        wout = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        woff = torch.empty(nk + 1, dtype=torch.int64, device=dev)
+        g.to_bytes_device(wout, woff)
+        r = kd.GroupedTDigest.from_bytes_device(wout, woff, scale=kd.ScaleK3, session=session)
+        del r
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.to_bytes_device(wout, woff)
+        session.synchronize()
+        t_ser = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = kd.GroupedTDigest.from_bytes_device(wout, woff, scale=kd.ScaleK3, session=session)
+            del r
+        session.synchronize()
+        t_de = (time.perf_counter() - t0) / reps
+        wire = {"workload": f"every key of the {nk}-key grouped digest, per-digest wire layout, "
+                            "device to device",
+                "bytes": nbytes, "to_bytes_ms": t_ser * 1e3, "from_bytes_ms": t_de * 1e3,
+                "to_bytes_gb_s": nbytes / t_ser / 1e9, "from_bytes_gb_s": nbytes / t_de / 1e9}
+        del wout, woff
+        print(f"[bench] wire {wire['to_bytes_gb_s']:.1f} / {wire['from_bytes_gb_s']:.1f} GB/s",
+              file=sys.stderr, flush=True)
     del digs
 
     # ------------------------------------------------ plain per-key launches
@@ -535,6 +565,7 @@ def main():
         "plain_per_key": plain,
         "single_digest": single,
         "grouped_queries": queries,
+        "grouped_wire_format": wire,
         "batched_decay": batched,
         "serving": serve,
     }

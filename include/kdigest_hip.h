@@ -205,6 +205,27 @@ int kd_grouped_centroids(kd_grouped *g, int64_t key, double *means, double *weig
                          int64_t cap, int64_t *count);
 /* Per-key scalar state {stored_total, log_scale, clock, min, max}. */
 int kd_grouped_key_state(kd_grouped *g, int64_t key, double *out5);
+/* Wire format of every key at once: key k's digest in the per-digest layout
+ * (to_bytes, tdigest.mojo:627-652; byte-identical to kd_grouped_to_bytes for
+ * key k) occupies bytes [off[k], off[k+1]) of one buffer, keys in order.
+ * kd_grouped_wire_size gives the total (off[num_keys]).  The restore builds a
+ * grouped digest of num_keys keys from such blobs: every blob is checked
+ * first (KD_ETRUNC for a short one, KD_ESERDE for a bad version byte or
+ * count, or for a compression or decay rate that differs from key 0's) and
+ * nothing is created on an error.  The reference serializes one digest at a
+ * time and has no grouped restore (tdigest.mojo:627-698, grouped.mojo:
+ * 176-180): these batch it for checkpoint and resume of device-resident
+ * state.  _device: buffers and offsets in device memory; _all: host arrays. */
+int kd_grouped_wire_size(kd_grouped *g, int64_t *out_bytes);
+/* Compression and decay rate of a grouped digest (as created or restored). */
+int kd_grouped_params(kd_grouped *g, double *compression, double *lambda);
+int kd_grouped_to_bytes_device(kd_grouped *g, uint8_t *d_out, int64_t cap, int64_t *d_off);
+int kd_grouped_to_bytes_all(kd_grouped *g, uint8_t *out, int64_t cap, int64_t *off);
+int kd_grouped_from_bytes_device(kd_session *s, int scale, int64_t num_keys,
+                                 const uint8_t *d_data, int64_t n, const int64_t *d_off,
+                                 kd_grouped **out);
+int kd_grouped_from_bytes_all(kd_session *s, int scale, int64_t num_keys, const uint8_t *data,
+                              int64_t n, const int64_t *off, kd_grouped **out);
 
 /* ------------------------------------------- batched per-sample digests -- */
 /* num_digests independent TDigest[S] fed sample by sample with add_at

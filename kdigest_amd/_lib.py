@@ -72,6 +72,12 @@ SIGNATURES = {
     "kd_grouped_cdf": (_i, [_vp, _i64p, _dp, _i64, _dp]),
     "kd_grouped_quantile_device": (_i, [_vp, _vp, _vp, _i64, _vp]),
     "kd_grouped_to_bytes": (_i64, [_vp, _i64, _u8p, _i64]),
+    "kd_grouped_wire_size": (_i, [_vp, _i64p]),
+    "kd_grouped_params": (_i, [_vp, _dp, _dp]),
+    "kd_grouped_to_bytes_device": (_i, [_vp, _vp, _i64, _vp]),
+    "kd_grouped_to_bytes_all": (_i, [_vp, _u8p, _i64, _i64p]),
+    "kd_grouped_from_bytes_device": (_i, [_vp, _i, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
+    "kd_grouped_from_bytes_all": (_i, [_vp, _i, _i64, _u8p, _i64, _i64p, C.POINTER(_vp)]),
     "kd_grouped_centroid_count": (_i, [_vp, _i64, _i64p]),
     "kd_grouped_total_weight": (_i, [_vp, _i64, _dp]),
     "kd_grouped_centroids": (_i, [_vp, _i64, _dp, _dp, _i64, _i64p]),

@@ -1362,6 +1362,285 @@ extern "C" int64_t kd_grouped_to_bytes(kd_grouped* g, int64_t key, uint8_t* out,
   return rc == KD_OK ? r : rc;
 }
 
+// ------------------------------------------- wire format of every key at once
+// Every key's digest in the per-digest layout (tdigest.mojo:627-652, the
+// bytes kd_grouped_to_bytes gives for one key), concatenated in key order on
+// the device, and the inverse: a grouped digest restored from such blobs.
+// Checkpoint and resume of device-resident grouped state without a host
+// round trip per key.
+namespace {
+constexpr int kWireHeader = 65;  // version byte + 8 f64 (count as f64)
+
+__device__ inline uint64_t wire_u64(const uint8_t* p) {
+  uint64_t b = 0;
+  for (int i = 0; i < 8; i++) b |= (uint64_t)p[i] << (8 * i);
+  return b;
+}
+
+// Blob sizes: 65 + 16 * centroids.
+__global__ void k_wire_sizes(int64_t nk, const int64_t* off, int64_t* size) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nk) size[k] = kWireHeader + 16 * (off[k + 1] - off[k]);
+}
+
+// One wave per key writes its blob byte by byte (a blob starts at any byte).
+__global__ void k_wire_write(int64_t nk, const int64_t* off, const double* cm, const double* cw,
+                             const double* state, double compression, double lambda,
+                             const int64_t* woff, uint8_t* out) {
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (k >= nk) return;
+  const int64_t c0 = off[k], c = off[k + 1] - c0, base = woff[k];
+  const int64_t size = kWireHeader + 16 * c;
+  // Header order: compression, lambda, log scale, clock, stored total, min,
+  // max, count (tdigest.mojo:640-651).
+  for (int64_t i = lane; i < size; i += 64) {
+    uint8_t b;
+    if (i == 0) {
+      b = kSerdeVersion;
+    } else {
+      double v;
+      int byte;
+      if (i < kWireHeader) {
+        const int f = (int)((i - 1) >> 3);
+        byte = (int)((i - 1) & 7);
+        switch (f) {
+          case 0: v = compression; break;
+          case 1: v = lambda; break;
+          case 2: v = state[1 * nk + k]; break;
+          case 3: v = state[2 * nk + k]; break;
+          case 4: v = state[0 * nk + k]; break;
+          case 5: v = state[3 * nk + k]; break;
+          case 6: v = state[4 * nk + k]; break;
+          default: v = (double)c; break;
+        }
+      } else {
+        const int64_t q = i - kWireHeader;
+        const int64_t pidx = q >> 4;
+        byte = (int)(q & 7);
+        v = (q & 8) ? cw[c0 + pidx] : cm[c0 + pidx];
+      }
+      b = (uint8_t)(kd_d2bits(v) >> (8 * byte));
+    }
+    out[base + i] = b;
+  }
+}
+
+// Per key: header checks and scalars.  err[0] the first truncated key,
+// err[1] the first with a bad version byte or count, err[2] the first whose
+// compression or decay differs from key 0's (all nk when none).
+__global__ void k_wire_parse(int64_t nk, const uint8_t* data, int64_t n, const int64_t* woff,
+                             int64_t* cnt, double* state, unsigned long long* err) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  const int64_t a = woff[k], b = woff[k + 1];
+  cnt[k] = 0;
+  if (a < 0 || b < a || b > n) {
+    atomicMin(&err[0], (unsigned long long)k);
+    return;
+  }
+  if (b == a || data[a] != kSerdeVersion) {
+    atomicMin(&err[1], (unsigned long long)k);
+    return;
+  }
+  if (b - a < kWireHeader) {
+    atomicMin(&err[0], (unsigned long long)k);
+    return;
+  }
+  double h[8];
+  for (int i = 0; i < 8; i++) h[i] = kd_bits2d(wire_u64(data + a + 1 + 8 * i));
+  // The count is stored as a double: a whole number of centroids.
+  if (!(h[7] >= 0.0) || !(h[7] < 4503599627370496.0) || h[7] != floor(h[7])) {
+    atomicMin(&err[1], (unsigned long long)k);
+    return;
+  }
+  const int64_t c = (int64_t)h[7];
+  if (b - a < kWireHeader + 16 * c) {
+    atomicMin(&err[0], (unsigned long long)k);
+    return;
+  }
+  // Key 0's compression and decay rate (read only when its own header is
+  // in bounds; otherwise its thread reports it).
+  const int64_t a0 = woff[0], b0 = woff[1];
+  if (a0 >= 0 && b0 <= n && b0 - a0 >= kWireHeader) {
+    const uint64_t c0 = wire_u64(data + a0 + 1), l0 = wire_u64(data + a0 + 9);
+    if (kd_d2bits(h[0]) != c0 || kd_d2bits(h[1]) != l0) atomicMin(&err[2], (unsigned long long)k);
+  }
+  cnt[k] = c;
+  state[0 * nk + k] = h[4];  // stored total
+  state[1 * nk + k] = h[2];  // log scale
+  state[2 * nk + k] = h[3];  // clock
+  state[3 * nk + k] = h[5];  // min
+  state[4 * nk + k] = h[6];  // max
+}
+
+// One wave per key: its centroid pairs into the CSR.
+__global__ void k_wire_pairs(int64_t nk, const uint8_t* data, const int64_t* woff,
+                             const int64_t* off, double* cm, double* cw) {
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (k >= nk) return;
+  const int64_t c0 = off[k], c = off[k + 1] - c0;
+  const uint8_t* p = data + woff[k] + kWireHeader;
+  for (int64_t i = lane; i < c; i += 64) {
+    cm[c0 + i] = kd_bits2d(wire_u64(p + 16 * i));
+    cw[c0 + i] = kd_bits2d(wire_u64(p + 16 * i + 8));
+  }
+}
+}  // namespace
+
+// Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total).
+static void scan_offsets(kd_session* s, const int64_t* cnt, int64_t n, int64_t* off, DevBuf& scratch) {
+  const int64_t nb = (n + kOffChunk - 1) / kOffChunk;
+  int64_t* bsum = static_cast<int64_t*>(scratch.reserve((nb + 1) * sizeof(int64_t)));
+  k_off_sums<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(cnt, n, bsum);
+  launched("k_off_sums");
+  k_off_top<<<1, kOffChunk, 0, s->st()>>>(bsum, nb, off + n);
+  launched("k_off_top");
+  k_off_down<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(cnt, n, bsum, off);
+  launched("k_off_down");
+}
+
+extern "C" int kd_grouped_params(kd_grouped* g, double* compression, double* lambda) {
+  *compression = g->compression;
+  *lambda = g->lambda;
+  return KD_OK;
+}
+
+extern "C" int kd_grouped_wire_size(kd_grouped* g, int64_t* out_bytes) {
+  return guarded([&] {
+    *out_bytes = kWireHeader * g->num_keys + 16 * g->total_cent;
+    return KD_OK;
+  });
+}
+
+static void grouped_to_wire(kd_grouped* g, uint8_t* d_out, int64_t cap, int64_t* d_off) {
+  const int64_t nk = g->num_keys;
+  const int64_t need = kWireHeader * nk + 16 * g->total_cent;
+  if (cap < need) throw KdError(KD_EINVAL, "output buffer smaller than kd_grouped_wire_size");
+  kd_session* s = g->s;
+  int64_t* sizes = s->buf<int64_t>(s->tmp, nk);
+  k_wire_sizes<<<blocks_for(nk), 256, 0, s->st()>>>(nk, g->doff(), sizes);
+  launched("k_wire_sizes");
+  scan_offsets(s, sizes, nk, d_off, s->tmp2);
+  k_wire_write<<<blocks_for(nk * 64), 256, 0, s->st()>>>(nk, g->doff(), g->dm(), g->dw(),
+                                                         static_cast<double*>(g->state.p),
+                                                         g->compression, g->lambda, d_off, d_out);
+  launched("k_wire_write");
+}
+
+extern "C" int kd_grouped_to_bytes_device(kd_grouped* g, uint8_t* d_out, int64_t cap, int64_t* d_off) {
+  return guarded([&] {
+    grouped_to_wire(g, d_out, cap, d_off);
+    g->s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_to_bytes_all(kd_grouped* g, uint8_t* out, int64_t cap, int64_t* off) {
+  return guarded([&] {
+    const int64_t nk = g->num_keys;
+    const int64_t need = kWireHeader * nk + 16 * g->total_cent;
+    if (cap < need) throw KdError(KD_EINVAL, "output buffer smaller than kd_grouped_wire_size");
+    kd_session* s = g->s;
+    uint8_t* dout = s->buf<uint8_t>(s->v_buf, need);
+    int64_t* doff = s->buf<int64_t>(s->q_off, nk + 1);
+    grouped_to_wire(g, dout, need, doff);
+    s->d2h(out, dout, need);
+    s->d2h(off, doff, (nk + 1) * sizeof(int64_t));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+static kd_grouped* grouped_from_wire(kd_session* s, int scale, int64_t nk, const uint8_t* d_data,
+                                     int64_t n, const int64_t* d_off) {
+  if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+  if (nk <= 0 || nk > 0xffffffffLL) throw KdError(KD_EINVAL, "num_keys out of range");
+  std::unique_ptr<kd_grouped> g(new kd_grouped());
+  g->s = s;
+  g->scale = scale;
+  g->num_keys = nk;
+  g->stored_total.assign(nk, 0.0);
+  g->log_scale.assign(nk, 0.0);
+  g->clock.assign(nk, 0.0);
+  g->vmin.assign(nk, 0.0);
+  g->vmax.assign(nk, 0.0);
+  g->hoff.assign(nk + 1, 0);
+  g->off.reserve((nk + 1) * sizeof(int64_t));
+  g->state.reserve(5 * nk * sizeof(double));
+  void* p = nullptr;
+  hip_ok(hipHostMalloc(&p, 4 * sizeof(int64_t), hipHostMallocDefault), "hipHostMalloc");
+  g->h_tot = static_cast<int64_t*>(p);
+  int64_t* cnt = s->buf<int64_t>(s->tmp, nk + 3);
+  auto* err = reinterpret_cast<unsigned long long*>(cnt + nk);
+  const unsigned long long none[3] = {(unsigned long long)nk, (unsigned long long)nk,
+                                      (unsigned long long)nk};
+  s->h2d(err, none, sizeof(none));
+  k_wire_parse<<<blocks_for(nk), 256, 0, s->st()>>>(nk, d_data, n, d_off, cnt,
+                                                    static_cast<double*>(g->state.p), err);
+  launched("k_wire_parse");
+  unsigned long long herr[3];
+  s->d2h(herr, err, sizeof(herr));
+  s->sync();
+  if (herr[0] < (unsigned long long)nk)
+    throw KdError(KD_ETRUNC, "truncated digest for key " + std::to_string(herr[0]));
+  if (herr[1] < (unsigned long long)nk)
+    throw KdError(KD_ESERDE, "unrecognized digest serialization (version or count) for key " +
+                                 std::to_string(herr[1]));
+  if (herr[2] < (unsigned long long)nk)
+    throw KdError(KD_ESERDE, "key " + std::to_string(herr[2]) +
+                                 " has another compression or decay rate than key 0");
+  uint8_t h0[17];
+  int64_t w0 = 0;
+  s->d2h(&w0, d_off, sizeof(int64_t));
+  s->sync();
+  s->d2h(h0, d_data + w0, sizeof(h0));
+  scan_offsets(s, cnt, nk, g->doff(), s->tmp2);
+  s->d2h(g->h_tot, g->doff() + nk, sizeof(int64_t));
+  s->sync();
+  uint64_t cb = 0, lb = 0;
+  for (int i = 0; i < 8; i++) {
+    cb |= (uint64_t)h0[1 + i] << (8 * i);
+    lb |= (uint64_t)h0[9 + i] << (8 * i);
+  }
+  g->compression = kd_bits2d(cb);
+  g->lambda = kd_bits2d(lb);
+  g->total_cent = *g->h_tot;
+  g->cm.reserve(std::max<int64_t>(g->total_cent, 1) * sizeof(double));
+  g->cw.reserve(std::max<int64_t>(g->total_cent, 1) * sizeof(double));
+  k_wire_pairs<<<blocks_for(nk * 64), 256, 0, s->st()>>>(nk, d_data, d_off, g->doff(), g->dm(),
+                                                         g->dw());
+  launched("k_wire_pairs");
+  s->sync();
+  g->dirty = true;
+  g->prefix_valid = false;
+  return g.release();
+}
+
+extern "C" int kd_grouped_from_bytes_device(kd_session* s, int scale, int64_t num_keys,
+                                            const uint8_t* d_data, int64_t n, const int64_t* d_off,
+                                            kd_grouped** out) {
+  return guarded([&] {
+    *out = grouped_from_wire(s, scale, num_keys, d_data, n, d_off);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_grouped_from_bytes_all(kd_session* s, int scale, int64_t num_keys,
+                                         const uint8_t* data, int64_t n, const int64_t* off,
+                                         kd_grouped** out) {
+  return guarded([&] {
+    if (num_keys <= 0 || num_keys > 0xffffffffLL) throw KdError(KD_EINVAL, "num_keys out of range");
+    uint8_t* dd = s->buf<uint8_t>(s->v_buf, std::max<int64_t>(n, 1));
+    int64_t* doff = s->buf<int64_t>(s->q_off, num_keys + 1);
+    s->h2d(dd, data, n);
+    s->h2d(doff, off, (num_keys + 1) * sizeof(int64_t));
+    *out = grouped_from_wire(s, scale, num_keys, dd, n, doff);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_grouped_centroid_count(kd_grouped* g, int64_t key, int64_t* out) {
   return guarded([&] {
     g->check_key(key);

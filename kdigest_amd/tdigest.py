@@ -448,6 +448,70 @@ class GroupedTDigest:
         L.kd_grouped_to_bytes(self._h, int(key), buf, need)
         return bytes(buf)
 
+    def wire_size(self) -> int:
+        """Bytes of every key's serialized digest (to_bytes_all)."""
+        o = C.c_int64()
+        check(load().kd_grouped_wire_size(self._h, C.byref(o)))
+        return o.value
+
+    def to_bytes_all(self):
+        """Every key's to_bytes, concatenated in key order, and the offsets
+        (num_keys + 1) of each key's bytes."""
+        need = self.wire_size()
+        buf = (C.c_uint8 * max(need, 1))()
+        off = np.empty(self.num_keys + 1, dtype=np.int64)
+        check(load().kd_grouped_to_bytes_all(self._h, buf, need, off.ctypes.data_as(_i64p)))
+        return bytes(buf)[:need], off
+
+    def to_bytes_device(self, out, off, cap: int | None = None) -> None:
+        """to_bytes_all into device memory (torch tensors or addresses): out
+        of at least wire_size() bytes (cap: its size when an address), off of
+        num_keys + 1 int64."""
+        cap = int(out.numel()) if cap is None else int(cap)
+        check(load().kd_grouped_to_bytes_device(self._h, _device_ptr(out), cap, _device_ptr(off)))
+
+    @classmethod
+    def _adopt(cls, h, num_keys, scale, session):
+        g = cls.__new__(cls)
+        g._h = h
+        g._scale = _scale(scale)
+        g.num_keys = int(num_keys)
+        g.session = session
+        comp, lam = C.c_double(), C.c_double()
+        check(load().kd_grouped_params(h, C.byref(comp), C.byref(lam)))
+        g.compression = comp.value
+        g.decay = Decay(lam.value)
+        return g
+
+    @classmethod
+    def from_bytes_all(cls, data: bytes, off, scale=ScaleK1,
+                       session: GpuSession | None = None) -> "GroupedTDigest":
+        """A grouped digest restored from to_bytes_all's bytes and offsets."""
+        session = session or default_session()
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        nk = off.size - 1
+        buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+        h = C.c_void_p()
+        sc = _scale(scale)
+        check(load().kd_grouped_from_bytes_all(session.handle, sc.id, nk, buf, len(data),
+                                               off.ctypes.data_as(_i64p), C.byref(h)))
+        return cls._adopt(h.value, nk, sc, session)
+
+    @classmethod
+    def from_bytes_device(cls, data, off, scale=ScaleK1, session: GpuSession | None = None,
+                          n: int | None = None,
+                          num_keys: int | None = None) -> "GroupedTDigest":
+        """A grouped digest restored from device memory (torch tensors, or
+        addresses with n bytes of data and num_keys + 1 int64 offsets)."""
+        session = session or default_session()
+        nk = int(off.numel()) - 1 if num_keys is None else int(num_keys)
+        n = int(data.numel()) if n is None else int(n)
+        h = C.c_void_p()
+        sc = _scale(scale)
+        check(load().kd_grouped_from_bytes_device(session.handle, sc.id, nk, _device_ptr(data), n,
+                                                  _device_ptr(off), C.byref(h)))
+        return cls._adopt(h.value, nk, sc, session)
+
     def centroid_count(self, key: int) -> int:
         o = C.c_int64()
         check(load().kd_grouped_centroid_count(self._h, int(key), C.byref(o)))
