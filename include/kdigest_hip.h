@@ -260,6 +260,23 @@ int kd_batch_centroids(kd_batch *b, int64_t d, double *means, double *weights, i
                        int64_t *count);
 /* A TDigest holding a copy of digest d (centroids, buffer and state). */
 int kd_batch_to_tdigest(kd_batch *b, int64_t d, kd_tdigest **out);
+/* Compression and decay rate of a batch (as created or loaded). */
+int kd_batch_params(kd_batch *b, double *compression, double *lambda);
+/* Map-reduce finalize (test_edge.mojo:183-215, query.mojo:345-369): a batch
+ * of num_digests digests loaded from their serialized bytes (the per-digest
+ * layout of to_bytes, tdigest.mojo:627-652; digest d at bytes [off[d],
+ * off[d+1]) of one buffer): centroids and state, empty buffers, as from_bytes
+ * gives them (tdigest.mojo:656-698).  kd_batch_merge_tree then reduces them
+ * into digest 0.  Every blob is checked before the batch is created
+ * (KD_ETRUNC for a short one, KD_ESERDE for a bad version byte or count or a
+ * compression or decay rate other than digest 0's, KD_EINVAL for more
+ * centroids than a batched digest holds or a compression outside the
+ * batched range).  _device: device buffers; _all: host arrays. */
+int kd_batch_from_bytes_device(kd_session *s, int scale, int64_t num_digests,
+                               const uint8_t *d_data, int64_t n, const int64_t *d_off,
+                               kd_batch **out);
+int kd_batch_from_bytes_all(kd_session *s, int scale, int64_t num_digests, const uint8_t *data,
+                            int64_t n, const int64_t *off, kd_batch **out);
 
 /* --------------------------------------------------------------- serving -- */
 /* Continuous-batching server over num_keys digests resident in device

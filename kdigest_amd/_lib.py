@@ -74,6 +74,9 @@ SIGNATURES = {
     "kd_grouped_to_bytes": (_i64, [_vp, _i64, _u8p, _i64]),
     "kd_grouped_wire_size": (_i, [_vp, _i64p]),
     "kd_grouped_params": (_i, [_vp, _dp, _dp]),
+    "kd_batch_params": (_i, [_vp, _dp, _dp]),
+    "kd_batch_from_bytes_device": (_i, [_vp, _i, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
+    "kd_batch_from_bytes_all": (_i, [_vp, _i, _i64, _u8p, _i64, _i64p, C.POINTER(_vp)]),
     "kd_grouped_to_bytes_device": (_i, [_vp, _vp, _i64, _vp]),
     "kd_grouped_to_bytes_all": (_i, [_vp, _u8p, _i64, _i64p]),
     "kd_grouped_from_bytes_device": (_i, [_vp, _i, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),

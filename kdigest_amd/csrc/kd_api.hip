@@ -1726,37 +1726,156 @@ struct kd_batch {
   }
 };
 
+static std::unique_ptr<kd_batch> new_batch(kd_session* s, int scale, int64_t num_digests,
+                                           double compression, double lambda) {
+  if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+  if (num_digests <= 0) throw KdError(KD_EINVAL, "num_digests must be positive");
+  int64_t limit = (int64_t)compression * 5;  // tdigest.mojo:146
+  if (limit < 1 || limit > kd::batch_capacity_buffer())
+    throw KdError(KD_EINVAL, "compression out of range for the batched path (5*compression <= " +
+                                 std::to_string(kd::batch_capacity_buffer()) + ")");
+  std::unique_ptr<kd_batch> b(new kd_batch());
+  b->s = s;
+  b->scale = scale;
+  b->nd = num_digests;
+  b->compression = compression;
+  b->lambda = lambda;
+  b->limit = (int)limit;
+  size_t cc = (size_t)num_digests * kd::batch_capacity_centroids();
+  size_t bc = (size_t)num_digests * kd::batch_capacity_buffer();
+  b->st.reserve(num_digests * sizeof(kd::BatchState));
+  b->cm.reserve(cc * sizeof(double));
+  b->cw.reserve(cc * sizeof(double));
+  b->bm.reserve(bc * sizeof(double));
+  b->bw.reserve(bc * sizeof(double));
+  hip_ok(hipMemsetAsync(b->st.p, 0, num_digests * sizeof(kd::BatchState), s->st()), "memset");
+  s->sync();
+  return b;
+}
+
 extern "C" int kd_batch_create(kd_session* s, int scale, int64_t num_digests, double compression,
                                double lambda, kd_batch** out) {
   return guarded([&] {
-    if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
-    if (num_digests <= 0) throw KdError(KD_EINVAL, "num_digests must be positive");
-    int64_t limit = (int64_t)compression * 5;  // tdigest.mojo:146
-    if (limit < 1 || limit > kd::batch_capacity_buffer())
-      throw KdError(KD_EINVAL, "compression out of range for the batched path (5*compression <= " +
-                                   std::to_string(kd::batch_capacity_buffer()) + ")");
-    std::unique_ptr<kd_batch> b(new kd_batch());
-    b->s = s;
-    b->scale = scale;
-    b->nd = num_digests;
-    b->compression = compression;
-    b->lambda = lambda;
-    b->limit = (int)limit;
-    size_t cc = (size_t)num_digests * kd::batch_capacity_centroids();
-    size_t bc = (size_t)num_digests * kd::batch_capacity_buffer();
-    b->st.reserve(num_digests * sizeof(kd::BatchState));
-    b->cm.reserve(cc * sizeof(double));
-    b->cw.reserve(cc * sizeof(double));
-    b->bm.reserve(bc * sizeof(double));
-    b->bw.reserve(bc * sizeof(double));
-    hip_ok(hipMemsetAsync(b->st.p, 0, num_digests * sizeof(kd::BatchState), s->st()), "memset");
-    s->sync();
-    *out = b.release();
+    *out = new_batch(s, scale, num_digests, compression, lambda).release();
     return KD_OK;
   });
 }
 
 extern "C" void kd_batch_destroy(kd_batch* b) { delete b; }
+
+extern "C" int kd_batch_params(kd_batch* b, double* compression, double* lambda) {
+  *compression = b->compression;
+  *lambda = b->lambda;
+  return KD_OK;
+}
+
+// Map-reduce finalize (test_edge.mojo:183-215, query.mojo:345-369): many
+// serialized digests loaded into a batch, for kd_batch_merge_tree to reduce.
+namespace {
+__global__ void k_batch_load(int64_t nd, const uint8_t* data, const int64_t* woff,
+                             const int64_t* cnt, const double* st5, int64_t capc,
+                             kd::BatchState* st, double* cm, double* cw,
+                             unsigned long long* over) {
+  const int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (d >= nd) return;
+  const int64_t c = cnt[d];
+  if (c > capc) {
+    if (lane == 0) atomicMin(over, (unsigned long long)d);
+    return;
+  }
+  if (lane == 0) {
+    kd::BatchState b;
+    b.stored_total = st5[d];
+    b.log_scale = st5[nd + d];
+    b.clock = st5[2 * nd + d];
+    b.vmin = st5[3 * nd + d];
+    b.vmax = st5[4 * nd + d];
+    b.nc = (int)c;
+    b.nb = 0;
+    b.err = 0;
+    b.pad = 0;
+    st[d] = b;
+  }
+  const uint8_t* p = data + woff[d] + kWireHeader;
+  for (int64_t i = lane; i < c; i += 64) {
+    cm[d * capc + i] = kd_bits2d(wire_u64(p + 16 * i));
+    cw[d * capc + i] = kd_bits2d(wire_u64(p + 16 * i + 8));
+  }
+}
+}  // namespace
+
+static kd_batch* batch_from_wire(kd_session* s, int scale, int64_t nd, const uint8_t* d_data,
+                                 int64_t n, const int64_t* d_off) {
+  if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
+  if (nd <= 0) throw KdError(KD_EINVAL, "num_digests must be positive");
+  int64_t* cnt = s->buf<int64_t>(s->tmp, nd + 4);
+  auto* err = reinterpret_cast<unsigned long long*>(cnt + nd);
+  double* st5 = s->buf<double>(s->tmp2, 5 * nd);
+  const unsigned long long none[4] = {(unsigned long long)nd, (unsigned long long)nd,
+                                      (unsigned long long)nd, (unsigned long long)nd};
+  s->h2d(err, none, sizeof(none));
+  k_wire_parse<<<blocks_for(nd), 256, 0, s->st()>>>(nd, d_data, n, d_off, cnt, st5, err);
+  launched("k_wire_parse");
+  unsigned long long herr[3];
+  s->d2h(herr, err, sizeof(herr));
+  s->sync();
+  if (herr[0] < (unsigned long long)nd)
+    throw KdError(KD_ETRUNC, "truncated digest " + std::to_string(herr[0]));
+  if (herr[1] < (unsigned long long)nd)
+    throw KdError(KD_ESERDE, "unrecognized digest serialization (version or count) for digest " +
+                                 std::to_string(herr[1]));
+  if (herr[2] < (unsigned long long)nd)
+    throw KdError(KD_ESERDE, "digest " + std::to_string(herr[2]) +
+                                 " has another compression or decay rate than digest 0");
+  int64_t w0 = 0;
+  s->d2h(&w0, d_off, sizeof(int64_t));
+  s->sync();
+  uint8_t h0[17];
+  s->d2h(h0, d_data + w0, sizeof(h0));
+  s->sync();
+  uint64_t cb = 0, lb = 0;
+  for (int i = 0; i < 8; i++) {
+    cb |= (uint64_t)h0[1 + i] << (8 * i);
+    lb |= (uint64_t)h0[9 + i] << (8 * i);
+  }
+  std::unique_ptr<kd_batch> b = new_batch(s, scale, nd, kd_bits2d(cb), kd_bits2d(lb));
+  const int64_t capc = kd::batch_capacity_centroids();
+  k_batch_load<<<blocks_for(nd * 64), 256, 0, s->st()>>>(
+      nd, d_data, d_off, cnt, st5, capc, static_cast<kd::BatchState*>(b->st.p),
+      static_cast<double*>(b->cm.p), static_cast<double*>(b->cw.p), err + 3);
+  launched("k_batch_load");
+  unsigned long long hover = 0;
+  s->d2h(&hover, err + 3, sizeof(hover));
+  s->sync();
+  if (hover < (unsigned long long)nd)
+    throw KdError(KD_EINVAL, "digest " + std::to_string(hover) + " has more than " +
+                                 std::to_string(capc) + " centroids (the batch capacity)");
+  return b.release();
+}
+
+extern "C" int kd_batch_from_bytes_device(kd_session* s, int scale, int64_t num_digests,
+                                          const uint8_t* d_data, int64_t n, const int64_t* d_off,
+                                          kd_batch** out) {
+  return guarded([&] {
+    *out = batch_from_wire(s, scale, num_digests, d_data, n, d_off);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_batch_from_bytes_all(kd_session* s, int scale, int64_t num_digests,
+                                       const uint8_t* data, int64_t n, const int64_t* off,
+                                       kd_batch** out) {
+  return guarded([&] {
+    if (num_digests <= 0) throw KdError(KD_EINVAL, "num_digests must be positive");
+    uint8_t* dd = s->buf<uint8_t>(s->v_buf, std::max<int64_t>(n, 1));
+    int64_t* doff = s->buf<int64_t>(s->q_off, num_digests + 1);
+    s->h2d(dd, data, n);
+    s->h2d(doff, off, (num_digests + 1) * sizeof(int64_t));
+    *out = batch_from_wire(s, scale, num_digests, dd, n, doff);
+    return KD_OK;
+  });
+}
 
 namespace {
 __global__ void k_validate_samples(const double* v, const double* w, int64_t n, int* flag) {

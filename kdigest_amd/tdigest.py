@@ -643,6 +643,50 @@ class BatchedDigests:
     def flush(self) -> None:
         check(load().kd_batch_flush(self._h))
 
+    @classmethod
+    def _adopt(cls, h, num_digests, scale, session):
+        b = cls.__new__(cls)
+        b._h = h
+        b._scale = _scale(scale)
+        b.num_digests = int(num_digests)
+        b.session = session
+        comp, lam = C.c_double(), C.c_double()
+        check(load().kd_batch_params(h, C.byref(comp), C.byref(lam)))
+        b.compression = comp.value
+        b.decay = Decay(lam.value)
+        return b
+
+    @classmethod
+    def from_bytes_all(cls, data: bytes, off, scale=ScaleK1,
+                       session: GpuSession | None = None) -> "BatchedDigests":
+        """Digests loaded from their serialized bytes (digest d at
+        data[off[d]:off[d+1]]), for merge_tree to reduce (map-reduce
+        finalize)."""
+        session = session or default_session()
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        nd = off.size - 1
+        buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+        h = C.c_void_p()
+        sc = _scale(scale)
+        check(load().kd_batch_from_bytes_all(session.handle, sc.id, nd, buf, len(data),
+                                             off.ctypes.data_as(_i64p), C.byref(h)))
+        return cls._adopt(h.value, nd, sc, session)
+
+    @classmethod
+    def from_bytes_device(cls, data, off, scale=ScaleK1, session: GpuSession | None = None,
+                          n: int | None = None,
+                          num_digests: int | None = None) -> "BatchedDigests":
+        """from_bytes_all from device memory (torch tensors, or addresses
+        with n bytes of data and num_digests + 1 int64 offsets)."""
+        session = session or default_session()
+        nd = int(off.numel()) - 1 if num_digests is None else int(num_digests)
+        n = int(data.numel()) if n is None else int(n)
+        h = C.c_void_p()
+        sc = _scale(scale)
+        check(load().kd_batch_from_bytes_device(session.handle, sc.id, nd, _device_ptr(data), n,
+                                                _device_ptr(off), C.byref(h)))
+        return cls._adopt(h.value, nd, sc, session)
+
     def set_graph(self, enable: bool = True) -> None:
         """Run merge_tree as a captured HIP graph (default) or as eager launches."""
         check(load().kd_batch_set_graph(self._h, 1 if enable else 0))

@@ -1,5 +1,7 @@
 """GPU parity of the batched per-sample path (add_at replay of many digests)
 and of the merge_into tree, against the CPU oracle."""
+import struct
+
 import numpy as np
 import pytest
 
@@ -134,3 +136,82 @@ def test_batch_merge_tree_graph_replays_match_eager(kd):
             assert np.array_equal(mg.view(np.uint64), me.view(np.uint64)), (rnd, d)
             assert np.array_equal(wg.view(np.uint64), we.view(np.uint64)), (rnd, d)
             assert g.state(d) == e.state(d), (rnd, d)
+
+
+def _oracle_tree(refs):
+    for r in refs:
+        r.merge()
+    stride = 1
+    while stride < len(refs):
+        for i in range(0, len(refs), 2 * stride):
+            if i + stride < len(refs):
+                refs[i].merge_into(refs[i + stride])
+        stride *= 2
+    return refs[0]
+
+
+@pytest.mark.parametrize("lam", [0.0, 1e-3])
+def test_batch_map_reduce_from_bytes(kd, oracle, lam):
+    """Map-reduce finalize (test_edge.mojo:183-215): digests built elsewhere
+    (here by the oracle) and serialized are loaded into a batch from their
+    bytes, each exactly as serialized, and the merge_into tree reduces them
+    as the oracle's tree does."""
+    rng = np.random.default_rng(31)
+    nd = 21
+    refs = []
+    for d in range(nd):
+        r = oracle.Digest(2, 300.0, lam)
+        n = 0 if d == 5 else int(rng.integers(1, 6000))
+        vals = rng.standard_t(3, n) + d
+        times = np.cumsum(rng.exponential(1e-3, n))
+        wts = rng.integers(1, 4, n).astype(np.float64) if lam == 0.0 else rng.uniform(0.1, 10, n)
+        for i in range(n):
+            r.add_at(vals[i], times[i], wts[i])
+        refs.append(r)
+    blobs = [r.to_bytes() for r in refs]
+    off = np.zeros(nd + 1, np.int64)
+    off[1:] = np.cumsum([len(x) for x in blobs])
+    b = kd.BatchedDigests.from_bytes_all(b"".join(blobs), off, scale=kd.ScaleK2)
+    assert b.num_digests == nd and b.compression == 300.0 and b.decay.rate_lambda == lam
+    for d in range(nd):
+        m, w = b.centroids(d)
+        rm, rw = refs[d].centroids()
+        assert np.array_equal(m, rm) and np.array_equal(w, rw), d
+        st, rs = b.state(d), refs[d].state()
+        for k in ("stored_total", "log_scale", "clock", "min", "max"):
+            assert st[k] == rs[k], (d, k)
+        assert st["buffered"] == 0 and st["error"] == 0
+    b.merge_tree()
+    root = _oracle_tree(refs)
+    m, w = b.centroids(0)
+    rm, rw = root.centroids()
+    check_centroids(m, w, rm, rw, 50.0, lam == 0.0, "map-reduce root")
+    st, rs = b.state(0), root.state()
+    for k in ("clock", "min", "max"):
+        assert st[k] == rs[k], k
+    assert abs(st["stored_total"] - rs["stored_total"]) <= 1e-12 * abs(rs["stored_total"])
+
+
+def test_batch_from_bytes_rejects(kd, oracle):
+    a = oracle.Digest(2, 100.0)
+    a.add_batch_b(np.linspace(0.0, 1.0, 1000))
+    c = oracle.Digest(2, 200.0)
+    c.add_batch_b(np.linspace(0.0, 1.0, 1000))
+    ba, bc = a.to_bytes(), c.to_bytes()
+    # Another compression than digest 0's.
+    with pytest.raises(kd.KdigestError) as e:
+        kd.BatchedDigests.from_bytes_all(ba + bc, np.array([0, len(ba), len(ba) + len(bc)]),
+                                         scale=kd.ScaleK2)
+    assert e.value.code == -4
+    # More centroids than a batched digest holds.
+    big = bytearray(ba[:65])
+    struct.pack_into("<d", big, 57, 2000.0)
+    big += np.column_stack([np.arange(2000.0), np.ones(2000)]).astype("<f8").tobytes()
+    with pytest.raises(kd.KdigestError) as e:
+        kd.BatchedDigests.from_bytes_all(ba + bytes(big), np.array([0, len(ba), len(ba) + len(big)]),
+                                         scale=kd.ScaleK2)
+    assert e.value.code == -1
+    # Truncated.
+    with pytest.raises(kd.KdigestError) as e:
+        kd.BatchedDigests.from_bytes_all(ba[:-3], np.array([0, len(ba)]), scale=kd.ScaleK2)
+    assert e.value.code == -5

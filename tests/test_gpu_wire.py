@@ -20,7 +20,7 @@ def _grouped(kd, nk, lam, seed, batches=2):
         n = 200_000
         # Keys above nk // 2 + 50 stay empty in the first batch; the last key
         # is never touched (an empty digest in the middle of the blobs).
-        hi = nk // 2 + 50 if b == 0 else nk - 1
+        hi = min(nk // 2 + 50, nk - 1) if b == 0 else nk - 1
         keys = ((rng.zipf(1.3, n) - 1) % hi).astype(np.uint32)
         vals = rng.standard_normal(n) * 2.0 + keys * 0.01
         g.add_batch(keys, vals)
@@ -174,3 +174,36 @@ def test_wire_restore_rejects_bad_blobs(kd):
     # The untouched blob still restores.
     r = kd.GroupedTDigest.from_bytes_all(blob, off, scale=kd.ScaleK2)
     assert r.to_bytes_all()[0] == blob
+
+
+def test_wire_blobs_reduce_in_a_batch(kd, oracle):
+    """The grouped wire buffer on the device loaded into a batch (one digest
+    per key) and reduced by the merge_into tree, as the oracle reduces the
+    same blobs."""
+    nk = 40
+    g, _ = _grouped(kd, nk, 0.0, 41)
+    blob, off = g.to_bytes_all()
+    hip = _Hip()
+    try:
+        n = g.wire_size()
+        dd = hip.alloc(n)
+        doff = hip.alloc(8 * (nk + 1))
+        g.to_bytes_device(dd, doff, cap=n)
+        b = kd.BatchedDigests.from_bytes_device(dd, doff, scale=kd.ScaleK2, n=n, num_digests=nk)
+    finally:
+        hip.free()
+    for k in range(nk):
+        m, w = b.centroids(k)
+        gm, gw = g.centroids(k)
+        assert np.array_equal(m, gm) and np.array_equal(w, gw), k
+    b.merge_tree()
+    refs = [oracle.Digest.from_bytes(blob[off[k]:off[k + 1]], 2) for k in range(nk)]
+    stride = 1
+    while stride < nk:
+        for i in range(0, nk, 2 * stride):
+            if i + stride < nk:
+                refs[i].merge_into(refs[i + stride])
+        stride *= 2
+    m, w = b.centroids(0)
+    rm, rw = refs[0].centroids()
+    check_centroids(m, w, rm, rw, float(np.abs(rm).max()) + 1.0, True, "wire tree root")
