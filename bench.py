@@ -332,6 +332,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-wire", action="store_true")
+    ap.add_argument("--no-keymap", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", type=str, default=None,
                     help="fixed grouped launch shape items,chunks,tile_min,tiles_max,overlap "
@@ -528,6 +529,26 @@ def main():
                              "note": "same sample, keys partitioned over host threads"},
                "gpu_over_cpu": value / rate}
         del kh, vh
+    # ------------------------------------------------ key map (sparse keys)
+    keymap = None
+    if not args.no_keymap:
+        ids = torch.empty_like(keys)
+        km = kd.KeyMap(nk, session=session)
+        session.synchronize()
+        t0 = time.perf_counter()
+        km.map_device(keys, ids, insert=True)
+        t_new = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        km.map_device(keys, ids, insert=True)
+        t_known = time.perf_counter() - t0
+        keymap = {"workload": f"the {n} grouped keys through a {nk}-key map (uint32 keys to dense "
+                              "ids): first batch inserting every key, then the same batch with every "
+                              "key known",
+                  "keys": len(km), "first_ms": t_new * 1e3, "known_ms": t_known * 1e3,
+                  "known_keys_per_s": n / t_known}
+        del ids, km
+        print(f"[bench] key map {keymap['known_keys_per_s'] / 1e9:.1f} G keys/s", file=sys.stderr,
+              flush=True)
     del keys, vals
     torch.cuda.empty_cache()
 
@@ -584,6 +605,7 @@ def main():
         "single_digest": single,
         "grouped_queries": queries,
         "grouped_wire_format": wire,
+        "key_map": keymap,
         "batched_decay": batched,
         "serving": serve,
     }

@@ -35,6 +35,7 @@ typedef struct kd_tdigest kd_tdigest;
 typedef struct kd_grouped kd_grouped;
 typedef struct kd_batch kd_batch;
 typedef struct kd_server kd_server;
+typedef struct kd_keymap kd_keymap;
 
 /* Thread-local description of the last error on this thread. */
 const char *kd_last_error(void);
@@ -277,6 +278,24 @@ int kd_batch_from_bytes_device(kd_session *s, int scale, int64_t num_digests,
                                kd_batch **out);
 int kd_batch_from_bytes_all(kd_session *s, int scale, int64_t num_digests, const uint8_t *data,
                             int64_t n, const int64_t *off, kd_batch **out);
+
+/* --------------------------------------------------------------- key map -- */
+/* Dense ids for sparse uint32 keys, held on the device, for grouped digests
+ * over keys that are not 0..num_keys-1 (the reference takes dense ids,
+ * grouped.mojo:85-104, and assembles its per-key blocks on the host,
+ * gpu/grouped.mojo:454-538).  At most capacity keys; a key gets its id the
+ * first time it is mapped with insert != 0, ids 0, 1, 2, ... in order of
+ * first occurrence within that batch.  A batch that would exceed the
+ * capacity fails with KD_ENOMEM and leaves the map unchanged.  Without
+ * insert, an unknown key maps to 0xffffffff. */
+int kd_keymap_create(kd_session *s, int64_t capacity, kd_keymap **out);
+void kd_keymap_destroy(kd_keymap *m);
+int kd_keymap_map_device(kd_keymap *m, const uint32_t *d_keys, int64_t n, uint32_t *d_ids,
+                         int insert);
+int kd_keymap_map(kd_keymap *m, const uint32_t *keys, int64_t n, uint32_t *ids, int insert);
+int kd_keymap_size(kd_keymap *m, int64_t *out);
+/* The key of every id 0..size-1 (cap >= size). */
+int kd_keymap_keys(kd_keymap *m, uint32_t *out, int64_t cap);
 
 /* --------------------------------------------------------------- serving -- */
 /* Continuous-batching server over num_keys digests resident in device

@@ -13,6 +13,7 @@ from .tdigest import (
     DigestServer,
     GpuSession,
     GroupedTDigest,
+    KeyMap,
     ScaleFunction,
     ScaleK0,
     ScaleK1,
@@ -29,7 +30,7 @@ from .tdigest import (
 
 __all__ = [
     "BatchedDigests", "DigestServer",
-    "Decay", "GpuSession", "GroupedTDigest", "KdigestError", "LIB_PATH", "ScaleFunction",
+    "Decay", "GpuSession", "GroupedTDigest", "KdigestError", "KeyMap", "LIB_PATH", "ScaleFunction",
     "ScaleK0", "ScaleK1", "ScaleK2", "ScaleK3", "TDigest", "default_session", "gpu_cdf",
     "gpu_compress", "gpu_math_eval", "gpu_quantile", "gpu_scale_eval", "load",
 ]

@@ -75,6 +75,12 @@ SIGNATURES = {
     "kd_grouped_wire_size": (_i, [_vp, _i64p]),
     "kd_grouped_params": (_i, [_vp, _dp, _dp]),
     "kd_batch_params": (_i, [_vp, _dp, _dp]),
+    "kd_keymap_create": (_i, [_vp, _i64, C.POINTER(_vp)]),
+    "kd_keymap_destroy": (None, [_vp]),
+    "kd_keymap_map_device": (_i, [_vp, _vp, _i64, _vp, _i]),
+    "kd_keymap_map": (_i, [_vp, C.POINTER(C.c_uint32), _i64, C.POINTER(C.c_uint32), _i]),
+    "kd_keymap_size": (_i, [_vp, _i64p]),
+    "kd_keymap_keys": (_i, [_vp, C.POINTER(C.c_uint32), _i64]),
     "kd_batch_from_bytes_device": (_i, [_vp, _i, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
     "kd_batch_from_bytes_all": (_i, [_vp, _i, _i64, _u8p, _i64, _i64p, C.POINTER(_vp)]),
     "kd_grouped_to_bytes_device": (_i, [_vp, _vp, _i64, _vp]),

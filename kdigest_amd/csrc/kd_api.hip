@@ -2083,6 +2083,246 @@ extern "C" int kd_batch_to_tdigest(kd_batch* b, int64_t d, kd_tdigest** out) {
   });
 }
 
+// ================================================================ key map
+// Dense ids for sparse uint32 keys, on the device (upstream of the grouped
+// ingest: the reference takes dense ids, grouped.mojo:85-104, and builds its
+// per-key blocks on the host, gpu/grouped.mojo:454-538).  An open-addressing
+// table of 64-bit words (key << 1 | 1; 0 is empty) with at least twice the
+// capacity in slots; a key's id is assigned when it is first mapped with
+// insert, in order of first occurrence in that batch (a flag per position,
+// scanned), so the ids do not depend on thread timing.  A batch that would
+// exceed the capacity changes nothing: its pending slots are cleared again
+// (slots inserted by one batch never sit inside an older key's probe chain,
+// so clearing them keeps every older key reachable).
+struct kd_keymap {
+  kd_session* s;
+  int64_t capacity = 0, size = 0, slots = 0;
+  DevBuf word, id, first, rev, slot_of, flag, scan, scratch;
+  uint64_t* h_res = nullptr;  // pinned: probe overflow, new keys
+  ~kd_keymap() {
+    if (h_res) (void)hipHostFree(h_res);
+  }
+};
+
+namespace {
+constexpr int32_t kKmPending = -1;
+constexpr uint32_t kKmUnknown = 0xffffffffu;
+
+__device__ inline uint64_t km_hash(uint32_t k, uint64_t mask) {
+  return ((uint64_t)k * 0x9E3779B97F4A7C15ull >> 17) & mask;
+}
+
+__global__ void k_km_init(uint64_t* word, int32_t* id, unsigned long long* first, int64_t slots) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  word[i] = 0ull;
+  id[i] = kKmPending;
+  first[i] = ~0ull;
+}
+
+// Finds (or, with insert, claims) each key's slot; a pending slot records
+// the first position that mapped it.  No slot is claimed once the map would
+// hold more than `capacity` keys (res[1] counts the claims; res[0] is set
+// instead), so the table (at least twice the capacity) never fills beyond
+// the claims in flight; and at most `slots` probes per key.
+__global__ void k_km_probe(const uint32_t* keys, int64_t n, unsigned long long* word,
+                           const int32_t* id, unsigned long long* first, int64_t slots, int insert,
+                           int64_t size, int64_t capacity, int64_t* slot_of,
+                           unsigned long long* res) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const unsigned long long w = ((unsigned long long)k << 1) | 1ull;
+  const uint64_t mask = (uint64_t)slots - 1;
+  uint64_t h = km_hash(k, mask);
+  int64_t found = -1;
+  bool over = false;
+  // Table words and counters are read from the L2 (other workgroups claim
+  // slots during the launch; a stale cached 0 would only cost an atomic).
+  for (int64_t p = 0; p < slots; p++) {
+    unsigned long long cur = __hip_atomic_load(&word[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == 0ull) {
+      if (!insert) break;
+      const unsigned long long claimed =
+          __hip_atomic_load(&res[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (size + (int64_t)claimed >= capacity) {
+        // Full: unless this key has just been claimed here by another
+        // position (the slot is read again), the batch does not fit.
+        cur = atomicCAS(&word[h], 0ull, 0ull);
+        if (cur == 0ull) {
+          over = true;
+          break;
+        }
+      } else {
+        cur = atomicCAS(&word[h], 0ull, w);
+        if (cur == 0ull) {
+          atomicAdd(&res[1], 1ull);
+          cur = w;
+        }
+      }
+    }
+    if (cur == w) {
+      found = (int64_t)h;
+      break;
+    }
+    h = (h + 1) & mask;
+  }
+  if (insert && (over || found < 0)) atomicMax(&res[0], 1ull);
+  slot_of[i] = found;
+  // The first position of a new key: most positions come after the one
+  // already recorded and skip the atomic (a hot key's occurrences would
+  // otherwise all serialize on one word).
+  if (found >= 0 && id[found] == kKmPending) {
+    const unsigned long long f =
+        __hip_atomic_load(&first[found], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned long long)i < f) atomicMin(&first[found], (unsigned long long)i);
+  }
+}
+
+// flag[i] = 1 where position i is the first to map a new key.
+__global__ void k_km_first(int64_t n, const int64_t* slot_of, const int32_t* id,
+                           const unsigned long long* first, int64_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t h = slot_of[i];
+  flag[i] = (h >= 0 && id[h] == kKmPending && first[h] == (unsigned long long)i) ? 1 : 0;
+}
+
+__global__ void k_km_assign(const uint32_t* keys, int64_t n, const int64_t* slot_of,
+                            const int64_t* flag, const int64_t* rank, int64_t base, int32_t* id,
+                            unsigned long long* first, uint32_t* rev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const int64_t h = slot_of[i];
+  id[h] = (int32_t)(base + rank[i]);
+  first[h] = ~0ull;
+  rev[base + rank[i]] = keys[i];
+}
+
+// Clears the slots this batch claimed (still pending).
+__global__ void k_km_rollback(int64_t slots, unsigned long long* word, const int32_t* id,
+                              unsigned long long* first) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= slots || id[h] != kKmPending) return;
+  word[h] = 0ull;
+  first[h] = ~0ull;
+}
+
+__global__ void k_km_lookup(int64_t n, const int64_t* slot_of, const int32_t* id, uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t h = slot_of[i];
+  out[i] = (h >= 0 && id[h] != kKmPending) ? (uint32_t)id[h] : kKmUnknown;
+}
+}  // namespace
+
+extern "C" int kd_keymap_create(kd_session* s, int64_t capacity, kd_keymap** out) {
+  return guarded([&] {
+    if (capacity <= 0 || capacity > 0x7fffffffLL)
+      throw KdError(KD_EINVAL, "key map capacity must be in [1, 2^31)");
+    std::unique_ptr<kd_keymap> m(new kd_keymap());
+    m->s = s;
+    m->capacity = capacity;
+    int64_t slots = 64;
+    while (slots < 2 * capacity) slots <<= 1;
+    m->slots = slots;
+    m->word.reserve(slots * sizeof(uint64_t));
+    m->id.reserve(slots * sizeof(int32_t));
+    m->first.reserve(slots * sizeof(uint64_t));
+    m->rev.reserve(capacity * sizeof(uint32_t));
+    k_km_init<<<blocks_for(slots), 256, 0, s->st()>>>(static_cast<uint64_t*>(m->word.p),
+                                                      static_cast<int32_t*>(m->id.p),
+                                                      static_cast<unsigned long long*>(m->first.p),
+                                                      slots);
+    launched("k_km_init");
+    void* p = nullptr;
+    hip_ok(hipHostMalloc(&p, 2 * sizeof(uint64_t), hipHostMallocDefault), "hipHostMalloc");
+    m->h_res = static_cast<uint64_t*>(p);
+    s->sync();
+    *out = m.release();
+    return KD_OK;
+  });
+}
+
+extern "C" void kd_keymap_destroy(kd_keymap* m) { delete m; }
+
+static void keymap_map(kd_keymap* m, const uint32_t* d_keys, int64_t n, uint32_t* d_ids, int insert) {
+  kd_session* s = m->s;
+  if (n <= 0) return;
+  auto* word = static_cast<unsigned long long*>(m->word.p);
+  auto* id = static_cast<int32_t*>(m->id.p);
+  auto* first = static_cast<unsigned long long*>(m->first.p);
+  int64_t* slot_of = static_cast<int64_t*>(m->slot_of.reserve(n * sizeof(int64_t)));
+  auto* res = static_cast<unsigned long long*>(m->scratch.reserve(2 * sizeof(uint64_t)));
+  hip_ok(hipMemsetAsync(res, 0, 2 * sizeof(uint64_t), s->st()), "memset");
+  k_km_probe<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, n, word, id, first, m->slots, insert,
+                                                 m->size, m->capacity, slot_of, res);
+  launched("k_km_probe");
+  if (insert) {
+    int64_t* flag = static_cast<int64_t*>(m->flag.reserve(n * sizeof(int64_t)));
+    int64_t* rank = static_cast<int64_t*>(m->scan.reserve((n + 1) * sizeof(int64_t)));
+    k_km_first<<<blocks_for(n), 256, 0, s->st()>>>(n, slot_of, id, first, flag);
+    launched("k_km_first");
+    DevBuf bsum;
+    scan_offsets(s, flag, n, rank, bsum);
+    s->d2h(m->h_res, res, sizeof(uint64_t));
+    s->d2h(m->h_res + 1, rank + n, sizeof(int64_t));
+    s->sync();
+    const int64_t fresh = (int64_t)m->h_res[1];
+    if (m->h_res[0] || m->size + fresh > m->capacity) {
+      k_km_rollback<<<blocks_for(m->slots), 256, 0, s->st()>>>(m->slots, word, id, first);
+      launched("k_km_rollback");
+      s->sync();
+      throw KdError(KD_ENOMEM, "key map capacity exceeded (" + std::to_string(m->capacity) +
+                                   " keys; " + std::to_string(m->size) + " held)");
+    }
+    k_km_assign<<<blocks_for(n), 256, 0, s->st()>>>(d_keys, n, slot_of, flag, rank, m->size, id,
+                                                    first, static_cast<uint32_t*>(m->rev.p));
+    launched("k_km_assign");
+    m->size += fresh;
+  }
+  k_km_lookup<<<blocks_for(n), 256, 0, s->st()>>>(n, slot_of, id, d_ids);
+  launched("k_km_lookup");
+  s->sync();
+}
+
+extern "C" int kd_keymap_map_device(kd_keymap* m, const uint32_t* d_keys, int64_t n,
+                                    uint32_t* d_ids, int insert) {
+  return guarded([&] {
+    keymap_map(m, d_keys, n, d_ids, insert);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_keymap_map(kd_keymap* m, const uint32_t* keys, int64_t n, uint32_t* ids,
+                             int insert) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    kd_session* s = m->s;
+    uint32_t* dk = s->buf<uint32_t>(s->tmp, n);
+    uint32_t* di = s->buf<uint32_t>(s->tmp2, n);
+    s->h2d(dk, keys, n * sizeof(uint32_t));
+    keymap_map(m, dk, n, di, insert);
+    s->d2h(ids, di, n * sizeof(uint32_t));
+    s->sync();
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_keymap_size(kd_keymap* m, int64_t* out) {
+  *out = m->size;
+  return KD_OK;
+}
+
+extern "C" int kd_keymap_keys(kd_keymap* m, uint32_t* out, int64_t cap) {
+  return guarded([&] {
+    if (cap < m->size) throw KdError(KD_EINVAL, "output smaller than the key map's size");
+    m->s->d2h(out, m->rev.p, m->size * sizeof(uint32_t));
+    m->s->sync();
+    return KD_OK;
+  });
+}
+
 // ================================================================ serving
 struct kd_server {
   kd_session* s;

@@ -606,6 +606,56 @@ def gpu_math_eval(fn: str, xs, session=None) -> np.ndarray:
 
 
 # ------------------------------------------------------ batched digests
+class KeyMap:
+    """Dense ids 0..size-1 for sparse uint32 keys, held on the device: a key
+    gets the next id the first time it is mapped with insert, in order of
+    first occurrence in its batch (for GroupedTDigest over sparse keys)."""
+
+    UNKNOWN = 0xFFFFFFFF
+
+    def __init__(self, capacity: int, session: GpuSession | None = None):
+        self.capacity = int(capacity)
+        self.session = session or default_session()
+        h = C.c_void_p()
+        check(load().kd_keymap_create(self.session.handle, self.capacity, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().kd_keymap_destroy(h)
+            self._h = None
+
+    def __len__(self) -> int:
+        o = C.c_int64()
+        check(load().kd_keymap_size(self._h, C.byref(o)))
+        return o.value
+
+    def map(self, keys, insert: bool = True) -> np.ndarray:
+        """Dense ids of host keys (UNKNOWN for keys not in the map when
+        insert is off)."""
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        out = np.empty(k.size, dtype=np.uint32)
+        u32 = C.POINTER(C.c_uint32)
+        check(load().kd_keymap_map(self._h, k.ctypes.data_as(u32), k.size, out.ctypes.data_as(u32),
+                                   1 if insert else 0))
+        return out
+
+    def map_device(self, keys, ids, insert: bool = True, n: int | None = None) -> None:
+        """Dense ids of device keys into device ids (uint32, torch tensors or
+        addresses with n keys)."""
+        n = int(keys.numel()) if n is None else int(n)
+        check(load().kd_keymap_map_device(self._h, _device_ptr(keys), n, _device_ptr(ids),
+                                          1 if insert else 0))
+
+    def keys(self) -> np.ndarray:
+        """The key of every id, in id order."""
+        n = len(self)
+        out = np.empty(max(n, 1), dtype=np.uint32)
+        check(load().kd_keymap_keys(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        return out[:n]
+
+
 class BatchedDigests:
     """num_digests independent TDigest fed sample by sample with add_at
     (tdigest.mojo:163-175) and reduced by a pairwise merge_into tree
