@@ -87,6 +87,12 @@ int kd_session_set_overlap(kd_session *s, int enable);
  * alone; 0 sends every key through both passes.  Same results either way.
  * No counterpart in the reference. */
 int kd_session_set_heavy_keys(kd_session *s, int64_t min_pairs);
+/* Grouped batches over at most 2^bits keys (bits 0..27, default 27) are
+ * grouped by the radix partition (one to three passes, stable: each key's
+ * values keep their batch order); more keys by the counting partition, whose
+ * order within a key follows its atomics (the selection compress may then
+ * differ in the last bit between runs).  No counterpart in the reference. */
+int kd_session_set_radix_key_bits(kd_session *s, int bits);
 /* Autotuner of the grouped ingest: times ingests of the device-resident batch
  * (d_keys, d_values, n pairs, keys below num_keys) into fresh digests of the
  * given scale and compression under candidate launch shapes (coordinate

@@ -287,6 +287,14 @@ extern "C" int kd_session_set_heavy_keys(kd_session* s, int64_t min_pairs) {
   });
 }
 
+extern "C" int kd_session_set_radix_key_bits(kd_session* s, int bits) {
+  return guarded([&] {
+    if (bits < 0 || bits > 27) throw KdError(KD_EINVAL, "bits must be in [0, 27]");
+    s->eng.set_radix_key_bits(bits);
+    return KD_OK;
+  });
+}
+
 extern "C" int kd_session_path_counts(kd_session* s, int64_t* out3) {
   for (int i = 0; i < 3; i++) out3[i] = s->eng.path_counts()[i];
   return KD_OK;

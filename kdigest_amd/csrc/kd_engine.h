@@ -173,8 +173,10 @@ class Engine {
   const SegResult* dev_results() const { return static_cast<const SegResult*>(res_.p); }
 
   // Key partition of (key, value) pairs: counts per key and values grouped by
-  // key (batch order within a key is not kept: all of a key's new values carry
-  // the same weight, so their order among equals cannot change the result).
+  // key.  The radix partition (up to 2^radix_key_bits() keys) keeps every
+  // key's values in batch order; the counting partition beyond it does not,
+  // and the selection compress of a key's values can differ in the last bit
+  // with their order.
   // With d_bad != nullptr and key_partition_validates(n, num_keys), the
   // batch is also validated on the way (a key >= num_keys or a non-finite
   // value sets *d_bad; the grouping is then meaningless but stays in bounds).
@@ -194,6 +196,12 @@ class Engine {
   // this off.  Results are the same either way.
   void set_heavy_keys(int64_t min_pairs) { heavy_min_ = std::max<int64_t>(min_pairs, 0); }
   int64_t heavy_keys() const { return heavy_min_; }
+  // Grouped batches over at most 2^bits keys (bits <= 27) take the radix
+  // partition (stable: every key's values keep their batch order); more
+  // keys take the counting partition, whose order within a key follows its
+  // atomics.
+  void set_radix_key_bits(int bits) { rx_max_bits_ = bits; }
+  int radix_key_bits() const { return rx_max_bits_; }
   // Tiles of the selection count / gather passes: at least tile_min values,
   // at most tiles_max tiles per segment (one workgroup per tile).
   // One-block compress of small segments on a second stream, overlapping the
@@ -285,10 +293,12 @@ class Engine {
   static constexpr int kSmallClasses = 5;
   std::vector<int> small_[kSmallClasses], h_small_;
   DevBuf small_list_;
-  // Radix key partition: pass-1 output, tile count matrix, scan partials,
-  // segment and tile offsets.
+  // Radix key partition: pass-1 (and, over 2^18 keys, pass-2) output, tile
+  // count matrix, scan partials, segment and tile offsets.
   DevBuf rx_keys_, rx_vals_, rx_mat_, rx_bsum_, rx_seg1_, rx_seg2_, rx_tile_;
+  DevBuf rx_keys32_, rx_vals2_;
   int rx_items_ = 8;       // items per thread in a scatter sub-tile (4, 8 or 16)
+  int rx_max_bits_ = 27;       // radix partition up to 2^27 keys (three passes)
   int64_t heavy_min_ = 32768;  // pairs that make a key heavy (0: no heavy split)
   DevBuf heavy_cnt_, heavy_list_, heavy_dig_;
   int64_t rx_chunks_ = 4096;  // target number of chunks per pass

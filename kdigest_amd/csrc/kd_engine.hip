@@ -3981,16 +3981,24 @@ __global__ void k_rx_init(uint32_t* seg, uint32_t n) {
   seg[1] = n;
 }
 
-// tile_start[s] = sum over earlier segments of ceil(len / chunk); nseg <= 1024.
+// tile_start[s] = sum over earlier segments of ceil(len / chunk); one block,
+// each thread owning ceil(nseg / 1024) consecutive segments.
 __global__ __launch_bounds__(1024) void k_rx_tiles(const uint32_t* seg_start, int nseg,
                                                    uint32_t chunk, uint32_t* tile_start) {
   __shared__ uint32_t v[1024];
-  int s = threadIdx.x;
-  v[s] = s < nseg ? (seg_start[s + 1] - seg_start[s] + chunk - 1) / chunk : 0u;
+  const int per = (nseg + 1023) / 1024;
+  const int lo = min(nseg, (int)threadIdx.x * per), hi = min(nseg, lo + per);
+  uint32_t s = 0;
+  for (int i = lo; i < hi; i++) s += (seg_start[i + 1] - seg_start[i] + chunk - 1) / chunk;
+  v[threadIdx.x] = s;
   __syncthreads();
-  uint32_t total = block_excl_scan(v, nseg);
-  if (s < nseg) tile_start[s] = v[s];
-  if (s == 0) tile_start[nseg] = total;
+  const uint32_t total = block_excl_scan(v, 1024);
+  uint32_t run = v[threadIdx.x];
+  for (int i = lo; i < hi; i++) {
+    tile_start[i] = run;
+    run += (seg_start[i + 1] - seg_start[i] + chunk - 1) / chunk;
+  }
+  if (threadIdx.x == 0) tile_start[nseg] = total;
 }
 
 // Exclusive scan of cnt[0..R) into out (R <= 4 * kRxThreads); all threads call it.
@@ -4342,14 +4350,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* a, const u
 // kValidate (first pass) a key at or above key_limit or a non-finite value
 // sets *bad (grouped.mojo:91-97); positions stay in bounds either way because
 // only the key's digit bits place it.  With kWriteKeys (first of two passes)
-// only the next pass's digit (key & out_mask, at most 16 bits) is written
-// beside each value.
-template <int ITEMS, bool kWriteKeys, bool kValidate, typename KT, bool kHeavy>
+// only the key bits the later passes read (key & out_mask, at most 18 bits,
+// as KO) are written beside each value.
+template <int ITEMS, bool kWriteKeys, bool kValidate, typename KT, bool kHeavy, typename KO>
 __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const double* vals,
                                                            const uint32_t* seg_start,
                                                            const uint32_t* tile_start, int nseg,
                                                            uint32_t chunk, int shift, int bits,
-                                                           const uint32_t* dst, uint16_t* keys_out,
+                                                           const uint32_t* dst, KO* keys_out,
                                                            uint32_t out_mask, double* vals_out,
                                                            uint32_t key_limit, int* bad,
                                                            const uint16_t* dtab, uint32_t dkeys,
@@ -4367,7 +4375,8 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
   // share this LDS (the masks are all zero again when the ranking ends and
   // are cleared again after the write-out).
   __shared__ unsigned long long stage[kSub > kPm ? kSub : kPm];
-  // Staged digit (high half) and next-pass key bits (low half) per item.
+  // Staged digit (bits 18 and up; below 512) and later-pass key bits (the
+  // low 18) per item.
   __shared__ uint32_t sdk[kSub];
   double* sv = reinterpret_cast<double*>(stage);
   auto pm = [&](int wv, uint32_t d) -> unsigned long long& { return stage[wv * kRxBins + (int)d]; };
@@ -4469,7 +4478,7 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
         uint32_t d = dg[r];
         uint32_t pos = lc[d] + wc[w][d] + rk[r];
         sv[pos] = vr[r];
-        sdk[pos] = (d << 16) | (kWriteKeys ? ((uint32_t)kr[r] & out_mask & 0xffffu) : 0u);
+        sdk[pos] = (d << 18) | (kWriteKeys ? ((uint32_t)kr[r] & out_mask & 0x3ffffu) : 0u);
       }
     }
     __syncthreads();
@@ -4477,13 +4486,13 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
     const int len = (int)(e - sub);
     for (int i = threadIdx.x; i < len; i += kRxThreads) {
       const uint32_t dk = sdk[i];
-      const uint32_t d = dk >> 16, k = dk & 0xffffu;
+      const uint32_t d = dk >> 18, k = dk & 0x3ffffu;
       uint32_t o = run[d] + ((uint32_t)i - lc[d]);
       if (kHeavy && d < (uint32_t)kHeavyLight) {
         vals_heavy[o] = sv[i];  // final place: no second pass
       } else {
         vals_out[o] = sv[i];
-        if (kWriteKeys) keys_out[o] = (uint16_t)(k & out_mask);
+        if (kWriteKeys) keys_out[o] = (KO)(k & out_mask);
       }
     }
     __syncthreads();
@@ -4540,7 +4549,10 @@ __global__ void k_rx_final(const uint32_t* seg, int64_t num_keys, int64_t* off, 
 void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, int64_t n,
                                  int bits, int64_t num_keys, int64_t* d_counts,
                                  int64_t* d_offsets, double* d_out, int* d_bad) {
-  const int hb = bits > kRxBits ? bits - kRxBits : 0;  // high digit bits (pass 1)
+  // Up to 2^18 keys: one or two passes (hb high bits, then lb low bits).
+  // More: three passes of bits-18, 9 and 9 bits.
+  const bool three = bits > 2 * kRxBits;
+  const int hb = bits > kRxBits ? bits - kRxBits : 0;  // bits above the last pass's
   const int lb = bits > kRxBits ? kRxBits : bits;      // low digit bits (last pass)
   // Heavy keys get their own first-pass digit when the batch is large
   // enough for a handful of keys to hold most of it.
@@ -4550,8 +4562,10 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   // Chunk: about n / rx_chunks_ items, a whole number of sub-tiles, at most 2^20.
   int64_t chunk = (n + std::max<int64_t>(rx_chunks_, 1) - 1) / std::max<int64_t>(rx_chunks_, 1);
   chunk = std::min<int64_t>(std::max<int64_t>((chunk + sub - 1) / sub * sub, sub), (int64_t)1 << 20);
+  // Tiles of a pass over nseg segments: at most one partial tile per
+  // segment beyond n / chunk.
   const int64_t max_tiles1 = (n + chunk - 1) / chunk;
-  const int64_t max_tiles2 = max_tiles1 + (1 << hb);
+  const int64_t max_tiles2 = max_tiles1 + ((int64_t)1 << hb);
   const int64_t mat_max = (int64_t)kRxBins * max_tiles2;
   const int64_t nb_max = (mat_max + kScanChunk - 1) / kScanChunk;
   uint32_t* mat = as<uint32_t>(rx_mat_, mat_max);
@@ -4596,20 +4610,25 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   const uint32_t low_mask = (1u << lb) - 1u;
   const uint32_t high_mask = (1u << hb) - 1u;
   auto scatter = [&](bool first, bool last, unsigned grid, const auto* kin, const double* vin,
-                     const uint32_t* seg, int nseg, int shift, int dbits, uint16_t* kout,
-                     double* vout, const uint32_t* pbase) {
+                     const uint32_t* seg, int nseg, int shift, int dbits, auto* kout,
+                     uint32_t out_mask, double* vout, const uint32_t* pbase) {
     using KT = std::remove_const_t<std::remove_pointer_t<decltype(kin)>>;
+    using KO = std::remove_pointer_t<decltype(kout)>;
     const uint32_t c = (uint32_t)chunk;
 #define KD_RX_LAUNCH(I, WK, VA, HV)                                                               \
-  k_rx_scatter<I, WK, VA, KT, HV><<<grid, kRxThreads, 0, stream_>>>(                             \
-      kin, vin, seg, tiles, nseg, c, shift, dbits, mat, kout, low_mask, vout, klim, d_bad, dtab,  \
+  k_rx_scatter<I, WK, VA, KT, HV, KO><<<grid, kRxThreads, 0, stream_>>>(                         \
+      kin, vin, seg, tiles, nseg, c, shift, dbits, mat, kout, out_mask, vout, klim, d_bad, dtab,  \
       dkeys, high_mask, pbase, d_out)
 #define KD_RX_ITEMS(WK, VA, HV)            \
   if (items == 4) KD_RX_LAUNCH(4, WK, VA, HV);       \
   else if (items == 16) KD_RX_LAUNCH(16, WK, VA, HV); \
   else KD_RX_LAUNCH(8, WK, VA, HV)
     const bool validate = first && d_bad != nullptr;
-    if constexpr (std::is_same<KT, uint32_t>::value) {
+    if constexpr (std::is_same<KT, uint32_t>::value && std::is_same<KO, uint32_t>::value) {
+      // First of three passes.
+      if (validate) { KD_RX_ITEMS(true, true, false); }
+      else { KD_RX_ITEMS(true, false, false); }
+    } else if constexpr (std::is_same<KT, uint32_t>::value) {
       if (!last && heavy) {
         if (validate) { KD_RX_ITEMS(true, true, true); }
         else { KD_RX_ITEMS(true, false, true); }
@@ -4630,8 +4649,8 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
   // device only; read back for the phase byte counts when profiling.
   int64_t n_light = n;
   auto pass = [&](const auto* kin, const double* vin, const uint32_t* seg, int nseg, int shift,
-                  int dbits, int64_t tiles_max, uint16_t* kout, double* vout, uint32_t* seg_out,
-                  bool first, bool hv, const uint32_t* pbase) {
+                  int dbits, int64_t tiles_max, auto* kout, uint32_t out_mask, double* vout,
+                  uint32_t* seg_out, bool first, bool hv, const uint32_t* pbase) {
     const int R = 1 << dbits;
     k_rx_tiles<<<1, 1024, 0, stream_>>>(seg, nseg, (uint32_t)chunk, tiles);
     launched("k_rx_tiles");
@@ -4664,13 +4683,14 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
       n_light = n - (int64_t)h;
     }
     {
-      // Bytes: the key (4 B, or the 2 B digit) and value read; the value and
-      // the next digit (first of two passes, light keys only) written.
+      // Bytes: the key (4 or 2 B) and value read; the value and the key bits
+      // of the later passes (light keys only) written.
       const double items = first ? (double)n : (double)n_light;
-      const double b = ((double)sizeof(*kin) + 16.0) * items + (kout ? 2.0 * (double)n_light : 0.0);
+      const double b = ((double)sizeof(*kin) + 16.0) * items +
+                       (kout ? (double)sizeof(*kout) * (double)n_light : 0.0);
       int ph_ = phase_begin("key_scatter", b);
       scatter(first, kout == nullptr, (unsigned)tiles_max, kin, vin, seg, nseg, shift, dbits, kout,
-              vout, pbase);
+              out_mask, vout, pbase);
       phase_end(ph_);
     }
     const int64_t m = (int64_t)nseg * R + 1;
@@ -4679,24 +4699,42 @@ void Engine::key_partition_radix(const uint32_t* d_keys, const double* d_vals, i
     launched("k_rx_segs");
   };
 
-  if (hb > 0) {
+  if (three) {
+    // bits-18, then 9, then 9 bits: the first pass keeps the low 18 key
+    // bits (32-bit), the second the low 9 (16-bit).
+    const int h1 = bits - 2 * kRxBits;
+    uint32_t* k1 = as<uint32_t>(rx_keys32_, n);
+    uint16_t* k2 = as<uint16_t>(rx_keys_, n);
+    double* v1 = as<double>(rx_vals_, n);
+    double* v2 = as<double>(rx_vals2_, n);
+    const int64_t max_tiles_mid = max_tiles1 + ((int64_t)1 << h1);
+    pass(d_keys, d_vals, segA, 1, 2 * kRxBits, h1, max_tiles1, k1, (1u << (2 * kRxBits)) - 1u, v1,
+         segB, true, false, nullptr);
+    pass((const uint32_t*)k1, v1, segB, 1 << h1, kRxBits, kRxBits, max_tiles_mid, k2, low_mask, v2,
+         segA, false, false, nullptr);
+    pass((const uint16_t*)k2, v2, segA, 1 << (h1 + kRxBits), 0, kRxBits, max_tiles2,
+         (uint16_t*)nullptr, 0u, d_out, segB, false, false, nullptr);
+    std::swap(segA, segB);
+  } else if (hb > 0) {
     uint16_t* k1 = as<uint16_t>(rx_keys_, n);
     double* v1 = as<double>(rx_vals_, n);
     if (heavy) {
       // Pass 1 over digits 0..255 (heavy keys, straight to the output) and
       // 256 + high bits; pass 2 over the light segments only, placed after
       // the heavy values.
-      pass(d_keys, d_vals, segA, 1, lb, kRxBits, max_tiles1, k1, v1, segB, true, true, nullptr);
+      pass(d_keys, d_vals, segA, 1, lb, kRxBits, max_tiles1, k1, low_mask, v1, segB, true, true,
+           nullptr);
       pass((const uint16_t*)k1, v1, segB + kHeavyLight, 1 << hb, 0, lb, max_tiles2,
-           (uint16_t*)nullptr, d_out, segA, false, false, segB + kHeavyLight);
+           (uint16_t*)nullptr, 0u, d_out, segA, false, false, segB + kHeavyLight);
     } else {
-      pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, v1, segB, true, false, nullptr);
-      pass((const uint16_t*)k1, v1, segB, 1 << hb, 0, lb, max_tiles2, (uint16_t*)nullptr, d_out,
+      pass(d_keys, d_vals, segA, 1, lb, hb, max_tiles1, k1, low_mask, v1, segB, true, false,
+           nullptr);
+      pass((const uint16_t*)k1, v1, segB, 1 << hb, 0, lb, max_tiles2, (uint16_t*)nullptr, 0u, d_out,
            segA, false, false, nullptr);
     }
   } else {
-    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, (uint16_t*)nullptr, d_out, segB, true, false,
-         nullptr);
+    pass(d_keys, d_vals, segA, 1, 0, lb, max_tiles1, (uint16_t*)nullptr, 0u, d_out, segB, true,
+         false, nullptr);
     std::swap(segA, segB);
   }
   k_rx_final<<<(unsigned)((num_keys + 1 + 255) / 256), 256, 0, stream_>>>(
@@ -4711,7 +4749,8 @@ static int key_bits(int64_t num_keys) {
 }
 
 bool Engine::key_partition_validates(int64_t n, int64_t num_keys) const {
-  return n > 0 && n <= ((int64_t)1 << 31) && key_bits(num_keys) <= 2 * kRxBits;
+  return n > 0 && n <= ((int64_t)1 << 31) &&
+         key_bits(num_keys) <= std::min(rx_max_bits_, 3 * kRxBits);
 }
 
 void Engine::key_partition(const uint32_t* d_keys, const double* d_vals, int64_t n,

@@ -162,6 +162,11 @@ class GpuSession:
         pairs are placed by the first partition pass alone (0: off)."""
         check(load().kd_session_set_heavy_keys(self.handle, int(min_pairs)))
 
+    def set_radix_key_bits(self, bits: int = 27) -> None:
+        """Grouped batches over at most 2^bits keys take the stable radix
+        partition; more keys the counting partition."""
+        check(load().kd_session_set_radix_key_bits(self.handle, int(bits)))
+
     def autotune_grouped(self, keys, values, num_keys: int, scale: int = 3,
                          compression: float = 100.0, reps: int = 2) -> dict:
         """Times the grouped ingest of a device-resident batch (torch tensors

@@ -204,17 +204,23 @@ def test_grouped_many_keys_zipf(kd, oracle):
     check_values(got[ok], ref[ok], 100.0, "grouped quantile")
 
 
-@pytest.mark.parametrize("nk", [1, 2, 511, 512, 513, 4097, 1 << 18, (1 << 18) + 1])
-def test_grouped_key_partition_sizes(kd, oracle, nk):
+@pytest.mark.parametrize("nk,radix_bits", [(1, 27), (2, 27), (511, 27), (512, 27), (513, 27),
+                                           (4097, 27), (1 << 18, 27), ((1 << 18) + 1, 27),
+                                           ((1 << 20) + 3, 27), ((1 << 18) + 1, 18),
+                                           (70_000, 12)])
+def test_grouped_key_partition_sizes(kd, oracle, nk, radix_bits):
     """Key grouping at the digit-width edges of the partition (one radix pass
-    up to 512 keys, two passes up to 2^18, hashed counting beyond)."""
-    rng = np.random.default_rng(nk)
+    up to 512 keys, two passes up to 2^18, three up to 2^27; the counting
+    partition beyond the session's radix limit)."""
+    rng = np.random.default_rng(nk + radix_bits)
     n = 300_001
     keys = (rng.zipf(1.3, n) - 1) % nk
     keys[:3] = [0, nk - 1, nk // 2]
     keys = keys.astype(np.uint32)
     vals = rng.standard_normal(n) + keys * 1e-3
-    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    s = kd.GpuSession(0)
+    s.set_radix_key_bits(radix_bits)
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s)
     g.add_batch(keys, vals)
     cnt = np.bincount(keys, minlength=nk)
     chosen = sorted(set([0, nk - 1, nk // 2] + list(range(0, nk, max(1, nk // 40)))))
@@ -229,7 +235,7 @@ def test_grouped_key_partition_sizes(kd, oracle, nk):
         check_centroids(m, w, rm, rw, 300.0, True, f"nk {nk} key {k}")
 
 
-@pytest.mark.parametrize("nk", [100, 5000])
+@pytest.mark.parametrize("nk", [100, 5000, 300_000])
 def test_grouped_rejects_bad_batches_without_change(kd, nk):
     # grouped.mojo:91-97: the whole batch is validated before any key changes.
     rng = np.random.default_rng(nk)
@@ -301,14 +307,18 @@ def test_grouped_heavy_keys_identical(kd, oracle, nk, zipf, min_pairs):
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
 
 
-@pytest.mark.parametrize("items,chunks", [(4, 1), (8, 3), (16, 7), (8, 4096), (4, 1 << 20)])
-def test_grouped_partition_shapes_identical(kd, oracle, items, chunks):
+@pytest.mark.parametrize("items,chunks,nk", [(4, 1, 3001), (8, 3, 3001), (16, 7, 3001),
+                                             (8, 4096, 3001), (4, 1 << 20, 3001),
+                                             (8, 4096, 300_001), (4, 3, 300_001),
+                                             (16, 1 << 20, 300_001)])
+def test_grouped_partition_shapes_identical(kd, oracle, items, chunks, nk):
     """Every launch shape of the key partition (sub-tile width, chunks per
-    pass: one chunk of many sub-tiles up to one sub-tile per chunk) groups
-    the batch identically: centroids are bit-identical to the default shape
-    and match the oracle."""
-    rng = np.random.default_rng(items * 131 + chunks)
-    nk = 3001
+    pass: one chunk of many sub-tiles up to one sub-tile per chunk; two and
+    three passes) groups the batch identically: centroids are bit-identical
+    to the default shape, the heaviest key's are bit-identical to a
+    one-key digest fed its values in batch order (the partition is stable),
+    and they match the oracle."""
+    rng = np.random.default_rng(items * 131 + chunks + nk)
     n = 250_003
     keys = ((rng.zipf(1.2, n) - 1) % nk).astype(np.uint32)
     vals = rng.standard_normal(n) * 3.0 + keys * 0.01
@@ -325,8 +335,13 @@ def test_grouped_partition_shapes_identical(kd, oracle, items, chunks):
         assert np.array_equal(m, bm) and np.array_equal(w, bw), k
         assert g.total_weight(k) == float(cnt[k])
     k = int(np.argmax(cnt))
-    r = oracle.Digest(3, 100.0)
-    r.add_batch_b(vals[keys == k])
+    one = kd.GroupedTDigest(1, 100.0, scale=kd.ScaleK3)
+    sel = vals[keys == k]
+    one.add_batch(np.zeros(sel.size, dtype=np.uint32), sel)
     m, w = g.centroids(k)
+    om, ow = one.centroids(0)
+    assert np.array_equal(m, om) and np.array_equal(w, ow)
+    r = oracle.Digest(3, 100.0)
+    r.add_batch_b(sel)
     rm, rw = r.centroids()
     check_centroids(m, w, rm, rw, 300.0, True, f"key {k}")
