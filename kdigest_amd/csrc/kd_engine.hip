@@ -1064,16 +1064,49 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
   __shared__ int s_nbig;
   __shared__ int s_big[kBigMax];
   __shared__ double s_bigw[kBigMax];
+  __shared__ int s_bigc[kBigMax + 1];
+  // Values per thread when a segment is loaded: all of them in flight at
+  // once (one memory round trip rather than one per value).
+  constexpr int kPer = CAP / BLOCK;
   for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
     const int si = list[li];
     const SegDesc sd = segs[si];
     const int nv = (int)sd.v_cnt, ne = (int)sd.e_cnt, n = nv + ne;
     const double* v = vin + sd.v_off;
-    for (int i = threadIdx.x; i < nv; i += BLOCK) sa[padi(i)] = v[i];
+    {
+      double r[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        r[u] = i < nv ? v[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        if (i < nv) sa[padi(i)] = r[u];
+      }
+    }
     __syncthreads();
     double* s = nv > 0 ? lds_sort<(CAP / BLOCK < 16 ? CAP / BLOCK : 16)>(sa, sb, nv) : sa;
     double* x = (s == sa) ? sb : sa;
-    const double* e = esm + sd.es_off;
+    // The sorted prior means staged behind the sorted new values (nv + ne
+    // <= CAP, so they fit in the padded buffer), for the merge's searches.
+    double* e = s + padi(nv);
+    if (ne > 0) {
+      const double* eg = esm + sd.es_off;
+      double r[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        r[u] = i < ne ? eg[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        if (i < ne) e[i] = r[u];
+      }
+      __syncthreads();
+    }
     for (int i = threadIdx.x; i < nv; i += BLOCK) {
       double xv = s[padi(i)];
       int lo = 0, hi = ne;  // prior records <= xv
@@ -1108,15 +1141,22 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
     // kv[j] = k(P(j) / N): the left and right ends of every sweep test
     // (P(j) = sf * (new values before j) + prior weight before j).
     double* kv = s;
-    for (int j = threadIdx.x; j <= n; j += BLOCK) {
-      int lo = 0, hi = ne;  // prior records before merged index j
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (pos[mid] < j) lo = mid + 1;
-        else hi = mid;
+    if (ne == 0) {
+      // No prior records: P(j) = sf * j + (prior weight before 0).
+      const double w0 = sw[0];
+      for (int j = threadIdx.x; j <= n; j += BLOCK)
+        kv[j] = kd_k(scale, (sd.sf * (double)j + w0) / sd.stored_n, sd.nrm);
+    } else {
+      for (int j = threadIdx.x; j <= n; j += BLOCK) {
+        int lo = 0, hi = ne;  // prior records before merged index j
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (pos[mid] < j) lo = mid + 1;
+          else hi = mid;
+        }
+        double P = sd.sf * (double)(j - lo) + sw[lo];
+        kv[j] = kd_k(scale, P / sd.stored_n, sd.nrm);
       }
-      double P = sd.sf * (double)(j - lo) + sw[lo];
-      kv[j] = kd_k(scale, P / sd.stored_n, sd.nrm);
     }
     __syncthreads();
     // nxt[s]: first i > s with NOT (kv[i + 1] - kv[s] <= 1), or n (the
@@ -1256,28 +1296,42 @@ __global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDe
     const int nbig = s_nbig;
     if (nbig > 0) {
       // Chunk sums of the long clusters (chunk k of cluster [a, b) stored at
-      // T[a + 2k], T[a + 2k + 1]: inside the cluster's own records).
-      for (int q = 0; q < nbig; q++) {
+      // T[a + 2k], T[a + 2k + 1]: inside the cluster's own records), the
+      // chunks of all of them spread over the block at once.
+      if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int q = 0; q < nbig; q++) {
+          const int c = s_big[q];
+          const int a = (int)cs[c];
+          const int b = (c + 1 < C) ? (int)cs[c + 1] : n;
+          s_bigc[q] = acc;
+          acc += (b - a + (int)kSumChunk - 1) / (int)kSumChunk;
+        }
+        s_bigc[nbig] = acc;
+      }
+      __syncthreads();
+      const int nflat = s_bigc[nbig];
+      for (int f = threadIdx.x; f < nflat; f += BLOCK) {
+        int q = 0;
+        while (s_bigc[q + 1] <= f) q++;
         const int c = s_big[q];
         const int64_t a = cs[c];
         const int64_t b = (c + 1 < C) ? (int64_t)cs[c + 1] : n;
-        const int64_t nch = (b - a + kSumChunk - 1) / kSumChunk;
-        for (int64_t k = threadIdx.x; k < nch; k += BLOCK) {
-          const int64_t j0 = a + k * kSumChunk, j1 = min(b, j0 + kSumChunk);
-          PCur pc{pos, sw, ne, sd.sf, 0};
-          int64_t ce = ne > 0 ? pcur_seek(pc, j0) : 0;
-          dd part{0.0, 0.0};
-          for (int64_t j = j0; j < j1; j++) {
-            double w = sd.sf;
-            if (ce < ne && pos[ce] == j) {
-              w = ew[ce];
-              ce++;
-            }
-            part = dd_add(part, two_prod(x[j], w));
+        const int64_t k = f - s_bigc[q];
+        const int64_t j0 = a + k * kSumChunk, j1 = min(b, j0 + kSumChunk);
+        PCur pc{pos, sw, ne, sd.sf, 0};
+        int64_t ce = ne > 0 ? pcur_seek(pc, j0) : 0;
+        dd part{0.0, 0.0};
+        for (int64_t j = j0; j < j1; j++) {
+          double w = sd.sf;
+          if (ce < ne && pos[ce] == j) {
+            w = ew[ce];
+            ce++;
           }
-          T[a + 2 * k] = part.hi;
-          T[a + 2 * k + 1] = part.lo;
+          part = dd_add(part, two_prod(x[j], w));
         }
+        T[a + 2 * k] = part.hi;
+        T[a + 2 * k + 1] = part.lo;
       }
       __syncthreads();
       for (int q = threadIdx.x; q < nbig; q += BLOCK) {
@@ -2331,6 +2385,42 @@ __device__ inline int sel_first_bound(const SelBound* bd, int C, int64_t r) {
   return l;
 }
 
+// The gathered buckets split by size into the lists of the kernels that
+// finish them (at most 128 values: one wave each; 1024 and 4096: LDS sort;
+// more: radix select), list c at out + c * stride with its length in
+// ccnt[c]; a wave claims its places with one atomic per list.
+constexpr int kSelClasses = 4;
+
+__global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const SelMark* mk,
+                                                     const int2* mlist,
+                                                     const unsigned int* mlist_cnt, int2* out,
+                                                     int64_t stride, unsigned int* ccnt) {
+  const unsigned int total = *mlist_cnt;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (unsigned int i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < total;
+       i0 += gridDim.x * blockDim.x) {
+    const unsigned int i = i0 + (unsigned int)lane;
+    int c = -1;
+    int2 e = make_int2(0, 0);
+    if (i < total) {
+      e = mlist[i];
+      const int64_t n = mk[sel[e.x].m_off + e.y].cnt;
+      c = n <= 128 ? 0 : n <= 1024 ? 1 : n <= 4096 ? 2 : 3;
+    }
+#pragma unroll
+    for (int q = 0; q < kSelClasses; q++) {
+      const uint64_t m = __ballot(c == q);
+      if (m == 0ull) continue;
+      unsigned int base = 0;
+      const int leader = __ffsll((long long)m) - 1;
+      if (lane == leader) base = atomicAdd(&ccnt[q], (unsigned int)__popcll(m));
+      base = __shfl(base, leader, 64);
+      if (c == q) out[(int64_t)q * stride + base + (unsigned int)__popcll(m & below)] = e;
+    }
+  }
+}
+
 // Per gathered bucket of at most CAP values (and more than LO): sorted in
 // LDS; its sum and extremes, and for every boundary rank inside it the
 // values at that rank and just below and the sum of the values below.
@@ -2352,7 +2442,21 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sorted(const SegDesc* segs, const
     if (n <= LO || n > CAP) continue;
     const SegDesc sd = segs[g.si];
     const double* src = X + sd.x_off + mp->goff;
-    for (int i = threadIdx.x; i < (int)n; i += BLOCK) sa[padi(i)] = src[i];
+    {
+      // All of the bucket's loads in flight at once (CAP / BLOCK per thread).
+      constexpr int kPer = CAP / BLOCK;
+      double r[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        r[u] = i < (int)n ? src[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int i = u * BLOCK + (int)threadIdx.x;
+        if (i < (int)n) sa[padi(i)] = r[u];
+      }
+    }
     __syncthreads();
     const double* s = lds_sort(sa, sb, (int)n);
     dd acc{0.0, 0.0};
@@ -2495,8 +2599,12 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
 }
 
 // Gathered buckets of more than LO values: radix select on order keys (eight
-// bits per pass) for each boundary rank inside, from LDS when the bucket fits
-// there, else from memory.
+// bits per pass) of the values at every boundary rank inside, up to
+// kMultiSel ranks per select (one histogram each), then one pass for the
+// sums below every selected key; from LDS when the bucket fits there, else
+// from memory.
+constexpr int kMultiSel = 16;
+
 template <int LO>
 __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, const SelSeg* sel,
                                                        const SelState* st, const int2* mlist,
@@ -2504,11 +2612,13 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
                                                        SelMark* mk, SelBound* bnd,
                                                        const double* X) {
   __shared__ double vals[kSelLdsVals];
-  __shared__ uint32_t hist[256];
+  __shared__ uint32_t mh[kMultiSel][256];
   __shared__ double rh[kBlock], rl[kBlock];
-  __shared__ unsigned long long smin, smax, sless;
-  __shared__ unsigned int scnt;
-  __shared__ unsigned long long s_prefix, s_rank;
+  __shared__ unsigned long long smin, smax;
+  __shared__ unsigned long long s_pre[kMultiSel], s_rk[kMultiSel], gmx[kMultiSel];
+  __shared__ double g1[kMultiSel], g2[kMultiSel];
+  __shared__ unsigned int gcn[kMultiSel];
+  __shared__ int s_ib0, s_ib1;
   const unsigned int total = *mlist_cnt;
   for (unsigned int gi = blockIdx.x; gi < total; gi += gridDim.x) {
     const int2 e = mlist[gi];
@@ -2521,8 +2631,25 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
     const SegDesc sd = segs[g.si];
     const double* src = X + sd.x_off + mp->goff;
     const double* a = src;
+    // f(x) for every value of the bucket, each thread's in index order;
+    // eight loads per thread in flight (memory latency, not bandwidth, sets
+    // the pace of a lone block).
+    auto each = [&](const double* p, auto&& f) {
+      constexpr int kU = 8;
+      for (int64_t i0 = threadIdx.x; i0 < n; i0 += kU * (int64_t)blockDim.x) {
+        double r[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          const int64_t i = i0 + u * (int64_t)blockDim.x;
+          r[u] = i < n ? p[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (i0 + u * (int64_t)blockDim.x < n) f(i0 + u * (int64_t)blockDim.x, r[u]);
+      }
+    };
     if (n <= kSelLdsVals) {
-      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) vals[i] = src[i];
+      each(src, [&](int64_t i, double x) { vals[i] = x; });
       a = vals;
     }
     if (threadIdx.x == 0) {
@@ -2534,22 +2661,22 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
     // fixed-point slices (with the bucket's magnitude bound), so it does not
     // depend on the order of the additions.
     unsigned long long kmn = ~0ull, kmx = 0ull;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      unsigned long long k = ord_key(a[i]);
+    each(a, [&](int64_t, double x) {
+      unsigned long long k = ord_key(x);
       kmn = k < kmn ? k : kmn;
       kmx = k > kmx ? k : kmx;
-    }
+    });
     atomicMin(&smin, kmn);
     atomicMax(&smax, kmx);
     __syncthreads();
     const int ex = sel_exponent(key_value(smin), key_value(smax));
     double t1 = 0.0, t2 = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    each(a, [&](int64_t, double x) {
       double s1, s2;
-      sel_slices(a[i], ex, s1, s2);
+      sel_slices(x, ex, s1, s2);
       t1 += s1;
       t2 += s2;
-    }
+    });
     dd tot = block_dd_sum(dd{t1, t2}, rh, rl);
     if (threadIdx.x == 0) {
       mp->sum_hi = tot.hi;
@@ -2559,75 +2686,119 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
     }
     const int C = st[j].ncl;
     SelBound* bd = bnd + g.c_off;
-    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
-      const int64_t rho = bd[i].vr - vstart;
-      if (rho == 0) {
-        if (threadIdx.x == 0) {
-          bd[i].x_hi = key_value(smin);
-          bd[i].sl_hi = 0.0;
-          bd[i].sl_lo = 0.0;
-        }
-        continue;
+    if (threadIdx.x == 0) {
+      int i0 = sel_first_bound(bd, C, vstart), i1 = i0;
+      while (i1 < C && bd[i1].vr < vstart + n) i1++;
+      // A boundary at the bucket's first rank: its value is the minimum and
+      // nothing of the bucket lies below it.
+      if (i0 < i1 && bd[i0].vr == vstart) {
+        bd[i0].x_hi = key_value(smin);
+        bd[i0].sl_hi = 0.0;
+        bd[i0].sl_lo = 0.0;
+        i0++;
       }
-      if (threadIdx.x == 0) {
-        s_prefix = 0ull;
-        s_rank = (unsigned long long)rho;
+      s_ib0 = i0;
+      s_ib1 = i1;
+    }
+    __syncthreads();
+    const int ib0 = s_ib0, ib1 = s_ib1;
+    const int lane = threadIdx.x & 63;
+    for (int gb = ib0; gb < ib1; gb += kMultiSel) {
+      const int nbg = min(kMultiSel, ib1 - gb);
+      if ((int)threadIdx.x < nbg) {
+        s_pre[threadIdx.x] = 0ull;
+        s_rk[threadIdx.x] = (unsigned long long)(bd[gb + threadIdx.x].vr - vstart);
       }
-      __syncthreads();
       unsigned long long pmask = 0ull;
       for (int shift = 56; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;  // blockDim.x == 256 bins
+        for (int i = threadIdx.x; i < nbg * 256; i += blockDim.x) (&mh[0][0])[i] = 0u;
         __syncthreads();
-        const unsigned long long pre = s_prefix;
-        for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
-          unsigned long long k = ord_key(a[q]);
-          if ((k & pmask) == pre) atomicAdd(&hist[(k >> shift) & 255], 1u);
-        }
+        unsigned long long pre[kMultiSel];
+#pragma unroll
+        for (int q = 0; q < kMultiSel; q++) pre[q] = q < nbg ? s_pre[q] : ~0ull;
+        each(a, [&](int64_t, double x) {
+          const unsigned long long k = ord_key(x);
+          const unsigned long long km = k & pmask;
+          const uint32_t d = (uint32_t)(k >> shift) & 255u;
+#pragma unroll
+          for (int q = 0; q < kMultiSel; q++)
+            if (q < nbg && km == pre[q]) atomicAdd(&mh[q][d], 1u);
+        });
         __syncthreads();
-        const uint32_t h = hist[threadIdx.x];
-        const unsigned long long r = s_rank;
-        __syncthreads();
-        block_excl_scan(hist, 256);
-        // The digit whose cumulative range holds rank r.
-        const uint32_t cum = hist[threadIdx.x];
-        if ((unsigned long long)cum <= r && r < (unsigned long long)cum + h) {
-          s_rank = r - cum;
-          s_prefix = pre | ((unsigned long long)threadIdx.x << shift);
+        // Wave w finds the digit holding the rank of boundaries w, w + 4,
+        // ...: lane l scans bins 4l .. 4l + 3 after the lanes before it.
+        for (int q = (int)(threadIdx.x >> 6); q < nbg; q += kBlock / 64) {
+          uint32_t c[4];
+          uint32_t sum = 0;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            c[u] = mh[q][4 * lane + u];
+            sum += c[u];
+          }
+          uint32_t inc = sum;
+          for (int dl = 1; dl < 64; dl <<= 1) {
+            const uint32_t y = __shfl_up(inc, dl, 64);
+            if (lane >= dl) inc += y;
+          }
+          const unsigned long long r = s_rk[q];
+          unsigned long long cum = inc - sum;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (cum <= r && r < cum + c[u]) {
+              s_rk[q] = r - cum;
+              s_pre[q] |= (unsigned long long)(4 * lane + u) << shift;
+            }
+            cum += c[u];
+          }
         }
         pmask |= 255ull << shift;
         __syncthreads();
       }
-      const unsigned long long K = s_prefix;
-      if (threadIdx.x == 0) {
-        sless = 0ull;
-        scnt = 0;
+      // The selected keys ascend with the ranks: a value goes to the group
+      // of the first key above it (it is below that key and every later
+      // one); the slice sums are exact, so the order of the adds is free.
+      unsigned long long K[kMultiSel];
+#pragma unroll
+      for (int q = 0; q < kMultiSel; q++) K[q] = q < nbg ? s_pre[q] : ~0ull;
+      if ((int)threadIdx.x < kMultiSel) {
+        g1[threadIdx.x] = 0.0;
+        g2[threadIdx.x] = 0.0;
+        gmx[threadIdx.x] = 0ull;
+        gcn[threadIdx.x] = 0u;
       }
       __syncthreads();
-      double l1 = 0.0, l2 = 0.0;
-      unsigned long long kl = 0ull;
-      unsigned int cl = 0;
-      for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
-        double x = a[q];
-        unsigned long long k = ord_key(x);
-        if (k < K) {
+      each(a, [&](int64_t, double x) {
+        const unsigned long long k = ord_key(x);
+        int q = 0;
+#pragma unroll
+        for (int u = 0; u < kMultiSel; u++) q += (u < nbg && !(k < K[u])) ? 1 : 0;
+        if (q < nbg) {
           double s1, s2;
           sel_slices(x, ex, s1, s2);
-          l1 += s1;
-          l2 += s2;
-          kl = k > kl ? k : kl;
-          cl++;
+          atomicAdd(&g1[q], s1);
+          atomicAdd(&g2[q], s2);
+          atomicMax(&gmx[q], k);
+          atomicAdd(&gcn[q], 1u);
         }
-      }
-      dd sl{l1, l2};
-      atomicMax(&sless, kl);
-      atomicAdd(&scnt, cl);
-      dd sls = block_dd_sum(sl, rh, rl);
+      });
+      __syncthreads();
       if (threadIdx.x == 0) {
-        bd[i].x_hi = key_value(K);
-        bd[i].x_lo = key_value(sless);
-        bd[i].sl_hi = sls.hi;
-        bd[i].sl_lo = sls.lo;
-        bd[i].dup = (int64_t)scnt != rho;
+        double l1 = 0.0, l2 = 0.0;
+        unsigned long long kl = 0ull;
+        unsigned int cl = 0;
+        for (int q = 0; q < nbg; q++) {
+          l1 += g1[q];
+          l2 += g2[q];
+          kl = gmx[q] > kl ? gmx[q] : kl;
+          cl += gcn[q];
+          const dd sls = two_sum(l1, l2);
+          SelBound& o = bd[gb + q];
+          o.x_hi = key_value(K[q]);
+          o.x_lo = key_value(kl);
+          o.sl_hi = sls.hi;
+          o.sl_lo = sls.lo;
+          o.dup = (int64_t)cl != o.vr - vstart;
+        }
       }
       __syncthreads();
     }
@@ -3223,6 +3394,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   SelBound* dbnd = nullptr;
   SelMark* dmk = nullptr;
   int2* dml = nullptr;
+  int2* dmlx = nullptr;
   uint16_t* dbix = nullptr;
   if (nsel > 0 && !fb_only) {
     dst = as<SelState>(sel_st_, nsel);
@@ -3236,9 +3408,12 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     dmk = as<SelMark>(sel_mk_, sel_m);
     dml = as<int2>(sel_ml_, sel_m);
     dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
-    dmlc = as<unsigned int>(sel_mlc_, 2);  // gathered-bucket list size, fallbacks
+    // Gathered-bucket list size, fallbacks, list sizes per size class.
+    dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
+    dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
     check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
-    check_hip(hipMemsetAsync(dmlc, 0, 2 * sizeof(unsigned int), stream_), "memset sel list");
+    check_hip(hipMemsetAsync(dmlc, 0, (2 + kSelClasses) * sizeof(unsigned int), stream_),
+              "memset sel list");
     k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
     launched("k_sel_init");
     {
@@ -3319,6 +3494,9 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, pose, dst,
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);
       launched("k_sel_plan");
+      k_sel_classes<<<(unsigned)std::min<int64_t>((sel_m + 255) / 256, 1024), 256, 0, stream_>>>(
+          dsel, dmk, dml, dmlc, dmlx, sel_m, dmlc + 2);
+      launched("k_sel_classes");
       phase_end(ph_);
     }
     {
@@ -3342,16 +3520,22 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     }
     {
       int ph_ = phase_begin("sel_select", 0.0);
+      // Each size class walks its own list (grids sized for the worst case;
+      // blocks past a list's end return at once).
       unsigned grid = (unsigned)std::min<int64_t>(sel_m, 4096);
-      k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd, x);
+      const int2* l0 = dmlx;
+      const int2* l1 = dmlx + sel_m;
+      const int2* l2 = dmlx + 2 * sel_m;
+      const int2* l3 = dmlx + 3 * sel_m;
+      k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, l0, dmlc + 2, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
-      k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk, dbnd,
-                                                                    x);
+      k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l1, dmlc + 3, dmk,
+                                                                    dbnd, x);
       launched("k_sel_sorted<1024>");
-      k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc, dmk,
+      k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l2, dmlc + 4, dmk,
                                                                      dbnd, x);
       launched("k_sel_sorted<4096>");
-      k_sel_select<4096><<<std::min(grid, 1024u), kBlock, 0, stream_>>>(dsegs, dsel, dst, dml, dmlc,
+      k_sel_select<4096><<<std::min(grid, 1024u), kBlock, 0, stream_>>>(dsegs, dsel, dst, l3, dmlc + 5,
                                                                          dmk, dbnd, x);
       launched("k_sel_select");
       phase_end(ph_);

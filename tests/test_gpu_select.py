@@ -122,6 +122,37 @@ def test_select_with_decay(kd, oracle, sessions, lam):
             assert sa[k] == rs[k], (k, sa[k], rs[k])
 
 
+@pytest.mark.parametrize("scale,compression", [(3, 100.0), (3, 300.0), (2, 200.0)])
+def test_select_large_buckets(kd, oracle, sessions, scale, compression):
+    """2^25 values in one batch: the 8191 splitters leave buckets of about
+    4096 values, so many cluster boundaries fall in buckets selected by
+    radix select rather than sorted, and the tail buckets hold ten to forty
+    boundaries each (selected up to 16 at a time); a second batch merges
+    into the result.  Same clusters as the sort path and the oracle."""
+    s_sel, s_srt = sessions
+    rng = np.random.default_rng(int(compression) + scale)
+    a = kd.TDigest(scale, compression, session=s_sel)
+    b = kd.TDigest(scale, compression, session=s_srt)
+    r = oracle.Digest(scale, compression)
+    for n in (1 << 25, (1 << 22) + 7):
+        x = np.where(rng.uniform(size=n) < 0.9, rng.standard_normal(n),
+                     (1.0 - rng.uniform(size=n)) ** (-1.0 / 1.5))
+        before = s_sel.path_counts()["selection"]
+        a.add_batch(x)
+        b.add_batch(x)
+        r.add_batch(x)
+        assert s_sel.path_counts()["selection"] > before
+        ma, wa = a.centroids()
+        mb, wb = b.centroids()
+        rm, rw = r.centroids()
+        scale_abs = float(np.abs(x).max())
+        check_centroids(ma, wa, mb, wb, scale_abs, True, f"K{scale} d{compression} n={n} sel vs sort")
+        check_centroids(ma, wa, rm, rw, scale_abs, True, f"K{scale} d{compression} n={n} sel vs oracle")
+        sa, rs = a.state(), r.state()
+        for k in ("min", "max", "stored_total"):
+            assert sa[k] == rs[k], (scale, compression, n, k, sa[k], rs[k])
+
+
 def test_select_ties_fall_back_to_sort(kd, oracle, sessions):
     s_sel, _ = sessions
     rng = np.random.default_rng(5)
