@@ -1481,7 +1481,8 @@ struct SelState {
 
 struct SelMark {
   int64_t bucket, goff, cnt, vstart;  // gathered at x_off + goff; first new-value rank
-  unsigned int cursor, pad;
+  unsigned int cursor;
+  int first_bound;                    // first boundary i >= 1 with vr >= vstart
   double sum_hi, sum_lo, vmin, vmax;
 };
 
@@ -2215,7 +2216,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
       m.cnt = c[i];
       m.vstart = B[i];
       m.cursor = 0;
-      m.pad = 0;
+      m.first_bound = 0;
       m.sum_hi = m.sum_lo = m.vmin = m.vmax = 0.0;
       mk[g.m_off + mbase] = m;
       mlist[s_base + mbase] = make_int2(j, (int)mbase);
@@ -2388,10 +2389,13 @@ __device__ inline int sel_first_bound(const SelBound* bd, int C, int64_t r) {
 // The gathered buckets split by size into the lists of the kernels that
 // finish them (at most 128 values: one wave each; 1024 and 4096: LDS sort;
 // more: radix select), list c at out + c * stride with its length in
-// ccnt[c]; a wave claims its places with one atomic per list.
+// ccnt[c]; a wave claims its places with one atomic per list.  Each
+// bucket's first boundary is found here, all buckets at once, rather than
+// by a chain of dependent loads in the kernel that finishes it.
 constexpr int kSelClasses = 4;
 
-__global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const SelMark* mk,
+__global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const SelState* st,
+                                                     SelMark* mk, const SelBound* bnd,
                                                      const int2* mlist,
                                                      const unsigned int* mlist_cnt, int2* out,
                                                      int64_t stride, unsigned int* ccnt) {
@@ -2405,8 +2409,11 @@ __global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const Se
     int2 e = make_int2(0, 0);
     if (i < total) {
       e = mlist[i];
-      const int64_t n = mk[sel[e.x].m_off + e.y].cnt;
+      const SelSeg g = sel[e.x];
+      SelMark* mp = mk + g.m_off + e.y;
+      const int64_t n = mp->cnt;
       c = n <= 128 ? 0 : n <= 1024 ? 1 : n <= 4096 ? 2 : 3;
+      mp->first_bound = sel_first_bound(bnd + g.c_off, st[e.x].ncl, mp->vstart);
     }
 #pragma unroll
     for (int q = 0; q < kSelClasses; q++) {
@@ -2471,7 +2478,7 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sorted(const SegDesc* segs, const
     const int C = st[j].ncl;
     SelBound* bd = bnd + g.c_off;
     const int64_t vstart = mp->vstart;
-    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
+    for (int i = mp->first_bound; i < C && bd[i].vr < vstart + n; i++) {
       const int rho = (int)(bd[i].vr - vstart);
       dd sl{0.0, 0.0};
       for (int q = threadIdx.x; q < rho; q += BLOCK) sl = dd_add(sl, dd{s[padi(q)], 0.0});
@@ -2579,7 +2586,7 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
     const int C = st[j].ncl;
     SelBound* bd = bnd + g.c_off;
     const int64_t vstart = mp->vstart;
-    for (int i = sel_first_bound(bd, C, vstart); i < C && bd[i].vr < vstart + n; i++) {
+    for (int i = mp->first_bound; i < C && bd[i].vr < vstart + n; i++) {
       const int rho = (int)(bd[i].vr - vstart);
       dd sl{0.0, 0.0};
 #pragma unroll
@@ -2603,17 +2610,38 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
 // kMultiSel ranks per select (one histogram each), then one pass for the
 // sums below every selected key; from LDS when the bucket fits there, else
 // from memory.
+// The bucket fills most of a CU's LDS, so one block runs per CU: it has
+// the most threads a block may have.
 constexpr int kMultiSel = 16;
+constexpr int kSelectThreads = 1024;
+
+// Sum over the block of exact fixed-point slice sums (any order of the adds
+// gives the same double-double): butterfly in each wave, then the waves'.
+__device__ inline dd block_exact_sum(dd v, double* wh, double* wl) {
+  v = wave_dd_sum(v);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wh[w] = v.hi;
+    wl[w] = v.lo;
+  }
+  __syncthreads();
+  dd r{0.0, 0.0};
+  for (int q = 0; q < nw; q++) r = dd_add(r, dd{wh[q], wl[q]});
+  __syncthreads();
+  return r;
+}
 
 template <int LO>
-__global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, const SelSeg* sel,
-                                                       const SelState* st, const int2* mlist,
-                                                       const unsigned int* mlist_cnt,
-                                                       SelMark* mk, SelBound* bnd,
-                                                       const double* X) {
+__global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* segs,
+                                                               const SelSeg* sel,
+                                                               const SelState* st,
+                                                               const int2* mlist,
+                                                               const unsigned int* mlist_cnt,
+                                                               SelMark* mk, SelBound* bnd,
+                                                               const double* X) {
   __shared__ double vals[kSelLdsVals];
   __shared__ uint32_t mh[kMultiSel][256];
-  __shared__ double rh[kBlock], rl[kBlock];
+  __shared__ double rh[kSelectThreads / 64], rl[kSelectThreads / 64];
   __shared__ unsigned long long smin, smax;
   __shared__ unsigned long long s_pre[kMultiSel], s_rk[kMultiSel], gmx[kMultiSel];
   __shared__ double g1[kMultiSel], g2[kMultiSel];
@@ -2677,7 +2705,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
       t1 += s1;
       t2 += s2;
     });
-    dd tot = block_dd_sum(dd{t1, t2}, rh, rl);
+    dd tot = block_exact_sum(dd{t1, t2}, rh, rl);
     if (threadIdx.x == 0) {
       mp->sum_hi = tot.hi;
       mp->sum_lo = tot.lo;
@@ -2687,7 +2715,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
     const int C = st[j].ncl;
     SelBound* bd = bnd + g.c_off;
     if (threadIdx.x == 0) {
-      int i0 = sel_first_bound(bd, C, vstart), i1 = i0;
+      int i0 = mp->first_bound, i1 = i0;
       while (i1 < C && bd[i1].vr < vstart + n) i1++;
       // A boundary at the bucket's first rank: its value is the minimum and
       // nothing of the bucket lies below it.
@@ -2727,7 +2755,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_select(const SegDesc* segs, cons
         __syncthreads();
         // Wave w finds the digit holding the rank of boundaries w, w + 4,
         // ...: lane l scans bins 4l .. 4l + 3 after the lanes before it.
-        for (int q = (int)(threadIdx.x >> 6); q < nbg; q += kBlock / 64) {
+        for (int q = (int)(threadIdx.x >> 6); q < nbg; q += (int)(blockDim.x >> 6)) {
           uint32_t c[4];
           uint32_t sum = 0;
 #pragma unroll
@@ -3495,7 +3523,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);
       launched("k_sel_plan");
       k_sel_classes<<<(unsigned)std::min<int64_t>((sel_m + 255) / 256, 1024), 256, 0, stream_>>>(
-          dsel, dmk, dml, dmlc, dmlx, sel_m, dmlc + 2);
+          dsel, dst, dmk, dbnd, dml, dmlc, dmlx, sel_m, dmlc + 2);
       launched("k_sel_classes");
       phase_end(ph_);
     }
@@ -3535,7 +3563,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l2, dmlc + 4, dmk,
                                                                      dbnd, x);
       launched("k_sel_sorted<4096>");
-      k_sel_select<4096><<<std::min(grid, 1024u), kBlock, 0, stream_>>>(dsegs, dsel, dst, l3, dmlc + 5,
+      k_sel_select<4096><<<std::min(grid, 1024u), kSelectThreads, 0, stream_>>>(dsegs, dsel, dst, l3, dmlc + 5,
                                                                          dmk, dbnd, x);
       launched("k_sel_select");
       phase_end(ph_);
