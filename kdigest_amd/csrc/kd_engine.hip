@@ -1767,11 +1767,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
     double x[U];
     uint32_t kx[U];
     int lo[U], len[U];
+    // All U loads issued before any is used (an index past the tile reads
+    // its last value, then is ignored).
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
-      x[u] = i < b ? v[i] : 0.0;
-      const bool in = S > 0 && !(x[u] < sp0) && x[u] < spl_last;
+      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
+      x[u] = v[i < b ? i : b - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
+      const bool in = i < b && S > 0 && !(x[u] < sp0) && x[u] < spl_last;
       kx[u] = in ? (uint32_t)((ord_key(x[u]) << sh) >> 32) : 0u;
       const int h = in ? slot_of(kx[u]) : 0;
       lo[u] = in ? (int)slot[h] : 0;
@@ -1799,9 +1805,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, 
     for (int u = 0; u < U; u++) nx[u] = sk[lo[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
       const bool below = x[u] < sp0, above = !(x[u] < spl_last);
       lo[u] = below ? 0 : (above ? S : lo[u]);
-      if (!below && !above && nx[u] == kx[u]) {
+      if (i < b && !below && !above && nx[u] == kx[u]) {
         int k = lo[u];
         while (k < S && sk[k] == kx[u] && sg[k] <= x[u]) k++;
         lo[u] = k;
@@ -2277,7 +2284,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
 // Second pass over the new values: marked buckets are gathered, the rest
 // are added to their cluster's slice sums.
 template <int SMAX, int CMAX>
-__global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs, const SelSeg* sel,
+__global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* segs, const SelSeg* sel,
                                                             int nsel, int64_t t0, int64_t t1,
                                                             const double* vin, const uint16_t* bix,
                                                             const int* binfo, const int* ce,
@@ -2289,11 +2296,15 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
   // the central clusters of a large digest hold a third of its values each,
   // so one copy would serialize a wave's atomic adds.  The slices are exact
   // fixed-point values, so any grouping of the sums gives the same bits.
+  // Replica r starts r doubles past a multiple of 2 * CMAX, so the replicas
+  // of one cluster sit in different LDS banks as well.
   constexpr int kRep = CMAX <= 256 ? 8 : 1;
+  constexpr int kRepStride = 2 * CMAX + 1;
   __shared__ int inf[SMAX + 1];
   __shared__ int ex[CMAX];
-  __shared__ double acc[kRep * 2 * CMAX];
-  __shared__ unsigned int lcnt[kMk], lbase[kMk];
+  __shared__ double acc[kRep * kRepStride];
+  __shared__ unsigned int lcnt[kMk];
+  __shared__ unsigned long long lbase[kMk];  // gather position of the chunk's first item
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
   const int j = sel_tile_seg(sel, nsel, t);
@@ -2303,8 +2314,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
   const int S = st[j].nspl, C = st[j].ncl, nmk = st[j].nmk;
   for (int i = threadIdx.x; i <= S; i += blockDim.x) inf[i] = binfo[g.b_off + i];
   for (int i = threadIdx.x; i < C; i += blockDim.x) ex[i] = ce[g.c_off + i];
-  for (int i = threadIdx.x; i < kRep * 2 * CMAX; i += blockDim.x) acc[i] = 0.0;
-  double* acc_r = acc + (threadIdx.x % kRep) * 2 * CMAX;
+  for (int i = threadIdx.x; i < kRep * kRepStride; i += blockDim.x) acc[i] = 0.0;
+  double* acc_r = acc + (threadIdx.x % kRep) * kRepStride;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
@@ -2318,41 +2329,53 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_gather(const SegDesc* segs,
     double xr[kItems];
     int fr[kItems];
     unsigned int rr[kItems];
+    uint16_t br[kItems];
+    // The chunk in two halves, each with its loads issued before any is
+    // used (an index past the tile reads its last item, then is ignored).
 #pragma unroll
-    for (int q = 0; q < kItems; q++) {
-      int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
-      fr[q] = 0x7fffffff;
-      if (i < b) {
-        double x = v[i];
-        int f = inf[bx[i]];
-        xr[q] = x;
-        fr[q] = f;
-        if (f < 0) {
-          rr[q] = atomicAdd(&lcnt[-1 - f], 1u);
-        } else {
-          double s1, s2;
-          sel_slices(x, ex[f], s1, s2);
-          unsafeAtomicAdd(&acc_r[2 * f], s1);
-          unsafeAtomicAdd(&acc_r[2 * f + 1], s2);
+    for (int h = 0; h < kItems; h += kItems / 2) {
+#pragma unroll
+      for (int q = h; q < h + kItems / 2; q++) {
+        const int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
+        const int64_t ic = i < b ? i : b - 1;
+        xr[q] = v[ic];
+        br[q] = bx[ic];
+      }
+#pragma unroll
+      for (int q = h; q < h + kItems / 2; q++) {
+        const int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
+        fr[q] = 0x7fffffff;
+        if (i < b) {
+          const double x = xr[q];
+          const int f = inf[br[q]];
+          fr[q] = f;
+          if (f < 0) {
+            rr[q] = atomicAdd(&lcnt[-1 - f], 1u);
+          } else {
+            double s1, s2;
+            sel_slices(x, ex[f], s1, s2);
+            unsafeAtomicAdd(&acc_r[2 * f], s1);
+            unsafeAtomicAdd(&acc_r[2 * f + 1], s2);
+          }
         }
       }
     }
     __syncthreads();
     // One reservation per gathered bucket and chunk.
     for (int i = threadIdx.x; i < nmk; i += blockDim.x)
-      if (lcnt[i]) lbase[i] = atomicAdd(&m[i].cursor, lcnt[i]);
+      if (lcnt[i]) lbase[i] = (unsigned long long)m[i].goff + atomicAdd(&m[i].cursor, lcnt[i]);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
       int f = fr[q];
-      if (f < 0) xg[m[-1 - f].goff + lbase[-1 - f] + rr[q]] = xr[q];
+      if (f < 0) xg[lbase[-1 - f] + rr[q]] = xr[q];
     }
     __syncthreads();
   }
   for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) {
     double sum = 0.0;
 #pragma unroll
-    for (int r = 0; r < kRep; r++) sum += acc[r * 2 * CMAX + i];
+    for (int r = 0; r < kRep; r++) sum += acc[r * kRepStride + i];
     if (sum != 0.0) unsafeAtomicAdd(&csl[2 * g.c_off + i], sum);
   }
 }
@@ -4609,7 +4632,10 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
     stage[i] = 0ull;
   }
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t kr[ITEMS], rk[ITEMS];
+  // Items past the chunk's end keep a key of an earlier sub-tile (or 0):
+  // any key indexes the digit table in bounds after the clamp below.
+  uint32_t kr[ITEMS] = {};
+  uint32_t rk[ITEMS];
   uint16_t dg[ITEMS];
   double vr[ITEMS];
   bool wrong = false;
@@ -4627,6 +4653,12 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
   __syncthreads();
   for (uint32_t sub = a; sub < b; sub += kSub) {
     const uint32_t e = min(b, sub + (uint32_t)kSub);
+    uint16_t dh[ITEMS];
+    if constexpr (kHeavy) {
+      // The sub-tile's digit-table lookups all issued before any is used.
+#pragma unroll
+      for (int r = 0; r < ITEMS; r++) dh[r] = dtab[kr[r] < dkeys ? kr[r] : 0u];
+    }
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
       uint32_t i = sub + (uint32_t)(w * kPerWave + r * 64 + lane);
@@ -4634,7 +4666,9 @@ __global__ __launch_bounds__(kRxThreads) void k_rx_scatter(const KT* keys, const
       uint32_t d = 0u;
       if (valid) {
         if constexpr (kHeavy) {
-          d = heavy_digit(dtab, dkeys, kr[r], shift, lmask);
+          // heavy_digit with the looked-up entry.
+          const uint32_t k = kr[r];
+          d = k < dkeys ? (uint32_t)dh[r] : kHeavyLight + ((k >> shift) & lmask);
         } else {
           d = (kr[r] >> shift) & mask;
         }
