@@ -3509,7 +3509,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
         case 1: KD_SMALL(512, 128); break;
         case 2: KD_SMALL(1024, 128); break;
         case 3: KD_SMALL(2048, kBlock); break;
-        default: KD_SMALL(4096, kBlock); break;
+        default: KD_SMALL(4096, 512); break;
       }
 #undef KD_SMALL
       launched("k_small_compress");
