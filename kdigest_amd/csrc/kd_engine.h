@@ -269,9 +269,11 @@ class Engine {
 
   int device_;
   hipStream_t stream_ = nullptr;
-  // Second stream (default priority): the one-block compress of small
-  // segments runs on it, overlapping the selection path of large segments;
-  // fork_/join_ order it after the shared inputs and before the finish.
+  // Second stream (default priority): the two smaller splitter-sample
+  // classes run on it beside the largest, and with set_overlap the
+  // one-block compress of small segments, overlapping the selection path of
+  // large segments; fork_/join_ order it after the shared inputs and before
+  // the work that reads its results.
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ = nullptr, join_ = nullptr;
   bool overlap_ = false;

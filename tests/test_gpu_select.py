@@ -270,3 +270,28 @@ def test_autotune_grouped_keeps_results(kd):
         bm, bw = base.centroids(k)
         assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
         assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
+
+
+def test_overlap_keeps_results(kd):
+    """Small digests compressed on the second stream while the selection
+    path runs (set_overlap) give the same digests bit for bit, over two
+    batches (the second merges into the first's centroids)."""
+    rng = np.random.default_rng(8)
+    nk = 3001
+    on, off = kd.GpuSession(0), kd.GpuSession(0)
+    on.set_overlap(True)
+    off.set_overlap(False)
+    a = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=on)
+    b = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=off)
+    for _ in range(2):
+        keys = ((rng.zipf(1.2, 2_000_000) - 1) % nk).astype(np.uint32)
+        vals = rng.standard_normal(keys.size) * 5.0 + keys * 0.01
+        a.add_batch(keys, vals)
+        b.add_batch(keys, vals)
+    paths = on.path_counts()
+    assert paths["selection"] > 0 and paths["one_block"] > 0, paths
+    for k in range(nk):
+        m, w = a.centroids(k)
+        bm, bw = b.centroids(k)
+        assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
+        assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
