@@ -337,6 +337,9 @@ def main():
     ap.add_argument("--shape", type=str, default=None,
                     help="fixed grouped launch shape items,chunks,tile_min,tiles_max,overlap "
                          "(implies --no-autotune)")
+    ap.add_argument("--radix-key-bits", type=int, default=None,
+                    help="grouped batches over more than 2^bits keys take the counting partition "
+                         "(default: the library's 27)")
     ap.add_argument("--only-single", action="store_true",
                     help="measure only the single-digest line (for its kernel profile)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -361,6 +364,8 @@ def main():
                           "dtype": "f64", "data": "synthetic (torch Philox, seeded), device-resident",
                           "config": {"workload": single["workload"]}, "single_digest": single}))
         return
+    if args.radix_key_bits is not None:
+        session.set_radix_key_bits(args.radix_key_bits)
     n, nk = args.n, args.keys
     print(f"[bench] generating {n} grouped pairs over {nk} keys", file=sys.stderr, flush=True)
     keys, vals = gen_grouped(n, nk, 0, dev)
