@@ -3469,20 +3469,27 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     launched("k_sel_init");
     {
       int ph_ = phase_begin("sel_sample", 0.0);
-      // The three sample sizes touch disjoint segments: the two smaller on
-      // the side stream, the largest beside them on the session stream.
-      check_hip(hipEventRecord(fork_, stream_), "fork samples");
-      check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork samples wait");
-      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, side_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      // The three sample sizes touch disjoint segments: with several
+      // segments the two smaller run on the side stream, the largest beside
+      // them on the session stream (a lone digest skips the fork's latency).
+      const bool fork_s = nsel > 1;
+      hipStream_t s2 = fork_s ? side_ : stream_;
+      if (fork_s) {
+        check_hip(hipEventRecord(fork_, stream_), "fork samples");
+        check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork samples wait");
+      }
+      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
       launched("k_sel_sample<1024>");
-      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, side_>>>(dsegs, dsel, d_v, esm, dspl, dst);
+      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
       launched("k_sel_sample<4096>");
       // The largest class fills a CU's LDS anyway: a block of kSelThreads.
       k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
                                                                           dst);
       launched("k_sel_sample<8192>");
-      check_hip(hipEventRecord(join_, side_), "join samples");
-      check_hip(hipStreamWaitEvent(stream_, join_, 0), "join samples wait");
+      if (fork_s) {
+        check_hip(hipEventRecord(join_, side_), "join samples");
+        check_hip(hipStreamWaitEvent(stream_, join_, 0), "join samples wait");
+      }
       phase_end(ph_);
     }
   }
@@ -3585,12 +3592,15 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       const int2* l2 = dmlx + 2 * sel_m;
       const int2* l3 = dmlx + 3 * sel_m;
       // The buckets of the radix select (the largest, few) are finished on
-      // the side stream beside the sorted classes: disjoint buckets and
-      // boundaries.
-      check_hip(hipEventRecord(fork_, stream_), "fork select");
-      check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork select wait");
-      k_sel_select<4096><<<std::min(grid, 1024u), kSelectThreads, 0, side_>>>(dsegs, dsel, dst, l3, dmlc + 5,
-                                                                             dmk, dbnd, x);
+      // the side stream beside the sorted classes (disjoint buckets and
+      // boundaries) when there are several digests.
+      const bool fork_b = nsel > 1;
+      if (fork_b) {
+        check_hip(hipEventRecord(fork_, stream_), "fork select");
+        check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork select wait");
+      }
+      k_sel_select<4096><<<std::min(grid, 1024u), kSelectThreads, 0, fork_b ? side_ : stream_>>>(
+          dsegs, dsel, dst, l3, dmlc + 5, dmk, dbnd, x);
       launched("k_sel_select");
       k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, l0, dmlc + 2, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
@@ -3600,8 +3610,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l2, dmlc + 4, dmk,
                                                                      dbnd, x);
       launched("k_sel_sorted<4096>");
-      check_hip(hipEventRecord(join_, side_), "join select");
-      check_hip(hipStreamWaitEvent(stream_, join_, 0), "join select wait");
+      if (fork_b) {
+        check_hip(hipEventRecord(join_, side_), "join select");
+        check_hip(hipStreamWaitEvent(stream_, join_, 0), "join select wait");
+      }
       phase_end(ph_);
     }
     {
