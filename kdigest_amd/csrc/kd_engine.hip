@@ -1050,7 +1050,7 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
 // means by all lanes with the arithmetic of k_cluster_mean.  Replaces the
 // sort / merge / chain / sums launches for small digests.
 template <int CAP, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_small_compress(int scale, const SegDesc* segs,
+__global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(int scale, const SegDesc* segs,
                                                           const int* list, int nlist,
                                                           const double* vin, const double* esm,
                                                           const double* esw, const double* se,
