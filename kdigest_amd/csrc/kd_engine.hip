@@ -3494,14 +3494,16 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     }
   }
   // Small segments after the selection samples (whose blocks then start
-  // first); on the side stream (set_overlap) they overlap the rest of the
-  // selection path.
+  // first); on the side stream (set_overlap) they start after the selection
+  // count (which keeps every CU busy by itself) and overlap the rest of the
+  // selection path, whose single-block steps leave most CUs idle.
   bool forked = false;
   int64_t n_small = 0;
   for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
-  if (n_small > 0 && !fb_only) {
+  const bool small_later = overlap_ && nsel > 0;
+  auto run_small = [&](bool side) {
     hipStream_t on = stream_;
-    if (overlap_) {
+    if (side) {
       check_hip(hipEventRecord(fork_, stream_), "fork");
       check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork wait");
       forked = true;
@@ -3530,7 +3532,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     }
     phase_end(ph_);
     if (forked) check_hip(hipEventRecord(join_, side_), "join");
-  }
+  };
+  if (n_small > 0 && !fb_only && !small_later) run_small(false);
   if (nsel > 0 && !fb_only) {
     const double sel_bytes = 8.0 * P.sel_v;
     {
@@ -3547,6 +3550,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       }
       phase_end(ph_);
     }
+    if (n_small > 0 && small_later) run_small(true);
     {
       int ph_ = phase_begin("sel_plan", 0.0);
       k_sel_ranks<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, esm, pose,
@@ -3594,7 +3598,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // The buckets of the radix select (the largest, few) are finished on
       // the side stream beside the sorted classes (disjoint buckets and
       // boundaries) when there are several digests.
-      const bool fork_b = nsel > 1;
+      const bool fork_b = nsel > 1 && !forked;
       if (fork_b) {
         check_hip(hipEventRecord(fork_, stream_), "fork select");
         check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork select wait");
