@@ -6,7 +6,7 @@
  * message.  Validation happens before any state changes, as in the reference
  * (kdigest/tdigest.mojo:252-254, kdigest/grouped.mojo:91-97).
  *
- * Scale ids: 0..3 = ScaleK0..ScaleK3 (kdigest/scale.mojo:121-243), a runtime
+ * Scale ids: 0..3 = ScaleK0..ScaleK3 (kdigest/scale.mojo:78-200), a runtime
  * int in place of the reference's compile-time type parameter.
  *
  * Citations are of the reference interface each entry point replaces (paths
@@ -146,7 +146,7 @@ int kd_scale_eval(kd_session *s, int scale, double nrm, const double *q, int64_t
  * pointers). */
 int kd_math_eval(kd_session *s, int fn, const double *x, int64_t n, double *out);
 
-/* Host-side scale functions (kdigest/scale.mojo:105-235), the same code the
+/* Host-side scale functions (kdigest/scale.mojo:78-192), the same code the
  * device evaluates: normalizer(compression, n), k(q, nrm), q(k, nrm). */
 double kd_scale_normalizer(int scale, double compression, double n);
 double kd_scale_k(int scale, double q, double nrm);
@@ -154,7 +154,7 @@ double kd_scale_q(int scale, double k, double nrm);
 
 /* ------------------------------------------------ TDigest (one digest) -- */
 /* TDigest[S] (kdigest/tdigest.mojo:97-726).  decay_lambda is Decay.rate_lambda
- * (kdigest/decay.mojo:267-294); 0 disables decay. */
+ * (kdigest/decay.mojo:23-51); 0 disables decay. */
 int kd_tdigest_create(kd_session *s, int scale, double compression, double decay_lambda,
                       kd_tdigest **out);
 void kd_tdigest_destroy(kd_tdigest *d);

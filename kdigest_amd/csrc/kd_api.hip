@@ -450,7 +450,7 @@ struct kd_tdigest {
   DevBuf cm, cw;               // centroids (device)
   int64_t nc = 0;
 
-  bool decay() const { return lambda > 0.0; }                       // decay.mojo:292-294
+  bool decay() const { return lambda > 0.0; }                       // decay.mojo:49-51
   double scale_factor() const { return decay() ? kd_exp(log_scale) : 1.0; }  // :352-357
   double effective_count() const {                                   // :373-377
     return decay() ? stored_total * kd_exp(-log_scale) : stored_total;
@@ -1699,7 +1699,7 @@ extern "C" double kd_scale_normalizer(int scale, double compression, double n) {
 
 extern "C" double kd_scale_k(int scale, double q, double nrm) { return kd_k(scale, q, nrm); }
 
-// Inverse maps of scale.mojo:139-142, 163-167, 195-200, 229-235.
+// Inverse maps of scale.mojo:96-99, 120-124, 152-157, 186-192.
 extern "C" double kd_scale_q(int scale, double k, double nrm) {
   switch (scale) {
     case KD_K0: return k / nrm;

@@ -17,7 +17,7 @@
  * given here by their IEEE-754 bit patterns.  Accuracy is < 1 ulp; the tests
  * compare against glibc on millions of arguments.
  *
- * Scale functions restate kdigest/scale.mojo:84-235 (K0..K3 normalizer, k, q
+ * Scale functions restate kdigest/scale.mojo:32-192 (K0..K3 normalizer, k, q
  * and the q clamp to [1e-15, 1 - 1e-15] for K2/K3).
  */
 #ifndef KD_MATH_H
@@ -200,7 +200,7 @@ KD_HD double kd_exp(double x) {
 enum { KD_K0 = 0, KD_K1 = 1, KD_K2 = 2, KD_K3 = 3 };
 
 KD_HD double kd_clamp_q(double q) {
-  /* scale.mojo:84-90 (FP64 eps = 1e-15). */
+  /* scale.mojo:32-47 (FP64 eps = 1e-15). */
   const double eps = 1e-15;
   if (q < eps) return eps;
   if (q > 1.0 - eps) return 1.0 - eps;
@@ -208,7 +208,7 @@ KD_HD double kd_clamp_q(double q) {
 }
 
 KD_HD double kd_normalizer(int scale, double compression, double n) {
-  /* scale.mojo:129-132, 152-155, 182-186, 214-218. */
+  /* scale.mojo:86-89, 109-112, 139-143, 171-175. */
   const double pi = kd_bits2d(0x400921fb54442d18ull);
   switch (scale) {
     case KD_K0: return compression / 2.0;
@@ -219,7 +219,7 @@ KD_HD double kd_normalizer(int scale, double compression, double n) {
 }
 
 KD_HD double kd_k(int scale, double q, double nrm) {
-  /* scale.mojo:134-137, 157-161, 188-193, 220-227. */
+  /* scale.mojo:91-94, 114-118, 145-150, 177-184. */
   switch (scale) {
     case KD_K0: return nrm * q;
     case KD_K1: return nrm * kd_asin(2.0 * q - 1.0);

@@ -32,7 +32,7 @@ def _ptr(a: np.ndarray, t=_dp):
 
 # ------------------------------------------------------------------ scales
 class ScaleFunction:
-    """k(q) scale function (scale.mojo:93-118); `id` selects it in the ABI."""
+    """k(q) scale function (scale.mojo:50-75); `id` selects it in the ABI."""
 
     id: int = -1
     name: str = ""
@@ -57,22 +57,22 @@ class ScaleFunction:
 
 
 class ScaleK0(ScaleFunction):
-    """Linear: k = normalizer * q (scale.mojo:121-142)."""
+    """Linear: k = normalizer * q (scale.mojo:78-99)."""
     id, name = 0, "ScaleK0"
 
 
 class ScaleK1(ScaleFunction):
-    """k = normalizer * asin(2q - 1) (scale.mojo:145-167)."""
+    """k = normalizer * asin(2q - 1) (scale.mojo:102-124)."""
     id, name = 1, "ScaleK1"
 
 
 class ScaleK2(ScaleFunction):
-    """k = normalizer * log(q / (1 - q)) (scale.mojo:170-200)."""
+    """k = normalizer * log(q / (1 - q)) (scale.mojo:127-157)."""
     id, name = 2, "ScaleK2"
 
 
 class ScaleK3(ScaleFunction):
-    """Piecewise log scale (scale.mojo:203-235)."""
+    """Piecewise log scale (scale.mojo:160-192)."""
     id, name = 3, "ScaleK3"
 
 
@@ -88,7 +88,7 @@ def _scale(s) -> ScaleFunction:
 
 # ------------------------------------------------------------------- decay
 class Decay:
-    """Exponential decay rate lambda (decay.mojo:267-294)."""
+    """Exponential decay rate lambda (decay.mojo:23-51)."""
 
     def __init__(self, rate_lambda: float = 0.0):
         self.rate_lambda = float(rate_lambda)

@@ -11,9 +11,9 @@
  * same header (fdlibm method), as the host layer and the device do.
  *
  * Reference files restated (paths relative to the reference root):
- *   src/kdigest/centroid.mojo:34-40    incremental weighted mean
- *   src/kdigest/scale.mojo:84-235      K0..K3, clamp
- *   src/kdigest/decay.mojo:267-294     Decay
+ *   src/kdigest/centroid.mojo:25-40    incremental weighted mean
+ *   src/kdigest/scale.mojo:32-192      K0..K3, clamp
+ *   src/kdigest/decay.mojo:23-54       Decay
  *   src/kdigest/tdigest.mojo:39-726    MergingDigest
  *   src/kdigest/grouped.mojo:60-162    per-key batches
  */
@@ -122,7 +122,7 @@ ko_digest *ko_clone(const ko_digest *s) {
   return d;
 }
 
-static inline int decay_active(const ko_digest *d) { return d->lambda > 0.0; } /* decay.mojo:292-294 */
+static inline int decay_active(const ko_digest *d) { return d->lambda > 0.0; } /* decay.mojo:49-51 */
 
 /* tdigest.mojo:317-322 */
 static inline int validate(double value, double weight) {

@@ -27,7 +27,7 @@ typedef struct {
 typedef struct {
   int scale;             /* 0..3 = ScaleK0..K3 */
   double compression;
-  double lambda;         /* Decay.rate_lambda (decay.mojo:267-294) */
+  double lambda;         /* Decay.rate_lambda (decay.mojo:23-37) */
   ko_centroid *cent;
   int64_t nc, cap_c;
   ko_centroid *buf;

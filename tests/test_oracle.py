@@ -111,7 +111,7 @@ def test_scale_k0_linear(oracle):
 
 
 def test_scale_closed_forms(oracle):
-    # scale.mojo:129-227 evaluated by hand.
+    # scale.mojo:86-184 evaluated by hand.
     assert oracle.normalizer(0, 100.0, 5.0) == 50.0
     assert oracle.normalizer(1, 100.0, 5.0) == 100.0 / (2 * math.pi)
     n = 1e6
@@ -119,7 +119,7 @@ def test_scale_closed_forms(oracle):
     assert abs(oracle.normalizer(3, 100.0, n) - 100.0 / (4 * math.log(n / 100.0) + 21.0)) <= 1e-15
     assert oracle.k(1, 0.5, 3.0) == 0.0
     assert abs(oracle.k(1, 1.0, 3.0) - 3.0 * math.pi / 2) <= 1e-15
-    # K2/K3 clamp q to [1e-15, 1 - 1e-15] (scale.mojo:84-90).
+    # K2/K3 clamp q to [1e-15, 1 - 1e-15] (scale.mojo:32-47).
     assert oracle.k(2, 0.0, 1.0) == oracle.k(2, 1e-15, 1.0)
     assert oracle.k(3, 1.0, 1.0) == oracle.k(3, 1.0 - 1e-15, 1.0)
     assert abs(oracle.k(3, 0.25, 2.0) - 2.0 * math.log(0.5)) <= 1e-15
