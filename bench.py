@@ -61,6 +61,180 @@ def gen_grouped(n: int, num_keys: int, seed: int, dev):
     return keys, vals
 
 
+RANK_QS = (0.001, 0.01, 0.5, 0.99, 0.999)
+
+
+def tail_quantiles(per_key: int = 1024) -> np.ndarray:
+    """The per-key tail quantiles of the grouped workload: per_key / 2
+    log-spaced points in [1e-4, 1e-2] and their mirror images near 1."""
+    lo = np.logspace(-4, -2, per_key // 2)
+    return np.concatenate([lo, 1.0 - lo[::-1]])
+
+
+def host_threads() -> int:
+    """Worker threads for host-side oracle runs: the cores this process may use,
+    at most 16 (one GPU's share of the host on the measurement boxes)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _parse_blobs(blob: np.ndarray, off: np.ndarray):
+    """Per-key headers (num_keys x 8 doubles, as raw uint64 bit patterns) and the
+    concatenated (mean, weight) pairs of consecutive wire blobs."""
+    nk = off.size - 1
+    starts = off[:-1].astype(np.int64)
+    hidx = starts[:, None] + np.arange(65, dtype=np.int64)[None, :]
+    hdr = np.ascontiguousarray(blob[hidx][:, 1:]).view("<u8").reshape(nk, 8)
+    mask = np.ones(blob.size, bool)
+    mask[hidx.ravel()] = False
+    pairs = np.ascontiguousarray(blob[mask]).view("<f8").reshape(-1, 2)
+    return hdr, pairs
+
+
+def grouped_parity(kd, g, keys, vals, nk: int, refs=None, threads: int | None = None,
+                   stride: int = 100, scale: int = 3, compression: float = 100.0):
+    """Compares every key of the grouped digest g (built from keys, vals on the
+    device) with the oracle's per-key batch merges of the same pairs: header
+    fields (stored total, min, max, clock, count) bit for bit, weights bit for
+    bit, means within 1e-12 relative (floor 1e-12 * max|x| of the key), Sum w
+    equal to the key's pair count and means sorted; then, on the 64 largest
+    keys and every `stride`-th key by size, the 1024 tail quantiles and the
+    quantiles at RANK_QS within 1e-9 relative, and the rank error of both
+    estimates against an exact sort of the key's values (on the device).
+    `refs` may hold the oracle digests already built; returns (report, refs)."""
+    import torch
+
+    from oracle import oracle as O
+
+    failures = []
+    t_all = time.perf_counter()
+    dev = vals.device
+    nbytes = g.wire_size()
+    wout = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    woff = torch.empty(nk + 1, dtype=torch.int64, device=dev)
+    g.to_bytes_device(wout, woff)
+    gblob, goff = wout.cpu().numpy(), woff.cpu().numpy()
+    del wout, woff
+    t_oracle = None
+    threads = threads or host_threads()
+    if refs is None:
+        kh = keys.cpu().numpy().astype(np.uint32)
+        vh = vals.cpu().numpy()
+        refs = [O.Digest(scale, compression) for _ in range(nk)]
+        t0 = time.perf_counter()
+        O.grouped_add_batch(refs, kh, vh, threads=threads)
+        t_oracle = time.perf_counter() - t0
+        del kh, vh
+    oparts = [r.to_bytes() for r in refs]
+    olen = np.fromiter((len(b) for b in oparts), np.int64, nk)
+    ooff = np.zeros(nk + 1, np.int64)
+    ooff[1:] = np.cumsum(olen)
+    oblob = np.frombuffer(b"".join(oparts), np.uint8)
+    del oparts
+    gh, gp = _parse_blobs(gblob, goff)
+    oh, op = _parse_blobs(oblob, ooff)
+    bad_hdr = np.nonzero(np.any(gh != oh, axis=1))[0]
+    names = ("compression", "lambda", "log_scale", "clock", "stored_total", "min", "max", "count")
+    for k in bad_hdr[:5].tolist():
+        diff = {names[i]: (gh[k, i:i + 1].view("<f8")[0], oh[k, i:i + 1].view("<f8")[0])
+                for i in range(8) if gh[k, i] != oh[k, i]}
+        failures.append(f"key {k}: header {diff}")
+    cnt = torch.bincount(keys.to(torch.int64), minlength=nk).cpu().numpy()
+    ghd = gh.view("<f8")
+    counts = ghd[:, 7].astype(np.int64)
+    rep = {"keys_compared": nk, "centroids": int(counts.sum()), "headers_equal": bad_hdr.size == 0,
+           "mean_bar": 1e-12, "query_bar": 1e-9}
+    sumw_ok = bool(np.array_equal(ghd[:, 4], cnt.astype(np.float64)))
+    rep["stored_total_equals_pairs"] = sumw_ok
+    if not sumw_ok:
+        failures.append("stored total differs from the key's pair count")
+    if bad_hdr.size == 0:
+        kid = np.repeat(np.arange(nk), counts)
+        wsum = np.bincount(kid, weights=gp[:, 1], minlength=nk)
+        rep["weights_equal"] = bool(np.array_equal(gp[:, 1].view("<u8"), op[:, 1].view("<u8")))
+        if not rep["weights_equal"]:
+            failures.append("centroid weights differ")
+        rep["sum_w_equals_pairs"] = bool(np.array_equal(wsum, cnt.astype(np.float64)))
+        if not rep["sum_w_equals_pairs"]:
+            failures.append("sum of centroid weights differs from the pair count")
+        same_key = kid[1:] == kid[:-1]
+        rep["means_sorted"] = bool(np.all(gp[1:, 0][same_key] >= gp[:-1, 0][same_key]))
+        if not rep["means_sorted"]:
+            failures.append("centroid means not sorted within a key")
+        scale_k = np.maximum(np.abs(ghd[:, 5]), np.abs(ghd[:, 6]))
+        floor = np.maximum(np.abs(op[:, 0]), np.repeat(scale_k, counts))
+        rel = np.abs(gp[:, 0] - op[:, 0]) / np.maximum(floor, 1e-300)
+        rep["max_mean_rel_err"] = float(rel.max()) if rel.size else 0.0
+        rep["means_bit_identical_frac"] = float(np.mean(gp[:, 0].view("<u8") == op[:, 0].view("<u8")))
+        if rep["max_mean_rel_err"] > 1e-12:
+            failures.append(f"mean relative error {rep['max_mean_rel_err']:.3g} > 1e-12")
+    # Queries and rank error on a stratified key sample.
+    by_size = np.argsort(-cnt, kind="stable")
+    sample = np.unique(np.concatenate([by_size[:64], by_size[64::stride]]))
+    sample = sample[cnt[sample] > 0]
+    tq = tail_quantiles()
+    qrow = np.concatenate([tq, np.array(RANK_QS)])
+    qk = torch.from_numpy(np.repeat(sample.astype(np.int64), qrow.size)).to(dev)
+    qv = torch.from_numpy(np.tile(qrow, sample.size)).to(dev)
+    qo = torch.empty_like(qv)
+    g.quantile_device(qk, qv, qo)
+    gq = qo.cpu().numpy().reshape(sample.size, qrow.size)
+    del qk, qv, qo
+    oq = np.stack([refs[int(k)].quantiles(qrow) for k in sample])
+    qrel = np.abs(gq - oq) / np.maximum(np.abs(oq), 1.0)
+    rep["query_keys"] = int(sample.size)
+    rep["queries_compared"] = int(gq.size)
+    rep["max_quantile_rel_err"] = float(qrel.max())
+    rep["quantiles_bit_identical_frac"] = float(np.mean(gq.view("<u8") == oq.view("<u8")))
+    if rep["max_quantile_rel_err"] > 1e-9:
+        failures.append(f"quantile relative error {rep['max_quantile_rel_err']:.3g} > 1e-9")
+    # Rank error: fraction of the key's values <= the estimate, minus q
+    # (test_tdigest.mojo:93-103), exact counts from the key's sorted values.
+    kk = keys.to(torch.int64)
+    vs, idx = torch.sort(vals)
+    ks = kk[idx]
+    del idx
+    ks, idx2 = torch.sort(ks, stable=True)
+    vs = vs[idx2]
+    del idx2, ks, kk
+    koff = np.zeros(nk + 1, np.int64)
+    koff[1:] = np.cumsum(cnt)
+    nq = len(RANK_QS)
+    est = torch.from_numpy(np.concatenate([gq[:, -nq:], oq[:, -nq:]], axis=1)).to(dev)
+    ranks = np.empty((sample.size, 2 * nq))
+    for i, k in enumerate(sample.tolist()):
+        seg = vs[koff[k]:koff[k + 1]]
+        ranks[i] = (torch.searchsorted(seg, est[i], right=True).cpu().numpy() / cnt[k])
+    del vs, est
+    qarr = np.array(RANK_QS)
+    err_g = np.abs(ranks[:, :nq] - qarr)
+    err_o = np.abs(ranks[:, nq:] - qarr)
+    big = cnt[sample] >= 100000
+    rep["rank_error"] = {
+        f"q={q}": {"gpu_max": float(err_g[:, j].max()), "gpu_mean": float(err_g[:, j].mean()),
+                   "oracle_max": float(err_o[:, j].max()), "oracle_mean": float(err_o[:, j].mean()),
+                   "gpu_mean_keys_ge_1e5": float(err_g[big, j].mean()) if big.any() else None,
+                   "oracle_mean_keys_ge_1e5": float(err_o[big, j].mean()) if big.any() else None}
+        for j, q in enumerate(RANK_QS)}
+    rep["rank_error_note"] = ("|#(x <= estimate)/n - q| per key against the key's sorted values, over "
+                              f"the {sample.size} sampled keys ({int(big.sum())} with >= 1e5 values)")
+    rep["oracle_s"] = t_oracle
+    rep["oracle_threads"] = threads
+    rep["check_s"] = time.perf_counter() - t_all
+    rep["failures"] = failures
+    return rep, refs
+
+
 def traffic_bytes(phase: str):
     """HBM bytes per launch of a kernel group, from the committed rocprof
     FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic.json), or None."""
@@ -323,7 +497,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--keys", type=int, default=100000)
     ap.add_argument("--single-n", type=int, default=1 << 26)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 27)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
@@ -332,6 +506,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-wire", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-keymap", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", type=str, default=None,
@@ -462,6 +637,16 @@ def main():
                    "request": "16 quantiles of one key, host arrays, synchronous"}
         del qk, qv, out
         print(f"[bench] queries {queries['queries_per_s'] / 1e9:.3f} G/s", file=sys.stderr, flush=True)
+    # ------------------------------------------------ parity at full size
+    parity = None
+    oracle_all = None
+    if not args.no_parity:
+        parity, refs = grouped_parity(kd, digs[-1], keys, vals, nk)
+        oracle_all = (parity["oracle_s"], parity["oracle_threads"])
+        del refs
+        print(f"[bench] parity: {len(parity['failures'])} failures, max mean rel err "
+              f"{parity.get('max_mean_rel_err')}, max quantile rel err {parity['max_quantile_rel_err']}",
+              file=sys.stderr, flush=True)
     # ------------------------------------------------ wire format, every key
     wire = None
     if not args.no_wire:
@@ -525,13 +710,25 @@ def main():
         kh = keys[:m].cpu().numpy().astype(np.uint32)
         vh = vals[:m].cpu().numpy()
         rate, dt = cpu_baseline_grouped(kh, vh, nk)
-        nthr = max(1, min(16, len(os.sched_getaffinity(0))))
-        rate_mt, dt_mt = cpu_baseline_grouped(kh, vh, nk, threads=nthr)
+        if oracle_all is not None:
+            dt_mt, nthr = oracle_all
+            all_cores = {"value": n / dt_mt, "cores": nthr, "s": dt_mt,
+                         "note": f"the whole {n}-pair batch (the parity run's oracle), keys "
+                                 "partitioned over host threads"}
+        else:
+            nthr = host_threads()
+            rate_mt, dt_mt = cpu_baseline_grouped(kh, vh, nk, threads=nthr)
+            all_cores = {"value": rate_mt, "cores": nthr, "s": dt_mt,
+                         "note": "same sample, keys partitioned over host threads"}
         cpu = {"value": rate, "unit": "samples/s", "cores": 1, "kind": "port",
                "sample": f"first {m} pairs of the same grouped workload, oracle per-key batch merge "
-                         f"(C restatement of the reference MergingDigest, gcc -O2), {dt:.1f} s",
-               "all_cores": {"value": rate_mt, "cores": nthr, "s": dt_mt,
-                             "note": "same sample, keys partitioned over host threads"},
+                         "(C restatement of the reference MergingDigest, gcc -O3 "
+                         f"-ffp-contract=off), {dt:.1f} s",
+               "all_cores": all_cores,
+               "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                        "usable_cores": len(os.sched_getaffinity(0)),
+                        "threads_note": "multi-threaded runs use at most 16 threads, one GPU's "
+                                        "share of the host"},
                "gpu_over_cpu": value / rate}
         del kh, vh
     # ------------------------------------------------ key map (sparse keys)
@@ -606,6 +803,7 @@ def main():
         "compress_paths": paths,
         "autotuned_shape": tuned,
         "cpu_baseline": cpu,
+        "parity": parity,
         "plain_per_key": plain,
         "single_digest": single,
         "grouped_queries": queries,

@@ -187,12 +187,13 @@ int kd_tdigest_state(kd_tdigest *d, double *out9);
 
 /* ------------------------------------------ GroupedTDigest (per key) ---- */
 /* GroupedTDigest[S] (kdigest/grouped.mojo:30-190): num_keys digests with
- * dense ids, state resident in device memory. */
+ * dense ids (1 <= num_keys <= 2^31 - 1, the reference's Int32 id range), state
+ * resident in device memory. */
 int kd_grouped_create(kd_session *s, int scale, int64_t num_keys, double compression,
                       double decay_lambda, kd_grouped **out);
 void kd_grouped_destroy(kd_grouped *g);
 /* add_batch (grouped.mojo:60-162): keys uint32 < num_keys (the reference's
- * Int32 dense ids), host arrays. */
+ * Int32 dense ids), host arrays; at most 2^32 - 2^16 pairs per call. */
 int kd_grouped_add_batch(kd_grouped *g, const uint32_t *keys, const double *values, int64_t n);
 /* Same with device-resident keys and values. */
 int kd_grouped_add_batch_device(kd_grouped *g, const uint32_t *d_keys, const double *d_values,
@@ -202,7 +203,10 @@ int kd_grouped_quantile(kd_grouped *g, const int64_t *keys, const double *qs, in
                         double *out);
 int kd_grouped_cdf(kd_grouped *g, const int64_t *keys, const double *xs, int64_t nq,
                    double *out);
-/* Device-resident query arrays (keys int64, values, output). */
+/* Device-resident query arrays (keys int64, values, output).  Every query is
+ * checked on the device: a key outside [0, num_keys) returns KD_EINVAL, a q
+ * outside [0, 1] KD_ERANGE, an empty key KD_EEMPTY (those outputs are NaN, the
+ * others are written).  Synchronous: returns after the results are in d_out. */
 int kd_grouped_quantile_device(kd_grouped *g, const int64_t *d_keys, const double *d_qs,
                                int64_t nq, double *d_out);
 int64_t kd_grouped_to_bytes(kd_grouped *g, int64_t key, uint8_t *out, int64_t cap); /* :176-180 */

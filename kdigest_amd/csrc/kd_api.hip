@@ -148,14 +148,27 @@ __global__ void k_query(int is_cdf, const int64_t* off, const double* m, const d
                   : quantile_one(m + a, w + a, S + a, nc, st, qs[i]);
 }
 
-// Grouped queries: the state arrays are separate per-key vectors.
-__global__ void k_query_grouped(int is_cdf, const int64_t* off, const double* m, const double* w,
-                                const double* S, const double* total, const double* vmin,
-                                const double* vmax, const double* eff, const int64_t* qk,
-                                const double* qs, int64_t nq, double* out) {
+// Grouped queries: the state arrays are separate per-key vectors.  A query
+// whose key is out of range, whose quantile q is outside [0, 1] or whose key
+// is empty gets NaN and sets its bit in *err (the reference raises for each,
+// grouped.mojo:164-174 via tdigest.mojo:492-498, 545-546).
+__global__ void k_query_grouped(int is_cdf, int64_t nk, const int64_t* off, const double* m,
+                                const double* w, const double* S, const double* total,
+                                const double* vmin, const double* vmax, const double* eff,
+                                const int64_t* qk, const double* qs, int64_t nq, double* out,
+                                int* err) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   int64_t d = qk[i];
+  int bad = 0;
+  if (d < 0 || d >= nk) bad = 1;
+  else if (!is_cdf && !(qs[i] >= 0.0 && qs[i] <= 1.0)) bad = 2;
+  else if (off[d + 1] == off[d]) bad = 4;
+  if (bad) {
+    out[i] = __builtin_nan("");
+    if (err) atomicOr(err, bad);
+    return;
+  }
   int64_t a = off[d], nc = off[d + 1] - a;
   QState st{total[d], vmin[d], vmax[d], eff[d]};
   out[i] = is_cdf ? cdf_one(m + a, w + a, S + a, nc, st, qs[i])
@@ -181,6 +194,15 @@ __global__ void k_validate(const uint32_t* keys, const double* vals, int64_t n, 
 
 
 unsigned blocks_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+// Grid of 256-thread blocks for kernels that give each item one wave and walk
+// the items with a wave stride: at most 2^20 blocks (2^22 waves), so the grid
+// stays far inside the 32-bit launch limits for any item count.
+// One-thread-per-pair launches stay below 2^32 work items per dimension.
+constexpr int64_t kMaxGroupedBatch = ((int64_t)1 << 32) - ((int64_t)1 << 16);
+unsigned wave_grid(int64_t items) {
+  const int64_t b = (items + 3) / 4;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, (int64_t)1 << 20));
+}
 
 }  // namespace
 
@@ -783,8 +805,14 @@ extern "C" int kd_tdigest_from_bytes(kd_session* s, const uint8_t* data, int64_t
     for (int i = 0; i < 8; i++)
       if (get_f64(data, n, 1 + 8 * i, &h[i]))
         throw KdError(KD_ETRUNC, "truncated digest: not enough bytes for a Float64");
+    // The count is stored as a double: a whole number of centroids, checked
+    // before anything is sized by it, and the blob must hold that many pairs.
+    if (!(h[7] >= 0.0) || !(h[7] < 4503599627370496.0) || h[7] != std::floor(h[7]))
+      throw KdError(KD_ESERDE, "digest centroid count is not a whole number >= 0");
     int64_t count = (int64_t)h[7];
-    std::vector<double> m(std::max<int64_t>(count, 0)), w(std::max<int64_t>(count, 0));
+    if (count > (n - 65) / 16)
+      throw KdError(KD_ETRUNC, "truncated digest: not enough bytes for a Float64");
+    std::vector<double> m(count), w(count);
     int64_t off = 65;
     for (int64_t i = 0; i < count; i++, off += 16)
       if (get_f64(data, n, off, &m[i]) || get_f64(data, n, off + 8, &w[i]))
@@ -863,7 +891,7 @@ struct kd_grouped {
   DevBuf off, cm, cw, S;          // device CSR of centroids + query prefix
   DevBuf noff, ncm, ncw;          // next CSR (swapped in after a batch)
   DevBuf state;                   // 5 x num_keys: stored total, log scale, clock, min, max
-  DevBuf qstate, seg_key, seg_of, ncnt, tot;
+  DevBuf qstate, seg_key, seg_of, ncnt, tot, qerr;
   int64_t* h_tot = nullptr;       // page-locked new centroid total
   bool prefix_valid = false;
 
@@ -900,7 +928,10 @@ struct kd_grouped {
   }
 
   void add_device(const uint32_t* d_keys, const double* d_values, int64_t n);
-  void query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq, double* d_out);
+  // d_err (may be null): bit 0 a key outside [0, num_keys), bit 1 a quantile
+  // q outside [0, 1], bit 2 a key with an empty digest; those outputs are NaN.
+  void query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq, double* d_out,
+             int* d_err = nullptr);
 };
 
 namespace {
@@ -1000,16 +1031,17 @@ __global__ void k_gather_keys(int64_t nk, const int64_t* old_off, const double* 
                               const double* ow, const int64_t* seg_of, const kd::SegDesc* segs,
                               const double* em, const double* ew, const int64_t* new_off,
                               double* nm, double* nw) {
-  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (k >= nk) return;
-  const int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
-  const int64_t sg = seg_of[k];
-  const double* sm = sg >= 0 ? em + segs[sg].x_off : om + old_off[k];
-  const double* sw = sg >= 0 ? ew + segs[sg].x_off : ow + old_off[k];
-  for (int64_t i = lane; i < cnt; i += 64) {
-    nm[dst + i] = sm[i];
-    nw[dst + i] = sw[i];
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nk; k += stride) {
+    const int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
+    const int64_t sg = seg_of[k];
+    const double* sm = sg >= 0 ? em + segs[sg].x_off : om + old_off[k];
+    const double* sw = sg >= 0 ? ew + segs[sg].x_off : ow + old_off[k];
+    for (int64_t i = lane; i < cnt; i += 64) {
+      nm[dst + i] = sm[i];
+      nw[dst + i] = sw[i];
+    }
   }
 }
 
@@ -1029,7 +1061,7 @@ extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, dou
                                  double lambda, kd_grouped** out) {
   return guarded([&] {
     if (scale < 0 || scale > 3) throw KdError(KD_EINVAL, "scale must be 0..3");
-    if (num_keys <= 0 || num_keys > 0xffffffffLL) throw KdError(KD_EINVAL, "num_keys out of range");
+    if (num_keys <= 0 || num_keys > 0x7fffffffLL) throw KdError(KD_EINVAL, "num_keys must be in [1, 2^31 - 1]");
     std::unique_ptr<kd_grouped> g(new kd_grouped());
     g->s = s;
     g->scale = scale;
@@ -1063,6 +1095,8 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   // grouped.mojo:60-162.  The whole batch is validated (:91-97) before any
   // key changes: the partition (or k_validate) sets the flag, and the device
   // plan changes nothing when it is set.
+  if (n > kMaxGroupedBatch)
+    throw KdError(KD_EINVAL, "grouped batch above 2^32 - 2^16 pairs: split it into several calls");
   clk::time_point tp = clk::now();
   int* flag = s->buf<int>(s->tmp, 1);
   hip_ok(hipMemsetAsync(flag, 0, sizeof(int), s->st()), "memset");
@@ -1128,7 +1162,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   const int64_t new_total = *h_tot;
   double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
   double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
-  k_gather_keys<<<blocks_for(num_keys * 64), 256, 0, s->st()>>>(
+  k_gather_keys<<<wave_grid(num_keys), 256, 0, s->st()>>>(
       num_keys, doff(), dm(), dw(), dso, segs, s->eng.cent_means(), s->eng.cent_weights(), dno,
       dnm, dnw);
   launched("k_gather_keys");
@@ -1143,7 +1177,7 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
 }
 
 void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int64_t nq,
-                       double* d_out) {
+                       double* d_out, int* d_err) {
   double* dS = static_cast<double*>(S.reserve(std::max<int64_t>(total_cent, 1) * sizeof(double)));
   double* st = static_cast<double*>(qstate.reserve(4 * num_keys * sizeof(double)));
   if (!prefix_valid) {
@@ -1154,9 +1188,10 @@ void kd_grouped::query(int is_cdf, const int64_t* d_keys, const double* d_q, int
     launched("k_grouped_qstate");
     prefix_valid = true;
   }
-  k_query_grouped<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, doff(), dm(), dw(), dS, st,
-                                                       st + num_keys, st + 2 * num_keys,
-                                                       st + 3 * num_keys, d_keys, d_q, nq, d_out);
+  k_query_grouped<<<blocks_for(nq), 256, 0, s->st()>>>(is_cdf, num_keys, doff(), dm(), dw(), dS,
+                                                       st, st + num_keys, st + 2 * num_keys,
+                                                       st + 3 * num_keys, d_keys, d_q, nq, d_out,
+                                                       d_err);
   launched("k_query_grouped");
   hip_ok(hipGetLastError(), "grouped query launch");
 }
@@ -1348,7 +1383,17 @@ extern "C" int kd_grouped_quantile_device(kd_grouped* g, const int64_t* d_keys, 
                                           int64_t nq, double* d_out) {
   return guarded([&] {
     if (nq <= 0) return KD_OK;
-    g->query(0, d_keys, d_qs, nq, d_out);
+    if (nq > kMaxGroupedBatch) throw KdError(KD_EINVAL, "too many queries in one call");
+    int* derr = static_cast<int*>(g->qerr.reserve(sizeof(int)));
+    hip_ok(hipMemsetAsync(derr, 0, sizeof(int), g->s->st()), "memset");
+    g->query(0, d_keys, d_qs, nq, d_out, derr);
+    int herr = 0;
+    g->s->d2h(&herr, derr, sizeof(int));
+    g->s->sync();
+    // Checked in the reference's order: key, then q range, then emptiness.
+    if (herr & 1) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    if (herr & 2) throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+    if (herr & 4) throw KdError(KD_EEMPTY, "quantile of an empty digest is undefined");
     return KD_OK;
   });
 }
@@ -1391,13 +1436,11 @@ __global__ void k_wire_sizes(int64_t nk, const int64_t* off, int64_t* size) {
   if (k < nk) size[k] = kWireHeader + 16 * (off[k + 1] - off[k]);
 }
 
-// One wave per key writes its blob byte by byte (a blob starts at any byte).
-__global__ void k_wire_write(int64_t nk, const int64_t* off, const double* cm, const double* cw,
-                             const double* state, double compression, double lambda,
-                             const int64_t* woff, uint8_t* out) {
-  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (k >= nk) return;
+// One key's blob, written byte by byte by one wave (a blob starts at any byte).
+__device__ void wire_write_key(int64_t k, int lane, int64_t nk, const int64_t* off,
+                               const double* cm, const double* cw, const double* state,
+                               double compression, double lambda, const int64_t* woff,
+                               uint8_t* out) {
   const int64_t c0 = off[k], c = off[k + 1] - c0, base = woff[k];
   const int64_t size = kWireHeader + 16 * c;
   // Header order: compression, lambda, log scale, clock, stored total, min,
@@ -1432,6 +1475,15 @@ __global__ void k_wire_write(int64_t nk, const int64_t* off, const double* cm, c
     }
     out[base + i] = b;
   }
+}
+
+__global__ void k_wire_write(int64_t nk, const int64_t* off, const double* cm, const double* cw,
+                             const double* state, double compression, double lambda,
+                             const int64_t* woff, uint8_t* out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nk; k += stride)
+    wire_write_key(k, lane, nk, off, cm, cw, state, compression, lambda, woff, out);
 }
 
 // Per key: header checks and scalars.  err[0] the first truncated key,
@@ -1485,14 +1537,15 @@ __global__ void k_wire_parse(int64_t nk, const uint8_t* data, int64_t n, const i
 // One wave per key: its centroid pairs into the CSR.
 __global__ void k_wire_pairs(int64_t nk, const uint8_t* data, const int64_t* woff,
                              const int64_t* off, double* cm, double* cw) {
-  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (k >= nk) return;
-  const int64_t c0 = off[k], c = off[k + 1] - c0;
-  const uint8_t* p = data + woff[k] + kWireHeader;
-  for (int64_t i = lane; i < c; i += 64) {
-    cm[c0 + i] = kd_bits2d(wire_u64(p + 16 * i));
-    cw[c0 + i] = kd_bits2d(wire_u64(p + 16 * i + 8));
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nk; k += stride) {
+    const int64_t c0 = off[k], c = off[k + 1] - c0;
+    const uint8_t* p = data + woff[k] + kWireHeader;
+    for (int64_t i = lane; i < c; i += 64) {
+      cm[c0 + i] = kd_bits2d(wire_u64(p + 16 * i));
+      cw[c0 + i] = kd_bits2d(wire_u64(p + 16 * i + 8));
+    }
   }
 }
 }  // namespace
@@ -1531,7 +1584,7 @@ static void grouped_to_wire(kd_grouped* g, uint8_t* d_out, int64_t cap, int64_t*
   k_wire_sizes<<<blocks_for(nk), 256, 0, s->st()>>>(nk, g->doff(), sizes);
   launched("k_wire_sizes");
   scan_offsets(s, sizes, nk, d_off, s->tmp2);
-  k_wire_write<<<blocks_for(nk * 64), 256, 0, s->st()>>>(nk, g->doff(), g->dm(), g->dw(),
+  k_wire_write<<<wave_grid(nk), 256, 0, s->st()>>>(nk, g->doff(), g->dm(), g->dw(),
                                                          static_cast<double*>(g->state.p),
                                                          g->compression, g->lambda, d_off, d_out);
   launched("k_wire_write");
@@ -1617,7 +1670,7 @@ static kd_grouped* grouped_from_wire(kd_session* s, int scale, int64_t nk, const
   g->total_cent = *g->h_tot;
   g->cm.reserve(std::max<int64_t>(g->total_cent, 1) * sizeof(double));
   g->cw.reserve(std::max<int64_t>(g->total_cent, 1) * sizeof(double));
-  k_wire_pairs<<<blocks_for(nk * 64), 256, 0, s->st()>>>(nk, d_data, d_off, g->doff(), g->dm(),
+  k_wire_pairs<<<wave_grid(nk), 256, 0, s->st()>>>(nk, d_data, d_off, g->doff(), g->dm(),
                                                          g->dw());
   launched("k_wire_pairs");
   s->sync();
@@ -1639,7 +1692,7 @@ extern "C" int kd_grouped_from_bytes_all(kd_session* s, int scale, int64_t num_k
                                          const uint8_t* data, int64_t n, const int64_t* off,
                                          kd_grouped** out) {
   return guarded([&] {
-    if (num_keys <= 0 || num_keys > 0xffffffffLL) throw KdError(KD_EINVAL, "num_keys out of range");
+    if (num_keys <= 0 || num_keys > 0x7fffffffLL) throw KdError(KD_EINVAL, "num_keys must be in [1, 2^31 - 1]");
     uint8_t* dd = s->buf<uint8_t>(s->v_buf, std::max<int64_t>(n, 1));
     int64_t* doff = s->buf<int64_t>(s->q_off, num_keys + 1);
     s->h2d(dd, data, n);
@@ -1780,13 +1833,10 @@ extern "C" int kd_batch_params(kd_batch* b, double* compression, double* lambda)
 // Map-reduce finalize (test_edge.mojo:183-215, query.mojo:345-369): many
 // serialized digests loaded into a batch, for kd_batch_merge_tree to reduce.
 namespace {
-__global__ void k_batch_load(int64_t nd, const uint8_t* data, const int64_t* woff,
-                             const int64_t* cnt, const double* st5, int64_t capc,
-                             kd::BatchState* st, double* cm, double* cw,
-                             unsigned long long* over) {
-  const int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (d >= nd) return;
+__device__ void batch_load_one(int64_t d, int lane, int64_t nd, const uint8_t* data,
+                               const int64_t* woff, const int64_t* cnt, const double* st5,
+                               int64_t capc, kd::BatchState* st, double* cm, double* cw,
+                               unsigned long long* over) {
   const int64_t c = cnt[d];
   if (c > capc) {
     if (lane == 0) atomicMin(over, (unsigned long long)d);
@@ -1810,6 +1860,16 @@ __global__ void k_batch_load(int64_t nd, const uint8_t* data, const int64_t* wof
     cm[d * capc + i] = kd_bits2d(wire_u64(p + 16 * i));
     cw[d * capc + i] = kd_bits2d(wire_u64(p + 16 * i + 8));
   }
+}
+
+__global__ void k_batch_load(int64_t nd, const uint8_t* data, const int64_t* woff,
+                             const int64_t* cnt, const double* st5, int64_t capc,
+                             kd::BatchState* st, double* cm, double* cw,
+                             unsigned long long* over) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t d = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; d < nd; d += stride)
+    batch_load_one(d, lane, nd, data, woff, cnt, st5, capc, st, cm, cw, over);
 }
 }  // namespace
 
@@ -1849,7 +1909,7 @@ static kd_batch* batch_from_wire(kd_session* s, int scale, int64_t nd, const uin
   }
   std::unique_ptr<kd_batch> b = new_batch(s, scale, nd, kd_bits2d(cb), kd_bits2d(lb));
   const int64_t capc = kd::batch_capacity_centroids();
-  k_batch_load<<<blocks_for(nd * 64), 256, 0, s->st()>>>(
+  k_batch_load<<<wave_grid(nd), 256, 0, s->st()>>>(
       nd, d_data, d_off, cnt, st5, capc, static_cast<kd::BatchState*>(b->st.p),
       static_cast<double*>(b->cm.p), static_cast<double*>(b->cw.p), err + 3);
   launched("k_batch_load");

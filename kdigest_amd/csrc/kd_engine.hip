@@ -2998,6 +2998,53 @@ __device__ inline void finish_result(const SegDesc& sd, int si, int64_t C, doubl
   res[si] = r;
 }
 
+// A batch minimum or maximum that is zero takes the sign of the first zero in
+// batch order: the reference starts from values[0] and replaces it only by a
+// strictly smaller (larger) value (tdigest.mojo:272-281), so among -0.0 and
+// +0.0 the first one seen stays.  The paths above find the extremes from
+// sorted or order-keyed values, which do not tell the two zeros apart; this
+// pass rereads only the segments whose extreme is zero, each up to its first
+// zero.  Blocks take 256 segments at a time and list the flagged ones.
+__global__ __launch_bounds__(256) void k_zero_signs(const SegDesc* segs, int nseg,
+                                                    const double* v, SegResult* res) {
+  __shared__ int list[256];
+  __shared__ int nl;
+  __shared__ unsigned long long first;
+  for (int base = blockIdx.x * 256; base < nseg; base += gridDim.x * 256) {
+    if (threadIdx.x == 0) nl = 0;
+    __syncthreads();
+    const int si = base + (int)threadIdx.x;
+    if (si < nseg && segs[si].v_cnt > 0 && (res[si].vmin == 0.0 || res[si].vmax == 0.0))
+      list[atomicAdd(&nl, 1)] = si;
+    __syncthreads();
+    const int n = nl;
+    for (int t = 0; t < n; t++) {
+      const int sg = list[t];
+      const double* x = v + segs[sg].v_off;
+      const int64_t cnt = segs[sg].v_cnt;
+      if (threadIdx.x == 0) first = (unsigned long long)cnt;
+      __syncthreads();
+      for (int64_t c0 = 0; c0 < cnt; c0 += 256 * 8) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int64_t i = c0 + j * 256 + threadIdx.x;
+          if (i < cnt && x[i] == 0.0) atomicMin(&first, (unsigned long long)i);
+        }
+        __syncthreads();
+        const bool found = first < (unsigned long long)cnt;
+        __syncthreads();
+        if (found) break;
+      }
+      if (threadIdx.x == 0 && first < (unsigned long long)cnt) {
+        const double z = x[first];
+        if (res[sg].vmin == 0.0) res[sg].vmin = z;
+        if (res[sg].vmax == 0.0) res[sg].vmax = z;
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Many segments: one lane each, walking its centroids.
 // only: -1 every segment, 0 the sort path's (mode 0), 1 the others.
 __global__ void k_finish_lane(const SegDesc* segs, int nseg, const int64_t* ccount, double* cm,
@@ -3712,6 +3759,9 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_finish_lane<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres,
                                                           finish_only);
     launched("k_finish");
+    k_zero_signs<<<(unsigned)std::min((nseg + 255) / 256, 1024), 256, 0, stream_>>>(dsegs, nseg, d_v,
+                                                                                   dres);
+    launched("k_zero_signs");
     phase_end(ph_);
   }
   check_hip(hipGetLastError(), "kernel launch");

@@ -60,7 +60,6 @@ class Server {
   };
   void refresh_state(const std::vector<int64_t>& keys);
   void finish(int64_t id);
-  void renorm_slot(int64_t key, double factor);
 
   Engine& eng_;
   int scale_;
@@ -69,7 +68,8 @@ class Server {
   int64_t cap_, budget_;
   std::vector<double> stored_, logs_, clock_, vmin_, vmax_;
   std::vector<int64_t> count_;
-  DevBuf m_, w_, S_, dcount_, dstate_, tmpk_, tmps_, qkind_, qkey_, qx_, qout_, vbuf_, meta_;
+  DevBuf m_, w_, S_, dcount_, dstate_, tmpk_, tmps_, qkind_, qkey_, qx_, qout_, vbuf_, meta_,
+      rkey_, rfac_, rbak_, rlist_;
   std::deque<int64_t> pending_;
   std::unordered_map<int64_t, Request> reqs_;
   int64_t next_id_ = 1;
@@ -79,8 +79,8 @@ class Server {
   std::vector<double> completed_t_;
   std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
   // Host staging of the asynchronous state and install copies.
-  std::vector<int64_t> h_keys_, h_meta_;
-  std::vector<double> h_state_;
+  std::vector<int64_t> h_keys_, h_meta_, h_rkeys_;
+  std::vector<double> h_state_, h_rfac_;
   SegVec segv_;
   ResVec resv_;
 };

@@ -85,6 +85,32 @@ __global__ void k_install(const int64_t* keys, const int64_t* xoff, const int64_
   if (threadIdx.x == 0) count[k] = c;
 }
 
+// _renormalize of one key per block (tdigest.mojo:366-367): its centroid
+// weights times e^(-log_scale).  The weights before it are kept in
+// backup[r * cap ...], so a request whose merge then fails can restore them.
+__global__ void k_renorm_slots(const int64_t* keys, const double* factor, int64_t nr, int64_t cap,
+                               const int64_t* count, double* w, double* backup) {
+  const int64_t r = blockIdx.x;
+  if (r >= nr) return;
+  const int64_t k = keys[r], c = count[k];
+  const double f = factor[r];
+  for (int64_t i = threadIdx.x; i < c; i += blockDim.x) {
+    const double x = w[k * cap + i];
+    backup[r * cap + i] = x;
+    w[k * cap + i] = x * f;
+  }
+}
+
+// Puts back the weights k_renorm_slots saved for the renormalized keys whose
+// merge failed: list[j] is the index r of such a key in that launch.
+__global__ void k_restore_slots(const int64_t* keys, const int64_t* list, int64_t nl, int64_t cap,
+                                const int64_t* count, const double* backup, double* w) {
+  const int64_t j = blockIdx.x;
+  if (j >= nl) return;
+  const int64_t r = list[j], k = keys[r], c = count[k];
+  for (int64_t i = threadIdx.x; i < c; i += blockDim.x) w[k * cap + i] = backup[r * cap + i];
+}
+
 unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -301,6 +327,11 @@ int64_t Server::step(int max_rounds) {
       double stored, logs, clock;
     };
     std::vector<Saved> saved;
+    std::vector<int64_t>& rkeys = h_rkeys_;
+    std::vector<double>& rfac = h_rfac_;
+    std::vector<int64_t> ridx;  // per segment: its index among the renormalized keys, or -1
+    rkeys.clear();
+    rfac.clear();
     for (int64_t id : round) {
       const Request& r = reqs_.at(id);
       int64_t k = r.key, n = (int64_t)r.data.size();
@@ -311,13 +342,12 @@ int64_t Server::step(int max_rounds) {
         logs_[k] += lambda_ * (nclock - clock_[k]);
         if (logs_[k] > 400.0) {  // _renormalize (:359-371)
           double f = kd_exp(-logs_[k]);
-          renorm_slot(k, f);
+          ridx.resize(keys.size(), -1);
+          ridx.push_back((int64_t)rkeys.size());
+          rkeys.push_back(k);  // the slot's weights are rescaled on the device below
+          rfac.push_back(f);
           stored_[k] *= f;
           logs_[k] = 0.0;
-          // A failed merge keeps the renormalized weights: the saved state
-          // is rescaled to match them (same digest, other representation).
-          saved.back().stored *= f;
-          saved.back().logs = 0.0;
         }
       }
       clock_[k] = nclock;
@@ -340,19 +370,33 @@ int64_t Server::step(int max_rounds) {
       vals.insert(vals.end(), r.data.begin(), r.data.end());
     }
     hipStream_t st = eng_.stream();
+    if (!rkeys.empty()) {
+      const int64_t nr = (int64_t)rkeys.size();
+      int64_t* drk = static_cast<int64_t*>(rkey_.reserve(nr * sizeof(int64_t)));
+      double* drf = static_cast<double*>(rfac_.reserve(nr * sizeof(double)));
+      check_hip(hipMemcpyAsync(drk, rkeys.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+      check_hip(hipMemcpyAsync(drf, rfac.data(), nr * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+      double* dbk = static_cast<double*>(rbak_.reserve(nr * cap_ * sizeof(double)));
+      k_renorm_slots<<<(unsigned)nr, 64, 0, st>>>(drk, drf, nr, cap_, static_cast<int64_t*>(dcount_.p),
+                                                  static_cast<double*>(w_.p), dbk);
+      launched("k_renorm_slots");
+    }
+    ridx.resize(keys.size(), -1);
     double* dv = static_cast<double*>(vbuf_.reserve(std::max<size_t>(vals.size(), 1) * sizeof(double)));
     check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
     ResVec& res = resv_;
     eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
     // A merge that would not fit its key's slot fails that request alone:
-    // the key keeps its state and nothing is installed for it.
-    std::vector<int64_t> ok;
+    // the key keeps its state (a renormalized slot gets its weights back)
+    // and nothing is installed for it.
+    std::vector<int64_t> ok, restore;
     for (size_t i = 0; i < segs.size(); i++) {
       if (res[i].count > cap_) {
         int64_t k = keys[i];
         stored_[k] = saved[i].stored;
         logs_[k] = saved[i].logs;
         clock_[k] = saved[i].clock;
+        if (ridx[i] >= 0) restore.push_back(ridx[i]);
         Request& r = reqs_.at(round[i]);
         r.err = KD_ENOMEM_SERVE;
         r.msg = "key " + std::to_string(k) + " needs " + std::to_string(res[i].count) +
@@ -361,13 +405,21 @@ int64_t Server::step(int max_rounds) {
         ok.push_back((int64_t)i);
       }
     }
+    if (!restore.empty()) {
+      const int64_t nl = (int64_t)restore.size();
+      int64_t* dl = static_cast<int64_t*>(rlist_.reserve(nl * sizeof(int64_t)));
+      check_hip(hipMemcpyAsync(dl, restore.data(), nl * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+      k_restore_slots<<<(unsigned)nl, 64, 0, st>>>(static_cast<int64_t*>(rkey_.p), dl, nl, cap_,
+                                                   static_cast<int64_t*>(dcount_.p),
+                                                   static_cast<double*>(rbak_.p), static_cast<double*>(w_.p));
+      launched("k_restore_slots");
+      check_hip(hipStreamSynchronize(st), "sync");  // `restore` is a local staging vector
+    }
     int64_t ns = (int64_t)ok.size();
     std::vector<int64_t>& meta = h_meta_;
     meta.resize(3 * std::max<int64_t>(ns, 1));
-    std::vector<int64_t> okeys(ns);
     for (int64_t q = 0; q < ns; q++) {
       int64_t i = ok[q];
-      okeys[q] = keys[i];
       meta[q] = keys[i];
       meta[ns + q] = segs[i].x_off;
       meta[2 * ns + q] = res[i].count;
@@ -395,7 +447,8 @@ int64_t Server::step(int max_rounds) {
       stored_[k] = res[i].total;  // :304-315
       count_[k] = res[i].count;
     }
-    keys.swap(okeys);
+    // Every key of the round gets its query state rewritten (a failed key's
+    // is unchanged, rewriting it keeps the code path single).
     refresh_state(keys);
     for (int64_t id : round) finish(id);
     done += (int64_t)round.size();
@@ -403,17 +456,6 @@ int64_t Server::step(int max_rounds) {
   check_hip(hipStreamSynchronize(eng_.stream()), "sync");
   if (eng_.profiling()) eng_.host_phase("serve_step", ms_since(t_step));
   return done;
-}
-
-void Server::renorm_slot(int64_t key, double factor) {
-  // Centroid weights of one key scale by e^(-log_scale) (tdigest.mojo:366-367).
-  std::vector<double> w(count_[key]);
-  double* dw = static_cast<double*>(w_.p) + key * cap_;
-  check_hip(hipMemcpyAsync(w.data(), dw, w.size() * sizeof(double), hipMemcpyDeviceToHost, eng_.stream()), "D2H");
-  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
-  for (double& x : w) x *= factor;
-  check_hip(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, eng_.stream()), "H2D");
-  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
 }
 
 bool Server::result(int64_t id, std::vector<double>& out, int& err, std::string& msg, bool erase) {

@@ -246,6 +246,19 @@ def _device_ptr(t) -> int:
     return int(t.data_ptr())
 
 
+def _device_array(t, dtype_name: str, n: int | None, what: str) -> int:
+    """Address of a contiguous device tensor of the given torch dtype name and
+    element count; a wrong dtype, length, layout or device raises ValueError
+    before anything reaches the library."""
+    if str(t.dtype) != "torch." + dtype_name:
+        raise ValueError(f"{what} must be a torch.{dtype_name} tensor (got {t.dtype})")
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{what} must be a contiguous device tensor")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{what} has {t.numel()} elements, expected {n}")
+    return int(t.data_ptr())
+
+
 # ----------------------------------------------------------------- TDigest
 class TDigest:
     """Merging t-digest with pluggable scale and decay (tdigest.mojo:97-726)."""
@@ -441,8 +454,14 @@ class GroupedTDigest:
         return float(out[0]) if scalar else out
 
     def quantile_device(self, keys, qs, out) -> None:
-        check(load().kd_grouped_quantile_device(self._h, _device_ptr(keys), _device_ptr(qs),
-                                                int(qs.numel()), _device_ptr(out)))
+        """quantile(keys[i], qs[i]) into out[i] for device tensors: keys int64,
+        qs and out float64, all of one length.  A key out of range, a q outside
+        [0, 1] or an empty key raises (the outputs of those queries are NaN)."""
+        n = int(qs.numel())
+        kp = _device_array(keys, "int64", n, "keys")
+        qp = _device_array(qs, "float64", n, "qs")
+        op = _device_array(out, "float64", n, "out")
+        check(load().kd_grouped_quantile_device(self._h, kp, qp, n, op))
 
     def to_bytes(self, key: int) -> bytes:
         L = load()

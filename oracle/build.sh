@@ -8,7 +8,7 @@ set -euo pipefail
 here="$(cd "$(dirname "$0")" && pwd)"
 out="$here/_build"
 mkdir -p "$out"
-CFLAGS="-O2 -std=gnu11 -fPIC -shared -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function"
+CFLAGS="-O3 -std=gnu11 -fPIC -shared -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function"
 gcc $CFLAGS -o "$out/libkd_oracle.so.tmp" "$here/kd_oracle.c" -lm -lpthread
 mv "$out/libkd_oracle.so.tmp" "$out/libkd_oracle.so"
 gcc $CFLAGS -DKD_ORACLE_GLIBC_K=1 -o "$out/libkd_oracle_glibc.so.tmp" "$here/kd_oracle.c" -lm -lpthread

@@ -673,3 +673,31 @@ double ko_rank_error(const double *sorted, int64_t n, double v, double q) {
   double r = (double)lo / (double)n - q;
   return r < 0 ? -r : r;
 }
+
+/* Vectors of queries: out[i] = quantile(qs[i]) (or cdf), stopping at the
+ * first error. */
+int ko_quantiles(ko_digest *d, const double *qs, int64_t n, double *out) {
+  for (int64_t i = 0; i < n; i++) {
+    int rc = ko_quantile(d, qs[i], &out[i]);
+    if (rc) return rc;
+  }
+  return KO_OK;
+}
+
+int ko_cdfs(ko_digest *d, const double *xs, int64_t n, double *out) {
+  for (int64_t i = 0; i < n; i++) {
+    int rc = ko_cdf(d, xs[i], &out[i]);
+    if (rc) return rc;
+  }
+  return KO_OK;
+}
+
+/* add_at per sample (tdigest.mojo:163-175), in order. */
+int ko_add_at_many(ko_digest *d, const double *values, const double *times,
+                   const double *weights, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    int rc = ko_add_at(d, values[i], times[i], weights[i]);
+    if (rc) return rc;
+  }
+  return KO_OK;
+}

@@ -62,6 +62,10 @@ void ko_merge(ko_digest *d);
 int ko_quantile(ko_digest *d, double q, double *out);
 int ko_cdf(ko_digest *d, double x, double *out);
 int ko_merge_into(ko_digest *self, ko_digest *other);
+int ko_quantiles(ko_digest *d, const double *qs, int64_t n, double *out);
+int ko_cdfs(ko_digest *d, const double *xs, int64_t n, double *out);
+int ko_add_at_many(ko_digest *d, const double *values, const double *times,
+                   const double *weights, int64_t n);
 double ko_total_weight(const ko_digest *d);
 
 /* Serde v1 (tdigest.mojo:627-698).  ko_to_bytes returns the size; writes when

@@ -47,6 +47,9 @@ def lib(variant: str = "kd") -> C.CDLL:
         "ko_quantile": (i32, [vp, d, _dp]),
         "ko_cdf": (i32, [vp, d, _dp]),
         "ko_merge_into": (i32, [vp, vp]),
+        "ko_quantiles": (i32, [vp, _dp, i64, _dp]),
+        "ko_cdfs": (i32, [vp, _dp, i64, _dp]),
+        "ko_add_at_many": (i32, [vp, _dp, _dp, _dp, i64]),
         "ko_total_weight": (d, [vp]),
         "ko_to_bytes": (i64, [vp, C.c_char_p, i64]),
         "ko_from_bytes": (i32, [C.c_char_p, i64, i32, C.POINTER(vp)]),
@@ -130,6 +133,24 @@ class Digest:
         out = C.c_double()
         _check(self.L.ko_cdf(self.h, x, C.byref(out)))
         return out.value
+
+    def quantiles(self, qs) -> np.ndarray:
+        a = np.ascontiguousarray(qs, dtype=np.float64)
+        out = np.empty(a.size)
+        _check(self.L.ko_quantiles(self.h, _ptr(a), a.size, _ptr(out)))
+        return out
+
+    def cdfs(self, xs) -> np.ndarray:
+        a = np.ascontiguousarray(xs, dtype=np.float64)
+        out = np.empty(a.size)
+        _check(self.L.ko_cdfs(self.h, _ptr(a), a.size, _ptr(out)))
+        return out
+
+    def add_at_many(self, values, times, weights):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        _check(self.L.ko_add_at_many(self.h, _ptr(v), _ptr(t), _ptr(w), v.size))
 
     def merge_into(self, other: "Digest"):
         _check(self.L.ko_merge_into(self.h, other.h))

@@ -22,6 +22,16 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def kd():
+    # torch ships its own HIP runtime; when a test also hands torch device
+    # tensors to the library, torch's runtime must be initialized before the
+    # library's (the reverse order leaves torch without a device).
+    try:
+        import torch
+
+        if torch.cuda.device_count() > 0:
+            torch.cuda.init()
+    except ImportError:
+        pass
     import kdigest_amd
 
     kdigest_amd.load()

@@ -201,3 +201,63 @@ def test_serve_slot_overflow_fails_only_that_request(kd):
     q = srv.submit_quantile(1, [0.5])
     srv.step(1)
     assert srv.result(q)[0] == 2.0
+
+
+def test_serve_renormalize_then_slot_overflow(kd, oracle):
+    """A request whose prepare renormalizes its key (the decay exponent passes
+    400, tdigest.mojo:324-332) and whose merge then overflows the key's slot
+    fails alone; the key is left exactly as before the request (weights,
+    stored total, exponent, clock), so the next queries agree with the oracle
+    digest that never saw the failed request."""
+    lam = 0.01
+    srv = kd.DigestServer(2, 100.0, kd.Decay.rate(lam), scale=kd.ScaleK1, slot_capacity=40,
+                          mem_budget=64 << 20)
+    rng = np.random.default_rng(12)
+    first = rng.standard_normal(30)
+    r = oracle.Digest(1, 100.0, lam)
+    r.add_batch_b(first)
+    a = srv.submit_ingest(0, first)
+    srv.step()
+    assert srv.result(a).size == 0
+    m0, w0 = srv.centroids(0)
+    st0 = srv.key_state(0)
+    b = srv.submit_ingest(0, rng.standard_normal(50000))  # +500 to the exponent, > 40 centroids
+    srv.step()
+    with pytest.raises(kd.KdigestError) as e:
+        srv.result(b)
+    assert e.value.code == -7
+    st, rs = srv.key_state(0), r.state()
+    assert st == st0
+    for k in ("log_scale", "clock", "min", "max"):
+        assert st[k] == rs[k], (k, st[k], rs[k])
+    assert abs(st["stored_total"] - rs["stored_total"]) <= 1e-12 * rs["stored_total"]
+    m, w = srv.centroids(0)
+    assert np.array_equal(m, m0) and np.array_equal(w, w0)
+    rm, rw = r.centroids()
+    check_centroids(m, w, rm, rw, 5.0, False, "kept")
+    qs = np.array([0.0, 0.01, 0.1, 0.5, 0.9, 0.99, 1.0])
+    xs = np.concatenate([first[:5], [-5.0, 0.0, 5.0]])
+    iq = srv.submit_quantile(0, qs)
+    ic = srv.submit_cdf(0, xs)
+    srv.step()
+    gq, gc = srv.result(iq), srv.result(ic)
+    eq = r.quantiles(qs)
+    ec = r.cdfs(xs)
+    assert np.all(np.abs(gq - eq) <= QUERY_RTOL * np.maximum(1.0, np.abs(eq))), (gq, eq)
+    assert np.all(np.abs(gc - ec) <= QUERY_RTOL), (gc, ec)
+    # A later request that renormalizes (+450) and fits goes through as on the
+    # CPU: values on a 0.5 grid (about 17 distinct) tie into few centroids.
+    more = np.round(2.0 * rng.standard_normal(45000)) / 2.0
+    r2 = oracle.Digest(1, 100.0, lam)
+    r2.add_batch_b(first)
+    r2.add_batch_b(more)
+    assert len(r2) <= 40
+    c = srv.submit_ingest(0, more)
+    srv.step()
+    assert srv.result(c).size == 0
+    st, rs = srv.key_state(0), r2.state()
+    for k in ("log_scale", "clock", "min", "max"):
+        assert st[k] == rs[k], (k, st[k], rs[k])
+    m, w = srv.centroids(0)
+    rm, rw = r2.centroids()
+    check_centroids(m, w, rm, rw, 5.0, False, "renormalized")

@@ -337,3 +337,20 @@ def test_add_batch_crossover(oracle, session, cross):
     hi.add_batch(x, with_session=session)
     hi_ref.add_batch_b(x)
     assert hi.to_bytes() == hi_ref.to_bytes() and hi.nbuffer() == 0
+
+
+def test_vector_helpers_match_scalar_calls(oracle):
+    # The vector query / add_at helpers are loops over the scalar entry points.
+    rng = np.random.default_rng(30)
+    d = oracle.Digest(1, 100.0)
+    d.add_loop(rng.standard_normal(5000))
+    qs = rng.uniform(0, 1, 50)
+    assert d.quantiles(qs).tolist() == [d.quantile(float(q)) for q in qs]
+    xs = rng.normal(0, 2, 50)
+    assert d.cdfs(xs).tolist() == [d.cdf(float(x)) for x in xs]
+    a, b = oracle.Digest(2, 300.0, 1e-3), oracle.Digest(2, 300.0, 1e-3)
+    v, t, w = rng.standard_t(3, 4000), np.cumsum(rng.exponential(1e-3, 4000)), rng.uniform(0.1, 10, 4000)
+    a.add_at_many(v, t, w)
+    for i in range(v.size):
+        b.add_at(float(v[i]), float(t[i]), float(w[i]))
+    assert a.to_bytes() == b.to_bytes()
