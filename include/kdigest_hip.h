@@ -209,6 +209,17 @@ int kd_grouped_cdf(kd_grouped *g, const int64_t *keys, const double *xs, int64_t
  * others are written).  Synchronous: returns after the results are in d_out. */
 int kd_grouped_quantile_device(kd_grouped *g, const int64_t *d_keys, const double *d_qs,
                                int64_t nq, double *d_out);
+/* No counterpart in the reference.  kd_grouped_quantile_device for callers
+ * that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid it): same arguments, checks and results, and
+ * no allocation of any kind.  Its scratch (every key's query prefix and
+ * state, the error flag) is a fixed 64 MiB array per GPU that the library
+ * holds until the process ends, so a digest may have at most
+ * 256 + 32 * num_keys + 8 * centroids <= 64 MiB bytes of it (else KD_EINVAL,
+ * nothing launched).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (several host threads or streams at once) race. */
+int kd_grouped_quantile_device_noalloc(kd_grouped *g, const int64_t *d_keys, const double *d_qs,
+                                       int64_t nq, double *d_out);
 int64_t kd_grouped_to_bytes(kd_grouped *g, int64_t key, uint8_t *out, int64_t cap); /* :176-180 */
 int kd_grouped_centroid_count(kd_grouped *g, int64_t key, int64_t *out);            /* :182-185 */
 int kd_grouped_total_weight(kd_grouped *g, int64_t key, double *out);               /* :187-190 */

@@ -72,6 +72,7 @@ SIGNATURES = {
     "kd_grouped_quantile": (_i, [_vp, _i64p, _dp, _i64, _dp]),
     "kd_grouped_cdf": (_i, [_vp, _i64p, _dp, _i64, _dp]),
     "kd_grouped_quantile_device": (_i, [_vp, _vp, _vp, _i64, _vp]),
+    "kd_grouped_quantile_device_noalloc": (_i, [_vp, _vp, _vp, _i64, _vp]),
     "kd_grouped_to_bytes": (_i64, [_vp, _i64, _u8p, _i64]),
     "kd_grouped_wire_size": (_i, [_vp, _i64p]),
     "kd_grouped_params": (_i, [_vp, _dp, _dp]),

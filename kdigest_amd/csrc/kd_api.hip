@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -235,13 +236,53 @@ struct kd_session {
 
 extern "C" const char* kd_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------- allocation-free scratch
+// Device memory of kd_grouped_quantile_device_noalloc: one fixed array per
+// GPU, held until the process ends, shared by every call on that GPU (calls
+// that overlap on one GPU race).  In host code `scratch` is only a stand-in
+// name; the device address of each GPU's copy is resolved once, with that GPU
+// current, when a session is created on it, and kept in g_scratch.
+namespace {
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch[kScratchDevices] = {};
+
+void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kScratchDevices)
+    throw KdError(KD_EINVAL, "device id beyond the scratch table");
+  if (g_scratch[device]) return;
+  hip_ok(hipSetDevice(device), "hipSetDevice");
+  void* p = nullptr;
+  hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!p) throw KdError(KD_EDEVICE, "no device address for the scratch array");
+  g_scratch[device] = p;
+}
+
+void* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kScratchDevices || !g_scratch[device])
+    throw KdError(KD_EDEVICE, "scratch not set up on this device (create a session on it first)");
+  return g_scratch[device];
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace
+
 extern "C" int kd_session_create(int device, kd_session** out) {
   return guarded([&] {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
       throw KdError(KD_EDEVICE, "no HIP device available");
     if (device < 0 || device >= n) throw KdError(KD_EINVAL, "device id out of range");
-    *out = new kd_session(device);
+    std::unique_ptr<kd_session> s(new kd_session(device));
+    resolve_scratch(device);
+    *out = s.release();
     return KD_OK;
   });
 }
@@ -1391,6 +1432,47 @@ extern "C" int kd_grouped_quantile_device(kd_grouped* g, const int64_t* d_keys, 
     g->s->d2h(&herr, derr, sizeof(int));
     g->s->sync();
     // Checked in the reference's order: key, then q range, then emptiness.
+    if (herr & 1) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    if (herr & 2) throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
+    if (herr & 4) throw KdError(KD_EEMPTY, "quantile of an empty digest is undefined");
+    return KD_OK;
+  });
+}
+
+// The same queries with no device allocation: the prefix of every key, the
+// per-key query state and the error flag live in the GPU's `scratch` array
+// (rebuilt by every call), so a digest needs 256 + 32 * num_keys + 8 * centroids
+// bytes of it at most.
+extern "C" int kd_grouped_quantile_device_noalloc(kd_grouped* g, const int64_t* d_keys,
+                                                  const double* d_qs, int64_t nq, double* d_out) {
+  return guarded([&] {
+    if (nq <= 0) return KD_OK;
+    if (nq > kMaxGroupedBatch) throw KdError(KD_EINVAL, "too many queries in one call");
+    const int64_t nk = g->num_keys, nc = g->total_cent;
+    const size_t need = 256 + (size_t)32 * (size_t)nk + (size_t)8 * (size_t)std::max<int64_t>(nc, 1);
+    if (need > kScratchBytes)
+      throw KdError(KD_EINVAL, "digest needs " + std::to_string(need) +
+                                   " bytes of query scratch; the allocation-free entry point has " +
+                                   std::to_string(kScratchBytes));
+    char* base = static_cast<char*>(scratch_of(g->s->eng.device()));
+    int* derr = reinterpret_cast<int*>(base);
+    double* st = reinterpret_cast<double*>(base + 256);
+    double* dS = st + 4 * nk;
+    hipStream_t sm = g->s->st();
+    k_fill_int<<<1, 64, 0, sm>>>(derr, 1, 0);
+    launched("k_fill_int");
+    k_query_prefix<<<blocks_for(nk, 64), 64, 0, sm>>>(nk, g->doff(), g->dw(), dS);
+    launched("k_query_prefix");
+    k_grouped_qstate<<<blocks_for(nk), 256, 0, sm>>>(nk, g->dstate(0), g->dstate(1), g->dstate(3),
+                                                     g->dstate(4), g->decay() ? 1 : 0, st);
+    launched("k_grouped_qstate");
+    k_query_grouped<<<blocks_for(nq), 256, 0, sm>>>(0, nk, g->doff(), g->dm(), g->dw(), dS, st,
+                                                    st + nk, st + 2 * nk, st + 3 * nk, d_keys, d_qs,
+                                                    nq, d_out, derr);
+    launched("k_query_grouped");
+    int herr = 0;
+    hip_ok(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, sm), "D2H");
+    hip_ok(hipStreamSynchronize(sm), "stream sync");
     if (herr & 1) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
     if (herr & 2) throw KdError(KD_ERANGE, "quantile q must be in [0, 1]");
     if (herr & 4) throw KdError(KD_EEMPTY, "quantile of an empty digest is undefined");

@@ -463,6 +463,15 @@ class GroupedTDigest:
         op = _device_array(out, "float64", n, "out")
         check(load().kd_grouped_quantile_device(self._h, kp, qp, n, op))
 
+    def quantile_device_noalloc(self, keys, qs, out) -> None:
+        """quantile_device without any device allocation (the library's fixed
+        per-GPU scratch; see kd_grouped_quantile_device_noalloc)."""
+        n = int(qs.numel())
+        kp = _device_array(keys, "int64", n, "keys")
+        qp = _device_array(qs, "float64", n, "qs")
+        op = _device_array(out, "float64", n, "out")
+        check(load().kd_grouped_quantile_device_noalloc(self._h, kp, qp, n, op))
+
     def to_bytes(self, key: int) -> bytes:
         L = load()
         need = L.kd_grouped_to_bytes(self._h, int(key), None, 0)
