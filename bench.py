@@ -235,6 +235,55 @@ def grouped_parity(kd, g, keys, vals, nk: int, refs=None, threads: int | None = 
     return rep, refs
 
 
+def plain_cpu(kd, session):
+    """Plain single digest, ScaleK1, compression 100, no decay: 2^20 N(0,1)
+    values (numpy seed 0) added in chunks of 4096 (below the crossover: the
+    per-sample add path, tdigest.mojo:199-202) on the oracle (ns per add,
+    merges included, one thread) and on the library (its merges on the GPU);
+    quantiles at RANK_QS, cdf at -3, 0, 3, and the rank error of both against
+    the sorted values."""
+    from oracle import oracle as O
+
+    x = np.random.default_rng(0).standard_normal(1 << 20)
+    chunk = 4096
+    best = None
+    for _ in range(3):
+        d = O.Digest(1, 100.0)
+        t0 = time.perf_counter()
+        for a in range(0, x.size, chunk):
+            d.add_batch(x[a:a + chunk])
+        d.merge()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    g = kd.TDigest(kd.ScaleK1, 100.0, session=session)
+    t0 = time.perf_counter()
+    for a in range(0, x.size, chunk):
+        g.add_batch(x[a:a + chunk])
+    g.merge()
+    dt_gpu = time.perf_counter() - t0
+    qs = np.array(RANK_QS)
+    xs = np.array([-3.0, 0.0, 3.0])
+    oq, oc = d.quantiles(qs), d.cdfs(xs)
+    gq, gc = g.quantile(qs), g.cdf(xs)
+    srt = np.sort(x)
+    rank = lambda v: np.searchsorted(srt, v, side="right") / srt.size
+    return {"workload": "one digest, ScaleK1, compression 100, 2^20 N(0,1) values (numpy seed 0) "
+                        "added in chunks of 4096 (per-sample add path)",
+            "cpu_ns_per_add": best / x.size * 1e9, "cpu_threads": 1,
+            "reference_published_ns_per_add": 88.0,
+            "reference_note": "M1 Max, one thread, 1M U(0,1) adds (bench/Benchmark.md:102)",
+            "gpu_library_ms": dt_gpu * 1e3,
+            "gpu_note": "the same adds through the library: host buffer, merges every 500 samples on "
+                        "the GPU (latency-bound; shown for parity, not speed)",
+            "quantiles": {"q": list(RANK_QS), "oracle": oq.tolist(), "gpu": gq.tolist()},
+            "cdf": {"x": xs.tolist(), "oracle": oc.tolist(), "gpu": gc.tolist()},
+            "max_query_rel_diff": float(max(np.max(np.abs(gq - oq) / np.maximum(np.abs(oq), 1.0)),
+                                            np.max(np.abs(gc - oc)))),
+            "rank_error_oracle": (np.abs(rank(oq) - qs)).tolist(),
+            "rank_error_gpu": (np.abs(rank(gq) - qs)).tolist(),
+            "centroids": {"oracle": len(d), "gpu": len(g)}}
+
+
 def traffic_bytes(phase: str):
     """HBM bytes per launch of a kernel group, from the committed rocprof
     FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic.json), or None."""
@@ -301,7 +350,7 @@ def plain_per_key(kd, session, keys, vals, nk, ms_grouped, dev):
             "est_ms": est * 1e3, "speedup_grouped": est * 1e3 / ms_grouped}
 
 
-def batched_decay(kd, session, dev, nd=8192, seed=3):
+def batched_decay(kd, session, dev, nd=8192, seed=3, with_parity=True):
     """8192 digests, geometric sizes (mean 32768, at most 2^22), Student-t(3)
     values, U(0.1, 10) weights, add_at timestamps with Exp(1e3/s) gaps,
     lambda 1e-3/s, ScaleK2, compression 300; flush; merge_into tree to one."""
@@ -344,13 +393,106 @@ def batched_decay(kd, session, dev, nd=8192, seed=3):
     b = run()
     dt = time.perf_counter() - t0
     st = b.state(0)
-    return {"workload": f"{nd} digests, geometric sizes mean 32768 (max 2^22), {n} samples, "
-                        "Student-t(3), weights U(0.1,10), add_at with Exp(1e3/s) gaps, "
-                        "lambda 1e-3/s, ScaleK2, compression 300, flush + merge_into tree",
-            "samples": n, "max_digest": int(sizes.max().item()), "ms": dt * 1e3,
-            "samples_per_s": n / dt, "bytes_per_sample": 56.0,
-            "roofline_frac": 56.0 * n / dt / 1e9 / HBM_PEAK_GBS,
-            "root_centroids": len(b.centroids(0)[0]), "root_total": st["stored_total"]}
+    out = {"workload": f"{nd} digests, geometric sizes mean 32768 (max 2^22), {n} samples, "
+                       "Student-t(3), weights U(0.1,10), add_at with Exp(1e3/s) gaps, "
+                       "lambda 1e-3/s, ScaleK2, compression 300, flush + merge_into tree",
+           "samples": n, "max_digest": int(sizes.max().item()), "ms": dt * 1e3,
+           "samples_per_s": n / dt, "bytes_per_sample": 56.0,
+           "roofline_frac": 56.0 * n / dt / 1e9 / HBM_PEAK_GBS,
+           "root_centroids": len(b.centroids(0)[0]), "root_total": st["stored_total"]}
+    del b
+    if with_parity:
+        out["parity"] = batched_parity(kd, session, nd, off, vals, times, w)
+    return out
+
+
+def _cmp_digest_arrays(m, w, st, rm, rw, rs):
+    """(max mean rel err, max weight rel err, state fields equal) or None when
+    the centroid counts differ."""
+    if m.size != rm.size:
+        return None
+    floor = 1e-12 * max(abs(rs["min"]), abs(rs["max"]), 1e-300)
+    me = float(np.max(np.abs(m - rm) / np.maximum(np.abs(rm), floor))) if m.size else 0.0
+    we = float(np.max(np.abs(w - rw) / np.abs(rw))) if w.size else 0.0
+    eq = all(st[k] == rs[k] for k in ("clock", "log_scale", "min", "max"))
+    te = abs(st["stored_total"] - rs["stored_total"]) / max(abs(rs["stored_total"]), 1e-300)
+    return me, we, eq, te
+
+
+def batched_parity(kd, session, nd, off, vals, times, w):
+    """The batched decayed digests against the oracle: every digest replayed
+    with add_at on the host (all host threads: the all-cores CPU baseline; a
+    bounded prefix of digests on one thread: the single-thread baseline); the
+    8 largest digests and every 256th are compared after the flush, then the
+    merge_into tree roots."""
+    from oracle import oracle as O
+
+    offh, vh, th, wh = off.cpu().numpy(), vals.cpu().numpy(), times.cpu().numpy(), w.cpu().numpy()
+    sizes = np.diff(offh)
+    b = kd.BatchedDigests(nd, 300.0, kd.Decay.rate(1e-3), scale=kd.ScaleK2, session=session)
+    b.add_at_device(off, vals, times, w)
+    b.flush()
+    session.synchronize()
+    pick = np.unique(np.concatenate([np.argsort(-sizes, kind="stable")[:8], np.arange(0, nd, 256)]))
+    got = {int(d): (b.centroids(int(d)), b.state(int(d))) for d in pick}
+    b.merge_tree()
+    root_m, root_w = b.centroids(0)
+    root_st = b.state(0)
+    del b
+    thr = host_threads()
+    refs = [O.Digest(2, 300.0, 1e-3) for _ in range(nd)]
+    t0 = time.perf_counter()
+    O.batch_add_at(refs, offh, vh, th, wh, threads=thr)
+    for r in refs:
+        r.merge()
+    t_all = time.perf_counter() - t0
+    # One thread on the first digests holding about 2^25 samples.
+    k1 = int(np.searchsorted(offh, 1 << 25)) + 1
+    k1 = min(k1, nd)
+    one = [O.Digest(2, 300.0, 1e-3) for _ in range(k1)]
+    t0 = time.perf_counter()
+    O.batch_add_at(one, offh[:k1 + 1], vh, th, wh, threads=1)
+    for r in one:
+        r.merge()
+    t_one = time.perf_counter() - t0
+    del one
+    worst = {"mean": 0.0, "weight": 0.0, "total": 0.0}
+    failures = []
+    for d, ((m, ww), st) in got.items():
+        rm, rw = refs[d].centroids()
+        c = _cmp_digest_arrays(m, ww, st, rm, rw, refs[d].state())
+        if c is None:
+            failures.append(f"digest {d}: {m.size} centroids, oracle {rm.size}")
+            continue
+        worst["mean"] = max(worst["mean"], c[0])
+        worst["weight"] = max(worst["weight"], c[1])
+        worst["total"] = max(worst["total"], c[3])
+        if not c[2]:
+            failures.append(f"digest {d}: state differs")
+    stride = 1
+    while stride < nd:
+        for i in range(0, nd, 2 * stride):
+            if i + stride < nd:
+                refs[i].merge_into(refs[i + stride])
+        stride *= 2
+    rm, rw = refs[0].centroids()
+    croot = _cmp_digest_arrays(root_m, root_w, root_st, rm, rw, refs[0].state())
+    if croot is None:
+        failures.append(f"tree root: {root_m.size} centroids, oracle {rm.size}")
+    n = int(offh[-1])
+    return {"digests_compared": int(pick.size), "max_mean_rel_err": worst["mean"],
+            "max_weight_rel_err": worst["weight"], "max_stored_total_rel_err": worst["total"],
+            "tree_root": None if croot is None else
+            {"centroids": int(rm.size), "max_mean_rel_err": croot[0], "max_weight_rel_err": croot[1],
+             "state_equal": croot[2], "stored_total_rel_err": croot[3]},
+            "failures": failures,
+            "cpu_baseline": {"one_thread": {"value": float(offh[k1]) / t_one, "unit": "samples/s",
+                                            "cores": 1, "kind": "port", "s": t_one,
+                                            "sample": f"the first {k1} digests ({int(offh[k1])} samples), "
+                                                      "add_at per sample then merge"},
+                             "all_cores": {"value": n / t_all, "unit": "samples/s", "cores": thr,
+                                           "s": t_all, "sample": "all digests, add_at then merge "
+                                                                 "(the tree not included)"}}}
 
 
 def _serve_requests(rng, nk, rate, seconds, cdf, perm, pool):
@@ -379,13 +521,17 @@ def _serve_requests(rng, nk, rate, seconds, cdf, perm, pool):
     return arr, keys, kinds, sizes, off, data
 
 
-def _serve_run(srv, reqs):
+def _serve_run(srv, reqs, watch=None):
     """Submits each request once its arrival time has passed and steps the
-    server; returns per-request latency (completion minus arrival, s) and the
-    wall time."""
+    server; returns per-request latency (completion minus arrival, s), the
+    wall time and, for the requests on keys in `watch`, their outcomes
+    (query values, "empty", "enomem" or None for a finished ingest)."""
+    import kdigest_amd as kd
+
     arr, keys, kinds, sizes, off, data = reqs
     nreq = arr.size
     lat = np.full(nreq, np.nan)
+    kept = {}
     first_id = None
     i = 0
     ndone = 0
@@ -408,12 +554,20 @@ def _serve_run(srv, reqs):
         for rid, tc in zip(cids.tolist(), ct.tolist()):
             r = rid - first_id
             lat[r] = (tc - c0) - arr[r]
-            srv.discard(rid)
+            if watch is not None and int(keys[r]) in watch:
+                try:
+                    res = srv.result(rid)
+                    kept[r] = res if kinds[r] != 0 else None
+                except kd.KdigestError as e:
+                    kept[r] = "empty" if e.code == -2 else "enomem"
+            else:
+                srv.discard(rid)
         ndone += len(cids)
-    return lat, time.perf_counter() - t0
+    return lat, time.perf_counter() - t0, kept
 
 
-def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_seconds=0.5):
+def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_seconds=0.5,
+            with_parity=True):
     """Open-loop serving: Poisson arrivals at `rate` req/s for `seconds`,
     keys Zipf(1.05) over 1M device-resident digests (ScaleK1, compression
     100, half-life 1e6 samples); half the requests ingest log-uniform 1..65536
@@ -433,10 +587,19 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_s
     reqs = _serve_requests(rng, nk, rate, seconds, cdf, perm, pool)
     srv = kd.DigestServer(nk, 100.0, kd.Decay.half_life(1e6), scale=kd.ScaleK1, slot_capacity=128,
                           mem_budget=64 << 20, session=session)
-    _serve_run(srv, warm)
-    s0 = srv.stats()
-    lat, wall = _serve_run(srv, reqs)
     arr, keys, kinds, sizes, off, data = reqs
+    # Watched keys (their outcomes are kept for the oracle replay): the four
+    # with the most requests and 28 others drawn at random.
+    uk, uc = np.unique(keys, return_counts=True)
+    hot = uk[np.argsort(-uc, kind="stable")[:4]]
+    rest = np.setdiff1d(uk[uc >= 2], hot)
+    watch = set(int(k) for k in np.concatenate([hot, rng.choice(rest, min(28, rest.size),
+                                                                replace=False)]))
+    if not with_parity:
+        watch = None
+    _, _, warm_kept = _serve_run(srv, warm, watch)
+    s0 = srv.stats()
+    lat, wall, kept = _serve_run(srv, reqs, watch)
     nreq = arr.size
     lat *= 1e3
     st = srv.stats()
@@ -457,10 +620,89 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_s
             "query_p99_ms": float(np.percentile(lat[~ing], 99)),
             "ingest_samples_per_s": float(sizes[ing].sum()) / wall,
             "queries_per_s": float(sizes[~ing].sum()) / wall,
-            "admission": st, "mem_budget_bytes": 64 << 20}
+            "admission": st, "mem_budget_bytes": 64 << 20,
+            "parity": serve_parity(srv, warm, warm_kept, reqs, kept, watch) if with_parity else None}
 
 
-def single_digest(kd, session, dev, single_n: int):
+def serve_parity(srv, warm, warm_kept, reqs, kept, watch):
+    """Oracle replay of the serving request log: every ingest (warm-up and
+    measured run, arrival order, one B batch merge per request) on all host
+    threads (the CPU baseline), and for the watched keys every query answer
+    and the final digest compared with the oracle at the same point of the
+    replay."""
+    from oracle import oracle as O
+
+    def ingest_log(rq):
+        arr, keys, kinds, sizes, off, data = rq
+        sel = np.nonzero(kinds == 0)[0]
+        return keys[sel], sel, off, data
+
+    # All host threads, every ingest of both runs (dense ids of touched keys).
+    kw, sw, ow, dw = ingest_log(warm)
+    km, sm, om, dm = ingest_log(reqs)
+    all_keys = np.concatenate([kw, km])
+    ids, dense = np.unique(all_keys, return_inverse=True)
+    pieces = [dw[ow[r]:ow[r + 1]] for r in sw] + [dm[om[r]:om[r + 1]] for r in sm]
+    lens = np.array([p.size for p in pieces], np.int64)
+    roff = np.zeros(lens.size + 1, np.int64)
+    roff[1:] = np.cumsum(lens)
+    rdata = np.concatenate(pieces)
+    thr = host_threads()
+    refs = [O.Digest(1, 100.0, math.log(2.0) / 1e6) for _ in range(ids.size)]
+    t0 = time.perf_counter()
+    O.replay_ingests(refs, dense, roff, rdata, threads=thr)
+    t_all = time.perf_counter() - t0
+    del refs, rdata, pieces
+    # Watched keys: replay in arrival order, answering queries on the way.
+    lam = math.log(2.0) / 1e6
+    ref = {k: O.Digest(1, 100.0, lam) for k in watch}
+    worst_q, nq, failures = 0.0, 0, []
+    for rq, kp, measured in ((warm, warm_kept, False), (reqs, kept, True)):
+        arr, keys, kinds, sizes, off, data = rq
+        for r in np.nonzero(np.isin(keys, list(watch)))[0]:
+            k, x = int(keys[r]), data[off[r]:off[r + 1]]
+            out = kp.get(int(r), "missing")
+            if kinds[r] == 0:
+                if out != "enomem":
+                    ref[k].add_batch_b(x)
+                continue
+            if not measured:
+                continue
+            d = ref[k]
+            if len(d) == 0 and d.nbuffer() == 0:
+                if not (isinstance(out, str) and out == "empty"):
+                    failures.append(f"request {r}: expected an empty-key error")
+                continue
+            exp = d.quantiles(x) if kinds[r] == 1 else d.cdfs(x)
+            if isinstance(out, str):
+                failures.append(f"request {r}: {out}")
+                continue
+            err = np.abs(out - exp) / np.maximum(np.abs(exp), 1.0)
+            worst_q = max(worst_q, float(err.max()))
+            nq += x.size
+    worst_m, worst_w = 0.0, 0.0
+    for k in watch:
+        m, w = srv.centroids(k)
+        st = srv.key_state(k)
+        rm, rw = ref[k].centroids()
+        c = _cmp_digest_arrays(m, w, st, rm, rw, ref[k].state())
+        if c is None:
+            failures.append(f"key {k}: {m.size} centroids, oracle {rm.size}")
+            continue
+        worst_m, worst_w = max(worst_m, c[0]), max(worst_w, c[1])
+        if not c[2]:
+            failures.append(f"key {k}: state differs")
+    n_ing = int(lens.size)
+    return {"watched_keys": len(watch), "queries_compared": nq, "max_query_rel_err": worst_q,
+            "max_mean_rel_err": worst_m, "max_weight_rel_err": worst_w, "failures": failures,
+            "cpu_baseline": {"value": n_ing / t_all, "unit": "ingest requests/s", "cores": thr,
+                             "kind": "port", "s": t_all,
+                             "samples_per_s": float(roff[-1]) / t_all,
+                             "sample": f"all {n_ing} ingest requests of the warm-up and measured "
+                                       "runs, replayed in arrival order (keys over host threads)"}}
+
+
+def single_digest(kd, session, dev, single_n: int, with_parity: bool = True):
     """One digest fed 2^20-value device batches (ScaleK2, compression 200,
     half-life 1e5 samples): one warm run, then one timed run."""
     import torch
@@ -482,11 +724,81 @@ def single_digest(kd, session, dev, single_n: int):
     d = run_single()
     session.synchronize()
     dt = time.perf_counter() - t0
-    return {"workload": f"one digest, {single_n} lognormal(0,1.5) FP64 values in batches "
-                        f"of {batch}, ScaleK2, compression 200, half-life 1e5",
-            "samples_per_s": single_n / dt, "ms": dt * 1e3, "centroids": len(d),
-            "batches": (single_n + batch - 1) // batch,
-            "roofline_frac": 24.0 * single_n / dt / 1e9 / HBM_PEAK_GBS}
+    out = {"workload": f"one digest, {single_n} lognormal(0,1.5) FP64 values in batches "
+                       f"of {batch}, ScaleK2, compression 200, half-life 1e5",
+           "samples_per_s": single_n / dt, "ms": dt * 1e3, "centroids": len(d),
+           "batches": (single_n + batch - 1) // batch,
+           "roofline_frac": 24.0 * single_n / dt / 1e9 / HBM_PEAK_GBS}
+    if with_parity:
+        out["queries_and_parity"] = single_parity(kd, d, xs, batch)
+    return out
+
+
+def single_parity(kd, d, xs, batch: int):
+    """The single digest's query batch (4096 q ~ U[0,1] plus 1e-4 and 1 - 1e-4;
+    4096 cdf values at sampled data values) timed on the GPU and compared with
+    the oracle fed the same batches (B semantics, one thread: the CPU
+    baseline), plus the digest state and centroids, and the decay-weighted
+    rank error of both against the sorted data."""
+    import torch
+
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(1)
+    qs = np.concatenate([rng.uniform(0, 1, 4096), [1e-4, 1 - 1e-4]])
+    xh = xs.cpu().numpy()
+    xq = xh[rng.integers(0, xh.size, 4096)]
+    d.quantile(qs[:8])
+    t0 = time.perf_counter()
+    gq = d.quantile(qs)
+    gc = d.cdf(xq)
+    t_q = time.perf_counter() - t0
+    lam = math.log(2.0) / 1e5
+    r = O.Digest(2, 200.0, lam)
+    t0 = time.perf_counter()
+    for a in range(0, xh.size, batch):
+        r.add_batch_b(xh[a:a + batch])
+    t_cpu = time.perf_counter() - t0
+    oq, oc = r.quantiles(qs), r.cdfs(xq)
+    st, rs = d.state(), r.state()
+    m, w = d.centroids()
+    rm, rw = r.centroids()
+    same_count = m.size == rm.size
+    mean_err = float(np.max(np.abs(m - rm) / np.maximum(np.abs(rm), 1e-12 * max(abs(rs["min"]),
+                                                                                   abs(rs["max"]))))) \
+        if same_count else None
+    w_err = float(np.max(np.abs(w - rw) / np.abs(rw))) if same_count else None
+    # Decay-weighted rank error: sample i of batch b weighs e^(-lambda (T - T_b)).
+    nb = (xh.size + batch - 1) // batch
+    bw = np.exp(-lam * batch * (nb - 1 - np.arange(nb)))
+    xt = torch.from_numpy(xh)
+    order = torch.argsort(xt)
+    sx = xt[order].numpy()
+    sw = torch.from_numpy(np.repeat(bw, batch)[:xh.size])[order].numpy()
+    cw = np.cumsum(sw) / sw.sum()
+    q5 = np.array(RANK_QS)
+
+    def wrank(v):
+        i = np.searchsorted(sx, v, side="right")
+        return np.where(i > 0, cw[np.maximum(i - 1, 0)], 0.0)
+
+    return {"query_batch": {"quantiles": int(qs.size), "cdfs": int(xq.size), "ms": t_q * 1e3,
+                            "queries_per_s": (qs.size + xq.size) / t_q,
+                            "note": "host arrays in and out, one call each"},
+            "cpu_baseline": {"value": xh.size / t_cpu, "unit": "samples/s", "cores": 1,
+                             "kind": "port", "s": t_cpu,
+                             "sample": "the whole workload, same batches, oracle B-semantics merges"},
+            "centroids": {"gpu": int(m.size), "oracle": int(rm.size)},
+            "state_equal": {k: bool(st[k] == rs[k]) for k in ("clock", "log_scale", "min", "max")},
+            "stored_total_rel_diff": abs(st["stored_total"] - rs["stored_total"]) / rs["stored_total"],
+            "max_mean_rel_err": mean_err, "max_weight_rel_err": w_err,
+            "max_quantile_rel_err": float(np.max(np.abs(gq - oq) / np.maximum(np.abs(oq), 1.0))),
+            "max_cdf_abs_err": float(np.max(np.abs(gc - oc))),
+            "weighted_rank_error": {"q": list(RANK_QS),
+                                    "gpu": np.abs(wrank(d.quantile(q5)) - q5).tolist(),
+                                    "oracle": np.abs(wrank(r.quantiles(q5)) - q5).tolist(),
+                                    "note": "each sample weighted by its batch's decay factor at "
+                                            "the end of the stream"}}
 
 
 def main():
@@ -506,7 +818,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-wire", action="store_true")
-    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip every parity / oracle comparison leg")
+    ap.add_argument("--no-plain-cpu", action="store_true")
+    ap.add_argument("--batched-digests", type=int, default=8192)
     ap.add_argument("--no-keymap", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", type=str, default=None,
@@ -533,7 +848,7 @@ def main():
     torch.cuda.set_device(dev)
     session = kd.default_session(0)
     if args.only_single:
-        single = single_digest(kd, session, dev, args.single_n)
+        single = single_digest(kd, session, dev, args.single_n, with_parity=False)
         print(json.dumps({"metric": "single-digest ingest samples/s", "value": single["samples_per_s"],
                           "unit": "samples/s", "n_gpus": 1, "higher_is_better": True,
                           "dtype": "f64", "data": "synthetic (torch Philox, seeded), device-resident",
@@ -757,21 +1072,29 @@ def main():
     # ------------------------------------------------ single digest
     single = None
     if not args.no_single:
-        single = single_digest(kd, session, dev, args.single_n)
+        single = single_digest(kd, session, dev, args.single_n, with_parity=not args.no_parity)
         print(f"[bench] single {single['samples_per_s'] / 1e9:.3f} G samples/s", file=sys.stderr, flush=True)
 
     # ------------------------------------------------ batched, decayed, merge tree
     batched = None
     if not args.no_batched:
-        batched = batched_decay(kd, session, dev)
+        batched = batched_decay(kd, session, dev, nd=args.batched_digests,
+                                with_parity=not args.no_parity)
         torch.cuda.empty_cache()
         print(f"[bench] batched {batched['samples_per_s'] / 1e9:.3f} G samples/s", file=sys.stderr,
               flush=True)
 
+    # ------------------------------------------------ plain single digest (CPU)
+    plain_single = None
+    if not args.no_plain_cpu:
+        plain_single = plain_cpu(kd, session)
+        print(f"[bench] plain single digest: oracle {plain_single['cpu_ns_per_add']:.1f} ns/add",
+              file=sys.stderr, flush=True)
+
     # ------------------------------------------------ serving
     serve = None
     if not args.no_serve:
-        serve = serving(kd, session, seconds=args.serve_seconds)
+        serve = serving(kd, session, seconds=args.serve_seconds, with_parity=not args.no_parity)
         print(f"[bench] serving p50 {serve['p50_ms']:.2f} ms p99 {serve['p99_ms']:.2f} ms",
               file=sys.stderr, flush=True)
 
@@ -805,6 +1128,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "plain_per_key": plain,
+        "plain_single_cpu": plain_single,
         "single_digest": single,
         "grouped_queries": queries,
         "grouped_wire_format": wire,

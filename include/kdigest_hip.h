@@ -151,6 +151,11 @@ int kd_math_eval(kd_session *s, int fn, const double *x, int64_t n, double *out)
 double kd_scale_normalizer(int scale, double compression, double n);
 double kd_scale_k(int scale, double q, double nrm);
 double kd_scale_q(int scale, double k, double nrm);
+/* The running weight of a cluster over L equal weights, as the reference
+ * sums it (centroid.mojo:34-36: L sequential W = W + x), in O(log) steps:
+ * the host side of the helper the device sweeps use to check their
+ * boundaries in the reference's summation order (tdigest.mojo:419-435). */
+double kd_add_repeated(double W, double x, int64_t L);
 
 /* ------------------------------------------------ TDigest (one digest) -- */
 /* TDigest[S] (kdigest/tdigest.mojo:97-726).  decay_lambda is Decay.rate_lambda

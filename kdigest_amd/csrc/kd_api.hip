@@ -22,6 +22,7 @@
 #include "../../include/kdigest_hip.h"
 #include "kd_batch.h"
 #include "kd_engine.h"
+#include "kd_exact.h"
 #include "kd_math.h"
 #include "kd_query.h"
 #include "kd_serve.h"
@@ -1833,6 +1834,8 @@ extern "C" double kd_scale_normalizer(int scale, double compression, double n) {
 }
 
 extern "C" double kd_scale_k(int scale, double q, double nrm) { return kd_k(scale, q, nrm); }
+
+extern "C" double kd_add_repeated(double W, double x, int64_t L) { return kd_repeat_add(W, x, L); }
 
 // Inverse maps of scale.mojo:96-99, 120-124, 152-157, 186-192.
 extern "C" double kd_scale_q(int scale, double k, double nrm) {

@@ -18,6 +18,7 @@
 #include <cstdint>
 
 #include "kd_batch.h"
+#include "kd_exact.h"
 #include "kd_math.h"
 
 namespace kd {
@@ -79,6 +80,7 @@ __device__ inline uint64_t ord_key(double x) {
 struct Sh {
   BatchState st;
   int m, renorm, nclusters;
+  int frac, bad;  // merge_lds: some weight is not an integer; first cluster in question
   double renorm_ls;
 };
 
@@ -218,8 +220,75 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
       s = e;
     }
     sh.nclusters = c;
+    sh.frac = 0;
+    sh.bad = 0x7fffffff;
   }
   __syncthreads();
+  // The prefix above is exact when every weight is an integer; otherwise
+  // every boundary is checked in the CPU's summation order (kd_exact.h):
+  // each cluster's running weight, the sums of the clusters before it (in
+  // order), then the last absorbed candidate must not close and the next one
+  // must.  From the first cluster in question on (rare: mostly the last
+  // candidate, where q_right is about 1), lane 0 runs the CPU sweep itself.
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    if (L.ww[j] != floor(L.ww[j])) sh.frac = 1;
+  __syncthreads();
+  if ((sh.frac || !(N < 0x1p53)) && n > 1) {
+    const int C0 = sh.nclusters;
+    double* T0 = L.P;  // free until the means
+    for (int c = threadIdx.x; c < C0; c += blockDim.x) {
+      const int a = L.starts[c], b = c + 1 < C0 ? L.starts[c + 1] : n;
+      double W = L.ww[a];
+      for (int j = a + 1; j < b; j++) W = W + L.ww[j];
+      T0[c] = W;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double acc = 0.0;
+      for (int c = 0; c < C0; c++) {
+        const double w = T0[c];
+        T0[c] = acc;
+        acc = acc + w;
+      }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < C0; c += blockDim.x) {
+      const int a = L.starts[c], b = c + 1 < C0 ? L.starts[c + 1] : n;
+      const double wsf = T0[c];
+      double W = L.ww[a];
+      bool bad = false;
+      if (b - a >= 2) {
+        for (int j = a + 1; j < b - 1; j++) W = W + L.ww[j];
+        bad = kd_sweep_closes(scale, nrm, N, wsf, W, L.ww[b - 1]) != 0;
+        W = W + L.ww[b - 1];
+      }
+      if (b < n) bad = bad || !kd_sweep_closes(scale, nrm, N, wsf, W, L.ww[b]);
+      if (bad) atomicMin(&sh.bad, c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh.bad < C0) {
+      int c = sh.bad;
+      int st0 = L.starts[c];
+      double wsf = T0[c];
+      double cur_w = L.ww[st0], cur_m = L.wm[st0];
+      L.starts[c++] = st0;
+      for (int i = st0 + 1; i < n; i++) {
+        const double wi = L.ww[i], xi = L.wm[i];
+        if (xi == cur_m || !kd_sweep_closes(scale, nrm, N, wsf, cur_w, wi)) {
+          const double nw = cur_w + wi;
+          cur_m += (xi - cur_m) * (wi / nw);
+          cur_w = nw;
+        } else {
+          wsf = wsf + cur_w;
+          L.starts[c++] = i;
+          cur_w = wi;
+          cur_m = xi;
+        }
+      }
+      sh.nclusters = c;
+    }
+    __syncthreads();
+  }
   const int C = sh.nclusters;
   // Running mean and weight of every cluster, record by record
   // (centroid.mojo:34-36).  P is free now: means go there; weights go to the

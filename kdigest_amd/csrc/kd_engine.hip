@@ -26,6 +26,7 @@
 #include <stdexcept>
 #include <type_traits>
 
+#include "kd_exact.h"
 #include "kd_math.h"
 
 namespace kd {
@@ -618,6 +619,66 @@ struct PCur {
   }
 };
 
+// First E record at or after merged index j (binary search over pos).
+__device__ inline int64_t pcur_seek(const PCur& pc, int64_t j) {
+  int64_t lo = 0, hi = pc.ne;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (pc.pos[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// A positive normal power of two: its multiples by integers below 2^53 are
+// exact, so a running sum of n such weights is exactly n * w.
+__device__ inline bool pow2_weight(double w) {
+  const uint64_t b = kd_d2bits(w);
+  return (b & 0x000fffffffffffffull) == 0 && ((b >> 52) & 0x7ff) != 0 && ((b >> 52) & 0x7ff) != 0x7ff;
+}
+
+// Running weight of the merged records [a, b) in the CPU's order (new values
+// weigh sf, the prior record at merged position pos[c] weighs ew[c]); ce is
+// the first prior record at or after a.  O(prior records + binades).
+__device__ inline double run_weight(int64_t a, int64_t b, double sf, const int64_t* pos,
+                                    const double* ew, int64_t ne, int64_t ce) {
+  double W;
+  if (ce < ne && pos[ce] == a) {
+    W = ew[ce];
+    ce++;
+  } else {
+    W = sf;
+  }
+  int64_t j = a + 1;
+  while (j < b) {
+    int64_t e = ce < ne ? pos[ce] : b;
+    if (e > b) e = b;
+    if (e > j) {
+      W = kd_repeat_add(W, sf, e - j);
+      j = e;
+    }
+    if (j < b) {
+      W = W + ew[ce];
+      ce++;
+      j++;
+    }
+  }
+  return W;
+}
+
+// Weight of merged record j.
+__device__ inline double rec_weight(int64_t j, double sf, const int64_t* pos, const double* ew,
+                                    int64_t ne) {
+  if (ne == 0) return sf;
+  int64_t lo = 0, hi = ne;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (pos[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < ne && pos[lo] == j) ? ew[lo] : sf;
+}
+
 __device__ inline double kinv_est(int scale, double k, double nrm) {
   double a = k / nrm;
   switch (scale) {
@@ -819,11 +880,34 @@ __global__ __launch_bounds__(64) void k_chain(const SegDesc* segs, int nseg, int
   }
   PCur pc{pose + sd.es_off, se + sd.es_off + si, sd.e_cnt, sd.sf, 0};
   int64_t replays = 0;
+  const int64_t* pos = pose + sd.es_off;
+  const double* ew = esw + sd.es_off;
+  const int64_t ne = sd.e_cnt;
+  // The closed-form prefix decides exactly when every weight is an integer
+  // (or a power of two with no prior records); otherwise each boundary is
+  // moved to where the CPU's running sums put it (kd_exact.h), with the
+  // finished clusters' weights summed in order in wsf.
+  bool closed_exact = ne == 0 ? pow2_weight(sd.sf) : (sd.sf == 1.0 && sd.stored_n < 0x1p53);
+  for (int64_t i = 0; closed_exact && i < ne; i++) closed_exact = ew[i] == floor(ew[i]);
+  double wsf = 0.0;
+  auto next = [&](int64_t s) {
+    int64_t b = chain_next(scale, pc, s, n, sd.stored_n, sd.nrm);
+    if (closed_exact) return b;
+    const int64_t ce = pcur_seek(pc, s);
+    auto closes = [&](int64_t i) {
+      const double cur = run_weight(s, i, sd.sf, pos, ew, ne, ce);
+      return kd_sweep_closes(scale, sd.nrm, sd.stored_n, wsf, cur, rec_weight(i, sd.sf, pos, ew, ne)) != 0;
+    };
+    while (b < n && !closes(b)) b++;
+    while (b - 1 > s && closes(b - 1)) b--;
+    return b;
+  };
   // Clusters spanning three or more checkpoint tiles are summed by
   // k_long_sum (one block each); the rest by k_cluster_mean.
-  int64_t c = sweep_segment(X + sd.x_off, n, pc, esw + sd.es_off, cs + sd.x_off, replays,
-                            [&](int64_t s) { return chain_next(scale, pc, s, n, sd.stored_n, sd.nrm); },
+  int64_t c = sweep_segment(X + sd.x_off, n, pc, esw + sd.es_off, cs + sd.x_off, replays, next,
                             [&](int64_t a, int64_t b, int64_t ci) {
+                              if (!closed_exact)
+                                wsf = wsf + run_weight(a, b, sd.sf, pos, ew, ne, pcur_seek(pc, a));
                               if (b / kTile - a / kTile >= 2) {
                                 unsigned long long slot = atomicAdd(long_cnt, 1ull);
                                 long_list[2 * slot] = si;
@@ -969,24 +1053,6 @@ __device__ inline dd direct_sum(const double* x, PCur& pc, const double* esw_seg
   return acc;
 }
 
-// First E record at or after merged index j (binary search over pos).
-__device__ inline int64_t pcur_seek(const PCur& pc, int64_t j) {
-  int64_t lo = 0, hi = pc.ne;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (pc.pos[mid] < j) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// A positive normal power of two: its multiples by integers below 2^53 are
-// exact, so a running sum of n such weights is exactly n * w.
-__device__ inline bool pow2_weight(double w) {
-  const uint64_t b = kd_d2bits(w);
-  return (b & 0x000fffffffffffffull) == 0 && ((b >> 52) & 0x7ff) != 0 && ((b >> 52) & 0x7ff) != 0x7ff;
-}
-
 // Clusters up to this many records replay the running centroid of the CPU
 // sweep record by record (bit-identical means); longer ones use an exact
 // double-double sum (means within an ulp or two).
@@ -1061,7 +1127,7 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
   __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
   __shared__ uint16_t nxt[CAP];
   __shared__ int64_t s_C;
-  __shared__ int s_nbig;
+  __shared__ int s_nbig, s_frac, s_bad;
   __shared__ int s_big[kBigMax];
   __shared__ double s_bigw[kBigMax];
   __shared__ int s_bigc[kBigMax + 1];
@@ -1189,11 +1255,99 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
         st = (v & 0x8000) ? tie_extend(x, n, st, b, pc, ew, replays) : b;
       }
       res[si].replays = replays;
-      ccount[si] = C;
       s_C = C;
       s_nbig = 0;
+      s_frac = 0;
     }
     __syncthreads();
+    // The closed-form prefix is exact when every weight is an integer (or a
+    // power of two with no prior records): the running sums of the CPU
+    // sweep are then the same numbers.  Otherwise each boundary is checked
+    // in the CPU's summation order (kd_exact.h): every cluster's running
+    // weight (and the one before its last record), the finished-cluster
+    // sums in order, then per cluster the last absorbed candidate must not
+    // close and the first one after it must.  A disagreement (rare: mostly
+    // the last candidate, where q_right is about 1) restarts the sweep at
+    // that cluster, record by record as the CPU runs it.
+    for (int i = threadIdx.x; i < ne; i += BLOCK)
+      if (ew[i] != floor(ew[i])) s_frac = 1;
+    __syncthreads();
+    const bool closed_exact = (ne == 0 && pow2_weight(sd.sf)) ||
+                              (sd.sf == 1.0 && s_frac == 0 && sd.stored_n < 0x1p53);
+    if (!closed_exact && n > 1) {
+      // kv is dead after the walk: T0[c] takes each cluster's running
+      // weight, then (lane 0, in order) the sum of the clusters before it.
+      double* T0 = s;
+      const int64_t C0 = s_C;
+      for (int64_t c = threadIdx.x; c < C0; c += BLOCK) {
+        const int64_t a = nxt[c];
+        const int64_t b = (c + 1 < C0) ? (int64_t)nxt[c + 1] : n;
+        const int64_t ce = ne > 0 ? pcur_seek(PCur{pos, sw, ne, sd.sf, 0}, a) : 0;
+        T0[c] = run_weight(a, b, sd.sf, pos, ew, ne, ce);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int64_t c = 0; c < C0; c++) {
+          const double w = T0[c];
+          T0[c] = acc;
+          acc = acc + w;
+        }
+        s_bad = (int)C0;
+      }
+      __syncthreads();
+      for (int64_t c = threadIdx.x; c < C0; c += BLOCK) {
+        const int64_t a = nxt[c];
+        const int64_t b = (c + 1 < C0) ? (int64_t)nxt[c + 1] : n;
+        const int64_t ce = ne > 0 ? pcur_seek(PCur{pos, sw, ne, sd.sf, 0}, a) : 0;
+        // Running weight before the last record, and with it (one more add,
+        // as the sweep takes it) the cluster's weight.
+        double W = rec_weight(a, sd.sf, pos, ew, ne);
+        bool bad = false;
+        if (b - a >= 2) {
+          const double wl = rec_weight(b - 1, sd.sf, pos, ew, ne);
+          const double Wp = run_weight(a, b - 1, sd.sf, pos, ew, ne, ce);
+          bad = kd_sweep_closes(scale, sd.nrm, sd.stored_n, T0[c], Wp, wl);
+          W = Wp + wl;
+        }
+        if (b < n)
+          bad = bad || !kd_sweep_closes(scale, sd.nrm, sd.stored_n, T0[c], W,
+                                        rec_weight(b, sd.sf, pos, ew, ne));
+        if (bad) atomicMin(&s_bad, (int)c);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0 && s_bad < (int)C0) {
+        // The CPU sweep (tdigest.mojo:419-435) from the first cluster in
+        // question: tie rule, running mean and weight, boundary test.
+        int64_t c = s_bad;
+        int64_t st = nxt[c];
+        double wsf = T0[c];
+        int64_t ce = ne > 0 ? pcur_seek(PCur{pos, sw, ne, sd.sf, 0}, st) : 0;
+        double cur_w = sd.sf;
+        if (ce < ne && pos[ce] == st) cur_w = ew[ce++];
+        double cur_m = x[st];
+        nxt[c++] = (uint16_t)st;
+        for (int64_t i = st + 1; i < n; i++) {
+          double wi = sd.sf;
+          if (ce < ne && pos[ce] == i) wi = ew[ce++];
+          const double xi = x[i];
+          if (xi == cur_m || !kd_sweep_closes(scale, sd.nrm, sd.stored_n, wsf, cur_w, wi)) {
+            const double nw = cur_w + wi;
+            cur_m += (xi - cur_m) * (wi / nw);
+            cur_w = nw;
+          } else {
+            wsf = wsf + cur_w;
+            nxt[c++] = (uint16_t)i;
+            cur_w = wi;
+            cur_m = xi;
+          }
+        }
+        s_C = c;
+        res[si].replays += 1;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) ccount[si] = s_C;
     const int64_t C = s_C;
     const uint16_t* cs = nxt;
     double* T = s;  // kv is no longer needed: per-record scratch of the means
@@ -1429,7 +1583,11 @@ __global__ __launch_bounds__(kBlock) void k_long_sum(const SegDesc* segs, const 
       double mean;
       if (!equal_run_mean(x, a, b, &mean)) mean = dd_div(dd{rh[0], rl[0]}, w);
       cm[sd.x_off + c] = mean;
-      cw[sd.x_off + c] = w;
+      // The stored weight is the cluster's running weight, summed in record
+      // order as the CPU sums it (the exact sum above is for the mean).
+      const int64_t* pos = pose + sd.es_off;
+      const int64_t ce = pcur_seek(PCur{pos, se, sd.e_cnt, sd.sf, 0}, a);
+      cw[sd.x_off + c] = run_weight(a, b, sd.sf, pos, esw + sd.es_off, sd.e_cnt, ce);
     }
     __syncthreads();
   }
@@ -1914,9 +2072,9 @@ constexpr int kChainLds = 512;  // prior records staged per segment
 
 __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const SegDesc* segs,
                                                                const SelSeg* sel, int nsel,
-                                                               const double* se,
+                                                               const double* se, const double* esw,
                                                                const int64_t* pose, SelState* st,
-                                                               SelBound* bnd) {
+                                                               SelBound* bnd, double* csl) {
   __shared__ int64_t lpos[kChainWaves][kChainLds];
   __shared__ double lse[kChainWaves][kChainLds + 1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2136,6 +2294,86 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     s = sp;
     ks = ksp;
     while (cur < ne && pos[cur] < s) cur++;
+  }
+  // Unless the closed-form prefix is exact (integer weights), every boundary
+  // is checked in the CPU's summation order (kd_exact.h): the running weight
+  // of each cluster, the sums of the clusters before each (in order), then
+  // the last absorbed candidate must not close and the first one after it
+  // must.  From the first cluster in question on (rare: mostly the last
+  // candidate, where q_right is about 1), the chain is walked again with the
+  // exact test, each boundary found from the closed-form estimate.  The
+  // cluster slices (written by k_sel_plan afterwards) hold the scratch.
+  const double* ew = esw + sd.es_off;
+  bool closed_exact = ne == 0 ? pow2_weight(sf) : (sf == 1.0 && N < 0x1p53);
+  if (closed_exact && ne > 0) {
+    bool frac = false;
+    for (int64_t i = lane; i < ne; i += 64) frac = frac || ew[i] != floor(ew[i]);
+    closed_exact = __ballot(frac) == 0ull;
+  }
+  if (!closed_exact) {
+    auto seek = [&](int64_t a) { return pcur_seek(PCur{pos, sep, ne, sf, 0}, a); };
+    double* wt = csl + 2 * g.c_off;
+    __threadfence_block();
+    for (int c = lane; c < C; c += 64) {
+      const int64_t a = bd[c].b, b = c + 1 < C ? bd[c + 1].b : n;
+      wt[2 * c] = run_weight(a, b, sf, pos, ew, ne, seek(a));
+    }
+    __threadfence_block();
+    if (lane == 0) {
+      double acc = 0.0;
+      for (int c = 0; c < C; c++) {
+        wt[2 * c + 1] = acc;
+        acc = acc + wt[2 * c];
+      }
+    }
+    __threadfence_block();
+    int first_bad = C;
+    for (int c0 = 0; c0 < C; c0 += 64) {
+      const int c = c0 + lane;
+      bool bad = false;
+      if (c < C) {
+        const int64_t a = bd[c].b, b = c + 1 < C ? bd[c + 1].b : n;
+        const double wsf = wt[2 * c + 1];
+        double W = rec_weight(a, sf, pos, ew, ne);
+        if (b - a >= 2) {
+          const double wl = rec_weight(b - 1, sf, pos, ew, ne);
+          const double Wp = run_weight(a, b - 1, sf, pos, ew, ne, seek(a));
+          bad = kd_sweep_closes(scale, nrm, N, wsf, Wp, wl) != 0;
+          W = Wp + wl;
+        }
+        if (b < n) bad = bad || !kd_sweep_closes(scale, nrm, N, wsf, W, rec_weight(b, sf, pos, ew, ne));
+      }
+      const uint64_t bm = __ballot(bad);
+      if (bm != 0ull) {
+        first_bad = c0 + __ffsll((long long)bm) - 1;
+        break;
+      }
+    }
+    if (first_bad < C) {
+      int64_t s2 = bd[first_bad].b;
+      double wsf = wt[2 * first_bad + 1];
+      C = first_bad;
+      for (;;) {
+        if (!record(s2)) {
+          if (lane == 0) st[j].fail = 1;
+          return;
+        }
+        if (s2 >= n - 1) break;
+        cur = seek(s2);
+        double kn;
+        int64_t b = exact_next(s2, kv(s2), &kn);
+        const int64_t ce = seek(s2);
+        auto closes = [&](int64_t i) {
+          const double w0 = run_weight(s2, i, sf, pos, ew, ne, ce);
+          return kd_sweep_closes(scale, nrm, N, wsf, w0, rec_weight(i, sf, pos, ew, ne)) != 0;
+        };
+        while (b < n && !closes(b)) b++;
+        while (b - 1 > s2 && closes(b - 1)) b--;
+        if (b >= n) break;
+        wsf = wsf + run_weight(s2, b, sf, pos, ew, ne, ce);
+        s2 = b;
+      }
+    }
   }
   if (lane == 0) st[j].ncl = C;
 }
@@ -2932,6 +3170,8 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
     }
     dd wsum = dd_add(two_prod(sd.sf, (double)nV), ewt);
     double w = wsum.hi + wsum.lo;
+    // Stored weight: the running weight in record order, as the CPU sums it.
+    const double wrun = run_weight(ba.b, bb, sd.sf, pos, ew, ne, ea);
     dd sv = two_sum(csl[2 * (g.c_off + c)], csl[2 * (g.c_off + c) + 1]);
     if (nV > 0) {
       int bf = sel_bucket_of_rank(B, S, vra), bl = sel_bucket_of_rank(B, S, vrb - 1);
@@ -2963,7 +3203,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
     if (xf == xl) mean = (xf == 0.0 && bb - ba.b >= 2) ? 0.0 : xf;
     else mean = dd_div(tot, w);
     cm[sd.x_off + c] = mean;
-    cw[sd.x_off + c] = w;
+    cw[sd.x_off + c] = wrun;
   }
   if (threadIdx.x == 0) {
     ccount[g.si] = C;
@@ -3604,7 +3844,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
                                                      dst);
       launched("k_sel_ranks");
       k_sel_chain<<<(nsel + kChainWaves - 1) / kChainWaves, 64 * kChainWaves, 0, stream_>>>(
-          scale, dsegs, dsel, nsel, se, pose, dst, dbnd);
+          scale, dsegs, dsel, nsel, se, esw, pose, dst, dbnd, dcsl);
       launched("k_sel_chain");
       k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, pose, dst,
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);

@@ -701,3 +701,94 @@ int ko_add_at_many(ko_digest *d, const double *values, const double *times,
   }
   return KO_OK;
 }
+
+/* ------------------------------------------------------- threaded replays */
+typedef struct {
+  ko_digest **ds;
+  const int64_t *off;
+  const double *v, *t, *w;
+  int64_t d0, d1;
+  int rc;
+} at_job;
+
+static void *at_worker(void *arg) {
+  at_job *j = (at_job *)arg;
+  for (int64_t d = j->d0; d < j->d1 && !j->rc; d++) {
+    int64_t a = j->off[d], b = j->off[d + 1];
+    j->rc = ko_add_at_many(j->ds[d], j->v + a, j->t + a, j->w + a, b - a);
+  }
+  return NULL;
+}
+
+/* Digest d gets add_at of samples [off[d], off[d+1]) in order
+ * (tdigest.mojo:163-175 per sample); digests split over `threads` host
+ * threads by sample count. */
+int ko_batch_add_at(ko_digest **ds, int64_t nd, const int64_t *off, const double *v,
+                    const double *t, const double *w, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  at_job jobs[256];
+  pthread_t th[256];
+  int64_t d = 0, n = off[nd];
+  for (int i = 0; i < threads; i++) {
+    int64_t target = (int64_t)((double)n * (i + 1) / threads);
+    jobs[i] = (at_job){ds, off, v, t, w, d, d, 0};
+    while (d < nd && (off[d + 1] <= target || i == threads - 1)) d++;
+    jobs[i].d1 = d;
+  }
+  for (int i = 0; i < threads; i++) pthread_create(&th[i], NULL, at_worker, &jobs[i]);
+  int rc = 0;
+  for (int i = 0; i < threads; i++) {
+    pthread_join(th[i], NULL);
+    if (jobs[i].rc && !rc) rc = jobs[i].rc;
+  }
+  return rc;
+}
+
+typedef struct {
+  ko_digest **ds;
+  int64_t nreq;
+  const int64_t *keys, *off;
+  const double *data;
+  int tid, threads;
+  int rc;
+} rp_job;
+
+static void *rp_worker(void *arg) {
+  rp_job *j = (rp_job *)arg;
+  for (int64_t r = 0; r < j->nreq && !j->rc; r++) {
+    if (j->keys[r] % j->threads != j->tid) continue;
+    int64_t a = j->off[r], b = j->off[r + 1];
+    j->rc = ko_add_batch_b(j->ds[j->keys[r]], j->data + a, b - a);
+  }
+  return NULL;
+}
+
+/* Ingest requests in arrival order, each one B-semantics batch merge of its
+ * key (tdigest.mojo:223-315): request r adds data[off[r], off[r+1]) to
+ * ds[keys[r]].  Keys are split over `threads` host threads (key mod
+ * threads), so each key sees its requests in order. */
+int ko_replay_ingests(ko_digest **ds, int64_t nreq, const int64_t *keys, const int64_t *off,
+                      const double *data, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  rp_job jobs[256];
+  pthread_t th[256];
+  for (int i = 0; i < threads; i++) {
+    jobs[i] = (rp_job){ds, nreq, keys, off, data, i, threads, 0};
+    pthread_create(&th[i], NULL, rp_worker, &jobs[i]);
+  }
+  int rc = 0;
+  for (int i = 0; i < threads; i++) {
+    pthread_join(th[i], NULL);
+    if (jobs[i].rc && !rc) rc = jobs[i].rc;
+  }
+  return rc;
+}
+
+/* L sequential additions W = W + x, one rounding each (the running weight of
+ * centroid.mojo:34-36 over equal weights). */
+double ko_add_loop_sum(double W, double x, int64_t L) {
+  for (int64_t i = 0; i < L; i++) W = W + x;
+  return W;
+}

@@ -98,6 +98,18 @@ void ko_stable_sort(ko_centroid *cs, int64_t n);
 int ko_grouped_add_batch(ko_digest **digests, int64_t num_keys, const uint32_t *keys,
                          const double *values, int64_t n, int threads);
 
+/* add_at of samples [off[d], off[d+1]) into ds[d], digests over `threads`
+ * host threads. */
+int ko_batch_add_at(ko_digest **ds, int64_t nd, const int64_t *off, const double *v,
+                    const double *t, const double *w, int threads);
+/* Ingest request r (data[off[r], off[r+1])) as one B batch merge of
+ * ds[keys[r]], in arrival order per key; keys over `threads` host threads. */
+int ko_replay_ingests(ko_digest **ds, int64_t nreq, const int64_t *keys, const int64_t *off,
+                      const double *data, int threads);
+
+/* L sequential additions W = W + x (one rounding each). */
+double ko_add_loop_sum(double W, double x, int64_t L);
+
 /* Scalar math exposed for tests (the same kd_math.h the device uses). */
 double ko_k(int scale, double q, double nrm);
 double ko_normalizer(int scale, double compression, double n);

@@ -61,7 +61,10 @@ def lib(variant: str = "kd") -> C.CDLL:
         "ko_set_state": (None, [vp, _dp, _dp, i64, _dp, _dp, i64, _dp]),
         "ko_sweep": (i64, [i32, d, d, _dp, _dp, i64, _dp, _dp]),
         "ko_grouped_add_batch": (i32, [C.POINTER(vp), i64, C.POINTER(C.c_uint32), _dp, i64, i32]),
+        "ko_batch_add_at": (i32, [C.POINTER(vp), i64, C.POINTER(i64), _dp, _dp, _dp, i32]),
+        "ko_replay_ingests": (i32, [C.POINTER(vp), i64, C.POINTER(i64), C.POINTER(i64), _dp, i32]),
         "ko_k": (d, [i32, d, d]),
+        "ko_add_loop_sum": (d, [d, d, i64]),
         "ko_normalizer": (d, [i32, d, d]),
         "ko_log": (d, [d]),
         "ko_asin": (d, [d]),
@@ -250,3 +253,31 @@ def rank_error(sorted_data: np.ndarray, v: float, q: float) -> float:
 
 def kd_exp(x):
     return lib("kd").ko_exp(x)
+
+
+def batch_add_at(digests, off, values, times, weights, threads: int = 1):
+    """Digest d gets add_at of samples [off[d], off[d+1]) in order (tdigest.mojo:163-175)."""
+    L = digests[0].L
+    o = np.ascontiguousarray(off, np.int64)
+    v = np.ascontiguousarray(values, np.float64)
+    t = np.ascontiguousarray(times, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    arr = (C.c_void_p * len(digests))(*[d.h for d in digests])
+    _check(L.ko_batch_add_at(arr, len(digests), o.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(v),
+                             _ptr(t), _ptr(w), threads))
+
+
+def replay_ingests(digests, keys, off, data, threads: int = 1):
+    """Ingest request r = one B batch merge of data[off[r]:off[r+1]] into
+    digests[keys[r]], in arrival order per key (tdigest.mojo:223-315)."""
+    L = digests[0].L
+    k = np.ascontiguousarray(keys, np.int64)
+    o = np.ascontiguousarray(off, np.int64)
+    d = np.ascontiguousarray(data, np.float64)
+    arr = (C.c_void_p * len(digests))(*[x.h for x in digests])
+    _check(L.ko_replay_ingests(arr, k.size, k.ctypes.data_as(C.POINTER(C.c_int64)),
+                               o.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(d), threads))
+
+
+def add_loop_sum(W: float, x: float, L: int) -> float:
+    return lib("kd").ko_add_loop_sum(W, x, L)

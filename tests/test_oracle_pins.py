@@ -354,3 +354,30 @@ def test_vector_helpers_match_scalar_calls(oracle):
     for i in range(v.size):
         b.add_at(float(v[i]), float(t[i]), float(w[i]))
     assert a.to_bytes() == b.to_bytes()
+
+
+def test_threaded_replays_match_sequential(oracle):
+    # The threaded helpers give each digest its samples / requests in order,
+    # whatever the thread count.
+    rng = np.random.default_rng(31)
+    nd = 9
+    sizes = rng.integers(0, 3000, nd)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    v = rng.standard_t(3, off[-1])
+    t = np.concatenate([np.cumsum(rng.exponential(1e-3, s)) for s in sizes])
+    w = rng.uniform(0.1, 10, off[-1])
+    a = [oracle.Digest(2, 300.0, 1e-3) for _ in range(nd)]
+    b = [oracle.Digest(2, 300.0, 1e-3) for _ in range(nd)]
+    oracle.batch_add_at(a, off, v, t, w, threads=4)
+    for d in range(nd):
+        b[d].add_at_many(v[off[d]:off[d + 1]], t[off[d]:off[d + 1]], w[off[d]:off[d + 1]])
+    assert all(x.to_bytes() == y.to_bytes() for x, y in zip(a, b))
+    keys = rng.integers(0, 5, 40)
+    roff = np.concatenate([[0], np.cumsum(rng.integers(1, 500, 40))])
+    data = rng.standard_normal(roff[-1])
+    c = [oracle.Digest(1, 100.0, 1e-4) for _ in range(5)]
+    e = [oracle.Digest(1, 100.0, 1e-4) for _ in range(5)]
+    oracle.replay_ingests(c, keys, roff, data, threads=3)
+    for r in range(40):
+        e[keys[r]].add_batch_b(data[roff[r]:roff[r + 1]])
+    assert all(x.to_bytes() == y.to_bytes() for x, y in zip(c, e))
