@@ -1302,17 +1302,22 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
         const int64_t ce = ne > 0 ? pcur_seek(PCur{pos, sw, ne, sd.sf, 0}, a) : 0;
         // Running weight before the last record, and with it (one more add,
         // as the sweep takes it) the cluster's weight.
+        const double wsf = T0[c];
+        const double kl = kd_k(scale, wsf / sd.stored_n, sd.nrm);
+        // The sweep's test with the reference's operands (kd_sweep_closes),
+        // k(q_left) shared by both candidates.
+        auto closes = [&](double cur, double w) {
+          return !(kd_k(scale, (wsf + cur + w) / sd.stored_n, sd.nrm) - kl <= 1.0);
+        };
         double W = rec_weight(a, sd.sf, pos, ew, ne);
         bool bad = false;
         if (b - a >= 2) {
           const double wl = rec_weight(b - 1, sd.sf, pos, ew, ne);
           const double Wp = run_weight(a, b - 1, sd.sf, pos, ew, ne, ce);
-          bad = kd_sweep_closes(scale, sd.nrm, sd.stored_n, T0[c], Wp, wl);
+          bad = closes(Wp, wl);
           W = Wp + wl;
         }
-        if (b < n)
-          bad = bad || !kd_sweep_closes(scale, sd.nrm, sd.stored_n, T0[c], W,
-                                        rec_weight(b, sd.sf, pos, ew, ne));
+        if (b < n) bad = bad || !closes(W, rec_weight(b, sd.sf, pos, ew, ne));
         if (bad) atomicMin(&s_bad, (int)c);
       }
       __syncthreads();
@@ -2314,16 +2319,35 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     auto seek = [&](int64_t a) { return pcur_seek(PCur{pos, sep, ne, sf, 0}, a); };
     double* wt = csl + 2 * g.c_off;
     __threadfence_block();
+    // wt[2c]: the cluster's running weight (then the sum of the clusters
+    // before it); wt[2c + 1]: the running weight before its last record.
     for (int c = lane; c < C; c += 64) {
       const int64_t a = bd[c].b, b = c + 1 < C ? bd[c + 1].b : n;
-      wt[2 * c] = run_weight(a, b, sf, pos, ew, ne, seek(a));
+      double W = rec_weight(a, sf, pos, ew, ne), Wp = 0.0;
+      if (b - a >= 2) {
+        Wp = run_weight(a, b - 1, sf, pos, ew, ne, seek(a));
+        W = Wp + rec_weight(b - 1, sf, pos, ew, ne);
+      }
+      wt[2 * c] = W;
+      wt[2 * c + 1] = Wp;
     }
     __threadfence_block();
-    if (lane == 0) {
+    {
+      // The sums of the clusters before each, in order: each lane loads one
+      // cluster's weight, the wave adds them one lane at a time (the same
+      // sum in every lane), each lane keeps the sum before its own.
       double acc = 0.0;
-      for (int c = 0; c < C; c++) {
-        wt[2 * c + 1] = acc;
-        acc = acc + wt[2 * c];
+      for (int c0 = 0; c0 < C; c0 += 64) {
+        const int c = c0 + lane;
+        const double w = c < C ? wt[2 * c] : 0.0;
+        const int cnt = C - c0 < 64 ? C - c0 : 64;
+        double mine = 0.0;
+        for (int l = 0; l < cnt; l++) {
+          const double wl = __shfl(w, l, 64);
+          if (lane == l) mine = acc;
+          acc = acc + wl;
+        }
+        if (c < C) wt[2 * c] = mine;
       }
     }
     __threadfence_block();
@@ -2333,15 +2357,21 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       bool bad = false;
       if (c < C) {
         const int64_t a = bd[c].b, b = c + 1 < C ? bd[c + 1].b : n;
-        const double wsf = wt[2 * c + 1];
+        const double wsf = wt[2 * c];
+        const double kl = kd_k(scale, wsf / N, nrm);
+        // The sweep's test with the reference's operands (kd_sweep_closes),
+        // k(q_left) shared by both candidates.
+        auto closes = [&](double cur, double w) {
+          return !(kd_k(scale, (wsf + cur + w) / N, nrm) - kl <= 1.0);
+        };
         double W = rec_weight(a, sf, pos, ew, ne);
         if (b - a >= 2) {
           const double wl = rec_weight(b - 1, sf, pos, ew, ne);
-          const double Wp = run_weight(a, b - 1, sf, pos, ew, ne, seek(a));
-          bad = kd_sweep_closes(scale, nrm, N, wsf, Wp, wl) != 0;
+          const double Wp = wt[2 * c + 1];
+          bad = closes(Wp, wl);
           W = Wp + wl;
         }
-        if (b < n) bad = bad || !kd_sweep_closes(scale, nrm, N, wsf, W, rec_weight(b, sf, pos, ew, ne));
+        if (b < n) bad = bad || !closes(W, rec_weight(b, sf, pos, ew, ne));
       }
       const uint64_t bm = __ballot(bad);
       if (bm != 0ull) {
@@ -2351,7 +2381,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     }
     if (first_bad < C) {
       int64_t s2 = bd[first_bad].b;
-      double wsf = wt[2 * first_bad + 1];
+      double wsf = wt[2 * first_bad];
       C = first_bad;
       for (;;) {
         if (!record(s2)) {

@@ -44,7 +44,7 @@ KD_HD int kd_exponent(double x) { return (int)((kd_d2bits(x) >> 52) & 0x7ff) - 1
  * d is constant until a sum reaches 2^(e+1). */
 KD_HD double kd_repeat_add(double W, double x, int64_t L) {
   while (L > 0) {
-    if (!(W >= 2.0 * x) || L < 3 || !(W >= 0x1p-1000)) {
+    if (!(W >= 2.0 * x) || L < 3 || !(W >= 0x1p-960)) {
       W = W + x;
       L--;
       continue;
@@ -55,19 +55,24 @@ KD_HD double kd_repeat_add(double W, double x, int64_t L) {
     if (kd_exponent(W) != e) continue;
     const double d = (W + x) - W;
     if (d == 0.0) return W; /* x below half an ulp: every further add is a no-op */
-    const double u = kd_pow2(e - 52);
+    /* Units of u = 2^(e-52): scaling by powers of two is exact, and every
+     * integer below 2^53 (and every product m * D below it) is an exact
+     * double, so the step count is found in doubles. */
+    const double u = kd_pow2(e - 52), ui = kd_pow2(52 - e);
     /* Steps k = 0, 1, ... with W + k d + x < 2^(e+1), all in units of u:
-     * k * D <= A - floor(x / u) - 1 (integers below 2^53). */
-    const int64_t A = (int64_t)((kd_pow2(e + 1) - W) / u);
-    const double xf = x / u;
-    const int64_t q = (int64_t)xf;  /* floor: xf >= 0 */
-    const int64_t D = (int64_t)(d / u);
-    const int64_t lim = A - q - 1;
-    if (lim < 0 || D <= 0) continue;
-    int64_t m = lim / D + 1;
-    if (m > L) m = L;
-    W = W + (double)(m * D) * u;
-    L -= m;
+     * k * D <= A - floor(x / u) - 1. */
+    const double A = (kd_pow2(e + 1) - W) * ui;
+    const double D = d * ui;
+    const double lim = A - __builtin_floor(x * ui) - 1.0;
+    if (lim < 0.0 || !(D > 0.0)) continue;
+    /* floor(lim / D) from a rounded quotient, corrected to the exact one. */
+    double m = __builtin_floor(lim / D);
+    while (m > 0.0 && m * D > lim) m -= 1.0;
+    while ((m + 1.0) * D <= lim) m += 1.0;
+    m += 1.0;
+    if (m > (double)L) m = (double)L;
+    W = W + (m * D) * u;
+    L -= (int64_t)m;
   }
   return W;
 }
