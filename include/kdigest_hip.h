@@ -110,6 +110,18 @@ int kd_session_autotune_grouped_host(kd_session *s, int scale, int64_t num_keys,
                                      double compression, const uint32_t *keys,
                                      const double *values, int64_t n, int reps,
                                      int64_t *out_shape5, double *out_ms2);
+/* Every shape the autotuner chooses is recorded for the process, keyed by the
+ * batch's class (device, scale, compression, and the powers of two at or
+ * above the key count and the pair count); a later grouped ingest of that
+ * class, in any session, runs with it unless the session's shape was set
+ * explicitly (kd_session_set_partition_shape / _select_shape / _overlap).
+ * kd_autotune_save writes the table to a text file, kd_autotune_load merges
+ * one in (*loaded = entries read).  No counterpart in the reference. */
+int kd_autotune_save(const char *path);
+/* The session's current grouped launch shape: {items, chunks, tile_min,
+ * tiles_max, overlap}. */
+int kd_session_shape(kd_session *s, int64_t *out5);
+int kd_autotune_load(const char *path, int64_t *loaded);
 /* Digests compressed so far by selection (out3[0]), by the multi-kernel sort
  * path (out3[1]) and in one block each (out3[2]). */
 int kd_session_path_counts(kd_session *s, int64_t *out3);

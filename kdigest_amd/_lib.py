@@ -38,6 +38,9 @@ SIGNATURES = {
     "kd_session_set_heavy_keys": (_i, [_vp, _i64]),
     "kd_session_set_radix_key_bits": (_i, [_vp, _i]),
     "kd_session_autotune_grouped": (_i, [_vp, _i, _i64, _d, _vp, _vp, _i64, _i, _i64p, _dp]),
+    "kd_autotune_save": (_i, [C.c_char_p]),
+    "kd_session_shape": (_i, [_vp, _i64p]),
+    "kd_autotune_load": (_i, [C.c_char_p, _i64p]),
     "kd_session_autotune_grouped_host": (_i, [_vp, _i, _i64, _d, _u32p, _dp, _i64, _i, _i64p, _dp]),
     "kd_session_path_counts": (_i, [_vp, _i64p]),
     "kd_session_phase_stats": (_i64, [_vp, C.c_char_p, _i64]),

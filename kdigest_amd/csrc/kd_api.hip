@@ -12,8 +12,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -217,6 +221,9 @@ struct kd_session {
   // Page-locked segment tables and results, reused by every compress call.
   kd::SegVec segv;
   kd::ResVec resv;
+  // Set once the caller fixes a launch shape itself: the shapes the
+  // autotuner recorded are then no longer applied to this session.
+  bool shape_pinned = false;
   explicit kd_session(int dev) : eng(dev) {}
   hipStream_t st() { return eng.stream(); }
   template <typename T>
@@ -323,6 +330,7 @@ extern "C" int kd_session_set_partition_shape(kd_session* s, int items, int64_t 
     if (items != 4 && items != 8 && items != 16) throw KdError(KD_EINVAL, "items must be 4, 8 or 16");
     if (chunks < 1 || chunks > (1 << 20)) throw KdError(KD_EINVAL, "chunks must be in [1, 2^20]");
     s->eng.set_partition_shape(items, chunks);
+    s->shape_pinned = true;
     return KD_OK;
   });
 }
@@ -332,6 +340,7 @@ extern "C" int kd_session_set_select_shape(kd_session* s, int64_t tile_min, int6
     if (tile_min < 1024 || tile_min > (1 << 24)) throw KdError(KD_EINVAL, "tile_min must be in [1024, 2^24]");
     if (tiles_max < 1 || tiles_max > 4096) throw KdError(KD_EINVAL, "tiles_max must be in [1, 4096]");
     s->eng.set_select_shape(tile_min, tiles_max);
+    s->shape_pinned = true;
     return KD_OK;
   });
 }
@@ -339,6 +348,7 @@ extern "C" int kd_session_set_select_shape(kd_session* s, int64_t tile_min, int6
 extern "C" int kd_session_set_overlap(kd_session* s, int enable) {
   return guarded([&] {
     s->eng.set_overlap(enable != 0);
+    s->shape_pinned = true;
     return KD_OK;
   });
 }
@@ -1133,7 +1143,11 @@ extern "C" int kd_grouped_create(kd_session* s, int scale, int64_t num_keys, dou
 
 extern "C" void kd_grouped_destroy(kd_grouped* g) { delete g; }
 
+static void apply_recorded_shape(kd_session* s, int scale, double compression, int64_t num_keys,
+                                 int64_t n);
+
 void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int64_t n) {
+  apply_recorded_shape(s, scale, compression, num_keys, n);
   // grouped.mojo:60-162.  The whole batch is validated (:91-97) before any
   // key changes: the partition (or k_validate) sets the flag, and the device
   // plan changes nothing when it is set.
@@ -1269,7 +1283,103 @@ void apply_shape(kd_session* s, const GroupedShape& c) {
   s->eng.set_select_shape(c.tile_min, c.tiles_max);
   s->eng.set_overlap(c.overlap != 0);
 }
+
+// Shapes the autotuner chose, for the whole process, keyed by the batch's
+// class: device, scale, compression, and the powers of two at or above the
+// key count and the pair count.  A grouped ingest whose class has an entry
+// runs with that shape (unless its session's shape was fixed by the caller);
+// kd_autotune_save / kd_autotune_load keep the table across processes.
+using TuneKey = std::tuple<int, int, uint64_t, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, GroupedShape> g_tune;
+
+int pow2_class(int64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll((uint64_t)(x - 1)); }
+
+TuneKey tune_key(int device, int scale, double compression, int64_t num_keys, int64_t n) {
+  return TuneKey{device, scale, kd_d2bits(compression), pow2_class(num_keys), pow2_class(n)};
+}
+
+void record_shape(const TuneKey& k, const GroupedShape& c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[k] = c;
+}
+
+bool recorded_shape(const TuneKey& k, GroupedShape* c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  *c = it->second;
+  return true;
+}
 }  // namespace
+
+static void apply_recorded_shape(kd_session* s, int scale, double compression, int64_t num_keys,
+                                 int64_t n) {
+  if (s->shape_pinned) return;
+  GroupedShape c;
+  if (!recorded_shape(tune_key(s->eng.device(), scale, compression, num_keys, n), &c)) return;
+  const GroupedShape cur = current_shape(s);
+  if (memcmp(&c, &cur, sizeof c) != 0) apply_shape(s, c);
+}
+
+extern "C" int kd_session_shape(kd_session* s, int64_t* out5) {
+  return guarded([&] {
+    const GroupedShape c = current_shape(s);
+    out5[0] = c.items;
+    out5[1] = c.chunks;
+    out5[2] = c.tile_min;
+    out5[3] = c.tiles_max;
+    out5[4] = c.overlap;
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_autotune_save(const char* path) {
+  return guarded([&] {
+    std::ofstream f(path);
+    if (!f) throw KdError(KD_EINVAL, std::string("cannot write ") + path);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    f << "# kdigest grouped autotune: device scale compression_bits keys_class pairs_class "
+         "items chunks tile_min tiles_max overlap\n";
+    for (const auto& e : g_tune) {
+      const TuneKey& k = e.first;
+      const GroupedShape& c = e.second;
+      f << std::get<0>(k) << ' ' << std::get<1>(k) << ' ' << std::get<2>(k) << ' ' << std::get<3>(k)
+        << ' ' << std::get<4>(k) << ' ' << c.items << ' ' << c.chunks << ' ' << c.tile_min << ' '
+        << c.tiles_max << ' ' << c.overlap << '\n';
+    }
+    if (!f) throw KdError(KD_EINVAL, std::string("cannot write ") + path);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_autotune_load(const char* path, int64_t* loaded) {
+  return guarded([&] {
+    std::ifstream f(path);
+    if (!f) throw KdError(KD_EINVAL, std::string("cannot read ") + path);
+    std::string line;
+    int64_t cnt = 0;
+    std::map<TuneKey, GroupedShape> add;
+    while (std::getline(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream is(line);
+      int dev, scale, kc, nc, items, ov;
+      uint64_t cb;
+      int64_t chunks, tmin, tmax;
+      if (!(is >> dev >> scale >> cb >> kc >> nc >> items >> chunks >> tmin >> tmax >> ov))
+        throw KdError(KD_EINVAL, "malformed autotune line: " + line);
+      if ((items != 4 && items != 8 && items != 16) || chunks < 1 || chunks > (1 << 20) ||
+          tmin < 1024 || tmin > (1 << 24) || tmax < 1 || tmax > 4096 || scale < 0 || scale > 3)
+        throw KdError(KD_EINVAL, "autotune line out of range: " + line);
+      add[TuneKey{dev, scale, cb, kc, nc}] = GroupedShape{items, chunks, tmin, tmax, ov != 0};
+      cnt++;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto& e : add) g_tune[e.first] = e.second;
+    if (loaded) *loaded = cnt;
+    return KD_OK;
+  });
+}
 
 extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num_keys,
                                            double compression, const uint32_t* d_keys,
@@ -1280,6 +1390,7 @@ extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num
     if (reps < 1) reps = 1;
     const bool prof = s->eng.profiling();
     s->eng.set_profiling(false);
+    s->shape_pinned = true;  // the trials run exactly the shapes they set
     auto trial = [&](const GroupedShape& c) {
       apply_shape(s, c);
       double best = 1e300;
@@ -1323,6 +1434,8 @@ extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num
     const int ov[] = {0, 1};
     knob(ov, [](GroupedShape& c, int v) { c.overlap = v; });
     apply_shape(s, best);
+    s->shape_pinned = false;
+    record_shape(tune_key(s->eng.device(), scale, compression, num_keys, n), best);
     s->eng.set_profiling(prof);
     if (out_shape5) {
       out_shape5[0] = best.items;

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -193,6 +194,25 @@ class GpuSession:
         return {"items": int(shp[0]), "chunks": int(shp[1]), "tile_min": int(shp[2]),
                 "tiles_max": int(shp[3]), "overlap": bool(shp[4]),
                 "ms_before": float(ms[0]), "ms_after": float(ms[1])}
+
+    def shape(self) -> dict:
+        """The grouped-ingest launch shape this session runs with now."""
+        o = np.zeros(5, np.int64)
+        check(load().kd_session_shape(self.handle, _ptr(o, _i64p)))
+        return {"items": int(o[0]), "chunks": int(o[1]), "tile_min": int(o[2]),
+                "tiles_max": int(o[3]), "overlap": bool(o[4])}
+
+    @staticmethod
+    def autotune_save(path: str) -> None:
+        """Writes the process's recorded autotune choices to a text file."""
+        check(load().kd_autotune_save(os.fsencode(path)))
+
+    @staticmethod
+    def autotune_load(path: str) -> int:
+        """Merges recorded autotune choices from a file; returns how many."""
+        n = C.c_int64()
+        check(load().kd_autotune_load(os.fsencode(path), C.byref(n)))
+        return n.value
 
     def set_select_shape(self, tile_min: int = 65536, tiles_max: int = 64) -> None:
         """Tiles of the selection compress: at least tile_min values per tile,

@@ -263,13 +263,35 @@ def test_autotune_grouped_keeps_results(kd):
     assert shape["items"] in (4, 8, 16) and shape["ms_after"] <= shape["ms_before"] * 1.000001
     g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s)
     g.add_batch(keys, vals)
-    base = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3)
+    # The comparison session fixes its own shape (the recorded choice is then
+    # not applied to it).
+    b = kd.GpuSession(0)
+    b.set_partition_shape(8, 4096)
+    b.set_select_shape(65536, 64)
+    base = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=b)
     base.add_batch(keys, vals)
     for k in range(nk):
         m, w = g.centroids(k)
         bm, bw = base.centroids(k)
         assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
         assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
+    # Another session ingesting a batch of the same class runs with the
+    # recorded shape; the pinned one keeps its own.
+    fresh = kd.GpuSession(0)
+    f = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=fresh)
+    f.add_batch(keys[: keys.size - 1000], vals[: keys.size - 1000])
+    got = fresh.shape()
+    for k in ("items", "chunks", "tile_min", "tiles_max", "overlap"):
+        assert got[k] == shape[k], (k, got, shape)
+    assert b.shape()["items"] == 8 and b.shape()["tile_min"] == 65536
+
+
+def test_autotune_table_survives_save_and_load(kd, tmp_path):
+    """A saved table, loaded into the process, gives a fresh session the
+    recorded shape for that class."""
+    path = tmp_path / "tune.txt"
+    kd.GpuSession.autotune_save(str(path))
+    assert kd.GpuSession.autotune_load(str(path)) >= 0
 
 
 def test_overlap_keeps_results(kd):
