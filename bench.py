@@ -521,11 +521,14 @@ def _serve_requests(rng, nk, rate, seconds, cdf, perm, pool):
     return arr, keys, kinds, sizes, off, data
 
 
-def _serve_run(srv, reqs, watch=None):
-    """Submits each request once its arrival time has passed and steps the
-    server; returns per-request latency (completion minus arrival, s), the
-    wall time and, for the requests on keys in `watch`, their outcomes
-    (query values, "empty", "enomem" or None for a finished ingest)."""
+def _serve_run(srv, reqs, watch=None, workers=True):
+    """Submits each request once its arrival time has passed; returns
+    per-request latency (completion minus arrival, s), the wall time and, for
+    the requests on keys in `watch`, their outcomes (query values, "empty",
+    "enomem" or None for a finished ingest).  With `workers` the server's
+    worker threads admit and run requests as they are submitted (this thread
+    only submits and collects completions); otherwise this thread steps the
+    server whenever requests are pending."""
     import kdigest_amd as kd
 
     arr, keys, kinds, sizes, off, data = reqs
@@ -535,6 +538,8 @@ def _serve_run(srv, reqs, watch=None):
     first_id = None
     i = 0
     ndone = 0
+    if workers:
+        srv.start(4)
     t0 = time.perf_counter()
     c0 = srv.clock()
     while ndone < nreq:
@@ -545,12 +550,18 @@ def _serve_run(srv, reqs, watch=None):
             if first_id is None:
                 first_id = int(ids[0])
             i = j
-        if srv.pending == 0:
-            if i < nreq:
-                time.sleep(max(0.0, min(arr[i] - now, 1e-3)))
-            continue
-        srv.step(4)
-        cids, ct = srv.take_completed()
+        if workers:
+            cids, ct = srv.take_completed()
+            if cids.size == 0:
+                nxt = arr[i] - now if i < nreq else 1e-4
+                time.sleep(max(0.0, min(nxt, 1e-4)))
+        else:
+            if srv.pending == 0:
+                if i < nreq:
+                    time.sleep(max(0.0, min(arr[i] - now, 1e-3)))
+                continue
+            srv.step(4)
+            cids, ct = srv.take_completed()
         for rid, tc in zip(cids.tolist(), ct.tolist()):
             r = rid - first_id
             lat[r] = (tc - c0) - arr[r]
@@ -563,20 +574,25 @@ def _serve_run(srv, reqs, watch=None):
             else:
                 srv.discard(rid)
         ndone += len(cids)
-    return lat, time.perf_counter() - t0, kept
+    wall = time.perf_counter() - t0
+    if workers:
+        srv.stop()
+    return lat, wall, kept
 
 
 def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_seconds=0.5,
-            with_parity=True):
+            with_parity=True, workers=True):
     """Open-loop serving: Poisson arrivals at `rate` req/s for `seconds`,
     keys Zipf(1.05) over 1M device-resident digests (ScaleK1, compression
     100, half-life 1e6 samples); half the requests ingest log-uniform 1..65536
     N(0,1) values shifted per key, a quarter ask quantiles and a quarter cdf
     values (log-uniform 1..4096 per request).  Requests are submitted when
     their arrival time has passed; latency = completion (server clock, when
-    the step that finished the request ends) minus scheduled arrival.  A
-    warm-up run of the same mix (its own draws, `warm_seconds` long) goes
-    first and is not measured."""
+    the worker batch or step that finished the request ends) minus scheduled
+    arrival.  With `workers` the server's two worker threads serve requests
+    as they arrive (queries beside ingest rounds); otherwise this thread
+    steps it.  A warm-up run of the same mix (its own draws, `warm_seconds`
+    long) goes first and is not measured."""
     rng = np.random.default_rng(seed)
     ranks = np.arange(1, nk + 1, dtype=np.float64)
     cdf = np.cumsum(ranks ** -1.05)
@@ -597,9 +613,9 @@ def serving(kd, session, seconds=2.0, rate=20000.0, nk=1_000_000, seed=5, warm_s
                                                                 replace=False)]))
     if not with_parity:
         watch = None
-    _, _, warm_kept = _serve_run(srv, warm, watch)
+    _, _, warm_kept = _serve_run(srv, warm, watch, workers)
     s0 = srv.stats()
-    lat, wall, kept = _serve_run(srv, reqs, watch)
+    lat, wall, kept = _serve_run(srv, reqs, watch, workers)
     nreq = arr.size
     lat *= 1e3
     st = srv.stats()
@@ -1095,8 +1111,17 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serving(kd, session, seconds=args.serve_seconds, with_parity=not args.no_parity)
-        print(f"[bench] serving p50 {serve['p50_ms']:.2f} ms p99 {serve['p99_ms']:.2f} ms",
+        print(f"[bench] serving (workers) p50 {serve['p50_ms']:.2f} ms p99 {serve['p99_ms']:.2f} ms",
               file=sys.stderr, flush=True)
+        # The same arrivals served by stepping the server from this thread
+        # (one continuous batch at a time), for comparison.
+        step_mode = serving(kd, session, seconds=args.serve_seconds, with_parity=False,
+                            workers=False)
+        serve["step_mode"] = {k: step_mode[k] for k in
+                              ("requests_per_s", "p50_ms", "p99_ms", "max_ms", "ingest_p99_ms",
+                               "query_p99_ms", "admission")}
+        print(f"[bench] serving (step) p50 {step_mode['p50_ms']:.2f} ms "
+              f"p99 {step_mode['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
 
     line = {
         "metric": "ingest samples/s (% HBM peak) single + 100k-key grouped; p99 query latency",

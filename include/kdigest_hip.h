@@ -361,6 +361,23 @@ int kd_server_clock(kd_server *v, double *now);
 /* Admits and runs one continuous batch: all admitted queries in one launch,
  * then up to max_rounds rounds of ingests (one per key per round). */
 int kd_server_step(kd_server *v, int max_rounds, int64_t *completed);
+/* Worker mode: two host threads serve requests as they are submitted, so a
+ * request that arrives while others are on the device is admitted without
+ * waiting for them.  Queries whose key has no earlier ingest outstanding run
+ * in batched launches on a stream of their own, beside the ingest rounds
+ * (up to max_rounds per key per batch) on the session's stream; each key
+ * still sees its requests in submission order, so every result equals the
+ * one kd_server_step gives.  Completions are read with
+ * kd_server_take_completed / kd_server_result as before; kd_server_step,
+ * kd_server_key_state and kd_server_key_centroids fail (KD_EINVAL) while the
+ * workers run, and the session must not be used for other work meanwhile.
+ * kd_server_stop waits until every submitted request completed, then joins
+ * the workers (a device error in a worker fails its requests with
+ * KD_EDEVICE and is returned by kd_server_stop).  kd_server_destroy stops
+ * them too.  No counterpart in the reference, whose digests are
+ * single-threaded objects (the serving loop is the caller's). */
+int kd_server_start(kd_server *v, int max_rounds);
+int kd_server_stop(kd_server *v);
 /* *done = 1 when the request completed; query results are copied to out
  * (cap entries) and the request is then forgotten.  A query that found its
  * key empty completes with KD_EEMPTY (tdigest.mojo:485-487). */

@@ -114,6 +114,8 @@ SIGNATURES = {
     "kd_server_take_completed": (_i, [_vp, _i64p, _dp, _i64, _i64p]),
     "kd_server_clock": (_i, [_vp, _dp]),
     "kd_server_step": (_i, [_vp, _i, _i64p]),
+    "kd_server_start": (_i, [_vp, _i]),
+    "kd_server_stop": (_i, [_vp]),
     "kd_server_result": (_i, [_vp, _i64, _dp, _i64, C.POINTER(_i)]),
     "kd_server_pending": (_i, [_vp, _i64p]),
     "kd_server_stats": (_i, [_vp, _i64p]),

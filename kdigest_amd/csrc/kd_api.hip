@@ -2667,9 +2667,30 @@ extern "C" int kd_server_clock(kd_server* v, double* now) {
   return KD_OK;
 }
 
+// Calls that need the server's state at rest.
+static void check_not_running(kd_server* v) {
+  if (v->srv->running()) throw KdError(KD_EINVAL, "the server's workers are running; stop them first");
+}
+
 extern "C" int kd_server_step(kd_server* v, int max_rounds, int64_t* completed) {
   return guarded([&] {
+    check_not_running(v);
     *completed = v->srv->step(max_rounds);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_start(kd_server* v, int max_rounds) {
+  return guarded([&] {
+    if (max_rounds < 1) throw KdError(KD_EINVAL, "max_rounds must be at least 1");
+    v->srv->start(max_rounds);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_server_stop(kd_server* v) {
+  return guarded([&] {
+    v->srv->stop();
     return KD_OK;
   });
 }
@@ -2706,6 +2727,7 @@ extern "C" int kd_server_stats(kd_server* v, int64_t* out6) {
 extern "C" int kd_server_key_state(kd_server* v, int64_t key, double* out6) {
   return guarded([&] {
     if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    check_not_running(v);
     v->srv->key_state(key, out6);
     return KD_OK;
   });
@@ -2715,6 +2737,7 @@ extern "C" int kd_server_key_centroids(kd_server* v, int64_t key, double* means,
                                        int64_t cap, int64_t* count) {
   return guarded([&] {
     if (key < 0 || key >= v->srv->num_keys()) throw KdError(KD_EINVAL, "key out of range [0, num_keys)");
+    check_not_running(v);
     std::vector<double> m, w;
     v->srv->key_centroids(key, m, w);
     *count = (int64_t)m.size();

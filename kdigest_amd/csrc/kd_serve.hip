@@ -20,6 +20,8 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -147,13 +149,21 @@ Server::Server(Engine& eng, int scale, int64_t num_keys, double compression, dou
 
 int64_t Server::submit(int kind, int64_t key, const double* data, int64_t n) {
   Request r;
-  r.id = next_id_++;
   r.kind = kind;
   r.key = key;
   r.data.assign(data, data + n);
-  pending_.push_back(r.id);
-  reqs_.emplace(r.id, std::move(r));
-  return next_id_ - 1;
+  int64_t id;
+  bool workers;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    id = r.id = next_id_++;
+    Request& q = reqs_.emplace(id, std::move(r)).first->second;
+    workers = running_;
+    if (workers) enqueue(q);
+    else pending_.push_back(id);
+  }
+  if (workers) cv_.notify_all();
+  return id;
 }
 
 // Pushes the query state of `keys` (host scalars, centroid prefix sums) to
@@ -192,16 +202,16 @@ double Server::now() const {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
 }
 
-void Server::finish(int64_t id) {
-  Request& r = reqs_.at(id);
+void Server::finish(Request& r) {
   r.done = true;
   r.data.clear();
   r.data.shrink_to_fit();
-  completed_.push_back(id);
+  completed_.push_back(r.id);
   completed_t_.push_back(now());
 }
 
 int64_t Server::take_completed(int64_t* ids, double* t, int64_t cap) {
+  std::lock_guard<std::mutex> lk(mu_);
   int64_t n = std::min<int64_t>(cap, (int64_t)completed_.size());
   std::copy(completed_.begin(), completed_.begin() + n, ids);
   std::copy(completed_t_.begin(), completed_t_.begin() + n, t);
@@ -215,10 +225,11 @@ static double ms_since(std::chrono::steady_clock::time_point a) {
 }
 
 int64_t Server::step(int max_rounds) {
+  if (running()) throw std::runtime_error("the server's workers are running (kd_server_start)");
   const auto t_step = std::chrono::steady_clock::now();
   // ---- admission (arrival order, memory budget, per-key Q* I{0..R}) ----
-  std::vector<int64_t> queries;
-  std::vector<std::vector<int64_t>> rounds(std::max(max_rounds, 1));
+  std::vector<Request*> queries;
+  std::vector<std::vector<Request*>> rounds(std::max(max_rounds, 1));
   std::unordered_map<int64_t, int> key_ingests;  // admitted ingests per key this step
   std::unordered_map<int64_t, bool> key_blocked;  // a later request must wait for the next step
   int64_t bytes = 0;
@@ -235,7 +246,7 @@ int64_t Server::step(int max_rounds) {
       } else if (r.kind == 0) {
         int& c = key_ingests[r.key];
         if (c < (int)rounds.size()) {
-          rounds[c].push_back(id);
+          rounds[c].push_back(&r);
           c++;
           admit = true;
         } else {
@@ -245,10 +256,10 @@ int64_t Server::step(int max_rounds) {
         if (count_[r.key] == 0) {
           r.err = KD_EEMPTY_SERVE;  // tdigest.mojo:485-487 raises on an empty digest
           r.msg = "query on a key whose digest is empty";
-          finish(id);
+          finish(r);
           n_empty++;
         } else {
-          queries.push_back(id);
+          queries.push_back(&r);
         }
         admit = true;
       } else {
@@ -267,173 +278,190 @@ int64_t Server::step(int max_rounds) {
   admitted_bytes_ += bytes;
   max_step_bytes_ = std::max(max_step_bytes_, bytes);
   int64_t done = n_empty;
-
-  // ---- queries: one launch ----
   if (!queries.empty()) {
-    std::vector<int> kinds;
-    std::vector<int64_t> qk;
-    std::vector<double> xs;
-    for (int64_t id : queries) {
-      const Request& r = reqs_.at(id);
-      for (double x : r.data) {
-        kinds.push_back(r.kind);
-        qk.push_back(r.key);
-        xs.push_back(x);
-      }
-    }
-    int64_t nq = (int64_t)xs.size();
-    int* dkind = static_cast<int*>(qkind_.reserve(std::max<int64_t>(nq, 1) * sizeof(int)));
-    int64_t* dqk = static_cast<int64_t*>(qkey_.reserve(std::max<int64_t>(nq, 1) * sizeof(int64_t)));
-    double* dx = static_cast<double*>(qx_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
-    double* dout = static_cast<double*>(qout_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
-    hipStream_t st = eng_.stream();
-    check_hip(hipMemcpyAsync(dkind, kinds.data(), nq * sizeof(int), hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemcpyAsync(dqk, qk.data(), nq * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemcpyAsync(dx, xs.data(), nq * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
-    int ph = eng_.phase_begin("serve_query", 24.0 * (double)nq);
-    k_slot_query<<<nblocks(nq), 256, 0, st>>>(dkind, dqk, dx, nq, cap_,
-                                              static_cast<int64_t*>(dcount_.p),
-                                              static_cast<double*>(m_.p), static_cast<double*>(w_.p),
-                                              static_cast<double*>(S_.p),
-                                              static_cast<double*>(dstate_.p), dout);
-    launched("k_slot_query");
-    eng_.phase_end(ph);
-    std::vector<double> out(nq);
-    check_hip(hipMemcpyAsync(out.data(), dout, nq * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-    check_hip(hipStreamSynchronize(st), "sync");
-    int64_t pos = 0;
-    for (int64_t id : queries) {
-      Request& r = reqs_.at(id);
-      r.result.assign(out.begin() + pos, out.begin() + pos + (int64_t)r.data.size());
-      pos += (int64_t)r.data.size();
-      finish(id);
-      done++;
-    }
+    run_queries(queries, eng_.stream());
+    for (Request* r : queries) finish(*r);
+    done += (int64_t)queries.size();
   }
-
-  // ---- ingest rounds ----
   for (auto& round : rounds) {
     if (round.empty()) continue;
-    rounds_++;
-    std::sort(round.begin(), round.end(),
-              [&](int64_t a, int64_t b) { return reqs_.at(a).key < reqs_.at(b).key; });
-    SegVec& segs = segv_;
-    segs.clear();
-    std::vector<double> vals;
-    std::vector<int64_t> keys;
-    std::vector<char> had;
-    std::vector<double> sfs;
-    struct Saved {
-      double stored, logs, clock;
-    };
-    std::vector<Saved> saved;
-    std::vector<int64_t>& rkeys = h_rkeys_;
-    std::vector<double>& rfac = h_rfac_;
-    std::vector<int64_t> ridx;  // per segment: its index among the renormalized keys, or -1
-    rkeys.clear();
-    rfac.clear();
-    for (int64_t id : round) {
-      const Request& r = reqs_.at(id);
-      int64_t k = r.key, n = (int64_t)r.data.size();
-      saved.push_back(Saved{stored_[k], logs_[k], clock_[k]});
-      // _gpu_batch_prepare (tdigest.mojo:236-296).
-      double nclock = clock_[k] + (double)n;
-      if (lambda_ > 0.0 && nclock > clock_[k]) {
-        logs_[k] += lambda_ * (nclock - clock_[k]);
-        if (logs_[k] > 400.0) {  // _renormalize (:359-371)
-          double f = kd_exp(-logs_[k]);
-          ridx.resize(keys.size(), -1);
-          ridx.push_back((int64_t)rkeys.size());
-          rkeys.push_back(k);  // the slot's weights are rescaled on the device below
-          rfac.push_back(f);
-          stored_[k] *= f;
-          logs_[k] = 0.0;
-        }
+    run_ingest_round(round);
+    for (Request* r : round) finish(*r);
+    done += (int64_t)round.size();
+  }
+  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+  if (eng_.profiling()) eng_.host_phase("serve_step", ms_since(t_step));
+  return done;
+}
+
+// All the given queries in one launch on stream st, results into each
+// request (the caller completes them).  Uses the query staging buffers.
+void Server::run_queries(const std::vector<Request*>& queries, hipStream_t st) {
+  std::vector<int>& kinds = q_kinds_;
+  std::vector<int64_t>& qk = q_keys_;
+  std::vector<double>& xs = q_xs_;
+  kinds.clear();
+  qk.clear();
+  xs.clear();
+  for (const Request* r : queries) {
+    for (double x : r->data) {
+      kinds.push_back(r->kind);
+      qk.push_back(r->key);
+      xs.push_back(x);
+    }
+  }
+  int64_t nq = (int64_t)xs.size();
+  int* dkind = static_cast<int*>(qkind_.reserve(std::max<int64_t>(nq, 1) * sizeof(int)));
+  int64_t* dqk = static_cast<int64_t*>(qkey_.reserve(std::max<int64_t>(nq, 1) * sizeof(int64_t)));
+  double* dx = static_cast<double*>(qx_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
+  double* dout = static_cast<double*>(qout_.reserve(std::max<int64_t>(nq, 1) * sizeof(double)));
+  check_hip(hipMemcpyAsync(dkind, kinds.data(), nq * sizeof(int), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(dqk, qk.data(), nq * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(dx, xs.data(), nq * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+  const bool main = st == eng_.stream();
+  int ph = main ? eng_.phase_begin("serve_query", 24.0 * (double)nq) : -1;
+  k_slot_query<<<nblocks(nq), 256, 0, st>>>(dkind, dqk, dx, nq, cap_,
+                                            static_cast<int64_t*>(dcount_.p),
+                                            static_cast<double*>(m_.p), static_cast<double*>(w_.p),
+                                            static_cast<double*>(S_.p),
+                                            static_cast<double*>(dstate_.p), dout);
+  launched("k_slot_query");
+  if (main) eng_.phase_end(ph);
+  std::vector<double> out(nq);
+  check_hip(hipMemcpyAsync(out.data(), dout, nq * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+  check_hip(hipStreamSynchronize(st), "sync");
+  int64_t pos = 0;
+  for (Request* r : queries) {
+    r->result.assign(out.begin() + pos, out.begin() + pos + (int64_t)r->data.size());
+    pos += (int64_t)r->data.size();
+  }
+}
+
+// One round of ingests (at most one per key): each its own batch merge
+// (_gpu_batch_prepare, the compress, _gpu_batch_finish), all in one batched
+// compress.  Sets err / msg of a request whose merge does not fit its slot.
+// Ends with the round's device work complete.
+void Server::run_ingest_round(std::vector<Request*>& round) {
+  rounds_++;
+  std::sort(round.begin(), round.end(), [](const Request* a, const Request* b) { return a->key < b->key; });
+  SegVec& segs = segv_;
+  segs.clear();
+  std::vector<double> vals;
+  std::vector<int64_t> keys;
+  std::vector<char> had;
+  std::vector<double> sfs;
+  struct Saved {
+    double stored, logs, clock;
+  };
+  std::vector<Saved> saved;
+  std::vector<int64_t>& rkeys = h_rkeys_;
+  std::vector<double>& rfac = h_rfac_;
+  std::vector<int64_t> ridx;  // per segment: its index among the renormalized keys, or -1
+  rkeys.clear();
+  rfac.clear();
+  for (const Request* r : round) {
+    int64_t k = r->key, n = (int64_t)r->data.size();
+    saved.push_back(Saved{stored_[k], logs_[k], clock_[k]});
+    // _gpu_batch_prepare (tdigest.mojo:236-296).
+    double nclock = clock_[k] + (double)n;
+    if (lambda_ > 0.0 && nclock > clock_[k]) {
+      logs_[k] += lambda_ * (nclock - clock_[k]);
+      if (logs_[k] > 400.0) {  // _renormalize (:359-371)
+        double f = kd_exp(-logs_[k]);
+        ridx.resize(keys.size(), -1);
+        ridx.push_back((int64_t)rkeys.size());
+        rkeys.push_back(k);  // the slot's weights are rescaled on the device below
+        rfac.push_back(f);
+        stored_[k] *= f;
+        logs_[k] = 0.0;
       }
-      clock_[k] = nclock;
-      double sf = lambda_ > 0.0 ? kd_exp(logs_[k]) : 1.0;
-      had.push_back(count_[k] > 0);
-      stored_[k] += (double)n * sf;
-      double effn = lambda_ > 0.0 ? stored_[k] * kd_exp(-logs_[k]) : stored_[k];
-      SegDesc sd{};
-      sd.v_off = (int64_t)vals.size();
-      sd.v_cnt = n;
-      sd.e_off = k * cap_;
-      sd.e_cnt = count_[k];
-      sd.sf = sf;
-      sd.stored_n = stored_[k];
-      sd.nrm = kd_normalizer(scale_, compression_, effn);
-      sd.floor = lambda_ > 0.0 ? 1e-6 * kd_exp(logs_[k]) : 0.0;
-      segs.push_back(sd);
-      keys.push_back(k);
-      sfs.push_back(sf);
-      vals.insert(vals.end(), r.data.begin(), r.data.end());
     }
-    hipStream_t st = eng_.stream();
-    if (!rkeys.empty()) {
-      const int64_t nr = (int64_t)rkeys.size();
-      int64_t* drk = static_cast<int64_t*>(rkey_.reserve(nr * sizeof(int64_t)));
-      double* drf = static_cast<double*>(rfac_.reserve(nr * sizeof(double)));
-      check_hip(hipMemcpyAsync(drk, rkeys.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-      check_hip(hipMemcpyAsync(drf, rfac.data(), nr * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
-      double* dbk = static_cast<double*>(rbak_.reserve(nr * cap_ * sizeof(double)));
-      k_renorm_slots<<<(unsigned)nr, 64, 0, st>>>(drk, drf, nr, cap_, static_cast<int64_t*>(dcount_.p),
-                                                  static_cast<double*>(w_.p), dbk);
-      launched("k_renorm_slots");
+    clock_[k] = nclock;
+    double sf = lambda_ > 0.0 ? kd_exp(logs_[k]) : 1.0;
+    had.push_back(count_[k] > 0);
+    stored_[k] += (double)n * sf;
+    double effn = lambda_ > 0.0 ? stored_[k] * kd_exp(-logs_[k]) : stored_[k];
+    SegDesc sd{};
+    sd.v_off = (int64_t)vals.size();
+    sd.v_cnt = n;
+    sd.e_off = k * cap_;
+    sd.e_cnt = count_[k];
+    sd.sf = sf;
+    sd.stored_n = stored_[k];
+    sd.nrm = kd_normalizer(scale_, compression_, effn);
+    sd.floor = lambda_ > 0.0 ? 1e-6 * kd_exp(logs_[k]) : 0.0;
+    segs.push_back(sd);
+    keys.push_back(k);
+    sfs.push_back(sf);
+    vals.insert(vals.end(), r->data.begin(), r->data.end());
+  }
+  hipStream_t st = eng_.stream();
+  if (!rkeys.empty()) {
+    const int64_t nr = (int64_t)rkeys.size();
+    int64_t* drk = static_cast<int64_t*>(rkey_.reserve(nr * sizeof(int64_t)));
+    double* drf = static_cast<double*>(rfac_.reserve(nr * sizeof(double)));
+    check_hip(hipMemcpyAsync(drk, rkeys.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(drf, rfac.data(), nr * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    double* dbk = static_cast<double*>(rbak_.reserve(nr * cap_ * sizeof(double)));
+    k_renorm_slots<<<(unsigned)nr, 64, 0, st>>>(drk, drf, nr, cap_, static_cast<int64_t*>(dcount_.p),
+                                                static_cast<double*>(w_.p), dbk);
+    launched("k_renorm_slots");
+  }
+  ridx.resize(keys.size(), -1);
+  double* dv = static_cast<double*>(vbuf_.reserve(std::max<size_t>(vals.size(), 1) * sizeof(double)));
+  check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+  ResVec& res = resv_;
+  eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
+  // A merge that would not fit its key's slot fails that request alone:
+  // the key keeps its state (a renormalized slot gets its weights back)
+  // and nothing is installed for it.
+  std::vector<int64_t> ok, restore;
+  for (size_t i = 0; i < segs.size(); i++) {
+    if (res[i].count > cap_) {
+      int64_t k = keys[i];
+      stored_[k] = saved[i].stored;
+      logs_[k] = saved[i].logs;
+      clock_[k] = saved[i].clock;
+      if (ridx[i] >= 0) restore.push_back(ridx[i]);
+      Request* r = round[i];
+      r->err = KD_ENOMEM_SERVE;
+      r->msg = "key " + std::to_string(k) + " needs " + std::to_string(res[i].count) +
+               " centroids; the slot capacity is " + std::to_string(cap_);
+    } else {
+      ok.push_back((int64_t)i);
     }
-    ridx.resize(keys.size(), -1);
-    double* dv = static_cast<double*>(vbuf_.reserve(std::max<size_t>(vals.size(), 1) * sizeof(double)));
-    check_hip(hipMemcpyAsync(dv, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
-    ResVec& res = resv_;
-    eng_.compress(scale_, segs, dv, static_cast<double*>(m_.p), static_cast<double*>(w_.p), res);
-    // A merge that would not fit its key's slot fails that request alone:
-    // the key keeps its state (a renormalized slot gets its weights back)
-    // and nothing is installed for it.
-    std::vector<int64_t> ok, restore;
-    for (size_t i = 0; i < segs.size(); i++) {
-      if (res[i].count > cap_) {
-        int64_t k = keys[i];
-        stored_[k] = saved[i].stored;
-        logs_[k] = saved[i].logs;
-        clock_[k] = saved[i].clock;
-        if (ridx[i] >= 0) restore.push_back(ridx[i]);
-        Request& r = reqs_.at(round[i]);
-        r.err = KD_ENOMEM_SERVE;
-        r.msg = "key " + std::to_string(k) + " needs " + std::to_string(res[i].count) +
-                " centroids; the slot capacity is " + std::to_string(cap_);
-      } else {
-        ok.push_back((int64_t)i);
-      }
-    }
-    if (!restore.empty()) {
-      const int64_t nl = (int64_t)restore.size();
-      int64_t* dl = static_cast<int64_t*>(rlist_.reserve(nl * sizeof(int64_t)));
-      check_hip(hipMemcpyAsync(dl, restore.data(), nl * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
-      k_restore_slots<<<(unsigned)nl, 64, 0, st>>>(static_cast<int64_t*>(rkey_.p), dl, nl, cap_,
-                                                   static_cast<int64_t*>(dcount_.p),
-                                                   static_cast<double*>(rbak_.p), static_cast<double*>(w_.p));
-      launched("k_restore_slots");
-      check_hip(hipStreamSynchronize(st), "sync");  // `restore` is a local staging vector
-    }
-    int64_t ns = (int64_t)ok.size();
-    std::vector<int64_t>& meta = h_meta_;
-    meta.resize(3 * std::max<int64_t>(ns, 1));
-    for (int64_t q = 0; q < ns; q++) {
-      int64_t i = ok[q];
-      meta[q] = keys[i];
-      meta[ns + q] = segs[i].x_off;
-      meta[2 * ns + q] = res[i].count;
-    }
-    if (ns > 0) {
-      int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
-      check_hip(hipMemcpyAsync(dmeta, meta.data(), 3 * ns * sizeof(int64_t), hipMemcpyHostToDevice, st),
-                "H2D");
-      k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
-                                             eng_.cent_means(), eng_.cent_weights(),
-                                             static_cast<double*>(m_.p), static_cast<double*>(w_.p),
-                                             static_cast<int64_t*>(dcount_.p));
-      launched("k_install");
-    }
+  }
+  if (!restore.empty()) {
+    const int64_t nl = (int64_t)restore.size();
+    int64_t* dl = static_cast<int64_t*>(rlist_.reserve(nl * sizeof(int64_t)));
+    check_hip(hipMemcpyAsync(dl, restore.data(), nl * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    k_restore_slots<<<(unsigned)nl, 64, 0, st>>>(static_cast<int64_t*>(rkey_.p), dl, nl, cap_,
+                                                 static_cast<int64_t*>(dcount_.p),
+                                                 static_cast<double*>(rbak_.p), static_cast<double*>(w_.p));
+    launched("k_restore_slots");
+    check_hip(hipStreamSynchronize(st), "sync");  // `restore` is a local staging vector
+  }
+  int64_t ns = (int64_t)ok.size();
+  std::vector<int64_t>& meta = h_meta_;
+  meta.resize(3 * std::max<int64_t>(ns, 1));
+  for (int64_t q = 0; q < ns; q++) {
+    int64_t i = ok[q];
+    meta[q] = keys[i];
+    meta[ns + q] = segs[i].x_off;
+    meta[2 * ns + q] = res[i].count;
+  }
+  if (ns > 0) {
+    int64_t* dmeta = static_cast<int64_t*>(meta_.reserve(3 * ns * sizeof(int64_t)));
+    check_hip(hipMemcpyAsync(dmeta, meta.data(), 3 * ns * sizeof(int64_t), hipMemcpyHostToDevice, st),
+              "H2D");
+    k_install<<<(unsigned)ns, 64, 0, st>>>(dmeta, dmeta + ns, dmeta + 2 * ns, ns, cap_,
+                                           eng_.cent_means(), eng_.cent_weights(),
+                                           static_cast<double*>(m_.p), static_cast<double*>(w_.p),
+                                           static_cast<int64_t*>(dcount_.p));
+    launched("k_install");
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu_);  // count_ and the extremes are read by other threads
     for (int64_t i : ok) {
       int64_t k = keys[i];
       double m = (double)segs[i].v_cnt;
@@ -447,18 +475,227 @@ int64_t Server::step(int max_rounds) {
       stored_[k] = res[i].total;  // :304-315
       count_[k] = res[i].count;
     }
-    // Every key of the round gets its query state rewritten (a failed key's
-    // is unchanged, rewriting it keeps the code path single).
-    refresh_state(keys);
-    for (int64_t id : round) finish(id);
-    done += (int64_t)round.size();
   }
-  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
-  if (eng_.profiling()) eng_.host_phase("serve_step", ms_since(t_step));
-  return done;
+  // Every key of the round gets its query state rewritten (a failed key's
+  // is unchanged, rewriting it keeps the code path single).
+  refresh_state(keys);
+  check_hip(hipStreamSynchronize(st), "sync");
+}
+
+// ------------------------------------------------------------ workers
+// Two host threads serve the queue while the caller keeps submitting: the
+// query worker answers every query whose key has no earlier ingest still
+// outstanding, in batches, on a stream of its own; the ingest worker runs
+// ingest rounds (at most max_rounds per key per batch) on the engine's
+// stream for every ingest whose key has no earlier request outstanding.  A
+// request's per-key predecessors are counted when it is submitted, so each
+// key still sees its requests in arrival order (results equal those of the
+// synchronous step), while queries of other keys run beside the ingests.
+void Server::start(int max_rounds) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (running_) return;
+  check_hip(hipStreamCreateWithFlags(&qstream_, hipStreamNonBlocking), "hipStreamCreate");
+  max_rounds_ = std::max(max_rounds, 1);
+  // Requests queued before the start join the workers' queues in order
+  // (every request counted in key_seq_ before has completed or is among them).
+  key_seq_.assign(4 * nk_, 0);
+  for (int64_t id : pending_) enqueue(reqs_.at(id));
+  pending_.clear();
+  stop_ = false;
+  running_ = true;
+  worker_err_.clear();
+  qthr_ = std::thread([this] { query_loop(); });
+  ithr_ = std::thread([this] { ingest_loop(); });
+}
+
+Server::~Server() {
+  try {
+    stop();
+  } catch (...) {
+  }
+}
+
+void Server::stop() {
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (!running_) return;
+    idle_cv_.wait(lk, [&] { return qq_.empty() && iq_.empty() && busy_ == 0; });
+    stop_ = true;
+  }
+  cv_.notify_all();
+  qthr_.join();
+  ithr_.join();
+  std::lock_guard<std::mutex> lk(mu_);
+  running_ = false;
+  // Requests submitted while stop() was under way stay queued for step() or
+  // the next start, in submission order.
+  for (Request* r : qq_) pending_.push_back(r->id);
+  for (Request* r : iq_) pending_.push_back(r->id);
+  qq_.clear();
+  iq_.clear();
+  std::sort(pending_.begin(), pending_.end());
+  check_hip(hipStreamDestroy(qstream_), "hipStreamDestroy");
+  qstream_ = nullptr;
+  if (!worker_err_.empty()) {
+    std::string e;
+    e.swap(worker_err_);
+    throw std::runtime_error(e);
+  }
+}
+
+// Sequencing of a new request among its key's requests (mu_ held).
+void Server::enqueue(Request& r) {
+  int64_t* ks = key_seq_.data() + 4 * r.key;  // ingests submitted, done; queries submitted, done
+  r.need_ing = ks[0];
+  r.need_qry = ks[2];
+  if (r.kind == 0) {
+    ks[0]++;
+    iq_.push_back(&r);
+  } else {
+    ks[2]++;
+    qq_.push_back(&r);
+  }
+}
+
+// A worker's batch that threw: each request fails with the error (the
+// server's first such error is also raised by stop()).
+void Server::fail_batch(const std::vector<Request*>& batch, const char* what) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (worker_err_.empty()) worker_err_ = what;
+  for (Request* r : batch) {
+    r->err = KD_EDEVICE_SERVE;
+    r->msg = what;
+  }
+}
+
+void Server::query_loop() {
+  check_hip(hipSetDevice(eng_.device()), "hipSetDevice");
+  std::vector<Request*> batch;
+  for (;;) {
+    batch.clear();
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      for (;;) {
+        if (stop_) return;
+        int64_t bytes = 0;
+        for (auto it = qq_.begin(); it != qq_.end();) {
+          Request* r = *it;
+          const int64_t* ks = key_seq_.data() + 4 * r->key;
+          const int64_t need = (int64_t)r->data.size() * 16;
+          if (ks[1] >= r->need_ing && (bytes == 0 || bytes + need <= budget_)) {
+            it = qq_.erase(it);
+            if (count_[r->key] == 0) {
+              r->err = KD_EEMPTY_SERVE;  // tdigest.mojo:485-487 raises on an empty digest
+              r->msg = "query on a key whose digest is empty";
+              finish(*r);
+              key_seq_[4 * r->key + 3]++;
+              admitted_++;
+              continue;
+            }
+            batch.push_back(r);
+            bytes += need;
+          } else {
+            ++it;
+          }
+        }
+        if (!batch.empty()) {
+          busy_++;
+          admitted_ += (int64_t)batch.size();
+          break;
+        }
+        idle_cv_.notify_all();
+        cv_.wait(lk);
+      }
+    }
+    try {
+      run_queries(batch, qstream_);
+    } catch (const std::exception& e) {
+      fail_batch(batch, e.what());
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Request* r : batch) {
+        finish(*r);
+        key_seq_[4 * r->key + 3]++;
+      }
+      busy_--;
+    }
+    cv_.notify_all();
+    idle_cv_.notify_all();
+  }
+}
+
+void Server::ingest_loop() {
+  check_hip(hipSetDevice(eng_.device()), "hipSetDevice");
+  std::vector<std::vector<Request*>> rounds;
+  for (;;) {
+    rounds.assign(max_rounds_, {});
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      for (;;) {
+        if (stop_) return;
+        // Per key, the ingests taken this batch (they run in consecutive
+        // rounds): each must follow every earlier request of its key.
+        std::unordered_map<int64_t, int> taken;
+        int64_t bytes = 0;
+        int64_t n = 0;
+        for (auto it = iq_.begin(); it != iq_.end();) {
+          Request* r = *it;
+          const int64_t* ks = key_seq_.data() + 4 * r->key;
+          int& c = taken[r->key];
+          const int64_t need = (int64_t)r->data.size() * 8;
+          const bool ready = ks[1] + c >= r->need_ing && ks[3] >= r->need_qry;
+          if (ready && c < max_rounds_ && (bytes == 0 || bytes + need <= budget_)) {
+            it = iq_.erase(it);
+            rounds[c].push_back(r);
+            c++;
+            bytes += need;
+            n++;
+          } else {
+            c = max_rounds_;  // later ingests of this key wait for the next batch
+            ++it;
+          }
+        }
+        if (n > 0) {
+          busy_++;
+          steps_++;
+          admitted_ += n;
+          admitted_bytes_ += bytes;
+          last_admitted_bytes_ = bytes;
+          max_step_bytes_ = std::max(max_step_bytes_, bytes);
+          break;
+        }
+        idle_cv_.notify_all();
+        cv_.wait(lk);
+      }
+    }
+    for (auto& round : rounds) {
+      if (round.empty()) continue;
+      try {
+        run_ingest_round(round);
+      } catch (const std::exception& e) {
+        fail_batch(round, e.what());
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Request* r : round) {
+          finish(*r);
+          key_seq_[4 * r->key + 1]++;
+        }
+      }
+      cv_.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      busy_--;
+    }
+    cv_.notify_all();
+    idle_cv_.notify_all();
+  }
 }
 
 bool Server::result(int64_t id, std::vector<double>& out, int& err, std::string& msg, bool erase) {
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = reqs_.find(id);
   if (it == reqs_.end()) throw std::runtime_error("unknown request id");
   if (!it->second.done) return false;
@@ -469,7 +706,9 @@ bool Server::result(int64_t id, std::vector<double>& out, int& err, std::string&
   return true;
 }
 
-void Server::key_state(int64_t key, double* o) const {
+void Server::key_state(int64_t key, double* o) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (running_) throw std::runtime_error("the server's workers are running; stop them first");
   o[0] = stored_[key];
   o[1] = logs_[key];
   o[2] = clock_[key];
@@ -479,6 +718,7 @@ void Server::key_state(int64_t key, double* o) const {
 }
 
 void Server::key_centroids(int64_t key, std::vector<double>& m, std::vector<double>& w) {
+  if (running()) throw std::runtime_error("the server's workers are running; stop them first");
   m.resize(count_[key]);
   w.resize(count_[key]);
   check_hip(hipMemcpyAsync(m.data(), static_cast<double*>(m_.p) + key * cap_, m.size() * sizeof(double),

@@ -905,6 +905,18 @@ class DigestServer:
         check(load().kd_server_step(self._h, int(max_rounds), C.byref(done)))
         return done.value
 
+    def start(self, max_rounds: int = 4) -> None:
+        """Serve requests on two worker threads as they are submitted: queries
+        run beside ingest rounds, each key still sees its requests in
+        submission order (results equal those of step()).  step(), key_state()
+        and centroids() raise until stop()."""
+        check(load().kd_server_start(self._h, int(max_rounds)))
+
+    def stop(self) -> None:
+        """Waits until every submitted request completed, then ends the
+        workers; raises the first device error a worker met."""
+        check(load().kd_server_stop(self._h))
+
     def result(self, rid: int):
         """Query values (np.ndarray), an empty array for a finished ingest, or
         None while the request is pending.  A finished request is forgotten

@@ -261,3 +261,93 @@ def test_serve_renormalize_then_slot_overflow(kd, oracle):
     m, w = srv.centroids(0)
     rm, rw = r2.centroids()
     check_centroids(m, w, rm, rw, 5.0, False, "renormalized")
+
+
+def _run_workers(kd, reqs, nkeys, scale, delta, decay, budget, rounds, seed):
+    """Submits while the server's workers run: bursts of requests with short
+    pauses between them, so later requests arrive while earlier ones are on
+    the device; polls completions meanwhile."""
+    import time
+
+    srv = kd.DigestServer(nkeys, delta, decay, scale=scale, slot_capacity=2048, mem_budget=budget)
+    rng = np.random.default_rng(seed)
+    head = int(rng.integers(0, 8))  # a few queued before the start join the workers' queues
+    ids = [srv.submit(k, key, d) for k, key, d in reqs[:head]]
+    srv.start(rounds)
+    seen = 0
+    i = head
+    while i < len(reqs):
+        burst = int(rng.integers(1, 12))
+        for k, key, d in reqs[i:i + burst]:
+            ids.append(srv.submit(k, key, d))
+        i += burst
+        seen += srv.take_completed()[0].size
+        time.sleep(float(rng.uniform(0.0, 0.002)))
+    srv.stop()
+    seen += srv.take_completed()[0].size
+    assert seen == len(reqs) and srv.pending == 0
+    out = [_take(kd, srv, j) for j in ids]
+    assert all(o is not None for o in out)
+    return srv, out
+
+
+@pytest.mark.parametrize("scale,delta,lam", CASES)
+def test_serve_workers_equal_alone(kd, scale, delta, lam):
+    """Worker mode (queries on their own stream beside the ingest rounds,
+    requests admitted as they arrive) gives every request the result it gets
+    alone, bit for bit, and leaves every key in the same state."""
+    nkeys = 37
+    reqs = _workload(5 + scale, nkeys, 400, 30000)
+    decay = kd.Decay.rate(lam)
+    sc = [kd.ScaleK0, kd.ScaleK1, kd.ScaleK2, kd.ScaleK3][scale]
+    alone, ref = _run_alone(kd, reqs, nkeys, sc, delta, decay)
+    srv, got = _run_workers(kd, reqs, nkeys, sc, delta, decay, 2 << 20, 3, 40 + scale)
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert _same_bits(a, b), (i, reqs[i][0], reqs[i][1], a, b)
+    st = srv.stats()
+    assert st["admitted"] == len(reqs) and st["queued"] == 0
+    for key in range(nkeys):
+        assert srv.key_state(key) == alone.key_state(key), key
+        m, w = srv.centroids(key)
+        am, aw = alone.centroids(key)
+        assert _same_bits(m, am) and _same_bits(w, aw), key
+
+
+def test_serve_workers_guards_and_restart(kd):
+    srv = kd.DigestServer(4, 100.0, scale=kd.ScaleK1, slot_capacity=8)
+    rng = np.random.default_rng(9)
+    srv.start(2)
+    srv.start(2)  # already running: no effect
+    with pytest.raises(kd.KdigestError):
+        srv.step()
+    with pytest.raises(kd.KdigestError):
+        srv.key_state(0)
+    with pytest.raises(kd.KdigestError):
+        srv.centroids(0)
+    e = srv.submit_quantile(2, [0.5])        # empty key
+    small = srv.submit_ingest(1, [1.0, 2.0, 3.0])
+    big = srv.submit_ingest(0, rng.standard_normal(20000))  # overflows its 8-centroid slot
+    q = srv.submit_quantile(1, [0.5])
+    srv.stop()
+    srv.stop()  # not running: no effect
+    with pytest.raises(kd.KdigestError) as ex:
+        srv.result(e)
+    assert ex.value.code == -2
+    with pytest.raises(kd.KdigestError) as ex:
+        srv.result(big)
+    assert ex.value.code == -7
+    assert srv.result(small).size == 0 and srv.result(q)[0] == 2.0
+    assert srv.key_state(0)["stored_total"] == 0.0
+    # Queued before a start, served after it; then the synchronous step again.
+    a = srv.submit_ingest(2, [5.0, 6.0])
+    srv.start(1)
+    b = srv.submit_quantile(2, [0.0, 1.0])
+    srv.stop()
+    assert srv.result(a).size == 0 and srv.result(b).tolist() == [5.0, 6.0]
+    c = srv.submit_cdf(2, [5.5])
+    assert srv.step() == 1 and srv.result(c)[0] == 0.5
+    del srv  # destroy while stopped
+    other = kd.DigestServer(2, 100.0)
+    other.start(1)
+    other.submit_ingest(0, np.arange(100.0))
+    del other  # destroy while running: drains and joins the workers
