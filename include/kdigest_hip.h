@@ -168,6 +168,15 @@ double kd_scale_q(int scale, double k, double nrm);
  * the host side of the helper the device sweeps use to check their
  * boundaries in the reference's summation order (tdigest.mojo:419-435). */
 double kd_add_repeated(double W, double x, int64_t L);
+/* The running sums S = S + x[i], i = 0..m-1 (x[i] >= 0), as the reference's
+ * per-sample loops form them (the decay exponent, tdigest.mojo:324-332; the
+ * stored total, :334-347), by the binade rounds the batched replay runs in
+ * parallel on the device: bit-identical to the loop.  With stop_on the first
+ * nonzero x[i] whose sum exceeds thr is not consumed (its sum goes to
+ * *stop_val).  Returns the number consumed; *S is the sum after them and
+ * out[i] (if out is given) the sum after x[i].  Host only, for tests. */
+int64_t kd_add_sequence(double *S, const double *x, int64_t m, int stop_on, double thr,
+                        double *out, double *stop_val);
 
 /* ------------------------------------------------ TDigest (one digest) -- */
 /* TDigest[S] (kdigest/tdigest.mojo:97-726).  decay_lambda is Decay.rate_lambda

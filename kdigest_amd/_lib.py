@@ -52,6 +52,7 @@ SIGNATURES = {
     "kd_scale_normalizer": (_d, [_i, _d, _d]),
     "kd_scale_k": (_d, [_i, _d, _d]),
     "kd_add_repeated": (_d, [_d, _d, _i64]),
+    "kd_add_sequence": (_i64, [_dp, _dp, _i64, _i, _d, _dp, _dp]),
     "kd_scale_q": (_d, [_i, _d, _d]),
     "kd_tdigest_create": (_i, [_vp, _i, _d, _d, C.POINTER(_vp)]),
     "kd_tdigest_destroy": (None, [_vp]),

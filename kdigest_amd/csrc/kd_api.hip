@@ -1950,6 +1950,11 @@ extern "C" double kd_scale_k(int scale, double q, double nrm) { return kd_k(scal
 
 extern "C" double kd_add_repeated(double W, double x, int64_t L) { return kd_repeat_add(W, x, L); }
 
+extern "C" int64_t kd_add_sequence(double* S, const double* x, int64_t m, int stop_on, double thr,
+                                   double* out, double* stop_val) {
+  return kd_running_sum(S, x, m, stop_on, thr, out, stop_val);
+}
+
 // Inverse maps of scale.mojo:96-99, 120-124, 152-157, 186-192.
 extern "C" double kd_scale_q(int scale, double k, double nrm) {
   switch (scale) {

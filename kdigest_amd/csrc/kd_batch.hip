@@ -3,9 +3,10 @@
 // pairwise merge_into tree (tdigest.mojo:589-623), all on the GPU.
 //
 // One workgroup per digest keeps the digest's centroids and ingest buffer in
-// LDS for the whole stream.  Per sample, the clock / decay exponent / stored
-// total bookkeeping is inherently sequential and runs in lane 0; the decay
-// factors, the stable sort, the weight prefix, the k values, the
+// LDS for the whole stream.  Per sample, the clock is a block max-scan; the
+// decay exponent and the stored total are sequential floating-point sums,
+// formed by the block with the loop's exact bits (block_running_sum); the
+// decay factors, the stable sort, the weight prefix, the k values, the
 // next-boundary search and the cluster means run across the block.  Each
 // buffer merge (tdigest.mojo:391-439) is the same greedy sweep as the CPU:
 // the chain of cluster starts is walked by lane 0 over a next-boundary table
@@ -83,6 +84,232 @@ struct Sh {
   int frac, bad;  // merge_lds: some weight is not an integer; first cluster in question
   double renorm_ls;
 };
+
+// ---------------------------------------------------------------- scans
+constexpr int kRWaves = kRB / 64;
+constexpr int kRPer = (kRBMax + kRB - 1) / kRB;  // values per thread in a running-sum round
+
+// Shared state of the block scans and running sums below.
+struct RunSh {
+  uint64_t ws[kRWaves];
+  double wd[kRWaves];
+  int code;  // round: 2 * index + 1 of the first step the units do not cover (2 m + 1: none)
+  int i0, done, consumed, stopped, kind;
+  double S, Sprev, stopv, Send;
+};
+
+// Exclusive sum of v over the block's threads in thread order.
+__device__ inline uint64_t block_excl_scan_u64(uint64_t v, RunSh& rs) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) rs.ws[w] = inc;
+  __syncthreads();
+  uint64_t base = 0;
+  for (int k = 0; k < w; k++) base += rs.ws[k];
+  __syncthreads();
+  return base + inc - v;
+}
+
+// max(init, v of every earlier thread): an exact exclusive max-scan.
+__device__ inline double block_excl_max(double v, double init, RunSh& rs) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc = y > inc ? y : inc;
+  }
+  if (lane == 63) rs.wd[w] = inc;
+  __syncthreads();
+  double base = init;
+  for (int k = 0; k < w; k++) base = rs.wd[k] > base ? rs.wd[k] : base;
+  const double before = __shfl_up(inc, 1, 64);
+  __syncthreads();
+  return lane > 0 && before > base ? before : base;
+}
+
+struct RunOut {
+  int consumed, stopped;
+  double S, stopv;
+};
+
+// The sequential loop S = S + x[i], i = 0..m-1 (x[i] >= 0; m <= kRB * kRPer),
+// bit for bit, by the block in binade rounds (kd_sum_units): every step of a
+// round is an integer number of units of the round's binade, so the sums
+// are an exclusive scan of those counts; the first step they do not cover
+// (an exact half, a sum leaving the binade) is added plainly by lane 0 and
+// the next round starts after it -- one round per call in practice.  A start
+// below 2^-960 (zero) passes zeros through and adds the first nonzero x
+// plainly.  With stop_on, the first nonzero x whose sum exceeds thr is not
+// consumed (its sum is stopv).  out[i] = the sum after x[i] when out is
+// given; out may alias x (an index is written after its round has read it).
+// kd_running_sum is the same algorithm sequentially (tested against the
+// loop on the host).
+__device__ RunOut block_running_sum(double S0, const double* x, double* out, int m, bool stop_on,
+                                    double thr, RunSh& rs) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    rs.i0 = 0;
+    rs.S = S0;
+    rs.done = 0;
+    rs.consumed = m;
+    rs.stopped = 0;
+  }
+  __syncthreads();
+  const int none = 2 * m + 1;
+  for (;;) {
+    const int i0 = rs.i0;
+    const double Sc = rs.S;
+    const int done = rs.done;
+    __syncthreads();  // every thread has read the round's start
+    if (done || i0 >= m) break;
+    if (!(Sc >= 0x1p-960)) {
+      if (tid == 0) rs.code = none;
+      __syncthreads();
+      for (int i = i0 + tid; i < m; i += kRB)
+        if (x[i] != 0.0) {
+          atomicMin(&rs.code, 2 * i + 1);
+          break;
+        }
+      __syncthreads();
+      const int F = rs.code >> 1;  // first nonzero x, or m
+      if (out)
+        for (int i = i0 + tid; i < F; i += kRB) out[i] = Sc;
+      if (tid == 0) {
+        if (F >= m) {
+          rs.i0 = m;
+        } else {
+          const double ns = Sc + x[F];
+          if (stop_on && ns > thr) {
+            rs.done = 1;
+            rs.consumed = F;
+            rs.stopped = 1;
+            rs.stopv = ns;
+          } else {
+            if (out) out[F] = ns;
+            rs.S = ns;
+            rs.i0 = F + 1;
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    const int e = kd_exponent(Sc);
+    const double u = kd_pow2(e - 52), ui = kd_pow2(52 - e);
+    const uint64_t A0 = (uint64_t)(Sc * ui);
+    const int cnt = m - i0;
+    const int per = (cnt + kRB - 1) / kRB;
+    const int a = i0 + min(cnt, tid * per), b = i0 + min(cnt, tid * per + per);
+    if (tid == 0) rs.code = none;
+    double xv[kRPer];
+    int64_t rv[kRPer];
+    unsigned upm = 0;
+    uint64_t part = 0;
+    bool live = true;
+#pragma unroll
+    for (int k = 0; k < kRPer; k++) {
+      const int i = a + k;
+      xv[k] = 0.0;
+      rv[k] = 0;
+      if (i < b) {
+        xv[k] = x[i];
+        int64_t fl = 0;
+        const int64_t r = kd_sum_units(xv[k], ui, &fl);
+        rv[k] = r;
+        if (r >= 0 && r != fl) upm |= 1u << k;
+        if (live) {
+          if (r < 0) live = false;
+          else part += (uint64_t)r;
+        }
+      }
+    }
+    uint64_t A = A0 + block_excl_scan_u64(part, rs);
+    int code = none, kind = 0;
+    double sv[kRPer];
+    double Sprev = 0.0, stopv = 0.0;
+    live = true;
+#pragma unroll
+    for (int k = 0; k < kRPer; k++) {
+      const int i = a + k;
+      sv[k] = 0.0;
+      if (live && i < b) {
+        const int64_t r = rv[k];
+        const uint64_t fl = r < 0 ? 0 : (uint64_t)(r - (int64_t)((upm >> k) & 1u));
+        if (r < 0 || A + fl >= (1ull << 53)) {
+          live = false;
+          code = 2 * i + 1;
+          kind = 2;
+          Sprev = (double)A * u;
+        } else {
+          const uint64_t A1 = A + (uint64_t)r;
+          const double s1 = (double)A1 * u;
+          if (stop_on && xv[k] != 0.0 && s1 > thr) {
+            live = false;
+            code = 2 * i + 1;
+            kind = 3;
+            Sprev = (double)A * u;
+            stopv = s1;
+          } else {
+            sv[k] = s1;
+            A = A1;
+          }
+        }
+      }
+    }
+    if (code < none) atomicMin(&rs.code, code);
+    __syncthreads();
+    const int cmin = rs.code;
+    const int F = cmin >> 1;  // m when every step was covered
+    if (cmin < none && code == cmin) {
+      rs.kind = kind;
+      rs.Sprev = Sprev;
+      rs.stopv = stopv;
+    }
+    if (cmin == none && b == m && b > a) rs.Send = (double)A * u;
+    if (out) {
+#pragma unroll
+      for (int k = 0; k < kRPer; k++) {
+        const int i = a + k;
+        if (i < b && i < F) out[i] = sv[k];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (cmin == none) {
+        rs.S = rs.Send;
+        rs.i0 = m;
+      } else if (rs.kind == 3) {
+        rs.S = rs.Sprev;
+        rs.done = 1;
+        rs.consumed = F;
+        rs.stopped = 1;
+      } else {
+        const double ns = rs.Sprev + x[F];
+        if (stop_on && x[F] != 0.0 && ns > thr) {
+          rs.S = rs.Sprev;
+          rs.done = 1;
+          rs.consumed = F;
+          rs.stopped = 1;
+          rs.stopv = ns;
+        } else {
+          if (out) out[F] = ns;
+          rs.S = ns;
+          rs.i0 = F + 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  RunOut o{rs.consumed, rs.stopped, rs.S, rs.stopv};
+  __syncthreads();
+  return o;
+}
 
 // Stable sort of the buffer by (mean, position): bitonic on the pair.
 __device__ void sort_buffer(Lds& L, int nb) {
@@ -408,6 +635,7 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
                                                 double* gcw, double* gbm, double* gbw) {
   __shared__ __attribute__((aligned(16))) char pool[kLds];
   __shared__ Sh sh;
+  __shared__ RunSh rsh;
   Lds L = carve(pool);
   const int64_t d = order[blockIdx.x];
   load_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
@@ -434,20 +662,7 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
     {
       double lm = -__builtin_inf();
       for (int i = i0; i < i1; i++) lm = L.P[i] > lm ? L.P[i] : lm;
-      L.wm[tid] = lm;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double run = sh.st.clock;
-      for (int t = 0; t < kRB; t++) {
-        const double x = L.wm[t];
-        L.wm[t] = run;
-        run = x > run ? x : run;
-      }
-    }
-    __syncthreads();
-    {
-      double prev = L.wm[tid];
+      double prev = block_excl_max(lm, sh.st.clock, rsh);
       for (int i = i0; i < i1; i++) {
         const double t = L.P[i];
         L.bw[nb0 + i] = (lambda > 0.0 && t > prev) ? lambda * (t - prev) : 0.0;
@@ -456,38 +671,26 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      int m = 0, renorm = 0;
-      double ls = sh.st.log_scale, rls = 0.0;
-      double* term = L.bw + nb0;
-      while (m < have && !renorm) {
-        double t8[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) t8[u] = m + u < have ? term[m + u] : 0.0;
-        const int m0 = m;
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          if (m0 + u < have && !renorm) {
-            const double nls = ls + t8[u];
-            if (t8[u] != 0.0 && nls > kRenorm) {
-              renorm = 1;
-              rls = nls;
-            } else {
-              ls = nls;
-              term[m] = ls;
-              m++;
-            }
-          }
-        }
+    if (lambda > 0.0) {
+      // The exponent's running sum, stopping where it crosses the
+      // renormalization threshold (block_running_sum: the sequential
+      // additions' bits).
+      const RunOut o = block_running_sum(sh.st.log_scale, L.bw + nb0, L.bw + nb0, have, true,
+                                         kRenorm, rsh);
+      if (tid == 0) {
+        // The clock after the last consumed sample, or after the one whose
+        // advance triggers the renormalization.
+        const int last = o.stopped ? o.consumed : o.consumed - 1;
+        if (last >= 0) sh.st.clock = L.kv[last];
+        sh.renorm = o.stopped;
+        sh.renorm_ls = o.stopv;
+        sh.m = o.consumed;
+        sh.st.log_scale = o.S;
       }
-      // The clock after the last consumed sample, or after the one whose
-      // advance triggers the renormalization.
-      const int last = renorm ? m : m - 1;
-      if (last >= 0) sh.st.clock = L.kv[last];
-      sh.renorm = renorm;
-      sh.renorm_ls = rls;
-      sh.m = m;
-      sh.st.log_scale = ls;
+    } else if (tid == 0) {
+      if (have > 0) sh.st.clock = L.kv[have - 1];
+      sh.renorm = 0;
+      sh.m = have;
     }
     __syncthreads();
     const int m = sh.m;
@@ -499,9 +702,9 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
     }
     __syncthreads();
     // 3. Insert bookkeeping in order (tdigest.mojo:334-347): min and max by a
-    //    block reduction that keeps the earliest of equal values (as the
-    //    sequential strict comparisons do); the stored total by one addition
-    //    per sample in order (lane 0).
+    //    block reduction in sample order that keeps the earliest of equal
+    //    values (as the sequential strict comparisons do); the stored total
+    //    by the sequential additions' bits (block_running_sum).
     {
       const int pm = (m + kRB - 1) / kRB;
       const int j0 = min(m, tid * pm), j1 = min(m, j0 + pm);
@@ -518,16 +721,35 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
           if (v > mx) mx = v;
         }
       }
-      L.wm[tid] = mn;
-      L.ww[tid] = mx;
-      L.nx[tid] = has;
+      // Lane l takes lane l + o's range (the later samples) at step o.
+      const int lane = tid & 63;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const double omn = __shfl_down(mn, o, 64), omx = __shfl_down(mx, o, 64);
+        const int oh = __shfl_down(has, o, 64);
+        if ((lane & (2 * o - 1)) == 0 && oh) {
+          if (!has) {
+            mn = omn;
+            mx = omx;
+            has = 1;
+          } else {
+            if (omn < mn) mn = omn;
+            if (omx > mx) mx = omx;
+          }
+        }
+      }
+      if (lane == 0) {
+        L.wm[tid >> 6] = mn;
+        L.ww[tid >> 6] = mx;
+        L.nx[tid >> 6] = has;
+      }
     }
-    __syncthreads();
+    const bool first = sh.st.stored_total == 0.0 && nb0 == 0;
+    const RunOut tot = block_running_sum(sh.st.stored_total, L.bw + nb0, nullptr, m, false, 0.0, rsh);
     if (tid == 0 && m > 0) {
-      double total = sh.st.stored_total, vmin = sh.st.vmin, vmax = sh.st.vmax;
-      const bool first = total == 0.0 && nb0 == 0;
+      double vmin = sh.st.vmin, vmax = sh.st.vmax;
       bool init = !first;
-      for (int t = 0; t < kRB; t++) {
+      for (int t = 0; t < kRWaves; t++) {
         if (!L.nx[t]) continue;
         const double a = L.wm[t], c = L.ww[t];
         if (!init) {
@@ -539,16 +761,7 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
           if (c > vmax) vmax = c;
         }
       }
-      const double* w = L.bw + nb0;
-      for (int i0w = 0; i0w < m; i0w += 8) {
-        double w8[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) w8[u] = i0w + u < m ? w[i0w + u] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-          if (i0w + u < m) total += w8[u];
-      }
-      sh.st.stored_total = total;
+      sh.st.stored_total = tot.S;
       sh.st.vmin = vmin;
       sh.st.vmax = vmax;
       sh.st.nb = nb0 + m;

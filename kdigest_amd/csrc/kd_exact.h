@@ -77,6 +77,98 @@ KD_HD double kd_repeat_add(double W, double x, int64_t L) {
   return W;
 }
 
+/* A running sum S = S + x (x >= 0) inside the binade [2^e, 2^(e+1)) of S,
+ * counted in units u = 2^(e-52) (ui = 1/u): S = A u with A an integer in
+ * [2^52, 2^53).  While A + floor(x/u) < 2^53 the rounded sum is (A + r) u
+ * with r the integer nearest to x/u, whatever A is, except at an exact
+ * half, where the tie goes to the even sum.  So a stretch of such steps is
+ * an integer prefix sum, which a block can form in parallel.  Returns r and
+ * *fl = floor(x/u), or -1 when the step needs the plain addition (an exact
+ * half, or x/u >= 2^52).  x * ui is exact (a power-of-two scaling) unless it
+ * underflows, and then r = 0 either way. */
+KD_HD int64_t kd_sum_units(double x, double ui, int64_t* fl) {
+  const double q = x * ui;
+  if (!(q < 0x1p52)) return -1;
+  const double f = __builtin_floor(q);
+  const double d = q - f; /* exact */
+  if (d == 0.5) return -1;
+  *fl = (int64_t)f;
+  return (int64_t)f + (d > 0.5 ? 1 : 0);
+}
+
+/* The sums S_i = S_(i-1) + x_i, i = 0..m-1 (x_i >= 0), computed round by
+ * round as a block does it: within a round every step is counted in units of
+ * the round's binade (kd_sum_units); the first step that the units do not
+ * cover (a tie, an increment of 2^52 units, a sum leaving the binade) is
+ * added plainly and the next round starts after it (a sum that reached
+ * 2^(e+1) exactly makes the next step such a one).  Starting values below 2^-960 (zero
+ * included) pass zeros through and add the first nonzero x plainly.  With
+ * stop_on, the first nonzero x whose sum exceeds thr is not consumed: its
+ * index is returned and the exceeding sum goes to *stop_val.  Returns the
+ * number of x consumed (m without a stop); *S is the sum after them and
+ * out[i] = S_i when out is given.  Equal to the sequential loop bit for bit
+ * (the host build is tested against it). */
+KD_HD int64_t kd_running_sum(double* S, const double* x, int64_t m, int stop_on, double thr,
+                             double* out, double* stop_val) {
+  double s = *S;
+  int64_t i = 0;
+  while (i < m) {
+    if (!(s >= 0x1p-960)) {
+      while (i < m && x[i] == 0.0) {
+        if (out) out[i] = s;
+        i++;
+      }
+      if (i == m) break;
+      const double ns = s + x[i];
+      if (stop_on && ns > thr) {
+        *stop_val = ns;
+        *S = s;
+        return i;
+      }
+      if (out) out[i] = ns;
+      s = ns;
+      i++;
+      continue;
+    }
+    const int e = kd_exponent(s);
+    const double u = kd_pow2(e - 52), ui = kd_pow2(52 - e);
+    uint64_t A = (uint64_t)(s * ui);
+    int plain = 0;
+    while (i < m) {
+      int64_t fl = 0;
+      const int64_t r = kd_sum_units(x[i], ui, &fl);
+      if (r < 0 || A + (uint64_t)fl >= (1ull << 53)) {
+        plain = 1;
+        break;
+      }
+      const uint64_t A1 = A + (uint64_t)r;
+      const double s1 = (double)A1 * u;
+      if (stop_on && x[i] != 0.0 && s1 > thr) {
+        *stop_val = s1;
+        *S = (double)A * u;
+        return i;
+      }
+      if (out) out[i] = s1;
+      A = A1;
+      i++;
+    }
+    s = (double)A * u;
+    if (plain) {
+      const double ns = s + x[i];
+      if (stop_on && x[i] != 0.0 && ns > thr) {
+        *stop_val = ns;
+        *S = s;
+        return i;
+      }
+      if (out) out[i] = ns;
+      s = ns;
+      i++;
+    }
+  }
+  *S = s;
+  return m;
+}
+
 /* The sweep's test at a candidate of weight w: true when it closes the
  * current cluster, i.e. NOT (k(q_right) - k(q_left) <= 1) with the
  * reference's operands (a NaN span closes it, as on the CPU).  The tie rule
