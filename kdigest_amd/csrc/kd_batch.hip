@@ -26,7 +26,7 @@ namespace kd {
 
 namespace {
 
-constexpr int kRB = 256;          // threads per digest
+constexpr int kRB = 512;          // threads per digest (8 waves: two per SIMD)
 constexpr int kRCMax = 1536;      // centroids kept per digest
 constexpr int kRBMax = 1536;      // ingest buffer capacity (>= 5 * compression)
 constexpr int kRNMax = kRCMax + kRBMax;
@@ -96,6 +96,8 @@ struct RunSh {
   int code;  // round: 2 * index + 1 of the first step the units do not cover (2 m + 1: none)
   int i0, done, consumed, stopped, kind;
   double S, Sprev, stopv, Send;
+  int total;                  // merge_lds: clusters kept
+  uint64_t pmask[kRWaves];    // merge_lds: threads that pruned a cluster
 };
 
 // Exclusive sum of v over the block's threads in thread order.
@@ -131,6 +133,25 @@ __device__ inline double block_excl_max(double v, double init, RunSh& rs) {
   const double before = __shfl_up(inc, 1, 64);
   __syncthreads();
   return lane > 0 && before > base ? before : base;
+}
+
+// Exclusive sum of v over the block's threads (a fixed tree: wave scan,
+// then the wave totals in order).
+__device__ inline double block_excl_sum(double v, RunSh& rs) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc = y + inc;
+  }
+  if (lane == 63) rs.wd[w] = inc;
+  __syncthreads();
+  double base = 0.0;
+  for (int k = 0; k < w; k++) base = base + rs.wd[k];
+  const double before = __shfl_up(inc, 1, 64);
+  __syncthreads();
+  return lane > 0 ? base + before : base;
 }
 
 struct RunOut {
@@ -311,39 +332,122 @@ __device__ RunOut block_running_sum(double S0, const double* x, double* out, int
   return o;
 }
 
-// Stable sort of the buffer by (mean, position): bitonic on the pair.
-__device__ void sort_buffer(Lds& L, int nb) {
-  int npad = 2;
-  while (npad < nb) npad <<= 1;
-  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-    L.skey[i] = i < nb ? L.bm[i] : __builtin_inf();
-    L.sidx[i] = i < nb ? i : 0x7fffffff;
-  }
-  __syncthreads();
-  for (int k = 2; k <= npad; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < (npad >> 1); t += blockDim.x) {
-        int i = 2 * t - (t & (j - 1));
-        int l = i + j;
-        double a = L.skey[i], b = L.skey[l];
-        int ia = L.sidx[i], ib = L.sidx[l];
-        bool a_first = (a < b) || (!(b < a) && ia < ib);
-        bool asc = (i & k) == 0;
-        if (asc != a_first) {
-          L.skey[i] = b;
-          L.skey[l] = a;
-          L.sidx[i] = ib;
-          L.sidx[l] = ia;
-        }
+// Stable sort of the buffer by (mean, position): a bitonic network on the
+// pair (order key of the mean, position) over kRB * kSortE slots, padded with
+// keys above every finite mean.  Thread t holds slots [kSortE t, kSortE t +
+// kSortE) in registers: partners closer than kSortE are in the same thread,
+// closer than 64 kSortE in the same wave (exchanged by lane shuffles); only
+// the widest stages go through LDS.  Leaves the sorted means in skey and
+// their buffer positions in sidx.
+constexpr int kSortSlots = 2048;
+constexpr int kSortE = kSortSlots / kRB;
+static_assert(kSortSlots >= kRBMax && kSortE >= 1 && kRB * kSortE == kSortSlots, "sort slots");
+
+__device__ inline bool sort_before(uint64_t ka, int ia, uint64_t kb, int ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+// In-thread stage of partner distance J (< kSortE).
+template <int J>
+__device__ inline void sort_reg_stage(uint64_t (&key)[kSortE], int (&idx)[kSortE], int base, int k) {
+#pragma unroll
+  for (int r = 0; r < kSortE; r++) {
+    if ((r & J) == 0 && (r | J) < kSortE) {
+      const int r2 = r | J;
+      const bool asc = ((base + r) & k) == 0;
+      const bool sw = asc ? sort_before(key[r2], idx[r2], key[r], idx[r])
+                          : sort_before(key[r], idx[r], key[r2], idx[r2]);
+      if (sw) {
+        const uint64_t tk = key[r];
+        key[r] = key[r2];
+        key[r2] = tk;
+        const int ti = idx[r];
+        idx[r] = idx[r2];
+        idx[r2] = ti;
       }
-      __syncthreads();
     }
   }
 }
 
+__device__ void sort_buffer(Lds& L, int nb) {
+  const int base = (int)threadIdx.x * kSortE;
+  uint64_t key[kSortE];
+  int idx[kSortE];
+#pragma unroll
+  for (int r = 0; r < kSortE; r++) {
+    const int i = base + r;
+    key[r] = i < nb ? ord_key(L.bm[i]) : ~0ull;
+    idx[r] = i < nb ? i : 0x7fffffff;
+  }
+  uint64_t* sk = reinterpret_cast<uint64_t*>(L.skey);
+  for (int k = 2; k <= kSortSlots; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < kSortE) {
+        if constexpr (kSortE > 4) {
+          if (j == 4) sort_reg_stage<4>(key, idx, base, k);
+        }
+        if constexpr (kSortE > 2) {
+          if (j == 2) sort_reg_stage<2>(key, idx, base, k);
+        }
+        if constexpr (kSortE > 1) {
+          if (j == 1) sort_reg_stage<1>(key, idx, base, k);
+        }
+        continue;
+      }
+      // Slot p takes the first of (its pair) when it is the lower slot of an
+      // ascending pair or the upper of a descending one, else the second.
+      uint64_t pk[kSortE];
+      int pi[kSortE];
+      if (j < 64 * kSortE) {
+        const int lm = j / kSortE;
+#pragma unroll
+        for (int r = 0; r < kSortE; r++) {
+          pk[r] = __shfl_xor(key[r], lm, 64);
+          pi[r] = __shfl_xor(idx[r], lm, 64);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < kSortE; r++) {
+          sk[base + r] = key[r];
+          L.sidx[base + r] = idx[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortE; r++) {
+          const int l = (base + r) ^ j;
+          pk[r] = sk[l];
+          pi[r] = L.sidx[l];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < kSortE; r++) {
+        const int p = base + r;
+        const bool lower = (p & j) == 0;
+        const bool asc = (p & k) == 0;
+        const bool mine_first = sort_before(key[r], idx[r], pk[r], pi[r]);
+        const bool take = (lower == asc) ? !mine_first : mine_first;
+        if (take) {
+          key[r] = pk[r];
+          idx[r] = pi[r];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSortE; r++) {
+    const int i = base + r;
+    if (i < nb) {
+      L.skey[i] = L.bm[idx[r]];
+      L.sidx[i] = idx[r];
+    }
+  }
+  __syncthreads();
+}
+
 // merge() of tdigest.mojo:391-439 (+ prune 441-458) on the digest in LDS.
 // Returns false when the result would not fit kRCMax centroids.
-__device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double lambda) {
+__device__ bool merge_lds(Lds& L, Sh& sh, RunSh& rs, int scale, double compression, double lambda) {
   const int nc = sh.st.nc, nb = sh.st.nb;
   if (nb == 0) return true;
   const int n = nc + nb;
@@ -372,25 +476,15 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
     L.ww[i + lo] = L.cw[i];
   }
   __syncthreads();
-  // Weight prefix P[j] = sum of the first j weights (blocked scan).
+  // Weight prefix P[j] = sum of the first j weights (blocked scan; exact for
+  // integer weights, and every boundary found from it is checked in the
+  // CPU's order below otherwise).
   {
     const int chunk = (n + blockDim.x - 1) / blockDim.x;
     const int a = min(n, (int)threadIdx.x * chunk), b = min(n, a + chunk);
     double s = 0.0;
     for (int j = a; j < b; j++) s += L.ww[j];
-    L.kv[threadIdx.x] = s;  // partials (kv is free here)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double run = 0.0;
-      for (int t = 0; t < (int)blockDim.x; t++) {
-        double v = L.kv[t];
-        L.kv[t] = run;
-        run += v;
-      }
-    }
-    __syncthreads();
-    double run = L.kv[threadIdx.x];
-    __syncthreads();
+    double run = block_excl_sum(s, rs);
     for (int j = a; j < b; j++) {
       L.P[j] = run;
       run += L.ww[j];
@@ -470,18 +564,33 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
       T0[c] = W;
     }
     __syncthreads();
+    // T0[c] = the sum of the cluster weights up to c, in cluster order (the
+    // CPU's weight_so_far after c); wsf of cluster c is T0[c - 1].
+    // (One lane: the sum grows from one cluster's weight to the total, so
+    // it crosses a binade every few clusters and the block's rounds would
+    // not pay.)
     if (threadIdx.x == 0) {
       double acc = 0.0;
-      for (int c = 0; c < C0; c++) {
-        const double w = T0[c];
+      int c = 0;
+      for (; c + 8 <= C0; c += 8) {
+        double t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t8[u] = T0[c + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          acc = acc + t8[u];
+          T0[c + u] = acc;
+        }
+      }
+      for (; c < C0; c++) {
+        acc = acc + T0[c];
         T0[c] = acc;
-        acc = acc + w;
       }
     }
     __syncthreads();
     for (int c = threadIdx.x; c < C0; c += blockDim.x) {
       const int a = L.starts[c], b = c + 1 < C0 ? L.starts[c + 1] : n;
-      const double wsf = T0[c];
+      const double wsf = c > 0 ? T0[c - 1] : 0.0;
       double W = L.ww[a];
       bool bad = false;
       if (b - a >= 2) {
@@ -496,7 +605,7 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
     if (threadIdx.x == 0 && sh.bad < C0) {
       int c = sh.bad;
       int st0 = L.starts[c];
-      double wsf = T0[c];
+      double wsf = c > 0 ? T0[c - 1] : 0.0;
       double cur_w = L.ww[st0], cur_m = L.wm[st0];
       L.starts[c++] = st0;
       for (int i = st0 + 1; i < n; i++) {
@@ -537,37 +646,39 @@ __device__ bool merge_lds(Lds& L, Sh& sh, int scale, double compression, double 
   // Prune (decay active, tdigest.mojo:441-458) and install: each thread
   // compacts a contiguous run of clusters at its scanned offset; the pruned
   // weight is summed in cluster order by lane 0 over the runs that have any.
-  __shared__ int kept[kRB], pruned[kRB];
   {
     const double floor = lambda > 0.0 ? kPruneEps * kd_exp(sh.st.log_scale) : 0.0;
     const int per = (C + kRB - 1) / kRB;
     const int c0 = min(C, (int)threadIdx.x * per), c1 = min(C, c0 + per);
     int kc = 0;
     for (int c = c0; c < c1; c++) kc += (lambda > 0.0 && !(ow[c] >= floor)) ? 0 : 1;
-    kept[threadIdx.x] = kc;
-    pruned[threadIdx.x] = (c1 - c0) - kc;
+    const bool pr = (c1 - c0) - kc > 0;
+    const uint64_t pmask = __ballot(pr);
+    int k = (int)block_excl_scan_u64((uint64_t)kc, rs);
+    if (threadIdx.x == kRB - 1) rs.total = k + kc;
+    if ((threadIdx.x & 63) == 0) rs.pmask[threadIdx.x >> 6] = pmask;
     __syncthreads();
     if (threadIdx.x == 0) {
-      int run = 0;
+      // The pruned weight, summed in cluster order over the threads that
+      // pruned any.
       double removed = 0.0;
-      for (int t = 0; t < kRB; t++) {
-        const int x = kept[t];
-        kept[t] = run;
-        run += x;
-        if (pruned[t]) {
+      for (int w = 0; w < kRWaves; w++) {
+        uint64_t mk = rs.pmask[w];
+        while (mk) {
+          const int t = w * 64 + __builtin_ctzll(mk);
+          mk &= mk - 1;
           const int a0 = min(C, t * per), a1 = min(C, a0 + per);
           for (int c = a0; c < a1; c++)
             if (!(ow[c] >= floor)) removed += ow[c];
         }
       }
       if (removed > 0.0) sh.st.stored_total -= removed;
+      const int run = rs.total;
       const bool fits = run <= kRCMax;
       sh.st.nc = fits ? run : kRCMax;
       sh.st.nb = 0;
       if (!fits) sh.st.err = 1;
     }
-    __syncthreads();
-    int k = kept[threadIdx.x];
     for (int c = c0; c < c1; c++) {
       if (lambda > 0.0 && !(ow[c] >= floor)) continue;
       if (k < kRCMax) {
@@ -775,7 +886,7 @@ __global__ __launch_bounds__(kRB) void k_replay(int scale, double compression, d
       continue;  // the sample that triggered it is inserted by the next walk
     }
     if (sh.st.nb >= limit) {
-      if (!merge_lds(L, sh, scale, compression, lambda)) break;
+      if (!merge_lds(L, sh, rsh, scale, compression, lambda)) break;
     }
   }
   store_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
@@ -787,11 +898,12 @@ __global__ __launch_bounds__(kRB) void k_flush(int scale, double compression, do
                                                double* gbm, double* gbw) {
   __shared__ __attribute__((aligned(16))) char pool[kLds];
   __shared__ Sh sh;
+  __shared__ RunSh rsh;
   Lds L = carve(pool);
   const int64_t d = blockIdx.x;
   load_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
   if (sh.st.err || sh.st.nb == 0) return;
-  merge_lds(L, sh, scale, compression, lambda);
+  merge_lds(L, sh, rsh, scale, compression, lambda);
   store_digest(L, sh, st, gcm, gcw, gbm, gbw, d);
 }
 
@@ -803,6 +915,7 @@ __global__ __launch_bounds__(kRB) void k_merge_pairs(int scale, double compressi
                                                      double* gbw) {
   __shared__ __attribute__((aligned(16))) char pool[kLds];
   __shared__ Sh sh;
+  __shared__ RunSh rsh;
   __shared__ BatchState other;
   Lds L = carve(pool);
   const int64_t self = (int64_t)blockIdx.x * 2 * stride;
@@ -845,7 +958,7 @@ __global__ __launch_bounds__(kRB) void k_merge_pairs(int scale, double compressi
     sh.st.clock = fmax(sh.st.clock, other.clock);
   }
   __syncthreads();
-  merge_lds(L, sh, scale, compression, lambda);
+  merge_lds(L, sh, rsh, scale, compression, lambda);
   store_digest(L, sh, st, gcm, gcw, gbm, gbw, self);
   // other is renormalized in place (tdigest.mojo:608-609 mutates it).
   if (lambda > 0.0) {
