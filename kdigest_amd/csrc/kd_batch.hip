@@ -82,6 +82,7 @@ struct Sh {
   BatchState st;
   int m, renorm, nclusters;
   int frac, bad;  // merge_lds: some weight is not an integer; first cluster in question
+  int anytie;     // merge_lds: some next-boundary link carries the tie flag
   double renorm_ls;
 };
 
@@ -495,6 +496,7 @@ __device__ bool merge_lds(Lds& L, Sh& sh, RunSh& rs, int scale, double compressi
   const double N = sh.st.stored_total;
   const double eff = lambda > 0.0 ? N * kd_exp(-sh.st.log_scale) : N;
   const double nrm = kd_normalizer(scale, compression, eff);
+  if (threadIdx.x == 0) sh.anytie = 0;
   for (int j = threadIdx.x; j <= n; j += blockDim.x) L.kv[j] = kd_k(scale, L.P[j] / N, nrm);
   __syncthreads();
   // nx[s]: first candidate i > s with NOT(k(P[i+1]/N) - k(P[s]/N) <= 1), or n.
@@ -511,10 +513,51 @@ __device__ bool merge_lds(Lds& L, Sh& sh, RunSh& rs, int scale, double compressi
     const int b = lo - 1;
     const bool tie = b < n && ord_key(L.wm[b]) - ord_key(L.wm[b - 1]) <= 2;
     L.nx[s] = b | (tie ? 0x40000000 : 0);
+    if (tie) sh.anytie = 1;
   }
   __syncthreads();
-  // Chain of cluster starts with the tie rule (tdigest.mojo:429), lane 0.
-  if (threadIdx.x == 0) {
+  // Chain of cluster starts s -> nx[s] from 0.  Without a tie flag the links
+  // are plain, and one wave walks 64 links at a time: jump tables f^2 ..
+  // f^32 (16-bit, n absorbing) in the free prefix and k-value regions, lane
+  // i forms f^i(s) from the bits of i.
+  if (!sh.anytie) {
+    constexpr int kJs = kRNMax + 2;  // table stride (entries 0..n)
+    uint16_t* jt = reinterpret_cast<uint16_t*>(L.P);  // f^2, f^4, f^8, f^16
+    uint16_t* j32 = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(L.kv) + 4 * kRNMax);
+    static_assert(4 * kJs * 2 <= 8 * (kRNMax + 8) && kJs * 2 <= 4 * (kRNMax + 8), "jump tables");
+    auto f1 = [&](int x) { return x < n ? (L.nx[x] & 0x3fffffff) : n; };
+    for (int x = threadIdx.x; x <= n; x += blockDim.x) jt[x] = (uint16_t)f1(f1(x));
+    __syncthreads();
+    for (int t = 1; t < 5; t++) {
+      const uint16_t* src = jt + (t - 1) * kJs;
+      uint16_t* dst = t < 4 ? jt + t * kJs : j32;
+      for (int x = threadIdx.x; x <= n; x += blockDim.x) dst[x] = src[src[x]];
+      __syncthreads();
+    }
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      int s = 0, c = 0;
+      while (s < n) {
+        int x = s;
+        if (lane & 1) x = f1(x);
+        if (lane & 2) x = jt[x];
+        if (lane & 4) x = jt[kJs + x];
+        if (lane & 8) x = jt[2 * kJs + x];
+        if (lane & 16) x = jt[3 * kJs + x];
+        if (lane & 32) x = j32[x];
+        if (x < n) L.starts[c + lane] = x;  // f^lane(s): the chain is a prefix of the lanes
+        c += __popcll(__ballot(x < n));
+        const int last = __shfl(x, 63, 64);
+        s = f1(last);
+      }
+      if (lane == 0) {
+        sh.nclusters = c;
+        sh.frac = 0;
+        sh.bad = 0x7fffffff;
+      }
+    }
+  } else if (threadIdx.x == 0) {
+    // With the tie rule (tdigest.mojo:429), lane 0.
     int s = 0, c = 0;
     while (s < n) {
       L.starts[c++] = s;
