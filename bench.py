@@ -846,6 +846,8 @@ def main():
     ap.add_argument("--radix-key-bits", type=int, default=None,
                     help="grouped batches over more than 2^bits keys take the counting partition "
                          "(default: the library's 27)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only the batched decayed digests (config 4) line, no grouped run")
     ap.add_argument("--only-single", action="store_true",
                     help="measure only the single-digest line (for its kernel profile)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -869,6 +871,13 @@ def main():
                           "unit": "samples/s", "n_gpus": 1, "higher_is_better": True,
                           "dtype": "f64", "data": "synthetic (torch Philox, seeded), device-resident",
                           "config": {"workload": single["workload"]}, "single_digest": single}))
+        return
+    if args.only_batched:
+        batched = batched_decay(kd, session, dev, nd=args.batched_digests, with_parity=False)
+        print(json.dumps({"metric": "batched decayed digests samples/s", "value": batched["samples_per_s"],
+                          "unit": "samples/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f64", "data": "synthetic (torch Philox, seeded), device-resident",
+                          "config": {"workload": batched["workload"]}, "batched_decay": batched}))
         return
     if args.radix_key_bits is not None:
         session.set_radix_key_bits(args.radix_key_bits)
