@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libkdigest_hip.so")
+# KDIGEST_AMD_LIB names another build of the library (an experiment or an
+# installed copy); by default the in-tree build.
+LIB_PATH = os.environ.get("KDIGEST_AMD_LIB") or os.path.join(_HERE, "libkdigest_hip.so")
 
 KD_OK, KD_EINVAL, KD_EEMPTY, KD_ERANGE, KD_ESERDE, KD_ETRUNC, KD_EDEVICE, KD_ENOMEM = (
     0, -1, -2, -3, -4, -5, -6, -7)

@@ -41,9 +41,10 @@ def _compare(kd_b, refs, integer_weights, tag):
         st, rs = kd_b.state(d), r.state()
         assert st["error"] == 0 and st["buffered"] == 0
         check_centroids(m, w, rm, rw, 20.0, integer_weights, f"{tag} digest {d}")
-        for k in ("clock", "log_scale", "min", "max"):
+        # The exponent and the stored total are the reference's sequential
+        # sums, bit for bit.
+        for k in ("clock", "log_scale", "min", "max", "stored_total"):
             assert st[k] == rs[k], (tag, d, k, st[k], rs[k])
-        assert abs(st["stored_total"] - rs["stored_total"]) <= 1e-12 * abs(rs["stored_total"]), (tag, d)
 
 
 @pytest.mark.parametrize("lam,int_weights", [(0.0, True), (1e-3, False), (0.5, False)])
@@ -215,3 +216,44 @@ def test_batch_from_bytes_rejects(kd, oracle):
     with pytest.raises(kd.KdigestError) as e:
         kd.BatchedDigests.from_bytes_all(ba[:-3], np.array([0, len(ba)]), scale=kd.ScaleK2)
     assert e.value.code == -5
+
+
+def test_batch_running_sums_with_exact_halves(kd, oracle):
+    """The replay's decay exponent and stored total are the reference's
+    sequential sums (tdigest.mojo:324-347) formed by the block in binade
+    rounds; a step whose increment is exactly half a unit of the sum's last
+    place rounds to the even sum, which depends on every step before it, and
+    is added on its own.  Digest 0: lambda 0.75 and steps of 2^-43 at a clock
+    near 350, so every decay increment is 1.5 units of the exponent's last
+    place (about 262 to 400); digest 1 (no decay): a first weight of 2^30,
+    then weights of 1.5 units of the total's last place.  Both exactly as the
+    oracle, and the exponent crosses the renormalization threshold."""
+    lam = 0.75
+    n0 = 3000
+    t = [350.0]
+    for i in range(1, n0):
+        t.append(t[-1] + (2.0 ** -43 if i % 50 else 3.5))
+    times0 = np.array(t)
+    vals0 = np.random.default_rng(2).standard_normal(n0)
+    w0 = np.ones(n0)
+    n1 = 2500
+    w1 = np.full(n1, 1.5 * 2.0 ** -22)
+    w1[0] = 2.0 ** 30
+    w1[1::7] = 3.0 * 2.0 ** -23  # 1.5 units as well, written another way
+    vals1 = np.random.default_rng(3).standard_normal(n1)
+    times1 = np.arange(n1, dtype=np.float64)
+    for d, (vals, times, wts, l) in enumerate([(vals0, times0, w0, lam), (vals1, times1, w1, 0.0)]):
+        b = kd.BatchedDigests(1, 100.0, kd.Decay.rate(l), scale=kd.ScaleK1)
+        off = np.array([0, vals.size], np.int64)
+        b.add_at(off, vals, times, wts)
+        b.flush()
+        r = oracle.Digest(1, 100.0, l)
+        for i in range(vals.size):
+            r.add_at(vals[i], times[i], wts[i])
+        r.merge()
+        st, rs = b.state(0), r.state()
+        for k in ("clock", "log_scale", "min", "max", "stored_total"):
+            assert st[k] == rs[k], (d, k, st[k], rs[k])
+        m, w = b.centroids(0)
+        rm, rw = r.centroids()
+        check_centroids(m, w, rm, rw, 5.0, False, f"digest {d}")
