@@ -64,6 +64,16 @@ void DevBuf::release() {
 }
 
 // ----------------------------------------------------------------- helpers
+// v of lane l (l the same in every lane), read through the scalar unit: a
+// lane-serial loop over it is a chain of scalar reads rather than of LDS
+// permutes.
+__device__ inline double lane_value(double v, int l) {
+  const uint64_t b = kd_d2bits(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return kd_bits2d(((uint64_t)hi << 32) | lo);
+}
+
 struct dd {
   double hi, lo;
 };
@@ -273,11 +283,64 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
     return;
   }
   int npad = pow2_at_least(n);
-  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-    key[i] = i < n ? em[sd.e_off + i] : __builtin_inf();
-    idx[i] = i < n ? i : 0x7fffffff;
+  if (n <= 128) {
+    // One wave, two slots per lane in registers: the same network over 128
+    // slots, partners across lanes exchanged by shuffles (no barriers).
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      double kk[2];
+      int ii[2];
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        const int i = 2 * lane + r;
+        kk[r] = i < n ? em[sd.e_off + i] : __builtin_inf();
+        ii[r] = i < n ? i : 0x7fffffff;
+      }
+      auto first = [](double a, int ia, double b, int ib) { return (a < b) || (!(b < a) && ia < ib); };
+      for (int k = 2; k <= 128; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          if (j == 1) {
+            const bool asc = ((2 * lane) & k) == 0;
+            const bool sw = asc ? !first(kk[0], ii[0], kk[1], ii[1]) : first(kk[0], ii[0], kk[1], ii[1]);
+            if (sw) {
+              const double tk = kk[0];
+              kk[0] = kk[1];
+              kk[1] = tk;
+              const int ti = ii[0];
+              ii[0] = ii[1];
+              ii[1] = ti;
+            }
+            continue;
+          }
+#pragma unroll
+          for (int r = 0; r < 2; r++) {
+            const double pk = __shfl_xor(kk[r], j >> 1, 64);
+            const int pi = __shfl_xor(ii[r], j >> 1, 64);
+            const int p = 2 * lane + r;
+            const bool lower = (p & j) == 0, asc = (p & k) == 0;
+            const bool mine = first(kk[r], ii[r], pk, pi);
+            if ((lower == asc) ? !mine : mine) {
+              kk[r] = pk;
+              ii[r] = pi;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        key[2 * lane + r] = kk[r];
+        idx[2 * lane + r] = ii[r];
+      }
+    }
+    __syncthreads();
+    npad = 0;  // sorted: the network below does nothing
+  } else {
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+      key[i] = i < n ? em[sd.e_off + i] : __builtin_inf();
+      idx[i] = i < n ? i : 0x7fffffff;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // Bitonic on (mean, input index): the stable order of the reference's
   // insertion sort (tdigest.mojo:64-77).
   for (int k = 2; k <= npad; k <<= 1) {
@@ -321,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
       const int cnt = n - c0 < 64 ? n - c0 : 64;
       double mine = 0.0;
       for (int l = 0; l < cnt; l++) {
-        acc += __shfl(wl, l, 64);
+        acc += lane_value(wl, l);
         if (lane == l) mine = acc;
       }
       if (lane < cnt) p[c0 + lane + 1] = mine;
@@ -2343,7 +2406,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
         const int cnt = C - c0 < 64 ? C - c0 : 64;
         double mine = 0.0;
         for (int l = 0; l < cnt; l++) {
-          const double wl = __shfl(w, l, 64);
+          const double wl = lane_value(w, l);
           if (lane == l) mine = acc;
           acc = acc + wl;
         }
@@ -3377,7 +3440,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* seg
   for (int64_t c0 = 0; c0 < C; c0 += 64) {
     const double wl = c0 + lane < C ? w[c0 + lane] : 0.0;
     const int cnt = (int)std::min<int64_t>(64, C - c0);
-    for (int l = 0; l < cnt; l++) total += __shfl(wl, l, 64);
+    for (int l = 0; l < cnt; l++) total += lane_value(wl, l);
   }
   // _prune (tdigest.mojo:441-458) when decay is active (floor > 0): kept
   // centroids move down in order (each chunk is read before it is written,
@@ -3395,7 +3458,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* seg
       uint64_t rm = __ballot(valid && !keep);
       while (rm) {
         const int l = __ffsll((long long)rm) - 1;
-        removed += __shfl(wl, l, 64);
+        removed += lane_value(wl, l);
         rm &= rm - 1ull;
       }
       if (keep) {
