@@ -348,7 +348,11 @@ int kd_keymap_keys(kd_keymap *m, uint32_t *out, int64_t cap);
 /* Continuous-batching server over num_keys digests resident in device
  * memory (GroupedTDigest semantics per key, grouped.mojo:60-190).  Each key
  * holds at most slot_capacity centroids; requests are admitted in arrival
- * order while their data fits mem_budget bytes.  kind: 0 ingest batch (one B
+ * order while the device memory their batch occupies fits mem_budget bytes
+ * (a query 28 B per value: key, x, result; an ingest 8 B per value staged
+ * plus its merge's workspace, 24 B per record over its values and the key's
+ * slot_capacity prior centroids; one request larger than the budget runs
+ * alone).  kind: 0 ingest batch (one B
  * merge, tdigest.mojo:223-315), 1 quantile batch, 2 cdf batch
  * (tdigest.mojo:485-587).  Requests of one key run in arrival order. */
 int kd_server_create(kd_session *s, int scale, int64_t num_keys, double compression,

@@ -86,6 +86,7 @@ class Server {
   };
   void refresh_state(const std::vector<int64_t>& keys);
   void finish(Request& r);
+  int64_t request_bytes(const Request& r) const;
   void run_queries(const std::vector<Request*>& queries, hipStream_t st);
   void run_ingest_round(std::vector<Request*>& round);
   void enqueue(Request& r);

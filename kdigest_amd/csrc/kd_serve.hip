@@ -147,6 +147,15 @@ Server::Server(Engine& eng, int scale, int64_t num_keys, double compression, dou
   check_hip(hipStreamSynchronize(eng_.stream()), "sync");
 }
 
+// Device memory a request occupies while its batch runs: a query's key, x
+// and result (plus its kind); an ingest's staged values, and the compress
+// workspace of its merge: the merged records and centroid outputs of its
+// new values and of the key's prior centroids (at most the slot capacity).
+int64_t Server::request_bytes(const Request& r) const {
+  const int64_t n = (int64_t)r.data.size();
+  return r.kind == 0 ? 8 * n + 24 * (n + cap_) : 28 * n;
+}
+
 int64_t Server::submit(int kind, int64_t key, const double* data, int64_t n) {
   Request r;
   r.kind = kind;
@@ -239,7 +248,7 @@ int64_t Server::step(int max_rounds) {
   for (int64_t id : pending_) {
     Request& r = reqs_.at(id);
     bool admit = false;
-    int64_t need = (int64_t)r.data.size() * (r.kind == 0 ? 8 : 16);
+    const int64_t need = request_bytes(r);
     if (!budget_full && !key_blocked[r.key]) {
       if (bytes + need > budget_ && bytes > 0) {
         budget_full = true;  // FIFO: nothing behind it jumps the queue
@@ -581,7 +590,7 @@ void Server::query_loop() {
         for (auto it = qq_.begin(); it != qq_.end();) {
           Request* r = *it;
           const int64_t* ks = key_seq_.data() + 4 * r->key;
-          const int64_t need = (int64_t)r->data.size() * 16;
+          const int64_t need = request_bytes(*r);
           if (ks[1] >= r->need_ing && (bytes == 0 || bytes + need <= budget_)) {
             it = qq_.erase(it);
             if (count_[r->key] == 0) {
@@ -643,7 +652,7 @@ void Server::ingest_loop() {
           Request* r = *it;
           const int64_t* ks = key_seq_.data() + 4 * r->key;
           int& c = taken[r->key];
-          const int64_t need = (int64_t)r->data.size() * 8;
+          const int64_t need = request_bytes(*r);
           const bool ready = ks[1] + c >= r->need_ing && ks[3] >= r->need_qry;
           if (ready && c < max_rounds_ && (bytes == 0 || bytes + need <= budget_)) {
             it = iq_.erase(it);

@@ -823,8 +823,9 @@ class DigestServer:
 
     Requests are ingest batches (one batch merge each, tdigest.mojo:223-315)
     and quantile / cdf batches (tdigest.mojo:485-587), each for one key.  A
-    step admits queued requests in arrival order while their data fits the
-    memory budget and runs them as one batch; every request's result is the
+    step admits queued requests in arrival order while the device memory
+    their batch occupies (staged data and merge workspace) fits the memory
+    budget and runs them as one batch; every request's result is the
     one it would have alone, in its key's arrival order."""
 
     INGEST, QUANTILE, CDF = 0, 1, 2

@@ -117,7 +117,11 @@ def test_serve_mixed_equals_alone_and_oracle(kd, oracle, scale, delta, lam):
 
 
 def test_serve_admission_order_and_budget(kd):
-    srv = kd.DigestServer(4, 100.0, mem_budget=8 * 1000)
+    # Device memory of a batch: an ingest of n values 8 n + 24 (n + slot
+    # capacity) bytes, a query 28 bytes per value.
+    cap = 128
+    need_a = 8 * 1000 + 24 * (1000 + cap)
+    srv = kd.DigestServer(4, 100.0, slot_capacity=cap, mem_budget=need_a + 10)
     a = srv.submit_ingest(0, np.arange(1000.0))   # fills the budget by itself
     q = srv.submit_quantile(0, [0.5])             # waits: budget full
     b = srv.submit_ingest(1, np.arange(10.0))     # FIFO: waits behind q
