@@ -158,27 +158,74 @@ __global__ void k_query(int is_cdf, const int64_t* off, const double* m, const d
 // whose key is out of range, whose quantile q is outside [0, 1] or whose key
 // is empty gets NaN and sets its bit in *err (the reference raises for each,
 // grouped.mojo:164-174 via tdigest.mojo:492-498, 545-546).
-__global__ void k_query_grouped(int is_cdf, int64_t nk, const int64_t* off, const double* m,
-                                const double* w, const double* S, const double* total,
-                                const double* vmin, const double* vmax, const double* eff,
-                                const int64_t* qk, const double* qs, int64_t nq, double* out,
-                                int* err) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  int64_t d = qk[i];
+// Queries of many keys.  When every query of a block names the same key (a
+// batch of quantiles per key, in key order), the block stages that key's
+// centroids, prefix and state in LDS once and every search runs there; the
+// arithmetic is the same either way.
+constexpr int kQStage = 512;  // centroids a block stages
+
+__global__ __launch_bounds__(256) void k_query_grouped(int is_cdf, int64_t nk, const int64_t* off,
+                                                       const double* m, const double* w,
+                                                       const double* S, const double* total,
+                                                       const double* vmin, const double* vmax,
+                                                       const double* eff, const int64_t* qk,
+                                                       const double* qs, int64_t nq, double* out,
+                                                       int* err) {
+  __shared__ double lm[kQStage], lw[kQStage], lS[kQStage];
+  __shared__ int64_t bkey;
+  __shared__ int mixed;
+  __shared__ QState bst;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t i = i0 + threadIdx.x;
+  const bool live = i < nq;
+  const int64_t d = live ? qk[i] : -1;
+  const double q = live ? qs[i] : 0.0;  // issued before the key's data is needed
+  if (threadIdx.x == 0) {
+    bkey = qk[i0];
+    mixed = 0;
+  }
+  __syncthreads();
+  if (live && d != bkey) mixed = 1;
+  __syncthreads();
+  const int64_t kb = bkey;
+  bool staged = false;
+  int64_t a = 0, nc = 0;
+  if (!mixed && kb >= 0 && kb < nk) {
+    a = off[kb];
+    nc = off[kb + 1] - a;
+    if (nc > 0 && nc <= kQStage) {
+      for (int64_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        lm[c] = m[a + c];
+        lw[c] = w[a + c];
+        lS[c] = S[a + c];
+      }
+      if (threadIdx.x == 0) bst = QState{total[kb], vmin[kb], vmax[kb], eff[kb]};
+      staged = true;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
   int bad = 0;
-  if (d < 0 || d >= nk) bad = 1;
-  else if (!is_cdf && !(qs[i] >= 0.0 && qs[i] <= 1.0)) bad = 2;
-  else if (off[d + 1] == off[d]) bad = 4;
+  if (!staged) {
+    if (d < 0 || d >= nk) {
+      bad = 1;
+    } else {
+      a = off[d];
+      nc = off[d + 1] - a;
+    }
+  }
+  if (!bad && !is_cdf && !(q >= 0.0 && q <= 1.0)) bad = 2;
+  else if (!bad && nc == 0) bad = 4;
   if (bad) {
     out[i] = __builtin_nan("");
     if (err) atomicOr(err, bad);
     return;
   }
-  int64_t a = off[d], nc = off[d + 1] - a;
-  QState st{total[d], vmin[d], vmax[d], eff[d]};
-  out[i] = is_cdf ? cdf_one(m + a, w + a, S + a, nc, st, qs[i])
-                  : quantile_one(m + a, w + a, S + a, nc, st, qs[i]);
+  const QState st = staged ? bst : QState{total[d], vmin[d], vmax[d], eff[d]};
+  const double* pm = staged ? lm : m + a;
+  const double* pw = staged ? lw : w + a;
+  const double* pS = staged ? lS : S + a;
+  out[i] = is_cdf ? cdf_one(pm, pw, pS, nc, st, q) : quantile_one(pm, pw, pS, nc, st, q);
 }
 
 __global__ void k_scale_weights(double* w, int64_t n, double factor) {
