@@ -602,12 +602,20 @@ struct kd_tdigest {
   // Compress E = centroids ++ buffer (++ device values with weight sf).
   SegResult run(const double* d_values, int64_t n, double sf) {
     int64_t ne = nc + (int64_t)bm.size();
-    double* em = s->buf<double>(s->e_m, ne);
-    double* ew = s->buf<double>(s->e_w, ne);
-    s->d2d(em, dm(), nc * sizeof(double));
-    s->d2d(ew, dw(), nc * sizeof(double));
-    s->h2d(em + nc, bm.data(), bm.size() * sizeof(double));
-    s->h2d(ew + nc, bw.data(), bw.size() * sizeof(double));
+    // The engine only reads its explicit records: with nothing buffered on
+    // the host they are the digest's own centroids, read in place.
+    const double* em = dm();
+    const double* ew = dw();
+    if (!bm.empty()) {
+      double* tm = s->buf<double>(s->e_m, ne);
+      double* tw = s->buf<double>(s->e_w, ne);
+      s->d2d(tm, dm(), nc * sizeof(double));
+      s->d2d(tw, dw(), nc * sizeof(double));
+      s->h2d(tm + nc, bm.data(), bm.size() * sizeof(double));
+      s->h2d(tw + nc, bw.data(), bw.size() * sizeof(double));
+      em = tm;
+      ew = tw;
+    }
     bm.clear();
     bw.clear();
     double eff = effective_count();

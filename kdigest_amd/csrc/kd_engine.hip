@@ -1819,10 +1819,15 @@ __device__ inline dd dd_mul_d(dd a, double b) {
   return two_sum(p.hi, p.lo + a.lo * b);
 }
 
-__global__ void k_sel_init(SelState* st, int nsel) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nsel) return;
-  st[j] = SelState{0, 0, 0, 0, ~0ull, 0ull};
+// Per-segment selection state, and the bucket counters and list sizes
+// cleared (grid-stride).
+__global__ void k_sel_init(SelState* st, int nsel, unsigned int* cnt, int64_t ncnt,
+                           unsigned int* mlc, int nmlc) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
+  if (j < nmlc) mlc[j] = 0u;
+  for (int64_t i = j; i < nsel; i += stride) st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull};
 }
 
 // Sample size for a segment of nv new values (about two per splitter).
@@ -3702,6 +3707,10 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
       g.tile_len = sel_tile_len(sd.v_cnt, sel_tile_min_, sel_tiles_max_);
       sd.mode = 1;
       sd.sel = (int32_t)hsel.size();
+      {
+        const int64_t need = std::max<int64_t>(sel_sample_size(sd.v_cnt), g.cap_s);
+        P.sample_cls |= 8 | (need <= 1024 ? 1 : need <= 4096 ? 2 : 4);
+      }
       hsel.push_back(g);
       P.sel_v += (double)sd.v_cnt;
     } else {
@@ -3842,11 +3851,12 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     // Gathered-bucket list size, fallbacks, list sizes per size class.
     dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
     dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
-    check_hip(hipMemsetAsync(dcnt, 0, sel_b * sizeof(unsigned int), stream_), "memset sel counts");
-    check_hip(hipMemsetAsync(dmlc, 0, (2 + kSelClasses) * sizeof(unsigned int), stream_),
-              "memset sel list");
-    k_sel_init<<<(nsel + 255) / 256, 256, 0, stream_>>>(dst, nsel);
-    launched("k_sel_init");
+    {
+      const int64_t items = std::max<int64_t>(std::max<int64_t>(sel_b, nsel), 2 + kSelClasses);
+      const unsigned grid = (unsigned)std::min<int64_t>((items + 255) / 256, 1024);
+      k_sel_init<<<grid, 256, 0, stream_>>>(dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses);
+      launched("k_sel_init");
+    }
     {
       int ph_ = phase_begin("sel_sample", 0.0);
       // The three sample sizes touch disjoint segments: with several
@@ -3858,14 +3868,22 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
         check_hip(hipEventRecord(fork_, stream_), "fork samples");
         check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork samples wait");
       }
-      k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
-      launched("k_sel_sample<1024>");
-      k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
-      launched("k_sel_sample<4096>");
+      // A host plan says which classes have segments; skip the others.
+      const bool cls_known = (P.sample_cls & 8) != 0;
+      if (!cls_known || (P.sample_cls & 1)) {
+        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
+        launched("k_sel_sample<1024>");
+      }
+      if (!cls_known || (P.sample_cls & 2)) {
+        k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
+        launched("k_sel_sample<4096>");
+      }
       // The largest class fills a CU's LDS anyway: a block of kSelThreads.
-      k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
-                                                                          dst);
-      launched("k_sel_sample<8192>");
+      if (!cls_known || (P.sample_cls & 4)) {
+        k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
+                                                                            dst);
+        launched("k_sel_sample<8192>");
+      }
       if (fork_s) {
         check_hip(hipEventRecord(join_, side_), "join samples");
         check_hip(hipStreamWaitEvent(stream_, join_, 0), "join samples wait");
