@@ -333,29 +333,31 @@ __device__ RunOut block_running_sum(double S0, const double* x, double* out, int
   return o;
 }
 
-// Stable sort of the buffer by (mean, position): a bitonic network on the
-// pair (order key of the mean, position) over kRB * kSortE slots, padded with
-// keys above every finite mean.  Thread t holds slots [kSortE t, kSortE t +
-// kSortE) in registers: partners closer than kSortE are in the same thread,
-// closer than 64 kSortE in the same wave (exchanged by lane shuffles); only
-// the widest stages go through LDS.  Leaves the sorted means in skey and
-// their buffer positions in sidx.
+// Stable sort of the buffer by (mean, position): the order of the pair
+// (order key of the mean, position), over kSortSlots slots padded with keys
+// above every finite mean.  Thread t holds slots [kSortE t, kSortE t +
+// kSortE) in registers.  Each wave first sorts its kRun slots ascending with
+// a bitonic network (partners closer than kSortE in the same thread, the
+// others exchanged by lane shuffles), then pairs of runs are merged through
+// LDS, each thread finding its outputs on the merge path.  Leaves the sorted
+// means in skey and their buffer positions in sidx.
 constexpr int kSortSlots = 2048;
 constexpr int kSortE = kSortSlots / kRB;
+constexpr int kRun = 64 * kSortE;
 static_assert(kSortSlots >= kRBMax && kSortE >= 1 && kRB * kSortE == kSortSlots, "sort slots");
 
 __device__ inline bool sort_before(uint64_t ka, int ia, uint64_t kb, int ib) {
   return ka < kb || (ka == kb && ia < ib);
 }
 
-// In-thread stage of partner distance J (< kSortE).
+// In-thread stage of partner distance J (< kSortE); kd: the direction bit.
 template <int J>
-__device__ inline void sort_reg_stage(uint64_t (&key)[kSortE], int (&idx)[kSortE], int base, int k) {
+__device__ inline void sort_reg_stage(uint64_t (&key)[kSortE], int (&idx)[kSortE], int base, int kd) {
 #pragma unroll
   for (int r = 0; r < kSortE; r++) {
     if ((r & J) == 0 && (r | J) < kSortE) {
       const int r2 = r | J;
-      const bool asc = ((base + r) & k) == 0;
+      const bool asc = ((base + r) & kd) == 0;
       const bool sw = asc ? sort_before(key[r2], idx[r2], key[r], idx[r])
                           : sort_before(key[r], idx[r], key[r2], idx[r2]);
       if (sw) {
@@ -380,58 +382,82 @@ __device__ void sort_buffer(Lds& L, int nb) {
     key[r] = i < nb ? ord_key(L.bm[i]) : ~0ull;
     idx[r] = i < nb ? i : 0x7fffffff;
   }
-  uint64_t* sk = reinterpret_cast<uint64_t*>(L.skey);
-  for (int k = 2; k <= kSortSlots; k <<= 1) {
+  // 1. One ascending run of kRun slots per wave.
+  for (int k = 2; k <= kRun; k <<= 1) {
+    const int kd = k == kRun ? 0 : k;  // the last level sorts every run ascending
     for (int j = k >> 1; j > 0; j >>= 1) {
       if (j < kSortE) {
         if constexpr (kSortE > 4) {
-          if (j == 4) sort_reg_stage<4>(key, idx, base, k);
+          if (j == 4) sort_reg_stage<4>(key, idx, base, kd);
         }
         if constexpr (kSortE > 2) {
-          if (j == 2) sort_reg_stage<2>(key, idx, base, k);
+          if (j == 2) sort_reg_stage<2>(key, idx, base, kd);
         }
         if constexpr (kSortE > 1) {
-          if (j == 1) sort_reg_stage<1>(key, idx, base, k);
+          if (j == 1) sort_reg_stage<1>(key, idx, base, kd);
         }
         continue;
       }
-      // Slot p takes the first of (its pair) when it is the lower slot of an
+      // Slot p takes the first of its pair when it is the lower slot of an
       // ascending pair or the upper of a descending one, else the second.
-      uint64_t pk[kSortE];
-      int pi[kSortE];
-      if (j < 64 * kSortE) {
-        const int lm = j / kSortE;
-#pragma unroll
-        for (int r = 0; r < kSortE; r++) {
-          pk[r] = __shfl_xor(key[r], lm, 64);
-          pi[r] = __shfl_xor(idx[r], lm, 64);
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < kSortE; r++) {
-          sk[base + r] = key[r];
-          L.sidx[base + r] = idx[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kSortE; r++) {
-          const int l = (base + r) ^ j;
-          pk[r] = sk[l];
-          pi[r] = L.sidx[l];
-        }
-        __syncthreads();
-      }
+      const int lm = j / kSortE;
 #pragma unroll
       for (int r = 0; r < kSortE; r++) {
+        const uint64_t pk = __shfl_xor(key[r], lm, 64);
+        const int pi = __shfl_xor(idx[r], lm, 64);
         const int p = base + r;
         const bool lower = (p & j) == 0;
-        const bool asc = (p & k) == 0;
-        const bool mine_first = sort_before(key[r], idx[r], pk[r], pi[r]);
+        const bool asc = (p & kd) == 0;
+        const bool mine_first = sort_before(key[r], idx[r], pk, pi);
         const bool take = (lower == asc) ? !mine_first : mine_first;
         if (take) {
-          key[r] = pk[r];
-          idx[r] = pi[r];
+          key[r] = pk;
+          idx[r] = pi;
         }
+      }
+    }
+  }
+  // 2. Runs merged pairwise: outputs base.. of the merged pair come from
+  //    the first i slots of run A and the first (d - i) of run B, i the first
+  //    A index that B's (d - 1 - i)-th slot precedes.
+  uint64_t* sk = reinterpret_cast<uint64_t*>(L.skey);
+  int* si = L.sidx;
+#pragma unroll
+  for (int r = 0; r < kSortE; r++) {
+    sk[base + r] = key[r];
+    si[base + r] = idx[r];
+  }
+  for (int run = kRun; run < kSortSlots; run <<= 1) {
+    __syncthreads();
+    const int rb = base & ~(2 * run - 1), d = base - rb;
+    const uint64_t* ak = sk + rb;
+    const int* ai = si + rb;
+    const uint64_t* bk = sk + rb + run;
+    const int* bi = si + rb + run;
+    int lo = d > run ? d - run : 0, hi = d < run ? d : run;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sort_before(ak[mid], ai[mid], bk[d - 1 - mid], bi[d - 1 - mid])) lo = mid + 1;
+      else hi = mid;
+    }
+    int ia = lo, ib = d - lo;
+#pragma unroll
+    for (int r = 0; r < kSortE; r++) {
+      const int ca = ia < run ? ia : run - 1, cb = ib < run ? ib : run - 1;
+      const uint64_t xa = ak[ca], xb = bk[cb];
+      const int ya = ai[ca], yb = bi[cb];
+      const bool ta = ib >= run || (ia < run && sort_before(xa, ya, xb, yb));
+      key[r] = ta ? xa : xb;
+      idx[r] = ta ? ya : yb;
+      ia += ta ? 1 : 0;
+      ib += ta ? 0 : 1;
+    }
+    __syncthreads();
+    if (2 * run < kSortSlots) {
+#pragma unroll
+      for (int r = 0; r < kSortE; r++) {
+        sk[base + r] = key[r];
+        si[base + r] = idx[r];
       }
     }
   }
