@@ -73,6 +73,12 @@ __device__ inline double lane_value(double v, int l) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
   return kd_bits2d(((uint64_t)hi << 32) | lo);
 }
+__device__ inline int64_t lane_value(int64_t v, int l) {
+  const uint64_t b = (uint64_t)v;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 struct dd {
   double hi, lo;
@@ -2140,7 +2146,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
 constexpr int kChainWaves = 4;  // segments per block
 constexpr int kChainGroups = 6;  // clusters tested per speculative pass
 // First lane of each speculative group (and 64).
-__constant__ const int kChainGroup[kChainGroups + 1] = {0, 4, 12, 22, 34, 48, 64};
+// Lane ranges of the groups (a table the compiler folds into the unrolled
+// group loop).
+__device__ constexpr int kChainGroup[kChainGroups + 1] = {0, 4, 12, 22, 34, 48, 64};
 constexpr int kChainLds = 512;  // prior records staged per segment
 
 __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const SegDesc* segs,
@@ -2266,14 +2274,14 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       uint64_t below = fm;
       if (tm != 0ull) {
         const int ft = __ffsll((long long)tm) - 1;
-        const int64_t cand_ft = __shfl(cand, ft, 64);
-        if (__shfl((int)live, ft, 64) != 0) hi = cand_ft;
+        const int64_t cand_ft = lane_value(cand, ft);
+        if (__builtin_amdgcn_readlane((int)live, ft) != 0) hi = cand_ft;
         below = fm & ((1ull << ft) - 1ull);  // ft < 64; ft == 0 leaves none
       }
       if (below != 0ull) {
         const int lf = 63 - __clzll((long long)below);
-        lo = __shfl(cand, lf, 64);
-        klo = __shfl(kr, lf, 64);
+        lo = lane_value(cand, lf);
+        klo = lane_value(kr, lf);
         klo_set = true;
       }
     }
@@ -2316,6 +2324,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     double ksp = ks;
     int done = 0;
     bool end = false;
+#pragma unroll
     for (int d = 0; d < kChainGroups; d++) {
       const uint64_t gm = ((d + 1 < kChainGroups ? 1ull << kChainGroup[d + 1] : 0ull) - 1ull) &
                           ~((1ull << kChainGroup[d]) - 1ull);
@@ -2324,12 +2333,12 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       const uint64_t tm = __ballot(valid && p) & gm;
       if (tm == 0ull) break;
       const int f = __ffsll((long long)tm) - 1;
-      const int64_t b = __shfl(cand, f, 64);
+      const int64_t b = lane_value(cand, f);
       // Bracketed when b is the first candidate after sp or the lane before
       // f (candidate b - 1 >= sp, evaluated) is in the group: then its
       // k(P(b) / N) is the next start's k.
       double kb;
-      const double kprev = __shfl(kr, f > 0 ? f - 1 : 0, 64);
+      const double kprev = lane_value(kr, f > 0 ? f - 1 : 0);
       if (f == kChainGroup[d]) {
         if (b != sp + 1) break;
         kb = b < n ? kv(b) : 0.0;
