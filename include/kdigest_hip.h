@@ -191,6 +191,16 @@ int kd_tdigest_add_at(kd_tdigest *d, double value, double timestamp, double weig
 int kd_tdigest_add_batch(kd_tdigest *d, const double *values, int64_t n, int with_session);
 /* One batch merge of device-resident values (B semantics, :223-315). */
 int kd_tdigest_add_batch_device(kd_tdigest *d, const double *d_values, int64_t n);
+/* n device-resident values as consecutive batch merges of `batch` values
+ * each (the last one shorter): the same digest as kd_tdigest_add_batch_device
+ * called on each slice in order (B semantics, :223-315, per slice), except
+ * that every value is validated before any slice is merged (a non-finite
+ * value anywhere leaves the digest unchanged).  The work that does not
+ * depend on the merges before it (validation, each slice's splitter sample)
+ * runs for all slices up front.  No counterpart in the reference, whose
+ * callers loop over add_batch. */
+int kd_tdigest_add_batches_device(kd_tdigest *d, const double *d_values, int64_t n,
+                                  int64_t batch);
 int kd_tdigest_merge(kd_tdigest *d);                                          /* :391-439 */
 int kd_tdigest_quantile(kd_tdigest *d, const double *qs, int64_t nq, double *out); /* :485-538 */
 int kd_tdigest_cdf(kd_tdigest *d, const double *xs, int64_t nq, double *out);      /* :540-587 */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "kd_tdigest_add_at": (_i, [_vp, _d, _d, _d]),
     "kd_tdigest_add_batch": (_i, [_vp, _dp, _i64, _i]),
     "kd_tdigest_add_batch_device": (_i, [_vp, _vp, _i64]),
+    "kd_tdigest_add_batches_device": (_i, [_vp, _vp, _i64, _i64]),
     "kd_tdigest_merge": (_i, [_vp]),
     "kd_tdigest_quantile": (_i, [_vp, _dp, _i64, _dp]),
     "kd_tdigest_cdf": (_i, [_vp, _dp, _i64, _dp]),

@@ -262,7 +262,7 @@ unsigned wave_grid(int64_t items) {
 // =================================================================== session
 struct kd_session {
   kd::Engine eng;
-  DevBuf q_off, q_m, q_w, q_s, q_state, q_qd, q_q, q_out, e_m, e_w, v_buf, tmp, tmp2;
+  DevBuf q_off, q_m, q_w, q_s, q_state, q_qd, q_q, q_out, e_m, e_w, v_buf, tmp, tmp2, presort;
   // Grouped-ingest scratch shared by every grouped digest of the session.
   DevBuf g_counts, g_koff, g_vals, g_keys, g_valsin, g_segof, g_xoff;
   // Page-locked segment tables and results, reused by every compress call.
@@ -803,6 +803,40 @@ extern "C" int kd_tdigest_add_batch_device(kd_tdigest* d, const double* d_values
     d->s->sync();
     if (hflag) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
     d->batch(d_values, n, false, 0.0, 0.0);
+    return KD_OK;
+  });
+}
+
+extern "C" int kd_tdigest_add_batches_device(kd_tdigest* d, const double* d_values, int64_t n,
+                                             int64_t batch) {
+  return guarded([&] {
+    if (n <= 0) return KD_OK;
+    if (batch <= 0) throw KdError(KD_EINVAL, "batch size must be positive");
+    int* flag = d->s->buf<int>(d->s->tmp, 1);
+    hip_ok(hipMemsetAsync(flag, 0, sizeof(int), d->s->st()), "memset");
+    k_validate<<<blocks_for(n), 256, 0, d->s->st()>>>(nullptr, d_values, n, 0, flag);
+    launched("k_validate");
+    // Slices that can take the selection path get their splitter samples
+    // sorted up front, all in one launch (they do not depend on the merges).
+    const int64_t nb = (n + batch - 1) / batch;
+    double* ps = nullptr;
+    if (batch > kd::kCap) {
+      ps = d->s->buf<double>(d->s->presort, nb * kd::Engine::kSelSampleMax);
+      d->s->eng.presort_samples(d_values, n, batch, ps);
+    }
+    int hflag = 0;
+    d->s->d2h(&hflag, flag, sizeof(int));
+    d->s->sync();
+    if (hflag) throw KdError(KD_EINVAL, "value must be finite (got NaN or inf)");
+    struct Clear {
+      kd::Engine& e;
+      ~Clear() { e.set_presorted_sample(nullptr); }
+    } clear{d->s->eng};
+    for (int64_t b = 0; b < nb; b++) {
+      const int64_t v0 = b * batch, nv = std::min<int64_t>(batch, n - v0);
+      d->s->eng.set_presorted_sample(ps ? ps + b * kd::Engine::kSelSampleMax : nullptr);
+      d->batch(d_values + v0, nv, false, 0.0, 0.0);
+    }
     return KD_OK;
   });
 }

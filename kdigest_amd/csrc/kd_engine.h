@@ -253,6 +253,16 @@ class Engine {
   }
   bool profiling() const { return prof_; }
 
+  // Consecutive batches of one digest's values (batch values each, the last
+  // one shorter), merged one after the other: their splitter samples do not
+  // depend on the merges before them, so all of them are sorted up front in
+  // one launch (out: kSelSampleMax doubles per batch).  A later compress of
+  // one segment with exactly batch b's values takes batch b's sorted sample
+  // when set_presorted_sample() names it (null: none).
+  static constexpr int kSelSampleMax = 8192;
+  void presort_samples(const double* d_v, int64_t n, int64_t batch, double* out);
+  void set_presorted_sample(const double* p) { presorted_ = p; }
+
  private:
   void sort_values(int nseg, const double* d_v, int64_t total_v);
   // Launches of a planned compress: the segment table, small-class lists,
@@ -267,6 +277,7 @@ class Engine {
   bool defer_pending_ = false;
   DevBuf plan_sum_, plan_part_;
   PlanSummary* h_plan_ = nullptr;  // page-locked copy of the device plan summary
+  const double* presorted_ = nullptr;
 
   int device_;
   hipStream_t stream_ = nullptr;

@@ -1849,22 +1849,11 @@ __host__ __device__ inline int sel_sample_size(int64_t nv) {
 // the sorted prior means (each kept exactly), repeats dropped.  Size classes:
 // this instance takes the segments whose sample size and splitter capacity
 // are at most CAP (and not both at most LO).
-template <int CAP, int BLOCK, int LO>
-__global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const SelSeg* sel,
-                                                      const double* vin, const double* esm,
-                                                      double* spl, SelState* st) {
-  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
-  __shared__ uint32_t part[BLOCK];
-  const int j = blockIdx.x;
-  const SelSeg g = sel[j];
-  const SegDesc sd = segs[g.si];
-  const int64_t nv = sd.v_cnt;
-  const int m = sel_sample_size(nv);
-  const int need = m > g.cap_s ? m : g.cap_s;
-  if (need <= LO || need > CAP) return;
-  const double* v = vin + sd.v_off;
-  // Sample i sits at ((2i + 1) nv) / (2m); eight loads in flight per thread
-  // (each is a separate cache line).
+// The regular sample of a segment's m new values (sample i at
+// ((2i + 1) nv) / (2m)), loaded into sa at padded indices, eight loads in
+// flight per thread (each is a separate cache line).
+template <int BLOCK>
+__device__ inline void sel_load_sample(const double* v, int64_t nv, int m, double* sa) {
   for (int i0 = 0; i0 < m; i0 += 8 * BLOCK) {
     double xs[8];
 #pragma unroll
@@ -1878,8 +1867,52 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
       if (i < m) sa[padi(i)] = xs[u];
     }
   }
+}
+
+// The sorted regular samples of consecutive batches of one digest's values
+// (batch values each, the last one shorter), ahead of their merges: block b
+// sorts batch b's sample as k_sel_sample's largest class would (same block
+// size, same sort) into out + b * kSelSample.
+__global__ __launch_bounds__(kSelThreads) void k_sel_presort(const double* vin, int64_t n,
+                                                             int64_t batch, double* out) {
+  __shared__ double sa[kSelSample + kSelSample / 16], sb[kSelSample + kSelSample / 16];
+  const int64_t v0 = (int64_t)blockIdx.x * batch;
+  const int64_t nv = std::min<int64_t>(batch, n - v0);
+  const int m = sel_sample_size(nv);
+  sel_load_sample<kSelThreads>(vin + v0, nv, m, sa);
   __syncthreads();
-  double* s = lds_sort(sa, sb, m);
+  const double* s = lds_sort(sa, sb, m);
+  double* o = out + (int64_t)blockIdx.x * kSelSample;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) o[i] = s[padi(i)];
+}
+
+// presorted: the segment's sorted sample from k_sel_presort, or null (only
+// the largest class, whose block size and sort k_sel_presort shares, and only
+// for a lone segment: j == 0).
+template <int CAP, int BLOCK, int LO>
+__global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const SelSeg* sel,
+                                                      const double* vin, const double* esm,
+                                                      double* spl, SelState* st,
+                                                      const double* presorted) {
+  __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
+  __shared__ uint32_t part[BLOCK];
+  const int j = blockIdx.x;
+  const SelSeg g = sel[j];
+  const SegDesc sd = segs[g.si];
+  const int64_t nv = sd.v_cnt;
+  const int m = sel_sample_size(nv);
+  const int need = m > g.cap_s ? m : g.cap_s;
+  if (need <= LO || need > CAP) return;
+  double* s;
+  if (presorted) {
+    for (int i = threadIdx.x; i < m; i += BLOCK) sa[padi(i)] = presorted[i];
+    __syncthreads();
+    s = sa;
+  } else {
+    sel_load_sample<BLOCK>(vin + sd.v_off, nv, m, sa);
+    __syncthreads();
+    s = lds_sort(sa, sb, m);
+  }
   double* f = (s == sa) ? sb : sa;
   const int ne = (int)sd.e_cnt;
   int T = (int)std::min<int64_t>(std::max<int64_t>(nv / 32, 63), (int64_t)(g.cap_s - ne));
@@ -3681,6 +3714,15 @@ void Engine::sort_values(int nseg, const double* d_v, int64_t total_v) {
   }
 }
 
+void Engine::presort_samples(const double* d_v, int64_t n, int64_t batch, double* out) {
+  static_assert(kSelSampleMax == kSelSample, "presorted sample size");
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  const int64_t nb = (n + batch - 1) / batch;
+  if (nb <= 0) return;
+  k_sel_presort<<<(unsigned)nb, kSelThreads, 0, stream_>>>(d_v, n, batch, out);
+  launched("k_sel_presort");
+}
+
 void Engine::compress(int scale, SegVec& segs, const double* d_v,
                       const double* d_em, const double* d_ew, ResVec& res,
                       bool sync_results) {
@@ -3880,17 +3922,18 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // A host plan says which classes have segments; skip the others.
       const bool cls_known = (P.sample_cls & 8) != 0;
       if (!cls_known || (P.sample_cls & 1)) {
-        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
+        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst, nullptr);
         launched("k_sel_sample<1024>");
       }
       if (!cls_known || (P.sample_cls & 2)) {
-        k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst);
+        k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst,
+                                                                  nullptr);
         launched("k_sel_sample<4096>");
       }
       // The largest class fills a CU's LDS anyway: a block of kSelThreads.
       if (!cls_known || (P.sample_cls & 4)) {
-        k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, d_v, esm, dspl,
-                                                                            dst);
+        k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(
+            dsegs, dsel, d_v, esm, dspl, dst, nsel == 1 ? presorted_ : nullptr);
         launched("k_sel_sample<8192>");
       }
       if (fork_s) {

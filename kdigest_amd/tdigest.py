@@ -325,6 +325,14 @@ class TDigest:
         n = int(values.numel())
         check(load().kd_tdigest_add_batch_device(self._h, _device_ptr(values), n))
 
+    def add_batches_device(self, values, batch: int) -> None:
+        """Consecutive batch merges of `batch` values each (the last one
+        shorter) of a device-resident float64 tensor: the digest
+        add_batch_device gives slice by slice, with every value validated
+        first and the merge-independent work of all slices done up front."""
+        n = int(values.numel())
+        check(load().kd_tdigest_add_batches_device(self._h, _device_ptr(values), n, int(batch)))
+
     def merge(self) -> None:
         check(load().kd_tdigest_merge(self._h))
 

@@ -345,3 +345,62 @@ def test_grouped_partition_shapes_identical(kd, oracle, items, chunks, nk):
     r.add_batch_b(sel)
     rm, rw = r.centroids()
     check_centroids(m, w, rm, rw, 300.0, True, f"key {k}")
+
+
+@pytest.mark.parametrize("n,batch,lam,pre", [
+    (1 << 20, 1 << 17, 0.0, 0),          # selection path, presorted samples
+    ((1 << 20) + 12345, 1 << 17, 1e-5, 0),  # decay, a short last slice
+    (300_000, 65_536, 2e-3, 37),         # samples buffered on the host first; renormalizes
+    (50_000, 3000, 0.0, 0),              # one-block slices (no presort)
+    (9000, 20_000, 1e-4, 5),             # one slice
+])
+def test_add_batches_device_equals_slices(kd, oracle, n, batch, lam, pre):
+    """kd_tdigest_add_batches_device (consecutive batch merges, splitter
+    samples of every slice sorted up front) gives the digest that
+    add_batch_device gives slice by slice, bit for bit, and the oracle's."""
+    import torch
+
+    rng = np.random.default_rng(n + batch)
+    x = np.exp(1.5 * rng.standard_normal(n))
+    x[rng.integers(0, n, n // 50)] = 1.0  # ties
+    head = rng.standard_normal(pre)
+    dx = torch.from_numpy(x).cuda()
+    a = kd.TDigest(kd.ScaleK2, 200.0, kd.Decay.rate(lam))
+    b = kd.TDigest(kd.ScaleK2, 200.0, kd.Decay.rate(lam))
+    r = oracle.Digest(2, 200.0, lam)
+    for v in head:
+        a.add(float(v))
+        b.add(float(v))
+        r.add(float(v))
+    a.add_batches_device(dx, batch)
+    for s0 in range(0, n, batch):
+        b.add_batch_device(dx[s0:s0 + batch])
+        r.add_batch_b(x[s0:s0 + batch])
+    ma, wa = a.centroids()
+    mb, wb = b.centroids()
+    assert np.array_equal(ma.view(np.uint64), mb.view(np.uint64))
+    assert np.array_equal(wa.view(np.uint64), wb.view(np.uint64))
+    assert a.state() == b.state()
+    rm, rw = r.centroids()
+    check_centroids(ma, wa, rm, rw, float(x.max()), lam == 0.0, "batches")
+    st, rs = a.state(), r.state()
+    for k in ("clock", "log_scale", "min", "max", "stored_total"):
+        assert st[k] == rs[k], (k, st[k], rs[k])
+
+
+def test_add_batches_device_rejects_without_change(kd):
+    import torch
+
+    rng = np.random.default_rng(4)
+    d = kd.TDigest(kd.ScaleK1, 100.0)
+    d.add_batch_device(torch.from_numpy(rng.standard_normal(100_000)).cuda())
+    before = (d.centroids(), d.state())
+    x = rng.standard_normal(400_000)
+    x[350_000] = np.nan  # in the last slice: nothing before it is merged either
+    with pytest.raises(kd.KdigestError):
+        d.add_batches_device(torch.from_numpy(x).cuda(), 1 << 17)
+    (m0, w0), s0 = before
+    m1, w1 = d.centroids()
+    assert np.array_equal(m0, m1) and np.array_equal(w0, w1) and d.state() == s0
+    with pytest.raises(kd.KdigestError):
+        d.add_batches_device(torch.from_numpy(x[:10]).cuda(), 0)
