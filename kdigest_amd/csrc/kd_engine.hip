@@ -1709,6 +1709,7 @@ struct SelSeg {
 struct SelState {
   int32_t nspl, ncl, nmk, fail;
   unsigned long long kmin, kmax;  // order keys of the smallest / largest new value
+  int32_t wdone, pad_;            // 1: the chain left every cluster's running weight in cwv
 };
 
 struct SelMark {
@@ -1833,7 +1834,7 @@ __global__ void k_sel_init(SelState* st, int nsel, unsigned int* cnt, int64_t nc
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
   if (j < nmlc) mlc[j] = 0u;
-  for (int64_t i = j; i < nsel; i += stride) st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull};
+  for (int64_t i = j; i < nsel; i += stride) st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
 }
 
 // Sample size for a segment of nv new values (about two per splitter).
@@ -2188,7 +2189,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
                                                                const SelSeg* sel, int nsel,
                                                                const double* se, const double* esw,
                                                                const int64_t* pose, SelState* st,
-                                                               SelBound* bnd, double* csl) {
+                                                               SelBound* bnd, double* csl,
+                                                               double* cwv) {
   __shared__ int64_t lpos[kChainWaves][kChainLds];
   __shared__ double lse[kChainWaves][kChainLds + 1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2440,6 +2442,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       }
       wt[2 * c] = W;
       wt[2 * c + 1] = Wp;
+      cwv[g.c_off + c] = W;
     }
     __threadfence_block();
     {
@@ -2498,7 +2501,11 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
           if (lane == 0) st[j].fail = 1;
           return;
         }
-        if (s2 >= n - 1) break;
+        const int ci = C - 1;  // the cluster starting at s2
+        if (s2 >= n - 1) {
+          if (lane == 0) cwv[g.c_off + ci] = run_weight(s2, n, sf, pos, ew, ne, seek(s2));
+          break;
+        }
         cur = seek(s2);
         double kn;
         int64_t b = exact_next(s2, kv(s2), &kn);
@@ -2509,11 +2516,17 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
         };
         while (b < n && !closes(b)) b++;
         while (b - 1 > s2 && closes(b - 1)) b--;
-        if (b >= n) break;
-        wsf = wsf + run_weight(s2, b, sf, pos, ew, ne, ce);
+        if (b >= n) {
+          if (lane == 0) cwv[g.c_off + ci] = run_weight(s2, n, sf, pos, ew, ne, ce);
+          break;
+        }
+        const double wc = run_weight(s2, b, sf, pos, ew, ne, ce);
+        if (lane == 0) cwv[g.c_off + ci] = wc;
+        wsf = wsf + wc;
         s2 = b;
       }
     }
+    if (lane == 0) st[j].wdone = 1;
   }
   if (lane == 0) st[j].ncl = C;
 }
@@ -3255,8 +3268,10 @@ __device__ inline dd sel_prefix(const SelMark& m, int64_t r, const SelBound* bd,
 }
 
 // Tie checks at every boundary, then each cluster's weight and mean.
+// cwv: the running weights the chain verified (when st[j].wdone).
 __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const SelSeg* sel,
-                                                         const SelState* st, const SelBound* bnd,
+                                                         const SelState* st, const double* cwv,
+                                                         const SelBound* bnd,
                                                          const SelMark* mk, const int* binfo,
                                                          const unsigned int* bst, const double* csl,
                                                          const double* esm, const double* esw,
@@ -3310,8 +3325,9 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
     }
     dd wsum = dd_add(two_prod(sd.sf, (double)nV), ewt);
     double w = wsum.hi + wsum.lo;
-    // Stored weight: the running weight in record order, as the CPU sums it.
-    const double wrun = run_weight(ba.b, bb, sd.sf, pos, ew, ne, ea);
+    // Stored weight: the running weight in record order, as the CPU sums it
+    // (the chain's when it checked the boundaries with it).
+    const double wrun = st[j].wdone ? cwv[g.c_off + c] : run_weight(ba.b, bb, sd.sf, pos, ew, ne, ea);
     dd sv = two_sum(csl[2 * (g.c_off + c)], csl[2 * (g.c_off + c) + 1]);
     if (nV > 0) {
       int bf = sel_bucket_of_rank(B, S, vra), bl = sel_bucket_of_rank(B, S, vrb - 1);
@@ -3879,7 +3895,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   const int64_t t_small = P.t_small, t_mid = P.t_mid, sel_t = P.sel_t;
   SelSeg* dsel = static_cast<SelSeg*>(sel_seg_.p);
   SelState* dst = nullptr;
-  double *dspl = nullptr, *dcsl = nullptr;
+  double *dspl = nullptr, *dcsl = nullptr, *dcwv = nullptr;
   unsigned int *dcnt = nullptr, *dbst = nullptr, *dmlc = nullptr;
   int *dinfo = nullptr, *dce = nullptr;
   SelBound* dbnd = nullptr;
@@ -3896,6 +3912,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     dbnd = as<SelBound>(sel_bnd_, sel_c);
     dce = as<int>(sel_ce_, sel_c);
     dcsl = as<double>(sel_csl_, 2 * sel_c);
+    dcwv = as<double>(sel_cwv_, sel_c);
     dmk = as<SelMark>(sel_mk_, sel_m);
     dml = as<int2>(sel_ml_, sel_m);
     dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
@@ -4007,7 +4024,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
                                                      dst);
       launched("k_sel_ranks");
       k_sel_chain<<<(nsel + kChainWaves - 1) / kChainWaves, 64 * kChainWaves, 0, stream_>>>(
-          scale, dsegs, dsel, nsel, se, esw, pose, dst, dbnd, dcsl);
+          scale, dsegs, dsel, nsel, se, esw, pose, dst, dbnd, dcsl, dcwv);
       launched("k_sel_chain");
       k_sel_plan<<<nsel, kSelThreads, 0, stream_>>>(dsegs, dsel, dspl, dcnt, dbst, dinfo, pose, dst,
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);
@@ -4072,7 +4089,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     }
     {
       int ph_ = phase_begin("sel_finalize", 0.0);
-      k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dbnd, dmk, dinfo, dbst, dcsl,
+      k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dcwv, dbnd, dmk, dinfo, dbst, dcsl,
                                                    esm, esw, pose, cm, cw, ccount, dres, dmlc + 1);
       launched("k_sel_finalize");
       phase_end(ph_);
