@@ -275,6 +275,67 @@ __device__ inline double* lds_sort(double* a, double* b, int n) {
 }
 
 // ---------------------------------------------------- 1. explicit records
+// Stable sort of n <= 64 PER explicit records by (mean, input index) in one
+// wave: PER slots per lane in registers, a bitonic network whose partners
+// closer than PER are in the same lane and the others one shuffle away.
+// Writes the sorted order to key / idx (slots 0 .. 64 PER - 1).
+template <int PER>
+__device__ inline void sort_e_wave(const double* em, int n, double* key, int* idx) {
+  const int lane = threadIdx.x & 63, base = lane * PER;
+  double kk[PER];
+  int ii[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int i = base + r;
+    kk[r] = i < n ? em[i] : __builtin_inf();
+    ii[r] = i < n ? i : 0x7fffffff;
+  }
+  auto first = [](double a, int ia, double b, int ib) { return (a < b) || (!(b < a) && ia < ib); };
+  for (int k = 2; k <= 64 * PER; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < PER) {
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+#pragma unroll
+          for (int jj = 1; jj < PER; jj <<= 1) {
+            if (jj == j && (r & jj) == 0) {
+              const int r2 = r | jj;
+              const bool asc = ((base + r) & k) == 0;
+              const bool sw = asc ? !first(kk[r], ii[r], kk[r2], ii[r2]) : first(kk[r], ii[r], kk[r2], ii[r2]);
+              if (sw) {
+                const double tk = kk[r];
+                kk[r] = kk[r2];
+                kk[r2] = tk;
+                const int ti = ii[r];
+                ii[r] = ii[r2];
+                ii[r2] = ti;
+              }
+            }
+          }
+        }
+        continue;
+      }
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const double pk = __shfl_xor(kk[r], j / PER, 64);
+        const int pi = __shfl_xor(ii[r], j / PER, 64);
+        const int p = base + r;
+        const bool lower = (p & j) == 0, asc = (p & k) == 0;
+        const bool mine = first(kk[r], ii[r], pk, pi);
+        if ((lower == asc) ? !mine : mine) {
+          kk[r] = pk;
+          ii[r] = pi;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    key[base + r] = kk[r];
+    idx[base + r] = ii[r];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg,
                                                     const double* em, const double* ew,
                                                     double* esm, double* esw, double* se) {
@@ -289,54 +350,13 @@ __global__ __launch_bounds__(kBlock) void k_sort_e(const SegDesc* segs, int nseg
     return;
   }
   int npad = pow2_at_least(n);
-  if (n <= 128) {
-    // One wave, two slots per lane in registers: the same network over 128
+  if (n <= 512) {
+    // One wave, 2-8 slots per lane in registers: the same network over 64 PER
     // slots, partners across lanes exchanged by shuffles (no barriers).
     if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
-      double kk[2];
-      int ii[2];
-#pragma unroll
-      for (int r = 0; r < 2; r++) {
-        const int i = 2 * lane + r;
-        kk[r] = i < n ? em[sd.e_off + i] : __builtin_inf();
-        ii[r] = i < n ? i : 0x7fffffff;
-      }
-      auto first = [](double a, int ia, double b, int ib) { return (a < b) || (!(b < a) && ia < ib); };
-      for (int k = 2; k <= 128; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-          if (j == 1) {
-            const bool asc = ((2 * lane) & k) == 0;
-            const bool sw = asc ? !first(kk[0], ii[0], kk[1], ii[1]) : first(kk[0], ii[0], kk[1], ii[1]);
-            if (sw) {
-              const double tk = kk[0];
-              kk[0] = kk[1];
-              kk[1] = tk;
-              const int ti = ii[0];
-              ii[0] = ii[1];
-              ii[1] = ti;
-            }
-            continue;
-          }
-#pragma unroll
-          for (int r = 0; r < 2; r++) {
-            const double pk = __shfl_xor(kk[r], j >> 1, 64);
-            const int pi = __shfl_xor(ii[r], j >> 1, 64);
-            const int p = 2 * lane + r;
-            const bool lower = (p & j) == 0, asc = (p & k) == 0;
-            const bool mine = first(kk[r], ii[r], pk, pi);
-            if ((lower == asc) ? !mine : mine) {
-              kk[r] = pk;
-              ii[r] = pi;
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 2; r++) {
-        key[2 * lane + r] = kk[r];
-        idx[2 * lane + r] = ii[r];
-      }
+      if (n <= 128) sort_e_wave<2>(em + sd.e_off, n, key, idx);
+      else if (n <= 256) sort_e_wave<4>(em + sd.e_off, n, key, idx);
+      else sort_e_wave<8>(em + sd.e_off, n, key, idx);
     }
     __syncthreads();
     npad = 0;  // sorted: the network below does nothing
