@@ -245,6 +245,7 @@ int64_t Server::step(int max_rounds) {
   std::deque<int64_t> keep;
   bool budget_full = false;
   int64_t n_empty = 0;
+  std::unique_lock<std::mutex> lk(mu_);  // submit() may run on other threads
   for (int64_t id : pending_) {
     Request& r = reqs_.at(id);
     bool admit = false;
@@ -286,19 +287,37 @@ int64_t Server::step(int max_rounds) {
   steps_++;
   admitted_bytes_ += bytes;
   max_step_bytes_ = std::max(max_step_bytes_, bytes);
+  lk.unlock();
   int64_t done = n_empty;
-  if (!queries.empty()) {
-    run_queries(queries, eng_.stream());
-    for (Request* r : queries) finish(*r);
-    done += (int64_t)queries.size();
+  try {
+    if (!queries.empty()) {
+      run_queries(queries, eng_.stream());
+      std::lock_guard<std::mutex> g(mu_);
+      for (Request* r : queries) finish(*r);
+      done += (int64_t)queries.size();
+    }
+    for (auto& round : rounds) {
+      if (round.empty()) continue;
+      run_ingest_round(round);
+      std::lock_guard<std::mutex> g(mu_);
+      for (Request* r : round) finish(*r);
+      done += (int64_t)round.size();
+    }
+    check_hip(hipStreamSynchronize(eng_.stream()), "sync");
+  } catch (const std::exception& e) {
+    // The step's requests that did not complete fail with the error.
+    std::lock_guard<std::mutex> g(mu_);
+    auto fail = [&](Request* r) {
+      if (r->done) return;
+      r->err = KD_EDEVICE_SERVE;
+      r->msg = e.what();
+      finish(*r);
+    };
+    for (Request* r : queries) fail(r);
+    for (auto& round : rounds)
+      for (Request* r : round) fail(r);
+    throw;
   }
-  for (auto& round : rounds) {
-    if (round.empty()) continue;
-    run_ingest_round(round);
-    for (Request* r : round) finish(*r);
-    done += (int64_t)round.size();
-  }
-  check_hip(hipStreamSynchronize(eng_.stream()), "sync");
   if (eng_.profiling()) eng_.host_phase("serve_step", ms_since(t_step));
   return done;
 }
