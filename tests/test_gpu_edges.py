@@ -189,3 +189,60 @@ def test_from_bytes_rejects_bad_counts(kd, count, code):
     assert e.value.code == code
     ok = kd.TDigest.from_bytes(_blob(2.0, [(1.0, 1.0), (2.0, 2.0)]), kd.ScaleK1)
     assert len(ok) == 2 and ok.quantile(0.0) == 1.0
+
+
+@pytest.mark.parametrize("scale,delta", [(1, 100.0), (0, 2000.0)])
+def test_grouped_device_queries_staged_equal_unstaged(kd, scale, delta):
+    """A block of device queries that all name one key stages that key's
+    centroids in LDS (keys of up to 512 centroids); the same queries in a
+    shuffled order take the unstaged path.  Both give the same bits, for
+    quantiles and cdfs, including keys of more than 512 centroids (ScaleK0,
+    compression 2000) that a uniform block does not stage."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(scale + 7)
+    nk = 24
+    sizes = rng.integers(1, 200_000, nk)
+    sizes[5] = 3          # tiny
+    keys = np.repeat(np.arange(nk, dtype=np.uint32), sizes)
+    vals = rng.standard_t(4, keys.size) + keys.astype(np.float64)
+    perm = rng.permutation(keys.size)
+    sc = [kd.ScaleK0, kd.ScaleK1, kd.ScaleK2, kd.ScaleK3][scale]
+    g = kd.GroupedTDigest(nk, delta, scale=sc)
+    g.add_batch(keys[perm], vals[perm])
+    if scale == 0:
+        assert max(len(g.centroids(k)[0]) for k in range(nk)) > 512
+    per = 1000  # not a multiple of the block size: some blocks straddle two keys
+    qk = np.repeat(np.arange(nk, dtype=np.int64), per)
+    qs = rng.uniform(0, 1, qk.size)
+    qs[::97] = 0.0
+    qs[::89] = 1.0
+    xs = np.concatenate([g.quantile(np.full(per, k, np.int64), rng.uniform(0, 1, per))
+                         for k in range(nk)])
+    shuf = rng.permutation(qk.size)
+    for fn, arg in (("quantile_device", qs), ("cdf_device", xs)):
+        if not hasattr(g, fn):
+            continue
+        dev = getattr(g, fn)
+        a = torch.empty(qk.size, dtype=torch.float64, device="cuda")
+        b = torch.empty(qk.size, dtype=torch.float64, device="cuda")
+        dev(torch.from_numpy(qk).cuda(), torch.from_numpy(arg).cuda(), a)
+        dev(torch.from_numpy(qk[shuf]).cuda(), torch.from_numpy(arg[shuf]).cuda(), b)
+        got_sorted = a.cpu().numpy()
+        got_shuf = np.empty(qk.size)
+        got_shuf[shuf] = b.cpu().numpy()
+        assert np.array_equal(got_sorted.view(np.uint64), got_shuf.view(np.uint64)), fn
+
+
+def test_grouped_device_queries_empty_key_block(kd):
+    """A block whose queries all name an empty key is not staged: every
+    query comes back NaN with KD_EEMPTY (tdigest.mojo:485-487)."""
+    torch = pytest.importorskip("torch")
+    g = kd.GroupedTDigest(3, 100.0, scale=kd.ScaleK1)
+    g.add_batch(np.array([0, 2, 2], np.uint32), np.array([1.0, 2.0, 3.0]))
+    k = torch.full((300,), 1, dtype=torch.int64, device="cuda")
+    q = torch.full((300,), 0.5, dtype=torch.float64, device="cuda")
+    out = torch.zeros(300, dtype=torch.float64, device="cuda")
+    with pytest.raises(kd.KdigestError) as e:
+        g.quantile_device(k, q, out)
+    assert e.value.code == -2
+    assert np.isnan(out.cpu().numpy()).all()
