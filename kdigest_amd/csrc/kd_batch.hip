@@ -181,6 +181,7 @@ __device__ RunOut block_running_sum(double S0, const double* x, double* out, int
     rs.done = 0;
     rs.consumed = m;
     rs.stopped = 0;
+    rs.stopv = 0.0;
   }
   __syncthreads();
   const int none = 2 * m + 1;
