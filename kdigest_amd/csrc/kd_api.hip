@@ -271,6 +271,10 @@ struct kd_session {
   // Set once the caller fixes a launch shape itself: the shapes the
   // autotuner recorded are then no longer applied to this session.
   bool shape_pinned = false;
+  // Centroid total after the last grouped batch (keys, pairs) of the
+  // session: the first guess of the next CSR's size for a batch of that
+  // shape.
+  int64_t csr_keys = -1, csr_pairs = -1, csr_total = 0;
   explicit kd_session(int dev) : eng(dev) {}
   hipStream_t st() { return eng.stream(); }
   template <typename T>
@@ -1078,11 +1082,14 @@ struct kd_grouped {
 namespace {
 // Per segment of a grouped batch: min / max (grouped.mojo via
 // tdigest.mojo:285-295) and the reconciled stored total (:298-315).
+// only: -1 every segment, 0 those on the sort path (mode 0), 1 the others.
 __global__ void k_grouped_keys(int64_t nseg, const kd::SegDesc* segs, const kd::SegResult* res,
-                               const int64_t* seg_key, double* total, double* vmin, double* vmax) {
+                               const int64_t* seg_key, double* total, double* vmin, double* vmax,
+                               int only) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nseg) return;
   const kd::SegDesc sd = segs[i];
+  if (only >= 0 && (sd.mode == 0) != (only == 0)) return;
   const kd::SegResult r = res[i];
   const int64_t k = seg_key[i];
   if (sd.stored_n == (double)sd.v_cnt * sd.sf && sd.e_cnt == 0) {
@@ -1095,13 +1102,21 @@ __global__ void k_grouped_keys(int64_t nseg, const kd::SegDesc* segs, const kd::
   total[k] = r.total;
 }
 
-// Centroid count of every key after the batch.
+// Whether segment sg is left out under only (-1 none, 0 all but the sort
+// path's, 1 the sort path's).
+__device__ inline bool seg_skipped(const kd::SegDesc* segs, int64_t sg, int only) {
+  return only >= 0 && (segs[sg].mode == 0) != (only == 0);
+}
+
+// Centroid count of every key after the batch (0 for a key whose segment is
+// left out under only).
 __global__ void k_grouped_counts(int64_t nk, const int64_t* off, const int64_t* seg_of,
-                                 const kd::SegResult* res, int64_t* cnt) {
+                                 const kd::SegDesc* segs, const kd::SegResult* res, int only,
+                                 int64_t* cnt) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nk) return;
   int64_t sg = seg_of[k];
-  cnt[k] = sg >= 0 ? res[sg].count : off[k + 1] - off[k];
+  cnt[k] = sg >= 0 ? (seg_skipped(segs, sg, only) ? 0 : res[sg].count) : off[k + 1] - off[k];
 }
 
 // Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total) in three
@@ -1167,16 +1182,20 @@ __global__ __launch_bounds__(1024) void k_off_down(const int64_t* cnt, int64_t n
 }
 
 // New CSR: one wave per key copies its centroids (the engine's output for a
-// touched key, the old ones otherwise).
+// touched key, the old ones otherwise).  Nothing is written when the new
+// total exceeds cap, the capacity of nm and nw, nor for a key whose segment
+// is left out under only (its count is 0 then).
 __global__ void k_gather_keys(int64_t nk, const int64_t* old_off, const double* om,
                               const double* ow, const int64_t* seg_of, const kd::SegDesc* segs,
                               const double* em, const double* ew, const int64_t* new_off,
-                              double* nm, double* nw) {
+                              double* nm, double* nw, int64_t cap, int only) {
+  if (new_off[nk] > cap) return;
   const int lane = threadIdx.x & 63;
   const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nk; k += stride) {
     const int64_t dst = new_off[k], cnt = new_off[k + 1] - dst;
     const int64_t sg = seg_of[k];
+    if (sg >= 0 && seg_skipped(segs, sg, only)) continue;
     const double* sm = sg >= 0 ? em + segs[sg].x_off : om + old_off[k];
     const double* sw = sg >= 0 ? ew + segs[sg].x_off : ow + old_off[k];
     for (int64_t i = lane; i < cnt; i += 64) {
@@ -1276,7 +1295,9 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   a.bad = flag;
   a.seg_key = dsk;
   a.seg_of = dso;
-  const kd::PlanSummary P = s->eng.compress_grouped(a, dv, dm(), dw());
+  // Selection fallbacks (rare) are finished after the sync below, not with
+  // a sync of their own.
+  const kd::PlanSummary P = s->eng.compress_grouped(a, dv, dm(), dw(), true);
   s->eng.host_phase("compress", ms_since(tp));
   if (P.bad == 1) throw KdError(KD_EINVAL, "key out of range [0, num_keys) or non-finite value");
   if (P.bad) throw std::runtime_error("too many explicit records in one digest");
@@ -1286,32 +1307,63 @@ void kd_grouped::add_device(const uint32_t* d_keys, const double* d_values, int6
   // their centroids).
   const kd::SegDesc* segs = s->eng.dev_segments();
   const kd::SegResult* res = s->eng.dev_results();
-  k_grouped_keys<<<blocks_for(P.nseg), 256, 0, s->st()>>>(P.nseg, segs, res, dsk, dstate(0),
-                                                          dstate(3), dstate(4));
-  launched("k_grouped_keys");
   int64_t* dc = static_cast<int64_t*>(ncnt.reserve(num_keys * sizeof(int64_t)));
-  k_grouped_counts<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dso, res, dc);
-  launched("k_grouped_counts");
   int64_t* dno = static_cast<int64_t*>(noff.reserve((num_keys + 1) * sizeof(int64_t)));
   const int64_t nb = (num_keys + kOffChunk - 1) / kOffChunk;
   int64_t* bsum = static_cast<int64_t*>(tot.reserve((nb + 1) * sizeof(int64_t)));
-  k_off_sums<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum);
-  launched("k_off_sums");
-  k_off_top<<<1, kOffChunk, 0, s->st()>>>(bsum, nb, dno + num_keys);
-  launched("k_off_top");
-  k_off_down<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum, dno);
-  launched("k_off_down");
-  // The exact new total sizes the next CSR.
-  s->d2h(h_tot, dno + num_keys, sizeof(int64_t));
-  s->sync();
+  // The new total is known on the device only.  The next CSR is sized from
+  // a guess (the current total, or the session's last batch of this shape,
+  // at most the bound total + n: a key ends with at most its old centroids
+  // plus its new values) and gathered at once; the gather writes nothing
+  // when the total exceeds the guess, and then runs again after the sync,
+  // with the exact size.
+  int64_t guess = total_cent;
+  if (s->csr_keys == num_keys && s->csr_pairs == n) guess = std::max(guess, s->csr_total);
+  guess = std::max<int64_t>(std::min<int64_t>(guess, total_cent + n), 1);
+  // The key state of the segments under only_keys, and the CSR with the
+  // segments under only_csr (-1 every one; with fallen-back segments
+  // pending, 1 all but those, which count as empty; 0 those).
+  auto gather = [&](int64_t cap_items, int only_csr) {
+    double* dnm = static_cast<double*>(ncm.reserve(cap_items * sizeof(double)));
+    double* dnw = static_cast<double*>(ncw.reserve(cap_items * sizeof(double)));
+    const int64_t cap = (int64_t)(std::min(ncm.cap, ncw.cap) / sizeof(double));
+    k_gather_keys<<<wave_grid(num_keys), 256, 0, s->st()>>>(
+        num_keys, doff(), dm(), dw(), dso, segs, s->eng.cent_means(), s->eng.cent_weights(), dno,
+        dnm, dnw, cap, only_csr);
+    launched("k_gather_keys");
+    return cap;
+  };
+  auto rebuild = [&](int only_keys, int only_csr, int64_t cap_items) {
+    k_grouped_keys<<<blocks_for(P.nseg), 256, 0, s->st()>>>(P.nseg, segs, res, dsk, dstate(0),
+                                                            dstate(3), dstate(4), only_keys);
+    launched("k_grouped_keys");
+    k_grouped_counts<<<blocks_for(num_keys), 256, 0, s->st()>>>(num_keys, doff(), dso, segs, res,
+                                                                only_csr, dc);
+    launched("k_grouped_counts");
+    k_off_sums<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum);
+    launched("k_off_sums");
+    k_off_top<<<1, kOffChunk, 0, s->st()>>>(bsum, nb, dno + num_keys);
+    launched("k_off_top");
+    k_off_down<<<(unsigned)nb, kOffChunk, 0, s->st()>>>(dc, num_keys, bsum, dno);
+    launched("k_off_down");
+    const int64_t cap = gather(cap_items, only_csr);
+    s->d2h(h_tot, dno + num_keys, sizeof(int64_t));
+    s->sync();
+    return cap;
+  };
+  const int first = s->eng.grouped_deferred() ? 1 : -1;
+  int64_t cap = rebuild(first, first, guess);
+  // Fallen-back segments finished now: their keys' state, then every count
+  // and the gather again.
+  if (s->eng.finish_grouped(scale, P, dv, dm(), dw())) cap = rebuild(0, -1, guess);
   const int64_t new_total = *h_tot;
-  double* dnm = static_cast<double*>(ncm.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
-  double* dnw = static_cast<double*>(ncw.reserve(std::max<int64_t>(new_total, 1) * sizeof(double)));
-  k_gather_keys<<<wave_grid(num_keys), 256, 0, s->st()>>>(
-      num_keys, doff(), dm(), dw(), dso, segs, s->eng.cent_means(), s->eng.cent_weights(), dno,
-      dnm, dnw);
-  launched("k_gather_keys");
-  s->sync();
+  if (new_total > cap) {
+    gather(new_total, -1);
+    s->sync();
+  }
+  s->csr_keys = num_keys;
+  s->csr_pairs = n;
+  s->csr_total = new_total;
   off.swap(noff);
   cm.swap(ncm);
   cw.swap(ncw);

@@ -163,9 +163,16 @@ class Engine {
   // (per-key prepare, segment table, classes) runs in one kernel and only
   // its summary comes back to the host; the results stay on the device
   // (dev_results(), dev_segments()).  Returns the summary; summary.bad != 0
-  // means nothing was changed or launched.
+  // means nothing was changed or launched.  With defer, segments of the
+  // selection path that fall back to sorting are left unfinished (the host
+  // learns of them only after a sync): grouped_deferred() tells the caller
+  // that their results are not final yet, and after its own sync the caller
+  // runs finish_grouped(), which finishes them (true) or finds none (false).
   PlanSummary compress_grouped(const GroupedPlanArgs& a, const double* d_v, const double* d_em,
-                               const double* d_ew);
+                               const double* d_ew, bool defer = false);
+  bool grouped_deferred() const { return defer_pending_; }
+  bool finish_grouped(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
+                      const double* d_ew);
   const SegDesc* dev_segments() const { return static_cast<const SegDesc*>(seg_.p); }
 
   const double* cent_means() const { return static_cast<const double*>(cm_.p); }
@@ -275,6 +282,7 @@ class Engine {
   void execute(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
                const double* d_ew, int part = 0);
   bool defer_pending_ = false;
+  bool fb_count_pending_ = false;  // the fallback count of a deferred grouped batch is unread
   DevBuf plan_sum_, plan_part_;
   PlanSummary* h_plan_ = nullptr;  // page-locked copy of the device plan summary
   const double* presorted_ = nullptr;

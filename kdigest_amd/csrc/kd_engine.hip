@@ -4534,8 +4534,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_write(GroupedPlanArgs a, 
 }
 
 PlanSummary Engine::compress_grouped(const GroupedPlanArgs& a, const double* d_v,
-                                     const double* d_em, const double* d_ew) {
+                                     const double* d_em, const double* d_ew, bool defer) {
   check_hip(hipSetDevice(device_), "hipSetDevice");
+  defer_pending_ = false;
   const int64_t max_seg = std::max<int64_t>(a.num_keys, 1);
   SegDesc* dsegs = as<SegDesc>(seg_, max_seg);
   int* dl = as<int>(small_list_, max_seg);
@@ -4568,20 +4569,40 @@ PlanSummary Engine::compress_grouped(const GroupedPlanArgs& a, const double* d_v
   check_hip(hipStreamSynchronize(stream_), "plan sync");
   const PlanSummary P = *h_plan_;
   if (P.bad || P.nseg == 0) return P;
-  execute(a.scale, P, d_v, d_em, d_ew);
-  // execute() reads the fallback count back only when it decides the sort
-  // path from it; otherwise it is still in flight.
-  if (P.nsel > 0 && P.n_mode0 > 0) check_hip(hipStreamSynchronize(stream_), "fallback sync");
   int64_t n_small = 0;
   for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
   path_counts_[2] += n_small;
   path_counts_[0] += P.nsel;
   path_counts_[1] += P.n_mode0;
+  if (defer) {
+    execute(a.scale, P, d_v, d_em, d_ew, 1);
+    // The fallback count is still in flight: counted by finish_grouped.
+    fb_count_pending_ = P.nsel > 0;
+    return P;
+  }
+  execute(a.scale, P, d_v, d_em, d_ew);
+  // execute() reads the fallback count back only when it decides the sort
+  // path from it; otherwise it is still in flight.
+  if (P.nsel > 0 && P.n_mode0 > 0) check_hip(hipStreamSynchronize(stream_), "fallback sync");
   if (P.nsel > 0 && *h_fallbacks_ > 0) {  // read by execute after its fallback sync
     path_counts_[0] -= *h_fallbacks_;
     path_counts_[1] += *h_fallbacks_;
   }
   return P;
+}
+
+bool Engine::finish_grouped(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
+                            const double* d_ew) {
+  // The caller has synchronized the stream: the fallback count is here.
+  if (fb_count_pending_ && *h_fallbacks_ > 0) {
+    path_counts_[0] -= *h_fallbacks_;
+    path_counts_[1] += *h_fallbacks_;
+  }
+  fb_count_pending_ = false;
+  const bool run = defer_pending_ && *h_fallbacks_ > 0;
+  defer_pending_ = false;
+  if (run) execute(scale, P, d_v, d_em, d_ew, 2);
+  return run;
 }
 
 // ------------------------------------------------------ key partition

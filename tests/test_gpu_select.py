@@ -195,6 +195,72 @@ def test_select_grouped_zipf(kd, oracle, sessions):
         check_centroids(ma, wa, rm, rw, 300.0, True, f"key {k} sel vs oracle")
 
 
+@pytest.mark.parametrize("lam", [0.0, 2e-5])
+def test_select_grouped_fallbacks(kd, oracle, lam):
+    """Grouped batches whose large digests partly fall back from the
+    selection path to sorting (few distinct values: boundaries tie): those
+    digests are finished after the batch's sync, and every key (state
+    included) equals the oracle's, before and after the fallen-back
+    digests' keys take prior centroids."""
+    rng = np.random.default_rng(int(lam * 1e6) + 31)
+    s = kd.GpuSession(0)
+    nk = 24
+    g = kd.GroupedTDigest(nk, 100.0, kd.Decay.rate(lam), scale=kd.ScaleK1, session=s)
+    refs = [oracle.Digest(1, 100.0, lam) for _ in range(nk)]
+    for batch in range(2):
+        # Keys 0-5 tied (selection fallbacks), 6-11 large and distinct
+        # (selection), 12-23 small (one block).
+        parts_k, parts_v = [], []
+        for k in range(nk):
+            n = 60000 + 1000 * k if k < 12 else 500 + 37 * k
+            x = rng.standard_normal(n) * 3.0 + k
+            if k < 6:
+                x = np.round(x)
+            parts_k.append(np.full(n, k, np.uint32))
+            parts_v.append(x)
+        keys = np.concatenate(parts_k)
+        vals = np.concatenate(parts_v)
+        perm = rng.permutation(keys.size)
+        keys, vals = keys[perm], vals[perm]
+        before = s.path_counts()
+        g.add_batch(keys, vals)
+        after = s.path_counts()
+        oracle.grouped_add_batch(refs, keys, vals, threads=8)
+        assert after["sorted"] > before["sorted"], (batch, before, after)
+        assert after["selection"] > before["selection"], (batch, before, after)
+        for k in range(nk):
+            m, w = g.centroids(k)
+            rm, rw = refs[k].centroids()
+            check_centroids(m, w, rm, rw, 40.0, lam == 0.0, f"batch {batch} key {k}")
+            st, rs = g.key_state(k), refs[k].state()
+            for f in ("min", "max", "stored_total", "clock", "log_scale"):
+                assert st[f] == rs[f], (batch, k, f, st[f], rs[f])
+
+
+def test_grouped_csr_larger_than_last_batch(kd, oracle):
+    """A batch of the same shape as the session's last one that ends with
+    more centroids than it did (so the next CSR, sized from that batch,
+    is gathered again with its exact size) equals the oracle."""
+    s = kd.GpuSession(0)
+    nk, n = 50, 200000
+    rng = np.random.default_rng(8)
+    keys = rng.integers(0, nk, n).astype(np.uint32)
+    few = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK2, session=s)
+    few.add_batch(keys, np.round(rng.standard_normal(n)))  # few distinct values: few centroids
+    g = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK2, session=s)
+    refs = [oracle.Digest(2, 100.0) for _ in range(nk)]
+    vals = rng.standard_normal(n) + keys
+    g.add_batch(keys, vals)
+    oracle.grouped_add_batch(refs, keys, vals, threads=8)
+    total = 0
+    for k in range(nk):
+        m, w = g.centroids(k)
+        rm, rw = refs[k].centroids()
+        check_centroids(m, w, rm, rw, 60.0, True, f"key {k}")
+        total += m.size
+    assert total > sum(few.centroid_count(k) for k in range(nk))
+
+
 def test_one_block_path_matches_sort_path(kd, oracle):
     """Digests of at most 4096 merged records compressed in one block give
     the multi-kernel path's clusters bit for bit (same arithmetic, same
