@@ -1808,16 +1808,6 @@ __device__ inline int sel_bucket_of_rank(const unsigned int* bst, int S, int64_t
   return lo;
 }
 
-__device__ inline int sel_tile_seg(const SelSeg* sel, int nsel, int64_t t) {
-  int lo = 0, hi = nsel - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (sel[mid].tile0 <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // 1.5 * 2^k: adding and subtracting it rounds to a multiple of 2^(k - 52).
 __device__ inline double sel_magic(int k) {
   return kd_bits2d(((uint64_t)(k + 1023) << 52) | 0x0008000000000000ull);
@@ -1846,15 +1836,21 @@ __device__ inline dd dd_mul_d(dd a, double b) {
   return two_sum(p.hi, p.lo + a.lo * b);
 }
 
-// Per-segment selection state, and the bucket counters and list sizes
-// cleared (grid-stride).
-__global__ void k_sel_init(SelState* st, int nsel, unsigned int* cnt, int64_t ncnt,
-                           unsigned int* mlc, int nmlc) {
+// Per-segment selection state, the bucket counters and list sizes cleared,
+// and the segment of every tile of the count and gather passes (grid-stride).
+__global__ void k_sel_init(const SegDesc* segs, const SelSeg* sel, SelState* st, int nsel,
+                           unsigned int* cnt, int64_t ncnt, unsigned int* mlc, int nmlc,
+                           int* tmap) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
   if (j < nmlc) mlc[j] = 0u;
-  for (int64_t i = j; i < nsel; i += stride) st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
+  for (int64_t i = j; i < nsel; i += stride) {
+    st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
+    const SelSeg g = sel[i];
+    const int64_t nt = (segs[g.si].v_cnt + g.tile_len - 1) / g.tile_len;
+    for (int64_t k = 0; k < nt; k++) tmap[g.tile0 + k] = (int)i;
+  }
 }
 
 // Sample size for a segment of nv new values (about two per splitter).
@@ -1985,143 +1981,300 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
   if (threadIdx.x == 0) st[j].nspl = (int)total;
 }
 
-// SMAX: the largest splitter count of the segments this instance serves
-// (their tiles are [t0, t1)); the LDS footprint follows it.
-template <int SMAX>
-__global__ __launch_bounds__(kSelThreads) void k_sel_count(const SegDesc* segs, const SelSeg* sel,
-                                                           int nsel, int64_t t0, int64_t t1,
-                                                           const double* vin, const double* spl,
-                                                           SelState* st, unsigned int* cnt,
-                                                           uint16_t* bix) {
-  // The search runs on 32 bits of each splitter's order key (4-byte LDS
-  // entries: half the bank traffic of the doubles): the bits right after the
-  // prefix that every splitter shares, so that nearly every value differs
-  // from every splitter there.  A table of kSlots slots over the key range
-  // of the central splitters (monotone float map) gives each value the few
-  // splitters it must still be compared with, by a binary search that each
-  // wave ends when none of its values has any left.  Values below the first
-  // or at or above the last splitter are placed directly; splitters sharing
-  // the 32 bits with a value are compared exactly (from global memory:
-  // rare).
-  constexpr int kSlots = 4096;
-  __shared__ uint32_t sk[SMAX + 1];
+// Two-level lookup of a value's bucket.  The splitters' order keys are
+// searched in full (64 bits, so no comparison needs the doubles), guided by
+// a table on 32 bits of each key: the bits right after the prefix that every
+// splitter shares (its window).  The window range [wA, wB] of the splitters
+// is cut into cells of 2^csh window values (at most 2^log_cells cells);
+// cell c is cut again into 2^p fine slots, p chosen from the number n of
+// splitters in the cell (n < 2^p <= 2n, so a slot holds one splitter or
+// none on average), and fs[s] counts the splitters whose slot is below s.
+// A value's bucket then lies among the splitters of its fine slot, fs[s] ..
+// fs[s + 1] (every splitter with the value's window is there), found by a
+// bisection of as many rounds as the fullest slot needs, the same for every
+// value of the block.
+__host__ __device__ inline int sel_cell_shift(uint32_t span, int log_cells) {
+  int bits = 0;
+  while (bits < 32 && (span >> bits) != 0u) bits++;
+  return bits > log_cells ? bits - log_cells : 0;
+}
+
+// log2 of the fine slots of a cell of n splitters and 2^csh window values.
+__host__ __device__ inline int sel_fine_bits(uint32_t n, int csh) {
+  int p = 0;
+  while (p < 30 && (n >> p) != 0u) p++;  // n < 2^p
+  return p < csh ? p : csh;
+}
+
+__host__ __device__ inline uint32_t sel_window(uint64_t key, int sh) {
+  return (uint32_t)((sh >= 64 ? 0ull : (key << sh)) >> 32);
+}
+
+// Fine slot of window offset d (= window - wA) under packed cell entries
+// (first slot | (csh - p) << 16).
+__device__ inline int sel_slot(const uint32_t* cell, int csh, uint32_t cmask, uint32_t d) {
+  const uint32_t ce = cell[d >> csh];
+  return (int)(ce & 0xffffu) + (int)((d & cmask) >> (ce >> 16));
+}
+
+// x[q] = min(x[q], ..., x[n]) for q <= n, by the whole block: entries
+// still at their initial maximum take the value of the next set entry
+// (x[n] must be set).  Each thread takes a run of at most PER consecutive
+// entries; the runs' minima are combined right to left across the block.
+template <int BLOCK, int PER, typename T>
+__device__ inline void block_suffix_min(T* x, int n, unsigned int* wmin) {
+  const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
+  const int per = (n + BLOCK) / BLOCK;  // n + 1 entries
+  const int q0 = (int)threadIdx.x * per;
+  const unsigned int kMax = (unsigned int)(T)~(T)0;
+  unsigned int v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) v[k] = (k < per && q0 + k <= n) ? (unsigned int)x[q0 + k] : kMax;
+  unsigned int m = kMax;
+#pragma unroll
+  for (int k = PER - 1; k >= 0; k--) {
+    m = min(m, v[k]);
+    v[k] = m;
+  }
+  // c = minimum of the runs of this lane and the lanes above it in the wave.
+  unsigned int c = m;
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned int y = __shfl_down(c, d, 64);
+    if (lane + d < 64) c = min(c, y);
+  }
+  if (lane == 0) wmin[wid] = c;
+  __syncthreads();
+  unsigned int carry = kMax;
+  for (int w = wid + 1; w < BLOCK / 64; w++) carry = min(carry, wmin[w]);
+  const unsigned int above = __shfl_down(c, 1, 64);
+  if (lane < 63) carry = min(carry, above);
+#pragma unroll
+  for (int k = 0; k < PER; k++)
+    if (k < per && q0 + k <= n) x[q0 + k] = (T)min(v[k], carry);
+}
+
+// SMAX: the largest splitter count of the segments this instance serves;
+// the LDS footprint follows it.  An instance serves the segments of tiles
+// [t0, t1) whose splitter capacity is in (SMIN, SMAX]; the others' blocks
+// return at once.
+template <int SMAX, int SMIN, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const SelSeg* sel,
+                                                     int nsel, int64_t t0, int64_t t1,
+                                                     const double* vin, const double* spl,
+                                                     SelState* st, unsigned int* cnt,
+                                                     uint16_t* bix, const int* tmap) {
+  constexpr int kLogCells = SMAX >= 4096 ? 9 : 7;
+  constexpr int kCells = 1 << kLogCells;
+  constexpr int kFine = 2 * SMAX + kCells + 1;  // fine slots at most, plus the end
+  static_assert(kCells + 1 <= SMAX + 1, "cell counts live in the counters");
+  static_assert(kCells % 64 == 0 && kFine < 65536, "table layout");
+  __shared__ uint64_t sk[SMAX + 1];
   __shared__ unsigned int lc[SMAX + 1];
-  __shared__ uint16_t slot[kSlots + 1];
+  __shared__ uint16_t fs[kFine + 1];
+  __shared__ uint32_t cell[kCells];
   __shared__ unsigned long long smin, smax;
+  __shared__ int s_nslot, s_maxlen;
+  __shared__ unsigned int wmin[BLOCK / 64];
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
-  const int j = sel_tile_seg(sel, nsel, t);
+  const int j = tmap[t];
   const SelSeg g = sel[j];
+  if (g.cap_s <= SMIN || g.cap_s > SMAX) return;
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
   const double* sg = spl + g.b_off;
-  const double sp0 = S > 0 ? sg[0] : 0.0, spl_last = S > 0 ? sg[S - 1] : 0.0;
-  const unsigned long long kf = ord_key(sp0), kl = ord_key(spl_last);
-  const int sh = kf == kl ? 32 : min(32, __clzll(kf ^ kl));
-  for (int i = threadIdx.x; i < S; i += blockDim.x) sk[i] = (uint32_t)((ord_key(sg[i]) << sh) >> 32);
-  for (int i = threadIdx.x; i <= S; i += blockDim.x) lc[i] = 0;
-  if (threadIdx.x == 0) {
-    smin = ~0ull;
-    smax = 0ull;
-  }
-  __syncthreads();
-  // The slots span the splitters tl..th: all but the outer 1/256 on each
-  // side when there are many, since a heavy tail would otherwise squeeze
-  // the bulk of the splitters into a few slots.  Keys outside that range
-  // land in the two edge slots (clamped), whose searches stay exact.
-  const int tl = S >= 512 ? S / 256 : 0;
-  const int th = S >= 512 ? S - 1 - S / 256 : (S > 0 ? S - 1 : 0);
-  const uint32_t k0 = S > 0 ? sk[tl] : 0u, kS = S > 0 ? sk[th] : 0u;
-  const float fscale = (float)kSlots / ((float)(kS - k0) + 1.0f);
-  auto slot_of = [&](uint32_t k) {
-    if (k <= k0) return 0;
-    int h = (int)((float)(k - k0) * fscale);
-    return h < kSlots - 1 ? h : kSlots - 1;
-  };
-  // slot[h] = number of splitters mapped below h (slot[kSlots] = S): each
-  // splitter fills the slots between its predecessor's and its own.
-  for (int i = threadIdx.x; i <= S; i += blockDim.x) {
-    const int hp = i > 0 ? slot_of(sk[i - 1]) : -1;
-    const int hi = i < S ? slot_of(sk[i]) : kSlots;
-    for (int h = hp + 1; h <= hi; h++) slot[h] = (uint16_t)i;
-  }
-  __syncthreads();
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
-  unsigned long long kmn = ~0ull, kmx = 0ull;
+  // The tile's first values are loaded before the table is built.
+  constexpr int U = 8;
+  double xn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int64_t i = a + (int64_t)u * BLOCK + threadIdx.x;
+    xn[u] = v[i < b ? i : b - 1];
+  }
+  // Setup loops unrolled over a thread's kPer entries, so that their loads
+  // are all in flight together.
+  constexpr int kPer = (SMAX + 1 + BLOCK - 1) / BLOCK;
+  {
+    double tv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int i = u * BLOCK + (int)threadIdx.x;
+      tv[u] = i < S ? sg[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int i = u * BLOCK + (int)threadIdx.x;
+      if (i < S) sk[i] = ord_key(tv[u]);
+    }
+  }
+  for (int c = threadIdx.x; c <= kCells; c += BLOCK) lc[c] = ~0u;
+  if (threadIdx.x == 0) {
+    smin = ~0ull;
+    smax = 0ull;
+    s_maxlen = 0;
+  }
+  __syncthreads();
+  const uint64_t kf = S > 0 ? sk[0] : 0ull, kl = S > 0 ? sk[S - 1] : 0ull;
+  const int sh = kf == kl ? 64 : __clzll(kf ^ kl);
+  const uint32_t wA = sel_window(kf, sh);
+  const int csh = sel_cell_shift(sel_window(kl, sh) - wA, kLogCells);
+  const uint32_t cmask = (1u << csh) - 1u;
+  // cf[c] (in lc) = splitters in cells below c: splitter i writes it for
+  // the cells after splitter i - 1's cell up to its own (the keys are
+  // sorted, so no counting is needed); then each cell's fine slots and first
+  // slot, scanned by the first wave.
+  unsigned int* cf = lc;
+  uint32_t wd[kPer], wq[kPer];  // window offsets of splitters i and i - 1
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int i = u * BLOCK + (int)threadIdx.x;
+    wd[u] = i < S ? sel_window(sk[i], sh) - wA : 0u;
+    wq[u] = (i > 0 && i <= S) ? sel_window(sk[i - 1], sh) - wA : 0u;
+  }
+  // The first splitter of each cell marks it; cells without one take the
+  // next cell's mark (a suffix minimum).
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int i = u * BLOCK + (int)threadIdx.x;
+    if (i <= S) {
+      const int cp = i > 0 ? (int)(wq[u] >> csh) : -1;
+      const int ci = i < S ? (int)(wd[u] >> csh) : kCells;
+      if (ci != cp) cf[ci] = (unsigned)i;
+    }
+  }
+  __syncthreads();
+  block_suffix_min<BLOCK, (kCells + BLOCK) / BLOCK>(cf, kCells, wmin);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    constexpr int per = kCells / 64;
+    const int c0 = (int)threadIdx.x * per;
+    uint32_t m[per], sum = 0;
+#pragma unroll
+    for (int u = 0; u < per; u++) {
+      m[u] = 1u << sel_fine_bits(cf[c0 + u + 1] - cf[c0 + u], csh);
+      sum += m[u];
+    }
+    uint32_t inc = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(inc, d, 64);
+      if ((int)threadIdx.x >= d) inc += y;
+    }
+    uint32_t base = inc - sum;
+#pragma unroll
+    for (int u = 0; u < per; u++) {
+      const int p = 31 - __clz((int)m[u]);
+      cell[c0 + u] = base | ((uint32_t)(csh - p) << 16);
+      base += m[u];
+    }
+    if (threadIdx.x == 63) s_nslot = (int)inc;
+  }
+  __syncthreads();
+  const int nslot = s_nslot;
+  for (int q = threadIdx.x; q <= nslot; q += BLOCK) fs[q] = 0xffffu;
+  __syncthreads();
+  int sp[kPer], si[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int i = u * BLOCK + (int)threadIdx.x;
+    sp[u] = (i > 0 && i <= S) ? sel_slot(cell, csh, cmask, wq[u]) : -1;
+    si[u] = i < S ? sel_slot(cell, csh, cmask, wd[u]) : nslot;
+  }
+  // fs[s] = i for the first splitter i of slot s; slots without one take
+  // the next slot's (a suffix minimum); fs[nslot] = S.
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int i = u * BLOCK + (int)threadIdx.x;
+    if (i <= S && si[u] != sp[u]) fs[si[u]] = (uint16_t)i;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int i = u * BLOCK + (int)threadIdx.x;
+    if (i <= S) lc[i] = 0u;
+  }
+  __syncthreads();
+  block_suffix_min<BLOCK, (kFine + BLOCK) / BLOCK>(fs, nslot, wmin);
+  __syncthreads();
+  int ml = 0;
+  constexpr int kPerF = (kFine + BLOCK - 1) / BLOCK;
+#pragma unroll
+  for (int u = 0; u < kPerF; u++) {
+    const int q = u * BLOCK + (int)threadIdx.x;
+    if (q < nslot) ml = max(ml, (int)fs[q + 1] - (int)fs[q]);
+  }
+  for (int d = 32; d > 0; d >>= 1) ml = max(ml, __shfl_xor(ml, d, 64));
+  if ((threadIdx.x & 63) == 0 && ml > 0) atomicMax(&s_maxlen, ml);
+  __syncthreads();
+  int rounds = 0;
+  while ((s_maxlen >> rounds) != 0) rounds++;
+  double mn = __builtin_inf(), mx = -__builtin_inf();
   uint16_t* bx = bix + sd.v_off;
   // U values per thread searched together (independent LDS reads in
-  // flight): bucket = number of splitters <= x, as sel_bucket.
-  constexpr int U = 5;
-  for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * kSelThreads) {
+  // flight), the next U loaded meanwhile: bucket = number of splitters <=
+  // x, as sel_bucket (the order key maps -0.0 and +0.0 together, as the
+  // double comparison does).
+  for (int64_t i0 = a; i0 < b; i0 += (int64_t)U * BLOCK) {
     double x[U];
-    uint32_t kx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) x[u] = xn[u];
+    // The next loads go out before this round's stores (an index past the
+    // tile reads its last value, then is ignored).
+    const int64_t i1 = i0 + (int64_t)U * BLOCK;
+    if (i1 < b) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t i = i1 + (int64_t)u * BLOCK + threadIdx.x;
+        xn[u] = v[i < b ? i : b - 1];
+      }
+    }
+    uint64_t kx[U];
     int lo[U], len[U];
-    // All U loads issued before any is used (an index past the tile reads
-    // its last value, then is ignored).
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
-      x[u] = v[i < b ? i : b - 1];
+      kx[u] = ord_key(x[u]);
+      const bool below = kx[u] < kf, above = kx[u] >= kl;
+      const uint32_t d = (below || above) ? 0u : sel_window(kx[u], sh) - wA;
+      const int q = sel_slot(cell, csh, cmask, d);
+      const int f0 = fs[q], f1 = fs[q + 1];
+      lo[u] = below ? 0 : (above ? S : f0);
+      len[u] = (below || above) ? 0 : f1 - f0;
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
-      const bool in = i < b && S > 0 && !(x[u] < sp0) && x[u] < spl_last;
-      kx[u] = in ? (uint32_t)((ord_key(x[u]) << sh) >> 32) : 0u;
-      const int h = in ? slot_of(kx[u]) : 0;
-      lo[u] = in ? (int)slot[h] : 0;
-      len[u] = in ? (int)slot[h + 1] - lo[u] : 0;
-    }
-    // lo = number of splitters whose searched bits are below the value's:
-    // all of them are below the value.
-    for (;;) {
-      bool more = false;
-#pragma unroll
-      for (int u = 0; u < U; u++) more = more || len[u] > 0;
-      if (__ballot(more) == 0ull) break;
+    // Upper bound of the value's key among the slot's splitters.
+    for (int r = 0; r < rounds; r++) {
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int half = len[u] >> 1;
-        const bool right = len[u] > 0 && sk[lo[u] + half] < kx[u];
+        const bool right = len[u] > 0 && sk[lo[u] + half] <= kx[u];
         lo[u] = right ? lo[u] + half + 1 : lo[u];
         len[u] = right ? len[u] - half - 1 : half;
       }
     }
-    // The first splitter not below the value in the searched bits: equal
-    // bits (rare) need the exact comparison.
-    uint32_t nx[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) nx[u] = sk[lo[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
-      const bool below = x[u] < sp0, above = !(x[u] < spl_last);
-      lo[u] = below ? 0 : (above ? S : lo[u]);
-      if (i < b && !below && !above && nx[u] == kx[u]) {
-        int k = lo[u];
-        while (k < S && sk[k] == kx[u] && sg[k] <= x[u]) k++;
-        lo[u] = k;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      int64_t i = i0 + (int64_t)u * kSelThreads + threadIdx.x;
+      const int64_t i = i0 + (int64_t)u * BLOCK + threadIdx.x;
       if (i < b) {
         const int k = lo[u];
         atomicAdd(&lc[k], 1u);
         bx[i] = (uint16_t)k;  // the gather pass reads it back instead of searching again
-        unsigned long long kk = ord_key(x[u]);
-        kmn = kk < kmn ? kk : kmn;
-        kmx = kk > kmx ? kk : kmx;
+        mn = fmin(mn, x[u]);
+        mx = fmax(mx, x[u]);
       }
     }
   }
-  atomicMin(&smin, kmn);
-  atomicMax(&smax, kmx);
+  // Zero extremes keep no sign here (the order key maps -0.0 and +0.0
+  // together); the batch extremes' zero signs are set afterwards.
+  for (int d = 32; d > 0; d >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, d, 64));
+    mx = fmax(mx, __shfl_xor(mx, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && mn <= mx) {
+    atomicMin(&smin, ord_key(mn));
+    atomicMax(&smax, ord_key(mx));
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i <= S; i += blockDim.x)
+  for (int i = threadIdx.x; i <= S; i += BLOCK)
     if (lc[i]) atomicAdd(&cnt[g.b_off + i], lc[i]);
   if (threadIdx.x == 0) {
     atomicMin(&st[j].kmin, smin);
@@ -2693,16 +2846,22 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
 }
 
 // Second pass over the new values: marked buckets are gathered, the rest
-// are added to their cluster's slice sums.
+// are added to their cluster's slice sums.  A block stages its tile's
+// gathered values (and their marked-bucket indices) in stv / stm at the
+// tile's own range, takes one place range per marked bucket once the tile is
+// done, and copies them there (the order inside a bucket is free: each
+// gathered bucket is sorted or selected afterwards).
 template <int SMAX, int CMAX>
 __global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* segs, const SelSeg* sel,
                                                             int nsel, int64_t t0, int64_t t1,
                                                             const double* vin, const uint16_t* bix,
                                                             const int* binfo, const int* ce,
                                                             const SelState* st, SelMark* mk,
-                                                            double* csl, double* X) {
-  constexpr int kItems = 8;                 // values per thread per chunk
+                                                            double* csl, double* X, double* stv,
+                                                            uint16_t* stm, const int* tmap) {
+  constexpr int kItems = 4;                 // values per thread per chunk
   constexpr int kMk = 2 * CMAX + 2;          // gathered buckets of a segment at most
+  static_assert(kMk < 65536, "marked-bucket index width");
   // Replicas of the cluster accumulators (lane r adds into replica r % kRep):
   // the central clusters of a large digest hold a third of its values each,
   // so one copy would serialize a wave's atomic adds.  The slices are exact
@@ -2715,73 +2874,127 @@ __global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* se
   __shared__ int ex[CMAX];
   __shared__ double acc[kRep * kRepStride];
   __shared__ unsigned int lcnt[kMk];
-  __shared__ unsigned long long lbase[kMk];  // gather position of the chunk's first item
+  __shared__ unsigned long long lbase[kMk];  // place of the bucket's first staged value
+  __shared__ unsigned int nst;
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
-  const int j = sel_tile_seg(sel, nsel, t);
+  const int j = tmap[t];
   if (st[j].fail) return;
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl, C = st[j].ncl, nmk = st[j].nmk;
-  for (int i = threadIdx.x; i <= S; i += blockDim.x) inf[i] = binfo[g.b_off + i];
-  for (int i = threadIdx.x; i < C; i += blockDim.x) ex[i] = ce[g.c_off + i];
-  for (int i = threadIdx.x; i < kRep * kRepStride; i += blockDim.x) acc[i] = 0.0;
-  double* acc_r = acc + (threadIdx.x % kRep) * kRepStride;
   const int64_t a = (t - g.tile0) * g.tile_len;
   const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
   const double* v = vin + sd.v_off;
   const uint16_t* bx = bix + sd.v_off;
-  double* xg = X + sd.x_off;
-  SelMark* m = mk + g.m_off;
+  // The tile's first values go out before the tables are loaded.
+  double xn[kItems];
+  uint16_t bn[kItems];
+#pragma unroll
+  for (int q = 0; q < kItems; q++) {
+    const int64_t i = a + (int64_t)q * kSelThreads + threadIdx.x;
+    const int64_t ic = i < b ? i : b - 1;
+    xn[q] = v[ic];
+    bn[q] = bx[ic];
+  }
+  {
+    // Table loads unrolled so that they are all in flight together.
+    constexpr int kPerI = (SMAX + 1 + kSelThreads - 1) / kSelThreads;
+    constexpr int kPerC = (CMAX + kSelThreads - 1) / kSelThreads;
+    int ti[kPerI], tc[kPerC];
+#pragma unroll
+    for (int u = 0; u < kPerI; u++) {
+      const int i = u * kSelThreads + (int)threadIdx.x;
+      ti[u] = i <= S ? binfo[g.b_off + i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerC; u++) {
+      const int i = u * kSelThreads + (int)threadIdx.x;
+      tc[u] = i < C ? ce[g.c_off + i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerI; u++) {
+      const int i = u * kSelThreads + (int)threadIdx.x;
+      if (i <= S) inf[i] = ti[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerC; u++) {
+      const int i = u * kSelThreads + (int)threadIdx.x;
+      if (i < C) ex[i] = tc[u];
+    }
+  }
+  for (int i = threadIdx.x; i < kRep * kRepStride; i += blockDim.x) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < nmk; i += blockDim.x) lcnt[i] = 0u;
+  if (threadIdx.x == 0) nst = 0u;
+  __syncthreads();
+  double* acc_r = acc + (threadIdx.x % kRep) * kRepStride;
+  double* sv = stv + sd.v_off + a;
+  uint16_t* sm = stm + sd.v_off + a;
+  const int lane = (int)(threadIdx.x & 63);
   const int64_t chunk = (int64_t)kItems * kSelThreads;
   for (int64_t c0 = a; c0 < b; c0 += chunk) {
-    for (int i = threadIdx.x; i < nmk; i += blockDim.x) lcnt[i] = 0;
-    __syncthreads();
     double xr[kItems];
-    int fr[kItems];
-    unsigned int rr[kItems];
     uint16_t br[kItems];
-    // The chunk in two halves, each with its loads issued before any is
-    // used (an index past the tile reads its last item, then is ignored).
 #pragma unroll
-    for (int h = 0; h < kItems; h += kItems / 2) {
+    for (int q = 0; q < kItems; q++) {
+      xr[q] = xn[q];
+      br[q] = bn[q];
+    }
+    // The next chunk's loads go out before this chunk's stores.
+    const int64_t c1 = c0 + chunk;
+    if (c1 < b) {
 #pragma unroll
-      for (int q = h; q < h + kItems / 2; q++) {
-        const int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
+      for (int q = 0; q < kItems; q++) {
+        const int64_t i = c1 + (int64_t)q * kSelThreads + threadIdx.x;
         const int64_t ic = i < b ? i : b - 1;
-        xr[q] = v[ic];
-        br[q] = bx[ic];
+        xn[q] = v[ic];
+        bn[q] = bx[ic];
       }
+    }
 #pragma unroll
-      for (int q = h; q < h + kItems / 2; q++) {
-        const int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
-        fr[q] = 0x7fffffff;
-        if (i < b) {
-          const double x = xr[q];
-          const int f = inf[br[q]];
-          fr[q] = f;
-          if (f < 0) {
-            rr[q] = atomicAdd(&lcnt[-1 - f], 1u);
-          } else {
-            double s1, s2;
-            sel_slices(x, ex[f], s1, s2);
-            unsafeAtomicAdd(&acc_r[2 * f], s1);
-            unsafeAtomicAdd(&acc_r[2 * f + 1], s2);
-          }
+    for (int q = 0; q < kItems; q++) {
+      const int64_t i = c0 + (int64_t)q * kSelThreads + threadIdx.x;
+      const bool in = i < b;
+      const int f = in ? inf[br[q]] : 0;
+      if (in && f >= 0) {
+        double s1, s2;
+        sel_slices(xr[q], ex[f], s1, s2);
+        unsafeAtomicAdd(&acc_r[2 * f], s1);
+        unsafeAtomicAdd(&acc_r[2 * f + 1], s2);
+      }
+      // Staged places claimed once per wave.
+      const unsigned long long mask = __ballot(in && f < 0);
+      if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned int base = 0u;
+        if (lane == leader) base = atomicAdd(&nst, (unsigned int)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (in && f < 0) {
+          const unsigned int o = base + __builtin_amdgcn_mbcnt_hi(
+                                            (uint32_t)(mask >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+          sv[o] = xr[q];
+          sm[o] = (uint16_t)(-1 - f);
+          atomicAdd(&lcnt[-1 - f], 1u);
         }
       }
     }
-    __syncthreads();
-    // One reservation per gathered bucket and chunk.
-    for (int i = threadIdx.x; i < nmk; i += blockDim.x)
-      if (lcnt[i]) lbase[i] = (unsigned long long)m[i].goff + atomicAdd(&m[i].cursor, lcnt[i]);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kItems; q++) {
-      int f = fr[q];
-      if (f < 0) xg[lbase[-1 - f] + rr[q]] = xr[q];
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+  // One place range per gathered bucket and tile; the counters then hand
+  // out the places inside it.
+  for (int i = threadIdx.x; i < nmk; i += blockDim.x) {
+    const unsigned int c = lcnt[i];
+    if (c) lbase[i] = (unsigned long long)mk[g.m_off + i].goff + atomicAdd(&mk[g.m_off + i].cursor, c);
+    lcnt[i] = 0u;
+  }
+  __syncthreads();
+  double* xg = X + sd.x_off;
+  const int ns = (int)nst;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int m = sm[i];
+    const double x = sv[i];
+    xg[lbase[m] + atomicAdd(&lcnt[m], 1u)] = x;
   }
   for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) {
     double sum = 0.0;
@@ -3923,6 +4136,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   int2* dml = nullptr;
   int2* dmlx = nullptr;
   uint16_t* dbix = nullptr;
+  uint16_t* dstm = nullptr;
+  int* dtmap = nullptr;
   if (nsel > 0 && !fb_only) {
     dst = as<SelState>(sel_st_, nsel);
     dspl = as<double>(sel_spl_, sel_b);
@@ -3936,13 +4151,18 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     dmk = as<SelMark>(sel_mk_, sel_m);
     dml = as<int2>(sel_ml_, sel_m);
     dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
+    // Staged gathered values: their marked-bucket indices here, the values
+    // in vs (which only the sorting path uses, for other segments).
+    dstm = as<uint16_t>(sel_stm_, std::max<int64_t>(total_v, 1));
+    dtmap = as<int>(sel_tmap_, std::max<int64_t>(sel_t, 1));
     // Gathered-bucket list size, fallbacks, list sizes per size class.
     dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
     dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
     {
       const int64_t items = std::max<int64_t>(std::max<int64_t>(sel_b, nsel), 2 + kSelClasses);
       const unsigned grid = (unsigned)std::min<int64_t>((items + 255) / 256, 1024);
-      k_sel_init<<<grid, 256, 0, stream_>>>(dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses);
+      k_sel_init<<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses,
+                                            dtmap);
       launched("k_sel_init");
     }
     {
@@ -4025,16 +4245,15 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     const double sel_bytes = 8.0 * P.sel_v;
     {
       int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
-      if (t_small > 0) {
-        k_sel_count<kSelSmallS><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, 0, t_small, d_v, dspl, dst, dcnt, dbix);
-        launched("k_sel_count<small>");
-      }
-      if (sel_t > t_small) {
-        k_sel_count<kSelSplMax><<<(unsigned)(sel_t - t_small), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_small, sel_t, d_v, dspl, dst, dcnt, dbix);
-        launched("k_sel_count");
-      }
+      // Two LDS classes by splitter capacity over every tile (a block whose
+      // segment belongs to the other class returns at once): small tables
+      // run several blocks per CU.
+      k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, stream_>>>(
+          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
+      launched("k_sel_count<small>");
+      k_sel_count<kSelSplMax, kSelSmallS, kSelThreads><<<(unsigned)sel_t, kSelThreads, 0, stream_>>>(
+          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
+      launched("k_sel_count");
       phase_end(ph_);
     }
     if (n_small > 0 && small_later) run_small(true);
@@ -4058,17 +4277,17 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       int ph_ = phase_begin("sel_gather", sel_bytes * 1.25);
       if (t_small > 0) {
         k_sel_gather<kSelSmallS, kSelSmallC><<<(unsigned)t_small, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, 0, t_small, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
+            dsegs, dsel, nsel, 0, t_small, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x, vs, dstm, dtmap);
         launched("k_sel_gather<small>");
       }
       if (t_mid > t_small) {
         k_sel_gather<kSelSplMax, kSelSmallC><<<(unsigned)(t_mid - t_small), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_small, t_mid, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
+            dsegs, dsel, nsel, t_small, t_mid, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x, vs, dstm, dtmap);
         launched("k_sel_gather<mid>");
       }
       if (sel_t > t_mid) {
         k_sel_gather<kSelSplMax, kSelCmax><<<(unsigned)(sel_t - t_mid), kSelThreads, 0, stream_>>>(
-            dsegs, dsel, nsel, t_mid, sel_t, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x);
+            dsegs, dsel, nsel, t_mid, sel_t, d_v, dbix, dinfo, dce, dst, dmk, dcsl, x, vs, dstm, dtmap);
         launched("k_sel_gather");
       }
       phase_end(ph_);
