@@ -915,8 +915,8 @@ def main():
         it, ch, tmin, tmax, ov = (int(x) for x in args.shape.split(","))
         session.set_partition_shape(it, ch)
         session.set_select_shape(tmin, tmax)
-        session.set_overlap(bool(ov))
-        tuned = {"items": it, "chunks": ch, "tile_min": tmin, "tiles_max": tmax, "overlap": bool(ov),
+        session.set_overlap(int(ov))
+        tuned = {"items": it, "chunks": ch, "tile_min": tmin, "tiles_max": tmax, "overlap": int(ov),
                  "fixed": True}
     elif not args.no_autotune:
         # Launch shape picked by the library's autotuner on this batch (not

@@ -76,10 +76,12 @@ int kd_session_set_partition_shape(kd_session *s, int items, int64_t chunks);
  * is split into at most tiles_max tiles (1..4096).  Results do not depend on
  * it.  No counterpart in the reference. */
 int kd_session_set_select_shape(kd_session *s, int64_t tile_min, int64_t tiles_max);
-/* Small digests (at most 4096 merged records) compressed on a second stream,
- * overlapping the selection compress of large ones (enable != 0) or on the
- * session stream after it (0, the default).  Same results either way.  No
- * counterpart in the reference. */
+/* Small digests (at most 4096 merged records): 0 (the default) compressed on
+ * the session stream after the selection samples of large ones; 1 on a second
+ * stream once the selection count is done, overlapping the rest of the
+ * selection compress; 2 on a second stream from the start of the compress.
+ * Other values: KD_EINVAL.  Same results every way.  No counterpart in the
+ * reference. */
 int kd_session_set_overlap(kd_session *s, int enable);
 /* Grouped key partition of batches of 2^22 pairs or more (and keys below
  * 2^17): keys estimated from a sample to hold at least min_pairs of the

@@ -398,7 +398,8 @@ extern "C" int kd_session_set_select_shape(kd_session* s, int64_t tile_min, int6
 
 extern "C" int kd_session_set_overlap(kd_session* s, int enable) {
   return guarded([&] {
-    s->eng.set_overlap(enable != 0);
+    if (enable < 0 || enable > 2) return KD_EINVAL;
+    s->eng.set_overlap(enable);
     s->shape_pinned = true;
     return KD_OK;
   });
@@ -1415,14 +1416,14 @@ GroupedShape current_shape(kd_session* s) {
   c.chunks = s->eng.partition_chunks();
   c.tile_min = s->eng.select_tile_min();
   c.tiles_max = s->eng.select_tiles_max();
-  c.overlap = s->eng.overlap() ? 1 : 0;
+  c.overlap = s->eng.overlap();
   return c;
 }
 
 void apply_shape(kd_session* s, const GroupedShape& c) {
   s->eng.set_partition_shape(c.items, c.chunks);
   s->eng.set_select_shape(c.tile_min, c.tiles_max);
-  s->eng.set_overlap(c.overlap != 0);
+  s->eng.set_overlap(c.overlap);
 }
 
 // Shapes the autotuner chose, for the whole process, keyed by the batch's
@@ -1572,7 +1573,7 @@ extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num
     knob(chunks, [](GroupedShape& c, int64_t v) { c.chunks = v; });
     knob(tmins, [](GroupedShape& c, int64_t v) { c.tile_min = v; });
     knob(tmaxs, [](GroupedShape& c, int64_t v) { c.tiles_max = v; });
-    const int ov[] = {0, 1};
+    const int ov[] = {0, 1, 2};
     knob(ov, [](GroupedShape& c, int v) { c.overlap = v; });
     apply_shape(s, best);
     s->shape_pinned = false;

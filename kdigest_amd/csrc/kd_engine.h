@@ -215,8 +215,8 @@ class Engine {
   // One-block compress of small segments on a second stream, overlapping the
   // selection path (off by default: both are bound by the same CU resources
   // on the grouped workloads measured so far).
-  void set_overlap(bool on) { overlap_ = on; }
-  bool overlap() const { return overlap_; }
+  void set_overlap(int mode) { overlap_ = mode; }
+  int overlap() const { return overlap_; }
   void set_select_shape(int64_t tile_min, int64_t tiles_max) {
     sel_tile_min_ = tile_min;
     sel_tiles_max_ = tiles_max;
@@ -296,7 +296,7 @@ class Engine {
   // the work that reads its results.
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ = nullptr, join_ = nullptr;
-  bool overlap_ = false;
+  int overlap_ = 0;  // small segments: 0 session stream, 1 side stream after the count, 2 side stream from the start
   DevBuf seg_, res_, em_s_, ew_s_, se_, pose_, vs_, x_, cm_, cw_, cs_, cnt_, t1_, t2_;
   DevBuf parts_, parts2_, spl_, nspl_, bcnt_, bcur_, tiles_, tcnt_, fb_, fbl_, route_, tsum_, long_,
       order_, clist_;

@@ -4138,76 +4138,17 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   uint16_t* dbix = nullptr;
   uint16_t* dstm = nullptr;
   int* dtmap = nullptr;
-  if (nsel > 0 && !fb_only) {
-    dst = as<SelState>(sel_st_, nsel);
-    dspl = as<double>(sel_spl_, sel_b);
-    dcnt = as<unsigned int>(sel_cnt_, sel_b);
-    dbst = as<unsigned int>(sel_bst_, sel_b);
-    dinfo = as<int>(sel_info_, sel_b);
-    dbnd = as<SelBound>(sel_bnd_, sel_c);
-    dce = as<int>(sel_ce_, sel_c);
-    dcsl = as<double>(sel_csl_, 2 * sel_c);
-    dcwv = as<double>(sel_cwv_, sel_c);
-    dmk = as<SelMark>(sel_mk_, sel_m);
-    dml = as<int2>(sel_ml_, sel_m);
-    dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
-    // Staged gathered values: their marked-bucket indices here, the values
-    // in vs (which only the sorting path uses, for other segments).
-    dstm = as<uint16_t>(sel_stm_, std::max<int64_t>(total_v, 1));
-    dtmap = as<int>(sel_tmap_, std::max<int64_t>(sel_t, 1));
-    // Gathered-bucket list size, fallbacks, list sizes per size class.
-    dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
-    dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
-    {
-      const int64_t items = std::max<int64_t>(std::max<int64_t>(sel_b, nsel), 2 + kSelClasses);
-      const unsigned grid = (unsigned)std::min<int64_t>((items + 255) / 256, 1024);
-      k_sel_init<<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses,
-                                            dtmap);
-      launched("k_sel_init");
-    }
-    {
-      int ph_ = phase_begin("sel_sample", 0.0);
-      // The three sample sizes touch disjoint segments: with several
-      // segments the two smaller run on the side stream, the largest beside
-      // them on the session stream (a lone digest skips the fork's latency).
-      const bool fork_s = nsel > 1;
-      hipStream_t s2 = fork_s ? side_ : stream_;
-      if (fork_s) {
-        check_hip(hipEventRecord(fork_, stream_), "fork samples");
-        check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork samples wait");
-      }
-      // A host plan says which classes have segments; skip the others.
-      const bool cls_known = (P.sample_cls & 8) != 0;
-      if (!cls_known || (P.sample_cls & 1)) {
-        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst, nullptr);
-        launched("k_sel_sample<1024>");
-      }
-      if (!cls_known || (P.sample_cls & 2)) {
-        k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst,
-                                                                  nullptr);
-        launched("k_sel_sample<4096>");
-      }
-      // The largest class fills a CU's LDS anyway: a block of kSelThreads.
-      if (!cls_known || (P.sample_cls & 4)) {
-        k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, d_v, esm, dspl, dst, nsel == 1 ? presorted_ : nullptr);
-        launched("k_sel_sample<8192>");
-      }
-      if (fork_s) {
-        check_hip(hipEventRecord(join_, side_), "join samples");
-        check_hip(hipStreamWaitEvent(stream_, join_, 0), "join samples wait");
-      }
-      phase_end(ph_);
-    }
-  }
-  // Small segments after the selection samples (whose blocks then start
-  // first); on the side stream (set_overlap) they start after the selection
-  // count (which keeps every CU busy by itself) and overlap the rest of the
-  // selection path, whose single-block steps leave most CUs idle.
+  // Small segments: overlap mode 0 runs them on the session stream after the
+  // selection samples (whose blocks then start first); mode 1 on the side
+  // stream once the selection count is done (the count keeps every CU busy
+  // by itself), overlapping the rest of the selection path, whose
+  // single-block steps leave most CUs idle; mode 2 on the side stream from
+  // the start, beside the selection path's samples and count.
   bool forked = false;
   int64_t n_small = 0;
   for (int c = 0; c < kSmallClasses; c++) n_small += P.n_small[c];
-  const bool small_later = overlap_ && nsel > 0;
+  const bool small_later = overlap_ == 1 && nsel > 0;
+  const bool small_first = overlap_ == 2 && nsel > 0;
   auto run_small = [&](bool side) {
     hipStream_t on = stream_;
     if (side) {
@@ -4240,7 +4181,70 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     phase_end(ph_);
     if (forked) check_hip(hipEventRecord(join_, side_), "join");
   };
-  if (n_small > 0 && !fb_only && !small_later) run_small(false);
+  if (n_small > 0 && !fb_only && small_first) run_small(true);
+  if (nsel > 0 && !fb_only) {
+    dst = as<SelState>(sel_st_, nsel);
+    dspl = as<double>(sel_spl_, sel_b);
+    dcnt = as<unsigned int>(sel_cnt_, sel_b);
+    dbst = as<unsigned int>(sel_bst_, sel_b);
+    dinfo = as<int>(sel_info_, sel_b);
+    dbnd = as<SelBound>(sel_bnd_, sel_c);
+    dce = as<int>(sel_ce_, sel_c);
+    dcsl = as<double>(sel_csl_, 2 * sel_c);
+    dcwv = as<double>(sel_cwv_, sel_c);
+    dmk = as<SelMark>(sel_mk_, sel_m);
+    dml = as<int2>(sel_ml_, sel_m);
+    dbix = as<uint16_t>(sel_bix_, std::max<int64_t>(total_v, 1));
+    // Staged gathered values: their marked-bucket indices here, the values
+    // in vs (which only the sorting path uses, for other segments).
+    dstm = as<uint16_t>(sel_stm_, std::max<int64_t>(total_v, 1));
+    dtmap = as<int>(sel_tmap_, std::max<int64_t>(sel_t, 1));
+    // Gathered-bucket list size, fallbacks, list sizes per size class.
+    dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
+    dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
+    {
+      const int64_t items = std::max<int64_t>(std::max<int64_t>(sel_b, nsel), 2 + kSelClasses);
+      const unsigned grid = (unsigned)std::min<int64_t>((items + 255) / 256, 1024);
+      k_sel_init<<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses,
+                                            dtmap);
+      launched("k_sel_init");
+    }
+    {
+      int ph_ = phase_begin("sel_sample", 0.0);
+      // The three sample sizes touch disjoint segments: with several
+      // segments the two smaller run on the side stream, the largest beside
+      // them on the session stream (a lone digest skips the fork's latency).
+      const bool fork_s = nsel > 1 && !forked;
+      hipStream_t s2 = fork_s ? side_ : stream_;
+      if (fork_s) {
+        check_hip(hipEventRecord(fork_, stream_), "fork samples");
+        check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork samples wait");
+      }
+      // A host plan says which classes have segments; skip the others.
+      const bool cls_known = (P.sample_cls & 8) != 0;
+      if (!cls_known || (P.sample_cls & 1)) {
+        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst, nullptr);
+        launched("k_sel_sample<1024>");
+      }
+      if (!cls_known || (P.sample_cls & 2)) {
+        k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst,
+                                                                  nullptr);
+        launched("k_sel_sample<4096>");
+      }
+      // The largest class fills a CU's LDS anyway: a block of kSelThreads.
+      if (!cls_known || (P.sample_cls & 4)) {
+        k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(
+            dsegs, dsel, d_v, esm, dspl, dst, nsel == 1 ? presorted_ : nullptr);
+        launched("k_sel_sample<8192>");
+      }
+      if (fork_s) {
+        check_hip(hipEventRecord(join_, side_), "join samples");
+        check_hip(hipStreamWaitEvent(stream_, join_, 0), "join samples wait");
+      }
+      phase_end(ph_);
+    }
+  }
+  if (n_small > 0 && !fb_only && !small_later && !small_first) run_small(false);
   if (nsel > 0 && !fb_only) {
     const double sel_bytes = 8.0 * P.sel_v;
     {

@@ -153,10 +153,12 @@ class GpuSession:
         scatter sub-tile (4, 8, 16) and the target chunks per pass."""
         check(load().kd_session_set_partition_shape(self.handle, int(items), int(chunks)))
 
-    def set_overlap(self, enable: bool = True) -> None:
-        """Compress small digests on a second stream, overlapping the
-        selection compress of large ones (off by default)."""
-        check(load().kd_session_set_overlap(self.handle, 1 if enable else 0))
+    def set_overlap(self, mode: int | bool = 1) -> None:
+        """Where small digests are compressed: 0 (False, the default) on the
+        session stream after the selection samples of large ones; 1 (True) on
+        a second stream once the selection count is done; 2 on a second
+        stream from the start of the compress."""
+        check(load().kd_session_set_overlap(self.handle, int(mode)))
 
     def set_heavy_keys(self, min_pairs: int) -> None:
         """Keys of a large grouped batch estimated to hold at least min_pairs
@@ -192,7 +194,7 @@ class GpuSession:
                 self.handle, scale, int(num_keys), float(compression), _device_ptr(keys),
                 _device_ptr(values), kn, int(reps), _ptr(shp, _i64p), _ptr(ms)))
         return {"items": int(shp[0]), "chunks": int(shp[1]), "tile_min": int(shp[2]),
-                "tiles_max": int(shp[3]), "overlap": bool(shp[4]),
+                "tiles_max": int(shp[3]), "overlap": int(shp[4]),
                 "ms_before": float(ms[0]), "ms_after": float(ms[1])}
 
     def shape(self) -> dict:
@@ -200,7 +202,7 @@ class GpuSession:
         o = np.zeros(5, np.int64)
         check(load().kd_session_shape(self.handle, _ptr(o, _i64p)))
         return {"items": int(o[0]), "chunks": int(o[1]), "tile_min": int(o[2]),
-                "tiles_max": int(o[3]), "overlap": bool(o[4])}
+                "tiles_max": int(o[3]), "overlap": int(o[4])}
 
     @staticmethod
     def autotune_save(path: str) -> None:

@@ -362,24 +362,28 @@ def test_autotune_table_survives_save_and_load(kd, tmp_path):
 
 def test_overlap_keeps_results(kd):
     """Small digests compressed on the second stream while the selection
-    path runs (set_overlap) give the same digests bit for bit, over two
+    path runs (set_overlap 1: after the selection count; 2: from the start)
+    give the same digests bit for bit as on the session stream, over two
     batches (the second merges into the first's centroids)."""
     rng = np.random.default_rng(8)
     nk = 3001
-    on, off = kd.GpuSession(0), kd.GpuSession(0)
-    on.set_overlap(True)
-    off.set_overlap(False)
-    a = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=on)
-    b = kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=off)
+    sessions = [kd.GpuSession(0) for _ in range(3)]
+    for mode, s in enumerate(sessions):
+        s.set_overlap(mode)
+        assert s.shape()["overlap"] == mode
+    with pytest.raises(kd.KdigestError):
+        sessions[0].set_overlap(3)
+    digs = [kd.GroupedTDigest(nk, 100.0, scale=kd.ScaleK3, session=s) for s in sessions]
     for _ in range(2):
         keys = ((rng.zipf(1.2, 2_000_000) - 1) % nk).astype(np.uint32)
         vals = rng.standard_normal(keys.size) * 5.0 + keys * 0.01
-        a.add_batch(keys, vals)
-        b.add_batch(keys, vals)
-    paths = on.path_counts()
+        for g in digs:
+            g.add_batch(keys, vals)
+    paths = sessions[1].path_counts()
     assert paths["selection"] > 0 and paths["one_block"] > 0, paths
     for k in range(nk):
-        m, w = a.centroids(k)
-        bm, bw = b.centroids(k)
-        assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
-        assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
+        bm, bw = digs[0].centroids(k)
+        for g in digs[1:]:
+            m, w = g.centroids(k)
+            assert np.array_equal(m.view(np.uint64), bm.view(np.uint64)), k
+            assert np.array_equal(w.view(np.uint64), bw.view(np.uint64)), k
