@@ -2307,18 +2307,25 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   const int nb = S + 1;
   const int per = (nb + kSelThreads - 1) / kSelThreads;
   const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
+  // A thread's counts are loaded together (at most kPerB of them).
+  constexpr int kPerB = (kSelSplMax + 1 + kSelThreads - 1) / kSelThreads;
+  uint32_t cv[kPerB];
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
   uint32_t acc = 0;
-  for (int i = lo0; i < hi0; i++) {
-    acc += c[i];
-    info[i] = 0;
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) {
+    acc += cv[u];
+    if (lo0 + u < hi0) info[lo0 + u] = 0;
   }
   part[tid] = acc;
   __syncthreads();
   uint32_t total = block_excl_scan(part, kSelThreads);
   uint32_t run = part[tid];
-  for (int i = lo0; i < hi0; i++) {
-    B[i] = run;
-    run += c[i];
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) {
+    if (lo0 + u < hi0) B[lo0 + u] = run;
+    run += cv[u];
   }
   if (tid == 0) {
     B[nb] = total;
@@ -2734,7 +2741,20 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
   if (tid == 0) s_fail = 0;
   const int C = st[j].ncl;
-  for (int i = tid; i <= S + 1; i += blockDim.x) sB[i] = B[i];
+  {
+    constexpr int kPerS = (kSelSplMax + 2 + kSelThreads - 1) / kSelThreads;
+    uint32_t tb[kPerS];
+#pragma unroll
+    for (int u = 0; u < kPerS; u++) {
+      const int i = u * kSelThreads + tid;
+      tb[u] = i <= S + 1 ? B[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerS; u++) {
+      const int i = u * kSelThreads + tid;
+      if (i <= S + 1) sB[i] = tb[u];
+    }
+  }
   __syncthreads();
   // 4. Boundary records and the buckets to gather.
   for (int i = tid; i < C; i += blockDim.x) {
@@ -2757,12 +2777,22 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     }
   }
   __syncthreads();
-  // 5. Gather layout of the marked buckets, in bucket order.
+  // 5. Gather layout of the marked buckets, in bucket order.  A thread's
+  // consecutive buckets (at most kPerB) are held in registers from here on.
+  constexpr int kPerB = (kSelSplMax + 1 + kSelThreads - 1) / kSelThreads;
+  int iv[kPerB];
+  uint32_t cv[kPerB];
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) {
+    iv[u] = lo0 + u < hi0 ? info[lo0 + u] : 0;
+    cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
+  }
   uint32_t mc = 0, mv = 0;
-  for (int i = lo0; i < hi0; i++)
-    if (info[i] < 0) {
+#pragma unroll
+  for (int u = 0; u < kPerB; u++)
+    if (lo0 + u < hi0 && iv[u] < 0) {
       mc++;
-      mv += c[i];
+      mv += cv[u];
     }
   __syncthreads();
   part[tid] = mc;
@@ -2779,28 +2809,32 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     s_base = (int)atomicAdd(mlist_cnt, nmk);
   }
   __syncthreads();
-  for (int i = lo0; i < hi0; i++) {
-    if (info[i] < 0) {
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) {
+    const int i = lo0 + u;
+    if (i < hi0 && iv[u] < 0) {
       SelMark m;
       m.bucket = i;
       m.goff = gbase;
-      m.cnt = c[i];
-      m.vstart = B[i];
+      m.cnt = cv[u];
+      m.vstart = sB[i];
       m.cursor = 0;
       m.first_bound = 0;
       m.sum_hi = m.sum_lo = m.vmin = m.vmax = 0.0;
       mk[g.m_off + mbase] = m;
       mlist[s_base + mbase] = make_int2(j, (int)mbase);
-      info[i] = -1 - (int)mbase;
+      iv[u] = -1 - (int)mbase;
       mbase++;
-      gbase += c[i];
+      gbase += cv[u];
     }
   }
   // 6. Cluster of every other bucket: boundaries i >= 1 with vr_i <= its
   // start (a thread's buckets are consecutive, so a cursor follows them).
   int l6 = -1;
-  for (int i = lo0; i < hi0; i++) {
-    if (info[i] >= 0) {
+#pragma unroll
+  for (int u = 0; u < kPerB; u++) {
+    const int i = lo0 + u;
+    if (i < hi0 && iv[u] >= 0) {
       const int64_t r = sB[i];
       if (l6 < 0) {
         int l = 1, h = C;
@@ -2813,9 +2847,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
       } else {
         while (l6 < C && svr[l6] <= r) l6++;
       }
-      info[i] = l6 - 1;
+      iv[u] = l6 - 1;
     }
   }
+#pragma unroll
+  for (int u = 0; u < kPerB; u++)
+    if (lo0 + u < hi0) info[lo0 + u] = iv[u];
   // 7. Slice exponent per cluster from the value range of its buckets.
   const double vmin = key_value(st[j].kmin), vmax = key_value(st[j].kmax);
   for (int k = tid; k < C; k += blockDim.x) {
@@ -4316,7 +4353,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_select<4096><<<std::min(grid, 1024u), kSelectThreads, 0, fork_b ? side_ : stream_>>>(
           dsegs, dsel, dst, l3, dmlc + 5, dmk, dbnd, x);
       launched("k_sel_select");
-      k_sel_sorted_wave<2, 0><<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, l0, dmlc + 2, dmk, dbnd, x);
+      // One wave per bucket of the smallest class (the list is long; every
+      // bucket's chain of dependent descriptor loads then overlaps the others').
+      const unsigned grid_w = (unsigned)std::min<int64_t>((sel_m + 3) / 4, 65535);
+      k_sel_sorted_wave<2, 0><<<grid_w, 256, 0, stream_>>>(dsegs, dsel, dst, l0, dmlc + 2, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
       k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l1, dmlc + 3, dmk,
                                                                     dbnd, x);
