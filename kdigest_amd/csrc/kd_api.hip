@@ -1556,7 +1556,7 @@ extern "C" int kd_session_autotune_grouped(kd_session* s, int scale, int64_t num
     const int items[] = {4, 8, 16};
     const int64_t chunks[] = {1024, 2048, 4096, 8192, 16384};
     const int64_t tmins[] = {16384, 32768, 65536, 131072};
-    const int64_t tmaxs[] = {16, 32, 64, 128, 256};
+    const int64_t tmaxs[] = {16, 32, 64, 128, 256, 512, 1024};
     auto knob = [&](const auto& values, auto set) {
       for (auto v : values) {
         GroupedShape c = best;

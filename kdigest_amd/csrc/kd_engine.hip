@@ -1775,6 +1775,14 @@ __host__ __device__ inline int64_t sel_tile_len(int64_t v_cnt, int64_t tile_min,
   return t > lo ? t : lo;
 }
 
+// Tiles counted by one block of the count pass: enough for at least
+// kSelCountMin values (building a block's search table costs about as much
+// as counting tens of thousands of values).
+constexpr int64_t kSelCountMin = 65536;
+__host__ __device__ inline int64_t sel_count_group(int64_t tile_len) {
+  return tile_len >= kSelCountMin ? 1 : (kSelCountMin + tile_len - 1) / tile_len;
+}
+
 // LDS footprint class of the count and gather passes: 0 few splitters and
 // clusters, 1 many splitters and few clusters, 2 the rest.
 __host__ __device__ inline int sel_class(int cap_s, int cap_c) {
@@ -1991,8 +1999,8 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
 // none on average), and fs[s] counts the splitters whose slot is below s.
 // A value's bucket then lies among the splitters of its fine slot, fs[s] ..
 // fs[s + 1] (every splitter with the value's window is there), found by a
-// bisection of as many rounds as the fullest slot needs, the same for every
-// value of the block.
+// bisection of as many rounds as the fullest slot that the wave's values hit
+// needs.
 __host__ __device__ inline int sel_cell_shift(uint32_t span, int log_cells) {
   int bits = 0;
   while (bits < 32 && (span >> bits) != 0u) bits++;
@@ -2073,18 +2081,24 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
   __shared__ uint16_t fs[kFine + 1];
   __shared__ uint32_t cell[kCells];
   __shared__ unsigned long long smin, smax;
-  __shared__ int s_nslot, s_maxlen;
+  __shared__ int s_nslot;
   __shared__ unsigned int wmin[BLOCK / 64];
   const int64_t t = t0 + blockIdx.x;
   if (t >= t1) return;
   const int j = tmap[t];
   const SelSeg g = sel[j];
   if (g.cap_s <= SMIN || g.cap_s > SMAX) return;
+  // A block counts sel_count_group consecutive tiles of its segment (the
+  // blocks of the other tiles return), so that building the table is paid
+  // for at least kSelCountMin values.
+  const int64_t lt = t - g.tile0;
+  const int64_t grp = sel_count_group(g.tile_len);
+  if (lt % grp != 0) return;
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
   const double* sg = spl + g.b_off;
-  const int64_t a = (t - g.tile0) * g.tile_len;
-  const int64_t b = std::min<int64_t>(sd.v_cnt, a + g.tile_len);
+  const int64_t a = lt * g.tile_len;
+  const int64_t b = std::min<int64_t>(sd.v_cnt, a + grp * g.tile_len);
   const double* v = vin + sd.v_off;
   // The tile's first values are loaded before the table is built.
   constexpr int U = 8;
@@ -2114,7 +2128,6 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
   if (threadIdx.x == 0) {
     smin = ~0ull;
     smax = 0ull;
-    s_maxlen = 0;
   }
   __syncthreads();
   const uint64_t kf = S > 0 ? sk[0] : 0ull, kl = S > 0 ? sk[S - 1] : 0ull;
@@ -2197,18 +2210,6 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
   __syncthreads();
   block_suffix_min<BLOCK, (kFine + BLOCK) / BLOCK>(fs, nslot, wmin);
   __syncthreads();
-  int ml = 0;
-  constexpr int kPerF = (kFine + BLOCK - 1) / BLOCK;
-#pragma unroll
-  for (int u = 0; u < kPerF; u++) {
-    const int q = u * BLOCK + (int)threadIdx.x;
-    if (q < nslot) ml = max(ml, (int)fs[q + 1] - (int)fs[q]);
-  }
-  for (int d = 32; d > 0; d >>= 1) ml = max(ml, __shfl_xor(ml, d, 64));
-  if ((threadIdx.x & 63) == 0 && ml > 0) atomicMax(&s_maxlen, ml);
-  __syncthreads();
-  int rounds = 0;
-  while ((s_maxlen >> rounds) != 0) rounds++;
   double mn = __builtin_inf(), mx = -__builtin_inf();
   uint16_t* bx = bix + sd.v_off;
   // U values per thread searched together (independent LDS reads in
@@ -2241,7 +2242,14 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
       lo[u] = below ? 0 : (above ? S : f0);
       len[u] = (below || above) ? 0 : f1 - f0;
     }
-    // Upper bound of the value's key among the slot's splitters.
+    // Upper bound of the value's key among the slot's splitters, in as many
+    // rounds as the fullest slot this wave's values hit needs.
+    int ml = len[0];
+#pragma unroll
+    for (int u = 1; u < U; u++) ml = max(ml, len[u]);
+    for (int d = 32; d > 0; d >>= 1) ml = max(ml, __shfl_xor(ml, d, 64));
+    int rounds = 0;
+    while ((ml >> rounds) != 0) rounds++;
     for (int r = 0; r < rounds; r++) {
 #pragma unroll
       for (int u = 0; u < U; u++) {
