@@ -1779,6 +1779,7 @@ __host__ __device__ inline int64_t sel_tile_len(int64_t v_cnt, int64_t tile_min,
 // kSelCountMin values (building a block's search table costs about as much
 // as counting tens of thousands of values).
 constexpr int64_t kSelCountMin = 65536;
+constexpr int64_t kSelCountGroupTiles = 2048;  // tiles of a launch from which blocks group them
 __host__ __device__ inline int64_t sel_count_group(int64_t tile_len) {
   return tile_len >= kSelCountMin ? 1 : (kSelCountMin + tile_len - 1) / tile_len;
 }
@@ -2088,11 +2089,12 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
   const int j = tmap[t];
   const SelSeg g = sel[j];
   if (g.cap_s <= SMIN || g.cap_s > SMAX) return;
-  // A block counts sel_count_group consecutive tiles of its segment (the
-  // blocks of the other tiles return), so that building the table is paid
-  // for at least kSelCountMin values.
+  // With many tiles (the GPU is full anyway), a block counts
+  // sel_count_group consecutive tiles of its segment (the blocks of the
+  // other tiles return), so that building the table is paid for at least
+  // kSelCountMin values; a few tiles keep one block each.
   const int64_t lt = t - g.tile0;
-  const int64_t grp = sel_count_group(g.tile_len);
+  const int64_t grp = t1 - t0 >= kSelCountGroupTiles ? sel_count_group(g.tile_len) : 1;
   if (lt % grp != 0) return;
   const SegDesc sd = segs[g.si];
   const int S = st[j].nspl;
