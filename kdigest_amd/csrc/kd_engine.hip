@@ -4298,13 +4298,23 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       int ph_ = phase_begin("sel_count", sel_bytes * 1.25);
       // Two LDS classes by splitter capacity over every tile (a block whose
       // segment belongs to the other class returns at once): small tables
-      // run several blocks per CU.
-      k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, stream_>>>(
+      // run several blocks per CU, on the side stream when it is free (beside
+      // the large tables' blocks, one per CU, with LDS to spare).
+      const bool fork_c = nsel > 1 && !forked;
+      if (fork_c) {
+        check_hip(hipEventRecord(fork_, stream_), "fork count");
+        check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork count wait");
+      }
+      k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, fork_c ? side_ : stream_>>>(
           dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
       launched("k_sel_count<small>");
       k_sel_count<kSelSplMax, kSelSmallS, kSelThreads><<<(unsigned)sel_t, kSelThreads, 0, stream_>>>(
           dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
       launched("k_sel_count");
+      if (fork_c) {
+        check_hip(hipEventRecord(join_, side_), "join count");
+        check_hip(hipStreamWaitEvent(stream_, join_, 0), "join count wait");
+      }
       phase_end(ph_);
     }
     if (n_small > 0 && small_later) run_small(true);
