@@ -2909,17 +2909,18 @@ __global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* se
   constexpr int kItems = 4;                 // values per thread per chunk
   constexpr int kMk = 2 * CMAX + 2;          // gathered buckets of a segment at most
   static_assert(kMk < 65536, "marked-bucket index width");
-  // Replicas of the cluster accumulators (lane r adds into replica r % kRep):
+  // Replicas of the cluster accumulators (lane l adds into replica l % rep):
   // the central clusters of a large digest hold a third of its values each,
   // so one copy would serialize a wave's atomic adds.  The slices are exact
   // fixed-point values, so any grouping of the sums gives the same bits.
-  // Replica r starts r doubles past a multiple of 2 * CMAX, so the replicas
-  // of one cluster sit in different LDS banks as well.
-  constexpr int kRep = CMAX <= 256 ? 8 : 1;
-  constexpr int kRepStride = 2 * CMAX + 1;
+  // As many replicas as the segment's clusters leave room for (at most
+  // kMaxRep), each 2C + 1 doubles long, so the replicas of one cluster sit
+  // in different LDS banks as well.
+  constexpr int kAcc = (CMAX <= 256 ? 8 : 1) * (2 * CMAX + 1);
+  constexpr int kMaxRep = 16;
   __shared__ int inf[SMAX + 1];
   __shared__ int ex[CMAX];
-  __shared__ double acc[kRep * kRepStride];
+  __shared__ double acc[kAcc];
   __shared__ unsigned int lcnt[kMk];
   __shared__ unsigned long long lbase[kMk];  // place of the bucket's first staged value
   __shared__ unsigned int nst;
@@ -2970,11 +2971,13 @@ __global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* se
       if (i < C) ex[i] = tc[u];
     }
   }
-  for (int i = threadIdx.x; i < kRep * kRepStride; i += blockDim.x) acc[i] = 0.0;
+  const int stride = 2 * C + 1;
+  const int nrep = std::max(1, std::min(kMaxRep, kAcc / stride));
+  for (int i = threadIdx.x; i < nrep * stride; i += blockDim.x) acc[i] = 0.0;
   for (int i = threadIdx.x; i < nmk; i += blockDim.x) lcnt[i] = 0u;
   if (threadIdx.x == 0) nst = 0u;
   __syncthreads();
-  double* acc_r = acc + (threadIdx.x % kRep) * kRepStride;
+  double* acc_r = acc + (int)(threadIdx.x % (unsigned)nrep) * stride;
   double* sv = stv + sd.v_off + a;
   uint16_t* sm = stm + sd.v_off + a;
   const int lane = (int)(threadIdx.x & 63);
@@ -3045,8 +3048,7 @@ __global__ __launch_bounds__(kSelThreads, 8) void k_sel_gather(const SegDesc* se
   }
   for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) {
     double sum = 0.0;
-#pragma unroll
-    for (int r = 0; r < kRep; r++) sum += acc[r * kRepStride + i];
+    for (int r = 0; r < nrep; r++) sum += acc[r * stride + i];
     if (sum != 0.0) unsafeAtomicAdd(&csl[2 * g.c_off + i], sum);
   }
 }
