@@ -3339,9 +3339,13 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
   __shared__ uint32_t mh[kMultiSel][256];
   __shared__ double rh[kSelectThreads / 64], rl[kSelectThreads / 64];
   __shared__ unsigned long long smin, smax;
-  __shared__ unsigned long long s_pre[kMultiSel], s_rk[kMultiSel], gmx[kMultiSel];
-  __shared__ double g1[kMultiSel], g2[kMultiSel];
-  __shared__ unsigned int gcn[kMultiSel];
+  __shared__ unsigned long long s_pre[kMultiSel], s_rk[kMultiSel];
+  // Group sums, largest keys and counts in kGrpRep replicas (lane l adds into
+  // replica l % kGrpRep): a bucket's values mostly fall in one or two groups.
+  constexpr int kGrpRep = 8;
+  __shared__ double g1[kMultiSel * kGrpRep], g2[kMultiSel * kGrpRep];
+  __shared__ unsigned long long gmr[kMultiSel * kGrpRep];
+  __shared__ unsigned int gcn[kMultiSel * kGrpRep];
   __shared__ int s_ib0, s_ib1;
   const unsigned int total = *mlist_cnt;
   for (unsigned int gi = blockIdx.x; gi < total; gi += gridDim.x) {
@@ -3390,8 +3394,15 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
       kmn = k < kmn ? k : kmn;
       kmx = k > kmx ? k : kmx;
     });
-    atomicMin(&smin, kmn);
-    atomicMax(&smax, kmx);
+    for (int d = 32; d > 0; d >>= 1) {
+      const unsigned long long on = __shfl_xor(kmn, d, 64), ox = __shfl_xor(kmx, d, 64);
+      kmn = on < kmn ? on : kmn;
+      kmx = ox > kmx ? ox : kmx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&smin, kmn);
+      atomicMax(&smax, kmx);
+    }
     __syncthreads();
     const int ex = sel_exponent(key_value(smin), key_value(smax));
     double t1 = 0.0, t2 = 0.0;
@@ -3427,14 +3438,22 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
     __syncthreads();
     const int ib0 = s_ib0, ib1 = s_ib1;
     const int lane = threadIdx.x & 63;
+    // Every key of the bucket shares the leading bits of its smallest and
+    // largest: the radix passes start at the first bit where those differ.
+    const unsigned long long kmin_b = smin, kmax_b = smax;
+    const int common = kmin_b == kmax_b ? 64 : __clzll(kmin_b ^ kmax_b);
+    const unsigned long long cmask = common == 0 ? 0ull : (~0ull << (64 - common));
     for (int gb = ib0; gb < ib1; gb += kMultiSel) {
       const int nbg = min(kMultiSel, ib1 - gb);
       if ((int)threadIdx.x < nbg) {
-        s_pre[threadIdx.x] = 0ull;
+        s_pre[threadIdx.x] = kmin_b & cmask;
         s_rk[threadIdx.x] = (unsigned long long)(bd[gb + threadIdx.x].vr - vstart);
       }
-      unsigned long long pmask = 0ull;
-      for (int shift = 56; shift >= 0; shift -= 8) {
+      unsigned long long pmask = cmask;
+      for (int top = 64 - common; top > 0; top -= 8) {
+        const int width = top < 8 ? top : 8;
+        const int shift = top - width;
+        const uint32_t dmask = (1u << width) - 1u;
         for (int i = threadIdx.x; i < nbg * 256; i += blockDim.x) (&mh[0][0])[i] = 0u;
         __syncthreads();
         unsigned long long pre[kMultiSel];
@@ -3443,7 +3462,7 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
         each(a, [&](int64_t, double x) {
           const unsigned long long k = ord_key(x);
           const unsigned long long km = k & pmask;
-          const uint32_t d = (uint32_t)(k >> shift) & 255u;
+          const uint32_t d = (uint32_t)(k >> shift) & dmask;
 #pragma unroll
           for (int q = 0; q < kMultiSel; q++)
             if (q < nbg && km == pre[q]) atomicAdd(&mh[q][d], 1u);
@@ -3475,7 +3494,7 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
             cum += c[u];
           }
         }
-        pmask |= 255ull << shift;
+        pmask |= (unsigned long long)dmask << shift;
         __syncthreads();
       }
       // The selected keys ascend with the ranks: a value goes to the group
@@ -3484,13 +3503,14 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
       unsigned long long K[kMultiSel];
 #pragma unroll
       for (int q = 0; q < kMultiSel; q++) K[q] = q < nbg ? s_pre[q] : ~0ull;
-      if ((int)threadIdx.x < kMultiSel) {
+      if ((int)threadIdx.x < kMultiSel * kGrpRep) {
         g1[threadIdx.x] = 0.0;
         g2[threadIdx.x] = 0.0;
-        gmx[threadIdx.x] = 0ull;
+        gmr[threadIdx.x] = 0ull;
         gcn[threadIdx.x] = 0u;
       }
       __syncthreads();
+      const int rr = lane % kGrpRep;
       each(a, [&](int64_t, double x) {
         const unsigned long long k = ord_key(x);
         int q = 0;
@@ -3499,10 +3519,11 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
         if (q < nbg) {
           double s1, s2;
           sel_slices(x, ex, s1, s2);
-          atomicAdd(&g1[q], s1);
-          atomicAdd(&g2[q], s2);
-          atomicMax(&gmx[q], k);
-          atomicAdd(&gcn[q], 1u);
+          const int gi = q * kGrpRep + rr;
+          unsafeAtomicAdd(&g1[gi], s1);
+          unsafeAtomicAdd(&g2[gi], s2);
+          atomicMax(&gmr[gi], k);
+          atomicAdd(&gcn[gi], 1u);
         }
       });
       __syncthreads();
@@ -3511,10 +3532,12 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
         unsigned long long kl = 0ull;
         unsigned int cl = 0;
         for (int q = 0; q < nbg; q++) {
-          l1 += g1[q];
-          l2 += g2[q];
-          kl = gmx[q] > kl ? gmx[q] : kl;
-          cl += gcn[q];
+          for (int r = 0; r < kGrpRep; r++) {
+            l1 += g1[q * kGrpRep + r];
+            l2 += g2[q * kGrpRep + r];
+            kl = gmr[q * kGrpRep + r] > kl ? gmr[q * kGrpRep + r] : kl;
+            cl += gcn[q * kGrpRep + r];
+          }
           const dd sls = two_sum(l1, l2);
           SelBound& o = bd[gb + q];
           o.x_hi = key_value(K[q]);
