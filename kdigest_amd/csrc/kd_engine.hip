@@ -1739,6 +1739,17 @@ struct SelMark {
   double sum_hi, sum_lo, vmin, vmax;
 };
 
+// A gathered bucket of the smallest size class, as its wave reads it: where
+// its values are, its first new-value rank, its boundaries [b0, b1) (indices
+// into the segment's boundary table at c_off) and its mark.
+struct SelWave {
+  int64_t src;     // index of the bucket's first gathered value in X
+  int64_t vstart;  // new-value rank of the bucket's first value
+  int64_t c_off;   // the segment's boundary table
+  int64_t mark;    // index of the bucket's SelMark
+  int32_t n, b0, b1, pad;
+};
+
 struct SelBound {
   int64_t b;           // merged rank of the cluster's first record
   int64_t vr;          // new values before it
@@ -3090,11 +3101,12 @@ __device__ inline int sel_first_bound(const SelBound* bd, int C, int64_t r) {
 // by a chain of dependent loads in the kernel that finishes it.
 constexpr int kSelClasses = 4;
 
-__global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const SelState* st,
-                                                     SelMark* mk, const SelBound* bnd,
-                                                     const int2* mlist,
+__global__ __launch_bounds__(256) void k_sel_classes(const SegDesc* segs, const SelSeg* sel,
+                                                     const SelState* st, SelMark* mk,
+                                                     const SelBound* bnd, const int2* mlist,
                                                      const unsigned int* mlist_cnt, int2* out,
-                                                     int64_t stride, unsigned int* ccnt) {
+                                                     SelWave* wout, int64_t stride,
+                                                     unsigned int* ccnt) {
   const unsigned int total = *mlist_cnt;
   const int lane = threadIdx.x & 63;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -3103,13 +3115,26 @@ __global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const Se
     const unsigned int i = i0 + (unsigned int)lane;
     int c = -1;
     int2 e = make_int2(0, 0);
+    SelWave wv{};
     if (i < total) {
       e = mlist[i];
       const SelSeg g = sel[e.x];
       SelMark* mp = mk + g.m_off + e.y;
-      const int64_t n = mp->cnt;
-      c = n <= 128 ? 0 : n <= 1024 ? 1 : n <= 4096 ? 2 : 3;
-      mp->first_bound = sel_first_bound(bnd + g.c_off, st[e.x].ncl, mp->vstart);
+      const int64_t n = mp->cnt, vs = mp->vstart;
+      const int C = st[e.x].ncl;
+      const int fb = sel_first_bound(bnd + g.c_off, C, vs);
+      mp->first_bound = fb;
+      // Buckets of failed segments are finished by the sorting path.
+      c = st[e.x].fail ? -1 : (n <= 128 ? 0 : n <= 1024 ? 1 : n <= 4096 ? 2 : 3);
+      if (c == 0) {
+        wv.src = segs[g.si].x_off + mp->goff;
+        wv.vstart = vs;
+        wv.c_off = g.c_off;
+        wv.mark = g.m_off + e.y;
+        wv.n = (int)n;
+        wv.b0 = fb;
+        wv.b1 = sel_first_bound(bnd + g.c_off, C, vs + n);
+      }
     }
 #pragma unroll
     for (int q = 0; q < kSelClasses; q++) {
@@ -3119,7 +3144,11 @@ __global__ __launch_bounds__(256) void k_sel_classes(const SelSeg* sel, const Se
       const int leader = __ffsll((long long)m) - 1;
       if (lane == leader) base = atomicAdd(&ccnt[q], (unsigned int)__popcll(m));
       base = __shfl(base, leader, 64);
-      if (c == q) out[(int64_t)q * stride + base + (unsigned int)__popcll(m & below)] = e;
+      const unsigned int at = base + (unsigned int)__popcll(m & below);
+      if (c == q) {
+        if (q == 0) wout[at] = wv;
+        else out[(int64_t)q * stride + at] = e;
+      }
     }
   }
 }
@@ -3205,26 +3234,19 @@ __device__ inline dd wave_dd_sum(dd v) {
   return v;
 }
 
-template <int PER, int LO>
-__global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, const SelSeg* sel,
-                                                         const SelState* st, const int2* mlist,
-                                                         const unsigned int* mlist_cnt, SelMark* mk,
+template <int PER>
+__global__ __launch_bounds__(256) void k_sel_sorted_wave(const SelWave* wl,
+                                                         const unsigned int* wl_cnt, SelMark* mk,
                                                          SelBound* bnd, const double* X) {
-  const unsigned int total = *mlist_cnt;
+  const unsigned int total = *wl_cnt;
   const int lane = threadIdx.x & 63;
   const unsigned int waves = gridDim.x * (blockDim.x >> 6);
   const double inf = __builtin_inf();
   for (unsigned int gi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gi < total;
        gi += waves) {
-    const int2 e = mlist[gi];
-    const int j = e.x;
-    if (st[j].fail) continue;
-    const SelSeg g = sel[j];
-    SelMark* mp = mk + g.m_off + e.y;
-    const int64_t n = mp->cnt;
-    if (n <= LO || n > 64 * PER) continue;
-    const SegDesc sd = segs[g.si];
-    const double* src = X + sd.x_off + mp->goff;
+    const SelWave w = wl[gi];
+    const int64_t n = w.n;
+    const double* src = X + w.src;
     // Element lane * PER + p sits in v[p].
     double v[PER];
 #pragma unroll
@@ -3232,6 +3254,11 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
       const int idx = lane * PER + p;
       v[p] = idx < n ? src[idx] : inf;
     }
+    SelBound* bd = bnd + w.c_off;
+    // The boundaries' ranks, one per lane (64 at a time), the first 64
+    // loaded beside the values.
+    const int nb = w.b1 - w.b0;
+    int64_t vr_l = lane < nb ? bd[w.b0 + lane].vr : 0;
     for (int k = 2; k <= 64 * PER; k <<= 1) {
       for (int jj = k >> 1; jj > 0; jj >>= 1) {
         if (jj < PER) {
@@ -3274,16 +3301,15 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
     };
     const double vmin = elem(0), vmax = elem((int)n - 1);
     if (lane == 0) {
+      SelMark* mp = mk + w.mark;
       mp->sum_hi = tot.hi;
       mp->sum_lo = tot.lo;
       mp->vmin = vmin;
       mp->vmax = vmax;
     }
-    const int C = st[j].ncl;
-    SelBound* bd = bnd + g.c_off;
-    const int64_t vstart = mp->vstart;
-    for (int i = mp->first_bound; i < C && bd[i].vr < vstart + n; i++) {
-      const int rho = (int)(bd[i].vr - vstart);
+    for (int q = 0; q < nb; q++) {
+      if (q > 0 && (q & 63) == 0) vr_l = q + lane < nb ? bd[w.b0 + q + lane].vr : 0;
+      const int rho = (int)(lane_value(vr_l, q & 63) - w.vstart);
       dd sl{0.0, 0.0};
 #pragma unroll
       for (int p = 0; p < PER; p++)
@@ -3291,11 +3317,12 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SegDesc* segs, co
       const dd sls = wave_dd_sum(sl);
       const double xh = elem(rho), xl = rho > 0 ? elem(rho - 1) : 0.0;
       if (lane == 0) {
-        bd[i].x_hi = xh;
-        bd[i].x_lo = xl;
-        bd[i].dup = rho > 0 && xl == xh;
-        bd[i].sl_hi = sls.hi;
-        bd[i].sl_lo = sls.lo;
+        SelBound& o = bd[w.b0 + q];
+        o.x_hi = xh;
+        o.x_lo = xl;
+        o.dup = rho > 0 && xl == xh;
+        o.sl_hi = sls.hi;
+        o.sl_lo = sls.lo;
       }
     }
   }
@@ -4210,6 +4237,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   uint16_t* dbix = nullptr;
   uint16_t* dstm = nullptr;
   int* dtmap = nullptr;
+  SelWave* dwv = nullptr;
   // Small segments: overlap mode 0 runs them on the session stream after the
   // selection samples (whose blocks then start first); mode 1 on the side
   // stream once the selection count is done (the count keeps every CU busy
@@ -4271,6 +4299,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     // in vs (which only the sorting path uses, for other segments).
     dstm = as<uint16_t>(sel_stm_, std::max<int64_t>(total_v, 1));
     dtmap = as<int>(sel_tmap_, std::max<int64_t>(sel_t, 1));
+    dwv = as<SelWave>(sel_wv_, std::max<int64_t>(sel_m, 1));
     // Gathered-bucket list size, fallbacks, list sizes per size class.
     dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
     dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
@@ -4355,7 +4384,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
                                                     dbnd, dce, dcsl, dmk, dml, dmlc);
       launched("k_sel_plan");
       k_sel_classes<<<(unsigned)std::min<int64_t>((sel_m + 255) / 256, 1024), 256, 0, stream_>>>(
-          dsel, dst, dmk, dbnd, dml, dmlc, dmlx, sel_m, dmlc + 2);
+          dsegs, dsel, dst, dmk, dbnd, dml, dmlc, dmlx, dwv, sel_m, dmlc + 2);
       launched("k_sel_classes");
       phase_end(ph_);
     }
@@ -4383,7 +4412,6 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // Each size class walks its own list (grids sized for the worst case;
       // blocks past a list's end return at once).
       unsigned grid = (unsigned)std::min<int64_t>(sel_m, 4096);
-      const int2* l0 = dmlx;
       const int2* l1 = dmlx + sel_m;
       const int2* l2 = dmlx + 2 * sel_m;
       const int2* l3 = dmlx + 3 * sel_m;
@@ -4399,9 +4427,9 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
           dsegs, dsel, dst, l3, dmlc + 5, dmk, dbnd, x);
       launched("k_sel_select");
       // One wave per bucket of the smallest class (the list is long; every
-      // bucket's chain of dependent descriptor loads then overlaps the others').
+      // bucket's chain of dependent loads then overlaps the others').
       const unsigned grid_w = (unsigned)std::min<int64_t>((sel_m + 3) / 4, 65535);
-      k_sel_sorted_wave<2, 0><<<grid_w, 256, 0, stream_>>>(dsegs, dsel, dst, l0, dmlc + 2, dmk, dbnd, x);
+      k_sel_sorted_wave<2><<<grid_w, 256, 0, stream_>>>(dwv, dmlc + 2, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
       k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l1, dmlc + 3, dmk,
                                                                     dbnd, x);
