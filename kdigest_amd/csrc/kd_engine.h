@@ -303,7 +303,8 @@ class Engine {
   DevBuf extra_[4];
   // Selection path (large segments compressed without a full sort).
   DevBuf sel_seg_, sel_st_, sel_spl_, sel_cnt_, sel_bst_, sel_info_, sel_bnd_, sel_ce_, sel_csl_, sel_cwv_,
-      sel_mk_, sel_ml_, sel_mlc_, sel_bix_, sel_mlx_, sel_stm_, sel_tmap_, sel_wv_;
+      sel_mk_, sel_ml_, sel_mlc_, sel_bix_, sel_mlx_, sel_stm_, sel_tmap_, sel_wv_,
+      sel_lcnt_, sel_slist_, sel_clist_;
   std::vector<char> h_sel_;
   unsigned int* h_fallbacks_ = nullptr;  // pinned: selection fallbacks of the last batch
   bool sel_on_ = true;

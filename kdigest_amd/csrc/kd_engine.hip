@@ -1856,23 +1856,6 @@ __device__ inline dd dd_mul_d(dd a, double b) {
   return two_sum(p.hi, p.lo + a.lo * b);
 }
 
-// Per-segment selection state, the bucket counters and list sizes cleared,
-// and the segment of every tile of the count and gather passes (grid-stride).
-__global__ void k_sel_init(const SegDesc* segs, const SelSeg* sel, SelState* st, int nsel,
-                           unsigned int* cnt, int64_t ncnt, unsigned int* mlc, int nmlc,
-                           int* tmap) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
-  if (j < nmlc) mlc[j] = 0u;
-  for (int64_t i = j; i < nsel; i += stride) {
-    st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
-    const SelSeg g = sel[i];
-    const int64_t nt = (segs[g.si].v_cnt + g.tile_len - 1) / g.tile_len;
-    for (int64_t k = 0; k < nt; k++) tmap[g.tile0 + k] = (int)i;
-  }
-}
-
 // Sample size for a segment of nv new values (about two per splitter).
 __host__ __device__ inline int sel_sample_size(int64_t nv) {
   int64_t m = nv / 16;
@@ -1880,6 +1863,37 @@ __host__ __device__ inline int sel_sample_size(int64_t nv) {
   if (m > kSelSample) m = kSelSample;
   if (m > nv) m = nv;
   return (int)m;
+}
+
+// Per-segment selection state, the bucket counters and list sizes cleared,
+// the segment of every tile of the count and gather passes, and the work
+// lists (grid-stride): per splitter-sample size class its segments
+// (slist[c * nsel ...], lcnt[c]), per count instance its blocks' first tiles
+// (clist[i * sel_t ...], lcnt[3 + i]), so that each launch's working blocks
+// come first.  lcnt is zero on entry.
+__global__ void k_sel_init(const SegDesc* segs, const SelSeg* sel, SelState* st, int nsel,
+                           unsigned int* cnt, int64_t ncnt, unsigned int* mlc, int nmlc,
+                           int* tmap, int64_t sel_t, unsigned int* lcnt, int* slist, int* clist) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
+  if (j < nmlc) mlc[j] = 0u;
+  const int64_t grp_min = sel_t >= kSelCountGroupTiles ? kSelCountMin : 0;
+  for (int64_t i = j; i < nsel; i += stride) {
+    st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
+    const SelSeg g = sel[i];
+    const int64_t nv = segs[g.si].v_cnt;
+    const int64_t nt = (nv + g.tile_len - 1) / g.tile_len;
+    for (int64_t k = 0; k < nt; k++) tmap[g.tile0 + k] = (int)i;
+    const int need = std::max(sel_sample_size(nv), g.cap_s);
+    const int c = need <= 1024 ? 0 : need <= 4096 ? 1 : 2;
+    slist[(int64_t)c * nsel + atomicAdd(&lcnt[c], 1u)] = (int)i;
+    const int64_t grp = grp_min > 0 ? sel_count_group(g.tile_len) : 1;
+    const int64_t nblk = (nt + grp - 1) / grp;
+    const int ci = g.cap_s <= kSelSmallS ? 0 : 1;
+    const unsigned int base = atomicAdd(&lcnt[3 + ci], (unsigned int)nblk);
+    for (int64_t k = 0; k < nblk; k++) clist[ci * sel_t + base + k] = (int)(g.tile0 + k * grp);
+  }
 }
 
 // Splitters: a regular sample of the new values, sorted in LDS, merged with
@@ -1930,10 +1944,12 @@ template <int CAP, int BLOCK, int LO>
 __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const SelSeg* sel,
                                                       const double* vin, const double* esm,
                                                       double* spl, SelState* st,
-                                                      const double* presorted) {
+                                                      const double* presorted, const int* list,
+                                                      const unsigned int* lcnt) {
   __shared__ double sa[CAP + CAP / 16], sb[CAP + CAP / 16];
   __shared__ uint32_t part[BLOCK];
-  const int j = blockIdx.x;
+  if (blockIdx.x >= *lcnt) return;
+  const int j = list[blockIdx.x];
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
   const int64_t nv = sd.v_cnt;
@@ -2082,7 +2098,8 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
                                                      int nsel, int64_t t0, int64_t t1,
                                                      const double* vin, const double* spl,
                                                      SelState* st, unsigned int* cnt,
-                                                     uint16_t* bix, const int* tmap) {
+                                                     uint16_t* bix, const int* tmap,
+                                                     const int* list, const unsigned int* lcnt) {
   constexpr int kLogCells = SMAX >= 4096 ? 9 : 7;
   constexpr int kCells = 1 << kLogCells;
   constexpr int kFine = 2 * SMAX + kCells + 1;  // fine slots at most, plus the end
@@ -2095,8 +2112,10 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(const SegDesc* segs, const 
   __shared__ unsigned long long smin, smax;
   __shared__ int s_nslot;
   __shared__ unsigned int wmin[BLOCK / 64];
-  const int64_t t = t0 + blockIdx.x;
-  if (t >= t1) return;
+  // The blocks of this instance's work list (first tiles of its segments'
+  // tile groups) come first; the rest return.
+  if (blockIdx.x >= *lcnt) return;
+  const int64_t t = list[blockIdx.x];
   const int j = tmap[t];
   const SelSeg g = sel[j];
   if (g.cap_s <= SMIN || g.cap_s > SMAX) return;
@@ -4238,6 +4257,8 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   uint16_t* dstm = nullptr;
   int* dtmap = nullptr;
   SelWave* dwv = nullptr;
+  unsigned int* dlcnt = nullptr;
+  int *dslist = nullptr, *dclist = nullptr;
   // Small segments: overlap mode 0 runs them on the session stream after the
   // selection samples (whose blocks then start first); mode 1 on the side
   // stream once the selection count is done (the count keeps every CU busy
@@ -4300,14 +4321,19 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     dstm = as<uint16_t>(sel_stm_, std::max<int64_t>(total_v, 1));
     dtmap = as<int>(sel_tmap_, std::max<int64_t>(sel_t, 1));
     dwv = as<SelWave>(sel_wv_, std::max<int64_t>(sel_m, 1));
+    // Work lists of the sample and count launches and their sizes.
+    dlcnt = as<unsigned int>(sel_lcnt_, 8);
+    dslist = as<int>(sel_slist_, 3 * (int64_t)nsel);
+    dclist = as<int>(sel_clist_, 2 * std::max<int64_t>(sel_t, 1));
     // Gathered-bucket list size, fallbacks, list sizes per size class.
     dmlc = as<unsigned int>(sel_mlc_, 2 + kSelClasses);
     dmlx = as<int2>(sel_mlx_, kSelClasses * sel_m);
     {
       const int64_t items = std::max<int64_t>(std::max<int64_t>(sel_b, nsel), 2 + kSelClasses);
       const unsigned grid = (unsigned)std::min<int64_t>((items + 255) / 256, 1024);
+      check_hip(hipMemsetAsync(dlcnt, 0, 8 * sizeof(unsigned int), stream_), "memset lists");
       k_sel_init<<<grid, 256, 0, stream_>>>(dsegs, dsel, dst, nsel, dcnt, sel_b, dmlc, 2 + kSelClasses,
-                                            dtmap);
+                                            dtmap, sel_t, dlcnt, dslist, dclist);
       launched("k_sel_init");
     }
     {
@@ -4324,18 +4350,20 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // A host plan says which classes have segments; skip the others.
       const bool cls_known = (P.sample_cls & 8) != 0;
       if (!cls_known || (P.sample_cls & 1)) {
-        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst, nullptr);
+        k_sel_sample<1024, 64, 0><<<nsel, 64, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst, nullptr, dslist,
+                                                        dlcnt);
         launched("k_sel_sample<1024>");
       }
       if (!cls_known || (P.sample_cls & 2)) {
         k_sel_sample<4096, kBlock, 1024><<<nsel, kBlock, 0, s2>>>(dsegs, dsel, d_v, esm, dspl, dst,
-                                                                  nullptr);
+                                                                  nullptr, dslist + nsel, dlcnt + 1);
         launched("k_sel_sample<4096>");
       }
       // The largest class fills a CU's LDS anyway: a block of kSelThreads.
       if (!cls_known || (P.sample_cls & 4)) {
         k_sel_sample<kSelSample, kSelThreads, 4096><<<nsel, kSelThreads, 0, stream_>>>(
-            dsegs, dsel, d_v, esm, dspl, dst, nsel == 1 ? presorted_ : nullptr);
+            dsegs, dsel, d_v, esm, dspl, dst, nsel == 1 ? presorted_ : nullptr, dslist + 2 * (int64_t)nsel,
+            dlcnt + 2);
         launched("k_sel_sample<8192>");
       }
       if (fork_s) {
@@ -4360,10 +4388,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
         check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork count wait");
       }
       k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, fork_c ? side_ : stream_>>>(
-          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
+          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist, dlcnt + 3);
       launched("k_sel_count<small>");
       k_sel_count<kSelSplMax, kSelSmallS, kSelThreads><<<(unsigned)sel_t, kSelThreads, 0, stream_>>>(
-          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap);
+          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist + sel_t, dlcnt + 4);
       launched("k_sel_count");
       if (fork_c) {
         check_hip(hipEventRecord(join_, side_), "join count");
