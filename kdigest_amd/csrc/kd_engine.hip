@@ -1879,20 +1879,54 @@ __global__ void k_sel_init(const SegDesc* segs, const SelSeg* sel, SelState* st,
   for (int64_t i = j; i < ncnt; i += stride) cnt[i] = 0u;
   if (j < nmlc) mlc[j] = 0u;
   const int64_t grp_min = sel_t >= kSelCountGroupTiles ? kSelCountMin : 0;
-  for (int64_t i = j; i < nsel; i += stride) {
-    st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
-    const SelSeg g = sel[i];
-    const int64_t nv = segs[g.si].v_cnt;
-    const int64_t nt = (nv + g.tile_len - 1) / g.tile_len;
-    for (int64_t k = 0; k < nt; k++) tmap[g.tile0 + k] = (int)i;
-    const int need = std::max(sel_sample_size(nv), g.cap_s);
-    const int c = need <= 1024 ? 0 : need <= 4096 ? 1 : 2;
-    slist[(int64_t)c * nsel + atomicAdd(&lcnt[c], 1u)] = (int)i;
-    const int64_t grp = grp_min > 0 ? sel_count_group(g.tile_len) : 1;
-    const int64_t nblk = (nt + grp - 1) / grp;
-    const int ci = g.cap_s <= kSelSmallS ? 0 : 1;
-    const unsigned int base = atomicAdd(&lcnt[3 + ci], (unsigned int)nblk);
-    for (int64_t k = 0; k < nblk; k++) clist[ci * sel_t + base + k] = (int)(g.tile0 + k * grp);
+  // Segments in whole waves: list places are claimed with one atomic per
+  // wave and list (the counters are shared by every block).
+  const int lane = (int)(threadIdx.x & 63);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t ib = j - lane; ib < nsel; ib += stride) {
+    const int64_t i = ib + lane;
+    const bool act = i < nsel;
+    int c = -1, ci = -1;
+    int64_t nblk = 0, grp = 1, tile0 = 0;
+    if (act) {
+      st[i] = SelState{0, 0, 0, 0, ~0ull, 0ull, 0, 0};
+      const SelSeg g = sel[i];
+      const int64_t nv = segs[g.si].v_cnt;
+      const int64_t nt = (nv + g.tile_len - 1) / g.tile_len;
+      for (int64_t k = 0; k < nt; k++) tmap[g.tile0 + k] = (int)i;
+      const int need = std::max(sel_sample_size(nv), g.cap_s);
+      c = need <= 1024 ? 0 : need <= 4096 ? 1 : 2;
+      grp = grp_min > 0 ? sel_count_group(g.tile_len) : 1;
+      nblk = (nt + grp - 1) / grp;
+      ci = g.cap_s <= kSelSmallS ? 0 : 1;
+      tile0 = g.tile0;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const uint64_t m = __ballot(c == q);
+      if (m == 0ull) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned int base = 0u;
+      if (lane == leader) base = atomicAdd(&lcnt[q], (unsigned int)__popcll(m));
+      base = __shfl(base, leader, 64);
+      if (c == q) slist[(int64_t)q * nsel + base + (unsigned int)__popcll(m & below)] = (int)i;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      // Exclusive prefix of the blocks of this wave's segments of instance q.
+      const unsigned int mine = ci == q ? (unsigned int)nblk : 0u;
+      unsigned int inc = mine;
+      for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+      }
+      const unsigned int tot = __shfl(inc, 63, 64);
+      if (tot == 0u) continue;
+      unsigned int base = 0u;
+      if (lane == 0) base = atomicAdd(&lcnt[3 + q], tot);
+      base = __shfl(base, 0, 64) + inc - mine;
+      for (int64_t k = 0; k < (int64_t)mine; k++) clist[q * sel_t + base + k] = (int)(tile0 + k * grp);
+    }
   }
 }
 
