@@ -228,6 +228,16 @@ __global__ __launch_bounds__(256) void k_query_grouped(int is_cdf, int64_t nk, c
   out[i] = is_cdf ? cdf_one(pm, pw, pS, nc, st, q) : quantile_one(pm, pw, pS, nc, st, q);
 }
 
+// A digest's new centroids (means and weights) copied out of the engine's
+// output in one launch.
+__global__ void k_copy_pair(double* a, const double* sa, double* b, const double* sb, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    a[i] = sa[i];
+    b[i] = sb[i];
+  }
+}
+
 __global__ void k_scale_weights(double* w, int64_t n, double factor) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) w[i] *= factor;
@@ -634,8 +644,11 @@ struct kd_tdigest {
     int64_t c = res[0].count;
     cm.reserve(std::max<int64_t>(c, 1) * sizeof(double));
     cw.reserve(std::max<int64_t>(c, 1) * sizeof(double));
-    s->d2d(dm(), s->eng.cent_means() + segs[0].x_off, c * sizeof(double));
-    s->d2d(dw(), s->eng.cent_weights() + segs[0].x_off, c * sizeof(double));
+    if (c > 0) {
+      k_copy_pair<<<blocks_for(c), 256, 0, s->st()>>>(dm(), s->eng.cent_means() + segs[0].x_off, dw(),
+                                                       s->eng.cent_weights() + segs[0].x_off, c);
+      launched("k_copy_pair");
+    }
     nc = c;
     return res[0];
   }

@@ -143,6 +143,8 @@ struct DevBuf {
 // named after the kernel).
 void launched(const char* kernel);
 
+struct SelSeg;  // selection-path segment (kd_engine.hip)
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -173,7 +175,7 @@ class Engine {
   bool grouped_deferred() const { return defer_pending_; }
   bool finish_grouped(int scale, const PlanSummary& P, const double* d_v, const double* d_em,
                       const double* d_ew);
-  const SegDesc* dev_segments() const { return static_cast<const SegDesc*>(seg_.p); }
+  const SegDesc* dev_segments() const { return dsegs_; }
 
   const double* cent_means() const { return static_cast<const double*>(cm_.p); }
   const double* cent_weights() const { return static_cast<const double*>(cw_.p); }
@@ -305,8 +307,22 @@ class Engine {
   DevBuf sel_seg_, sel_st_, sel_spl_, sel_cnt_, sel_bst_, sel_info_, sel_bnd_, sel_ce_, sel_csl_, sel_cwv_,
       sel_mk_, sel_ml_, sel_mlc_, sel_bix_, sel_mlx_, sel_stm_, sel_tmap_, sel_wv_,
       sel_lcnt_, sel_slist_, sel_clist_;
-  std::vector<char> h_sel_;
   unsigned int* h_fallbacks_ = nullptr;  // pinned: selection fallbacks of the last batch
+  // compress(): the segment table, E-segment list, small-class lists and
+  // selection segments go up in one copy from page-locked staging (a copy
+  // from pageable memory holds the host for tens of microseconds), and the
+  // results come back through page-locked staging.
+  DevBuf up_;
+  void* h_up_ = nullptr;
+  size_t h_up_cap_ = 0;
+  void* h_res_ = nullptr;
+  size_t h_res_cap_ = 0;
+  // The device plan arrays of the current batch (in up_ after compress(), in
+  // seg_ / order_ / small_list_ / sel_seg_ after compress_grouped()).
+  SegDesc* dsegs_ = nullptr;
+  int* deseg_ = nullptr;
+  int* dsmall_ = nullptr;
+  SelSeg* dsel_ = nullptr;
   bool sel_on_ = true;
   int64_t sel_tile_min_ = 65536, sel_tiles_max_ = 64;
   int64_t path_counts_[3] = {0, 0, 0};

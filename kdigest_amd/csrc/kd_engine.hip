@@ -1729,7 +1729,8 @@ struct SelSeg {
 struct SelState {
   int32_t nspl, ncl, nmk, fail;
   unsigned long long kmin, kmax;  // order keys of the smallest / largest new value
-  int32_t wdone, pad_;            // 1: the chain left every cluster's running weight in cwv
+  int32_t wdone;                  // 1: the chain left every cluster's running weight in cwv
+  int32_t blast;                  // bucket of the last new-value rank (nv - 1), from the plan
 };
 
 struct SelMark {
@@ -1757,6 +1758,7 @@ struct SelBound {
   int32_t dup, pad;    // the value at new-value rank vr repeats below that rank
   double x_hi, x_lo;   // values at new-value ranks vr and vr - 1 (same bucket only)
   double sl_hi, sl_lo; // sum of the bucket's values below rank vr
+  int32_t bk_hi, bk_lo;  // buckets of new-value ranks vr and vr - 1 (-1: no such rank)
 };
 
 // Selection-path sizing of one digest (shared by the host and device plans).
@@ -2365,6 +2367,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
                                                            SelState* st) {
   __shared__ uint32_t part[kSelThreads];
   __shared__ int s_fail;
+  // Every 16th splitter (the last of each run of 16): a prior centroid's
+  // search runs there, then over its run's 16 splitters loaded at once.
+  constexpr int kFence = 16;
+  __shared__ double sfence[(kSelSplMax + kFence - 1) / kFence];
   const int j = blockIdx.x;
   const SelSeg g = sel[j];
   const SegDesc sd = segs[g.si];
@@ -2386,6 +2392,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   uint32_t cv[kPerB];
 #pragma unroll
   for (int u = 0; u < kPerB; u++) cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
+  const int nf = (S + kFence - 1) / kFence;
+  if (ne > 0)
+    for (int k = tid; k < nf; k += blockDim.x) sfence[k] = sp[std::min(k * kFence + kFence - 1, S - 1)];
   uint32_t acc = 0;
 #pragma unroll
   for (int u = 0; u < kPerB; u++) {
@@ -2409,11 +2418,26 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   // 2. Merged positions of the prior centroids: E precede equal new values.
   for (int i = tid; i < ne; i += blockDim.x) {
     double e = esm[sd.es_off + i];
-    int l = 0, h = S;  // first splitter >= e (e itself is a splitter)
+    int l = 0, h = nf;  // first run whose last splitter is >= e
     while (l < h) {
       int mid = (l + h) >> 1;
-      if (sp[mid] < e) l = mid + 1;
+      if (sfence[mid] < e) l = mid + 1;
       else h = mid;
+    }
+    // First splitter >= e (e itself is a splitter): inside run l, whose
+    // last splitter is >= e; S when there is no such run.
+    if (l < nf) {
+      const int r0 = l * kFence;
+      double v[kFence];
+#pragma unroll
+      for (int u = 0; u < kFence; u++) v[u] = r0 + u < S ? sp[r0 + u] : __builtin_inf();
+      int f = kFence - 1;
+#pragma unroll
+      for (int u = kFence - 1; u >= 0; u--)
+        if (!(v[u] < e)) f = u;
+      l = r0 + f;
+    } else {
+      l = S;
     }
     if (l >= S || !(sp[l] == e)) s_fail = 1;
     else pose[sd.es_off + i] = i + (int64_t)B[l + 1];
@@ -2796,6 +2820,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   // Bucket ranks and boundary ranks in LDS: every search below runs there.
   __shared__ unsigned int sB[kSelSplMax + 2];
   __shared__ int64_t svr[kSelCmax];
+  // The prior centroids' merged positions, when they fit.
+  constexpr int kPosLds = 2048;
+  __shared__ int64_t spos[kPosLds];
   const int j = blockIdx.x;
   if (st[j].fail) return;
   const SelSeg g = sel[j];
@@ -2829,25 +2856,35 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
       if (i <= S + 1) sB[i] = tb[u];
     }
   }
+  const int64_t* ps = pos;
+  if (ne <= kPosLds) {
+    for (int i = tid; i < ne; i += blockDim.x) spos[i] = pos[i];
+    ps = spos;
+  }
   __syncthreads();
-  // 4. Boundary records and the buckets to gather.
+  // 4. Boundary records and the buckets to gather (the buckets of the new
+  // values at and just below each boundary, which the finalize reads).
   for (int i = tid; i < C; i += blockDim.x) {
     int64_t b = bd[i].b;
     int l = 0, h = ne;  // prior centroids before merged rank b
     while (l < h) {
       int mid = (l + h) >> 1;
-      if (pos[mid] < b) l = mid + 1;
+      if (ps[mid] < b) l = mid + 1;
       else h = mid;
     }
     int64_t vr = b - l;
     bd[i].vr = vr;
     svr[i] = vr;
-    bd[i].v_hi = !(l < ne && pos[l] == b);
-    bd[i].v_lo = b > 0 && !(l > 0 && pos[l - 1] == b - 1);
+    bd[i].v_hi = !(l < ne && ps[l] == b);
+    bd[i].v_lo = b > 0 && !(l > 0 && ps[l - 1] == b - 1);
     bd[i].dup = 0;
+    const int bh = vr < nv ? sel_bucket_of_rank(sB, S, vr) : -1;
+    const int bl = vr >= 1 ? sel_bucket_of_rank(sB, S, vr - 1) : -1;
+    bd[i].bk_hi = bh;
+    bd[i].bk_lo = bl;
     if (i > 0) {
-      if (vr < nv) info[sel_bucket_of_rank(sB, S, vr)] = -1;
-      if (vr >= 1) info[sel_bucket_of_rank(sB, S, vr - 1)] = -1;
+      if (bh >= 0) info[bh] = -1;
+      if (bl >= 0) info[bl] = -1;
     }
   }
   __syncthreads();
@@ -2930,7 +2967,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   // 7. Slice exponent per cluster from the value range of its buckets.
   const double vmin = key_value(st[j].kmin), vmax = key_value(st[j].kmax);
   for (int k = tid; k < C; k += blockDim.x) {
-    int64_t va = bd[k].vr, vb = k + 1 < C ? bd[k + 1].vr : nv;
+    int64_t va = svr[k], vb = k + 1 < C ? svr[k + 1] : nv;
     int e = 0;
     if (vb > va) {
       int bf = sel_bucket_of_rank(sB, S, va), bl = sel_bucket_of_rank(sB, S, vb - 1);
@@ -2952,6 +2989,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   __syncthreads();
   if (tid == 0) {
     st[j].ncl = C;
+    st[j].blast = nv > 0 ? sel_bucket_of_rank(sB, S, nv - 1) : -1;
     if (s_fail) st[j].fail = 1;
   }
 }
@@ -3635,11 +3673,9 @@ __global__ __launch_bounds__(kSelectThreads) void k_sel_select(const SegDesc* se
 
 // Value of the new value at rank vr - 1 for boundary i (the previous
 // bucket's largest value when vr starts its bucket).
-__device__ inline double sel_value_below(const SelBound& bd, const unsigned int* B, int S,
-                                         const int* info, const SelMark* m) {
-  int bh = sel_bucket_of_rank(B, S, bd.vr);
-  int bl = sel_bucket_of_rank(B, S, bd.vr - 1);
-  if (bd.vr < (int64_t)B[S + 1] && bh == bl) return bd.x_lo;
+__device__ inline double sel_value_below(const SelBound& bd, const int* info, const SelMark* m) {
+  const int bh = bd.bk_hi, bl = bd.bk_lo;  // bh < 0: rank vr is past the new values
+  if (bh >= 0 && bh == bl) return bd.x_lo;
   if (info[bl] >= 0) return __builtin_nan("");  // not gathered: cannot happen by construction
   return m[-1 - info[bl]].vmax;
 }
@@ -3658,7 +3694,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
                                                          const SelState* st, const double* cwv,
                                                          const SelBound* bnd,
                                                          const SelMark* mk, const int* binfo,
-                                                         const unsigned int* bst, const double* csl,
+                                                         const double* csl,
                                                          const double* esm, const double* esw,
                                                          const int64_t* pose, double* cm,
                                                          double* cw, int64_t* ccount,
@@ -3669,10 +3705,9 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
   const SegDesc sd = segs[g.si];
   if (threadIdx.x == 0) s_fail = st[j].fail;
   __syncthreads();
-  const int S = st[j].nspl, C = st[j].ncl;
+  const int C = st[j].ncl;
   const int64_t nv = sd.v_cnt, ne = sd.e_cnt, n = nv + ne;
   const SelBound* bd = bnd + g.c_off;
-  const unsigned int* B = bst + g.b_off;
   const int* info = binfo + g.b_off;
   const SelMark* m = mk + g.m_off;
   const double* em = esm + sd.es_off;
@@ -3683,7 +3718,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
       const SelBound b = bd[i];
       const int64_t ce_ = b.b - b.vr;  // prior centroids before the boundary
       double xh = b.v_hi ? b.x_hi : em[ce_];
-      double xl = b.v_lo ? sel_value_below(b, B, S, info, m) : em[ce_ - 1];
+      double xl = b.v_lo ? sel_value_below(b, info, m) : em[ce_ - 1];
       bool dup = b.v_hi && b.v_lo && b.dup;
       if (dup || xh != xh || xl != xl || !(xl <= xh) || ord_key(xh) - ord_key(xl) <= 2ull) s_fail = 1;
     }
@@ -3715,7 +3750,8 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
     const double wrun = st[j].wdone ? cwv[g.c_off + c] : run_weight(ba.b, bb, sd.sf, pos, ew, ne, ea);
     dd sv = two_sum(csl[2 * (g.c_off + c)], csl[2 * (g.c_off + c) + 1]);
     if (nV > 0) {
-      int bf = sel_bucket_of_rank(B, S, vra), bl = sel_bucket_of_rank(B, S, vrb - 1);
+      // Buckets of the first and last new values of the cluster.
+      const int bf = ba.bk_hi, bl = c + 1 < C ? bd[c + 1].bk_lo : st[j].blast;
       if (info[bf] < 0) {
         const SelMark& ma = m[-1 - info[bf]];
         int64_t hi_r = std::min<int64_t>(vrb, ma.vstart + ma.cnt) - ma.vstart;
@@ -3735,7 +3771,7 @@ __global__ __launch_bounds__(kBlock) void k_sel_finalize(SegDesc* segs, const Se
     double xl;
     if (c + 1 < C) {
       const SelBound b1 = bd[c + 1];
-      xl = b1.v_lo ? sel_value_below(b1, B, S, info, m) : em[eb - 1];
+      xl = b1.v_lo ? sel_value_below(b1, info, m) : em[eb - 1];
     } else {
       bool last_v = !(ne > 0 && pos[ne - 1] == n - 1);
       xl = last_v ? vmax : em[ne - 1];
@@ -3992,6 +4028,8 @@ Engine::~Engine() {
   for (auto e : pool_) (void)hipEventDestroy(e);
   if (h_plan_) (void)hipHostFree(h_plan_);
   if (h_fallbacks_) (void)hipHostFree(h_fallbacks_);
+  if (h_up_) (void)hipHostFree(h_up_);
+  if (h_res_) (void)hipHostFree(h_res_);
   if (fork_) (void)hipEventDestroy(fork_);
   if (join_) (void)hipEventDestroy(join_);
   if (side_) (void)hipStreamDestroy(side_);
@@ -4008,7 +4046,7 @@ void Engine::sort_values(int nseg, const double* d_v, int64_t total_v) {
   double* t2 = as<double>(t2_, std::max<int64_t>(total_v, 1));
   double* x = static_cast<double*>(x_.p);
   double* vs = static_cast<double*>(vs_.p);
-  SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
+  SegDesc* dsegs = dsegs_;
   auto* rc = as<RouteCounters>(route_, 1);
   RouteCounters hrc;
 
@@ -4124,6 +4162,20 @@ void Engine::presort_samples(const double* d_v, int64_t n, int64_t batch, double
   launched("k_sel_presort");
 }
 
+// Page-locked host buffer of at least `need` bytes (grown by reallocation;
+// callers only grow it when no copy from it is in flight).
+static void* pinned_at_least(void*& p, size_t& cap, size_t need) {
+  if (need > cap) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t c = std::max<size_t>(need, 4096) * 2;
+    check_hip(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc");
+    cap = c;
+  }
+  return p;
+}
+
 void Engine::compress(int scale, SegVec& segs, const double* d_v,
                       const double* d_em, const double* d_ew, ResVec& res,
                       bool sync_results) {
@@ -4203,41 +4255,45 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
     }
     P.sel_t = t;
   }
-  // Upload the plan: segment table, E-segment list, small-class lists
-  // (largest class first), selection segments.
-  SegDesc* dsegs = as<SegDesc>(seg_, nseg);
-  check_hip(hipMemcpyAsync(dsegs, segs.data(), nseg * sizeof(SegDesc), hipMemcpyHostToDevice, stream_), "H2D segs");
-  int* deseg = as<int>(order_, std::max<size_t>(eseg.size(), 1));
-  if (!eseg.empty())
-    check_hip(hipMemcpyAsync(deseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D eseg");
+  // Upload the plan in one copy: segment table, E-segment list, small-class
+  // lists (largest class first), selection segments, each at a 256-byte
+  // aligned offset of the staging buffer and of up_.
   h_small_.clear();
   for (int c = kSmallClasses - 1; c >= 0; c--) h_small_.insert(h_small_.end(), small_[c].begin(), small_[c].end());
-  int* dl = as<int>(small_list_, std::max<size_t>(h_small_.size(), 1));
-  if (!h_small_.empty())
-    check_hip(hipMemcpyAsync(dl, h_small_.data(), h_small_.size() * sizeof(int), hipMemcpyHostToDevice,
-                             stream_),
-              "H2D small list");
-  if (!hsel.empty()) {
-    h_sel_.resize(hsel.size() * sizeof(SelSeg));
-    memcpy(h_sel_.data(), hsel.data(), h_sel_.size());
-    SelSeg* dsel = as<SelSeg>(sel_seg_, hsel.size());
-    check_hip(hipMemcpyAsync(dsel, h_sel_.data(), h_sel_.size(), hipMemcpyHostToDevice, stream_), "H2D sel");
-  }
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_e = al((size_t)nseg * sizeof(SegDesc));
+  const size_t o_sm = o_e + al(std::max<size_t>(eseg.size(), 1) * sizeof(int));
+  const size_t o_sel = o_sm + al(std::max<size_t>(h_small_.size(), 1) * sizeof(int));
+  const size_t up = o_sel + al(std::max<size_t>(hsel.size(), 1) * sizeof(SelSeg));
+  char* hu = static_cast<char*>(pinned_at_least(h_up_, h_up_cap_, up));
+  memcpy(hu, segs.data(), (size_t)nseg * sizeof(SegDesc));
+  if (!eseg.empty()) memcpy(hu + o_e, eseg.data(), eseg.size() * sizeof(int));
+  if (!h_small_.empty()) memcpy(hu + o_sm, h_small_.data(), h_small_.size() * sizeof(int));
+  if (!hsel.empty()) memcpy(hu + o_sel, hsel.data(), hsel.size() * sizeof(SelSeg));
+  char* du = as<char>(up_, up);
+  check_hip(hipMemcpyAsync(du, hu, up, hipMemcpyHostToDevice, stream_), "H2D plan");
+  dsegs_ = reinterpret_cast<SegDesc*>(du);
+  deseg_ = reinterpret_cast<int*>(du + o_e);
+  dsmall_ = reinterpret_cast<int*>(du + o_sm);
+  dsel_ = reinterpret_cast<SelSeg*>(du + o_sel);
   defer_pending_ = false;
   execute(scale, P, d_v, d_em, d_ew, 1);
   SegResult* dres = static_cast<SegResult*>(res_.p);
-  // The call ends synchronized: host staging read by the asynchronous copies
-  // above (segment table, lists) is then free to change.
+  SegResult* hres = static_cast<SegResult*>(
+      pinned_at_least(h_res_, h_res_cap_, (size_t)nseg * sizeof(SegResult)));
+  // The call ends synchronized: the staging read and written by the
+  // asynchronous copies (plan, results) is then free to change.
   if (sync_results)
-    check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
+    check_hip(hipMemcpyAsync(hres, dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
   check_hip(hipStreamSynchronize(stream_), "sync results");
   // Selection fallbacks (rare): their segments go through the sort path now.
   if (defer_pending_ && *h_fallbacks_ > 0) {
     execute(scale, P, d_v, d_em, d_ew, 2);
     if (sync_results)
-      check_hip(hipMemcpyAsync(res.data(), dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
+      check_hip(hipMemcpyAsync(hres, dres, nseg * sizeof(SegResult), hipMemcpyDeviceToHost, stream_), "D2H results");
     check_hip(hipStreamSynchronize(stream_), "sync results");
   }
+  if (sync_results) memcpy(res.data(), hres, (size_t)nseg * sizeof(SegResult));
   defer_pending_ = false;
   if (sync_results)
     for (const SegResult& r : res) path_counts_[r.path == 1 ? 0 : (r.path == 2 ? 2 : 1)]++;
@@ -4248,9 +4304,9 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   const bool fb_only = part == 2;
   const int nseg = (int)P.nseg;
   const int64_t total_e = P.total_e, total_x = P.total_x, total_t = P.total_t, total_v = P.total_v;
-  SegDesc* dsegs = static_cast<SegDesc*>(seg_.p);
-  int* deseg = static_cast<int*>(order_.p);
-  const int* dl = static_cast<const int*>(small_list_.p);
+  SegDesc* dsegs = dsegs_;
+  int* deseg = deseg_;
+  const int* dl = dsmall_;
   SegResult* dres = as<SegResult>(res_, nseg);
   double* esm = as<double>(em_s_, std::max<int64_t>(total_e, 1));
   double* esw = as<double>(ew_s_, std::max<int64_t>(total_e, 1));
@@ -4278,7 +4334,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   const int nsel = (int)P.nsel;
   const int64_t sel_b = P.sel_b, sel_c = P.sel_c, sel_m = P.sel_m;
   const int64_t t_small = P.t_small, t_mid = P.t_mid, sel_t = P.sel_t;
-  SelSeg* dsel = static_cast<SelSeg*>(sel_seg_.p);
+  SelSeg* dsel = dsel_;
   SelState* dst = nullptr;
   double *dspl = nullptr, *dcsl = nullptr, *dcwv = nullptr;
   unsigned int *dcnt = nullptr, *dbst = nullptr, *dmlc = nullptr;
@@ -4507,7 +4563,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
     }
     {
       int ph_ = phase_begin("sel_finalize", 0.0);
-      k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dcwv, dbnd, dmk, dinfo, dbst, dcsl,
+      k_sel_finalize<<<nsel, kBlock, 0, stream_>>>(dsegs, dsel, dst, dcwv, dbnd, dmk, dinfo, dcsl,
                                                    esm, esw, pose, cm, cw, ccount, dres, dmlc + 1);
       launched("k_sel_finalize");
       phase_end(ph_);
@@ -4940,6 +4996,10 @@ PlanSummary Engine::compress_grouped(const GroupedPlanArgs& a, const double* d_v
   int* dl = as<int>(small_list_, max_seg);
   int* deseg = as<int>(order_, max_seg);
   SelSeg* dsel = as<SelSeg>(sel_seg_, max_seg);
+  dsegs_ = dsegs;
+  dsmall_ = dl;
+  deseg_ = deseg;
+  dsel_ = dsel;
   const int64_t nb = (a.num_keys + kPlanThreads - 1) / kPlanThreads;
   int64_t* part = as<int64_t>(plan_part_, nb * kQCount + 3 * nb + 1);
   int64_t* pmax = part + nb * kQCount;
