@@ -783,6 +783,20 @@ __device__ inline double kinv_est(int scale, double k, double nrm) {
   }
 }
 
+// Out-of-line copies of the scale function, its inverse estimate and the
+// sweep's test for the long kernels (chain walk, one-block compress).
+// Inlined at each call site (every branch of the scale switch, with its
+// logarithms and divisions) they made such a kernel's code 50-100 KB, near or
+// above the size of the instruction cache.
+__device__ __noinline__ double chain_k(int scale, double q, double nrm) { return kd_k(scale, q, nrm); }
+__device__ __noinline__ double chain_kinv(int scale, double k, double nrm) {
+  return kinv_est(scale, k, nrm);
+}
+__device__ __noinline__ int chain_closes(int scale, double nrm, double stored_n, double wsf, double cur,
+                                         double w) {
+  return kd_sweep_closes(scale, nrm, stored_n, wsf, cur, w);
+}
+
 // Smallest j with P(j) >= T (an estimate: the exact predicate search below
 // corrects any rounding slack).
 __device__ int64_t index_for_weight(PCur& pc, int64_t n, double T) {
@@ -1300,7 +1314,7 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
       // No prior records: P(j) = sf * j + (prior weight before 0).
       const double w0 = sw[0];
       for (int j = threadIdx.x; j <= n; j += BLOCK)
-        kv[j] = kd_k(scale, (sd.sf * (double)j + w0) / sd.stored_n, sd.nrm);
+        kv[j] = chain_k(scale, (sd.sf * (double)j + w0) / sd.stored_n, sd.nrm);
     } else {
       for (int j = threadIdx.x; j <= n; j += BLOCK) {
         int lo = 0, hi = ne;  // prior records before merged index j
@@ -1310,7 +1324,7 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
           else hi = mid;
         }
         double P = sd.sf * (double)(j - lo) + sw[lo];
-        kv[j] = kd_k(scale, P / sd.stored_n, sd.nrm);
+        kv[j] = chain_k(scale, P / sd.stored_n, sd.nrm);
       }
     }
     __syncthreads();
@@ -1392,11 +1406,11 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
         // Running weight before the last record, and with it (one more add,
         // as the sweep takes it) the cluster's weight.
         const double wsf = T0[c];
-        const double kl = kd_k(scale, wsf / sd.stored_n, sd.nrm);
+        const double kl = chain_k(scale, wsf / sd.stored_n, sd.nrm);
         // The sweep's test with the reference's operands (kd_sweep_closes),
         // k(q_left) shared by both candidates.
         auto closes = [&](double cur, double w) {
-          return !(kd_k(scale, (wsf + cur + w) / sd.stored_n, sd.nrm) - kl <= 1.0);
+          return !(chain_k(scale, (wsf + cur + w) / sd.stored_n, sd.nrm) - kl <= 1.0);
         };
         double W = rec_weight(a, sd.sf, pos, ew, ne);
         bool bad = false;
@@ -1425,7 +1439,7 @@ __global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(
           double wi = sd.sf;
           if (ce < ne && pos[ce] == i) wi = ew[ce++];
           const double xi = x[i];
-          if (xi == cur_m || !kd_sweep_closes(scale, sd.nrm, sd.stored_n, wsf, cur_w, wi)) {
+          if (xi == cur_m || !chain_closes(scale, sd.nrm, sd.stored_n, wsf, cur_w, wi)) {
             const double nw = cur_w + wi;
             cur_m += (xi - cur_m) * (wi / nw);
             cur_w = nw;
@@ -1994,7 +2008,20 @@ __global__ __launch_bounds__(BLOCK) void k_sel_sample(const SegDesc* segs, const
   if (need <= LO || need > CAP) return;
   double* s;
   if (presorted) {
-    for (int i = threadIdx.x; i < m; i += BLOCK) sa[padi(i)] = presorted[i];
+    // Eight loads in flight per thread before their LDS stores.
+    for (int i0 = 0; i0 < m; i0 += 8 * BLOCK) {
+      double xs[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u * BLOCK + (int)threadIdx.x;
+        xs[u] = i < m ? presorted[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u * BLOCK + (int)threadIdx.x;
+        if (i < m) sa[padi(i)] = xs[u];
+      }
+    }
     __syncthreads();
     s = sa;
   } else {
@@ -2523,7 +2550,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     const int64_t c = gallop([&](int64_t i) { return pos[i] < jj; });
     return sf * (double)(jj - c) + sep[c];
   };
-  auto kv = [&](int64_t jj) -> double { return kd_k(scale, P(jj) / N, nrm); };
+  auto kv = [&](int64_t jj) -> double { return chain_k(scale, P(jj) / N, nrm); };
   // Smallest index with P >= T, estimated (index_for_weight), for T at or
   // above the weight before s.
   auto index_est = [&](double T) -> int64_t {
@@ -2555,7 +2582,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     int64_t lo = s, hi = n;
     double klo = 0.0;
     bool klo_set = false;
-    const double qt = kinv_est(scale, ks + 1.0, nrm);
+    const double qt = chain_kinv(scale, ks + 1.0, nrm);
     int64_t gg;
     if (!(qt < 1.0)) gg = n - 1;
     else if (!(qt > 0.0)) gg = s + 1;
@@ -2623,7 +2650,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     const int grp = (lane >= 4) + (lane >= 12) + (lane >= 22) + (lane >= 34) + (lane >= 48);
     const int gstart = kChainGroup[grp];
     const int gwidth = kChainGroup[grp + 1] - gstart;
-    const double qs = kinv_est(scale, ks + 1.0 + (double)grp, nrm);
+    const double qs = chain_kinv(scale, ks + 1.0 + (double)grp, nrm);
     int64_t gs;
     if (!(qs < 1.0)) gs = n - 1;
     else if (!(qs > 0.0)) gs = s + 1;
@@ -2749,11 +2776,11 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       if (c < C) {
         const int64_t a = bd[c].b, b = c + 1 < C ? bd[c + 1].b : n;
         const double wsf = wt[2 * c];
-        const double kl = kd_k(scale, wsf / N, nrm);
+        const double kl = chain_k(scale, wsf / N, nrm);
         // The sweep's test with the reference's operands (kd_sweep_closes),
         // k(q_left) shared by both candidates.
         auto closes = [&](double cur, double w) {
-          return !(kd_k(scale, (wsf + cur + w) / N, nrm) - kl <= 1.0);
+          return !(chain_k(scale, (wsf + cur + w) / N, nrm) - kl <= 1.0);
         };
         double W = rec_weight(a, sf, pos, ew, ne);
         if (b - a >= 2) {
@@ -2790,7 +2817,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
         const int64_t ce = seek(s2);
         auto closes = [&](int64_t i) {
           const double w0 = run_weight(s2, i, sf, pos, ew, ne, ce);
-          return kd_sweep_closes(scale, nrm, N, wsf, w0, rec_weight(i, sf, pos, ew, ne)) != 0;
+          return chain_closes(scale, nrm, N, wsf, w0, rec_weight(i, sf, pos, ew, ne)) != 0;
         };
         while (b < n && !closes(b)) b++;
         while (b - 1 > s2 && closes(b - 1)) b--;
