@@ -1219,7 +1219,7 @@ __global__ void k_cluster_mean(const SegDesc* segs, int nseg, const int64_t* cba
 // means by all lanes with the arithmetic of k_cluster_mean.  Replaces the
 // sort / merge / chain / sums launches for small digests.
 template <int CAP, int BLOCK>
-__global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 8 : 1)) void k_small_compress(int scale, const SegDesc* segs,
+__global__ __launch_bounds__(BLOCK, (CAP <= 512 ? 4 : 1)) void k_small_compress(int scale, const SegDesc* segs,
                                                           const int* list, int nlist,
                                                           const double* vin, const double* esm,
                                                           const double* esw, const double* se,
@@ -2546,11 +2546,17 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     return b;
   };
   // P(j) for j >= s: weight of the merged records before index j (PCur::P).
-  auto P = [&](int64_t jj) -> double {
+  // *cp: the prior records before jj.
+  auto Pc = [&](int64_t jj, int64_t* cp) -> double {
     const int64_t c = gallop([&](int64_t i) { return pos[i] < jj; });
+    *cp = c;
     return sf * (double)(jj - c) + sep[c];
   };
-  auto kv = [&](int64_t jj) -> double { return chain_k(scale, P(jj) / N, nrm); };
+  auto kvc = [&](int64_t jj, int64_t* cp) -> double { return chain_k(scale, Pc(jj, cp) / N, nrm); };
+  auto kv = [&](int64_t jj) -> double {
+    int64_t c;
+    return kvc(jj, &c);
+  };
   // Smallest index with P >= T, estimated (index_for_weight), for T at or
   // above the weight before s.
   auto index_est = [&](double T) -> int64_t {
@@ -2659,9 +2665,11 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
     const bool low = cand < s;    // before the current start: not evaluated
     const bool over = cand >= n;  // past the last record: a virtual true
     double kr = 0.0;
-    if (!low && !over) kr = kv(cand + 1);
+    int64_t cc = cur;  // prior records before cand + 1
+    if (!low && !over) kr = kvc(cand + 1, &cc);
     int64_t sp = s;
     double ksp = ks;
+    int64_t csp = cur;  // prior records before sp (the cursor for the next pass)
     int done = 0;
     bool end = false;
 #pragma unroll
@@ -2678,12 +2686,14 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       // f (candidate b - 1 >= sp, evaluated) is in the group: then its
       // k(P(b) / N) is the next start's k.
       double kb;
+      int64_t cb = csp;
       const double kprev = lane_value(kr, f > 0 ? f - 1 : 0);
       if (f == kChainGroup[d]) {
         if (b != sp + 1) break;
-        kb = b < n ? kv(b) : 0.0;
+        kb = b < n ? kvc(b, &cb) : 0.0;
       } else {
         kb = kprev;
+        cb = lane_value(cc, f - 1);
       }
       done++;
       if (b >= n) {  // the open cluster runs to the end
@@ -2696,6 +2706,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       }
       sp = b;
       ksp = kb;
+      csp = cb;
       if (b == n - 1) {  // the last record closes its own cluster
         end = true;
         break;
@@ -2712,10 +2723,13 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_sel_chain(int scale, const
       }
       sp = b;
       ksp = kn;
+      csp = -1;
     }
     s = sp;
     ks = ksp;
-    while (cur < ne && pos[cur] < s) cur++;
+    if (csp >= 0) cur = csp;
+    else
+      while (cur < ne && pos[cur] < s) cur++;
   }
   // Unless the closed-form prefix is exact (integer weights), every boundary
   // is checked in the CPU's summation order (kd_exact.h): the running weight
