@@ -2861,6 +2861,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   // Bucket ranks and boundary ranks in LDS: every search below runs there.
   __shared__ unsigned int sB[kSelSplMax + 2];
   __shared__ int64_t svr[kSelCmax];
+  __shared__ int sbh[kSelCmax], sbl[kSelCmax];  // buckets at and below each boundary
   // The prior centroids' merged positions, when they fit.
   constexpr int kPosLds = 2048;
   __shared__ int64_t spos[kPosLds];
@@ -2920,9 +2921,14 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     bd[i].v_lo = b > 0 && !(l > 0 && ps[l - 1] == b - 1);
     bd[i].dup = 0;
     const int bh = vr < nv ? sel_bucket_of_rank(sB, S, vr) : -1;
-    const int bl = vr >= 1 ? sel_bucket_of_rank(sB, S, vr - 1) : -1;
+    // Rank vr - 1 is mostly in vr's bucket (one LDS read tells).
+    const int bl = vr < 1 ? -1
+                   : (bh >= 0 && (int64_t)sB[bh] <= vr - 1) ? bh
+                                                             : sel_bucket_of_rank(sB, S, vr - 1);
     bd[i].bk_hi = bh;
     bd[i].bk_lo = bl;
+    sbh[i] = bh;
+    sbl[i] = bl;
     if (i > 0) {
       if (bh >= 0) info[bh] = -1;
       if (bl >= 0) info[bl] = -1;
@@ -3011,7 +3017,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     int64_t va = svr[k], vb = k + 1 < C ? svr[k + 1] : nv;
     int e = 0;
     if (vb > va) {
-      int bf = sel_bucket_of_rank(sB, S, va), bl = sel_bucket_of_rank(sB, S, vb - 1);
+      // Buckets of the cluster's first and last new values (va < vb <= nv).
+      const int bf = sbh[k], bl = k + 1 < C ? sbl[k + 1] : sel_bucket_of_rank(sB, S, nv - 1);
       double lo = bf == 0 ? vmin : sp[bf - 1];
       double hi = bl == S ? vmax : sp[bl];
       double m = fmax(fabs(lo), fabs(hi));
