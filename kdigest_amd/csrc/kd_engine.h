@@ -61,7 +61,8 @@ struct PlanSummary {
   int64_t total_e, total_x, total_t, total_v;
   double sel_v;                   // new values on the selection path
   int64_t bad;                    // 1: invalid batch; 2: too many prior centroids in a digest
-  int64_t sample_cls;             // 8 | bit c: some segment samples in class c (0: unknown)
+  int64_t sample_cls;             // 8 | bit c: some segment samples in class c; 16 / 32: some
+                                  // segment counts with the small / large table (0: unknown)
 };
 
 // Inputs of the device-side plan of a grouped batch (per-key B-semantics

@@ -4262,6 +4262,7 @@ void Engine::compress(int scale, SegVec& segs, const double* d_v,
       {
         const int64_t need = std::max<int64_t>(sel_sample_size(sd.v_cnt), g.cap_s);
         P.sample_cls |= 8 | (need <= 1024 ? 1 : need <= 4096 ? 2 : 4);
+        P.sample_cls |= g.cap_s <= kSelSmallS ? 16 : 32;  // count instance of the segment
       }
       hsel.push_back(g);
       P.sel_v += (double)sd.v_cnt;
@@ -4520,17 +4521,26 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // segment belongs to the other class returns at once): small tables
       // run several blocks per CU, on the side stream when it is free (beside
       // the large tables' blocks, one per CU, with LDS to spare).
-      const bool fork_c = nsel > 1 && !forked;
+      // With a host plan, an instance none of whose segments is there is not
+      // launched.
+      const bool cnt_known = (P.sample_cls & 8) != 0;
+      const bool cnt_small = !cnt_known || (P.sample_cls & 16);
+      const bool cnt_large = !cnt_known || (P.sample_cls & 32);
+      const bool fork_c = nsel > 1 && !forked && cnt_small && cnt_large;
       if (fork_c) {
         check_hip(hipEventRecord(fork_, stream_), "fork count");
         check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork count wait");
       }
-      k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, fork_c ? side_ : stream_>>>(
-          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist, dlcnt + 3);
-      launched("k_sel_count<small>");
-      k_sel_count<kSelSplMax, kSelSmallS, kSelThreads><<<(unsigned)sel_t, kSelThreads, 0, stream_>>>(
-          dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist + sel_t, dlcnt + 4);
-      launched("k_sel_count");
+      if (cnt_small) {
+        k_sel_count<kSelSmallS, 0, 256><<<(unsigned)sel_t, 256, 0, fork_c ? side_ : stream_>>>(
+            dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist, dlcnt + 3);
+        launched("k_sel_count<small>");
+      }
+      if (cnt_large) {
+        k_sel_count<kSelSplMax, kSelSmallS, kSelThreads><<<(unsigned)sel_t, kSelThreads, 0, stream_>>>(
+            dsegs, dsel, nsel, 0, sel_t, d_v, dspl, dst, dcnt, dbix, dtmap, dclist + sel_t, dlcnt + 4);
+        launched("k_sel_count");
+      }
       if (fork_c) {
         check_hip(hipEventRecord(join_, side_), "join count");
         check_hip(hipStreamWaitEvent(stream_, join_, 0), "join count wait");
