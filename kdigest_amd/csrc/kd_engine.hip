@@ -3954,11 +3954,44 @@ __global__ void k_finish_lane(const SegDesc* segs, int nseg, const int64_t* ccou
 constexpr int kFinishWaves = 4;
 constexpr int kFinishWaveSegs = 2048;  // at most this many segments take the wave form
 
+// The zero-extreme rule of k_zero_signs for one segment, by one wave (the
+// few-segment finish does it in the same launch).  Lane 0 wrote res[si].
+__device__ inline void wave_zero_sign(const SegDesc& sd, int si, const double* v, SegResult* res,
+                                      int lane) {
+  int need = 0;
+  if (lane == 0) need = sd.v_cnt > 0 && (res[si].vmin == 0.0 || res[si].vmax == 0.0);
+  if (__shfl(need, 0, 64) == 0) return;
+  const double* x = v + sd.v_off;
+  const int64_t cnt = sd.v_cnt;
+  for (int64_t c0 = 0; c0 < cnt; c0 += 64 * 8) {
+    double r[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int64_t i = c0 + j * 64 + lane;
+      r[j] = i < cnt ? x[i] : 1.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const uint64_t m = __ballot(c0 + j * 64 + lane < cnt && r[j] == 0.0);
+      if (m != 0ull) {
+        if (lane == 0) {
+          const double z = x[c0 + j * 64 + (__ffsll((long long)m) - 1)];
+          if (res[si].vmin == 0.0) res[si].vmin = z;
+          if (res[si].vmax == 0.0) res[si].vmax = z;
+        }
+        return;
+      }
+    }
+  }
+}
+
+// v: the batch's values when the finish also applies the zero-extreme rule
+// (nullptr: k_zero_signs runs after it).
 __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* segs, int nseg,
                                                              const int64_t* ccount, double* cm,
                                                              double* cw, const double* X,
                                                              const double* vs, SegResult* res,
-                                                             int only) {
+                                                             int only, const double* v) {
   const int lane = threadIdx.x & 63;
   const int si = blockIdx.x * kFinishWaves + (threadIdx.x >> 6);
   if (si >= nseg) return;
@@ -4006,6 +4039,7 @@ __global__ __launch_bounds__(64 * kFinishWaves) void k_finish(const SegDesc* seg
     }
   }
   if (lane == 0) finish_result(sd, si, C, total, removed, X, vs, res);
+  if (v) wave_zero_sign(sd, si, v, res, lane);
 }
 
 // ---------------------------------------------------------------- engine
@@ -4704,16 +4738,19 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
   }
   {
     int ph_ = phase_begin("finish", 0.0);
-    if (nseg <= kFinishWaveSegs)
+    if (nseg <= kFinishWaveSegs) {
+      // Few segments: the zero-extreme rule in the same launch.
       k_finish<<<(nseg + kFinishWaves - 1) / kFinishWaves, 64 * kFinishWaves, 0, stream_>>>(
-          dsegs, nseg, ccount, cm, cw, x, vs, dres, finish_only);
-    else
+          dsegs, nseg, ccount, cm, cw, x, vs, dres, finish_only, d_v);
+      launched("k_finish");
+    } else {
       k_finish_lane<<<(nseg + 63) / 64, 64, 0, stream_>>>(dsegs, nseg, ccount, cm, cw, x, vs, dres,
                                                           finish_only);
-    launched("k_finish");
-    k_zero_signs<<<(unsigned)std::min((nseg + 255) / 256, 1024), 256, 0, stream_>>>(dsegs, nseg, d_v,
-                                                                                   dres);
-    launched("k_zero_signs");
+      launched("k_finish");
+      k_zero_signs<<<(unsigned)std::min((nseg + 255) / 256, 1024), 256, 0, stream_>>>(dsegs, nseg,
+                                                                                     d_v, dres);
+      launched("k_zero_signs");
+    }
     phase_end(ph_);
   }
   check_hip(hipGetLastError(), "kernel launch");
