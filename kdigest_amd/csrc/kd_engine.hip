@@ -2863,8 +2863,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   __shared__ int64_t svr[kSelCmax];
   __shared__ int sbh[kSelCmax], sbl[kSelCmax];  // buckets at and below each boundary
   // The prior centroids' merged positions, when they fit.
-  constexpr int kPosLds = 2048;
+  constexpr int kPosLds = 2 * kSelThreads;
   __shared__ int64_t spos[kPosLds];
+  // Buckets marked for the gather (one bit each).
+  __shared__ uint32_t smark[(kSelSplMax + 1 + 31) / 32];
   const int j = blockIdx.x;
   if (st[j].fail) return;
   const SelSeg g = sel[j];
@@ -2884,30 +2886,48 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
   const int lo0 = std::min(nb, tid * per), hi0 = std::min(nb, lo0 + per);
   if (tid == 0) s_fail = 0;
   const int C = st[j].ncl;
+  // The block's first global reads all in flight together: bucket starts,
+  // bucket counts (a thread's consecutive buckets, at most kPerB, held in
+  // registers from here on), its boundary's merged rank, prior positions.
+  constexpr int kPerB = (kSelSplMax + 1 + kSelThreads - 1) / kSelThreads;
+  uint32_t cv[kPerB];
+  int64_t b_own = 0;
+  const bool pos_lds = ne <= kPosLds;
   {
     constexpr int kPerS = (kSelSplMax + 2 + kSelThreads - 1) / kSelThreads;
     uint32_t tb[kPerS];
+    int64_t pv[2];
 #pragma unroll
     for (int u = 0; u < kPerS; u++) {
       const int i = u * kSelThreads + tid;
       tb[u] = i <= S + 1 ? B[i] : 0u;
     }
 #pragma unroll
+    for (int u = 0; u < kPerB; u++) cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
+    if (tid < C) b_own = bd[tid].b;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int i = u * kSelThreads + tid;
+      pv[u] = pos_lds && i < ne ? pos[i] : 0;
+    }
+#pragma unroll
     for (int u = 0; u < kPerS; u++) {
       const int i = u * kSelThreads + tid;
       if (i <= S + 1) sB[i] = tb[u];
     }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int i = u * kSelThreads + tid;
+      if (pos_lds && i < ne) spos[i] = pv[u];
+    }
+    for (int i = tid; i < (kSelSplMax + 1 + 31) / 32; i += kSelThreads) smark[i] = 0u;
   }
-  const int64_t* ps = pos;
-  if (ne <= kPosLds) {
-    for (int i = tid; i < ne; i += blockDim.x) spos[i] = pos[i];
-    ps = spos;
-  }
+  const int64_t* ps = pos_lds ? spos : pos;
   __syncthreads();
   // 4. Boundary records and the buckets to gather (the buckets of the new
   // values at and just below each boundary, which the finalize reads).
   for (int i = tid; i < C; i += blockDim.x) {
-    int64_t b = bd[i].b;
+    const int64_t b = i == tid ? b_own : bd[i].b;
     int l = 0, h = ne;  // prior centroids before merged rank b
     while (l < h) {
       int mid = (l + h) >> 1;
@@ -2930,20 +2950,18 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_plan(
     sbh[i] = bh;
     sbl[i] = bl;
     if (i > 0) {
-      if (bh >= 0) info[bh] = -1;
-      if (bl >= 0) info[bl] = -1;
+      if (bh >= 0) atomicOr(&smark[bh >> 5], 1u << (bh & 31));
+      if (bl >= 0) atomicOr(&smark[bl >> 5], 1u << (bl & 31));
     }
   }
   __syncthreads();
   // 5. Gather layout of the marked buckets, in bucket order.  A thread's
   // consecutive buckets (at most kPerB) are held in registers from here on.
-  constexpr int kPerB = (kSelSplMax + 1 + kSelThreads - 1) / kSelThreads;
   int iv[kPerB];
-  uint32_t cv[kPerB];
 #pragma unroll
   for (int u = 0; u < kPerB; u++) {
-    iv[u] = lo0 + u < hi0 ? info[lo0 + u] : 0;
-    cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
+    const int i = lo0 + u;
+    iv[u] = i < hi0 && ((smark[i >> 5] >> (i & 31)) & 1u) ? -1 : 0;
   }
   uint32_t mc = 0, mv = 0;
 #pragma unroll
