@@ -2419,6 +2419,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   uint32_t cv[kPerB];
 #pragma unroll
   for (int u = 0; u < kPerB; u++) cv[u] = lo0 + u < hi0 ? c[lo0 + u] : 0u;
+  // This thread's first prior mean, read beside the counts.
+  const double e_own = tid < ne ? esm[sd.es_off + tid] : 0.0;
   const int nf = (S + kFence - 1) / kFence;
   if (ne > 0)
     for (int k = tid; k < nf; k += blockDim.x) sfence[k] = sp[std::min(k * kFence + kFence - 1, S - 1)];
@@ -2444,7 +2446,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_ranks(const SegDesc* segs, 
   __syncthreads();
   // 2. Merged positions of the prior centroids: E precede equal new values.
   for (int i = tid; i < ne; i += blockDim.x) {
-    double e = esm[sd.es_off + i];
+    const double e = i == tid ? e_own : esm[sd.es_off + i];
     int l = 0, h = nf;  // first run whose last splitter is >= e
     while (l < h) {
       int mid = (l + h) >> 1;
