@@ -3393,97 +3393,107 @@ __device__ inline dd wave_dd_sum(dd v) {
   return v;
 }
 
+// One bucket of at most 64 PER values, sorted by its wave (element
+// lane * PER + p in v[p]).
 template <int PER>
+__device__ inline void sel_sorted_wave_one(const SelWave& w, int lane, SelMark* mk, SelBound* bnd,
+                                           const double* X) {
+  const double inf = __builtin_inf();
+  const int64_t n = w.n;
+  const double* src = X + w.src;
+  // Element lane * PER + p sits in v[p].
+  double v[PER];
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int idx = lane * PER + p;
+    v[p] = idx < n ? src[idx] : inf;
+  }
+  SelBound* bd = bnd + w.c_off;
+  // The boundaries' ranks, one per lane (64 at a time), the first 64
+  // loaded beside the values.
+  const int nb = w.b1 - w.b0;
+  int64_t vr_l = lane < nb ? bd[w.b0 + lane].vr : 0;
+  for (int k = 2; k <= 64 * PER; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      if (jj < PER) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+          const int q = p ^ jj;
+          if (q > p) {
+            const int idx = lane * PER + p;
+            const bool asc = (idx & k) == 0;
+            const double lo = __builtin_fmin(v[p], v[q]), hi = __builtin_fmax(v[p], v[q]);
+            v[p] = asc ? lo : hi;
+            v[q] = asc ? hi : lo;
+          }
+        }
+      } else {
+        const int lm = jj / PER;
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+          const int idx = lane * PER + p;
+          const double o = __shfl_xor(v[p], lm, 64);
+          const bool asc = (idx & k) == 0, lower = (idx & jj) == 0;
+          const double lo = __builtin_fmin(v[p], o), hi = __builtin_fmax(v[p], o);
+          v[p] = (asc == lower) ? lo : hi;
+        }
+      }
+    }
+  }
+  dd acc{0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < PER; p++)
+    if (lane * PER + p < n) acc = dd_add(acc, dd{v[p], 0.0});
+  const dd tot = wave_dd_sum(acc);
+  // Element r of the sorted bucket (every lane gets it).
+  auto elem = [&](int r) {
+    double x = v[0];
+#pragma unroll
+    for (int p = 1; p < PER; p++)
+      if ((r % PER) == p) x = v[p];
+    return __shfl(x, r / PER, 64);
+  };
+  const double vmin = elem(0), vmax = elem((int)n - 1);
+  if (lane == 0) {
+    SelMark* mp = mk + w.mark;
+    mp->sum_hi = tot.hi;
+    mp->sum_lo = tot.lo;
+    mp->vmin = vmin;
+    mp->vmax = vmax;
+  }
+  for (int q = 0; q < nb; q++) {
+    if (q > 0 && (q & 63) == 0) vr_l = q + lane < nb ? bd[w.b0 + q + lane].vr : 0;
+    const int rho = (int)(lane_value(vr_l, q & 63) - w.vstart);
+    dd sl{0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < PER; p++)
+      if (lane * PER + p < rho) sl = dd_add(sl, dd{v[p], 0.0});
+    const dd sls = wave_dd_sum(sl);
+    const double xh = elem(rho), xl = rho > 0 ? elem(rho - 1) : 0.0;
+    if (lane == 0) {
+      SelBound& o = bd[w.b0 + q];
+      o.x_hi = xh;
+      o.x_lo = xl;
+      o.dup = rho > 0 && xl == xh;
+      o.sl_hi = sls.hi;
+      o.sl_lo = sls.lo;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SelWave* wl,
                                                          const unsigned int* wl_cnt, SelMark* mk,
                                                          SelBound* bnd, const double* X) {
   const unsigned int total = *wl_cnt;
   const int lane = threadIdx.x & 63;
   const unsigned int waves = gridDim.x * (blockDim.x >> 6);
-  const double inf = __builtin_inf();
   for (unsigned int gi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); gi < total;
        gi += waves) {
     const SelWave w = wl[gi];
-    const int64_t n = w.n;
-    const double* src = X + w.src;
-    // Element lane * PER + p sits in v[p].
-    double v[PER];
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
-      const int idx = lane * PER + p;
-      v[p] = idx < n ? src[idx] : inf;
-    }
-    SelBound* bd = bnd + w.c_off;
-    // The boundaries' ranks, one per lane (64 at a time), the first 64
-    // loaded beside the values.
-    const int nb = w.b1 - w.b0;
-    int64_t vr_l = lane < nb ? bd[w.b0 + lane].vr : 0;
-    for (int k = 2; k <= 64 * PER; k <<= 1) {
-      for (int jj = k >> 1; jj > 0; jj >>= 1) {
-        if (jj < PER) {
-#pragma unroll
-          for (int p = 0; p < PER; p++) {
-            const int q = p ^ jj;
-            if (q > p) {
-              const int idx = lane * PER + p;
-              const bool asc = (idx & k) == 0;
-              const double lo = __builtin_fmin(v[p], v[q]), hi = __builtin_fmax(v[p], v[q]);
-              v[p] = asc ? lo : hi;
-              v[q] = asc ? hi : lo;
-            }
-          }
-        } else {
-          const int lm = jj / PER;
-#pragma unroll
-          for (int p = 0; p < PER; p++) {
-            const int idx = lane * PER + p;
-            const double o = __shfl_xor(v[p], lm, 64);
-            const bool asc = (idx & k) == 0, lower = (idx & jj) == 0;
-            const double lo = __builtin_fmin(v[p], o), hi = __builtin_fmax(v[p], o);
-            v[p] = (asc == lower) ? lo : hi;
-          }
-        }
-      }
-    }
-    dd acc{0.0, 0.0};
-#pragma unroll
-    for (int p = 0; p < PER; p++)
-      if (lane * PER + p < n) acc = dd_add(acc, dd{v[p], 0.0});
-    const dd tot = wave_dd_sum(acc);
-    // Element r of the sorted bucket (every lane gets it).
-    auto elem = [&](int r) {
-      double x = v[0];
-#pragma unroll
-      for (int p = 1; p < PER; p++)
-        if ((r % PER) == p) x = v[p];
-      return __shfl(x, r / PER, 64);
-    };
-    const double vmin = elem(0), vmax = elem((int)n - 1);
-    if (lane == 0) {
-      SelMark* mp = mk + w.mark;
-      mp->sum_hi = tot.hi;
-      mp->sum_lo = tot.lo;
-      mp->vmin = vmin;
-      mp->vmax = vmax;
-    }
-    for (int q = 0; q < nb; q++) {
-      if (q > 0 && (q & 63) == 0) vr_l = q + lane < nb ? bd[w.b0 + q + lane].vr : 0;
-      const int rho = (int)(lane_value(vr_l, q & 63) - w.vstart);
-      dd sl{0.0, 0.0};
-#pragma unroll
-      for (int p = 0; p < PER; p++)
-        if (lane * PER + p < rho) sl = dd_add(sl, dd{v[p], 0.0});
-      const dd sls = wave_dd_sum(sl);
-      const double xh = elem(rho), xl = rho > 0 ? elem(rho - 1) : 0.0;
-      if (lane == 0) {
-        SelBound& o = bd[w.b0 + q];
-        o.x_hi = xh;
-        o.x_lo = xl;
-        o.dup = rho > 0 && xl == xh;
-        o.sl_hi = sls.hi;
-        o.sl_lo = sls.lo;
-      }
-    }
+    // At most 64 values: one per lane (fewer network stages, half the
+    // shuffles of two per lane).
+    if (w.n <= 64) sel_sorted_wave_one<1>(w, lane, mk, bnd, X);
+    else sel_sorted_wave_one<2>(w, lane, mk, bnd, X);
   }
 }
 
@@ -4659,7 +4669,7 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       // One wave per bucket of the smallest class (the list is long; every
       // bucket's chain of dependent loads then overlaps the others').
       const unsigned grid_w = (unsigned)std::min<int64_t>((sel_m + 3) / 4, 65535);
-      k_sel_sorted_wave<2><<<grid_w, 256, 0, stream_>>>(dwv, dmlc + 2, dmk, dbnd, x);
+      k_sel_sorted_wave<<<grid_w, 256, 0, stream_>>>(dwv, dmlc + 2, dmk, dbnd, x);
       launched("k_sel_sorted_wave<128>");
       k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l1, dmlc + 3, dmk,
                                                                     dbnd, x);
