@@ -3393,9 +3393,10 @@ __device__ inline dd wave_dd_sum(dd v) {
   return v;
 }
 
-// One bucket of at most 64 PER values, sorted by its wave (element
-// lane * PER + p in v[p]).
-template <int PER>
+// One bucket of at most KMAX <= 64 PER values, sorted by its wave (element
+// lane * PER + p in v[p]).  The network stops at runs of KMAX: the first
+// run (ascending) holds the bucket, the padding above it is all +inf.
+template <int PER, int KMAX>
 __device__ inline void sel_sorted_wave_one(const SelWave& w, int lane, SelMark* mk, SelBound* bnd,
                                            const double* X) {
   const double inf = __builtin_inf();
@@ -3413,7 +3414,7 @@ __device__ inline void sel_sorted_wave_one(const SelWave& w, int lane, SelMark* 
   // loaded beside the values.
   const int nb = w.b1 - w.b0;
   int64_t vr_l = lane < nb ? bd[w.b0 + lane].vr : 0;
-  for (int k = 2; k <= 64 * PER; k <<= 1) {
+  for (int k = 2; k <= KMAX; k <<= 1) {
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
       if (jj < PER) {
 #pragma unroll
@@ -3491,9 +3492,10 @@ __global__ __launch_bounds__(256) void k_sel_sorted_wave(const SelWave* wl,
        gi += waves) {
     const SelWave w = wl[gi];
     // At most 64 values: one per lane (fewer network stages, half the
-    // shuffles of two per lane).
-    if (w.n <= 64) sel_sorted_wave_one<1>(w, lane, mk, bnd, X);
-    else sel_sorted_wave_one<2>(w, lane, mk, bnd, X);
+    // shuffles of two per lane); at most 32: the network's runs stop at 32.
+    if (w.n <= 32) sel_sorted_wave_one<1, 32>(w, lane, mk, bnd, X);
+    else if (w.n <= 64) sel_sorted_wave_one<1, 64>(w, lane, mk, bnd, X);
+    else sel_sorted_wave_one<2, 128>(w, lane, mk, bnd, X);
   }
 }
 
