@@ -4659,8 +4659,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       const int2* l3 = dmlx + 3 * sel_m;
       // The buckets of the radix select (the largest, few) are finished on
       // the side stream beside the sorted classes (disjoint buckets and
-      // boundaries) when there are several digests.
-      const bool fork_b = nsel > 1 && !forked;
+      // boundaries) when there are several digests, also behind small
+      // digests there (by now they are mostly done; the finish waits for
+      // that stream in any case).
+      const bool fork_b = nsel > 1;
       if (fork_b) {
         check_hip(hipEventRecord(fork_, stream_), "fork select");
         check_hip(hipStreamWaitEvent(side_, fork_, 0), "fork select wait");
@@ -4676,8 +4678,10 @@ void Engine::execute(int scale, const PlanSummary& P, const double* d_v, const d
       k_sel_sorted<1024, kBlock, 128><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l1, dmlc + 3, dmk,
                                                                     dbnd, x);
       launched("k_sel_sorted<1024>");
-      k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, stream_>>>(dsegs, dsel, dst, l2, dmlc + 4, dmk,
-                                                                     dbnd, x);
+      // With several digests the 1025..4096-value class follows the radix
+      // select on the side stream (the two smaller classes take longer).
+      k_sel_sorted<4096, kBlock, 1024><<<grid, kBlock, 0, fork_b ? side_ : stream_>>>(
+          dsegs, dsel, dst, l2, dmlc + 4, dmk, dbnd, x);
       launched("k_sel_sorted<4096>");
       if (fork_b) {
         check_hip(hipEventRecord(join_, side_), "join select");
